@@ -1,0 +1,189 @@
+/*
+ * CPU oracle for the fluxflow pressure-projection path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, in plain C, the
+ * algorithms of the reference's WebGPU/TSL code so that the HIP product path
+ * can be checked against it.  Only the test-suite, the smoke check and the
+ * CPU-baseline leg of the benchmark may load it, and there only as the
+ * checker (or as the CPU baseline being timed), never as a fallback.
+ *
+ * Layout conventions (packages/tsl_array_n/src/array.js:24-38): every field is
+ * first-index-fastest, cell (i,j,k) lives at i + nx*(j + ny*k).  MAC faces:
+ *   U [nx+1, ny, nz]   V [nx, ny+1, nz]   W [nx, ny, nz+1]
+ * and the face array of axis a at index I is the LOWER face of cell I
+ * (packages/fluxflow/src/linalg/multigrid.js:316-324).
+ *
+ * Every real-valued routine exists twice: suffix _f64 (double storage and
+ * arithmetic: the FP64 truth) and _f32 (float storage and arithmetic; dot
+ * products accumulate in double, like the GPU path).
+ */
+#ifndef FLUX_ORACLE_H
+#define FLUX_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_LEVELS 10
+
+/* PCG stop codes: packages/fluxflow/src/linalg/linalg.js:1056-1103 */
+enum {
+    ORC_STOP_NONE = 0,
+    ORC_STOP_DEGENERATE_PAP = 1,
+    ORC_STOP_PAP_GROWTH = 2,
+    ORC_STOP_ALPHA_MAGNITUDE = 3,
+    ORC_STOP_DEGENERATE_OLD_RZ = 4,
+    ORC_STOP_CONVERGED = 5,
+    ORC_STOP_RESIDUAL_GROWING = 6
+};
+
+enum { ORC_PREC_MULTIGRID = 0, ORC_PREC_JACOBI = 1, ORC_PREC_NONE = 2 };
+
+typedef struct {
+    int ndim;        /* 2 or 3; a 2-D grid has n[2] == 1 */
+    int n[3];
+    double h[3];
+} orc_grid;
+
+typedef struct {
+    int levels;          /* numberOfLevels (default 1) */
+    int nu_down;         /* numberOfSmoothingIterationsDown (2) */
+    int nu_up;           /* numberOfSmoothingIterationsUp (2) */
+    int n_coarsest;      /* numberOfCoarsestIterations (20) */
+    int n_final;         /* numberOfFinalIterations (2) */
+    double omega;        /* sorFactor (1.0) */
+} orc_mg_options;
+
+typedef struct {
+    int iterations;
+    int stopped_by;      /* ORC_STOP_* ; CONVERGED when the threshold was met */
+    int converged;
+    double residual_sq;  /* r.r of the last true residual */
+    double threshold;    /* absolute stop threshold used */
+    double b_norm;
+} orc_pcg_state;
+
+/* ---------------- deterministic inputs ---------------- */
+/* mulberry32: packages/fluxflow/src/grid/jitter_random.js:29-42 */
+typedef struct { uint32_t state; } orc_rng;
+void orc_rng_seed(orc_rng* r, uint32_t seed);
+uint32_t orc_rng_next_u32(orc_rng* r);
+double orc_rng_next(orc_rng* r);           /* [0,1) exactly as the JS */
+void orc_rng_fill(uint32_t seed, double* out, int64_t n);
+
+/* computeFlipBoxSeed3: packages/fluxflow/src/grid/grid_flip_solver3.js:70-130.
+ * Writes count*3 floats (positions); returns the count.  Pass out=NULL to
+ * only count. */
+int64_t orc_flip_box_seed3(const double box_min[3], const double box_max[3],
+                           const double h[3], int ppa, double jitter,
+                           uint32_t seed, float* out);
+
+/* cellIndexOf: grid_flip_solver3.js:227-235 (f32, IEEE division). */
+void orc_cell_index3(const float* pos, int64_t count, const float origin[3],
+                     const float h[3], const int n[3], int32_t* cell_out);
+
+/* Stable counting sort of particles by cell (the deterministic binning the
+ * HIP path builds with a radix sort): perm_out[rank] = particle id. */
+void orc_sort_by_cell(const int32_t* cell, int64_t count, int64_t n_cells,
+                      int32_t* perm_out, int32_t* cell_start_out);
+
+/* P2G, fixed-point int32 mode exactly as buildScatter/buildFinalize
+ * (grid_flip_solver3.js:508-564); component = 0,1,2 (U,V,W).  numer/denom are
+ * int32 face accumulators (not cleared here). */
+void orc_p2g_fixed3(const float* pos, const float* vel, int64_t count, int component,
+                    const int face_n[3], const float data_origin[3], const float h[3],
+                    float scale, int32_t* numer, int32_t* denom);
+void orc_p2g_finalize_fixed(const int32_t* numer, const int32_t* denom, int64_t n_faces,
+                            float scale, float eps, float* out, int32_t* valid);
+/* P2G, deterministic float mode: gather over cell-sorted particles (perm and
+ * cell_start from orc_sort_by_cell), f32 sums in a fixed order, finalized
+ * with the same eps rule. */
+void orc_p2g_gather3(const float* pos, const float* vel, const int32_t* perm,
+                     const int32_t* cell_start, int component, const int n[3],
+                     const float data_origin[3], const float h[3], float eps,
+                     float* out, int32_t* valid);
+/* P2G float mode: the same trilinear weights, summed in double in particle
+ * order, then divided (the value the sorted segmented sum must reproduce). */
+void orc_p2g_float3(const float* pos, const float* vel, int64_t count, int component,
+                    const int face_n[3], const float data_origin[3], const float h[3],
+                    double* numer, double* denom);
+void orc_p2g_finalize_float(const double* numer, const double* denom, int64_t n_faces,
+                            float eps, float* out, int32_t* valid);
+
+/* trilinear MAC sample: grid_math3.js:61-126 */
+float orc_sample_trilinear3(const float* data, const int shape[3], const float data_origin[3],
+                            const float h[3], const float pos[3]);
+/* G2P: grid_flip_solver3.js:903-922 */
+void orc_g2p3(const float* pos, float* vel, int64_t count,
+              const float* u, const float* v, const float* w,
+              const float* u_old, const float* v_old, const float* w_old,
+              const int n[3], const float h[3], const float origin[3],
+              float flip_ratio, float damping);
+/* particle advection, RK2 back-trace with no collider:
+ * grid_flip_solver3.js:933-940 and advection_solver3.js:20-62 */
+void orc_advect_particles3(float* pos, int64_t count,
+                           const float* u, const float* v, const float* w,
+                           const int n[3], const float h[3], const float origin[3],
+                           float dt, int max_substeps);
+/* semi-Lagrangian MAC advection, order 1, no collider:
+ * advection_solver3.js:130-176 with grid_math3.js:137-219 */
+void orc_advect_face3(const float* u, const float* v, const float* w,
+                      float* u_out, float* v_out, float* w_out,
+                      const int n[3], const float h[3], const float origin[3],
+                      float dt, int max_substeps);
+/* two-phase face beta: grid_two_phase_flip_solver2.js:693-762 (3-D form) */
+void orc_two_phase_beta3(const float* liquid_fraction, const int n[3],
+                         float rho_gas, float rho_liquid,
+                         float* beta_u, float* beta_v, float* beta_w);
+
+/* ---------------- real-templated solver (suffix _f64 / _f32) ---------------- */
+#define ORC_DECL(real, S)                                                                        \
+    void orc_apply_##S(const orc_grid* g, const real* x, real* out, const uint8_t* mask,          \
+                       const real* const* w);                                                     \
+    void orc_diag_##S(const orc_grid* g, real* out, const uint8_t* mask, const real* const* w);   \
+    void orc_relax_##S(const orc_grid* g, real* x, const real* b, const uint8_t* mask,            \
+                       const real* const* w, double omega, int color);                            \
+    void orc_residual_##S(const orc_grid* g, const real* x, const real* b, real* r,               \
+                          const uint8_t* mask, const real* const* w);                             \
+    void orc_restrict_##S(const orc_grid* fine, const real* r_fine, const uint8_t* fine_mask,     \
+                          real* b_coarse, const uint8_t* coarse_mask);                            \
+    void orc_correct_##S(const orc_grid* fine, const real* x_coarse, real* x_fine,                \
+                         const uint8_t* fine_mask);                                               \
+    void orc_coarsen_weights_##S(const orc_grid* fine, const real* const* w_fine,                 \
+                                 real* const* w_coarse);                                          \
+    void orc_coarsen_mask_##S(const orc_grid* fine, const uint8_t* fine_src,                      \
+                              const real* const* w_coarse, uint8_t* coarse_mask);                 \
+    void* orc_mg_create_##S(const orc_grid* g, const orc_mg_options* o, const uint8_t* mask,      \
+                            const uint8_t* coarse_src_mask, const real* const* w);                \
+    void orc_mg_destroy_##S(void* mg);                                                            \
+    void orc_mg_apply_##S(void* mg, const real* r, real* z);                                      \
+    int orc_mg_level_info_##S(void* mg, int level, int* n_out, uint8_t** mask, real** w);         \
+    int orc_pcg_solve_##S(const orc_grid* g, const uint8_t* mask, const real* const* w,           \
+                          int prec, void* mg, const real* b, real* x, double tol, int maxiter,    \
+                          int relative, orc_pcg_state* st);                                       \
+    double orc_true_residual_##S(const orc_grid* g, const uint8_t* mask, const real* const* w,    \
+                                 const real* b, const real* x);                                   \
+    void orc_pressure3_assemble_##S(const orc_grid* g, const uint8_t* active, const real* target, \
+                                    const real* const* beta, const real* const* cw,               \
+                                    const real* const* vel, uint8_t* mask_out, uint8_t* vent_out, \
+                                    real* target_out, real* const* wsys_out, real* b_out,         \
+                                    real* pressure);                                              \
+    void orc_pressure3_correct_##S(const orc_grid* g, const real* pressure,                       \
+                                   const real* const* beta, const real* const* cw,                \
+                                   const real* const* vel_in, real* const* vel_out);              \
+    int64_t orc_count_bad_##S(const real* p, int64_t n, double max_plausible);                    \
+    void orc_divergence3_##S(const orc_grid* g, const real* const* vel, real* out);
+
+ORC_DECL(double, f64)
+ORC_DECL(float, f32)
+
+/* threads for the OpenMP arms (1 = the reference's serial model) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

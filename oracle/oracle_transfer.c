@@ -1,0 +1,480 @@
+/*
+ * Oracle for the particle/grid transfers and advection (TEST INFRASTRUCTURE
+ * ONLY).  f32 arithmetic in the reference's operation order, built with
+ * -ffp-contract=off so that no multiply-add is fused behind our back.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle_util.h"
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---- mulberry32: packages/fluxflow/src/grid/jitter_random.js:29-42 ---- */
+void orc_rng_seed(orc_rng* r, uint32_t seed) { r->state = seed; }
+
+uint32_t orc_rng_next_u32(orc_rng* r) {
+    r->state += 0x6D2B79F5u;
+    uint32_t t = r->state;
+    t = (t ^ (t >> 15)) * (t | 1u);
+    t ^= t + (t ^ (t >> 7)) * (t | 61u);
+    return t ^ (t >> 14);
+}
+
+double orc_rng_next(orc_rng* r) { return (double)orc_rng_next_u32(r) / 4294967296.0; }
+
+void orc_rng_fill(uint32_t seed, double* out, int64_t n) {
+    orc_rng r;
+    orc_rng_seed(&r, seed);
+    for (int64_t q = 0; q < n; ++q) out[q] = orc_rng_next(&r);
+}
+
+/* ---- computeFlipBoxSeed3: grid_flip_solver3.js:70-130 ---- */
+static int js_round_pos(double v) { return (int)floor(v + 0.5); } /* Math.round */
+
+int64_t orc_flip_box_seed3(const double bmin[3], const double bmax[3], const double h[3],
+                           int ppa, double jitter, uint32_t seed, float* out) {
+    int cells[3];
+    double sub[3];
+    for (int a = 0; a < 3; ++a) {
+        int c = js_round_pos((bmax[a] - bmin[a]) / h[a]);
+        cells[a] = c > 1 ? c : 1;
+        sub[a] = h[a] / ppa;
+    }
+    int64_t count = (int64_t)cells[0] * cells[1] * cells[2] * ppa * ppa * ppa;
+    if (!out) return count;
+    orc_rng r;
+    orc_rng_seed(&r, seed);
+    int64_t q = 0;
+    for (int ck = 0; ck < cells[2]; ++ck)
+        for (int cj = 0; cj < cells[1]; ++cj)
+            for (int ci = 0; ci < cells[0]; ++ci)
+                for (int sk = 0; sk < ppa; ++sk)
+                    for (int sj = 0; sj < ppa; ++sj)
+                        for (int si = 0; si < ppa; ++si) {
+                            double jx = (orc_rng_next(&r) * 2 - 1) * jitter * sub[0];
+                            double jy = (orc_rng_next(&r) * 2 - 1) * jitter * sub[1];
+                            double jz = (orc_rng_next(&r) * 2 - 1) * jitter * sub[2];
+                            double x = bmin[0] + ci * h[0] + (si + 0.5) * sub[0] + jx;
+                            double y = bmin[1] + cj * h[1] + (sj + 0.5) * sub[1] + jy;
+                            double z = bmin[2] + ck * h[2] + (sk + 0.5) * sub[2] + jz;
+                            out[3 * q + 0] = (float)x;
+                            out[3 * q + 1] = (float)y;
+                            out[3 * q + 2] = (float)z;
+                            ++q;
+                        }
+    return count;
+}
+
+/* ---- cellIndexOf: grid_flip_solver3.js:227-235 ---- */
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void orc_cell_index3(const float* pos, int64_t count, const float origin[3], const float h[3],
+                     const int n[3], int32_t* cell_out) {
+    for (int64_t p = 0; p < count; ++p) {
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+            float g = (pos[3 * p + a] - origin[a]) / h[a];
+            c[a] = clampi((int)floorf(g), 0, n[a] - 1);
+        }
+        cell_out[p] = (int32_t)orc_cidx(n, c[0], c[1], c[2]);
+    }
+}
+
+/* stable counting sort: the permutation a stable LSD radix sort produces */
+void orc_sort_by_cell(const int32_t* cell, int64_t count, int64_t n_cells, int32_t* perm,
+                      int32_t* cell_start) {
+    int64_t* cnt = (int64_t*)calloc(n_cells + 1, sizeof(int64_t));
+    for (int64_t p = 0; p < count; ++p) cnt[cell[p] + 1]++;
+    for (int64_t c = 0; c < n_cells; ++c) cnt[c + 1] += cnt[c];
+    for (int64_t c = 0; c <= n_cells; ++c) cell_start[c] = (int32_t)cnt[c];
+    for (int64_t p = 0; p < count; ++p) perm[cnt[cell[p]]++] = (int32_t)p;
+    free(cnt);
+}
+
+/* ---- trilinearCoordsAndWeights3: grid_math3.js:61-95 ---- */
+typedef struct {
+    int i0c, j0c, k0c, i1c, j1c, k1c;
+    float w[8]; /* 000,100,010,110,001,101,011,111 */
+} orc_tri;
+
+static inline void tri_weights(const float pos[3], const float o[3], const float h[3],
+                               const int shape[3], orc_tri* t) {
+    float g[3], f[3];
+    int i0[3];
+    for (int a = 0; a < 3; ++a) {
+        g[a] = (pos[a] - o[a]) / h[a];
+        i0[a] = (int)floorf(g[a]);
+        f[a] = g[a] - (float)i0[a];
+    }
+    t->i0c = clampi(i0[0], 0, shape[0] - 1);
+    t->i1c = clampi(i0[0] + 1, 0, shape[0] - 1);
+    t->j0c = clampi(i0[1], 0, shape[1] - 1);
+    t->j1c = clampi(i0[1] + 1, 0, shape[1] - 1);
+    t->k0c = clampi(i0[2], 0, shape[2] - 1);
+    t->k1c = clampi(i0[2] + 1, 0, shape[2] - 1);
+    float gx = 1.0f - f[0], gy = 1.0f - f[1], gz = 1.0f - f[2];
+    float fx = f[0], fy = f[1], fz = f[2];
+    t->w[0] = (gx * gy) * gz;
+    t->w[1] = (fx * gy) * gz;
+    t->w[2] = (gx * fy) * gz;
+    t->w[3] = (fx * fy) * gz;
+    t->w[4] = (gx * gy) * fz;
+    t->w[5] = (fx * gy) * fz;
+    t->w[6] = (gx * fy) * fz;
+    t->w[7] = (fx * fy) * fz;
+}
+
+static inline void tri_corner(const orc_tri* t, int c, int* i, int* j, int* k) {
+    *i = (c & 1) ? t->i1c : t->i0c;
+    *j = (c & 2) ? t->j1c : t->j0c;
+    *k = (c & 4) ? t->k1c : t->k0c;
+}
+
+/* ---- P2G fixed-point: buildScatter / buildFinalize 508-564 ---- */
+static inline int32_t f32_to_i32_sat(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483520.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+
+void orc_p2g_fixed3(const float* pos, const float* vel, int64_t count, int comp,
+                    const int face_n[3], const float data_origin[3], const float h[3], float scale,
+                    int32_t* numer, int32_t* denom) {
+    for (int64_t p = 0; p < count; ++p) {
+        orc_tri t;
+        tri_weights(&pos[3 * p], data_origin, h, face_n, &t);
+        float value = vel[3 * p + comp];
+        for (int c = 0; c < 8; ++c) {
+            int i, j, k;
+            tri_corner(&t, c, &i, &j, &k);
+            int64_t f = orc_cidx(face_n, i, j, k);
+            float w = t.w[c];
+            /* round() in WGSL is round-half-to-even: rintf in the default mode */
+            uint32_t nu = (uint32_t)numer[f] + (uint32_t)f32_to_i32_sat(rintf((value * w) * scale));
+            uint32_t de = (uint32_t)denom[f] + (uint32_t)f32_to_i32_sat(rintf(w * scale));
+            numer[f] = (int32_t)nu; /* atomicAdd wraps */
+            denom[f] = (int32_t)de;
+        }
+    }
+}
+
+void orc_p2g_finalize_fixed(const int32_t* numer, const int32_t* denom, int64_t n_faces, float scale,
+                            float eps, float* out, int32_t* valid) {
+    for (int64_t f = 0; f < n_faces; ++f) {
+        float nu = (float)numer[f] / scale;
+        float de = (float)denom[f] / scale;
+        if (de > eps) {
+            out[f] = nu / de;
+            valid[f] = 1;
+        } else {
+            out[f] = 0;
+            valid[f] = 0;
+        }
+    }
+}
+
+/* ---- P2G as a deterministic gather over cell-sorted particles ----
+ * For each face the contributing cells are visited in (k, j, i) order over
+ * the 2x3x3 neighbourhood that can reach it, the particles of a cell in
+ * sorted order, and each particle's eight corners in the reference's order;
+ * every corner that lands on this face adds v*w and w in f32.  The HIP gather
+ * kernel uses exactly this order, so the two agree bit for bit. */
+void orc_p2g_gather3(const float* pos, const float* vel, const int32_t* perm,
+                     const int32_t* cell_start, int comp, const int n[3],
+                     const float data_origin[3], const float h[3], float eps,
+                     float* out, int32_t* valid) {
+    int fn[3] = {n[0] + (comp == 0), n[1] + (comp == 1), n[2] + (comp == 2)};
+#pragma omp parallel for schedule(static)
+    for (int fk = 0; fk < fn[2]; ++fk)
+        for (int fj = 0; fj < fn[1]; ++fj)
+            for (int fi = 0; fi < fn[0]; ++fi) {
+                int F[3] = {fi, fj, fk};
+                int lo[3], hi[3];
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = (a == comp) ? F[a] - 1 : F[a] - 1;
+                    hi[a] = (a == comp) ? F[a] : F[a] + 1;
+                    if (lo[a] < 0) lo[a] = 0;
+                    if (hi[a] > n[a] - 1) hi[a] = n[a] - 1;
+                }
+                float nu = 0.0f, de = 0.0f;
+                for (int ck = lo[2]; ck <= hi[2]; ++ck)
+                    for (int cj = lo[1]; cj <= hi[1]; ++cj)
+                        for (int ci = lo[0]; ci <= hi[0]; ++ci) {
+                            int64_t c = orc_cidx(n, ci, cj, ck);
+                            for (int32_t s = cell_start[c]; s < cell_start[c + 1]; ++s) {
+                                int32_t p = perm[s];
+                                orc_tri t;
+                                tri_weights(&pos[3 * p], data_origin, h, fn, &t);
+                                float value = vel[3 * p + comp];
+                                for (int q = 0; q < 8; ++q) {
+                                    int i, j, k;
+                                    tri_corner(&t, q, &i, &j, &k);
+                                    if (i == fi && j == fj && k == fk) {
+                                        nu = nu + value * t.w[q];
+                                        de = de + t.w[q];
+                                    }
+                                }
+                            }
+                        }
+                int64_t f = orc_cidx(fn, fi, fj, fk);
+                if (de > eps) {
+                    out[f] = nu / de;
+                    valid[f] = 1;
+                } else {
+                    out[f] = 0;
+                    valid[f] = 0;
+                }
+            }
+}
+
+void orc_p2g_float3(const float* pos, const float* vel, int64_t count, int comp,
+                    const int face_n[3], const float data_origin[3], const float h[3],
+                    double* numer, double* denom) {
+    for (int64_t p = 0; p < count; ++p) {
+        orc_tri t;
+        tri_weights(&pos[3 * p], data_origin, h, face_n, &t);
+        double value = vel[3 * p + comp];
+        for (int c = 0; c < 8; ++c) {
+            int i, j, k;
+            tri_corner(&t, c, &i, &j, &k);
+            int64_t f = orc_cidx(face_n, i, j, k);
+            numer[f] += value * (double)t.w[c];
+            denom[f] += (double)t.w[c];
+        }
+    }
+}
+
+void orc_p2g_finalize_float(const double* numer, const double* denom, int64_t n_faces, float eps,
+                            float* out, int32_t* valid) {
+    for (int64_t f = 0; f < n_faces; ++f) {
+        if (denom[f] > eps) {
+            out[f] = (float)(numer[f] / denom[f]);
+            valid[f] = 1;
+        } else {
+            out[f] = 0;
+            valid[f] = 0;
+        }
+    }
+}
+
+/* ---- collocatedValueAtPosition3: grid_math3.js:99-113 ---- */
+float orc_sample_trilinear3(const float* data, const int shape[3], const float o[3], const float h[3],
+                            const float pos[3]) {
+    orc_tri t;
+    tri_weights(pos, o, h, shape, &t);
+    float s = 0;
+    for (int c = 0; c < 8; ++c) {
+        int i, j, k;
+        tri_corner(&t, c, &i, &j, &k);
+        float term = data[orc_cidx(shape, i, j, k)] * t.w[c];
+        s = (c == 0) ? term : s + term;
+    }
+    return s;
+}
+
+static void mac_origins(const float origin[3], const float h[3], float ou[3], float ov[3], float ow[3]) {
+    /* createFaceCenteredGrid3: grid_data3.js:128-130 */
+    ou[0] = origin[0];             ou[1] = origin[1] + h[1] * 0.5f; ou[2] = origin[2] + h[2] * 0.5f;
+    ov[0] = origin[0] + h[0] * 0.5f; ov[1] = origin[1];             ov[2] = origin[2] + h[2] * 0.5f;
+    ow[0] = origin[0] + h[0] * 0.5f; ow[1] = origin[1] + h[1] * 0.5f; ow[2] = origin[2];
+}
+
+static void sample_mac(const float* u, const float* v, const float* w, const int n[3], const float h[3],
+                       const float ou[3], const float ov[3], const float ow[3], const float pos[3],
+                       float out[3]) {
+    int su[3] = {n[0] + 1, n[1], n[2]}, sv[3] = {n[0], n[1] + 1, n[2]}, sw[3] = {n[0], n[1], n[2] + 1};
+    out[0] = orc_sample_trilinear3(u, su, ou, h, pos);
+    out[1] = orc_sample_trilinear3(v, sv, ov, h, pos);
+    out[2] = orc_sample_trilinear3(w, sw, ow, h, pos);
+}
+
+/* ---- G2P: grid_flip_solver3.js:903-922 ---- */
+void orc_g2p3(const float* pos, float* vel, int64_t count, const float* u, const float* v,
+              const float* w, const float* uo, const float* vo, const float* wo, const int n[3],
+              const float h[3], const float origin[3], float flip_ratio, float damping) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    float dmp = damping < 0 ? 0 : (damping > 1 ? 1 : damping);
+    float one_r = 1.0f - flip_ratio, one_d = 1.0f - dmp;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < count; ++p) {
+        float nv[3], ov3[3];
+        sample_mac(u, v, w, n, h, ou, ov, ow, &pos[3 * p], nv);
+        sample_mac(uo, vo, wo, n, h, ou, ov, ow, &pos[3 * p], ov3);
+        float res[3];
+        int bad = 0;
+        for (int a = 0; a < 3; ++a) {
+            float delta = nv[a] - ov3[a];
+            float flip = vel[3 * p + a] + delta;
+            float blended = flip * flip_ratio + nv[a] * one_r;
+            res[a] = blended * one_d;
+            if (!isfinite(res[a])) bad = 1;
+        }
+        for (int a = 0; a < 3; ++a) {
+            float r = res[a];
+            r = r < -500.0f ? -500.0f : (r > 500.0f ? 500.0f : r);
+            vel[3 * p + a] = bad ? 0.0f : r;
+        }
+    }
+}
+
+/* ---- backTrace (no collider): advection_solver3.js:20-62 ---- */
+static void back_trace(const float* u, const float* v, const float* w, const int n[3], const float h[3],
+                       const float ou[3], const float ov[3], const float ow[3], const float start[3],
+                       float dt, float hmin, int max_sub, float direction, float out[3]) {
+    float pt[3] = {start[0], start[1], start[2]};
+    float rem = dt;
+    for (int s = 0; s < max_sub; ++s) {
+        if (rem <= 1e-6f) break;
+        float v0[3];
+        sample_mac(u, v, w, n, h, ou, ov, ow, pt, v0);
+        float len = sqrtf((v0[0] * v0[0] + v0[1] * v0[1]) + v0[2] * v0[2]);
+        float ns = ceilf((len * rem) / hmin);
+        if (ns < 1.0f) ns = 1.0f;
+        float sub = rem / ns;
+        float half = sub * 0.5f;
+        float mid[3];
+        for (int a = 0; a < 3; ++a) mid[a] = pt[a] - (v0[a] * half) * direction;
+        float mv[3];
+        sample_mac(u, v, w, n, h, ou, ov, ow, mid, mv);
+        for (int a = 0; a < 3; ++a) pt[a] = pt[a] - (mv[a] * sub) * direction;
+        rem = rem - sub;
+    }
+    out[0] = pt[0];
+    out[1] = pt[1];
+    out[2] = pt[2];
+}
+
+/* ---- advectParticles: grid_flip_solver3.js:933-940 ---- */
+void orc_advect_particles3(float* pos, int64_t count, const float* u, const float* v, const float* w,
+                           const int n[3], const float h[3], const float origin[3], float dt,
+                           int max_sub) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    float hmin = fminf(h[0], fminf(h[1], h[2]));
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = origin[a] + 1e-4f;
+        hi[a] = (origin[a] + (float)n[a] * h[a]) - 1e-4f;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < count; ++p) {
+        float tr[3];
+        back_trace(u, v, w, n, h, ou, ov, ow, &pos[3 * p], dt, hmin, max_sub, -1.0f, tr);
+        int bad = !isfinite(tr[0]) || !isfinite(tr[1]) || !isfinite(tr[2]);
+        for (int a = 0; a < 3; ++a) {
+            float s = bad ? pos[3 * p + a] : tr[a];
+            pos[3 * p + a] = s < lo[a] ? lo[a] : (s > hi[a] ? hi[a] : s);
+        }
+    }
+}
+
+/* ---- monotonicCubic1d3 / collocatedCubicValueAtPosition3: grid_math3.js:137-207 ---- */
+static inline float mono_cubic(float f0, float f1, float f2, float f3, float f) {
+    float D1 = f2 - f1;
+    float rawD1 = (f2 - f0) * 0.5f;
+    float rawD2 = (f3 - f1) * 0.5f;
+    int flat = fabsf(D1) < 1e-12f;
+    float d1 = (flat || rawD1 * D1 < 0) ? 0.0f : rawD1;
+    float d2 = (flat || rawD2 * D1 < 0) ? 0.0f : rawD2;
+    float a3 = (d1 + d2) - D1 * 2.0f;
+    float a2 = (D1 * 3.0f - d1 * 2.0f) - d2;
+    float a1 = d1, a0 = f1;
+    return (((a3 * f) * f) * f + (a2 * f) * f) + a1 * f + a0;
+}
+
+static inline void cubic_idx(float coord, int nn, int idx[4], float* f) {
+    int i0 = (int)floorf(coord);
+    *f = coord - (float)i0;
+    idx[0] = clampi(i0 - 1, 0, nn - 1);
+    idx[1] = clampi(i0, 0, nn - 1);
+    idx[2] = clampi(i0 + 1, 0, nn - 1);
+    idx[3] = clampi(i0 + 2, 0, nn - 1);
+}
+
+static float sample_cubic(const float* data, const int s[3], const float o[3], const float h[3],
+                          const float pos[3]) {
+    int xi[4], yi[4], zi[4];
+    float fx, fy, fz;
+    cubic_idx((pos[0] - o[0]) / h[0], s[0], xi, &fx);
+    cubic_idx((pos[1] - o[1]) / h[1], s[1], yi, &fy);
+    cubic_idx((pos[2] - o[2]) / h[2], s[2], zi, &fz);
+    float zv[4];
+    for (int c = 0; c < 4; ++c) {
+        float rv[4];
+        for (int b = 0; b < 4; ++b) {
+            int j = yi[b], k = zi[c];
+            rv[b] = mono_cubic(data[orc_cidx(s, xi[0], j, k)], data[orc_cidx(s, xi[1], j, k)],
+                               data[orc_cidx(s, xi[2], j, k)], data[orc_cidx(s, xi[3], j, k)], fx);
+        }
+        zv[c] = mono_cubic(rv[0], rv[1], rv[2], rv[3], fy);
+    }
+    return mono_cubic(zv[0], zv[1], zv[2], zv[3], fz);
+}
+
+/* ---- advectFaceCentered3, order 1, no collider: advection_solver3.js:130-176 ---- */
+void orc_advect_face3(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
+                      const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float* src[3] = {u, v, w};
+    float* dst[3] = {uo_, vo_, wo_};
+    const float* org[3] = {ou, ov, ow};
+    for (int a = 0; a < 3; ++a) {
+        int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
+                                    org[a][2] + (float)k * h[2]};
+                    float tr[3];
+                    back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
+                    dst[a][orc_cidx(s, i, j, k)] = sample_cubic(src[a], s, org[a], h, tr);
+                }
+    }
+}
+
+/* ---- two-phase face beta (3-D form of grid_two_phase_flip_solver2.js:693-762) ---- */
+void orc_two_phase_beta3(const float* frac, const int n[3], float rho_gas, float rho_liq, float* bu,
+                         float* bv, float* bw) {
+    const float min_d = 1e-4f;
+    float r0 = rho_gas > min_d ? rho_gas : min_d;
+    float r1 = rho_liq > min_d ? rho_liq : min_d;
+    float ref = r0 > r1 ? r0 : r1;
+    float* out[3] = {bu, bv, bw};
+    for (int a = 0; a < 3; ++a) {
+        int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    int A[3] = {i, j, k}, B[3] = {i, j, k};
+                    A[a] = clampi(A[a] - 1, 0, n[a] - 1);
+                    B[a] = clampi(B[a], 0, n[a] - 1);
+                    float ra = r0 + (r1 - r0) * frac[orc_cidx(n, A[0], A[1], A[2])];
+                    float rb = r0 + (r1 - r0) * frac[orc_cidx(n, B[0], B[1], B[2])];
+                    float rf = (ra + rb) * 0.5f;
+                    out[a][orc_cidx(s, i, j, k)] = ref / (rf > min_d ? rf : min_d);
+                }
+    }
+}

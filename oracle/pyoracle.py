@@ -1,0 +1,486 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``, as the checker.  The product path
+(``tsl_array_n_amd``) never imports this module.
+
+Arrays are numpy, first index fastest: a cell field of shape (nx, ny, nz) is
+stored as ``arr.ravel(order="F")``; helpers below take and return flat
+contiguous arrays plus the shape.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+STOP_NAMES = {
+    0: "none",
+    1: "degenerate-pAp",
+    2: "pAp-growth",
+    3: "alpha-magnitude",
+    4: "degenerate-oldRZ",
+    5: "converged",
+    6: "residual-growing",
+}
+PREC = {"multigrid": 0, "jacobi": 1, "none": 2}
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+class Grid(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("n", C.c_int * 3), ("h", C.c_double * 3)]
+
+
+class MGOptions(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int),
+        ("nu_down", C.c_int),
+        ("nu_up", C.c_int),
+        ("n_coarsest", C.c_int),
+        ("n_final", C.c_int),
+        ("omega", C.c_double),
+    ]
+
+
+class PCGState(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("stopped_by", C.c_int),
+        ("converged", C.c_int),
+        ("residual_sq", C.c_double),
+        ("threshold", C.c_double),
+        ("b_norm", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.orc_mg_create_f64.restype = C.c_void_p
+        _lib.orc_mg_create_f32.restype = C.c_void_p
+        _lib.orc_true_residual_f64.restype = C.c_double
+        _lib.orc_true_residual_f32.restype = C.c_double
+        _lib.orc_count_bad_f64.restype = C.c_int64
+        _lib.orc_count_bad_f32.restype = C.c_int64
+        _lib.orc_flip_box_seed3.restype = C.c_int64
+        _lib.orc_rng_next_u32.restype = C.c_uint32
+        _lib.orc_sample_trilinear3.restype = C.c_float
+        for s in ("f64", "f32"):
+            getattr(_lib, "orc_mg_destroy_" + s).argtypes = [C.c_void_p]
+            getattr(_lib, "orc_mg_apply_" + s).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(_lib, "orc_mg_level_info_" + s).argtypes = [
+                C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _dt(suffix):
+    return np.float64 if suffix == "f64" else np.float32
+
+
+def make_grid(shape, spacing):
+    g = Grid()
+    g.ndim = len(shape)
+    n = list(shape) + [1] * (3 - len(shape))
+    h = list(spacing) + [1.0] * (3 - len(spacing))
+    for a in range(3):
+        g.n[a] = int(n[a])
+        g.h[a] = float(h[a])
+    return g
+
+
+def face_shape(shape, axis):
+    s = list(shape) + [1] * (3 - len(shape))
+    s[axis] += 1
+    return tuple(s[: len(shape)])
+
+
+class _PtrArray:
+    """A (const real* const*) built from up to three numpy arrays."""
+
+    def __init__(self, arrays):
+        self.arrays = arrays
+        self.c = (C.c_void_p * 3)(*[(_p(a).value if a is not None else None) for a in list(arrays) + [None] * (3 - len(arrays))])
+
+
+def _w(weights):
+    if weights is None:
+        return None, None
+    pa = _PtrArray(weights)
+    return pa, C.cast(pa.c, C.c_void_p)
+
+
+def _u8(mask):
+    if mask is None:
+        return None
+    return np.ascontiguousarray(mask, dtype=np.uint8)
+
+
+def _real(a, suffix):
+    return np.ascontiguousarray(a, dtype=_dt(suffix))
+
+
+def apply(shape, spacing, x, mask=None, weights=None, suffix="f64"):
+    g = make_grid(shape, spacing)
+    x = _real(x, suffix)
+    out = np.zeros_like(x)
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    getattr(lib(), "orc_apply_" + suffix)(C.byref(g), _p(x), _p(out), _p(mask), wp)
+    return out
+
+
+def diag(shape, spacing, mask=None, weights=None, suffix="f64"):
+    g = make_grid(shape, spacing)
+    out = np.zeros(int(np.prod(shape)), dtype=_dt(suffix))
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    getattr(lib(), "orc_diag_" + suffix)(C.byref(g), _p(out), _p(mask), wp)
+    return out
+
+
+def relax(shape, spacing, x, b, color, mask=None, weights=None, omega=1.0, suffix="f64"):
+    g = make_grid(shape, spacing)
+    x = _real(x, suffix).copy()
+    b = _real(b, suffix)
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    getattr(lib(), "orc_relax_" + suffix)(C.byref(g), _p(x), _p(b), _p(mask), wp, C.c_double(omega), int(color))
+    return x
+
+
+def residual(shape, spacing, x, b, mask=None, weights=None, suffix="f64"):
+    g = make_grid(shape, spacing)
+    x = _real(x, suffix)
+    b = _real(b, suffix)
+    r = np.zeros_like(x)
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    getattr(lib(), "orc_residual_" + suffix)(C.byref(g), _p(x), _p(b), _p(r), _p(mask), wp)
+    return r
+
+
+def coarse_shape(shape):
+    return tuple(n // 2 for n in shape)
+
+
+def restrict(shape, r_fine, fine_mask=None, coarse_mask=None, suffix="f64"):
+    g = make_grid(shape, [1.0] * len(shape))
+    r_fine = _real(r_fine, suffix)
+    out = np.zeros(int(np.prod(coarse_shape(shape))), dtype=_dt(suffix))
+    getattr(lib(), "orc_restrict_" + suffix)(C.byref(g), _p(r_fine), _p(_u8(fine_mask)), _p(out), _p(_u8(coarse_mask)))
+    return out
+
+
+def correct(shape, x_coarse, x_fine, fine_mask=None, suffix="f64"):
+    g = make_grid(shape, [1.0] * len(shape))
+    xc = _real(x_coarse, suffix)
+    xf = _real(x_fine, suffix).copy()
+    getattr(lib(), "orc_correct_" + suffix)(C.byref(g), _p(xc), _p(xf), _p(_u8(fine_mask)))
+    return xf
+
+
+class Multigrid:
+    """createMultigridPreconditioner restated (multigrid.js:1213-1735)."""
+
+    def __init__(self, shape, spacing, levels=1, nu_down=2, nu_up=2, n_coarsest=20, n_final=2,
+                 omega=1.0, mask=None, coarse_mask=None, weights=None, suffix="f64"):
+        self.suffix = suffix
+        self.shape = tuple(shape)
+        self.g = make_grid(shape, spacing)
+        o = MGOptions()
+        o.levels, o.nu_down, o.nu_up, o.n_coarsest, o.n_final, o.omega = levels, nu_down, nu_up, n_coarsest, n_final, omega
+        self.o = o
+        self.mask = _u8(mask)
+        self.coarse_mask = _u8(coarse_mask)
+        self.weights = None if weights is None else [_real(w, suffix) for w in weights]
+        self._wkeep, wp = _w(self.weights)
+        self.h = getattr(lib(), "orc_mg_create_" + suffix)(C.byref(self.g), C.byref(o), _p(self.mask), _p(self.coarse_mask), wp)
+        if not self.h:
+            raise ValueError("multigrid: shape is not divisible by 2^(numberOfLevels-1)")
+
+    def apply(self, r):
+        r = _real(r, self.suffix)
+        z = np.zeros_like(r)
+        getattr(lib(), "orc_mg_apply_" + self.suffix)(C.c_void_p(self.h), _p(r), _p(z))
+        return z
+
+    def level(self, l):
+        n = (C.c_int * 3)()
+        mp = C.c_void_p()
+        wp = (C.c_void_p * 3)()
+        rc = getattr(lib(), "orc_mg_level_info_" + self.suffix)(C.c_void_p(self.h), l, n, C.byref(mp), wp)
+        if rc:
+            raise IndexError(l)
+        dims = [n[0], n[1], n[2]][: len(self.shape)]
+        cells = int(np.prod(dims))
+        mask = None
+        if mp.value:
+            mask = np.ctypeslib.as_array(C.cast(mp, C.POINTER(C.c_uint8)), shape=(cells,)).copy()
+        ws = []
+        for a in range(len(self.shape)):
+            if wp[a]:
+                ctype = C.c_double if self.suffix == "f64" else C.c_float
+                nf = int(np.prod(face_shape(dims, a)))
+                ws.append(np.ctypeslib.as_array(C.cast(wp[a], C.POINTER(ctype)), shape=(nf,)).copy())
+        return tuple(dims), mask, (ws or None)
+
+    def __del__(self):
+        try:
+            if self.h:
+                getattr(lib(), "orc_mg_destroy_" + self.suffix)(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+def pcg_solve(shape, spacing, b, x0=None, mask=None, weights=None, precond="multigrid", mg=None,
+              tol=1e-6, maxiter=100, relative=True, suffix="f64"):
+    g = make_grid(shape, spacing)
+    b = _real(b, suffix)
+    x = np.zeros_like(b) if x0 is None else _real(x0, suffix).copy()
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    st = PCGState()
+    getattr(lib(), "orc_pcg_solve_" + suffix)(
+        C.byref(g), _p(mask), wp, PREC[precond], C.c_void_p(mg.h if mg is not None else None),
+        _p(b), _p(x), C.c_double(tol), int(maxiter), int(bool(relative)), C.byref(st))
+    return x, {
+        "iterations": st.iterations,
+        "stoppedBy": STOP_NAMES[st.stopped_by],
+        "converged": bool(st.converged),
+        "residual": float(np.sqrt(abs(st.residual_sq))),
+        "threshold": st.threshold,
+        "bNorm": st.b_norm,
+    }
+
+
+def true_residual(shape, spacing, b, x, mask=None, weights=None, suffix="f64"):
+    g = make_grid(shape, spacing)
+    b = _real(b, suffix)
+    x = _real(x, suffix)
+    weights = None if weights is None else [_real(w, suffix) for w in weights]
+    keep, wp = _w(weights)
+    return getattr(lib(), "orc_true_residual_" + suffix)(C.byref(g), _p(_u8(mask)), wp, _p(b), _p(x))
+
+
+def pressure3_assemble(shape, spacing, vel, active=None, target=None, beta=None, cw=None,
+                       pressure=None, suffix="f64"):
+    """updateDirichletFields + combineAxisWeight + dispatchBuildSystem."""
+    g = make_grid(shape, spacing)
+    n = int(np.prod(shape))
+    dt = _dt(suffix)
+    vel = [_real(v, suffix) for v in vel]
+    active = _u8(active)
+    target = None if target is None else _real(target, suffix)
+    beta = None if beta is None else [_real(v, suffix) for v in beta]
+    cw = None if cw is None else [_real(v, suffix) for v in cw]
+    mask_out = np.zeros(n, np.uint8)
+    vent_out = np.zeros(n, np.uint8)
+    target_out = np.zeros(n, dt)
+    wsys = [np.zeros(int(np.prod(face_shape(shape, a))), dt) for a in range(3)]
+    b = np.zeros(n, dt)
+    p = np.zeros(n, dt) if pressure is None else _real(pressure, suffix).copy()
+    kv, vp = _w(vel)
+    kb, bp = _w(beta)
+    kc, cp = _w(cw)
+    kw, wp = _w(wsys)
+    getattr(lib(), "orc_pressure3_assemble_" + suffix)(
+        C.byref(g), _p(active), _p(target), bp, cp, vp, _p(mask_out), _p(vent_out), _p(target_out), wp, _p(b), _p(p))
+    has_mask = active is not None or cw is not None
+    has_w = beta is not None or cw is not None
+    return {
+        "mask": mask_out if has_mask else None,
+        "vent": vent_out if has_mask else None,
+        "target": target_out,
+        "weights": wsys if has_w else None,
+        "b": b,
+        "pressure": p,
+    }
+
+
+def pressure3_correct(shape, spacing, pressure, vel_in, beta=None, cw=None, vel_out=None, suffix="f64"):
+    g = make_grid(shape, spacing)
+    pressure = _real(pressure, suffix)
+    vel_in = [_real(v, suffix) for v in vel_in]
+    out = [v.copy() for v in (vel_in if vel_out is None else vel_out)]
+    out = [_real(v, suffix) for v in out]
+    kb, bp = _w(None if beta is None else [_real(v, suffix) for v in beta])
+    kc, cp = _w(None if cw is None else [_real(v, suffix) for v in cw])
+    ki, ip = _w(vel_in)
+    ko, op = _w(out)
+    getattr(lib(), "orc_pressure3_correct_" + suffix)(C.byref(g), _p(pressure), bp, cp, ip, op)
+    return out
+
+
+def count_bad(p, max_plausible=1e6, suffix="f64"):
+    p = _real(p, suffix)
+    return int(getattr(lib(), "orc_count_bad_" + suffix)(_p(p), C.c_int64(p.size), C.c_double(max_plausible)))
+
+
+def divergence3(shape, spacing, vel, suffix="f64"):
+    g = make_grid(shape, spacing)
+    vel = [_real(v, suffix) for v in vel]
+    out = np.zeros(int(np.prod(shape)), _dt(suffix))
+    kv, vp = _w(vel)
+    getattr(lib(), "orc_divergence3_" + suffix)(C.byref(g), vp, _p(out))
+    return out
+
+
+# ---------------- transfers (f32) ----------------
+
+def rng(seed, n):
+    out = np.zeros(n, np.float64)
+    lib().orc_rng_fill(C.c_uint32(seed), _p(out), C.c_int64(n))
+    return out
+
+
+def rng_u32(seed, n):
+    class R(C.Structure):
+        _fields_ = [("state", C.c_uint32)]
+    r = R()
+    lib().orc_rng_seed(C.byref(r), C.c_uint32(seed))
+    return np.array([lib().orc_rng_next_u32(C.byref(r)) for _ in range(n)], np.uint32)
+
+
+def flip_box_seed3(box_min, box_max, h, ppa=2, jitter=0.2, seed=0x9E3779B9):
+    bmin = (C.c_double * 3)(*box_min)
+    bmax = (C.c_double * 3)(*box_max)
+    hh = (C.c_double * 3)(*h)
+    count = lib().orc_flip_box_seed3(bmin, bmax, hh, int(ppa), C.c_double(jitter), C.c_uint32(seed), None)
+    out = np.zeros(count * 3, np.float32)
+    lib().orc_flip_box_seed3(bmin, bmax, hh, int(ppa), C.c_double(jitter), C.c_uint32(seed), _p(out))
+    return out
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def _i3(v):
+    return (C.c_int * 3)(*[int(x) for x in v])
+
+
+def cell_index3(pos, origin, h, n):
+    pos = np.ascontiguousarray(pos, np.float32)
+    count = pos.size // 3
+    out = np.zeros(count, np.int32)
+    lib().orc_cell_index3(_p(pos), C.c_int64(count), _f3(origin), _f3(h), _i3(n), _p(out))
+    return out
+
+
+def sort_by_cell(cell, n_cells):
+    cell = np.ascontiguousarray(cell, np.int32)
+    perm = np.zeros(cell.size, np.int32)
+    start = np.zeros(n_cells + 1, np.int32)
+    lib().orc_sort_by_cell(_p(cell), C.c_int64(cell.size), C.c_int64(n_cells), _p(perm), _p(start))
+    return perm, start
+
+
+def mac_origin(origin, h, comp):
+    o = [np.float32(x) for x in origin]
+    hh = [np.float32(x) for x in h]
+    for a in range(3):
+        if a != comp:
+            o[a] = np.float32(o[a] + np.float32(hh[a] * np.float32(0.5)))
+    return o
+
+
+def p2g_fixed3(pos, vel, n, h, origin, comp, scale=65536.0, eps=1e-4):
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    fn = list(n)
+    fn[comp] += 1
+    nf = int(np.prod(fn))
+    numer = np.zeros(nf, np.int32)
+    denom = np.zeros(nf, np.int32)
+    lib().orc_p2g_fixed3(_p(pos), _p(vel), C.c_int64(pos.size // 3), comp, _i3(fn), _f3(mac_origin(origin, h, comp)),
+                         _f3(h), C.c_float(scale), _p(numer), _p(denom))
+    out = np.zeros(nf, np.float32)
+    valid = np.zeros(nf, np.int32)
+    lib().orc_p2g_finalize_fixed(_p(numer), _p(denom), C.c_int64(nf), C.c_float(scale), C.c_float(eps), _p(out), _p(valid))
+    return out, valid, numer, denom
+
+
+def p2g_gather3(pos, vel, perm, start, n, h, origin, comp, eps=1e-4):
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    fn = list(n)
+    fn[comp] += 1
+    nf = int(np.prod(fn))
+    out = np.zeros(nf, np.float32)
+    valid = np.zeros(nf, np.int32)
+    lib().orc_p2g_gather3(_p(pos), _p(vel), _p(np.ascontiguousarray(perm, np.int32)), _p(np.ascontiguousarray(start, np.int32)),
+                          comp, _i3(n), _f3(mac_origin(origin, h, comp)), _f3(h), C.c_float(eps), _p(out), _p(valid))
+    return out, valid
+
+
+def p2g_float3(pos, vel, n, h, origin, comp, eps=1e-4):
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    fn = list(n)
+    fn[comp] += 1
+    nf = int(np.prod(fn))
+    numer = np.zeros(nf, np.float64)
+    denom = np.zeros(nf, np.float64)
+    lib().orc_p2g_float3(_p(pos), _p(vel), C.c_int64(pos.size // 3), comp, _i3(fn), _f3(mac_origin(origin, h, comp)),
+                         _f3(h), _p(numer), _p(denom))
+    out = np.zeros(nf, np.float32)
+    valid = np.zeros(nf, np.int32)
+    lib().orc_p2g_finalize_float(_p(numer), _p(denom), C.c_int64(nf), C.c_float(eps), _p(out), _p(valid))
+    return out, valid
+
+
+def g2p3(pos, vel, u, v, w, uo, vo, wo, n, h, origin, flip_ratio=0.95, damping=0.0):
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w, uo, vo, wo)]
+    lib().orc_g2p3(_p(pos), _p(vel), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin),
+                   C.c_float(flip_ratio), C.c_float(damping))
+    return vel
+
+
+def advect_particles3(pos, u, v, w, n, h, origin, dt, max_substeps=32):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_advect_particles3(_p(pos), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin),
+                                C.c_float(dt), int(max_substeps))
+    return pos
+
+
+def advect_face3(u, v, w, n, h, origin, dt, max_substeps=32):
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    outs = [np.zeros_like(a) for a in arrs]
+    lib().orc_advect_face3(*[_p(a) for a in arrs], *[_p(a) for a in outs], _i3(n), _f3(h), _f3(origin),
+                           C.c_float(dt), int(max_substeps))
+    return outs
+
+
+def two_phase_beta3(frac, n, rho_gas, rho_liquid):
+    frac = np.ascontiguousarray(frac, np.float32)
+    outs = [np.zeros(int(np.prod(face_shape(n, a))), np.float32) for a in range(3)]
+    lib().orc_two_phase_beta3(_p(frac), _i3(n), C.c_float(rho_gas), C.c_float(rho_liquid), *[_p(o) for o in outs])
+    return outs
