@@ -1,0 +1,267 @@
+/*
+ * fluxflow_hip.h -- C-ABI of the MI355X (gfx950) pressure-projection library.
+ *
+ * This is the drop-in boundary: a host binding (N-API addon, ctypes, cgo ...)
+ * maps the reference's JavaScript factories onto these entry points one for
+ * one.  Every declaration cites the reference interface it replaces
+ * (paths relative to the reference repository).
+ *
+ * Conventions, kept from the reference:
+ *  - fields are first-index-fastest: element (i,j,k) of a field of shape
+ *    [n0,n1,n2] lives at i + n0*(j + n1*k)   (packages/tsl_array_n/src/array.js:24-38)
+ *  - MAC velocity: U [nx+1,ny,nz], V [nx,ny+1,nz], W [nx,ny,nz+1]; the face
+ *    array of an axis at index I is the LOWER face of cell I
+ *    (packages/fluxflow/src/grid/grid_data3.js:121-176)
+ *  - the library owns every field it creates; objects bind the fields they
+ *    are created with ("create once, solve many", linalg.js:42-53)
+ *  - every call returns FLUX_OK (0) or an error code; flux_last_error() gives
+ *    the message.  Construction errors keep the reference's messages.
+ *    Numerical failure never returns an error: it is reported through the
+ *    state/diagnostics structs (linalg.js:1056-1103).
+ *  - all work is ordered on one HIP stream owned by the library; calls are
+ *    blocking unless documented otherwise, and must not interleave.
+ */
+#ifndef FLUXFLOW_HIP_H
+#define FLUXFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLUX_OK 0
+#define FLUX_ERR_ARG 1     /* invalid argument / shape / type (reference: throw) */
+#define FLUX_ERR_HIP 2     /* HIP runtime error */
+#define FLUX_ERR_STATE 3   /* not initialised, wrong object state */
+#define FLUX_ERR_NOMEM 4
+
+/* Last error message of the calling thread ("" when none). */
+const char* flux_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+/* replaces tsl_array_n init() / getRenderer()  (packages/tsl_array_n/src/context.js:74-144):
+ * selects the device and creates the library's stream. */
+int flux_init(int device);
+/* releases every library object still alive and the stream */
+int flux_shutdown(void);
+/* waits for all queued work (the reference's awaited readbacks) */
+int flux_sync(void);
+/* device name and compute-unit count */
+int flux_device_info(char* name, size_t name_len, int* compute_units);
+
+/* ----------------------------------------------------------------- fields */
+/* element types: arrayN(type, shape)  (packages/tsl_array_n/src/array.js:94-133).
+ * 'float' -> FLUX_F32, 'int' -> FLUX_I32, 'vec3' -> FLUX_VEC3 (3 x f32 per
+ * element); FLUX_U8 holds masks, FLUX_F64/F16/BF16 are internal storage. */
+enum {
+    FLUX_F32 = 0,
+    FLUX_I32 = 1,
+    FLUX_U8 = 2,
+    FLUX_F64 = 3,
+    FLUX_F16 = 4,
+    FLUX_BF16 = 5,
+    FLUX_VEC3 = 6
+};
+typedef struct flux_field flux_field;
+/* arrayN(type, shape): ndim 1..3 */
+int flux_field_create(int dtype, int ndim, const int64_t* shape, flux_field** out);
+int flux_field_destroy(flux_field* f);
+/* fromArray(typed)  (array.js:124-129); bytes must equal the field's size */
+int flux_field_upload(flux_field* f, const void* src, size_t bytes);
+/* toArray()  (array.js:114-122) */
+int flux_field_download(const flux_field* f, void* dst, size_t bytes);
+/* fills every element with value (converted to the element type) */
+int flux_field_fill(flux_field* f, double value);
+/* device-to-device copy of equal-sized fields (createCopyKernel3) */
+int flux_field_copy(flux_field* dst, const flux_field* src);
+/* .shape / .count / .type  (array.js:94-112) */
+int flux_field_info(const flux_field* f, int* dtype, int* ndim, int64_t* shape, int64_t* count);
+/* raw device address, for bindings that share device memory */
+void* flux_field_data(const flux_field* f);
+
+/* ------------------------------------------------------ Laplacian operator */
+/* createLaplacianOperator(shape, gridSpacing, {dirichletMask, faceWeights})
+ * (packages/fluxflow/src/linalg/multigrid.js:457-471, stencil laplacianAt
+ * 273-339).  ndim 2 or 3; mask: FLUX_U8 cells (1 = Dirichlet) or NULL;
+ * face_weights: ndim FLUX_F32 face fields or NULL. */
+typedef struct flux_operator flux_operator;
+int flux_operator_create(int ndim, const int* shape, const double* spacing, const flux_field* mask,
+                         const flux_field* const* face_weights, flux_operator** out);
+/* applyLaplacian(input, output) */
+int flux_operator_apply(flux_operator* op, const flux_field* in, flux_field* out);
+/* the per-cell diagonal laplacianDiagonalAt (multigrid.js:399-437) */
+int flux_operator_diagonal(flux_operator* op, flux_field* out);
+int flux_operator_destroy(flux_operator* op);
+
+/* --------------------------------------------------------------- multigrid */
+/* V-cycle storage precision (no reference counterpart: the reference is FP32) */
+enum { FLUX_STORE_F32 = 0, FLUX_STORE_F16 = 1, FLUX_STORE_BF16 = 2 };
+/* createMultigridPreconditioner options (multigrid.js:1223-1250) */
+typedef struct {
+    int levels;        /* numberOfLevels, default 1 */
+    int nu_down;       /* numberOfSmoothingIterationsDown, default 2 */
+    int nu_up;         /* numberOfSmoothingIterationsUp, default 2 */
+    int n_coarsest;    /* numberOfCoarsestIterations, default 20 */
+    int n_final;       /* numberOfFinalIterations, default 2 */
+    double omega;      /* sorFactor, default 1.0 */
+    int storage;       /* FLUX_STORE_*, default FLUX_STORE_F32 */
+} flux_mg_options;
+/* fills the reference defaults */
+void flux_mg_options_default(flux_mg_options* o);
+typedef struct flux_multigrid flux_multigrid;
+/* createMultigridPreconditioner(shape, gridSpacing, {dirichletMask,
+ * coarseDirichletMask, faceWeights, ...}) -- throws the reference's
+ * "1D/2D/3D", "gridSpacing length" and "not divisible" errors as FLUX_ERR_ARG. */
+int flux_multigrid_create(int ndim, const int* shape, const double* spacing, const flux_mg_options* opts,
+                          const flux_field* mask, const flux_field* coarse_mask,
+                          const flux_field* const* face_weights, flux_multigrid** out);
+/* refreshCoarseLevels() (multigrid.js:1415-1451): re-coarsen masks and face
+ * weights after the caller changed the fine ones */
+int flux_multigrid_refresh(flux_multigrid* mg);
+/* applyMultigridPreconditioner(input, output): one V-cycle from output = 0 */
+int flux_multigrid_apply(flux_multigrid* mg, const flux_field* in, flux_field* out);
+/* copies level `level`'s coarse mask (u8) and face weights (f32, per axis)
+ * to host buffers; any pointer may be NULL.  Shape in shape_out[3]. */
+int flux_multigrid_level(flux_multigrid* mg, int level, int* shape_out, uint8_t* mask_out,
+                         float* const* weights_out);
+int flux_multigrid_destroy(flux_multigrid* mg);
+
+/* --------------------------------------------------------------------- PCG */
+enum { FLUX_PREC_MULTIGRID = 0, FLUX_PREC_JACOBI = 1, FLUX_PREC_NONE = 2 };
+/* stoppedBy codes (linalg.js:1050-1103, 'residual-growing' 2076) */
+enum {
+    FLUX_STOP_NONE = 0,
+    FLUX_STOP_DEGENERATE_PAP = 1,
+    FLUX_STOP_PAP_GROWTH = 2,
+    FLUX_STOP_ALPHA_MAGNITUDE = 3,
+    FLUX_STOP_DEGENERATE_OLD_RZ = 4,
+    FLUX_STOP_CONVERGED = 5,
+    FLUX_STOP_RESIDUAL_GROWING = 6
+};
+/* the solver's `state` (linalg.js:879) plus the FP64 acceptance check */
+typedef struct {
+    int iterations;
+    int stopped_by;          /* FLUX_STOP_* */
+    int converged;
+    double residual;         /* ||b - A x||_2 of the last true residual (FP64) */
+    double threshold;        /* max(tol*||b||, tol) when relative, else tol */
+    double b_norm;
+} flux_pcg_state;
+typedef struct flux_pcg flux_pcg;
+/* createPreconditionedConjugateGradientSolver(applyOperator, applyPreconditioner, b, x)
+ * (linalg.js:782-2469).  The preconditioner is picked by `precond`; mg is
+ * required for FLUX_PREC_MULTIGRID.  b and x are FLUX_F32 fields of the
+ * operator's shape.  x is the warm start and receives the solution. */
+int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_field* b, flux_field* x,
+                    flux_pcg** out);
+/* solve(tol, maxiter, ..., relativeTolerance)  (linalg.js:2176): the
+ * solution is accumulated in FP64, every iteration recomputes the true
+ * residual b - A x in FP64 arithmetic, dots are FP64, all scalars stay on
+ * the GPU; the host reads one scalar block per chunk of iterations. */
+int flux_pcg_solve(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* state);
+int flux_pcg_destroy(flux_pcg* s);
+
+/* ---------------------------------------------------- pressure projection */
+/* createGridPressureSolver3 options (grid_pressure_solver3.js:66-86).
+ * dirichlet(pos) of the reference is a shader function of position; here
+ * the caller passes its evaluation per cell: active (u8) and target (f32). */
+typedef struct {
+    int resolution[3];
+    double spacing[3];
+    double origin[3];
+    const flux_field* dirichlet_active;    /* FLUX_U8 cells or NULL */
+    const flux_field* dirichlet_target;    /* FLUX_F32 cells or NULL (target 0) */
+    const flux_field* face_weights[3];     /* density beta per face, or NULLs */
+    const flux_field* collider_weights[3]; /* open fraction per face, or NULLs */
+    flux_mg_options multigrid;
+    int preconditioner;                    /* FLUX_PREC_*, default multigrid */
+    double tolerance;                      /* default 1e-6 */
+    int max_iterations;                    /* <= 0: ceil(factor * max(res)) */
+    double max_iterations_factor;          /* default 1.5 */
+    int relative_tolerance;                /* default 1 */
+    double max_plausible_pressure;         /* default 1e6 */
+} flux_pressure3_desc;
+void flux_pressure3_desc_default(flux_pressure3_desc* d);
+/* diagnostics (grid_pressure_solver3.js:287, 526-551) */
+typedef struct {
+    int converged;
+    int rejected;
+    int iterations;
+    int stopped_by;
+    double residual;
+    double b_norm;
+    int64_t bad_cells;
+} flux_pressure_diag;
+typedef struct flux_pressure3 flux_pressure3;
+int flux_pressure3_create(const flux_pressure3_desc* d, flux_pressure3** out);
+/* project(input, output) + dispatch() (grid_pressure_solver3.js:339-558).
+ * The six velocity fields are FLUX_F32 MAC arrays; output may alias input. */
+int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in,
+                           const flux_field* w_in, flux_field* u_out, flux_field* v_out,
+                           flux_field* w_out, flux_pressure_diag* diag);
+/* .pressure.data, .b, .dirichletMask, .ventMask (grid_pressure_solver3.js:572) */
+flux_field* flux_pressure3_field(flux_pressure3* p, int which);
+enum { FLUX_P3_PRESSURE = 0, FLUX_P3_RHS = 1, FLUX_P3_MASK = 2, FLUX_P3_VENT = 3, FLUX_P3_TARGET = 4,
+       FLUX_P3_WEIGHT_U = 5, FLUX_P3_WEIGHT_V = 6, FLUX_P3_WEIGHT_W = 7 };
+int flux_pressure3_destroy(flux_pressure3* p);
+
+/* ------------------------------------------------- particles and transfers */
+typedef struct {
+    int n[3];          /* cells */
+    float h[3];        /* spacing */
+    float origin[3];
+} flux_grid3;
+
+/* cellIndexOf + a stable sort by cell (grid_flip_solver3.js:227-235): writes
+ * cell[p], perm[rank] = particle id (stable), cell_start[c] (n_cells+1
+ * entries).  Particles must lie inside the domain; the count of those that
+ * do not is returned in *outside (they are binned to the clamped cell). */
+int flux_bin_particles(const flux_grid3* g, const flux_field* positions, int64_t count,
+                       flux_field* cell, flux_field* perm, flux_field* cell_start, int64_t* outside);
+/* P2G of one component by a deterministic gather over cell-sorted particles
+ * (the float counterpart of buildScatter + buildFinalize,
+ * grid_flip_solver3.js:508-564): out/valid are the component's face fields. */
+int flux_p2g_gather(const flux_grid3* g, const flux_field* positions, const flux_field* velocities,
+                    const flux_field* perm, const flux_field* cell_start, int component, float eps,
+                    flux_field* out, flux_field* valid);
+/* P2G of one component with the reference's int32 fixed-point atomics
+ * (scale p2gAtomicScale, default 65536): bit-identical to the reference's
+ * arithmetic.  numer/denom are FLUX_I32 face scratch fields (cleared here). */
+int flux_p2g_fixed(const flux_grid3* g, const flux_field* positions, const flux_field* velocities,
+                   int64_t count, int component, float scale, float eps, flux_field* numer,
+                   flux_field* denom, flux_field* out, flux_field* valid);
+/* g2pUpdate (grid_flip_solver3.js:903-922) */
+int flux_g2p(const flux_grid3* g, const flux_field* positions, flux_field* velocities, int64_t count,
+             const flux_field* const* vel_new, const flux_field* const* vel_old, float flip_ratio,
+             float damping);
+/* advectParticles (grid_flip_solver3.js:933-940), no collider */
+int flux_advect_particles(const flux_grid3* g, flux_field* positions, int64_t count,
+                          const flux_field* const* vel, float dt, int max_substeps);
+/* advectFaceCentered3, order 1, no collider (advection_solver3.js:130-176) */
+int flux_advect_faces(const flux_grid3* g, const flux_field* const* vel_in, flux_field* const* vel_out,
+                      float dt, int max_substeps);
+/* liquid fraction per cell from sorted particles (count > 0 -> 1), then
+ * two-phase face beta = rho_ref / max(rho_face, 1e-4)
+ * (grid_two_phase_flip_solver2.js:693-762, in 3-D) */
+int flux_two_phase_beta(const flux_grid3* g, const flux_field* cell_start, float rho_gas, float rho_liquid,
+                        flux_field* fraction, flux_field* const* beta);
+/* MAC divergence at cell centres (grid_math3.js:397-410) */
+int flux_divergence(const flux_grid3* g, const flux_field* const* vel, flux_field* out);
+
+/* ----------------------------------------------------------------- timing */
+/* HIP-event timing of the library stream */
+int flux_timer_start(void);
+int flux_timer_stop(double* ms);
+/* per-kernel event timing: when enabled, every launch of the kernel family
+ * `name` ("apply", "update", "relax", "residual", "restrict", "correct") on
+ * the finest level is bracketed by events; read returns the summed
+ * duration and the launch count, then resets. */
+int flux_kprof_enable(const char* name);
+int flux_kprof_read(double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

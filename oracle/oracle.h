@@ -179,6 +179,12 @@ void orc_two_phase_beta3(const float* liquid_fraction, const int n[3],
 ORC_DECL(double, f64)
 ORC_DECL(float, f32)
 
+/* the HIP plain path's arithmetic: f32 vectors, double solution accumulator,
+ * double true residual, double dots (see oracle_f32.c) */
+int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec,
+                        void* mg, const float* b, float* x, double* x64_out, double tol, int maxiter,
+                        int relative, orc_pcg_state* st);
+
 /* threads for the OpenMP arms (1 = the reference's serial model) */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -484,3 +484,28 @@ def two_phase_beta3(frac, n, rho_gas, rho_liquid):
     outs = [np.zeros(int(np.prod(face_shape(n, a))), np.float32) for a in range(3)]
     lib().orc_two_phase_beta3(_p(frac), _i3(n), C.c_float(rho_gas), C.c_float(rho_liquid), *[_p(o) for o in outs])
     return outs
+
+
+def pcg_solve_mixed(shape, spacing, b, x0=None, mask=None, weights=None, precond="multigrid", mg=None,
+                    tol=1e-6, maxiter=100, relative=True):
+    """The HIP plain path's arithmetic: f32 vectors, f64 accumulator and residual."""
+    g = make_grid(shape, spacing)
+    b = _real(b, "f32")
+    x = np.zeros_like(b) if x0 is None else _real(x0, "f32").copy()
+    x64 = np.zeros(x.size, np.float64)
+    mask = _u8(mask)
+    weights = None if weights is None else [_real(w, "f32") for w in weights]
+    keep, wp = _w(weights)
+    st = PCGState()
+    if mg is not None and mg.suffix != "f32":
+        raise ValueError("pcg_solve_mixed needs an f32 multigrid")
+    lib().orc_pcg_solve_mixed(C.byref(g), _p(mask), wp, PREC[precond], C.c_void_p(mg.h if mg is not None else None),
+                              _p(b), _p(x), _p(x64), C.c_double(tol), int(maxiter), int(bool(relative)), C.byref(st))
+    return x, x64, {
+        "iterations": st.iterations,
+        "stoppedBy": STOP_NAMES[st.stopped_by],
+        "converged": bool(st.converged),
+        "residual": float(np.sqrt(abs(st.residual_sq))),
+        "threshold": st.threshold,
+        "bNorm": st.b_norm,
+    }

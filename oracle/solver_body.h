@@ -446,9 +446,21 @@ void FN(orc_mg_apply)(void* p, const REAL* r, REAL* z) {
  * first iteration, which is what is done here).  Guards: linalg.js:226, 244,
  * 284, 547-551, 1095.
  * ------------------------------------------------------------------------ */
+/* fixed 64K-element chunks summed in order: the same value for any thread count */
 static double FN(dot)(const REAL* a, const REAL* b, int64_t n) {
+    const int64_t CH = 65536;
+    int64_t nch = (n + CH - 1) / CH;
+    double* part = (double*)malloc(sizeof(double) * (nch > 0 ? nch : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nch; ++c) {
+        double s = 0;
+        int64_t e = (c + 1) * CH < n ? (c + 1) * CH : n;
+        for (int64_t q = c * CH; q < e; ++q) s += (double)a[q] * (double)b[q];
+        part[c] = s;
+    }
     double s = 0;
-    for (int64_t q = 0; q < n; ++q) s += (double)a[q] * (double)b[q];
+    for (int64_t c = 0; c < nch; ++c) s += part[c];
+    free(part);
     return s;
 }
 

@@ -1,0 +1,135 @@
+"""createGridPressureSolver3 on the GPU against the oracle (GPU tests).
+
+The oracle composes the same stages (updateDirichletFields, build system,
+MGPCG in the HIP path's arithmetic, circuit breaker, correction) on the CPU.
+Tolerance: relative L2 <= 1e-5 for b, pressure and the corrected velocity;
+the mask and vent mask are exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tsl_array_n_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def oracle_project(sc, levels, tol, maxiter, collider=None):
+    n, h = sc["shape"], sc["spacing"]
+    A = O.pressure3_assemble(n, h, sc["vel"], active=sc["active"], target=sc["target"], beta=sc["beta"], cw=collider,
+                             suffix="f32")
+    mg = O.Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f32")
+    x, x64, st = O.pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], weights=A["weights"], mg=mg,
+                                   tol=tol, maxiter=maxiter)
+    out = O.pressure3_correct(n, h, x, sc["vel"], beta=sc["beta"], cw=collider, suffix="f32")
+    return A, x, x64, st, out
+
+
+def gpu_project(F, sc, levels, tol, collider=None, **kw):
+    n, h = sc["shape"], sc["spacing"]
+    vel = [F.array_n("float", F.face_shape(n, a), sc["vel"][a]) for a in range(3)]
+    beta = [F.array_n("float", F.face_shape(n, a), sc["beta"][a]) for a in range(3)]
+    cw = None
+    if collider is not None:
+        cw = [F.array_n("float", F.face_shape(n, a), collider[a]) for a in range(3)]
+    act = F.array_n("u8", n, sc["active"])
+    tgt = F.array_n("float", n, sc["target"])
+    ps = F.create_grid_pressure_solver3(resolution=n, grid_spacing=h, dirichlet=(act, tgt), face_weights=beta,
+                                        collider_weights=cw, multigrid={"levels": levels}, tolerance=tol, **kw)
+    diag = ps.project(vel)
+    return ps, vel, diag
+
+
+@pytest.mark.parametrize("n,levels", [(32, 4), (64, 5)])
+def test_dam_break_projection_matches_oracle(flux, n, levels):
+    F = flux
+    sc = scenes.dam_break_system(n)
+    ps, vel, diag = gpu_project(F, sc, levels, 1e-6)
+    A, x, x64, st, out = oracle_project(sc, levels, 1e-6, int(np.ceil(1.5 * n)))
+    print("gpu", diag, "oracle", st)
+    assert diag["converged"] and st["converged"]
+    assert not diag["rejected"]
+    assert abs(diag["iterations"] - st["iterations"]) <= 1
+    assert diag["residual"] <= 1e-6 * diag["bNorm"]
+    np.testing.assert_array_equal(ps.dirichlet_mask.to_array(), A["mask"])
+    np.testing.assert_array_equal(ps.vent_mask.to_array(), A["vent"])
+    assert rel(ps.b.to_array(), A["b"]) <= 1e-6
+    assert rel(ps.pressure.to_array(), x) <= 1e-5
+    for a in range(3):
+        assert rel(vel[a].to_array(), out[a]) <= 1e-5
+
+
+def test_random_scene_with_collider_matches_oracle(flux):
+    F = flux
+    n = 32
+    sc = scenes.random_scene(n, 3)
+    rng = np.random.default_rng(8)
+    # collider open fractions: a solid slab at x < 4 (weight 0) and partial faces around it
+    collider = []
+    for a in range(3):
+        fs = list((n, n, n))
+        fs[a] += 1
+        wgt = np.ones(fs[::-1], np.float32)
+        if a == 0:
+            wgt[:, :, :5] = 0.0
+            wgt[:, :, 5] = rng.uniform(0.01, 1.0, wgt[:, :, 5].shape)
+        else:
+            wgt[:, :, :4] = 0.0
+        collider.append(wgt.ravel())
+    ps, vel, diag = gpu_project(F, sc, 4, 1e-6, collider=collider)
+    A, x, x64, st, out = oracle_project(sc, 4, 1e-6, 48, collider=collider)
+    print("gpu", diag, "oracle", st)
+    assert diag["converged"] == st["converged"]
+    np.testing.assert_array_equal(ps.dirichlet_mask.to_array(), A["mask"])
+    assert rel(ps.b.to_array(), A["b"]) <= 1e-6
+    assert rel(ps.pressure.to_array(), x) <= 1e-5
+    for a in range(3):
+        assert rel(vel[a].to_array(), out[a]) <= 1e-5
+
+
+def test_circuit_breaker_restores_snapshot(flux):
+    """maxPlausiblePressure tiny -> rejected, pressure restored to the last
+    accepted snapshot (zeros), velocity corrected with it (grid_pressure_solver3.js:545-554)."""
+    F = flux
+    sc = scenes.dam_break_system(16)
+    ps, vel, diag = gpu_project(F, sc, 3, 1e-6, max_plausible_pressure=1e-12)
+    assert diag["rejected"] and diag["badCells"] > 0
+    p = ps.pressure.to_array()
+    assert np.all(p == 0)
+    for a in range(3):
+        np.testing.assert_array_equal(vel[a].to_array(), sc["vel"][a])
+
+
+def test_warm_start_second_projection_is_immediate(flux):
+    """Projecting the projected field again: b ~ 0 relative to the pinned
+    hydrostatic pressure, warm-started from it."""
+    F = flux
+    sc = scenes.dam_break_system(32)
+    ps, vel, d1 = gpu_project(F, sc, 4, 1e-6)
+    assert d1["converged"]
+    d2 = ps.project(vel)
+    assert d2["iterations"] <= 2
+
+
+def test_large_projection_reaches_tolerance(flux):
+    """128^3 two-phase (1000:1): the FP64 true residual of the solve meets
+    1e-6 relative, and the FP32 pressure the oracle recomputes sits at the FP32
+    representability floor, not above it."""
+    F = flux
+    n = 128
+    sc = scenes.dam_break_system(n)
+    ps, vel, diag = gpu_project(F, sc, 6, 1e-6)
+    print(diag)
+    assert diag["converged"] and diag["residual"] <= 1e-6 * diag["bNorm"]
+    p = ps.pressure.to_array()
+    A = O.pressure3_assemble(sc["shape"], sc["spacing"], sc["vel"], active=sc["active"], target=sc["target"],
+                             beta=sc["beta"], suffix="f32")
+    r32 = O.true_residual(sc["shape"], sc["spacing"], A["b"], p, A["mask"], A["weights"], suffix="f32")
+    print("f32-rounded pressure relative residual:", r32 / diag["bNorm"])
+    assert r32 / diag["bNorm"] < 1e-3

@@ -1,0 +1,10 @@
+// Internal entry points of the solver translation unit used by the others.
+#pragma once
+#include "flux_internal.h"
+
+namespace flux {
+int fill_device(void* p, int dtype, int64_t n, double v);
+int check_shape(int ndim, const int* shape, const double* spacing, int spacing_len, const char* who);
+int mg_refresh(flux_multigrid* mg);
+int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
+}  // namespace flux

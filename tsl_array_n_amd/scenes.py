@@ -1,0 +1,120 @@
+"""Seeded synthetic inputs for the projection workloads (host-side setup).
+
+The 3-D two-phase dam break at n^3 on the unit domain (h = 1/n):
+  * liquid block x < 0.4, y < 0.6 seeded with 2x2x2 jittered particles per
+    cell (computeFlipBoxSeed3, grid_flip_solver3.js:70-130).  The jitter
+    (<= 0.2 sub-spacing) keeps every particle inside its own cell, so the
+    occupied cells are exactly the block's round(0.4 n) x round(0.6 n) x n
+    cells: that is what the liquid fraction below uses (the GPU binning of the
+    seeded particles reproduces it, see the transfer tests);
+  * two-phase density rho = rho_gas + (rho_liq - rho_gas) * fraction, face
+    beta = rho_ref / max(rho_face, 1e-4) with the arithmetic face mean
+    (grid_two_phase_flip_solver2.js:693-762, in 3-D), ratio 1000:1;
+  * the first frame's velocity u* = P2G(particles at rest) + g dt on every
+    face, walls closed (zero normal velocity on the domain faces);
+  * the singular all-Neumann system pinned at the top-centre cell
+    (pressurePin 'topCenter', grid_two_phase_flip_solver2.js:133-169, 770-796).
+All float arithmetic is f32, in the order of the reference expressions.
+"""
+import numpy as np
+
+
+def js_round(v):
+    return int(np.floor(v + 0.5))
+
+
+def liquid_block_cells(n, box_max=(0.4, 0.6, 1.0)):
+    h = 1.0 / n
+    return tuple(max(1, js_round(b / h)) for b in box_max)
+
+
+def dam_break_fraction(n, box_max=(0.4, 0.6, 1.0)):
+    cx, cy, cz = liquid_block_cells(n, box_max)
+    frac = np.zeros((n, n, n), np.float32)  # [k][j][i]
+    frac[:cz, :cy, :cx] = 1.0
+    return frac.ravel()
+
+
+def particle_count(n, ppa=2, box_max=(0.4, 0.6, 1.0)):
+    cx, cy, cz = liquid_block_cells(n, box_max)
+    return cx * cy * cz * ppa ** 3
+
+
+def two_phase_beta(frac, shape, rho_gas=1.0, rho_liq=1000.0):
+    """face beta per axis, flat first-index-fastest arrays"""
+    nx, ny, nz = shape
+    f = frac.reshape(nz, ny, nx)
+    min_d = np.float32(1e-4)
+    r0 = max(np.float32(rho_gas), min_d)
+    r1 = max(np.float32(rho_liq), min_d)
+    ref = max(r0, r1)
+    rho = (r0 + (r1 - r0) * f).astype(np.float32)
+    out = []
+    for a in range(3):
+        ax = 2 - a  # numpy axis of grid axis a
+        m = rho.shape[ax]
+        idx_hi = np.clip(np.arange(m + 1), 0, m - 1)
+        idx_lo = np.clip(np.arange(m + 1) - 1, 0, m - 1)
+        ra = np.take(rho, idx_lo, axis=ax)
+        rb = np.take(rho, idx_hi, axis=ax)
+        rf = ((ra + rb) * np.float32(0.5)).astype(np.float32)
+        out.append((ref / np.maximum(rf, min_d)).astype(np.float32).ravel())
+    return out
+
+
+def dam_break_system(n, dt=1.0 / 60.0, g=-9.8, rho_gas=1.0, rho_liq=1000.0):
+    shape = (n, n, n)
+    frac = dam_break_fraction(n)
+    beta = two_phase_beta(frac, shape, rho_gas, rho_liq)
+    U = np.zeros((n, n, n + 1), np.float32)
+    V = np.full((n, n + 1, n), np.float32(g) * np.float32(dt), np.float32)
+    W = np.zeros((n + 1, n, n), np.float32)
+    V[:, 0, :] = 0.0
+    V[:, n, :] = 0.0
+    active = np.zeros((n, n, n), np.uint8)
+    active[n // 2, n - 1, n // 2] = 1
+    return {
+        "shape": shape,
+        "spacing": (1.0 / n,) * 3,
+        "fraction": frac,
+        "beta": beta,
+        "vel": [U.ravel(), V.ravel(), W.ravel()],
+        "active": active.ravel(),
+        "target": np.zeros(n ** 3, np.float32),
+        "particles": particle_count(n),
+    }
+
+
+def random_scene(n, seed):
+    """A 128^3-style randomized variable-coefficient scene: random spheres and
+    boxes of denser fluid, band-limited random divergence, closed walls."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    frac = np.zeros((n, n, n), np.float32)
+    for _ in range(rng.integers(1, 7)):
+        c = rng.uniform(0.2, 0.8, 3) * n
+        r = rng.uniform(6, 30) * n / 128.0
+        if rng.random() < 0.5:
+            inside = (i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2 < r * r
+        else:
+            inside = (np.abs(i - c[0]) < r) & (np.abs(j - c[1]) < r) & (np.abs(k - c[2]) < r)
+        frac[inside] = 1.0
+    ratio = float(np.exp(rng.uniform(0, np.log(1000.0))))
+    beta = two_phase_beta(frac.ravel(), shape, 1.0, ratio)
+    vel = [rng.standard_normal(int(np.prod(s))).astype(np.float32) for s in
+           ((n + 1) * n * n, n * (n + 1) * n, n * n * (n + 1))]
+    for a, v in enumerate(vel):
+        fs = [n, n, n]
+        fs[a] += 1
+        arr = v.reshape(fs[::-1])
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[2 - a] = 0
+        sl1[2 - a] = -1
+        arr[tuple(sl0)] = 0
+        arr[tuple(sl1)] = 0
+    active = np.zeros(n ** 3, np.uint8)
+    active[(n // 2) + n * ((n - 1) + n * (n // 2))] = 1
+    return {"shape": shape, "spacing": (1.0 / n,) * 3, "fraction": frac.ravel(), "beta": beta, "vel": vel,
+            "active": active, "target": np.zeros(n ** 3, np.float32), "ratio": ratio}
