@@ -1,0 +1,70 @@
+"""The Node.js binding (N-API addon + ESM module): the reference's own host
+language driving the HIP library."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from oracle import pyoracle as O
+
+NODE = shutil.which("node")
+ADDON = os.path.join(ROOT, "tsl_array_n_amd", "lib", "fluxflow_napi.node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+def run_node(script, *args, timeout=120):
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", script)] + [str(a) for a in args],
+                       capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def test_api_surface_and_host_helpers(tmp_path):
+    assert os.path.exists(ADDON), "build the addon with __graft_entry__.build()"
+    out = tmp_path / "api.json"
+    run_node("check_api.mjs", out)
+    d = json.load(open(out))
+    for name in ["init", "arrayN", "array0", "createLaplacianOperator", "createMultigridPreconditioner",
+                 "createJacobiPreconditioner", "createIdentityPreconditioner",
+                 "createPreconditionedConjugateGradientSolver", "createGridPressureSolver3",
+                 "createFaceCenteredGrid3", "computeFlipBoxSeed3", "createJitterRandom", "binParticles",
+                 "particleToGrid", "gridToParticle", "advectParticles", "advectFaceCentered3"]:
+        assert name in d["exports"], name
+    assert d["strides"] == [1, 4, 12]  # TA/test/array.test.js:41-93
+    assert "1D/2D/3D" in d["shapeError"]
+    assert "flux_init" in d["initError"]
+    golden = json.load(open(os.path.join(GOLDEN, "mulberry32.json")))
+    np.testing.assert_array_equal(d["random"], golden["vectors"][0]["values"][:16])
+    ref = O.flip_box_seed3([0, 0, 0], [0.4, 0.6, 1.0], [1 / 16] * 3)
+    assert d["seedCount"] == ref.size // 3
+    np.testing.assert_array_equal(np.array(d["seedHead"], np.float32), ref[:24])
+    assert abs(d["seedSum"] - ref.astype(np.float64).sum()) < 1e-6 * abs(d["seedSum"])
+
+
+@pytest.mark.gpu
+def test_projection_through_node_matches_oracle(tmp_path):
+    from tsl_array_n_amd import scenes
+    n = 16
+    r = run_node("projection.mjs", n, tmp_path, timeout=300)
+    print(r.stdout)
+    meta = json.load(open(tmp_path / "diag.json"))
+    assert meta["outside"] == 0 and meta["diag"]["converged"]
+    sc = scenes.dam_break_system(n)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "fraction.bin", np.float32), sc["fraction"])
+    A = O.pressure3_assemble(sc["shape"], sc["spacing"], sc["vel"], active=sc["active"], target=sc["target"],
+                             beta=sc["beta"], suffix="f32")
+    mg = O.Multigrid(sc["shape"], sc["spacing"], levels=3, mask=A["mask"], coarse_mask=A["vent"],
+                     weights=A["weights"], suffix="f32")
+    x, x64, st = O.pcg_solve_mixed(sc["shape"], sc["spacing"], A["b"], x0=A["pressure"], mask=A["mask"],
+                                   weights=A["weights"], mg=mg, tol=1e-6, maxiter=24)
+    p = np.fromfile(tmp_path / "pressure.bin", np.float32)
+    assert np.linalg.norm(p - x) <= 1e-5 * np.linalg.norm(x)
+    out = O.pressure3_correct(sc["shape"], sc["spacing"], x, sc["vel"], beta=sc["beta"], suffix="f32")
+    for a, name in enumerate("uvw"):
+        got = np.fromfile(tmp_path / (name + ".bin"), np.float32)
+        assert np.linalg.norm(got - out[a]) <= 1e-5 * max(np.linalg.norm(out[a]), 1e-30)

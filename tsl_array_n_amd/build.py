@@ -57,7 +57,32 @@ def build(verbose=False):
         os.remove(o)
     if verbose:
         print("built", LIB)
+    build_addon(verbose)
     return LIB
+
+
+NODE_INCLUDE = "/usr/include/node"
+ADDON = os.path.join(OUT, "fluxflow_napi.node")
+
+
+def build_addon(verbose=False):
+    """The N-API addon (node/fluxflow_napi.c) over libfluxhip.so; skipped
+    when the Node headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node headers not found: addon skipped")
+        return None
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-Wno-unused-parameter",
+           "-DNODE_GYP_MODULE_NAME=fluxflow_napi", "-I" + NODE_INCLUDE, "-include", "stdio.h",
+           os.path.join(HERE, "node", "fluxflow_napi.c"), "-o", ADDON + ".tmp",
+           "-L" + OUT, "-lfluxhip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("addon build failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(ADDON + ".tmp", ADDON)
+    if verbose:
+        print("built", ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

@@ -1,0 +1,629 @@
+/*
+ * N-API addon: the thin JavaScript binding of include/fluxflow_hip.h.
+ *
+ * Every export maps one C-ABI entry point; handles are napi externals whose
+ * finalizers release the native object (fields the library owns elsewhere,
+ * like a pressure solver's own pressure field, are wrapped without one).
+ * Errors become JavaScript exceptions carrying flux_last_error(), so the
+ * reference's messages ("not divisible", "shape mismatch", ...) surface
+ * unchanged.  The ESM module index.mjs builds the reference's API on top.
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/fluxflow_hip.h"
+
+#define CHECK_NAPI(call)                                                      \
+    do {                                                                      \
+        if ((call) != napi_ok) {                                              \
+            napi_throw_error(env, NULL, "fluxflow addon: N-API call failed"); \
+            return NULL;                                                      \
+        }                                                                     \
+    } while (0)
+
+#define CHECK_FLUX(call)                                      \
+    do {                                                      \
+        if ((call) != FLUX_OK) {                              \
+            napi_throw_error(env, NULL, flux_last_error());   \
+            return NULL;                                      \
+        }                                                     \
+    } while (0)
+
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3 };
+
+typedef struct {
+    int kind;
+    void* ptr;
+} handle_t;
+
+static void fin_handle(napi_env env, void* data, void* hint) {
+    (void)env;
+    (void)hint;
+    handle_t* h = (handle_t*)data;
+    if (!h) return;
+    switch (h->kind) {
+        case K_FIELD: flux_field_destroy((flux_field*)h->ptr); break;
+        case K_OP: flux_operator_destroy((flux_operator*)h->ptr); break;
+        case K_MG: flux_multigrid_destroy((flux_multigrid*)h->ptr); break;
+        case K_PCG: flux_pcg_destroy((flux_pcg*)h->ptr); break;
+        case K_P3: flux_pressure3_destroy((flux_pressure3*)h->ptr); break;
+        default: break;
+    }
+    free(h);
+}
+
+static napi_value wrap(napi_env env, int kind, void* ptr) {
+    handle_t* h = (handle_t*)malloc(sizeof(handle_t));
+    h->kind = kind;
+    h->ptr = ptr;
+    napi_value v;
+    if (napi_create_external(env, h, fin_handle, NULL, &v) != napi_ok) return NULL;
+    return v;
+}
+
+static void* unwrap(napi_env env, napi_value v, int kind_a, int kind_b) {
+    napi_valuetype t;
+    if (napi_typeof(env, v, &t) != napi_ok || t != napi_external) return NULL;
+    handle_t* h = NULL;
+    if (napi_get_value_external(env, v, (void**)&h) != napi_ok || !h) return NULL;
+    if (h->kind != kind_a && h->kind != kind_b) return NULL;
+    return h->ptr;
+}
+
+static int is_nullish(napi_env env, napi_value v) {
+    napi_valuetype t;
+    if (napi_typeof(env, v, &t) != napi_ok) return 1;
+    return t == napi_null || t == napi_undefined;
+}
+
+static flux_field* get_field(napi_env env, napi_value v) { return (flux_field*)unwrap(env, v, K_FIELD, K_FIELD_VIEW); }
+
+static int get_int(napi_env env, napi_value v, int dflt) {
+    int32_t x;
+    if (napi_get_value_int32(env, v, &x) != napi_ok) return dflt;
+    return x;
+}
+
+static double get_double(napi_env env, napi_value v, double dflt) {
+    double x;
+    if (napi_get_value_double(env, v, &x) != napi_ok) return dflt;
+    return x;
+}
+
+static napi_value prop(napi_env env, napi_value obj, const char* name) {
+    napi_value v = NULL;
+    bool has = false;
+    if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return NULL;
+    napi_get_named_property(env, obj, name, &v);
+    return v;
+}
+
+static int get_int_array(napi_env env, napi_value arr, int* out, int max) {
+    uint32_t n = 0;
+    if (napi_get_array_length(env, arr, &n) != napi_ok) return -1;
+    if ((int)n > max) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value e;
+        napi_get_element(env, arr, i, &e);
+        out[i] = get_int(env, e, 0);
+    }
+    return (int)n;
+}
+
+static int get_double_array(napi_env env, napi_value arr, double* out, int max) {
+    uint32_t n = 0;
+    if (napi_get_array_length(env, arr, &n) != napi_ok) return -1;
+    if ((int)n > max) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value e;
+        napi_get_element(env, arr, i, &e);
+        out[i] = get_double(env, e, 0.0);
+    }
+    return (int)n;
+}
+
+/* array of up to 3 fields, or null -> returns count, fills out */
+static int get_field_array(napi_env env, napi_value arr, const flux_field** out) {
+    if (is_nullish(env, arr)) return 0;
+    uint32_t n = 0;
+    if (napi_get_array_length(env, arr, &n) != napi_ok || n > 3) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value e;
+        napi_get_element(env, arr, i, &e);
+        out[i] = get_field(env, e);
+        if (!out[i]) return -1;
+    }
+    return (int)n;
+}
+
+static napi_value args_n(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+    size_t argc = want;
+    napi_value self;
+    if (napi_get_cb_info(env, info, &argc, argv, &self, NULL) != napi_ok) return NULL;
+    for (size_t i = argc; i < want; ++i) napi_get_undefined(env, &argv[i]);
+    return self;
+}
+
+static napi_value undef(napi_env env) {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    return u;
+}
+
+static napi_value num(napi_env env, double x) {
+    napi_value v;
+    napi_create_double(env, x, &v);
+    return v;
+}
+
+/* ------------------------------------------------------------- context */
+static napi_value js_init(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    CHECK_FLUX(flux_init(is_nullish(env, a[0]) ? 0 : get_int(env, a[0], 0)));
+    return undef(env);
+}
+
+static napi_value js_sync(napi_env env, napi_callback_info info) {
+    (void)info;
+    CHECK_FLUX(flux_sync());
+    return undef(env);
+}
+
+static napi_value js_device_info(napi_env env, napi_callback_info info) {
+    (void)info;
+    char name[256];
+    int cus = 0;
+    CHECK_FLUX(flux_device_info(name, sizeof name, &cus));
+    napi_value o, s;
+    napi_create_object(env, &o);
+    napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &s);
+    napi_set_named_property(env, o, "name", s);
+    napi_set_named_property(env, o, "computeUnits", num(env, cus));
+    return o;
+}
+
+static napi_value js_last_error(napi_env env, napi_callback_info info) {
+    (void)info;
+    napi_value s;
+    napi_create_string_utf8(env, flux_last_error(), NAPI_AUTO_LENGTH, &s);
+    return s;
+}
+
+/* -------------------------------------------------------------- fields */
+static napi_value js_field_create(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    int dtype = get_int(env, a[0], -1);
+    int shp[3];
+    int nd = get_int_array(env, a[1], shp, 3);
+    if (nd <= 0) {
+        napi_throw_error(env, NULL, "arrayN: shape must be an array of 1..3 positive integers");
+        return NULL;
+    }
+    int64_t s64[3] = {1, 1, 1};
+    for (int i = 0; i < nd; ++i) s64[i] = shp[i];
+    flux_field* f = NULL;
+    CHECK_FLUX(flux_field_create(dtype, nd, s64, &f));
+    return wrap(env, K_FIELD, f);
+}
+
+static napi_value js_field_upload(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = get_field(env, a[0]);
+    void* data = NULL;
+    size_t len = 0;
+    napi_typedarray_type tt;
+    napi_value ab;
+    size_t off = 0;
+    if (!f || napi_get_typedarray_info(env, a[1], &tt, &len, &data, &ab, &off) != napi_ok) {
+        napi_throw_type_error(env, NULL, "fromArray: expected (field, TypedArray)");
+        return NULL;
+    }
+    size_t esize = (tt == napi_float64_array) ? 8 : (tt == napi_uint8_array || tt == napi_int8_array ||
+                                                     tt == napi_uint8_clamped_array) ? 1
+                                                   : (tt == napi_int16_array || tt == napi_uint16_array) ? 2 : 4;
+    CHECK_FLUX(flux_field_upload(f, data, len * esize));
+    return undef(env);
+}
+
+static napi_value js_field_download(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = get_field(env, a[0]);
+    void* data = NULL;
+    size_t len = 0;
+    napi_typedarray_type tt;
+    napi_value ab;
+    size_t off = 0;
+    if (!f || napi_get_typedarray_info(env, a[1], &tt, &len, &data, &ab, &off) != napi_ok) {
+        napi_throw_type_error(env, NULL, "toArray: expected (field, TypedArray)");
+        return NULL;
+    }
+    size_t esize = (tt == napi_float64_array) ? 8 : (tt == napi_uint8_array || tt == napi_int8_array ||
+                                                     tt == napi_uint8_clamped_array) ? 1
+                                                   : (tt == napi_int16_array || tt == napi_uint16_array) ? 2 : 4;
+    CHECK_FLUX(flux_field_download(f, data, len * esize));
+    return a[1];
+}
+
+static napi_value js_field_fill(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = get_field(env, a[0]);
+    if (!f) {
+        napi_throw_type_error(env, NULL, "fill: expected a field");
+        return NULL;
+    }
+    CHECK_FLUX(flux_field_fill(f, get_double(env, a[1], 0.0)));
+    return undef(env);
+}
+
+static napi_value js_field_copy(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* d = get_field(env, a[0]);
+    flux_field* s = get_field(env, a[1]);
+    if (!d || !s) {
+        napi_throw_type_error(env, NULL, "copy: expected two fields");
+        return NULL;
+    }
+    CHECK_FLUX(flux_field_copy(d, s));
+    CHECK_FLUX(flux_sync());
+    return undef(env);
+}
+
+/* ------------------------------------------------------------ operator */
+static napi_value js_operator_create(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    int shp[3];
+    double sp[3];
+    int nd = get_int_array(env, a[0], shp, 3);
+    int ns = get_double_array(env, a[1], sp, 3);
+    const flux_field* w[3] = {NULL, NULL, NULL};
+    int nw = get_field_array(env, a[3], w);
+    if (nd < 0 || ns < 0 || nw < 0) {
+        napi_throw_type_error(env, NULL, "createLaplacianOperator: bad arguments");
+        return NULL;
+    }
+    flux_operator* op = NULL;
+    CHECK_FLUX(flux_operator_create(nd, shp, sp, is_nullish(env, a[2]) ? NULL : get_field(env, a[2]),
+                                    nw ? w : NULL, &op));
+    return wrap(env, K_OP, op);
+}
+
+static napi_value js_operator_apply(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    flux_operator* op = (flux_operator*)unwrap(env, a[0], K_OP, K_OP);
+    CHECK_FLUX(flux_operator_apply(op, get_field(env, a[1]), get_field(env, a[2])));
+    return undef(env);
+}
+
+/* ----------------------------------------------------------- multigrid */
+static void mg_opts(napi_env env, napi_value o, flux_mg_options* m) {
+    flux_mg_options_default(m);
+    if (is_nullish(env, o)) return;
+    napi_value v;
+    if ((v = prop(env, o, "numberOfLevels"))) m->levels = get_int(env, v, m->levels);
+    if ((v = prop(env, o, "numberOfSmoothingIterationsDown"))) m->nu_down = get_int(env, v, m->nu_down);
+    if ((v = prop(env, o, "numberOfSmoothingIterationsUp"))) m->nu_up = get_int(env, v, m->nu_up);
+    if ((v = prop(env, o, "numberOfCoarsestIterations"))) m->n_coarsest = get_int(env, v, m->n_coarsest);
+    if ((v = prop(env, o, "numberOfFinalIterations"))) m->n_final = get_int(env, v, m->n_final);
+    if ((v = prop(env, o, "sorFactor"))) m->omega = get_double(env, v, m->omega);
+    if ((v = prop(env, o, "storage"))) m->storage = get_int(env, v, m->storage);
+}
+
+static napi_value js_multigrid_create(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    args_n(env, info, 6, a);
+    int shp[3];
+    double sp[3];
+    int nd = get_int_array(env, a[0], shp, 3);
+    int ns = get_double_array(env, a[1], sp, 3);
+    const flux_field* w[3] = {NULL, NULL, NULL};
+    int nw = get_field_array(env, a[5], w);
+    if (nd < 0 || ns < 0 || nw < 0) {
+        napi_throw_type_error(env, NULL, "multigrid: bad arguments");
+        return NULL;
+    }
+    if (nd != ns) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "multigrid: gridSpacing length (%d) must match shape length (%d).", ns, nd);
+        napi_throw_error(env, NULL, msg);
+        return NULL;
+    }
+    flux_mg_options m;
+    mg_opts(env, a[2], &m);
+    flux_multigrid* mg = NULL;
+    CHECK_FLUX(flux_multigrid_create(nd, shp, sp, &m, is_nullish(env, a[3]) ? NULL : get_field(env, a[3]),
+                                     is_nullish(env, a[4]) ? NULL : get_field(env, a[4]), nw ? w : NULL, &mg));
+    return wrap(env, K_MG, mg);
+}
+
+static napi_value js_multigrid_apply(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    CHECK_FLUX(flux_multigrid_apply((flux_multigrid*)unwrap(env, a[0], K_MG, K_MG), get_field(env, a[1]),
+                                    get_field(env, a[2])));
+    return undef(env);
+}
+
+static napi_value js_multigrid_refresh(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    CHECK_FLUX(flux_multigrid_refresh((flux_multigrid*)unwrap(env, a[0], K_MG, K_MG)));
+    return undef(env);
+}
+
+/* ----------------------------------------------------------------- PCG */
+static const char* stop_name(int s) {
+    switch (s) {
+        case FLUX_STOP_DEGENERATE_PAP: return "degenerate-pAp";
+        case FLUX_STOP_PAP_GROWTH: return "pAp-growth";
+        case FLUX_STOP_ALPHA_MAGNITUDE: return "alpha-magnitude";
+        case FLUX_STOP_DEGENERATE_OLD_RZ: return "degenerate-oldRZ";
+        case FLUX_STOP_CONVERGED: return "converged";
+        case FLUX_STOP_RESIDUAL_GROWING: return "residual-growing";
+        default: return "none";
+    }
+}
+
+static napi_value js_pcg_create(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    flux_pcg* s = NULL;
+    CHECK_FLUX(flux_pcg_create((flux_operator*)unwrap(env, a[0], K_OP, K_OP), get_int(env, a[1], 0),
+                               is_nullish(env, a[2]) ? NULL : (flux_multigrid*)unwrap(env, a[2], K_MG, K_MG),
+                               get_field(env, a[3]), get_field(env, a[4]), &s));
+    return wrap(env, K_PCG, s);
+}
+
+static napi_value js_pcg_solve(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_pcg_state st;
+    bool rel = false;
+    napi_get_value_bool(env, a[3], &rel);
+    CHECK_FLUX(flux_pcg_solve((flux_pcg*)unwrap(env, a[0], K_PCG, K_PCG), get_double(env, a[1], 1e-6),
+                              get_int(env, a[2], 100), rel ? 1 : 0, &st));
+    napi_value o, s, b;
+    napi_create_object(env, &o);
+    napi_set_named_property(env, o, "iterations", num(env, st.iterations));
+    napi_create_string_utf8(env, stop_name(st.stopped_by), NAPI_AUTO_LENGTH, &s);
+    napi_set_named_property(env, o, "stoppedBy", s);
+    napi_get_boolean(env, st.converged != 0, &b);
+    napi_set_named_property(env, o, "converged", b);
+    napi_set_named_property(env, o, "residualSquared", num(env, st.residual * st.residual));
+    napi_set_named_property(env, o, "residual", num(env, st.residual));
+    napi_set_named_property(env, o, "threshold", num(env, st.threshold));
+    napi_set_named_property(env, o, "bNorm", num(env, st.b_norm));
+    return o;
+}
+
+/* ------------------------------------------------------------ pressure */
+static napi_value js_pressure3_create(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    napi_value o = a[0];
+    flux_pressure3_desc d;
+    flux_pressure3_desc_default(&d);
+    napi_value v;
+    int tmp[3];
+    double dt[3];
+    if ((v = prop(env, o, "resolution")) && get_int_array(env, v, tmp, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.resolution[i] = tmp[i];
+    if ((v = prop(env, o, "gridSpacing")) && get_double_array(env, v, dt, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.spacing[i] = dt[i];
+    if ((v = prop(env, o, "origin")) && get_double_array(env, v, dt, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.origin[i] = dt[i];
+    if ((v = prop(env, o, "dirichletActive")) && !is_nullish(env, v)) d.dirichlet_active = get_field(env, v);
+    if ((v = prop(env, o, "dirichletTarget")) && !is_nullish(env, v)) d.dirichlet_target = get_field(env, v);
+    if ((v = prop(env, o, "faceWeights")) && get_field_array(env, v, d.face_weights) < 0) {
+        napi_throw_type_error(env, NULL, "createGridPressureSolver3: faceWeights must be three fields");
+        return NULL;
+    }
+    if ((v = prop(env, o, "colliderWeights")) && get_field_array(env, v, d.collider_weights) < 0) {
+        napi_throw_type_error(env, NULL, "createGridPressureSolver3: colliderWeights must be three fields");
+        return NULL;
+    }
+    if ((v = prop(env, o, "multigrid"))) mg_opts(env, v, &d.multigrid);
+    if ((v = prop(env, o, "preconditionerKind"))) d.preconditioner = get_int(env, v, d.preconditioner);
+    if ((v = prop(env, o, "tolerance"))) d.tolerance = get_double(env, v, d.tolerance);
+    if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.max_iterations = get_int(env, v, 0);
+    if ((v = prop(env, o, "maxIterationsFactor"))) d.max_iterations_factor = get_double(env, v, d.max_iterations_factor);
+    if ((v = prop(env, o, "relativeTolerance"))) {
+        bool b = true;
+        napi_get_value_bool(env, v, &b);
+        d.relative_tolerance = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "maxPlausiblePressure"))) d.max_plausible_pressure = get_double(env, v, d.max_plausible_pressure);
+    flux_pressure3* p = NULL;
+    CHECK_FLUX(flux_pressure3_create(&d, &p));
+    return wrap(env, K_P3, p);
+}
+
+static napi_value js_pressure3_field(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = flux_pressure3_field((flux_pressure3*)unwrap(env, a[0], K_P3, K_P3), get_int(env, a[1], 0));
+    if (!f) {
+        napi_value n;
+        napi_get_null(env, &n);
+        return n;
+    }
+    return wrap(env, K_FIELD_VIEW, f);
+}
+
+static napi_value js_pressure3_project(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    const flux_field* in[3] = {NULL, NULL, NULL};
+    const flux_field* out[3] = {NULL, NULL, NULL};
+    if (get_field_array(env, a[1], in) != 3 || get_field_array(env, a[2], out) != 3) {
+        napi_throw_type_error(env, NULL, "project: expected (solver, [u,v,w], [u,v,w])");
+        return NULL;
+    }
+    flux_pressure_diag dg;
+    CHECK_FLUX(flux_pressure3_project((flux_pressure3*)unwrap(env, a[0], K_P3, K_P3), in[0], in[1], in[2],
+                                      (flux_field*)out[0], (flux_field*)out[1], (flux_field*)out[2], &dg));
+    napi_value o, s, b1, b2;
+    napi_create_object(env, &o);
+    napi_get_boolean(env, dg.converged != 0, &b1);
+    napi_get_boolean(env, dg.rejected != 0, &b2);
+    napi_set_named_property(env, o, "converged", b1);
+    napi_set_named_property(env, o, "rejected", b2);
+    napi_set_named_property(env, o, "iterations", num(env, dg.iterations));
+    napi_create_string_utf8(env, stop_name(dg.stopped_by), NAPI_AUTO_LENGTH, &s);
+    napi_set_named_property(env, o, "stoppedBy", s);
+    napi_set_named_property(env, o, "residual", num(env, dg.residual));
+    napi_set_named_property(env, o, "bNorm", num(env, dg.b_norm));
+    napi_set_named_property(env, o, "badCells", num(env, (double)dg.bad_cells));
+    return o;
+}
+
+/* ------------------------------------------------------------ transfers */
+static int get_grid(napi_env env, napi_value o, flux_grid3* g) {
+    int n[3];
+    double h[3] = {1, 1, 1}, org[3] = {0, 0, 0};
+    napi_value v;
+    if (!(v = prop(env, o, "n")) || get_int_array(env, v, n, 3) != 3) return -1;
+    if ((v = prop(env, o, "h"))) get_double_array(env, v, h, 3);
+    if ((v = prop(env, o, "origin"))) get_double_array(env, v, org, 3);
+    for (int i = 0; i < 3; ++i) {
+        g->n[i] = n[i];
+        g->h[i] = (float)h[i];
+        g->origin[i] = (float)org[i];
+    }
+    return 0;
+}
+
+static napi_value js_bin_particles(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    args_n(env, info, 6, a);
+    flux_grid3 g;
+    if (get_grid(env, a[0], &g)) {
+        napi_throw_type_error(env, NULL, "binParticles: bad grid");
+        return NULL;
+    }
+    int64_t outside = 0;
+    CHECK_FLUX(flux_bin_particles(&g, get_field(env, a[1]), (int64_t)get_double(env, a[2], 0), get_field(env, a[3]),
+                                  get_field(env, a[4]), get_field(env, a[5]), &outside));
+    return num(env, (double)outside);
+}
+
+static napi_value js_p2g_gather(napi_env env, napi_callback_info info) {
+    napi_value a[9];
+    args_n(env, info, 9, a);
+    flux_grid3 g;
+    if (get_grid(env, a[0], &g)) {
+        napi_throw_type_error(env, NULL, "p2g: bad grid");
+        return NULL;
+    }
+    CHECK_FLUX(flux_p2g_gather(&g, get_field(env, a[1]), get_field(env, a[2]), get_field(env, a[3]),
+                               get_field(env, a[4]), get_int(env, a[5], 0), (float)get_double(env, a[6], 1e-4),
+                               get_field(env, a[7]), get_field(env, a[8])));
+    return undef(env);
+}
+
+static napi_value js_g2p(napi_env env, napi_callback_info info) {
+    napi_value a[8];
+    args_n(env, info, 8, a);
+    flux_grid3 g;
+    const flux_field* vn[3];
+    const flux_field* vo[3];
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[4], vn) != 3 || get_field_array(env, a[5], vo) != 3) {
+        napi_throw_type_error(env, NULL, "g2p: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_g2p(&g, get_field(env, a[1]), get_field(env, a[2]), (int64_t)get_double(env, a[3], 0), vn, vo,
+                        (float)get_double(env, a[6], 0.97), (float)get_double(env, a[7], 0.02)));
+    return undef(env);
+}
+
+static napi_value js_advect_particles(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    args_n(env, info, 6, a);
+    flux_grid3 g;
+    const flux_field* v[3];
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[3], v) != 3) {
+        napi_throw_type_error(env, NULL, "advectParticles: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_advect_particles(&g, get_field(env, a[1]), (int64_t)get_double(env, a[2], 0), v,
+                                     (float)get_double(env, a[4], 0.0), get_int(env, a[5], 32)));
+    return undef(env);
+}
+
+static napi_value js_advect_faces(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    flux_grid3 g;
+    const flux_field* vi[3];
+    const flux_field* vo[3];
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[1], vi) != 3 || get_field_array(env, a[2], vo) != 3) {
+        napi_throw_type_error(env, NULL, "advectFaceCentered3: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_advect_faces(&g, vi, (flux_field* const*)vo, (float)get_double(env, a[3], 0.0),
+                                 get_int(env, a[4], 32)));
+    return undef(env);
+}
+
+static napi_value js_two_phase_beta(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    args_n(env, info, 6, a);
+    flux_grid3 g;
+    const flux_field* beta[3];
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[5], beta) != 3) {
+        napi_throw_type_error(env, NULL, "twoPhaseBeta: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_two_phase_beta(&g, is_nullish(env, a[1]) ? NULL : get_field(env, a[1]),
+                                   (float)get_double(env, a[2], 1.0), (float)get_double(env, a[3], 1000.0),
+                                   get_field(env, a[4]), (flux_field* const*)beta));
+    return undef(env);
+}
+
+#define EXPORT(name, fn)                                                   \
+    do {                                                                   \
+        napi_value f_;                                                     \
+        napi_create_function(env, name, NAPI_AUTO_LENGTH, fn, NULL, &f_);  \
+        napi_set_named_property(env, exports, name, f_);                   \
+    } while (0)
+
+static napi_value module_init(napi_env env, napi_value exports) {
+    EXPORT("init", js_init);
+    EXPORT("sync", js_sync);
+    EXPORT("deviceInfo", js_device_info);
+    EXPORT("lastError", js_last_error);
+    EXPORT("fieldCreate", js_field_create);
+    EXPORT("fieldUpload", js_field_upload);
+    EXPORT("fieldDownload", js_field_download);
+    EXPORT("fieldFill", js_field_fill);
+    EXPORT("fieldCopy", js_field_copy);
+    EXPORT("operatorCreate", js_operator_create);
+    EXPORT("operatorApply", js_operator_apply);
+    EXPORT("multigridCreate", js_multigrid_create);
+    EXPORT("multigridApply", js_multigrid_apply);
+    EXPORT("multigridRefresh", js_multigrid_refresh);
+    EXPORT("pcgCreate", js_pcg_create);
+    EXPORT("pcgSolve", js_pcg_solve);
+    EXPORT("pressure3Create", js_pressure3_create);
+    EXPORT("pressure3Field", js_pressure3_field);
+    EXPORT("pressure3Project", js_pressure3_project);
+    EXPORT("binParticles", js_bin_particles);
+    EXPORT("p2gGather", js_p2g_gather);
+    EXPORT("g2p", js_g2p);
+    EXPORT("advectParticles", js_advect_particles);
+    EXPORT("advectFaces", js_advect_faces);
+    EXPORT("twoPhaseBeta", js_two_phase_beta);
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, module_init)

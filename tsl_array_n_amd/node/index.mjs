@@ -1,0 +1,471 @@
+// tsl_array_n / fluxflow API on the MI355X library (ESM, Node >= 12).
+//
+// A drop-in for the reference's projection path: the same factory names,
+// argument meanings and error messages as
+//   packages/tsl_array_n/src/{context,array}.js            (init, arrayN, array0)
+//   packages/fluxflow/src/linalg/{multigrid,linalg}.js      (operator, preconditioners, PCG)
+//   packages/fluxflow/src/grid/grid_data3.js                (createFaceCenteredGrid3)
+//   packages/fluxflow/src/grid/grid_pressure_solver3.js     (createGridPressureSolver3)
+//   packages/fluxflow/src/grid/grid_flip_solver3.js:70-130  (computeFlipBoxSeed3)
+//   packages/fluxflow/src/grid/jitter_random.js             (createJitterRandom)
+// What the reference expresses as shader functions (a Dirichlet mask, face
+// weights, `dirichlet(pos)`) is passed here as fields, or as a JavaScript
+// function evaluated per cell on the host.
+import { createRequire } from 'module';
+
+const require = createRequire( import.meta.url );
+const addon = require( '../lib/fluxflow_napi.node' );
+
+const TYPE_CODES = { float: 0, int: 1, u8: 2, f64: 3, vec3: 6 };
+const TYPED = { float: Float32Array, int: Int32Array, u8: Uint8Array, f64: Float64Array, vec3: Float32Array };
+const PRECONDITIONERS = { multigrid: 0, jacobi: 1, none: 2 };
+
+export const DEFAULT_JITTER_SEED = 0x9E3779B9;
+
+// ------------------------------------------------------------- context
+export async function init( options = {} ) {
+
+	addon.init( options.device === undefined ? 0 : options.device );
+	return { backend: 'hip', device: addon.deviceInfo() };
+
+}
+
+export function sync() {
+
+	addon.sync();
+
+}
+
+// -------------------------------------------------------------- fields
+export function normalizeShape( shape ) {
+
+	const dims = typeof shape === 'number' ? [ shape ] : Array.from( shape );
+	if ( dims.length < 1 || dims.length > 3 ) throw new Error( `arrayN: only 1D/2D/3D shapes are supported, got ${ dims.length }D.` );
+	for ( const d of dims ) {
+
+		if ( ! Number.isInteger( d ) || d <= 0 ) throw new Error( `arrayN: shape entries must be positive integers, got ${ d }.` );
+
+	}
+
+	return dims;
+
+}
+
+// first index fastest: strides [1, n0, n0*n1] (array.js:24-38)
+export function computeStrides( shape ) {
+
+	const strides = [];
+	let s = 1;
+	for ( const d of shape ) {
+
+		strides.push( s );
+		s *= d;
+
+	}
+
+	return strides;
+
+}
+
+function wrapField( handle, type, shape ) {
+
+	const strides = computeStrides( shape );
+	const count = shape.reduce( ( a, n ) => a * n, 1 );
+	// callable like the reference's field accessor: here it yields the flat index
+	const field = ( ...I ) => I.reduce( ( acc, v, a ) => acc + v * strides[ a ], 0 );
+	field.handle = handle;
+	field.type = type;
+	field.shape = shape;
+	field.count = count;
+	field.strides = strides;
+	field.at = field;
+	field.toArray = async () => {
+
+		const out = new TYPED[ type ]( type === 'vec3' ? count * 3 : count );
+		addon.fieldDownload( handle, out );
+		return out;
+
+	};
+
+	field.fromArray = ( typed ) => {
+
+		addon.fieldUpload( handle, typed );
+		return field;
+
+	};
+
+	field.fill = ( value ) => {
+
+		addon.fieldFill( handle, value );
+		return field;
+
+	};
+
+	return field;
+
+}
+
+export function arrayN( type, shape ) {
+
+	if ( ! ( type in TYPE_CODES ) ) throw new Error( `arrayN: unknown type "${ type }".` );
+	const dims = normalizeShape( shape );
+	return wrapField( addon.fieldCreate( TYPE_CODES[ type ], dims ), type, dims );
+
+}
+
+export function array0( type ) {
+
+	return arrayN( type, [ 1 ] );
+
+}
+
+// ---------------------------------------------------------------- linalg
+function handles( fields ) {
+
+	return fields ? fields.map( ( f ) => f.handle ) : null;
+
+}
+
+export function createLaplacianOperator( shape, gridSpacing, options = {} ) {
+
+	const op = addon.operatorCreate( shape, gridSpacing, options.dirichletMask ? options.dirichletMask.handle : null,
+		handles( options.faceWeights ) );
+	function applyLaplacian( input, output ) {
+
+		return function dispatch() {
+
+			addon.operatorApply( op, input.handle, output.handle );
+
+		};
+
+	}
+
+	applyLaplacian.native = { kind: 'operator', op };
+	return applyLaplacian;
+
+}
+
+export function createMultigridPreconditioner( shape, gridSpacing, options = {} ) {
+
+	if ( shape.length < 1 || shape.length > 3 ) throw new Error( `multigrid: only 1D/2D/3D shapes are supported, got ${ shape.length }D.` );
+	if ( gridSpacing.length !== shape.length ) throw new Error( `multigrid: gridSpacing length (${ gridSpacing.length }) must match shape length (${ shape.length }).` );
+	const mg = addon.multigridCreate( shape, gridSpacing, options,
+		options.dirichletMask ? options.dirichletMask.handle : null,
+		options.coarseDirichletMask ? options.coarseDirichletMask.handle : null,
+		handles( options.faceWeights ) );
+	function applyMultigridPreconditioner( input, output ) {
+
+		return function dispatch() {
+
+			addon.multigridApply( mg, input.handle, output.handle );
+
+		};
+
+	}
+
+	applyMultigridPreconditioner.native = { kind: 'multigrid', mg };
+	applyMultigridPreconditioner.refreshCoarseLevels = () => addon.multigridRefresh( mg );
+	applyMultigridPreconditioner.options = Object.assign( {
+		numberOfLevels: 1, numberOfSmoothingIterationsDown: 2, numberOfSmoothingIterationsUp: 2,
+		numberOfCoarsestIterations: 20, numberOfFinalIterations: 2, sorFactor: 1.0
+	}, options );
+	return applyMultigridPreconditioner;
+
+}
+
+export function createJacobiPreconditioner() {
+
+	const f = () => () => {
+
+		throw new Error( 'createJacobiPreconditioner: only usable inside the conjugate-gradient solver here' );
+
+	};
+
+	f.native = { kind: 'jacobi' };
+	return f;
+
+}
+
+export function createIdentityPreconditioner() {
+
+	const f = () => () => {
+
+		throw new Error( 'createIdentityPreconditioner: only usable inside the conjugate-gradient solver here' );
+
+	};
+
+	f.native = { kind: 'none' };
+	return f;
+
+}
+
+export function createPreconditionedConjugateGradientSolver( applyOperator, applyPreconditioner, b, x, options = {} ) {
+
+	if ( b.type !== x.type ) throw new Error( `conjugateGradient: element type mismatch, b.type(${ b.type }) != x.type(${ x.type }).` );
+	if ( b.shape.length !== x.shape.length || b.shape.some( ( v, i ) => v !== x.shape[ i ] ) ) {
+
+		throw new Error( `conjugateGradient: shape mismatch, b.shape(${ b.shape }) != x.shape(${ x.shape }).` );
+
+	}
+
+	if ( b.type !== 'float' ) throw new Error( `conjugateGradient: GPU atomic dot product only supports type "float", got "${ b.type }".` );
+	if ( ! applyOperator || ! applyOperator.native ) throw new Error( 'conjugateGradient: the operator must come from createLaplacianOperator' );
+	const pre = applyPreconditioner && applyPreconditioner.native ? applyPreconditioner.native : { kind: 'none' };
+	const kind = pre.kind === 'multigrid' ? PRECONDITIONERS.multigrid : pre.kind === 'jacobi' ? PRECONDITIONERS.jacobi : PRECONDITIONERS.none;
+	const solver = addon.pcgCreate( applyOperator.native.op, kind, pre.kind === 'multigrid' ? pre.mg : null, b.handle, x.handle );
+	const state = { residualSquared: 0, iterations: 0, stoppedBy: 'none' };
+	const settings = { reportNoiseFloor: options.reportNoiseFloor === true };
+	// solve(tol, maxiter, residualCheckInterval, gpuResidentScalars, batchIterations,
+	//       gpuResidentSetup, relativeTolerance, ...) -- linalg.js:2176
+	async function solve( tol, maxiter, residualCheckInterval = 1, gpuResidentScalars = false, batchIterations = true,
+		gpuResidentSetup = false, relativeTolerance = false ) {
+
+		const st = addon.pcgSolve( solver, tol, maxiter, relativeTolerance === true );
+		state.residualSquared = st.residualSquared;
+		state.iterations = st.iterations;
+		state.stoppedBy = st.stoppedBy === 'converged' ? 'none' : st.stoppedBy;
+		return st.converged;
+
+	}
+
+	return { solve, state, settings };
+
+}
+
+// ------------------------------------------------------------ grid data
+export function createFaceCenteredGrid3( nx, ny, nz, hx, hy, hz, ox = 0, oy = 0, oz = 0 ) {
+
+	const dataSizeU = [ nx + 1, ny, nz ];
+	const dataSizeV = [ nx, ny + 1, nz ];
+	const dataSizeW = [ nx, ny, nz + 1 ];
+	const grid = {
+		resolution: [ nx, ny, nz ], gridSpacing: [ hx, hy, hz ], origin: [ ox, oy, oz ],
+		dataSizeU, dataSizeV, dataSizeW,
+		dataOriginU: [ ox, oy + hy * 0.5, oz + hz * 0.5 ],
+		dataOriginV: [ ox + hx * 0.5, oy, oz + hz * 0.5 ],
+		dataOriginW: [ ox + hx * 0.5, oy + hy * 0.5, oz ],
+		dataU: arrayN( 'float', dataSizeU ),
+		dataV: arrayN( 'float', dataSizeV ),
+		dataW: arrayN( 'float', dataSizeW )
+	};
+	grid.clear = () => {
+
+		grid.dataU.fill( 0 );
+		grid.dataV.fill( 0 );
+		grid.dataW.fill( 0 );
+
+	};
+
+	return grid;
+
+}
+
+// ------------------------------------------------------- pressure solver
+const DEFAULT_MAX_PLAUSIBLE_PRESSURE = 1e6;
+const FIELD_PRESSURE = 0, FIELD_RHS = 1, FIELD_MASK = 2, FIELD_VENT = 3;
+
+function evaluateDirichlet( dirichlet, resolution, gridSpacing, origin ) {
+
+	// the reference evaluates dirichlet(pos) at cell centres (dataPosition,
+	// grid_pressure_solver3.js:177-181); a JavaScript function is evaluated here
+	const [ nx, ny, nz ] = resolution;
+	const active = new Uint8Array( nx * ny * nz );
+	const target = new Float32Array( nx * ny * nz );
+	for ( let k = 0; k < nz; k ++ ) for ( let j = 0; j < ny; j ++ ) for ( let i = 0; i < nx; i ++ ) {
+
+		const pos = [ origin[ 0 ] + ( i + 0.5 ) * gridSpacing[ 0 ], origin[ 1 ] + ( j + 0.5 ) * gridSpacing[ 1 ], origin[ 2 ] + ( k + 0.5 ) * gridSpacing[ 2 ] ];
+		const r = dirichlet( pos );
+		const c = i + nx * ( j + ny * k );
+		active[ c ] = r.active ? 1 : 0;
+		target[ c ] = r.target || 0;
+
+	}
+
+	return { active, target };
+
+}
+
+export function createGridPressureSolver3( {
+	resolution, gridSpacing, origin = [ 0, 0, 0 ],
+	dirichlet, faceWeights, colliderWeights,
+	multigrid = {}, preconditioner = 'multigrid',
+	tolerance = 1e-6, maxIterations = null, maxIterationsFactor = 1.5, relativeTolerance = true,
+	maxPlausiblePressure = DEFAULT_MAX_PLAUSIBLE_PRESSURE
+} = {} ) {
+
+	if ( ! Object.prototype.hasOwnProperty.call( PRECONDITIONERS, preconditioner ) ) {
+
+		throw new Error( `grid_pressure_solver3: unknown preconditioner '${ preconditioner }' (expected ${ Object.keys( PRECONDITIONERS ).join( ', ' ) }).` );
+
+	}
+
+	let dirichletActive = null, dirichletTarget = null;
+	if ( typeof dirichlet === 'function' ) {
+
+		const { active, target } = evaluateDirichlet( dirichlet, resolution, gridSpacing, origin );
+		dirichletActive = arrayN( 'u8', resolution ).fromArray( active );
+		dirichletTarget = arrayN( 'float', resolution ).fromArray( target );
+
+	} else if ( dirichlet ) {
+
+		dirichletActive = dirichlet.active;
+		dirichletTarget = dirichlet.target || null;
+
+	}
+
+	const triple = ( w ) => ( w ? [ w.u, w.v, w.w ].map( ( f ) => f.handle ) : null );
+	const solver = addon.pressure3Create( {
+		resolution, gridSpacing, origin,
+		dirichletActive: dirichletActive ? dirichletActive.handle : null,
+		dirichletTarget: dirichletTarget ? dirichletTarget.handle : null,
+		faceWeights: triple( faceWeights ),
+		colliderWeights: triple( colliderWeights ),
+		multigrid, preconditionerKind: PRECONDITIONERS[ preconditioner ],
+		tolerance, maxIterations, maxIterationsFactor, relativeTolerance, maxPlausiblePressure
+	} );
+	const view = ( which, type ) => {
+
+		const h = addon.pressure3Field( solver, which );
+		return h ? wrapField( h, type, resolution.slice() ) : null;
+
+	};
+
+	const pressure = { resolution, gridSpacing, origin, data: view( FIELD_PRESSURE, 'float' ) };
+	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
+	const settings = { tolerance, preconditioner, multigrid };
+	const keep = { dirichletActive, dirichletTarget, faceWeights, colliderWeights };
+
+	function project( input, output ) {
+
+		if ( input.resolution.join() !== resolution.join() ) {
+
+			throw new Error( `createGridPressureSolver3: project() input resolution [${ input.resolution }] does not match constructed resolution [${ resolution }].` );
+
+		}
+
+		const inFields = [ input.dataU.handle, input.dataV.handle, input.dataW.handle ];
+		const outFields = [ output.dataU.handle, output.dataV.handle, output.dataW.handle ];
+		return async function dispatch() {
+
+			const d = addon.pressure3Project( solver, inFields, outFields );
+			diagnostics.converged = d.converged;
+			diagnostics.rejected = d.rejected;
+			diagnostics.iterations = d.iterations;
+			diagnostics.residual = d.residual;
+			diagnostics.stoppedBy = d.stoppedBy === 'converged' ? 'none' : d.stoppedBy;
+			diagnostics.badCells = d.badCells;
+			return keep && diagnostics;
+
+		};
+
+	}
+
+	return {
+		project, pressure, b: view( FIELD_RHS, 'float' ), diagnostics, settings,
+		dirichletMask: view( FIELD_MASK, 'u8' ), ventMask: view( FIELD_VENT, 'u8' )
+	};
+
+}
+
+// ----------------------------------------------------- seeding, transfers
+export function createJitterRandom( seed ) {
+
+	// mulberry32 (jitter_random.js:29-42)
+	let state = seed >>> 0;
+	return function random() {
+
+		state = ( state + 0x6D2B79F5 ) >>> 0;
+		let t = state;
+		t = Math.imul( t ^ ( t >>> 15 ), t | 1 );
+		t ^= t + Math.imul( t ^ ( t >>> 7 ), t | 61 );
+		return ( ( t ^ ( t >>> 14 ) ) >>> 0 ) / 4294967296;
+
+	};
+
+}
+
+export function computeFlipBoxSeed3( { boxMin, boxMax, gridSpacingX, gridSpacingY, gridSpacingZ, particlesPerCellAxis = 2, jitter = 0.2, randomSeed = DEFAULT_JITTER_SEED } ) {
+
+	const cells = [ 0, 1, 2 ].map( ( a ) => Math.max( 1, Math.round( ( boxMax[ a ] - boxMin[ a ] ) / [ gridSpacingX, gridSpacingY, gridSpacingZ ][ a ] ) ) );
+	const h = [ gridSpacingX, gridSpacingY, gridSpacingZ ];
+	const ppa = particlesPerCellAxis;
+	const sub = h.map( ( x ) => x / ppa );
+	const count = cells[ 0 ] * cells[ 1 ] * cells[ 2 ] * ppa * ppa * ppa;
+	const positionsArray = new Float32Array( count * 3 );
+	const random = createJitterRandom( randomSeed );
+	let q = 0;
+	for ( let ck = 0; ck < cells[ 2 ]; ck ++ ) for ( let cj = 0; cj < cells[ 1 ]; cj ++ ) for ( let ci = 0; ci < cells[ 0 ]; ci ++ ) {
+
+		for ( let sk = 0; sk < ppa; sk ++ ) for ( let sj = 0; sj < ppa; sj ++ ) for ( let si = 0; si < ppa; si ++ ) {
+
+			const jx = ( random() * 2 - 1 ) * jitter * sub[ 0 ];
+			const jy = ( random() * 2 - 1 ) * jitter * sub[ 1 ];
+			const jz = ( random() * 2 - 1 ) * jitter * sub[ 2 ];
+			positionsArray[ q ++ ] = boxMin[ 0 ] + ci * h[ 0 ] + ( si + 0.5 ) * sub[ 0 ] + jx;
+			positionsArray[ q ++ ] = boxMin[ 1 ] + cj * h[ 1 ] + ( sj + 0.5 ) * sub[ 1 ] + jy;
+			positionsArray[ q ++ ] = boxMin[ 2 ] + ck * h[ 2 ] + ( sk + 0.5 ) * sub[ 2 ] + jz;
+
+		}
+
+	}
+
+	return { count, positionsArray, velocitiesArray: new Float32Array( count * 3 ) };
+
+}
+
+function gridArg( grid ) {
+
+	return { n: grid.resolution, h: grid.gridSpacing, origin: grid.origin };
+
+}
+
+// cellIndexOf + stable sort by cell: { cell, perm, cellStart, outside }
+export function binParticles( grid, positions, count ) {
+
+	const n = grid.resolution;
+	const bins = {
+		cell: arrayN( 'int', Math.max( count, 1 ) ), perm: arrayN( 'int', Math.max( count, 1 ) ),
+		cellStart: arrayN( 'int', n[ 0 ] * n[ 1 ] * n[ 2 ] + 1 )
+	};
+	bins.outside = addon.binParticles( gridArg( grid ), positions.handle, count, bins.cell.handle, bins.perm.handle, bins.cellStart.handle );
+	return bins;
+
+}
+
+export function particleToGrid( grid, positions, velocities, bins, weightEpsilon = 1e-4 ) {
+
+	const valid = [ grid.dataSizeU, grid.dataSizeV, grid.dataSizeW ].map( ( s ) => arrayN( 'int', s ) );
+	const data = [ grid.dataU, grid.dataV, grid.dataW ];
+	for ( let c = 0; c < 3; c ++ ) addon.p2gGather( gridArg( grid ), positions.handle, velocities.handle, bins.perm.handle, bins.cellStart.handle, c, weightEpsilon, data[ c ].handle, valid[ c ].handle );
+	return valid;
+
+}
+
+export function gridToParticle( grid, oldGrid, positions, velocities, count, flipRatio = 0.97, velocityDamping = 0.02 ) {
+
+	addon.g2p( gridArg( grid ), positions.handle, velocities.handle, count,
+		[ grid.dataU.handle, grid.dataV.handle, grid.dataW.handle ],
+		[ oldGrid.dataU.handle, oldGrid.dataV.handle, oldGrid.dataW.handle ], flipRatio, velocityDamping );
+
+}
+
+export function advectParticles( grid, positions, count, dt, maxSubsteps = 32 ) {
+
+	addon.advectParticles( gridArg( grid ), positions.handle, count, [ grid.dataU.handle, grid.dataV.handle, grid.dataW.handle ], dt, maxSubsteps );
+
+}
+
+export function advectFaceCentered3( input, output, dt, maxSubsteps = 32 ) {
+
+	addon.advectFaces( gridArg( input ), [ input.dataU.handle, input.dataV.handle, input.dataW.handle ],
+		[ output.dataU.handle, output.dataV.handle, output.dataW.handle ], dt, maxSubsteps );
+
+}
+
+export function twoPhaseBeta( grid, bins, gasDensity, liquidDensity, fraction ) {
+
+	const beta = { u: arrayN( 'float', grid.dataSizeU ), v: arrayN( 'float', grid.dataSizeV ), w: arrayN( 'float', grid.dataSizeW ) };
+	addon.twoPhaseBeta( gridArg( grid ), bins ? bins.cellStart.handle : null, gasDensity, liquidDensity, fraction.handle, [ beta.u.handle, beta.v.handle, beta.w.handle ] );
+	return beta;
+
+}

@@ -1,0 +1,98 @@
+"""Particle/grid transfers of the FLIP solver on the GPU
+(packages/fluxflow/src/grid/grid_flip_solver3.js, advection_solver3.js).
+
+Particles are ``vec3`` fields (3 x f32 per particle); MAC velocity is a triple
+of face fields.  ``Grid`` carries cells, spacing and origin.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import Field, Grid3, _check, _lib, array_n, face_shape
+
+
+class Grid:
+    def __init__(self, n, h, origin=(0.0, 0.0, 0.0)):
+        self.n = tuple(int(x) for x in n)
+        self.h = tuple(float(x) for x in h)
+        self.origin = tuple(float(x) for x in origin)
+        g = Grid3()
+        for a in range(3):
+            g.n[a] = self.n[a]
+            g.h[a] = self.h[a]
+            g.origin[a] = self.origin[a]
+        self.c = g
+
+    @property
+    def cells(self):
+        return int(np.prod(self.n))
+
+    def face_shape(self, axis):
+        return face_shape(self.n, axis)
+
+
+def _mac(fields):
+    return (C.c_void_p * 3)(*[f.h for f in fields])
+
+
+class Bins:
+    """Result of bin_particles: per-particle cell, stable order, cell starts."""
+
+    def __init__(self, grid, count):
+        self.count = count
+        self.cell = array_n("int", max(count, 1))
+        self.perm = array_n("int", max(count, 1))
+        self.cell_start = array_n("int", grid.cells + 1)
+        self.outside = 0
+
+
+def bin_particles(grid, positions, count, bins=None):
+    """cellIndexOf + stable sort by cell (deterministic binning)."""
+    bins = bins or Bins(grid, count)
+    out = C.c_int64()
+    _check(_lib.flux_bin_particles(C.byref(grid.c), C.c_void_p(positions.h), C.c_int64(count), C.c_void_p(bins.cell.h),
+                                   C.c_void_p(bins.perm.h), C.c_void_p(bins.cell_start.h), C.byref(out)),
+           "bin_particles")
+    bins.outside = out.value
+    return bins
+
+
+def p2g_gather(grid, positions, velocities, bins, component, out, valid, eps=1e-4):
+    _check(_lib.flux_p2g_gather(C.byref(grid.c), C.c_void_p(positions.h), C.c_void_p(velocities.h),
+                                C.c_void_p(bins.perm.h), C.c_void_p(bins.cell_start.h), int(component), C.c_float(eps),
+                                C.c_void_p(out.h), C.c_void_p(valid.h)), "p2g")
+
+
+def p2g_fixed(grid, positions, velocities, count, component, numer, denom, out, valid, scale=65536.0, eps=1e-4):
+    _check(_lib.flux_p2g_fixed(C.byref(grid.c), C.c_void_p(positions.h), C.c_void_p(velocities.h), C.c_int64(count),
+                               int(component), C.c_float(scale), C.c_float(eps), C.c_void_p(numer.h),
+                               C.c_void_p(denom.h), C.c_void_p(out.h), C.c_void_p(valid.h)), "p2g_fixed")
+
+
+def g2p(grid, positions, velocities, count, vel_new, vel_old, flip_ratio=0.97, damping=0.02):
+    _check(_lib.flux_g2p(C.byref(grid.c), C.c_void_p(positions.h), C.c_void_p(velocities.h), C.c_int64(count),
+                         _mac(vel_new), _mac(vel_old), C.c_float(flip_ratio), C.c_float(damping)), "g2p")
+
+
+def advect_particles(grid, positions, count, vel, dt, max_substeps=32):
+    _check(_lib.flux_advect_particles(C.byref(grid.c), C.c_void_p(positions.h), C.c_int64(count), _mac(vel),
+                                      C.c_float(dt), int(max_substeps)), "advectParticles")
+
+
+def advect_faces(grid, vel_in, vel_out, dt, max_substeps=32):
+    _check(_lib.flux_advect_faces(C.byref(grid.c), _mac(vel_in), _mac(vel_out), C.c_float(dt), int(max_substeps)),
+           "advectFaceCentered3")
+
+
+def two_phase_beta(grid, bins, rho_gas, rho_liquid, fraction, beta):
+    _check(_lib.flux_two_phase_beta(C.byref(grid.c), C.c_void_p(bins.cell_start.h if bins is not None else None),
+                                    C.c_float(rho_gas), C.c_float(rho_liquid), C.c_void_p(fraction.h), _mac(beta)),
+           "two_phase_beta")
+
+
+def divergence(grid, vel, out):
+    _check(_lib.flux_divergence(C.byref(grid.c), _mac(vel), C.c_void_p(out.h)), "divergence")
+
+
+def mac_fields(grid, arrays=None):
+    return [array_n("float", grid.face_shape(a), None if arrays is None else arrays[a]) for a in range(3)]
