@@ -133,3 +133,46 @@ def test_large_projection_reaches_tolerance(flux):
     r32 = O.true_residual(sc["shape"], sc["spacing"], A["b"], p, A["mask"], A["weights"], suffix="f32")
     print("f32-rounded pressure relative residual:", r32 / diag["bNorm"])
     assert r32 / diag["bNorm"] < 1e-3
+
+
+@pytest.mark.parametrize("storage,vtol", [("f16", 5e-3), ("bf16", 5e-2)])
+def test_mixed_precision_vcycle_tracks_fp32(flux, frame110, storage, vtol):
+    """FP16/BF16 storage with FP32 arithmetic: one V-cycle stays within the
+    storage precision of the FP32 oracle's V-cycle (a wrong residual scale or
+    conversion would be off by O(1))."""
+    F, c = flux, frame110
+    sh = c["shape"]
+    m = F.array_n("u8", sh, c["mask"])
+    v = F.array_n("u8", sh, c["vent"])
+    w = [F.array_n("float", s, c["weights"][a]) for a, s in enumerate(
+        [(sh[0] + 1, sh[1], sh[2]), (sh[0], sh[1] + 1, sh[2]), (sh[0], sh[1], sh[2] + 1)])]
+    mg = F.create_multigrid_preconditioner(sh, c["spacing"], dirichlet_mask=m, coarse_dirichlet_mask=v,
+                                           face_weights=w, levels=4, storage=storage)
+    z = F.array_n("float", sh)
+    mg(F.array_n("float", sh, c["b"]), z)
+    ref = O.Multigrid(sh, c["spacing"], levels=4, mask=c["mask"], coarse_mask=c["vent"], weights=c["weights"],
+                      suffix="f32").apply(c["b"])
+    err = rel(z.to_array(), ref)
+    print(storage, "V-cycle rel L2 vs FP32 oracle:", err)
+    assert err <= vtol
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_mixed_precision_projection_reaches_same_tolerance(flux, storage):
+    F = flux
+    n = 64
+    sc = scenes.dam_break_system(n)
+    ps, vel, diag = gpu_project(F, sc, 5, 1e-6)
+    ps2, vel2, diag2 = gpu_project(F, sc, 5, 1e-6) if False else (None, None, None)
+    n_, h = sc["shape"], sc["spacing"]
+    vin = [F.array_n("float", F.face_shape(n_, a), sc["vel"][a]) for a in range(3)]
+    beta = [F.array_n("float", F.face_shape(n_, a), sc["beta"][a]) for a in range(3)]
+    act = F.array_n("u8", n_, sc["active"])
+    tgt = F.array_n("float", n_, sc["target"])
+    pm = F.create_grid_pressure_solver3(resolution=n_, grid_spacing=h, dirichlet=(act, tgt), face_weights=beta,
+                                        multigrid={"levels": 5, "storage": storage}, tolerance=1e-6)
+    dm = pm.project(vin)
+    print("plain", diag, storage, dm)
+    assert dm["converged"] and dm["residual"] <= 1e-6 * dm["bNorm"]
+    assert dm["iterations"] <= diag["iterations"] + 3
+    assert rel(pm.pressure.to_array(), ps.pressure.to_array()) <= 1e-4

@@ -1,0 +1,731 @@
+// Multigrid preconditioner for gfx950: createMultigridPreconditioner
+// (packages/fluxflow/src/linalg/multigrid.js:1213-1735).
+//
+// Every kernel computes in FP32; storage is FP32 (the reference's form) or,
+// for the mixed-precision form, FP16/BF16 for all coarse-level fields and the
+// face weights.  The finest level's x and b are the PCG's z and r and stay
+// FP32.  With FP16 storage the restricted residual is scaled to unit RMS
+// (s = sqrt(N / r.r), undone in the prolongation) so late, small residuals
+// never fall into FP16's subnormal range; the V-cycle is linear, so the
+// scaling does not change what it computes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "flux_internal.h"
+#include "flux_solver.h"
+#include "flux_stencil.h"
+#include "flux_sweep.h"
+
+namespace flux {
+
+static constexpr int MTPB = 256;
+
+template <typename T>
+__global__ void k_fill_t(T* __restrict__ p, int n, float v) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = to_store<T>(v);
+}
+
+template <typename T>
+__global__ void k_copy_t(const T* __restrict__ a, T* __restrict__ b, int n, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = a[t];
+}
+
+template <typename I, typename O>
+__global__ void k_convert(const I* __restrict__ a, O* __restrict__ b, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = to_store<O>(to_f32(a[t]));
+}
+
+// residual scale for FP16 storage: unit RMS of the finest residual
+__device__ __forceinline__ float residual_scale(const double* rr, int n) {
+    if (!rr) return 1.0f;
+    double v = *rr;
+    if (!(v > 0.0) || !(v < 1e300)) return 1.0f;
+    return (float)sqrt((double)n / v);
+}
+
+// One red-black Gauss-Seidel colour (buildRelaxKernel, multigrid.js:626-640):
+// x_c += ((b_c - (A x)_c) / diag_c) * omega on cells with (i+j+k)%2 == color.
+// Thread t covers the t-th cell of that colour.  FROM_ZERO takes the field as
+// zero on entry and writes every cell (the colour's cells with omega*b/diag,
+// the others with 0), folding zeroX into the sweep.  FINE marks finest-level
+// launches so profilers report them as kernels of their own.
+template <int FINE, int FROM_ZERO, typename XT, typename WT>
+__global__ void k_relax(LevelDevT<WT> L, XT* __restrict__ x, const XT* __restrict__ b, float omega, int color,
+                        const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    const int half = (L.n0 + 1) >> 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int rows = L.n1 * L.n2;
+    if (t >= half * rows) return;
+    int row = t / half;
+    int m = t - row * half;
+    int k = row / L.n1;
+    int j = row - k * L.n1;
+    int i = 2 * m + ((j + k + color) & 1);
+    if (FROM_ZERO) {
+        int ip = 2 * m + ((j + k + color + 1) & 1);
+        if (ip < L.n0) x[ip + L.n0 * row] = to_store<XT>(0.0f);
+    }
+    if (i >= L.n0) return;
+    int c = i + L.n0 * row;
+    float d;
+    float ax;
+    if (FROM_ZERO) {
+        (void)stencil_row<float>(L, i, j, k, c, [&](int) { return 0.0f; }, &d);
+        ax = 0.0f;
+    } else {
+        ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(x[q]); }, &d);
+    }
+    float cur = FROM_ZERO ? 0.0f : to_f32(x[c]);
+    x[c] = to_store<XT>(cur + ((to_f32(b[c]) - ax) / d) * omega);
+}
+
+// r = b - A x  (buildResidualKernel, multigrid.js:741-749)
+template <int FINE, typename XT, typename RT, typename WT>
+__global__ void k_residual(LevelDevT<WT> L, const XT* __restrict__ x, const XT* __restrict__ b, RT* __restrict__ r,
+                           const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n0 * L.n1 * L.n2) return;
+    int i, j, k;
+    cell_coords(L, c, i, j, k);
+    float ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(x[q]); }, (float*)nullptr);
+    r[c] = to_store<RT>(to_f32(b[c]) - ax);
+}
+
+// Full-weighting restriction (restrictionTapsForAxis 766-780 + buildRestrictKernel
+// 866-914): per axis taps 1/8,3/8,3/8,1/8 at 2c-1 (clamped to 2c), 2c, 2c+1,
+// 2c+2 (clamped to 2c+1); masked fine cells contribute 0; masked coarse cells
+// get 0.  The result is multiplied by the residual scale when rr is given.
+template <int FINE, typename IT, typename OT>
+__global__ void k_restrict(int ndim, int f0, int f1, int f2, const IT* __restrict__ rf, const uint8_t* __restrict__ mf,
+                           int c0, int c1, int c2, OT* __restrict__ bc, const uint8_t* __restrict__ mc,
+                           const double* __restrict__ rr, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c0 * c1 * c2) return;
+    int row = t / c0;
+    int I = t - row * c0;
+    int K = row / c1;
+    int J = row - K * c1;
+    if (mc && mc[t]) {
+        bc[t] = to_store<OT>(0.0f);
+        return;
+    }
+    const float tw[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    int ti[4], tj[4], tk[4];
+    ti[0] = I > 0 ? 2 * I - 1 : 2 * I;
+    ti[1] = 2 * I;
+    ti[2] = 2 * I + 1;
+    ti[3] = (I + 1 < c0) ? 2 * I + 2 : 2 * I + 1;
+    tj[0] = J > 0 ? 2 * J - 1 : 2 * J;
+    tj[1] = 2 * J;
+    tj[2] = 2 * J + 1;
+    tj[3] = (J + 1 < c1) ? 2 * J + 2 : 2 * J + 1;
+    int nk = 1;
+    tk[0] = 0;
+    if (ndim == 3) {
+        nk = 4;
+        tk[0] = K > 0 ? 2 * K - 1 : 2 * K;
+        tk[1] = 2 * K;
+        tk[2] = 2 * K + 1;
+        tk[3] = (K + 1 < c2) ? 2 * K + 2 : 2 * K + 1;
+    }
+    float sum = 0.0f;
+    for (int a = 0; a < 4; ++a)
+        for (int bq = 0; bq < 4; ++bq)
+            for (int cq = 0; cq < nk; ++cq) {
+                float wgt = tw[a] * tw[bq] * (nk == 4 ? tw[cq] : 1.0f);
+                int f = ti[a] + f0 * (tj[bq] + f1 * tk[cq]);
+                float v = (mf && mf[f]) ? 0.0f : to_f32(rf[f]);
+                sum += v * wgt;
+            }
+    bc[t] = to_store<OT>(sum * residual_scale(rr, f0 * f1 * f2));
+}
+
+// Bilinear / trilinear prolongation added to the fine x except at masked fine
+// cells (correctionTapsForAxis 921-934, buildCorrectKernel 990-1040); the
+// residual scale is undone when rr is given.
+template <int FINE, typename CT, typename FT>
+__global__ void k_correct(int ndim, int c0, int c1, int c2, const CT* __restrict__ xc, int f0, int f1, int f2,
+                          FT* __restrict__ xf, const uint8_t* __restrict__ mf, const double* __restrict__ rr,
+                          const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= f0 * f1 * f2) return;
+    if (mf && mf[t]) return;
+    int row = t / f0;
+    int i = t - row * f0;
+    int k = row / f1;
+    int j = row - k * f1;
+    int F[3] = {i, j, k};
+    int nc[3] = {c0, c1, c2};
+    int ti[3][2];
+    float tw[3][2];
+    for (int a = 0; a < 3; ++a) {
+        if (a < ndim) {
+            int ci = F[a] >> 1;
+            bool even = (F[a] & 1) == 0;
+            int lower = ci > 0 ? ci - 1 : ci;
+            int upper = ci < nc[a] - 1 ? ci + 1 : ci;
+            ti[a][0] = even ? lower : ci;
+            tw[a][0] = even ? 0.25f : 0.75f;
+            ti[a][1] = even ? ci : upper;
+            tw[a][1] = even ? 0.75f : 0.25f;
+        } else {
+            ti[a][0] = ti[a][1] = 0;
+            tw[a][0] = 1.0f;
+            tw[a][1] = 0.0f;
+        }
+    }
+    int nk = (ndim == 3) ? 2 : 1;
+    float sum = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int bq = 0; bq < 2; ++bq)
+            for (int cq = 0; cq < nk; ++cq) {
+                float wgt = tw[0][a] * tw[1][bq] * (nk == 2 ? tw[2][cq] : 1.0f);
+                sum += to_f32(xc[ti[0][a] + c0 * (ti[1][bq] + c1 * ti[2][cq])]) * wgt;
+            }
+    if (rr) sum = sum / residual_scale(rr, f0 * f1 * f2);
+    xf[t] = to_store<FT>(to_f32(xf[t]) + sum);
+}
+
+// Coarsest level in one workgroup: `iters` x (colour 0, colour 1) sweeps from
+// x = 0 (buildCoarseSweepKernel, multigrid.js:679-737).  The block's barriers
+// order the colours; global writes are visible within the block after them.
+template <typename XT, typename WT>
+__global__ void __launch_bounds__(1024) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x, const XT* __restrict__ b,
+                                                       float omega, int iters, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    const int n = L.n0 * L.n1 * L.n2;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = to_store<XT>(0.0f);
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) {
+        for (int color = 0; color < 2; ++color) {
+            for (int c = threadIdx.x; c < n; c += blockDim.x) {
+                int i, j, k;
+                cell_coords(L, c, i, j, k);
+                if (((i + j + k) & 1) != color) continue;
+                float d;
+                float ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(x[q]); }, &d);
+                x[c] = to_store<XT>(to_f32(x[c]) + ((to_f32(b[c]) - ax) / d) * omega);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Coarse face weight = mean of the 2^(d-1) fine faces at index 2I along the
+// face axis (buildFaceWeightCoarsenKernel, multigrid.js:1138-1184).
+template <typename IT, typename OT>
+__global__ void k_coarsen_w(int ndim, int axis, int f0, int f1, int f2, const IT* __restrict__ wf, int c0, int c1,
+                            int c2, OT* __restrict__ wc) {
+    int s0 = c0 + (axis == 0), s1 = c1 + (axis == 1), s2 = c2 + (axis == 2);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= s0 * s1 * s2) return;
+    int row = t / s0;
+    int I = t - row * s0;
+    int K = row / s1;
+    int J = row - K * s1;
+    int fs0 = f0 + (axis == 0), fs1 = f1 + (axis == 1);
+    int C[3] = {I, J, K};
+    int cnt[3];
+    for (int a = 0; a < 3; ++a) cnt[a] = (a < ndim && a != axis) ? 2 : 1;
+    float sum = 0.0f;
+    for (int d2 = 0; d2 < cnt[2]; ++d2)
+        for (int d1 = 0; d1 < cnt[1]; ++d1)
+            for (int d0 = 0; d0 < cnt[0]; ++d0) {
+                int fi = 2 * C[0] + d0, fj = 2 * C[1] + d1, fk = (ndim == 3) ? 2 * C[2] + d2 : 0;
+                sum += to_f32(wf[fi + fs0 * (fj + fs1 * fk)]);
+            }
+    wc[t] = to_store<OT>(sum / (float)(1 << (ndim - 1)));
+}
+
+// Coarse mask (buildMaskCoarsenKernel, multigrid.js:1066-1122): ALL 2^d fine
+// children of `src` masked, OR the six coarse face weights sum below 1e-6.
+template <typename WT>
+__global__ void k_coarsen_mask(int ndim, int f0, int f1, const uint8_t* __restrict__ src, int c0, int c1, int c2,
+                               const WT* __restrict__ w0, const WT* __restrict__ w1, const WT* __restrict__ w2,
+                               uint8_t* __restrict__ mc) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c0 * c1 * c2) return;
+    int row = t / c0;
+    int I = t - row * c0;
+    int K = row / c1;
+    int J = row - K * c1;
+    bool have = false, masked = false;
+    if (src) {
+        masked = true;
+        int nk = (ndim == 3) ? 2 : 1;
+        for (int d2 = 0; d2 < nk; ++d2)
+            for (int d1 = 0; d1 < 2; ++d1)
+                for (int d0 = 0; d0 < 2; ++d0) {
+                    int fk = (ndim == 3) ? 2 * K + d2 : 0;
+                    masked = masked && src[(2 * I + d0) + f0 * ((2 * J + d1) + f1 * fk)];
+                }
+        have = true;
+    }
+    if (w0) {
+        float total = to_f32(w0[I + (c0 + 1) * (J + c1 * K)]) + to_f32(w0[I + 1 + (c0 + 1) * (J + c1 * K)]);
+        total += to_f32(w1[I + c0 * (J + (c1 + 1) * K)]) + to_f32(w1[I + c0 * (J + 1 + (c1 + 1) * K)]);
+        if (ndim == 3) total += to_f32(w2[t]) + to_f32(w2[t + c0 * c1]);
+        bool enclosed = total < 1e-6f;
+        masked = have ? (masked || enclosed) : enclosed;
+        have = true;
+    }
+    mc[t] = (uint8_t)(have && masked);
+}
+
+}  // namespace flux
+
+using namespace flux;
+
+// ------------------------------------------------------------ host objects
+struct MGLevel {
+    int n[3];
+    double h[3];
+    void* x = nullptr;    // level 0: the caller's output (FP32)
+    void* b = nullptr;    // level 0: the caller's input (FP32)
+    void* buf = nullptr;  // residual (FP32 on level 0)
+    void* alt = nullptr;  // ping-pong partner of x for the fused sweep
+    uint8_t* mask = nullptr;              // level 0: the caller's (not owned)
+    void* w[3] = {nullptr, nullptr, nullptr};  // level 0: caller's FP32 or own converted copy
+    bool own = false;
+    bool own_w = false;
+};
+
+struct flux_multigrid {
+    int ndim = 3;
+    flux_mg_options o;
+    std::vector<MGLevel> lv;
+    const flux_field* mask = nullptr;
+    const flux_field* coarse_mask = nullptr;
+    const flux_field* w[3] = {nullptr, nullptr, nullptr};
+    bool has_w = false, has_masks = false;
+    bool fused = true;
+};
+
+static constexpr int COARSE_SINGLE_BLOCK_MAX = 4096;
+static constexpr int SW_TX = 64, SW_TY = 4, SW_KZ = 32;
+
+namespace {
+
+size_t store_size(int st) { return st == FLUX_STORE_F32 ? 4 : 2; }
+
+template <typename WT>
+LevelDevT<WT> level_dev(const flux_multigrid* mg, const MGLevel& v) {
+    LevelDevT<WT> L;
+    L.n0 = v.n[0];
+    L.n1 = v.n[1];
+    L.n2 = v.n[2];
+    L.ndim = mg->ndim;
+    for (int a = 0; a < 3; ++a) {
+        double hh = (a < mg->ndim) ? v.h[a] : 1.0;
+        L.ih2d[a] = 1.0 / (hh * hh);
+        L.ih2[a] = (float)L.ih2d[a];
+    }
+    L.mask = v.mask;
+    L.w0 = (const WT*)v.w[0];
+    L.w1 = (const WT*)v.w[1];
+    L.w2 = mg->ndim == 3 ? (const WT*)v.w[2] : nullptr;
+    return L;
+}
+
+template <typename T>
+void fill_t(void* p, int64_t n, float v) {
+    if (n > 0) k_fill_t<T><<<blocks_for(n, MTPB), MTPB, 0, g_stream>>>((T*)p, (int)n, v);
+}
+
+// refreshCoarseLevels (multigrid.js:1415-1437): level-l weights from level
+// l-1, then the level-l mask from level l-1's source (level 1 reads the
+// vent-only mask), and in the mixed form the finest weights' storage copy.
+template <typename ST>
+int refresh_t(flux_multigrid* mg) {
+    if (mg->has_w && mg->lv[0].own_w)
+        for (int a = 0; a < mg->ndim; ++a) {
+            int64_t nf = faces_of(mg->lv[0].n, a);
+            k_convert<float, ST><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>((const float*)mg->w[a]->d,
+                                                                               (ST*)mg->lv[0].w[a], (int)nf);
+        }
+    for (size_t l = 1; l < mg->lv.size(); ++l) {
+        MGLevel& f = mg->lv[l - 1];
+        MGLevel& c = mg->lv[l];
+        if (mg->has_w) {
+            for (int a = 0; a < mg->ndim; ++a) {
+                int64_t nf = faces_of(c.n, a);
+                k_coarsen_w<ST, ST><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>(
+                    mg->ndim, a, f.n[0], f.n[1], f.n[2], (const ST*)f.w[a], c.n[0], c.n[1], c.n[2], (ST*)c.w[a]);
+            }
+        }
+        if (!mg->has_masks) continue;
+        const uint8_t* src = nullptr;
+        if (mg->mask) src = (l == 1) ? (const uint8_t*)(mg->coarse_mask ? mg->coarse_mask->d : mg->mask->d) : f.mask;
+        int64_t cc = cells_of(c.n);
+        k_coarsen_mask<ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, f.n[0], f.n[1], src, c.n[0], c.n[1], c.n[2], mg->has_w ? (const ST*)c.w[0] : nullptr,
+            mg->has_w ? (const ST*)c.w[1] : nullptr, mg->has_w ? (const ST*)c.w[2] : nullptr, c.mask);
+    }
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+bool use_fused(const flux_multigrid* mg, const MGLevel& v) {
+    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= 16 && v.n[1] >= SW_TY && v.n[2] >= 4;
+}
+
+// `iters` full sweeps (colour A then B each), ping-ponging so the result lands in v.x
+template <typename XT, typename WT>
+void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
+    MGLevel& v = mg->lv[l];
+    LevelDevT<WT> L = level_dev<WT>(mg, v);
+    int gx = (v.n[0] + SW_TX - 1) / SW_TX, gy = (v.n[1] + SW_TY - 1) / SW_TY, gz = (v.n[2] + SW_KZ - 1) / SW_KZ;
+    unsigned nb = (unsigned)(gx * gy * gz);
+    float om = (float)mg->o.omega;
+    int colorA = reversed ? 1 : 0;
+    const XT* src = (const XT*)v.x;
+    if (!from_zero && (iters & 1)) {
+        k_copy_t<XT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>((const XT*)v.x, (XT*)v.alt,
+                                                                              (int)cells_of(v.n), stop);
+        src = (const XT*)v.alt;
+    }
+    for (int it = 0; it < iters; ++it) {
+        XT* dst = (((iters - 1 - it) & 1) == 0) ? (XT*)v.x : (XT*)v.alt;
+        if (l == 0) kprof_begin("sweep");
+        if (from_zero && it == 0)
+            k_rbgs_fused<SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        else
+            k_rbgs_fused<SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        if (l == 0) kprof_end("sweep");
+        src = dst;
+    }
+}
+
+template <int FINE, typename XT, typename WT>
+void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
+    MGLevel& v = mg->lv[l];
+    if (iters <= 0) return;
+    if (use_fused(mg, v)) {
+        fused_sweeps<XT, WT>(mg, l, iters, reversed, from_zero, stop);
+        return;
+    }
+    LevelDevT<WT> L = level_dev<WT>(mg, v);
+    int64_t work = (int64_t)((v.n[0] + 1) / 2) * v.n[1] * v.n[2];
+    unsigned g = blocks_for(work, MTPB);
+    float om = (float)mg->o.omega;
+    XT* x = (XT*)v.x;
+    const XT* b = (const XT*)v.b;
+    for (int it = 0; it < iters; ++it) {
+        int c0 = reversed ? 1 : 0, c1 = reversed ? 0 : 1;
+        if (from_zero && it == 0) {
+            k_relax<FINE, 1, XT, WT><<<g, MTPB, 0, g_stream>>>(L, x, b, om, c0, stop);
+        } else {
+            if (FINE) kprof_begin("relax");
+            k_relax<FINE, 0, XT, WT><<<g, MTPB, 0, g_stream>>>(L, x, b, om, c0, stop);
+            if (FINE) kprof_end("relax");
+        }
+        if (FINE) kprof_begin("relax");
+        k_relax<FINE, 0, XT, WT><<<g, MTPB, 0, g_stream>>>(L, x, b, om, c1, stop);
+        if (FINE) kprof_end("relax");
+    }
+}
+
+template <int FINE, typename XT, typename WT>
+void level_smooth(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
+    relax_sweeps<FINE, XT, WT>(mg, l, iters, reversed, from_zero, stop);
+}
+
+// vCycle (multigrid.js:1609-1642).  Level 0 has FP32 vectors and weights of
+// type ST; coarse levels store everything in ST.
+template <typename ST>
+int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
+    MGLevel& v = mg->lv[l];
+    int nl = (int)mg->lv.size();
+    const bool fine = (l == 0);
+    if (l == nl - 1) {
+        int64_t cells = cells_of(v.n);
+        if (fine) {
+            if (cells <= COARSE_SINGLE_BLOCK_MAX)
+                k_coarse_solve<float, ST><<<1, 1024, 0, g_stream>>>(level_dev<ST>(mg, v), (float*)v.x,
+                                                                    (const float*)v.b, (float)mg->o.omega,
+                                                                    mg->o.n_coarsest, stop);
+            else if (mg->o.n_coarsest > 0)
+                level_smooth<1, float, ST>(mg, l, mg->o.n_coarsest, false, true, stop);
+            else
+                fill_t<float>(v.x, cells, 0.0f);
+        } else {
+            if (cells <= COARSE_SINGLE_BLOCK_MAX)
+                k_coarse_solve<ST, ST><<<1, 1024, 0, g_stream>>>(level_dev<ST>(mg, v), (ST*)v.x, (const ST*)v.b,
+                                                                 (float)mg->o.omega, mg->o.n_coarsest, stop);
+            else if (mg->o.n_coarsest > 0)
+                level_smooth<0, ST, ST>(mg, l, mg->o.n_coarsest, false, true, stop);
+            else
+                fill_t<ST>(v.x, cells, 0.0f);
+        }
+        return FLUX_OK;
+    }
+    MGLevel& c = mg->lv[l + 1];
+    int64_t cells = cells_of(v.n), cc = cells_of(c.n);
+    const double* scale = fine ? rr : nullptr;
+    if (fine) {
+        if (mg->o.nu_down > 0) level_smooth<1, float, ST>(mg, l, mg->o.nu_down, false, true, stop);
+        else fill_t<float>(v.x, cells, 0.0f);
+        kprof_begin("residual");
+        k_residual<1, float, float, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
+            level_dev<ST>(mg, v), (const float*)v.x, (const float*)v.b, (float*)v.buf, stop);
+        kprof_end("residual");
+        kprof_begin("restrict");
+        k_restrict<1, float, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, v.n[0], v.n[1], v.n[2], (const float*)v.buf, v.mask, c.n[0], c.n[1], c.n[2], (ST*)c.b, c.mask,
+            scale, stop);
+        kprof_end("restrict");
+    } else {
+        if (mg->o.nu_down > 0) level_smooth<0, ST, ST>(mg, l, mg->o.nu_down, false, true, stop);
+        else fill_t<ST>(v.x, cells, 0.0f);
+        k_residual<0, ST, ST, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
+            level_dev<ST>(mg, v), (const ST*)v.x, (const ST*)v.b, (ST*)v.buf, stop);
+        k_restrict<0, ST, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, v.n[0], v.n[1], v.n[2], (const ST*)v.buf, v.mask, c.n[0], c.n[1], c.n[2], (ST*)c.b, c.mask,
+            nullptr, stop);
+    }
+    FLUX_TRY(vcycle_t<ST>(mg, l + 1, stop, rr));
+    if (fine) {
+        kprof_begin("correct");
+        k_correct<1, ST, float><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, c.n[0], c.n[1], c.n[2], (const ST*)c.x, v.n[0], v.n[1], v.n[2], (float*)v.x, v.mask, scale, stop);
+        kprof_end("correct");
+        level_smooth<1, float, ST>(mg, l, mg->o.n_final, true, false, stop);
+    } else {
+        k_correct<0, ST, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, c.n[0], c.n[1], c.n[2], (const ST*)c.x, v.n[0], v.n[1], v.n[2], (ST*)v.x, v.mask, nullptr, stop);
+        level_smooth<0, ST, ST>(mg, l, mg->o.nu_up, true, false, stop);
+    }
+    return FLUX_OK;
+}
+
+}  // namespace
+
+namespace flux {
+
+int mg_refresh(flux_multigrid* mg) {
+    switch (mg->o.storage) {
+        case FLUX_STORE_F16: return refresh_t<__half>(mg);
+        case FLUX_STORE_BF16: return refresh_t<__hip_bfloat16>(mg);
+        default: return refresh_t<float>(mg);
+    }
+}
+
+// z = M r on the stream.  `stop` (device) short-circuits every kernel; `rr`
+// (device, may be null) is r.r, used for the FP16 residual scale.
+int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr) {
+    mg->lv[0].x = z;
+    mg->lv[0].b = (void*)r;
+    switch (mg->o.storage) {
+        case FLUX_STORE_F16: FLUX_TRY(vcycle_t<__half>(mg, 0, stop, rr)); break;
+        case FLUX_STORE_BF16: FLUX_TRY(vcycle_t<__hip_bfloat16>(mg, 0, stop, nullptr)); break;
+        default: FLUX_TRY(vcycle_t<float>(mg, 0, stop, nullptr)); break;
+    }
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int64_t mg_cells(const flux_multigrid* mg) { return cells_of(mg->lv[0].n); }
+
+}  // namespace flux
+
+extern "C" {
+
+void flux_mg_options_default(flux_mg_options* o) {
+    o->levels = 1;
+    o->nu_down = 2;
+    o->nu_up = 2;
+    o->n_coarsest = 20;
+    o->n_final = 2;
+    o->omega = 1.0;
+    o->storage = FLUX_STORE_F32;
+}
+
+int flux_multigrid_destroy(flux_multigrid* mg) {
+    if (!mg) return FLUX_OK;
+    if (g_init) (void)hipStreamSynchronize(g_stream);
+    for (size_t l = 0; l < mg->lv.size(); ++l) {
+        MGLevel& v = mg->lv[l];
+        dfree(v.buf);
+        dfree(v.alt);
+        if (v.own) {
+            dfree(v.x);
+            dfree(v.b);
+            dfree(v.mask);
+        }
+        if (v.own || v.own_w)
+            for (int a = 0; a < 3; ++a) dfree(v.w[a]);
+    }
+    delete mg;
+    return FLUX_OK;
+}
+
+int flux_multigrid_create(int ndim, const int* shape, const double* spacing, const flux_mg_options* opts,
+                          const flux_field* mask, const flux_field* coarse_mask, const flux_field* const* w,
+                          flux_multigrid** out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(out && shape && spacing, "multigrid: NULL argument");
+    FLUX_TRY(check_shape(ndim, shape, spacing, ndim, "multigrid"));
+    flux_mg_options o;
+    if (opts) o = *opts;
+    else flux_mg_options_default(&o);
+    FLUX_REQUIRE(o.levels >= 1 && o.levels <= 12, "multigrid: numberOfLevels %d out of range", o.levels);
+    FLUX_REQUIRE(o.storage == FLUX_STORE_F32 || o.storage == FLUX_STORE_F16 || o.storage == FLUX_STORE_BF16,
+                 "multigrid: unknown storage %d", o.storage);
+    FLUX_REQUIRE(o.nu_down >= 0 && o.nu_up >= 0 && o.n_coarsest >= 0 && o.n_final >= 0,
+                 "multigrid: iteration counts must be >= 0");
+    int div = 1 << (o.levels - 1);
+    for (int a = 0; a < ndim; ++a)
+        if (shape[a] % div)
+            return set_error(FLUX_ERR_ARG,
+                             "multigrid: shape [%d, %d, %d] is not divisible by 2^(numberOfLevels-1)=%d (numberOfLevels=%d).",
+                             shape[0], shape[1], ndim > 2 ? shape[2] : 1, div, o.levels);
+    int n0[3] = {shape[0], shape[1], ndim > 2 ? shape[2] : 1};
+    if (mask) FLUX_TRY(check_field(mask, FLUX_U8, cells_of(n0), "multigrid: dirichletMask"));
+    if (coarse_mask) FLUX_TRY(check_field(coarse_mask, FLUX_U8, cells_of(n0), "multigrid: coarseDirichletMask"));
+    if (w)
+        for (int a = 0; a < ndim; ++a) FLUX_TRY(check_field(w[a], FLUX_F32, faces_of(n0, a), "multigrid: faceWeights"));
+    flux_multigrid* mg = new flux_multigrid();
+    mg->ndim = ndim;
+    mg->o = o;
+    mg->mask = mask;
+    mg->coarse_mask = coarse_mask;
+    for (int a = 0; a < ndim; ++a) mg->w[a] = w ? w[a] : nullptr;
+    mg->has_w = (w != nullptr);
+    mg->has_masks = (mask != nullptr) || (w != nullptr);
+    const char* fenv = getenv("FLUX_FUSED_SWEEP");
+    mg->fused = !(fenv && fenv[0] == '0');
+    mg->lv.resize(o.levels);
+    const size_t es = store_size(o.storage);
+    bool ok = true;
+    for (int l = 0; l < o.levels; ++l) {
+        MGLevel& v = mg->lv[l];
+        for (int a = 0; a < 3; ++a) {
+            v.n[a] = (a < ndim) ? (n0[a] >> l) : 1;
+            v.h[a] = (a < ndim) ? spacing[a] * (double)(1 << l) : 1.0;
+        }
+        int64_t cells = cells_of(v.n);
+        size_t vs = (l == 0) ? 4 : es;  // finest vectors stay FP32
+        v.buf = dmalloc(cells * vs);
+        ok = ok && v.buf;
+        if (ndim == 3 && mg->fused) {
+            v.alt = dmalloc(cells * vs);
+            ok = ok && v.alt;
+        }
+        if (l == 0) {
+            v.mask = mask ? (uint8_t*)mask->d : nullptr;
+            for (int a = 0; a < ndim; ++a) {
+                if (!w) continue;
+                if (o.storage == FLUX_STORE_F32) {
+                    v.w[a] = w[a]->d;
+                } else {
+                    v.own_w = true;
+                    v.w[a] = dmalloc(faces_of(v.n, a) * es);
+                    ok = ok && v.w[a];
+                }
+            }
+        } else {
+            v.own = true;
+            v.x = dmalloc(cells * vs);
+            v.b = dmalloc(cells * vs);
+            ok = ok && v.x && v.b;
+            if (mg->has_masks) {
+                v.mask = (uint8_t*)dmalloc(cells);
+                ok = ok && v.mask;
+                if (v.mask) fill_device(v.mask, FLUX_U8, cells, 0);
+            }
+            if (mg->has_w)
+                for (int a = 0; a < ndim; ++a) {
+                    v.w[a] = dmalloc(faces_of(v.n, a) * es);
+                    ok = ok && v.w[a];
+                }
+        }
+    }
+    if (!ok) {
+        flux_multigrid_destroy(mg);
+        return set_error(FLUX_ERR_NOMEM, "multigrid: device allocation failed");
+    }
+    // coarse fields start at the reference's unrefreshed defaults (x, b 0; weights 1)
+    for (int l = 1; l < o.levels; ++l) {
+        MGLevel& v = mg->lv[l];
+        int64_t cells = cells_of(v.n);
+        for (void* p : {v.x, v.b}) {
+            if (o.storage == FLUX_STORE_F32) fill_t<float>(p, cells, 0.0f);
+            else if (o.storage == FLUX_STORE_F16) fill_t<__half>(p, cells, 0.0f);
+            else fill_t<__hip_bfloat16>(p, cells, 0.0f);
+        }
+    }
+    int rc = mg_refresh(mg);
+    if (rc == FLUX_OK && hipStreamSynchronize(g_stream) != hipSuccess) rc = set_error(FLUX_ERR_HIP, "multigrid: setup failed");
+    if (rc) {
+        flux_multigrid_destroy(mg);
+        return rc;
+    }
+    *out = mg;
+    return FLUX_OK;
+}
+
+int flux_multigrid_refresh(flux_multigrid* mg) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(mg, "refreshCoarseLevels: NULL multigrid");
+    FLUX_TRY(mg_refresh(mg));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    return FLUX_OK;
+}
+
+int flux_multigrid_apply(flux_multigrid* mg, const flux_field* in, flux_field* out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(mg, "multigrid: NULL");
+    int64_t cells = cells_of(mg->lv[0].n);
+    FLUX_TRY(check_field(in, FLUX_F32, cells, "multigrid: input"));
+    FLUX_TRY(check_field(out, FLUX_F32, cells, "multigrid: output"));
+    FLUX_REQUIRE(in->d != out->d, "multigrid: input and output must differ");
+    FLUX_TRY(mg_apply_raw(mg, (const float*)in->d, (float*)out->d, nullptr, nullptr));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    return FLUX_OK;
+}
+
+int flux_multigrid_level(flux_multigrid* mg, int level, int* shape_out, uint8_t* mask_out, float* const* w_out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(mg && level >= 0 && level < (int)mg->lv.size(), "multigrid: level out of range");
+    MGLevel& v = mg->lv[level];
+    if (shape_out)
+        for (int a = 0; a < 3; ++a) shape_out[a] = v.n[a];
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    if (mask_out && v.mask) FLUX_HIP(hipMemcpy(mask_out, v.mask, cells_of(v.n), hipMemcpyDeviceToHost));
+    if (w_out)
+        for (int a = 0; a < mg->ndim; ++a) {
+            if (!w_out[a] || !v.w[a]) continue;
+            int64_t nf = faces_of(v.n, a);
+            if (level == 0 && mg->o.storage == FLUX_STORE_F32) {
+                FLUX_HIP(hipMemcpy(w_out[a], v.w[a], nf * 4, hipMemcpyDeviceToHost));
+                continue;
+            }
+            float* tmp = (float*)dmalloc(nf * 4);
+            FLUX_REQUIRE(tmp, "multigrid: allocation failed");
+            if (mg->o.storage == FLUX_STORE_F16)
+                k_convert<__half, float><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>((const __half*)v.w[a], tmp, (int)nf);
+            else if (mg->o.storage == FLUX_STORE_BF16)
+                k_convert<__hip_bfloat16, float><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>(
+                    (const __hip_bfloat16*)v.w[a], tmp, (int)nf);
+            else
+                k_convert<float, float><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>((const float*)v.w[a], tmp, (int)nf);
+            hipError_t e = hipMemcpy(w_out[a], tmp, nf * 4, hipMemcpyDeviceToHost);
+            dfree(tmp);
+            FLUX_HIP(e);
+        }
+    return FLUX_OK;
+}
+
+}  // extern "C"

@@ -6,5 +6,7 @@ namespace flux {
 int fill_device(void* p, int dtype, int64_t n, double v);
 int check_shape(int ndim, const int* shape, const double* spacing, int spacing_len, const char* who);
 int mg_refresh(flux_multigrid* mg);
+int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr);
+int64_t mg_cells(const flux_multigrid* mg);
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
 }  // namespace flux

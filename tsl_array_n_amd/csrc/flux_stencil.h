@@ -8,24 +8,44 @@
 //   a Dirichlet row is -x_c with diagonal -1; the diagonal of a free row is
 //   -sum over present faces of w/h^2 (Dirichlet neighbours included).
 #pragma once
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace flux {
 
-// One multigrid level (or the operator's only level) as the kernels see it.
-struct LevelDev {
+// storage conversions: every kernel computes in FP32 (or FP64) and stores in
+// the level's storage type
+template <typename T>
+__device__ __forceinline__ T to_store(float v);
+template <>
+__device__ __forceinline__ float to_store<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ __half to_store<__half>(float v) { return __float2half(v); }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+template <typename T>
+__device__ __forceinline__ float to_f32(T v) { return static_cast<float>(v); }
+
+// One multigrid level (or the operator's only level) as the kernels see it;
+// WT is the storage type of the face weights.
+template <typename WT>
+struct LevelDevT {
     int n0, n1, n2;      // cells per axis (n2 == 1 in 2-D)
     int ndim;
     float ih2[3];        // 1/h^2 per axis
     double ih2d[3];
     const uint8_t* mask; // 1 = Dirichlet row; may be null
-    const float* w0;     // face weights per axis; all null = unit weights
-    const float* w1;
-    const float* w2;
+    const WT* w0;        // face weights per axis; all null = unit weights
+    const WT* w1;
+    const WT* w2;
 };
+using LevelDev = LevelDevT<float>;
 
-__device__ __forceinline__ void cell_coords(const LevelDev& L, int c, int& i, int& j, int& k) {
+template <typename WT>
+__device__ __forceinline__ void cell_coords(const LevelDevT<WT>& L, int c, int& i, int& j, int& k) {
     int row = c / L.n0;
     i = c - row * L.n0;
     k = row / L.n1;
@@ -33,19 +53,21 @@ __device__ __forceinline__ void cell_coords(const LevelDev& L, int c, int& i, in
 }
 
 // lower-face index of cell (i,j,k) on each axis; upper = lower + fstride
-__device__ __forceinline__ int face_lo(const LevelDev& L, int axis, int i, int j, int k, int c) {
+template <typename WT>
+__device__ __forceinline__ int face_lo(const LevelDevT<WT>& L, int axis, int i, int j, int k, int c) {
     if (axis == 0) return i + (L.n0 + 1) * (j + L.n1 * k);
     if (axis == 1) return i + L.n0 * (j + (L.n1 + 1) * k);
     return c;
 }
-__device__ __forceinline__ int face_up_stride(const LevelDev& L, int axis) {
+template <typename WT>
+__device__ __forceinline__ int face_up_stride(const LevelDevT<WT>& L, int axis) {
     return axis == 0 ? 1 : (axis == 1 ? L.n0 : L.n0 * L.n1);
 }
 
 // Row of A applied to a field read through `ld(index)`, arithmetic in T.
 // Writes the row's diagonal to *diag when diag is non-null.
-template <typename T, typename LD>
-__device__ __forceinline__ T stencil_row(const LevelDev& L, int i, int j, int k, int c, LD ld, T* diag) {
+template <typename T, typename WT, typename LD>
+__device__ __forceinline__ T stencil_row(const LevelDevT<WT>& L, int i, int j, int k, int c, LD ld, T* diag) {
     const uint8_t* m = L.mask;
     T center = ld(c);
     if (m && m[c]) {
@@ -55,7 +77,7 @@ __device__ __forceinline__ T stencil_row(const LevelDev& L, int i, int j, int k,
     const int n[3] = {L.n0, L.n1, L.n2};
     const int I[3] = {i, j, k};
     const int st[3] = {1, L.n0, L.n0 * L.n1};
-    const float* w[3] = {L.w0, L.w1, L.w2};
+    const WT* w[3] = {L.w0, L.w1, L.w2};
     T sum = T(0), dsum = T(0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -73,8 +95,8 @@ __device__ __forceinline__ T stencil_row(const LevelDev& L, int i, int j, int k,
             }
             if (w[a]) {
                 int fl = face_lo(L, a, i, j, k, c);
-                T wl = (T)w[a][fl];
-                T wu = (T)w[a][fl + face_up_stride(L, a)];
+                T wl = (T)to_f32(w[a][fl]);
+                T wu = (T)to_f32(w[a][fl + face_up_stride(L, a)]);
                 dLo = dLo * wl;
                 dUp = dUp * wu;
                 hLo = hLo * wl;
