@@ -33,7 +33,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--kernel", default="relax", help="kernel family timed for the roofline")
+    ap.add_argument("--kernel", default="sweep", help="kernel family timed for the roofline")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="V-cycle storage: f32 is the reference's plain method, f16/bf16 the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=2, help="PCG iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -46,6 +48,9 @@ ALGO_BYTES_PER_CELL = {
     # one colour of red-black GS: x read (4) + b on the colour (2) + mask (1)
     # + face weights (12) + x written on the colour (2)
     "relax": 21.0,
+    # one fused red-black sweep (both colours): x in (4) + b (4) + mask (1)
+    # + face weights (12) + x out (4)
+    "sweep": 25.0,
     # p = z + beta p (z 4 + p 4 read, p 4 written), Ap written (4), mask 1, weights 12
     "apply": 29.0,
     # x64 read+write (16), p 4, b 4, r written 4, mask 1, weights 12
@@ -109,7 +114,8 @@ def main():
     act = F.array_n("u8", shape, sc["active"])
     tgt = F.array_n("float", shape, sc["target"])
     ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
-                                        face_weights=beta, multigrid={"levels": args.levels}, tolerance=1e-6,
+                                        face_weights=beta, multigrid={"levels": args.levels, "storage": args.storage},
+                                        tolerance=1e-6,
                                         relative_tolerance=True)
 
     def step():
@@ -155,7 +161,7 @@ def main():
         cpu = cpu_baseline(sc, args.levels, int(np.median(iters)), args.cpu_iters, threads)
 
     line = {
-        "metric": "projections/sec to 1e-6 rel residual at 256^3 (plain FP32 MGPCG)",
+        "metric": "projections/sec to 1e-6 rel residual at 256^3",
         "value": args.steps / elapsed,
         "unit": "projections/s",
         "n_gpus": 1,
@@ -168,7 +174,7 @@ def main():
         "dtype": "f32 (f64 accumulator, residual and dots)",
         "data": "synthetic: seeded 3-D two-phase dam break, frame-0 system",
         "config": {"workload": "3D FLIP two-phase dam break %d^3, liquid:air 1000:1, pinned top-centre, "
-                               "MGPCG %d levels 2/2 RBGS, cold start" % (n, args.levels),
+                               "MGPCG %d levels 2/2 RBGS, V-cycle storage %s, cold start" % (n, args.levels, args.storage),
                    "cells": int(cells), "particles": sc["particles"], "iterations": int(np.median(iters)),
                    "parallelism": "single GPU", "device": dev},
         "roofline": roof,

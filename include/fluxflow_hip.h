@@ -250,6 +250,31 @@ int flux_two_phase_beta(const flux_grid3* g, const flux_field* cell_start, float
 /* MAC divergence at cell centres (grid_math3.js:397-410) */
 int flux_divergence(const flux_grid3* g, const flux_field* const* vel, flux_field* out);
 
+/* -------------------------------------------------- learned initial guess */
+/* No reference counterpart (the reference has no learned component): a 3-D
+ * conv U-Net (4 levels, 16/32/64/64 channels, 3x3x3, ~0.69M parameters)
+ * predicting the pressure from (b / rms(b), Dirichlet mask, log face weight),
+ * trained by flux_unet_train_step on pairs the MGPCG solver produces.
+ * Parameters are FP32, flattened layer by layer as w[co][ci][27] then b[co]. */
+typedef struct flux_unet flux_unet;
+int64_t flux_unet_param_count(void);
+/* grid divisible by 8; training != 0 also allocates gradients and Adam state */
+int flux_unet_create(int nx, int ny, int nz, int training, flux_unet** out);
+int flux_unet_destroy(flux_unet* u);
+int flux_unet_set_params(flux_unet* u, const float* host, int64_t count);
+/* which = 0: parameters, 1: gradients of the last train step */
+int flux_unet_get_params(flux_unet* u, float* host, int64_t count, int which);
+/* input FLUX_F32 [3][nz][ny][nx] -> out FLUX_F32 [nz][ny][nx] */
+int flux_unet_forward(flux_unet* u, const flux_field* input, flux_field* out);
+/* forward, MSE against target, backward, and an Adam step when lr > 0 */
+int flux_unet_train_step(flux_unet* u, const flux_field* input, const flux_field* target, float lr, float beta1,
+                         float beta2, float eps, double* loss);
+/* input channels from an assembled system (b, Dirichlet mask, face weights) */
+int flux_unet_features(flux_unet* u, const flux_field* b, const flux_field* mask, const flux_field* const* weights,
+                       float density_ratio, flux_field* input);
+/* pressure guess = network output x rms(b) */
+int flux_unet_guess(flux_unet* u, const flux_field* input, flux_field* guess);
+
 /* ----------------------------------------------------------------- timing */
 /* HIP-event timing of the library stream */
 int flux_timer_start(void);

@@ -163,7 +163,6 @@ def test_mixed_precision_projection_reaches_same_tolerance(flux, storage):
     n = 64
     sc = scenes.dam_break_system(n)
     ps, vel, diag = gpu_project(F, sc, 5, 1e-6)
-    ps2, vel2, diag2 = gpu_project(F, sc, 5, 1e-6) if False else (None, None, None)
     n_, h = sc["shape"], sc["spacing"]
     vin = [F.array_n("float", F.face_shape(n_, a), sc["vel"][a]) for a in range(3)]
     beta = [F.array_n("float", F.face_shape(n_, a), sc["beta"][a]) for a in range(3)]

@@ -442,3 +442,53 @@ def test_two_phase_beta_range():
     # an all-liquid column gives beta 1, an all-gas one 1000, the interface ~2
     assert np.isclose(bu.min(), 1.0) and np.isclose(bu.max(), 1000.0)
     assert np.any(np.isclose(bv, 1000.0 / 500.5, rtol=1e-6))
+
+
+# ------------------------------------------------------- learned-guess oracle
+def test_unet_conv_matches_scipy_correlate():
+    from scipy.signal import correlate
+    from oracle import unet_ref as U
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 6, 5, 7))
+    w = rng.standard_normal((4, 3, 3, 3, 3))
+    b = rng.standard_normal(4)
+    got = U.conv3d(x, w, b)
+    for co in range(4):
+        ref = sum(correlate(x[ci], w[co, ci], mode="same", method="direct") for ci in range(3)) + b[co]
+        np.testing.assert_allclose(got[co], ref, rtol=1e-12, atol=1e-12)
+
+
+def test_unet_gradients_match_finite_differences():
+    from oracle import unet_ref as U
+    assert 600_000 < U.param_count() < 720_000
+    rng = np.random.default_rng(1)
+    p = U.init_params(3)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.1
+    x = rng.standard_normal((3, 8, 8, 8))
+    t = rng.standard_normal((1, 8, 8, 8))
+    y, cache = U.forward(p, x)
+    loss, gy = U.mse_loss(y, t)
+    g, gx = U.backward(p, cache, gy)
+    eps = 1e-6
+    for name in ["enc0a.w", "mid.w", "dec2.b", "dec0a.w", "out.w", "enc1b.b"]:
+        arr = p[name]
+        for _ in range(3):
+            idx = tuple(rng.integers(0, s) for s in arr.shape)
+            old = arr[idx]
+            arr[idx] = old + eps
+            lp = U.mse_loss(U.forward(p, x)[0], t)[0]
+            arr[idx] = old - eps
+            lm = U.mse_loss(U.forward(p, x)[0], t)[0]
+            arr[idx] = old
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - g[name][idx]) <= 1e-6 + 1e-4 * abs(fd), (name, idx, fd, g[name][idx])
+    idx = (1, 3, 4, 5)
+    old = x[idx]
+    x[idx] = old + eps
+    lp = U.mse_loss(U.forward(p, x)[0], t)[0]
+    x[idx] = old - eps
+    lm = U.mse_loss(U.forward(p, x)[0], t)[0]
+    x[idx] = old
+    assert abs((lp - lm) / (2 * eps) - gx[idx]) <= 1e-6 + 1e-4 * abs(gx[idx])

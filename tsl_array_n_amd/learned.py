@@ -1,0 +1,108 @@
+"""Learned initial guess: the U-Net of csrc/flux_unet.hip (host side).
+
+Input channels (built on the GPU from an assembled system): b / rms(b), the
+Dirichlet mask, log(mean face weight) / log(density ratio).  Target: the
+pressure divided by rms(b).  The guess handed to the solver is the network
+output times rms(b); the solver then runs its usual PCG from it, so a learned
+solve is accepted only at the same FP64 true-residual tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _check, _lib, array_n
+
+_lib.flux_unet_param_count.restype = C.c_int64
+
+LAYERS = [("enc0a", 3, 16), ("enc0b", 16, 16), ("enc1a", 16, 32), ("enc1b", 32, 32), ("enc2a", 32, 64),
+          ("enc2b", 64, 64), ("mid", 64, 64), ("dec2", 128, 64), ("dec1a", 96, 32), ("dec1b", 32, 32),
+          ("dec0a", 48, 16), ("dec0b", 16, 16), ("out", 16, 1)]
+
+
+def param_count():
+    return int(_lib.flux_unet_param_count())
+
+
+def flatten(params):
+    """dict name.w [co, ci, 3, 3, 3] / name.b [co] -> flat FP32 vector"""
+    return np.concatenate([np.concatenate([np.asarray(params[n + ".w"], np.float32).ravel(),
+                                           np.asarray(params[n + ".b"], np.float32).ravel()]) for n, _, _ in LAYERS])
+
+
+def unflatten(vec):
+    out, o = {}, 0
+    for n, ci, co in LAYERS:
+        k = ci * co * 27
+        out[n + ".w"] = vec[o:o + k].reshape(co, ci, 3, 3, 3)
+        o += k
+        out[n + ".b"] = vec[o:o + co]
+        o += co
+    return out
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = {}
+    for n, ci, co in LAYERS:
+        bound = np.sqrt(6.0 / (ci * 27))
+        p[n + ".w"] = rng.uniform(-bound, bound, (co, ci, 3, 3, 3)).astype(np.float32)
+        p[n + ".b"] = np.zeros(co, np.float32)
+    # a small output layer: the guess starts near zero
+    p["out.w"] *= 0.1
+    return p
+
+
+class UNet:
+    def __init__(self, n, training=False, params=None):
+        self.n = tuple(n)
+        h = C.c_void_p()
+        _check(_lib.flux_unet_create(int(n[0]), int(n[1]), int(n[2]), int(bool(training)), C.byref(h)), "unet")
+        self.h = h.value
+        self.count = param_count()
+        self.set_params(init_params() if params is None else params)
+        cells = int(np.prod(n))
+        self.input = array_n("float", (3 * cells,))
+        self.output = array_n("float", (cells,))
+
+    def set_params(self, params):
+        vec = flatten(params) if isinstance(params, dict) else np.ascontiguousarray(params, np.float32)
+        _check(_lib.flux_unet_set_params(C.c_void_p(self.h), vec.ctypes.data_as(C.c_void_p), C.c_int64(vec.size)))
+
+    def get(self, which=0):
+        out = np.empty(self.count, np.float32)
+        _check(_lib.flux_unet_get_params(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p), C.c_int64(self.count),
+                                         int(which)))
+        return out
+
+    def forward(self, x=None):
+        if x is not None:
+            self.input.from_array(x)
+        _check(_lib.flux_unet_forward(C.c_void_p(self.h), C.c_void_p(self.input.h), C.c_void_p(self.output.h)))
+        return self.output
+
+    def train_step(self, target, x=None, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        if x is not None:
+            self.input.from_array(x)
+        loss = C.c_double()
+        _check(_lib.flux_unet_train_step(C.c_void_p(self.h), C.c_void_p(self.input.h), C.c_void_p(target.h),
+                                         C.c_float(lr), C.c_float(beta1), C.c_float(beta2), C.c_float(eps),
+                                         C.byref(loss)))
+        return loss.value
+
+    def features(self, b, mask, weights, density_ratio):
+        w = (C.c_void_p * 3)(*[f.h for f in weights]) if weights is not None else None
+        _check(_lib.flux_unet_features(C.c_void_p(self.h), C.c_void_p(b.h), C.c_void_p(mask.h if mask else None), w,
+                                       C.c_float(density_ratio), C.c_void_p(self.input.h)))
+        return self.input
+
+    def guess(self, out_field):
+        _check(_lib.flux_unet_guess(C.c_void_p(self.h), C.c_void_p(self.input.h), C.c_void_p(out_field.h)))
+        return out_field
+
+    def __del__(self):
+        try:
+            from . import _initialised
+            if _initialised and self.h:
+                _lib.flux_unet_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
