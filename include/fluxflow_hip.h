@@ -201,6 +201,13 @@ int flux_pressure3_create(const flux_pressure3_desc* d, flux_pressure3** out);
 int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in,
                            const flux_field* w_in, flux_field* u_out, flux_field* v_out,
                            flux_field* w_out, flux_pressure_diag* diag);
+/* the two halves of project(): buildSystemBatch + refreshPreconditionerMask
+ * (grid_pressure_solver3.js:496-512), then the solve from the current
+ * pressure (the warm start), circuit breaker and correction (513-556).  A
+ * caller may write an initial guess into the pressure field in between. */
+int flux_pressure3_assemble(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in);
+int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
+                         flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag);
 /* .pressure.data, .b, .dirichletMask, .ventMask (grid_pressure_solver3.js:572) */
 flux_field* flux_pressure3_field(flux_pressure3* p, int which);
 enum { FLUX_P3_PRESSURE = 0, FLUX_P3_RHS = 1, FLUX_P3_MASK = 2, FLUX_P3_VENT = 3, FLUX_P3_TARGET = 4,
@@ -274,6 +281,9 @@ int flux_unet_features(flux_unet* u, const flux_field* b, const flux_field* mask
                        float density_ratio, flux_field* input);
 /* pressure guess = network output x rms(b) */
 int flux_unet_guess(flux_unet* u, const flux_field* input, flux_field* guess);
+/* the same guess written into `pressure` at the free cells only (Dirichlet
+ * cells keep their targets), ready for flux_pressure3_solve */
+int flux_unet_guess_into(flux_unet* u, const flux_field* input, const flux_field* mask, flux_field* pressure);
 
 /* ----------------------------------------------------------------- timing */
 /* HIP-event timing of the library stream */

@@ -29,7 +29,7 @@ async function main() {
 			active: Math.floor( pos[ 0 ] / h ) === pin[ 0 ] && Math.floor( pos[ 1 ] / h ) === pin[ 1 ] && Math.floor( pos[ 2 ] / h ) === pin[ 2 ],
 			target: 0
 		} ),
-		faceWeights: beta, multigrid: { numberOfLevels: 3 }, tolerance: 1e-6
+		faceWeights: beta, multigrid: { numberOfLevels: 3 }, tolerance: 1e-6, maxIterations: 100
 	} );
 	const dispatch = solver.project( grid, grid );
 	const diag = await dispatch();

@@ -59,3 +59,32 @@ def test_training_reduces_loss(flux, L):
     for _ in range(60):
         last = net.train_step(target, lr=1e-3)
     assert last < 0.5 * first
+
+
+def test_learned_guess_is_accepted_at_the_same_tolerance(flux, L):
+    """Trained on MGPCG-produced pairs; on held-out scenes the learned start
+    reaches the same FP64 true-residual tolerance as the cold start, and the
+    iteration counts of both are reported."""
+    F = flux
+    from tsl_array_n_amd import scenes
+    n = 32
+    pairs = L.training_pairs(F, n, 9, seed=5, levels=4)
+    net, losses = L.train(F, n, pairs, 150, lr=2e-3)
+    assert losses[-1] < losses[0]
+    results = []
+    for seed in (101, 202):
+        sc = scenes.random_scene(n, seed)
+        shape = sc["shape"]
+        beta = [F.array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+        act = F.array_n("u8", shape, sc["active"])
+        tgt = F.array_n("float", shape, sc["target"])
+        out = []
+        for learned in (False, True):
+            vel = [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+            ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                                face_weights=beta, multigrid={"levels": 4}, tolerance=1e-6)
+            d = L.LearnedProjector(ps, net, sc["ratio"]).project(vel) if learned else ps.project(vel)
+            assert d["converged"] and d["residual"] <= 1e-6 * d["bNorm"]
+            out.append(d["iterations"])
+        results.append(out)
+    print("iterations cold / learned:", results)

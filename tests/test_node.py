@@ -53,7 +53,7 @@ def test_projection_through_node_matches_oracle(tmp_path):
     r = run_node("projection.mjs", n, tmp_path, timeout=300)
     print(r.stdout)
     meta = json.load(open(tmp_path / "diag.json"))
-    assert meta["outside"] == 0 and meta["diag"]["converged"]
+    assert meta["outside"] == 0 and meta["diag"]["converged"], meta
     sc = scenes.dam_break_system(n)
     np.testing.assert_array_equal(np.fromfile(tmp_path / "fraction.bin", np.float32), sc["fraction"])
     A = O.pressure3_assemble(sc["shape"], sc["spacing"], sc["vel"], active=sc["active"], target=sc["target"],
@@ -61,7 +61,7 @@ def test_projection_through_node_matches_oracle(tmp_path):
     mg = O.Multigrid(sc["shape"], sc["spacing"], levels=3, mask=A["mask"], coarse_mask=A["vent"],
                      weights=A["weights"], suffix="f32")
     x, x64, st = O.pcg_solve_mixed(sc["shape"], sc["spacing"], A["b"], x0=A["pressure"], mask=A["mask"],
-                                   weights=A["weights"], mg=mg, tol=1e-6, maxiter=24)
+                                   weights=A["weights"], mg=mg, tol=1e-6, maxiter=100)
     p = np.fromfile(tmp_path / "pressure.bin", np.float32)
     assert np.linalg.norm(p - x) <= 1e-5 * np.linalg.norm(x)
     out = O.pressure3_correct(sc["shape"], sc["spacing"], x, sc["vel"], beta=sc["beta"], suffix="f32")

@@ -207,6 +207,9 @@ class LaplacianOperator:
     """createLaplacianOperator(shape, gridSpacing, {dirichletMask, faceWeights})."""
 
     def __init__(self, shape, spacing, dirichlet_mask=None, face_weights=None):
+        if len(spacing) != len(shape):
+            raise FluxError("createLaplacianOperator: gridSpacing length (%d) must match shape length (%d)."
+                            % (len(spacing), len(shape)))
         self.shape = tuple(shape)
         self._keep = (dirichlet_mask, face_weights)
         h = C.c_void_p()
@@ -241,6 +244,8 @@ class MultigridPreconditioner:
     def __init__(self, shape, spacing, dirichlet_mask=None, coarse_dirichlet_mask=None, face_weights=None, **opts):
         if len(shape) < 1 or len(shape) > 3:
             raise FluxError("multigrid: only 1D/2D/3D shapes are supported, got %dD." % len(shape))
+        if len(spacing) != len(shape):
+            raise FluxError("multigrid: gridSpacing length (%d) must match shape length (%d)." % (len(spacing), len(shape)))
         self.shape = tuple(shape)
         self.options = mg_options(**opts)
         self._keep = (dirichlet_mask, coarse_dirichlet_mask, face_weights)
@@ -371,10 +376,25 @@ class GridPressureSolver3:
         self.b = self._view(1, F32, n)
         self.dirichlet_mask = self._view(2, U8, n)
         self.vent_mask = self._view(3, U8, n)
+        self.weights = None
+        if face_weights is not None or collider_weights is not None:
+            self.weights = [self._view(5 + a, F32, face_shape(n, a)) for a in range(3)]
 
     def _view(self, which, dtype, shape):
         ptr = _lib.flux_pressure3_field(C.c_void_p(self.h), which)
         return Field(dtype, shape, handle=ptr, owned=False) if ptr else None
+
+    def assemble(self, vin):
+        """first half of project(): Dirichlet fields, weights, right-hand side, coarse levels"""
+        _check(_lib.flux_pressure3_assemble(C.c_void_p(self.h), *[C.c_void_p(f.h) for f in vin]), "assemble")
+
+    def solve(self, vin, vout=None):
+        """second half of project(): PCG from the current pressure, breaker, correction"""
+        vout = vin if vout is None else vout
+        dg = PressureDiag()
+        _check(_lib.flux_pressure3_solve(C.c_void_p(self.h), *[C.c_void_p(f.h) for f in vin],
+                                         *[C.c_void_p(f.h) for f in vout], C.byref(dg)), "solve")
+        return self._diag(dg)
 
     def project(self, vin, vout=None):
         """project(input, output) + dispatch(): vin/vout are (U, V, W) fields."""
@@ -382,6 +402,9 @@ class GridPressureSolver3:
         dg = PressureDiag()
         _check(_lib.flux_pressure3_project(C.c_void_p(self.h), *[C.c_void_p(f.h) for f in vin],
                                            *[C.c_void_p(f.h) for f in vout], C.byref(dg)), "project")
+        return self._diag(dg)
+
+    def _diag(self, dg):
         self.diagnostics = {"converged": bool(dg.converged), "rejected": bool(dg.rejected), "iterations": dg.iterations,
                             "stoppedBy": STOP_NAMES.get(dg.stopped_by, "unknown"), "residual": dg.residual,
                             "bNorm": dg.b_norm, "badCells": dg.bad_cells}

@@ -397,12 +397,21 @@ void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
     for (int it = 0; it < iters; ++it) {
         XT* dst = (((iters - 1 - it) & 1) == 0) ? (XT*)v.x : (XT*)v.alt;
         if (l == 0) kprof_begin("sweep");
-        if (from_zero && it == 0)
-            k_rbgs_fused<SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
-        else
-            k_rbgs_fused<SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        if (from_zero && it == 0) {
+            if (l == 0)
+                k_rbgs_fused<1, SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                    L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+            else
+                k_rbgs_fused<0, SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                    L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        } else {
+            if (l == 0)
+                k_rbgs_fused<1, SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                    L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+            else
+                k_rbgs_fused<0, SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
+                    L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        }
         if (l == 0) kprof_end("sweep");
         src = dst;
     }
@@ -604,8 +613,10 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     for (int a = 0; a < ndim; ++a) mg->w[a] = w ? w[a] : nullptr;
     mg->has_w = (w != nullptr);
     mg->has_masks = (mask != nullptr) || (w != nullptr);
+    // the one-pass sweep is opt-in (FLUX_FUSED_SWEEP=1) until its z-march
+    // stops being latency-bound; the two-launch form is the default
     const char* fenv = getenv("FLUX_FUSED_SWEEP");
-    mg->fused = !(fenv && fenv[0] == '0');
+    mg->fused = (fenv && fenv[0] == '1');
     mg->lv.resize(o.levels);
     const size_t es = store_size(o.storage);
     bool ok = true;
@@ -703,13 +714,17 @@ int flux_multigrid_level(flux_multigrid* mg, int level, int* shape_out, uint8_t*
     if (shape_out)
         for (int a = 0; a < 3; ++a) shape_out[a] = v.n[a];
     FLUX_HIP(hipStreamSynchronize(g_stream));
-    if (mask_out && v.mask) FLUX_HIP(hipMemcpy(mask_out, v.mask, cells_of(v.n), hipMemcpyDeviceToHost));
+    if (mask_out && v.mask) {
+        FLUX_HIP(hipMemcpyAsync(mask_out, v.mask, cells_of(v.n), hipMemcpyDeviceToHost, g_stream));
+        FLUX_HIP(hipStreamSynchronize(g_stream));
+    }
     if (w_out)
         for (int a = 0; a < mg->ndim; ++a) {
             if (!w_out[a] || !v.w[a]) continue;
             int64_t nf = faces_of(v.n, a);
             if (level == 0 && mg->o.storage == FLUX_STORE_F32) {
-                FLUX_HIP(hipMemcpy(w_out[a], v.w[a], nf * 4, hipMemcpyDeviceToHost));
+                FLUX_HIP(hipMemcpyAsync(w_out[a], v.w[a], nf * 4, hipMemcpyDeviceToHost, g_stream));
+                FLUX_HIP(hipStreamSynchronize(g_stream));
                 continue;
             }
             float* tmp = (float*)dmalloc(nf * 4);
@@ -721,7 +736,8 @@ int flux_multigrid_level(flux_multigrid* mg, int level, int* shape_out, uint8_t*
                     (const __hip_bfloat16*)v.w[a], tmp, (int)nf);
             else
                 k_convert<float, float><<<blocks_for(nf, MTPB), MTPB, 0, g_stream>>>((const float*)v.w[a], tmp, (int)nf);
-            hipError_t e = hipMemcpy(w_out[a], tmp, nf * 4, hipMemcpyDeviceToHost);
+            hipError_t e = hipMemcpyAsync(w_out[a], tmp, nf * 4, hipMemcpyDeviceToHost, g_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
             dfree(tmp);
             FLUX_HIP(e);
         }

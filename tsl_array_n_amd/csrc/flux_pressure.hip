@@ -278,6 +278,12 @@ int flux_pressure3_create(const flux_pressure3_desc* d, flux_pressure3** out) {
     return FLUX_OK;
 }
 
+int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
+                           flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
+    FLUX_TRY(flux_pressure3_assemble(p, u_in, v_in, w_in));
+    return flux_pressure3_solve(p, u_in, v_in, w_in, u_out, v_out, w_out, diag);
+}
+
 flux_field* flux_pressure3_field(flux_pressure3* p, int which) {
     if (!p) return nullptr;
     switch (which) {
@@ -293,20 +299,23 @@ flux_field* flux_pressure3_field(flux_pressure3* p, int which) {
     }
 }
 
-int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
-                           flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
-    FLUX_REQUIRE_INIT();
-    FLUX_REQUIRE(p && diag, "project: NULL argument");
-    const flux_field* vin[3] = {u_in, v_in, w_in};
-    flux_field* vout[3] = {u_out, v_out, w_out};
+static int check_velocity(flux_pressure3* p, const flux_field* const* vin, flux_field* const* vout) {
     for (int a = 0; a < 3; ++a) {
         int rc = check_field(vin[a], FLUX_F32, faces_of(p->n, a), "project input");
         if (rc)
             return set_error(FLUX_ERR_ARG,
                              "createGridPressureSolver3: project() input resolution does not match constructed resolution [%d, %d, %d].",
                              p->n[0], p->n[1], p->n[2]);
-        FLUX_TRY(check_field(vout[a], FLUX_F32, faces_of(p->n, a), "project output"));
+        if (vout) FLUX_TRY(check_field(vout[a], FLUX_F32, faces_of(p->n, a), "project output"));
     }
+    return FLUX_OK;
+}
+
+int flux_pressure3_assemble(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p, "project: NULL argument");
+    const flux_field* vin[3] = {u_in, v_in, w_in};
+    FLUX_TRY(check_velocity(p, vin, nullptr));
     const flux_pressure3_desc& d = p->d;
     Grid3 g{p->n[0], p->n[1], p->n[2]};
     int64_t cells = cells_of(p->n);
@@ -340,6 +349,27 @@ int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux
     FLUX_LAUNCH_CHECK();
     // refreshPreconditionerMask (multigrid.js:1449-1451)
     if (p->mg) FLUX_TRY(mg_refresh(p->mg));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    return FLUX_OK;
+}
+
+int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
+                         flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p && diag, "project: NULL argument");
+    const flux_field* vin[3] = {u_in, v_in, w_in};
+    flux_field* vout[3] = {u_out, v_out, w_out};
+    FLUX_TRY(check_velocity(p, vin, vout));
+    const flux_pressure3_desc& d = p->d;
+    Grid3 g{p->n[0], p->n[1], p->n[2]};
+    int64_t cells = cells_of(p->n);
+    unsigned gc = blocks_for(cells, PTPB);
+    const float* cw[3] = {nullptr, nullptr, nullptr};
+    const float* beta[3] = {nullptr, nullptr, nullptr};
+    for (int a = 0; a < 3; ++a) {
+        if (d.collider_weights[0]) cw[a] = (const float*)d.collider_weights[a]->d;
+        if (d.face_weights[0]) beta[a] = (const float*)d.face_weights[a]->d;
+    }
     flux_pcg_state st;
     FLUX_TRY(pcg_solve_raw(p->pcg, d.tolerance, p->maxiter, d.relative_tolerance, &st));
     // circuit breaker (289-336, 545-554)

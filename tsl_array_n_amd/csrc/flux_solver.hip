@@ -104,17 +104,16 @@ __device__ __forceinline__ bool degenerate(double v) { return !(fabs(v) <= 1.797
 // Setup: x64 = x; r = b - A x (FP64 arithmetic); rr.
 __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, double* __restrict__ x64,
                                      const float* __restrict__ b, float* __restrict__ r, Scal* S, double* partials) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0;
     int n = L.n0 * L.n1 * L.n2;
-    if (c < n) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         int i, j, k;
         cell_coords(L, c, i, j, k);
         double ax = stencil_row<double>(L, i, j, k, c, [&](int q) { return (double)x32[q]; }, (double*)nullptr);
         double rv = (double)b[c] - ax;
         x64[c] = (double)x32[c];
         r[c] = (float)rv;
-        v = rv * rv;
+        v += rv * rv;
     }
     grid_sum_last(v, partials, &S->ticket[0], [&](double tot) {
         S->rr = tot;
@@ -128,9 +127,9 @@ __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, 
 }
 
 __global__ void k_pcg_dot_bb(const float* __restrict__ b, int n, Scal* S, double* partials) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0;
-    if (c < n) v = (double)b[c] * (double)b[c];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        v += (double)b[c] * (double)b[c];
     grid_sum_last(v, partials, &S->ticket[1], [&](double tot) {
         S->bb = tot;
         double nb = sqrt(fabs(tot));
@@ -149,9 +148,9 @@ __global__ void k_pcg_dot_bb(const float* __restrict__ b, int n, Scal* S, double
 __global__ void k_pcg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, int n, Scal* S,
                              double* partials, int first) {
     if (S->stop) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0;
-    if (c < n) v = (double)r[c] * (double)z[c];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        v += (double)r[c] * (double)z[c];
     grid_sum_last(v, partials, &S->ticket[2], [&](double tot) {
         S->rz = tot;
         if (first) {
@@ -183,10 +182,9 @@ __global__ void k_pcg_apply(LevelDev L, const float* __restrict__ z, float* __re
     const float* __restrict__ pold = S->psel ? pb : pa;
     float* __restrict__ pnew = S->psel ? pa : pb;
     const float beta = (float)S->beta;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     int n = L.n0 * L.n1 * L.n2;
     double v = 0;
-    if (c < n) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         int i, j, k;
         cell_coords(L, c, i, j, k);
         auto ld = [&](int q) { return z[q] + pold[q] * beta; };
@@ -194,7 +192,7 @@ __global__ void k_pcg_apply(LevelDev L, const float* __restrict__ z, float* __re
         float pc = ld(c);
         pnew[c] = pc;
         Ap[c] = ap;
-        v = (double)pc * (double)ap;
+        v += (double)pc * (double)ap;
     }
     grid_sum_last(v, partials, &S->ticket[3], [&](double pAp) {
         S->pAp = pAp;
@@ -221,10 +219,9 @@ __global__ void k_pcg_update(LevelDev L, double* __restrict__ xa, double* __rest
     double* __restrict__ xnew = S->xsel ? xa : xb;
     const float* __restrict__ p = S->psel ? pb : pa;  // the p written by k_pcg_apply
     const double alpha = S->alpha;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     int n = L.n0 * L.n1 * L.n2;
     double v = 0;
-    if (c < n) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         int i, j, k;
         cell_coords(L, c, i, j, k);
         auto ld = [&](int q) { return xold[q] + alpha * (double)p[q]; };
@@ -232,7 +229,7 @@ __global__ void k_pcg_update(LevelDev L, double* __restrict__ xa, double* __rest
         xnew[c] = ld(c);
         double rv = (double)b[c] - ax;
         r[c] = (float)rv;
-        v = rv * rv;
+        v += rv * rv;
     }
     grid_sum_last(v, partials, &S->ticket[0], [&](double rr) {
         S->rr = rr;
@@ -393,6 +390,12 @@ struct flux_pcg {
 
 namespace flux {
 
+// Reductions run on a capped grid (grid-stride loops): every block takes one
+// ticket of the last-block reduction, so the block count bounds the atomics
+// serialised on that ticket.
+static constexpr unsigned RED_MAX_BLOCKS = 2048;
+static unsigned red_blocks(int64_t n) { return std::min<unsigned>(blocks_for(n, TPB), RED_MAX_BLOCKS); }
+
 static int pcg_precondition(flux_pcg* s) {
     const int* stop = &s->S->stop;
     unsigned g = blocks_for(s->n, TPB);
@@ -403,7 +406,7 @@ static int pcg_precondition(flux_pcg* s) {
 }
 
 static int pcg_iteration(flux_pcg* s) {
-    unsigned g = blocks_for(s->n, TPB);
+    unsigned g = red_blocks(s->n);
     kprof_begin("apply");
     k_pcg_apply<<<g, TPB, 0, g_stream>>>(s->op->L, s->z, s->pa, s->pb, s->Ap, s->S, s->partials);
     kprof_end("apply");
@@ -412,13 +415,16 @@ static int pcg_iteration(flux_pcg* s) {
                                            s->partials);
     kprof_end("update");
     FLUX_TRY(pcg_precondition(s));
+    kprof_begin("dot");
     k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials, 0);
+    kprof_end("dot");
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
 
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st) {
-    unsigned g = blocks_for(s->n, TPB);
+    unsigned g = red_blocks(s->n);
+    unsigned gfull = blocks_for(s->n, TPB);
     // host-written fields of the scalar block
     memset(s->Sh, 0, sizeof(Scal));
     s->Sh->tol = tol;
@@ -431,8 +437,8 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->S, s->partials);
     FLUX_TRY(pcg_precondition(s));
     k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials, 1);
-    k_fill_f32<<<g, TPB, 0, g_stream>>>(s->pa, (int)s->n, 0.0f);
-    k_fill_f32<<<g, TPB, 0, g_stream>>>(s->pb, (int)s->n, 0.0f);
+    k_fill_f32<<<gfull, TPB, 0, g_stream>>>(s->pa, (int)s->n, 0.0f);
+    k_fill_f32<<<gfull, TPB, 0, g_stream>>>(s->pb, (int)s->n, 0.0f);
     FLUX_LAUNCH_CHECK();
     // chunked self-stopping loop (linalg.js:1854-1905): one scalar read per chunk
     int ran = 0;
@@ -446,7 +452,7 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
         FLUX_HIP(hipMemcpyAsync(s->Sh, s->S, sizeof(Scal), hipMemcpyDeviceToHost, g_stream));
         FLUX_HIP(hipStreamSynchronize(g_stream));
     }
-    k_pcg_copy_out<<<g, TPB, 0, g_stream>>>(s->xa, s->xb, (float*)s->x->d, (int)s->n, s->S);
+    k_pcg_copy_out<<<gfull, TPB, 0, g_stream>>>(s->xa, s->xb, (float*)s->x->d, (int)s->n, s->S);
     FLUX_LAUNCH_CHECK();
     FLUX_HIP(hipStreamSynchronize(g_stream));
     const Scal& h = *s->Sh;

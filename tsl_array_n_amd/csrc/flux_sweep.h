@@ -35,7 +35,7 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return (b % 8u) * (nb / 8u) + b / 8u;
 }
 
-template <int TX, int TY, bool ZERO, typename XT, typename WT>
+template <int FINE, int TX, int TY, bool ZERO, typename XT, typename WT>
 __global__ void __launch_bounds__(TX* TY) k_rbgs_fused(LevelDevT<WT> L, const XT* __restrict__ xin,
                                                        const XT* __restrict__ b, XT* __restrict__ xout,
                                                        float omega, int colorA, int kchunk,

@@ -11,6 +11,7 @@
 // (tap = dz*9 + dy*3 + dx); optimiser Adam.  Upsampling and concat are
 // folded into the conv input loader (channel ci < c1 reads the half-size
 // tensor at (z/2, y/2, x/2)).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -280,6 +281,15 @@ __global__ void k_scale_out(const float* __restrict__ y, const double* __restric
     p[c] = y[c] * rms;
 }
 
+__global__ void k_guess_into(const float* __restrict__ y, const double* __restrict__ bb, const uint8_t* __restrict__ mask,
+                             int64_t nv, float* __restrict__ p) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nv) return;
+    if (mask && mask[c]) return;
+    float rms = (float)sqrt(fmax(*bb / (double)nv, 1e-300));
+    p[c] = y[c] * rms;
+}
+
 // ------------------------------------------------------------------ layers
 struct LayerSpec {
     const char* name;
@@ -338,13 +348,23 @@ void launch_conv(const ConvIn& in, const float* w, const float* b, float* out, i
     k_conv3<C><<<blocks_for(nv, UTPB), UTPB, 0, g_stream>>>(in, w, b, out, cout, nz, ny, nx, relu);
 }
 
+// output channels beyond 64 (the data gradient of dec2 has 128) run as
+// further launches over channel groups of 64
 void conv(const ConvIn& in, const float* w, const float* b, float* out, int cout, int level, const flux_unet* u,
           int relu) {
     int nx = u->n[0] >> level, ny = u->n[1] >> level, nz = u->n[2] >> level;
-    if (cout <= 1) launch_conv<1>(in, w, b, out, cout, nz, ny, nx, relu);
-    else if (cout <= 16) launch_conv<16>(in, w, b, out, cout, nz, ny, nx, relu);
-    else if (cout <= 32) launch_conv<32>(in, w, b, out, cout, nz, ny, nx, relu);
-    else launch_conv<64>(in, w, b, out, cout, nz, ny, nx, relu);
+    const int cin = in.c1 + in.c2;
+    const int64_t nv = (int64_t)nz * ny * nx;
+    for (int co0 = 0; co0 < cout; co0 += 64) {
+        int cc = std::min(64, cout - co0);
+        const float* wg = w + (size_t)co0 * cin * 27;
+        const float* bg = b ? b + co0 : nullptr;
+        float* og = out + (int64_t)co0 * nv;
+        if (cc <= 1) launch_conv<1>(in, wg, bg, og, cc, nz, ny, nx, relu);
+        else if (cc <= 16) launch_conv<16>(in, wg, bg, og, cc, nz, ny, nx, relu);
+        else if (cc <= 32) launch_conv<32>(in, wg, bg, og, cc, nz, ny, nx, relu);
+        else launch_conv<64>(in, wg, bg, og, cc, nz, ny, nx, relu);
+    }
 }
 
 ConvIn cin_single(const float* a, int c) { return ConvIn{a, nullptr, c, 0, 0, nullptr}; }
@@ -644,6 +664,22 @@ int flux_unet_guess(flux_unet* u, const flux_field* input, flux_field* guess) {
     FLUX_TRY(check_field(guess, FLUX_F32, nv, "unet: guess"));
     FLUX_TRY(forward(u, (const float*)input->d));
     k_scale_out<<<blocks_for(nv, UTPB), UTPB, 0, g_stream>>>(u->y[L_OUT], u->scal + 1, nv, (float*)guess->d);
+    FLUX_LAUNCH_CHECK();
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    return FLUX_OK;
+}
+
+int flux_unet_guess_into(flux_unet* u, const flux_field* input, const flux_field* mask, flux_field* pressure) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(u, "unet: NULL");
+    int64_t nv = u->nvox[0];
+    FLUX_TRY(check_field(input, FLUX_F32, 3 * nv, "unet: input"));
+    FLUX_TRY(check_field(pressure, FLUX_F32, nv, "unet: pressure"));
+    if (mask) FLUX_TRY(check_field(mask, FLUX_U8, nv, "unet: mask"));
+    FLUX_TRY(forward(u, (const float*)input->d));
+    k_guess_into<<<blocks_for(nv, UTPB), UTPB, 0, g_stream>>>(u->y[L_OUT], u->scal + 1,
+                                                               mask ? (const uint8_t*)mask->d : nullptr, nv,
+                                                               (float*)pressure->d);
     FLUX_LAUNCH_CHECK();
     FLUX_HIP(hipStreamSynchronize(g_stream));
     return FLUX_OK;

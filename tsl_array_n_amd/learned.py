@@ -106,3 +106,72 @@ class UNet:
                 _lib.flux_unet_destroy(C.c_void_p(self.h))
         except Exception:
             pass
+
+
+class LearnedProjector:
+    """project() with the network's guess as the PCG start:
+    assemble -> features -> guess into the free cells -> solve.  The solve is
+    the plain solver's, so it is accepted only at the same FP64 true-residual
+    tolerance; a bad guess costs iterations, never correctness."""
+
+    def __init__(self, solver, net, density_ratio):
+        self.solver = solver
+        self.net = net
+        self.density_ratio = float(density_ratio)
+
+    def project(self, vin, vout=None):
+        s = self.solver
+        s.assemble(vin)
+        self.net.features(s.b, s.dirichlet_mask, s.weights, self.density_ratio)
+        _check(_lib.flux_unet_guess_into(C.c_void_p(self.net.h), C.c_void_p(self.net.input.h),
+                                         C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
+                                         C.c_void_p(s.pressure.h)), "guess")
+        return s.solve(vin, vout)
+
+
+def training_pairs(F, n, count, seed=0, levels=None):
+    """(input features, normalized pressure) pairs produced by the MGPCG
+    solver itself on randomized two-phase scenes (random dense blobs, density
+    ratio log-uniform in [1, 1000], random divergence) and dam-break variants;
+    each pair stays on the GPU."""
+    from . import scenes
+    levels = levels or max(2, int(np.log2(n)) - 2)
+    rng = np.random.default_rng(seed)
+    pairs = []
+    net = UNet((n, n, n), training=False)
+    for q in range(count):
+        if q % 3 == 2:
+            sc = scenes.dam_break_system(n, box_max=(rng.uniform(0.2, 0.7), rng.uniform(0.3, 0.8), 1.0))
+            ratio = 1000.0
+        else:
+            sc = scenes.random_scene(n, int(rng.integers(0, 2 ** 31)))
+            ratio = sc["ratio"]
+        shape = sc["shape"]
+        vel = [array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+        beta = [array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+        act = array_n("u8", shape, sc["active"])
+        tgt = array_n("float", shape, sc["target"])
+        ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                            face_weights=beta, multigrid={"levels": levels}, tolerance=1e-6)
+        d = ps.project(vel)
+        if not d["converged"]:
+            continue
+        net.features(ps.b, ps.dirichlet_mask, ps.weights, ratio)
+        x = array_n("float", (3 * n ** 3,), net.input.to_array())
+        b = ps.b.to_array().astype(np.float64)
+        rms = np.sqrt((b * b).mean())
+        t = array_n("float", (n ** 3,), (ps.pressure.to_array() / rms).astype(np.float32))
+        pairs.append((x, t))
+    return pairs
+
+
+def train(F, n, pairs, steps, lr=1e-3, seed=0, log=None):
+    net = UNet((n, n, n), training=True, params=init_params(seed))
+    losses = []
+    for s in range(steps):
+        x, t = pairs[s % len(pairs)]
+        net.input.copy_from(x)
+        losses.append(net.train_step(t, lr=lr))
+        if log and (s % 50 == 0 or s == steps - 1):
+            log("step %d loss %.4e" % (s, losses[-1]))
+    return net, losses

@@ -62,9 +62,9 @@ def two_phase_beta(frac, shape, rho_gas=1.0, rho_liq=1000.0):
     return out
 
 
-def dam_break_system(n, dt=1.0 / 60.0, g=-9.8, rho_gas=1.0, rho_liq=1000.0):
+def dam_break_system(n, dt=1.0 / 60.0, g=-9.8, rho_gas=1.0, rho_liq=1000.0, box_max=(0.4, 0.6, 1.0)):
     shape = (n, n, n)
-    frac = dam_break_fraction(n)
+    frac = dam_break_fraction(n, box_max)
     beta = two_phase_beta(frac, shape, rho_gas, rho_liq)
     U = np.zeros((n, n, n + 1), np.float32)
     V = np.full((n, n + 1, n), np.float32(g) * np.float32(dt), np.float32)
@@ -81,7 +81,7 @@ def dam_break_system(n, dt=1.0 / 60.0, g=-9.8, rho_gas=1.0, rho_liq=1000.0):
         "vel": [U.ravel(), V.ravel(), W.ravel()],
         "active": active.ravel(),
         "target": np.zeros(n ** 3, np.float32),
-        "particles": particle_count(n),
+        "particles": particle_count(n, 2, box_max),
     }
 
 
