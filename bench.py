@@ -33,12 +33,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--kernel", default="sweep", help="kernel family timed for the roofline")
+    ap.add_argument("--kernel", default="relax", help="kernel family timed for the roofline")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="V-cycle storage: f32 is the reference's plain method, f16/bf16 the mixed form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=2, help="PCG iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 512 MiB device-to-device copy three times (the known-size kernel that scales "
+                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
     return ap.parse_args()
 
 
@@ -60,6 +63,29 @@ ALGO_BYTES_PER_CELL = {
     # coarse x (1/8 of 4) + mask 1 read, fine x read+write 8
     "correct": 9.5,
 }
+
+
+# fine-level kernel of each family, as rocprofv3 names it
+KERNEL_SYMBOL = {"relax": "k_relax3<1, 0,", "sweep": "k_rbgs_fused<1,", "apply": "k_pcg_apply3",
+                 "update": "k_pcg_update3", "residual": "k_residual3<1,", "correct": "k_correct<1,"}
+
+
+def measured_traffic(kernel, storage):
+    """HBM bytes per dispatch of the family's fine-level kernel from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
+    None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_dambreak256_%s.json" % storage)
+    sym = KERNEL_SYMBOL.get(kernel)
+    if not sym or not os.path.exists(path):
+        return None
+    best = None
+    for name, grids in json.load(open(path))["kernels"].items():
+        if sym not in name:
+            continue
+        for grid, rec in grids.items():
+            if best is None or int(grid) > best[0]:
+                best = (int(grid), rec["hbm_bytes"])
+    return None if best is None else best[1]
 
 
 def cpu_baseline(sc, levels, gpu_iters, iters, threads):
@@ -103,6 +129,16 @@ def main():
     from tsl_array_n_amd import scenes
 
     F.init(int(os.environ.get("LOCAL_RANK", "0")))
+    if args.copy_calibration:
+        # device-to-device copies of a 512 MiB field (the runtime's blit kernel)
+        src = F.array_n("float", (128 << 20,))
+        src.fill(1.0)
+        dst = F.array_n("float", (128 << 20,))
+        for _ in range(3):
+            dst.copy_from(src)
+        F.sync()
+        print("bench: copy calibration: 3 x %d bytes read + written" % ((128 << 20) * 4), file=sys.stderr, flush=True)
+        del src, dst
     dev, cus = F.device_info()
     n = args.n
     print("bench: building the %d^3 dam-break system on %s" % (n, dev), file=sys.stderr, flush=True)
@@ -150,7 +186,7 @@ def main():
         algo = ALGO_BYTES_PER_CELL.get(args.kernel, 0.0) * cells
         achieved = algo / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.kernel, args.storage),
                 "kernel": args.kernel, "avg_us": round(avg_s * 1e6, 2), "launches": k_launches,
                 "algorithmic_bytes": algo}
 

@@ -64,7 +64,13 @@ def test_projection_through_node_matches_oracle(tmp_path):
                                    weights=A["weights"], mg=mg, tol=1e-6, maxiter=100)
     p = np.fromfile(tmp_path / "pressure.bin", np.float32)
     assert np.linalg.norm(p - x) <= 1e-5 * np.linalg.norm(x)
+    # the projected dam-break velocity is ~0 (hydrostatic balance): the
+    # correction is checked against the oracle's correction of the same
+    # pressure, relative to the size of the input velocity
+    own = O.pressure3_correct(sc["shape"], sc["spacing"], p, sc["vel"], beta=sc["beta"], suffix="f32")
     out = O.pressure3_correct(sc["shape"], sc["spacing"], x, sc["vel"], beta=sc["beta"], suffix="f32")
+    vin = np.linalg.norm(np.concatenate(sc["vel"]))
     for a, name in enumerate("uvw"):
         got = np.fromfile(tmp_path / (name + ".bin"), np.float32)
-        assert np.linalg.norm(got - out[a]) <= 1e-5 * max(np.linalg.norm(out[a]), 1e-30)
+        assert np.linalg.norm(got - own[a]) <= 1e-6 * vin
+        assert np.linalg.norm(got - out[a]) <= 1e-3 * vin

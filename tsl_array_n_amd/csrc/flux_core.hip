@@ -1,5 +1,6 @@
 // Context, fields, timers: the tsl_array_n side of the boundary
 // (packages/tsl_array_n/src/context.js, array.js).
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -11,6 +12,7 @@ hipStream_t g_stream = nullptr;
 bool g_init = false;
 int g_device = 0;
 int g_cus = 0;
+unsigned g_march_blocks = 4096;
 
 static thread_local std::string t_err;
 
@@ -113,6 +115,11 @@ int flux_init(int device) {
     hipDeviceProp_t prop;
     FLUX_HIP(hipGetDeviceProperties(&prop, device));
     g_cus = prop.multiProcessorCount;
+    g_march_blocks = 4096;
+    if (const char* e = getenv("FLUX_MARCH_BLOCKS")) {
+        long v = strtol(e, nullptr, 10);
+        if (v >= 64 && v <= (1 << 20)) g_march_blocks = (unsigned)v;
+    }
     g_device = device;
     g_init = true;
     return FLUX_OK;

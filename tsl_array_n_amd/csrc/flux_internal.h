@@ -26,6 +26,9 @@ extern hipStream_t g_stream;
 extern bool g_init;
 extern int g_device;
 extern int g_cus;
+// Blocks a column-march grid aims at (FLUX_MARCH_BLOCKS, default 4096:
+// 16 blocks of 256 threads per CU on 256 CUs).
+extern unsigned g_march_blocks;
 
 int set_error(int code, const char* fmt, ...);
 void clear_error();

@@ -16,6 +16,7 @@
 #include "flux_internal.h"
 #include "flux_solver.h"
 #include "flux_stencil.h"
+#include "flux_stencil3.h"
 #include "flux_sweep.h"
 
 namespace flux {
@@ -86,17 +87,112 @@ __global__ void k_relax(LevelDevT<WT> L, XT* __restrict__ x, const XT* __restric
     x[c] = to_store<XT>(cur + ((to_f32(b[c]) - ax) / d) * omega);
 }
 
-// r = b - A x  (buildResidualKernel, multigrid.js:741-749)
-template <int FINE, typename XT, typename RT, typename WT>
-__global__ void k_residual(LevelDevT<WT> L, const XT* __restrict__ x, const XT* __restrict__ b, RT* __restrict__ r,
-                           const int* __restrict__ stop) {
+// k_relax for levels with an even number of cells along x, in the column-march
+// shape: thread t owns the cell pair (2p, 2p+1) of row j and marches over a
+// chunk of planes.  In every plane exactly one cell of the pair has the
+// colour; the pair is read and written as one aligned vector (the partner's
+// value is written back unchanged: no other thread writes it in this colour),
+// so every lane updates a cell and the stores fill whole lines.
+template <int FINE, int FROM_ZERO, typename XT, typename WT>
+__global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* __restrict__ x,
+                                                 const XT* __restrict__ b, float omega, int color,
+                                                 const int* __restrict__ stop) {
     if (stop && *stop) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= L.n0 * L.n1 * L.n2) return;
-    int i, j, k;
-    cell_coords(L, c, i, j, k);
-    float ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(x[q]); }, (float*)nullptr);
-    r[c] = to_store<RT>(to_f32(b[c]) - ax);
+    int t, k0, k1;
+    if (!march_begin(M, t, k0, k1)) return;
+    const int half = M.run;
+    const int j = t / half;
+    const int p = t - j * half;
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const uint8_t* __restrict__ m = L.mask;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    for (int k = k0; k < k1; ++k) {
+        const int s = (j + k + color) & 1;  // which cell of the pair has the colour
+        const int i = 2 * p + s;
+        const int cp = 2 * p + n0 * (j + n1 * k);
+        const int c = cp + s;
+        float w[6];
+        face_w3(L, i, j, k, c, w);
+        const bool dir = m && m[c];
+        const float bc = to_f32(b[c]);
+        if (FROM_ZERO) {
+            // x = 0 on entry: A x = 0 on a free row and -0 on a Dirichlet row
+            float sum, dsum;
+            const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            row_terms<float>(0.0f, zn, w, ih2, sum, dsum);
+            const float d = dir ? -1.0f : dsum;
+            const float ax = dir ? -0.0f : sum;
+            const float v = 0.0f + ((bc - ax) / d) * omega;
+            Pair2<XT> o;
+            o.lo = to_store<XT>(s ? 0.0f : v);
+            o.hi = to_store<XT>(s ? v : 0.0f);
+            *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+        } else {
+            const Pair2<XT> xp = *reinterpret_cast<const Pair2<XT>*>(x + cp);
+            const float xc = to_f32(s ? xp.hi : xp.lo);
+            const float xpart = to_f32(s ? xp.lo : xp.hi);
+            int q[6];
+            nbr_index3(n0, n1, n2, i, j, k, c, q);
+            // the pair partner is the upper x neighbour of the even cell and the lower of the odd one
+            const int qo = s ? q[1] : q[0];
+            const float xo = to_f32(x[qo]);
+            float xn[6];
+            xn[0] = s ? xpart : xo;
+            xn[1] = s ? xo : xpart;
+            xn[2] = to_f32(x[q[2]]);
+            xn[3] = to_f32(x[q[3]]);
+            xn[4] = to_f32(x[q[4]]);
+            xn[5] = to_f32(x[q[5]]);
+            if (m) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
+            }
+            float sum, dsum;
+            row_terms<float>(xc, xn, w, ih2, sum, dsum);
+            const float d = dir ? -1.0f : dsum;
+            const float ax = dir ? -xc : sum;
+            const XT nv = to_store<XT>(xc + ((bc - ax) / d) * omega);
+            Pair2<XT> o = xp;
+            if (s) o.hi = nv;
+            else o.lo = nv;
+            *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+        }
+    }
+}
+
+// r = b - A x (buildResidualKernel, multigrid.js:741-749) in the column-march
+// shape (one cell per thread and plane).
+template <int FINE, typename XT, typename RT, typename WT>
+__global__ void __launch_bounds__(MTPB) k_residual3(LevelDevT<WT> L, March M, const XT* __restrict__ x,
+                                                    const XT* __restrict__ b, RT* __restrict__ r,
+                                                    const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t, k0, k1;
+    if (!march_begin(M, t, k0, k1)) return;
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const int j = t / n0;
+    const int i = t - j * n0;
+    const uint8_t* __restrict__ m = L.mask;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    for (int k = k0; k < k1; ++k) {
+        const int c = i + n0 * (j + n1 * k);
+        float w[6];
+        face_w3(L, i, j, k, c, w);
+        int q[6];
+        nbr_index3(n0, n1, n2, i, j, k, c, q);
+        const float xc = to_f32(x[c]);
+        float xn[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) xn[a] = to_f32(x[q[a]]);
+        if (m) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
+        }
+        float sum, dsum;
+        row_terms<float>(xc, xn, w, ih2, sum, dsum);
+        const float ax = (m && m[c]) ? -xc : sum;
+        r[c] = to_store<RT>(to_f32(b[c]) - ax);
+    }
 }
 
 // Full-weighting restriction (restrictionTapsForAxis 766-780 + buildRestrictKernel
@@ -426,11 +522,28 @@ void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
         return;
     }
     LevelDevT<WT> L = level_dev<WT>(mg, v);
-    int64_t work = (int64_t)((v.n[0] + 1) / 2) * v.n[1] * v.n[2];
-    unsigned g = blocks_for(work, MTPB);
     float om = (float)mg->o.omega;
     XT* x = (XT*)v.x;
     const XT* b = (const XT*)v.b;
+    if ((v.n[0] & 1) == 0) {
+        const March M = march_shape(v.n[0] / 2, v.n[1], v.n[2], MTPB, g_march_blocks);
+        for (int it = 0; it < iters; ++it) {
+            int c0 = reversed ? 1 : 0, c1 = reversed ? 0 : 1;
+            if (from_zero && it == 0) {
+                k_relax3<FINE, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop);
+            } else {
+                if (FINE) kprof_begin("relax");
+                k_relax3<FINE, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop);
+                if (FINE) kprof_end("relax");
+            }
+            if (FINE) kprof_begin("relax");
+            k_relax3<FINE, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop);
+            if (FINE) kprof_end("relax");
+        }
+        return;
+    }
+    int64_t work = (int64_t)((v.n[0] + 1) / 2) * v.n[1] * v.n[2];
+    unsigned g = blocks_for(work, MTPB);
     for (int it = 0; it < iters; ++it) {
         int c0 = reversed ? 1 : 0, c1 = reversed ? 0 : 1;
         if (from_zero && it == 0) {
@@ -487,8 +600,9 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
         if (mg->o.nu_down > 0) level_smooth<1, float, ST>(mg, l, mg->o.nu_down, false, true, stop);
         else fill_t<float>(v.x, cells, 0.0f);
         kprof_begin("residual");
-        k_residual<1, float, float, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
-            level_dev<ST>(mg, v), (const float*)v.x, (const float*)v.b, (float*)v.buf, stop);
+        const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+        k_residual3<1, float, float, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), M, (const float*)v.x,
+                                                                      (const float*)v.b, (float*)v.buf, stop);
         kprof_end("residual");
         kprof_begin("restrict");
         k_restrict<1, float, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
@@ -498,8 +612,9 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
     } else {
         if (mg->o.nu_down > 0) level_smooth<0, ST, ST>(mg, l, mg->o.nu_down, false, true, stop);
         else fill_t<ST>(v.x, cells, 0.0f);
-        k_residual<0, ST, ST, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
-            level_dev<ST>(mg, v), (const ST*)v.x, (const ST*)v.b, (ST*)v.buf, stop);
+        const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+        k_residual3<0, ST, ST, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), M, (const ST*)v.x,
+                                                                (const ST*)v.b, (ST*)v.buf, stop);
         k_restrict<0, ST, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
             mg->ndim, v.n[0], v.n[1], v.n[2], (const ST*)v.buf, v.mask, c.n[0], c.n[1], c.n[2], (ST*)c.b, c.mask,
             nullptr, stop);

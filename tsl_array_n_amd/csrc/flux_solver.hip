@@ -11,6 +11,7 @@
 #include "flux_internal.h"
 #include "flux_solver.h"
 #include "flux_stencil.h"
+#include "flux_stencil3.h"
 
 namespace flux {
 
@@ -96,14 +97,19 @@ struct Scal {
     double pAp, rr, rz, rz_old, bb, base, alpha, beta, best, thr, tol, rr0;
     int stop, iter, stop_iter, relative;
     int xsel, psel, pad0, pad1;
-    unsigned ticket[4];
 };
 
 __device__ __forceinline__ bool degenerate(double v) { return !(fabs(v) <= 1.7976931348623157e308) || v == 0.0; }
 
-// Setup: x64 = x; r = b - A x (FP64 arithmetic); rr.
+// Every reduction is two-stage and deterministic: each block of the producing
+// kernel stores its block sum, and a one-block finishing kernel adds the
+// partials in index order and runs the scalar step.  The kernel boundary
+// makes the partials visible across XCDs (no per-block cache write-back).
+static constexpr int FIN_TPB = 1024;
+
+// Setup: x64 = x; r = b - A x (FP64 arithmetic); partials of rr.
 __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, double* __restrict__ x64,
-                                     const float* __restrict__ b, float* __restrict__ r, Scal* S, double* partials) {
+                                     const float* __restrict__ b, float* __restrict__ r, double* partials) {
     double v = 0;
     int n = L.n0 * L.n1 * L.n2;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
@@ -115,135 +121,200 @@ __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, 
         r[c] = (float)rv;
         v += rv * rv;
     }
-    grid_sum_last(v, partials, &S->ticket[0], [&](double tot) {
-        S->rr = tot;
-        S->rr0 = tot;
-        S->xsel = 0;
-        S->psel = 0;
-        S->iter = 0;
-        S->stop_iter = 0;
-        S->stop = 0;
-    });
+    store_block_partial(v, partials);
 }
 
-__global__ void k_pcg_dot_bb(const float* __restrict__ b, int n, Scal* S, double* partials) {
+__global__ void __launch_bounds__(FIN_TPB) k_fin_setup(Scal* S, const double* partials, unsigned nb) {
+    const double tot = sum_partials(partials, nb);
+    if (threadIdx.x != 0) return;
+    S->rr = tot;
+    S->rr0 = tot;
+    S->xsel = 0;
+    S->psel = 0;
+    S->iter = 0;
+    S->stop_iter = 0;
+    S->stop = 0;
+}
+
+__global__ void k_pcg_dot_bb(const float* __restrict__ b, int n, double* partials) {
     double v = 0;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
         v += (double)b[c] * (double)b[c];
-    grid_sum_last(v, partials, &S->ticket[1], [&](double tot) {
-        S->bb = tot;
-        double nb = sqrt(fabs(tot));
-        S->thr = S->relative ? fmax(S->tol * nb, S->tol) : S->tol;
-        // host solve(): an initial residual already below the threshold
-        // skips the loop and reports convergence (linalg.js:2205)
-        if (sqrt(fabs(S->rr)) < S->thr) {
-            S->stop = 5;
-            S->stop_iter = 0;
-        }
-    });
+    store_block_partial(v, partials);
 }
 
-// rz = r.z; seed (first=1: seedScalarsKernel linalg.js:1238-1244) or beta
-// (betaKernel, linalg.js:1198-1226).
-__global__ void k_pcg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, int n, Scal* S,
-                             double* partials, int first) {
+__global__ void __launch_bounds__(FIN_TPB) k_fin_bb(Scal* S, const double* partials, unsigned nb) {
+    const double tot = sum_partials(partials, nb);
+    if (threadIdx.x != 0) return;
+    S->bb = tot;
+    double bn = sqrt(fabs(tot));
+    S->thr = S->relative ? fmax(S->tol * bn, S->tol) : S->tol;
+    // host solve(): an initial residual already below the threshold
+    // skips the loop and reports convergence (linalg.js:2205)
+    if (sqrt(fabs(S->rr)) < S->thr) {
+        S->stop = 5;
+        S->stop_iter = 0;
+    }
+}
+
+// partials of rz = r.z
+__global__ void k_pcg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, int n, const Scal* S,
+                             double* partials) {
     if (S->stop) return;
     double v = 0;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
         v += (double)r[c] * (double)z[c];
-    grid_sum_last(v, partials, &S->ticket[2], [&](double tot) {
-        S->rz = tot;
-        if (first) {
-            S->rz_old = tot;
-            S->base = fmax(fabs(S->rr0), 1e-12);
-            S->beta = 0.0;
-            S->best = -1.0;
-            return;
-        }
-        double rzOld = S->rz_old;
-        if (degenerate(rzOld)) {
-            S->stop = 4;  // degenerate-oldRZ
-            S->beta = 0.0;
-            return;
-        }
-        bool flip = (rzOld > 0) != (tot > 0);
-        double braw = tot / rzOld;
-        bool restart = flip || !(fabs(braw) <= 10.0);  // MAX_BETA_MAGNITUDE
-        S->beta = restart ? 0.0 : braw;
-        S->rz_old = tot;
-    });
+    store_block_partial(v, partials);
 }
 
-// Iteration step 1: p = z + beta p (updateP, fused); Ap = A p; pAp; alpha
-// with the guards of alphaKernel (linalg.js:1147-1196).
-__global__ void k_pcg_apply(LevelDev L, const float* __restrict__ z, float* __restrict__ pa, float* __restrict__ pb,
-                            float* __restrict__ Ap, Scal* S, double* partials) {
+// rz; seed (first=1: seedScalarsKernel linalg.js:1238-1244) or beta
+// (betaKernel, linalg.js:1198-1226).
+__global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* partials, unsigned nb, int first) {
+    if (S->stop) return;
+    const double tot = sum_partials(partials, nb);
+    if (threadIdx.x != 0) return;
+    S->rz = tot;
+    if (first) {
+        S->rz_old = tot;
+        S->base = fmax(fabs(S->rr0), 1e-12);
+        S->beta = 0.0;
+        S->best = -1.0;
+        return;
+    }
+    double rzOld = S->rz_old;
+    if (degenerate(rzOld)) {
+        S->stop = 4;  // degenerate-oldRZ
+        S->beta = 0.0;
+        return;
+    }
+    bool flip = (rzOld > 0) != (tot > 0);
+    double braw = tot / rzOld;
+    bool restart = flip || !(fabs(braw) <= 10.0);  // MAX_BETA_MAGNITUDE
+    S->beta = restart ? 0.0 : braw;
+    S->rz_old = tot;
+}
+
+// Iteration step 1: p = z + beta p (updateP, fused); Ap = A p; partials of
+// pAp.  Column-march shape: one cell per thread and plane.
+__global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const float* __restrict__ z,
+                                                    float* __restrict__ pa, float* __restrict__ pb,
+                                                    float* __restrict__ Ap, const Scal* S, double* partials) {
     if (S->stop) return;
     const float* __restrict__ pold = S->psel ? pb : pa;
     float* __restrict__ pnew = S->psel ? pa : pb;
     const float beta = (float)S->beta;
-    int n = L.n0 * L.n1 * L.n2;
     double v = 0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_coords(L, c, i, j, k);
-        auto ld = [&](int q) { return z[q] + pold[q] * beta; };
-        float ap = stencil_row<float>(L, i, j, k, c, ld, (float*)nullptr);
-        float pc = ld(c);
-        pnew[c] = pc;
-        Ap[c] = ap;
-        v += (double)pc * (double)ap;
+    int t, k0, k1;
+    if (march_begin(M, t, k0, k1)) {
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int j = t / n0;
+        const int i = t - j * n0;
+        const uint8_t* __restrict__ m = L.mask;
+        const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        for (int k = k0; k < k1; ++k) {
+            const int c = i + n0 * (j + n1 * k);
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            int q[6];
+            nbr_index3(n0, n1, n2, i, j, k, c, q);
+            const float pc = z[c] + pold[c] * beta;
+            float xn[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = z[q[a]] + pold[q[a]] * beta;
+            if (m) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
+            }
+            float sum, dsum;
+            row_terms<float>(pc, xn, w, ih2, sum, dsum);
+            const float ap = (m && m[c]) ? -pc : sum;
+            pnew[c] = pc;
+            Ap[c] = ap;
+            v += (double)pc * (double)ap;
+        }
     }
-    grid_sum_last(v, partials, &S->ticket[3], [&](double pAp) {
-        S->pAp = pAp;
-        S->psel ^= 1;
-        S->iter += 1;
-        double alpha = S->rz_old / pAp;
-        int code = 0;
-        if (degenerate(pAp)) code = 1;                            // degenerate-pAp
-        else if (fabs(pAp) > S->base * 1e8) code = 2;             // MAX_PAP_GROWTH_FACTOR
-        else if (!(fabs(alpha) <= 1e6)) code = 3;                 // MAX_ALPHA_MAGNITUDE
-        S->alpha = code ? 0.0 : alpha;
-        if (code) S->stop = code;
-    });
+    store_block_partial(v, partials);
+}
+
+// pAp; alpha with the guards of alphaKernel (linalg.js:1147-1196).
+__global__ void __launch_bounds__(FIN_TPB) k_fin_pap(Scal* S, const double* partials, unsigned nb) {
+    if (S->stop) return;
+    const double pAp = sum_partials(partials, nb);
+    if (threadIdx.x != 0) return;
+    S->pAp = pAp;
+    S->psel ^= 1;
+    S->iter += 1;
+    double alpha = S->rz_old / pAp;
+    int code = 0;
+    if (degenerate(pAp)) code = 1;                            // degenerate-pAp
+    else if (fabs(pAp) > S->base * 1e8) code = 2;             // MAX_PAP_GROWTH_FACTOR
+    else if (!(fabs(alpha) <= 1e6)) code = 3;                 // MAX_ALPHA_MAGNITUDE
+    S->alpha = code ? 0.0 : alpha;
+    if (code) S->stop = code;
 }
 
 // Iteration step 2: x += alpha p (FP64 accumulator, ping-pong buffers);
 // r = b - A x in FP64 arithmetic (the true residual, recomputed every
-// iteration); rr; convergence and residual-growth tests.
-__global__ void k_pcg_update(LevelDev L, double* __restrict__ xa, double* __restrict__ xb, const float* __restrict__ pa,
-                             const float* __restrict__ pb, const float* __restrict__ b, float* __restrict__ r, Scal* S,
-                             double* partials) {
+// iteration); partials of rr.
+__global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double* __restrict__ xa,
+                                                     double* __restrict__ xb, const float* __restrict__ pa,
+                                                     const float* __restrict__ pb, const float* __restrict__ b,
+                                                     float* __restrict__ r, const Scal* S, double* partials) {
     if (S->stop) return;
     const double* __restrict__ xold = S->xsel ? xb : xa;
     double* __restrict__ xnew = S->xsel ? xa : xb;
-    const float* __restrict__ p = S->psel ? pb : pa;  // the p written by k_pcg_apply
+    const float* __restrict__ p = S->psel ? pb : pa;  // the p written by k_pcg_apply3
     const double alpha = S->alpha;
-    int n = L.n0 * L.n1 * L.n2;
     double v = 0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_coords(L, c, i, j, k);
-        auto ld = [&](int q) { return xold[q] + alpha * (double)p[q]; };
-        double ax = stencil_row<double>(L, i, j, k, c, ld, (double*)nullptr);
-        xnew[c] = ld(c);
-        double rv = (double)b[c] - ax;
-        r[c] = (float)rv;
-        v += rv * rv;
-    }
-    grid_sum_last(v, partials, &S->ticket[0], [&](double rr) {
-        S->rr = rr;
-        S->xsel ^= 1;
-        double res = sqrt(fabs(rr));
-        if (res <= S->thr) {  // isNonFiniteOrAbove(residual, threshold).not()
-            S->stop = 5;
-            S->stop_iter = S->iter;
-            return;
+    int t, k0, k1;
+    if (march_begin(M, t, k0, k1)) {
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int j = t / n0;
+        const int i = t - j * n0;
+        const uint8_t* __restrict__ m = L.mask;
+        const double ih2[3] = {L.ih2d[0], L.ih2d[1], L.ih2d[2]};
+        for (int k = k0; k < k1; ++k) {
+            const int c = i + n0 * (j + n1 * k);
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            int q[6];
+            nbr_index3(n0, n1, n2, i, j, k, c, q);
+            const double xc = xold[c] + alpha * (double)p[c];
+            double xn[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = xold[q[a]] + alpha * (double)p[q[a]];
+            if (m) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0 : xn[a];
+            }
+            double sum, dsum;
+            row_terms<double>(xc, xn, w, ih2, sum, dsum);
+            const double ax = (m && m[c]) ? -xc : sum;
+            xnew[c] = xc;
+            const double rv = (double)b[c] - ax;
+            r[c] = (float)rv;
+            v += rv * rv;
         }
-        if (!(res == res)) return;
-        if (S->best < 0 || fabs(rr) < S->best) S->best = fabs(rr);
-        else if (S->best > 0 && fabs(rr) > S->best * 100.0) S->stop = 6;  // MAX_RESIDUAL_GROWTH
-    });
+    }
+    store_block_partial(v, partials);
+}
+
+// rr; convergence and residual-growth tests.
+__global__ void __launch_bounds__(FIN_TPB) k_fin_rr(Scal* S, const double* partials, unsigned nb) {
+    if (S->stop) return;
+    const double rr = sum_partials(partials, nb);
+    if (threadIdx.x != 0) return;
+    S->rr = rr;
+    S->xsel ^= 1;
+    double res = sqrt(fabs(rr));
+    if (res <= S->thr) {  // isNonFiniteOrAbove(residual, threshold).not()
+        S->stop = 5;
+        S->stop_iter = S->iter;
+        return;
+    }
+    if (!(res == res)) return;
+    if (S->best < 0 || fabs(rr) < S->best) S->best = fabs(rr);
+    else if (S->best > 0 && fabs(rr) > S->best * 100.0) S->stop = 6;  // MAX_RESIDUAL_GROWTH
 }
 
 __global__ void k_pcg_copy_out(const double* __restrict__ xa, const double* __restrict__ xb, float* __restrict__ x32,
@@ -385,14 +456,14 @@ struct flux_pcg {
     Scal* S = nullptr;
     Scal* Sh = nullptr;  // pinned host mirror
     double* partials = nullptr;
+    March M;  // column-march grid of the iteration kernels
     int last_iterations = 4;
 };
 
 namespace flux {
 
-// Reductions run on a capped grid (grid-stride loops): every block takes one
-// ticket of the last-block reduction, so the block count bounds the atomics
-// serialised on that ticket.
+// The grid-stride reductions (setup, b.b, r.z) run on a capped grid: the
+// block count is the number of partials the finishing kernel adds.
 static constexpr unsigned RED_MAX_BLOCKS = 2048;
 static unsigned red_blocks(int64_t n) { return std::min<unsigned>(blocks_for(n, TPB), RED_MAX_BLOCKS); }
 
@@ -408,16 +479,19 @@ static int pcg_precondition(flux_pcg* s) {
 static int pcg_iteration(flux_pcg* s) {
     unsigned g = red_blocks(s->n);
     kprof_begin("apply");
-    k_pcg_apply<<<g, TPB, 0, g_stream>>>(s->op->L, s->z, s->pa, s->pb, s->Ap, s->S, s->partials);
+    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->z, s->pa, s->pb, s->Ap, s->S, s->partials);
     kprof_end("apply");
+    k_fin_pap<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     kprof_begin("update");
-    k_pcg_update<<<g, TPB, 0, g_stream>>>(s->op->L, s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d, s->r, s->S,
-                                           s->partials);
+    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d,
+                                                 s->r, s->S, s->partials);
     kprof_end("update");
+    k_fin_rr<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     FLUX_TRY(pcg_precondition(s));
     kprof_begin("dot");
-    k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials, 0);
+    k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
     kprof_end("dot");
+    k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g, 0);
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
@@ -433,10 +507,13 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     FLUX_HIP(hipStreamSynchronize(g_stream));
     // setup (runGpuResidentSetup, linalg.js:1613-1637)
     k_pcg_setup_residual<<<g, TPB, 0, g_stream>>>(s->op->L, (const float*)s->x->d, s->xa, (const float*)s->b->d, s->r,
-                                                   s->S, s->partials);
-    k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->S, s->partials);
+                                                   s->partials);
+    k_fin_setup<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
+    k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->partials);
+    k_fin_bb<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
     FLUX_TRY(pcg_precondition(s));
-    k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials, 1);
+    k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
+    k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g, 1);
     k_fill_f32<<<gfull, TPB, 0, g_stream>>>(s->pa, (int)s->n, 0.0f);
     k_fill_f32<<<gfull, TPB, 0, g_stream>>>(s->pb, (int)s->n, 0.0f);
     FLUX_LAUNCH_CHECK();
@@ -499,7 +576,8 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     s->xa = (double*)dmalloc(cells * 8);
     s->xb = (double*)dmalloc(cells * 8);
     s->S = (Scal*)dmalloc(sizeof(Scal));
-    s->partials = (double*)dmalloc(sizeof(double) * (blocks_for(cells, TPB) + 64));
+    s->M = march_shape(op->n[0], op->n[1], op->n[2], TPB, g_march_blocks);
+    s->partials = (double*)dmalloc(sizeof(double) * (std::max<int64_t>(blocks_for(cells, TPB), s->M.nb) + 64));
     void* hp = nullptr;
     if (hipHostMalloc(&hp, sizeof(Scal), hipHostMallocDefault) != hipSuccess) hp = nullptr;
     s->Sh = (Scal*)hp;
@@ -507,7 +585,7 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
         flux_pcg_destroy(s);
         return set_error(FLUX_ERR_NOMEM, "conjugateGradient: allocation failed");
     }
-    // the ticket counters start at zero
+    // the scalar block starts at zero
     FLUX_HIP(hipMemsetAsync(s->S, 0, sizeof(Scal), g_stream));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     *out = s;

@@ -111,6 +111,16 @@ __device__ __forceinline__ T stencil_row(const LevelDevT<WT>& L, int i, int j, i
     return sum;
 }
 
+// XCD-aware block order: the hardware hands consecutive block ids to the 8
+// XCDs round robin; this bijection gives each XCD a contiguous run of logical
+// blocks (neighbouring tiles share their halos in that XCD's L2).  XCD x
+// receives hardware ids x, x+8, ...: ceil((nb-x)/8) of them.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    const unsigned q = nb >> 3, r = nb & 7u;
+    const unsigned xcd = b & 7u, slot = b >> 3;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 // Deterministic FP64 block sum (blockDim.x a multiple of 64); the result is
 // valid in thread 0.  All threads of the block must call it.
 __device__ __forceinline__ double block_sum(double v) {
@@ -129,33 +139,19 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-// Grid-wide FP64 sum finished by the last block to arrive, in a fixed order
-// (deterministic): every block stores its partial, takes a ticket, and the
-// block holding the last ticket sums the partials and runs `epi(total)` in
-// thread 0, then re-arms the ticket.  Returns nothing useful to the caller.
-template <typename EPI>
-__device__ __forceinline__ void grid_sum_last(double v, double* partials, unsigned* ticket, EPI epi) {
-    double s = block_sum(v);
-    __shared__ int s_last;
-    const unsigned nb = gridDim.x;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = s;
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        s_last = (t == nb - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+// Two-stage deterministic FP64 sum.  Stage 1: every block of the producing
+// kernel stores its block sum at partials[blockIdx.x] (all threads call).
+__device__ __forceinline__ void store_block_partial(double v, double* partials) {
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Stage 2, in a one-block kernel launched after the producer: the sum of the
+// nb partials in a fixed order, valid in thread 0 (all threads call).
+__device__ __forceinline__ double sum_partials(const double* partials, unsigned nb) {
     double acc = 0;
-    for (unsigned q = threadIdx.x; q < nb; q += blockDim.x)
-        acc += __hip_atomic_load(&partials[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double tot = block_sum(acc);
-    if (threadIdx.x == 0) {
-        *ticket = 0u;
-        epi(tot);
-        __threadfence();
-    }
+    for (unsigned q = threadIdx.x; q < nb; q += blockDim.x) acc += partials[q];
+    return block_sum(acc);
 }
 
 }  // namespace flux

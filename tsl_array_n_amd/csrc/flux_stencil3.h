@@ -1,118 +1,121 @@
-// 3-D specialisation of the operator for the hot kernels.
+// 3-D kernels' shared pieces: column-march geometry, pair loads and the
+// operator row evaluated branch-free.
 //
-// Same operator as stencil_row (laplacianAt / laplacianDiagonalAt,
-// multigrid.js:273-339, 399-437), written in its coupling form:
-//   (A x)_c = sum over present faces f of  g_f * (xn_f - x_c),
-//   g_f = w_f / h_axis^2,  xn_f = 0 when the neighbour is Dirichlet,
-//   diag_c = -sum over present faces of g_f;  a Dirichlet row is -x_c (diag -1).
-// (Expanding dLo/dUp of the reference gives exactly these terms.)
+// The row is the one of stencil_row (laplacianAt / laplacianDiagonalAt,
+// multigrid.js:273-339, 399-437) with the same operations in the same order:
+// per axis dLo = (x_c - x_lo') * w_lo', dUp = (x_up' - x_c) * w_up',
+// sum += (dUp - dLo) / h^2, diag -= (w_lo' + w_up') / h^2, where x_n' is 0
+// for a Dirichlet neighbour and w' is 0 for a face outside the grid (so an
+// outside face contributes an exact 0).  Neighbour reads use clamped,
+// always-valid indices, so the loads are issued unconditionally.
 //
-// Launch shape: blocks of 64 x 4 threads over (x, y); each thread marches
-// over a chunk of z-planes, so there is no index division and consecutive
-// planes share the thread's z-neighbour reads in cache.
+// Launch shape: a 1-D grid; block b (after the XCD remap) covers `tpb`
+// consecutive threads of one z-chunk; thread t of the chunk owns position
+// t of the (x-run, y) plane and marches over the chunk's planes, so the
+// cell index needs one division per thread instead of two per cell.
 #pragma once
+#include <algorithm>
+
 #include "flux_stencil.h"
 
 namespace flux {
 
-static constexpr int S3_BX = 64, S3_BY = 4;
-
-struct Row3 {
-    float acc;   // sum of g_f * (xn_f - x_c) over present faces
-    float diag;  // -sum of g_f
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair2 {
+    T lo, hi;
 };
 
-// Evaluates one free row; `ld(q)` reads x (as float) at flat index q.
-template <typename WT, typename LD>
-__device__ __forceinline__ Row3 row3(const LevelDevT<WT>& L, int i, int j, int k, int c, float xc, LD ld) {
-    const uint8_t* __restrict__ m = L.mask;
-    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
-    const int plane = n0 * n1;
-    float acc = 0.0f, gs = 0.0f;
-    // x faces
-    {
-        const int fl = i + (n0 + 1) * (j + n1 * k);
-        float glo = L.w0 ? to_f32(L.w0[fl]) : 1.0f;
-        float gup = L.w0 ? to_f32(L.w0[fl + 1]) : 1.0f;
-        glo *= L.ih2[0];
-        gup *= L.ih2[0];
-        if (i > 0) {
-            float xn = (m && m[c - 1]) ? 0.0f : ld(c - 1);
-            acc += glo * (xn - xc);
-            gs += glo;
-        }
-        if (i < n0 - 1) {
-            float xn = (m && m[c + 1]) ? 0.0f : ld(c + 1);
-            acc += gup * (xn - xc);
-            gs += gup;
-        }
-    }
-    // y faces
-    {
-        const int fl = i + n0 * (j + (n1 + 1) * k);
-        float glo = L.w1 ? to_f32(L.w1[fl]) : 1.0f;
-        float gup = L.w1 ? to_f32(L.w1[fl + n0]) : 1.0f;
-        glo *= L.ih2[1];
-        gup *= L.ih2[1];
-        if (j > 0) {
-            float xn = (m && m[c - n0]) ? 0.0f : ld(c - n0);
-            acc += glo * (xn - xc);
-            gs += glo;
-        }
-        if (j < n1 - 1) {
-            float xn = (m && m[c + n0]) ? 0.0f : ld(c + n0);
-            acc += gup * (xn - xc);
-            gs += gup;
-        }
-    }
-    // z faces
-    {
-        float glo = L.w2 ? to_f32(L.w2[c]) : 1.0f;
-        float gup = L.w2 ? to_f32(L.w2[c + plane]) : 1.0f;
-        glo *= L.ih2[2];
-        gup *= L.ih2[2];
-        if (k > 0) {
-            float xn = (m && m[c - plane]) ? 0.0f : ld(c - plane);
-            acc += glo * (xn - xc);
-            gs += glo;
-        }
-        if (k < n2 - 1) {
-            float xn = (m && m[c + plane]) ? 0.0f : ld(c + plane);
-            acc += gup * (xn - xc);
-            gs += gup;
-        }
-    }
-    return Row3{acc, -gs};
+struct March {
+    int run;        // threads per row of the plane
+    int rows;       // rows per plane (n1)
+    int kz;         // planes per chunk
+    int n2;         // planes
+    unsigned bxy;   // blocks per chunk
+    unsigned nb;    // blocks in the grid
+};
+
+// `want` is the block count aimed at (several waves of the 256 CUs).
+inline March march_shape(int run, int n1, int n2, int tpb, unsigned want) {
+    March m;
+    m.run = run;
+    m.rows = n1;
+    m.n2 = n2;
+    const int64_t per_plane = (int64_t)run * n1;
+    m.bxy = (unsigned)((per_plane + tpb - 1) / tpb);
+    unsigned bz = (want + m.bxy - 1) / m.bxy;
+    bz = std::max(1u, std::min<unsigned>(bz, (unsigned)n2));
+    m.kz = (n2 + (int)bz - 1) / (int)bz;
+    bz = (unsigned)((n2 + m.kz - 1) / m.kz);
+    m.nb = m.bxy * bz;
+    return m;
 }
 
-// The same row in FP64 arithmetic (the PCG's true residual).
-template <typename LD>
-__device__ __forceinline__ double row3d(const LevelDev& L, int i, int j, int k, int c, double xc, LD ld) {
-    const uint8_t* __restrict__ m = L.mask;
+// Thread position in the plane and its plane range; false when the thread
+// lies past the plane's end.
+__device__ __forceinline__ bool march_begin(const March& M, int& t, int& k0, int& k1) {
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned bz = lb / M.bxy, bxy = lb - bz * M.bxy;
+    t = (int)(bxy * blockDim.x + threadIdx.x);
+    k0 = (int)bz * M.kz;
+    k1 = min(k0 + M.kz, M.n2);
+    return t < M.run * M.rows;
+}
+
+// Face weights of cell (i,j,k) with outside faces zeroed: w[2a] lower and
+// w[2a+1] upper face on axis a.
+template <typename WT>
+__device__ __forceinline__ void face_w3(const LevelDevT<WT>& L, int i, int j, int k, int c, float w[6]) {
     const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
     const int plane = n0 * n1;
-    double acc = 0.0;
-    {
-        const int fl = i + (n0 + 1) * (j + n1 * k);
-        double glo = (L.w0 ? (double)L.w0[fl] : 1.0) * L.ih2d[0];
-        double gup = (L.w0 ? (double)L.w0[fl + 1] : 1.0) * L.ih2d[0];
-        if (i > 0) acc += glo * (((m && m[c - 1]) ? 0.0 : ld(c - 1)) - xc);
-        if (i < n0 - 1) acc += gup * (((m && m[c + 1]) ? 0.0 : ld(c + 1)) - xc);
+    w[0] = w[1] = w[2] = w[3] = w[4] = w[5] = 1.0f;
+    if (L.w0) {
+        const int f = i + (n0 + 1) * (j + n1 * k);
+        w[0] = to_f32(L.w0[f]);
+        w[1] = to_f32(L.w0[f + 1]);
     }
-    {
-        const int fl = i + n0 * (j + (n1 + 1) * k);
-        double glo = (L.w1 ? (double)L.w1[fl] : 1.0) * L.ih2d[1];
-        double gup = (L.w1 ? (double)L.w1[fl + n0] : 1.0) * L.ih2d[1];
-        if (j > 0) acc += glo * (((m && m[c - n0]) ? 0.0 : ld(c - n0)) - xc);
-        if (j < n1 - 1) acc += gup * (((m && m[c + n0]) ? 0.0 : ld(c + n0)) - xc);
+    if (L.w1) {
+        const int f = i + n0 * (j + (n1 + 1) * k);
+        w[2] = to_f32(L.w1[f]);
+        w[3] = to_f32(L.w1[f + n0]);
     }
-    {
-        double glo = (L.w2 ? (double)L.w2[c] : 1.0) * L.ih2d[2];
-        double gup = (L.w2 ? (double)L.w2[c + plane] : 1.0) * L.ih2d[2];
-        if (k > 0) acc += glo * (((m && m[c - plane]) ? 0.0 : ld(c - plane)) - xc);
-        if (k < n2 - 1) acc += gup * (((m && m[c + plane]) ? 0.0 : ld(c + plane)) - xc);
+    if (L.w2) {
+        w[4] = to_f32(L.w2[c]);
+        w[5] = to_f32(L.w2[c + plane]);
     }
-    return acc;
+    if (i == 0) w[0] = 0.0f;
+    if (i == n0 - 1) w[1] = 0.0f;
+    if (j == 0) w[2] = 0.0f;
+    if (j == n1 - 1) w[3] = 0.0f;
+    if (k == 0) w[4] = 0.0f;
+    if (k == n2 - 1) w[5] = 0.0f;
+}
+
+// Clamped neighbour indices of cell c: q[2a] lower, q[2a+1] upper on axis a
+// (the cell itself where the neighbour is outside).
+__device__ __forceinline__ void nbr_index3(int n0, int n1, int n2, int i, int j, int k, int c, int q[6]) {
+    const int plane = n0 * n1;
+    q[0] = i > 0 ? c - 1 : c;
+    q[1] = i < n0 - 1 ? c + 1 : c;
+    q[2] = j > 0 ? c - n0 : c;
+    q[3] = j < n1 - 1 ? c + n0 : c;
+    q[4] = k > 0 ? c - plane : c;
+    q[5] = k < n2 - 1 ? c + plane : c;
+}
+
+// Row sum and diagonal from the centre value, the six Dirichlet-masked
+// neighbour values and the six face weights (outside faces zero).
+template <typename T>
+__device__ __forceinline__ void row_terms(T xc, const T xn[6], const float w[6], const T ih2[3], T& sum, T& dsum) {
+    sum = T(0);
+    dsum = T(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T wl = (T)w[2 * a], wu = (T)w[2 * a + 1];
+        const T dLo = (xc - xn[2 * a]) * wl;
+        const T dUp = (xn[2 * a + 1] - xc) * wu;
+        sum = sum + (dUp - dLo) * ih2[a];
+        dsum = dsum - (wl + wu) * ih2[a];
+    }
 }
 
 }  // namespace flux

@@ -27,14 +27,6 @@ struct SweepTile {
     static constexpr int R1X = TX + 2, R1Y = TY + 2, R1 = R1X * R1Y;  // tile + 1 ring
 };
 
-// XCD-aware block order: consecutive hardware block ids land on different
-// XCDs (round robin over 8); remap so that each XCD works on a contiguous
-// run of tiles and shares their halos in its own L2.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
-    if (nb % 8u) return b;
-    return (b % 8u) * (nb / 8u) + b / 8u;
-}
-
 template <int FINE, int TX, int TY, bool ZERO, typename XT, typename WT>
 __global__ void __launch_bounds__(TX* TY) k_rbgs_fused(LevelDevT<WT> L, const XT* __restrict__ xin,
                                                        const XT* __restrict__ b, XT* __restrict__ xout,
