@@ -93,13 +93,21 @@ __global__ void k_relax(LevelDevT<WT> L, XT* __restrict__ x, const XT* __restric
 // colour; the pair is read and written as one aligned vector (the partner's
 // value is written back unchanged: no other thread writes it in this colour),
 // so every lane updates a cell and the stores fill whole lines.
-template <int FINE, int FROM_ZERO, typename XT, typename WT>
+//
+// DOT (level 0's last launch of the V-cycle): each block also stores its
+// partial of b.x over both cells of every pair (the PCG's r.z with the final
+// z), for a one-block finishing kernel.
+template <int FINE, int FROM_ZERO, int DOT, typename XT, typename WT>
 __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* __restrict__ x,
                                                  const XT* __restrict__ b, float omega, int color,
-                                                 const int* __restrict__ stop) {
+                                                 const int* __restrict__ stop, double* __restrict__ partials) {
     if (stop && *stop) return;
+    double dot = 0.0;
     int t, k0, k1;
-    if (!march_begin(M, t, k0, k1)) return;
+    if (!march_begin(M, t, k0, k1)) {
+        if (DOT) store_block_partial(dot, partials);
+        return;
+    }
     const int half = M.run;
     const int j = t / half;
     const int p = t - j * half;
@@ -114,7 +122,8 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
         float w[6];
         face_w3(L, i, j, k, c, w);
         const bool dir = m && m[c];
-        const float bc = to_f32(b[c]);
+        const Pair2<XT> bp = *reinterpret_cast<const Pair2<XT>*>(b + cp);
+        const float bc = to_f32(s ? bp.hi : bp.lo);
         if (FROM_ZERO) {
             // x = 0 on entry: A x = 0 on a free row and -0 on a Dirichlet row
             float sum, dsum;
@@ -156,8 +165,13 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
             if (s) o.hi = nv;
             else o.lo = nv;
             *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+            if (DOT) {
+                dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
+                dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
+            }
         }
     }
+    if (DOT) store_block_partial(dot, partials);
 }
 
 // r = b - A x (buildResidualKernel, multigrid.js:741-749) in the column-march
@@ -292,29 +306,177 @@ __global__ void k_correct(int ndim, int c0, int c1, int c2, const CT* __restrict
     xf[t] = to_store<FT>(to_f32(xf[t]) + sum);
 }
 
-// Coarsest level in one workgroup: `iters` x (colour 0, colour 1) sweeps from
-// x = 0 (buildCoarseSweepKernel, multigrid.js:679-737).  The block's barriers
-// order the colours; global writes are visible within the block after them.
-template <typename XT, typename WT>
-__global__ void __launch_bounds__(1024) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x, const XT* __restrict__ b,
-                                                       float omega, int iters, const int* __restrict__ stop) {
+// 3-D full-weighting restriction in the column-march shape: thread (I,J)
+// marches over coarse planes K.  Each fine plane f is first reduced over its
+// 4 x 4 in-plane taps to P(f); the coarse value is sum_c tz[c] * P(fk[c]).
+// Consecutive coarse planes share two fine planes, whose P values are carried
+// in registers.  Same taps, clamps and masking as k_restrict.
+template <int FINE, typename IT, typename OT>
+__global__ void __launch_bounds__(MTPB) k_restrict3(March M, int f0, int f1, int f2, const IT* __restrict__ rf,
+                                                    const uint8_t* __restrict__ mf, int c0, int c1, int c2,
+                                                    OT* __restrict__ bc, const uint8_t* __restrict__ mc,
+                                                    const double* __restrict__ rr, const int* __restrict__ stop) {
     if (stop && *stop) return;
+    int t, K0, K1;
+    if (!march_begin(M, t, K0, K1)) return;
+    const int J = t / c0;
+    const int I = t - J * c0;
+    const float tw[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    int ti[4], tj[4];
+    ti[0] = I > 0 ? 2 * I - 1 : 2 * I;
+    ti[1] = 2 * I;
+    ti[2] = 2 * I + 1;
+    ti[3] = (I + 1 < c0) ? 2 * I + 2 : 2 * I + 1;
+    tj[0] = J > 0 ? 2 * J - 1 : 2 * J;
+    tj[1] = 2 * J;
+    tj[2] = 2 * J + 1;
+    tj[3] = (J + 1 < c1) ? 2 * J + 2 : 2 * J + 1;
+    const int fplane = f0 * f1;
+    const float scale = residual_scale(rr, f0 * f1 * f2);
+    auto reduce_plane = [&](int fk) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int bq = 0; bq < 4; ++bq) {
+                const int f = ti[a] + f0 * tj[bq] + fplane * fk;
+                const float v = (mf && mf[f]) ? 0.0f : to_f32(rf[f]);
+                sum += v * (tw[a] * tw[bq]);
+            }
+        return sum;
+    };
+    float Pa = 0.0f, Pb = 0.0f;  // P at fine planes 2K+1 and 2K+2 of the previous K
+    for (int K = K0; K < K1; ++K) {
+        const int tk0 = K > 0 ? 2 * K - 1 : 2 * K;
+        const int tk3 = (K + 1 < c2) ? 2 * K + 2 : 2 * K + 1;
+        float P0, P1;
+        if (K == K0) {
+            P0 = reduce_plane(tk0);
+            P1 = (tk0 == 2 * K) ? P0 : reduce_plane(2 * K);
+        } else {
+            P0 = Pa;  // 2(K-1)+1 = 2K-1
+            P1 = Pb;  // 2(K-1)+2 = 2K
+        }
+        const float P2 = reduce_plane(2 * K + 1);
+        const float P3 = (tk3 == 2 * K + 1) ? P2 : reduce_plane(tk3);
+        Pa = P2;
+        Pb = P3;
+        const int c = I + c0 * (J + c1 * K);
+        const float v = P0 * tw[0] + P1 * tw[1] + P2 * tw[2] + P3 * tw[3];
+        bc[c] = to_store<OT>((mc && mc[c]) ? 0.0f : v * scale);
+    }
+}
+
+// 3-D trilinear prolongation added to the fine x in the column-march shape
+// (same taps and masking as k_correct).
+template <int FINE, typename CT, typename FT>
+__global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int c2, const CT* __restrict__ xc,
+                                                   int f0, int f1, int f2, FT* __restrict__ xf,
+                                                   const uint8_t* __restrict__ mf, const double* __restrict__ rr,
+                                                   const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t, k0, k1;
+    if (!march_begin(M, t, k0, k1)) return;
+    const int j = t / f0;
+    const int i = t - j * f0;
+    auto taps = [](int F, int nc, int* ti, float* tw) {
+        const int ci = F >> 1;
+        const bool even = (F & 1) == 0;
+        const int lower = ci > 0 ? ci - 1 : ci;
+        const int upper = ci < nc - 1 ? ci + 1 : ci;
+        ti[0] = even ? lower : ci;
+        tw[0] = even ? 0.25f : 0.75f;
+        ti[1] = even ? ci : upper;
+        tw[1] = even ? 0.75f : 0.25f;
+    };
+    int xi[2], yj[2];
+    float wx[2], wy[2];
+    taps(i, c0, xi, wx);
+    taps(j, c1, yj, wy);
+    const float scale = rr ? residual_scale(rr, f0 * f1 * f2) : 1.0f;
+    const int cplane = c0 * c1;
+    for (int k = k0; k < k1; ++k) {
+        const int c = i + f0 * (j + f1 * k);
+        if (mf && mf[c]) continue;
+        int zk[2];
+        float wz[2];
+        taps(k, c2, zk, wz);
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+                for (int cq = 0; cq < 2; ++cq) {
+                    const float wgt = wx[a] * wy[bq] * wz[cq];
+                    sum += to_f32(xc[xi[a] + c0 * yj[bq] + cplane * zk[cq]]) * wgt;
+                }
+        if (rr) sum = sum / scale;
+        xf[c] = to_store<FT>(to_f32(xf[c]) + sum);
+    }
+}
+
+// Coarsest level in one workgroup: `iters` x (colour 0, colour 1) sweeps from
+// x = 0 (buildCoarseSweepKernel, multigrid.js:679-737).  x lives in LDS for
+// the whole solve; each thread keeps its cells' b, face weights, neighbour
+// indices and Dirichlet bits in registers.  Every update is rounded through
+// the storage type, as a store to the level's x would be.
+static constexpr int COARSE_LDS_CELLS = 4096;
+static constexpr int COARSE_TPB = 1024;
+template <typename XT, typename WT>
+__global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x,
+                                                             const XT* __restrict__ b, float omega, int iters,
+                                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
+    __shared__ float sx[COARSE_LDS_CELLS];
     const int n = L.n0 * L.n1 * L.n2;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = to_store<XT>(0.0f);
+    const uint8_t* __restrict__ m = L.mask;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    float bc[PER], w[PER][6];
+    int q[PER][6];
+    unsigned bits[PER];  // bit a: neighbour a is Dirichlet; bit 6: the cell is; bit 7: colour
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int c = threadIdx.x + u * COARSE_TPB;
+        bits[u] = 0;
+        bc[u] = 0.0f;
+        if (c < n) {
+            int i, j, k;
+            cell_coords(L, c, i, j, k);
+            face_w3(L, i, j, k, c, w[u]);
+            nbr_index3(L.n0, L.n1, L.n2, i, j, k, c, q[u]);
+            if (m) {
+                for (int a = 0; a < 6; ++a) bits[u] |= m[q[u][a]] ? (1u << a) : 0u;
+                bits[u] |= m[c] ? 64u : 0u;
+            }
+            bits[u] |= ((i + j + k) & 1) ? 128u : 0u;
+            bc[u] = to_f32(b[c]);
+        }
+    }
+    for (int c = threadIdx.x; c < n; c += COARSE_TPB) sx[c] = 0.0f;
     __syncthreads();
     for (int it = 0; it < iters; ++it) {
         for (int color = 0; color < 2; ++color) {
-            for (int c = threadIdx.x; c < n; c += blockDim.x) {
-                int i, j, k;
-                cell_coords(L, c, i, j, k);
-                if (((i + j + k) & 1) != color) continue;
-                float d;
-                float ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(x[q]); }, &d);
-                x[c] = to_store<XT>(to_f32(x[c]) + ((to_f32(b[c]) - ax) / d) * omega);
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int c = threadIdx.x + u * COARSE_TPB;
+                if (c >= n || (int)((bits[u] >> 7) & 1u) != color) continue;
+                const float xc = sx[c];
+                float xn[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits[u] >> a) & 1u) ? 0.0f : sx[q[u][a]];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w[u], ih2, sum, dsum);
+                const bool dir = (bits[u] >> 6) & 1u;
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                sx[c] = to_f32(to_store<XT>(xc + ((bc[u] - ax) / d) * omega));
             }
             __syncthreads();
         }
     }
+    for (int c = threadIdx.x; c < n; c += COARSE_TPB) x[c] = to_store<XT>(sx[c]);
 }
 
 // Coarse face weight = mean of the 2^(d-1) fine faces at index 2I along the
@@ -407,7 +569,7 @@ struct flux_multigrid {
     bool fused = true;
 };
 
-static constexpr int COARSE_SINGLE_BLOCK_MAX = 4096;
+static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
 static constexpr int SW_TX = 64, SW_TY = 4, SW_KZ = 32;
 
 namespace {
@@ -514,7 +676,8 @@ void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
 }
 
 template <int FINE, typename XT, typename WT>
-void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
+void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop,
+                  DotOut* dot = nullptr) {
     MGLevel& v = mg->lv[l];
     if (iters <= 0) return;
     if (use_fused(mg, v)) {
@@ -530,14 +693,19 @@ void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
         for (int it = 0; it < iters; ++it) {
             int c0 = reversed ? 1 : 0, c1 = reversed ? 0 : 1;
             if (from_zero && it == 0) {
-                k_relax3<FINE, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop);
+                k_relax3<FINE, 1, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop, nullptr);
             } else {
                 if (FINE) kprof_begin("relax");
-                k_relax3<FINE, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop);
+                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop, nullptr);
                 if (FINE) kprof_end("relax");
             }
             if (FINE) kprof_begin("relax");
-            k_relax3<FINE, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop);
+            if (dot && it == iters - 1 && M.nb <= dot->capacity) {
+                k_relax3<FINE, 0, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop, dot->partials);
+                dot->blocks = M.nb;
+            } else {
+                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop, nullptr);
+            }
             if (FINE) kprof_end("relax");
         }
         return;
@@ -560,14 +728,42 @@ void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
 }
 
 template <int FINE, typename XT, typename WT>
-void level_smooth(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
-    relax_sweeps<FINE, XT, WT>(mg, l, iters, reversed, from_zero, stop);
+void level_smooth(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop,
+                  DotOut* dot = nullptr) {
+    relax_sweeps<FINE, XT, WT>(mg, l, iters, reversed, from_zero, stop, dot);
+}
+
+// Restriction of level v's residual `r` into level c's b.
+template <int FINE, typename IT, typename OT>
+void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const IT* r, const double* scale,
+                    const int* stop) {
+    if (mg->ndim == 3) {
+        const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, g_march_blocks);
+        k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(M, v.n[0], v.n[1], v.n[2], r, v.mask, c.n[0], c.n[1],
+                                                                c.n[2], (OT*)c.b, c.mask, scale, stop);
+    } else {
+        k_restrict<FINE, IT, OT><<<blocks_for(cells_of(c.n), MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, v.n[0], v.n[1], v.n[2], r, v.mask, c.n[0], c.n[1], c.n[2], (OT*)c.b, c.mask, scale, stop);
+    }
+}
+
+// Prolongation of level c's x added into level v's x.
+template <int FINE, typename CT, typename FT>
+void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const double* scale, const int* stop) {
+    if (mg->ndim == 3) {
+        const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+        k_correct3<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(M, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0],
+                                                               v.n[1], v.n[2], (FT*)v.x, v.mask, scale, stop);
+    } else {
+        k_correct<FINE, CT, FT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(
+            mg->ndim, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0], v.n[1], v.n[2], (FT*)v.x, v.mask, scale, stop);
+    }
 }
 
 // vCycle (multigrid.js:1609-1642).  Level 0 has FP32 vectors and weights of
 // type ST; coarse levels store everything in ST.
 template <typename ST>
-int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
+int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot) {
     MGLevel& v = mg->lv[l];
     int nl = (int)mg->lv.size();
     const bool fine = (l == 0);
@@ -575,7 +771,7 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
         int64_t cells = cells_of(v.n);
         if (fine) {
             if (cells <= COARSE_SINGLE_BLOCK_MAX)
-                k_coarse_solve<float, ST><<<1, 1024, 0, g_stream>>>(level_dev<ST>(mg, v), (float*)v.x,
+                k_coarse_solve<float, ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), (float*)v.x,
                                                                     (const float*)v.b, (float)mg->o.omega,
                                                                     mg->o.n_coarsest, stop);
             else if (mg->o.n_coarsest > 0)
@@ -584,7 +780,7 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
                 fill_t<float>(v.x, cells, 0.0f);
         } else {
             if (cells <= COARSE_SINGLE_BLOCK_MAX)
-                k_coarse_solve<ST, ST><<<1, 1024, 0, g_stream>>>(level_dev<ST>(mg, v), (ST*)v.x, (const ST*)v.b,
+                k_coarse_solve<ST, ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), (ST*)v.x, (const ST*)v.b,
                                                                  (float)mg->o.omega, mg->o.n_coarsest, stop);
             else if (mg->o.n_coarsest > 0)
                 level_smooth<0, ST, ST>(mg, l, mg->o.n_coarsest, false, true, stop);
@@ -605,9 +801,7 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
                                                                       (const float*)v.b, (float*)v.buf, stop);
         kprof_end("residual");
         kprof_begin("restrict");
-        k_restrict<1, float, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
-            mg->ndim, v.n[0], v.n[1], v.n[2], (const float*)v.buf, v.mask, c.n[0], c.n[1], c.n[2], (ST*)c.b, c.mask,
-            scale, stop);
+        restrict_level<1, float, ST>(mg, v, c, (const float*)v.buf, scale, stop);
         kprof_end("restrict");
     } else {
         if (mg->o.nu_down > 0) level_smooth<0, ST, ST>(mg, l, mg->o.nu_down, false, true, stop);
@@ -615,20 +809,16 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr) {
         const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
         k_residual3<0, ST, ST, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), M, (const ST*)v.x,
                                                                 (const ST*)v.b, (ST*)v.buf, stop);
-        k_restrict<0, ST, ST><<<blocks_for(cc, MTPB), MTPB, 0, g_stream>>>(
-            mg->ndim, v.n[0], v.n[1], v.n[2], (const ST*)v.buf, v.mask, c.n[0], c.n[1], c.n[2], (ST*)c.b, c.mask,
-            nullptr, stop);
+        restrict_level<0, ST, ST>(mg, v, c, (const ST*)v.buf, nullptr, stop);
     }
-    FLUX_TRY(vcycle_t<ST>(mg, l + 1, stop, rr));
+    FLUX_TRY(vcycle_t<ST>(mg, l + 1, stop, rr, nullptr));
     if (fine) {
         kprof_begin("correct");
-        k_correct<1, ST, float><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
-            mg->ndim, c.n[0], c.n[1], c.n[2], (const ST*)c.x, v.n[0], v.n[1], v.n[2], (float*)v.x, v.mask, scale, stop);
+        correct_level<1, ST, float>(mg, v, c, scale, stop);
         kprof_end("correct");
-        level_smooth<1, float, ST>(mg, l, mg->o.n_final, true, false, stop);
+        level_smooth<1, float, ST>(mg, l, mg->o.n_final, true, false, stop, dot);
     } else {
-        k_correct<0, ST, ST><<<blocks_for(cells, MTPB), MTPB, 0, g_stream>>>(
-            mg->ndim, c.n[0], c.n[1], c.n[2], (const ST*)c.x, v.n[0], v.n[1], v.n[2], (ST*)v.x, v.mask, nullptr, stop);
+        correct_level<0, ST, ST>(mg, v, c, nullptr, stop);
         level_smooth<0, ST, ST>(mg, l, mg->o.nu_up, true, false, stop);
     }
     return FLUX_OK;
@@ -648,13 +838,14 @@ int mg_refresh(flux_multigrid* mg) {
 
 // z = M r on the stream.  `stop` (device) short-circuits every kernel; `rr`
 // (device, may be null) is r.r, used for the FP16 residual scale.
-int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr) {
+int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr, DotOut* dot) {
     mg->lv[0].x = z;
     mg->lv[0].b = (void*)r;
+    if (dot) dot->blocks = 0;
     switch (mg->o.storage) {
-        case FLUX_STORE_F16: FLUX_TRY(vcycle_t<__half>(mg, 0, stop, rr)); break;
-        case FLUX_STORE_BF16: FLUX_TRY(vcycle_t<__hip_bfloat16>(mg, 0, stop, nullptr)); break;
-        default: FLUX_TRY(vcycle_t<float>(mg, 0, stop, nullptr)); break;
+        case FLUX_STORE_F16: FLUX_TRY(vcycle_t<__half>(mg, 0, stop, rr, dot)); break;
+        case FLUX_STORE_BF16: FLUX_TRY(vcycle_t<__hip_bfloat16>(mg, 0, stop, nullptr, dot)); break;
+        default: FLUX_TRY(vcycle_t<float>(mg, 0, stop, nullptr, dot)); break;
     }
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
@@ -817,7 +1008,7 @@ int flux_multigrid_apply(flux_multigrid* mg, const flux_field* in, flux_field* o
     FLUX_TRY(check_field(in, FLUX_F32, cells, "multigrid: input"));
     FLUX_TRY(check_field(out, FLUX_F32, cells, "multigrid: output"));
     FLUX_REQUIRE(in->d != out->d, "multigrid: input and output must differ");
-    FLUX_TRY(mg_apply_raw(mg, (const float*)in->d, (float*)out->d, nullptr, nullptr));
+    FLUX_TRY(mg_apply_raw(mg, (const float*)in->d, (float*)out->d, nullptr, nullptr, nullptr));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     return FLUX_OK;
 }

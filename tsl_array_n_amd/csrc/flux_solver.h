@@ -6,7 +6,15 @@ namespace flux {
 int fill_device(void* p, int dtype, int64_t n, double v);
 int check_shape(int ndim, const int* shape, const double* spacing, int spacing_len, const char* who);
 int mg_refresh(flux_multigrid* mg);
-int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr);
+// Optional r.z fused into the V-cycle's last finest-level launch: when the
+// V-cycle ends with such a launch it stores one partial per block in
+// `partials` and sets `blocks`; otherwise `blocks` stays 0.
+struct DotOut {
+    double* partials = nullptr;
+    unsigned capacity = 0;  // partials available; a launch with more blocks is not fused
+    unsigned blocks = 0;
+};
+int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr, DotOut* dot);
 int64_t mg_cells(const flux_multigrid* mg);
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
 }  // namespace flux

@@ -456,6 +456,7 @@ struct flux_pcg {
     Scal* S = nullptr;
     Scal* Sh = nullptr;  // pinned host mirror
     double* partials = nullptr;
+    unsigned n_partials = 0;
     March M;  // column-march grid of the iteration kernels
     int last_iterations = 4;
 };
@@ -467,10 +468,13 @@ namespace flux {
 static constexpr unsigned RED_MAX_BLOCKS = 2048;
 static unsigned red_blocks(int64_t n) { return std::min<unsigned>(blocks_for(n, TPB), RED_MAX_BLOCKS); }
 
-static int pcg_precondition(flux_pcg* s) {
+// z = M r.  With `dot`, the multigrid V-cycle may also produce the r.z
+// partials (dot->blocks > 0).
+static int pcg_precondition(flux_pcg* s, DotOut* dot) {
     const int* stop = &s->S->stop;
     unsigned g = blocks_for(s->n, TPB);
-    if (s->precond == FLUX_PREC_MULTIGRID) return mg_apply_raw(s->mg, s->r, s->z, stop, &s->S->rr);
+    if (dot) dot->blocks = 0;
+    if (s->precond == FLUX_PREC_MULTIGRID) return mg_apply_raw(s->mg, s->r, s->z, stop, &s->S->rr, dot);
     if (s->precond == FLUX_PREC_JACOBI) k_jacobi<<<g, TPB, 0, g_stream>>>(s->op->L, s->r, s->z, stop);
     else k_copy_f32<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, stop);
     return FLUX_OK;
@@ -487,11 +491,17 @@ static int pcg_iteration(flux_pcg* s) {
                                                  s->r, s->S, s->partials);
     kprof_end("update");
     k_fin_rr<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
-    FLUX_TRY(pcg_precondition(s));
-    kprof_begin("dot");
-    k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
-    kprof_end("dot");
-    k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g, 0);
+    DotOut dot;
+    dot.partials = s->partials;
+    dot.capacity = s->n_partials;
+    FLUX_TRY(pcg_precondition(s, &dot));
+    if (dot.blocks == 0) {
+        kprof_begin("dot");
+        k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
+        kprof_end("dot");
+        dot.blocks = g;
+    }
+    k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, dot.blocks, 0);
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
@@ -511,7 +521,7 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     k_fin_setup<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
     k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->partials);
     k_fin_bb<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
-    FLUX_TRY(pcg_precondition(s));
+    FLUX_TRY(pcg_precondition(s, nullptr));
     k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
     k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g, 1);
     k_fill_f32<<<gfull, TPB, 0, g_stream>>>(s->pa, (int)s->n, 0.0f);
@@ -577,7 +587,11 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     s->xb = (double*)dmalloc(cells * 8);
     s->S = (Scal*)dmalloc(sizeof(Scal));
     s->M = march_shape(op->n[0], op->n[1], op->n[2], TPB, g_march_blocks);
-    s->partials = (double*)dmalloc(sizeof(double) * (std::max<int64_t>(blocks_for(cells, TPB), s->M.nb) + 64));
+    // partials of every reduction: the capped grid-stride grids, the column-march
+    // grids of the iteration kernels and of the V-cycle's fused r.z (a march
+    // grid over cell pairs; larger grids fall back to the separate dot kernel)
+    s->n_partials = (unsigned)std::max<int64_t>(std::max<int64_t>(RED_MAX_BLOCKS, s->M.nb), 4096);
+    s->partials = (double*)dmalloc(sizeof(double) * s->n_partials);
     void* hp = nullptr;
     if (hipHostMalloc(&hp, sizeof(Scal), hipHostMallocDefault) != hipSuccess) hp = nullptr;
     s->Sh = (Scal*)hp;
