@@ -265,7 +265,7 @@ __global__ void k_restrict(int ndim, int f0, int f1, int f2, const IT* __restric
 template <int FINE, typename CT, typename FT>
 __global__ void k_correct(int ndim, int c0, int c1, int c2, const CT* __restrict__ xc, int f0, int f1, int f2,
                           FT* __restrict__ xf, const uint8_t* __restrict__ mf, const double* __restrict__ rr,
-                          const int* __restrict__ stop) {
+                          float mult, const int* __restrict__ stop) {
     if (stop && *stop) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= f0 * f1 * f2) return;
@@ -303,6 +303,7 @@ __global__ void k_correct(int ndim, int c0, int c1, int c2, const CT* __restrict
                 sum += to_f32(xc[ti[0][a] + c0 * (ti[1][bq] + c1 * ti[2][cq])]) * wgt;
             }
     if (rr) sum = sum / residual_scale(rr, f0 * f1 * f2);
+    if (mult != 1.0f) sum = sum * mult;
     xf[t] = to_store<FT>(to_f32(xf[t]) + sum);
 }
 
@@ -373,7 +374,7 @@ template <int FINE, typename CT, typename FT>
 __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int c2, const CT* __restrict__ xc,
                                                    int f0, int f1, int f2, FT* __restrict__ xf,
                                                    const uint8_t* __restrict__ mf, const double* __restrict__ rr,
-                                                   const int* __restrict__ stop) {
+                                                   float mult, const int* __restrict__ stop) {
     if (stop && *stop) return;
     int t, k0, k1;
     if (!march_begin(M, t, k0, k1)) return;
@@ -412,6 +413,7 @@ __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int 
                     sum += to_f32(xc[xi[a] + c0 * yj[bq] + cplane * zk[cq]]) * wgt;
                 }
         if (rr) sum = sum / scale;
+        if (mult != 1.0f) sum = sum * mult;
         xf[c] = to_store<FT>(to_f32(xf[c]) + sum);
     }
 }
@@ -556,6 +558,9 @@ struct MGLevel {
     void* w[3] = {nullptr, nullptr, nullptr};  // level 0: caller's FP32 or own converted copy
     bool own = false;
     bool own_w = false;
+    // the level stores x / xscale and applies xscale * A (1 except on FP16
+    // coarse levels); Gauss-Seidel and the residual are unchanged by it
+    double xscale = 1.0;
 };
 
 struct flux_multigrid {
@@ -585,7 +590,7 @@ LevelDevT<WT> level_dev(const flux_multigrid* mg, const MGLevel& v) {
     L.ndim = mg->ndim;
     for (int a = 0; a < 3; ++a) {
         double hh = (a < mg->ndim) ? v.h[a] : 1.0;
-        L.ih2d[a] = 1.0 / (hh * hh);
+        L.ih2d[a] = v.xscale / (hh * hh);
         L.ih2[a] = (float)L.ih2d[a];
     }
     L.mask = v.mask;
@@ -750,13 +755,15 @@ void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, cons
 // Prolongation of level c's x added into level v's x.
 template <int FINE, typename CT, typename FT>
 void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const double* scale, const int* stop) {
+    const float mult = (float)(c.xscale / v.xscale);  // stored coarse x to the fine level's storage scale
     if (mg->ndim == 3) {
         const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
         k_correct3<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(M, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0],
-                                                               v.n[1], v.n[2], (FT*)v.x, v.mask, scale, stop);
+                                                               v.n[1], v.n[2], (FT*)v.x, v.mask, scale, mult, stop);
     } else {
         k_correct<FINE, CT, FT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(
-            mg->ndim, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0], v.n[1], v.n[2], (FT*)v.x, v.mask, scale, stop);
+            mg->ndim, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0], v.n[1], v.n[2], (FT*)v.x, v.mask, scale, mult,
+            stop);
     }
 }
 
@@ -932,6 +939,9 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
             v.n[a] = (a < ndim) ? (n0[a] >> l) : 1;
             v.h[a] = (a < ndim) ? spacing[a] * (double)(1 << l) : 1.0;
         }
+        // FP16 coarse levels hold x / h_l^2 (operator scaled by h_l^2) so the
+        // solution, of size h^2 |b|, stays clear of FP16's subnormal range
+        v.xscale = (o.storage == FLUX_STORE_F16 && l > 0) ? v.h[0] * v.h[0] : 1.0;
         int64_t cells = cells_of(v.n);
         size_t vs = (l == 0) ? 4 : es;  // finest vectors stay FP32
         v.buf = dmalloc(cells * vs);
