@@ -257,6 +257,65 @@ int flux_two_phase_beta(const flux_grid3* g, const flux_field* cell_start, float
 /* MAC divergence at cell centres (grid_math3.js:397-410) */
 int flux_divergence(const flux_grid3* g, const flux_field* const* vel, flux_field* out);
 
+/* ------------------------------------------------ FLIP grid stages (3-D) */
+/* createExtrapolateToRegion3 (array_utils3.js:88-228): `iterations` Jacobi
+ * steps in which an invalid cell takes the mean of its valid 6-neighbours
+ * (summed +x,-x,+y,-y,+z,-z) and becomes valid.  value/out: FLUX_F32 of one
+ * shape (out may be value); valid: FLUX_I32, FLUX_U8 or FLUX_F32 (nonzero =
+ * valid).  The object owns the ping-pong scratch (create once, run many). */
+typedef struct flux_extrapolator flux_extrapolator;
+int flux_extrapolator3_create(const flux_field* value, const flux_field* valid, flux_field* out,
+                              flux_extrapolator** e);
+int flux_extrapolator3_run(flux_extrapolator* e, int iterations);
+int flux_extrapolator3_destroy(flux_extrapolator* e);
+/* clearFluidMask + markFluidCellsKernel (grid_flip_solver3.js:216-245):
+ * fluid_mask (FLUX_F32 cells) = 1 in every cell holding a particle, else 0 */
+int flux_mark_fluid_cells(const flux_grid3* g, const flux_field* positions, int64_t count, flux_field* fluid_mask);
+/* computeU/V/WFluidAdjacent (grid_flip_solver3.js:595-625): out (FLUX_I32,
+ * the axis's face shape) = 1 where either neighbouring cell (clamped) is fluid */
+int flux_fluid_adjacent(const flux_grid3* g, const flux_field* fluid_mask, int axis, flux_field* out);
+
+/* Static collider primitives (primitive_sdf3.js:21-70): type 0 sphere
+ * {center, radius}; type 1 box {center, half extents, optional rotation
+ * about `axis` by `angle` radians when rotated != 0}. */
+typedef struct {
+    int type;
+    int rotated;
+    double center[3];
+    double radius;
+    double half[3];
+    double axis[3];
+    double angle;
+} flux_primitive3;
+/* FP64 grid geometry for the collider set-up (the reference computes it on
+ * the host in double precision) */
+typedef struct {
+    int n[3];
+    double h[3];
+    double origin[3];
+} flux_grid3d;
+/* computeFaceWeights (sdf_collider3.js:131-262 with fractionInsideCpu,
+ * level_set_utils.js:89-161): open area fraction of every U/V/W face,
+ * floored at 0.01 when open and 0 when all four corners are on or inside
+ * the surface; weights[3] are FLUX_F32 face fields. */
+int flux_collider_face_weights(const flux_grid3d* g, const flux_primitive3* shapes, int count, int invert,
+                               flux_field* const* weights);
+/* addShapes (sdf_collider3.js:71-110): the signed distance at cell centres */
+int flux_collider_sdf(const flux_grid3d* g, const flux_primitive3* shapes, int count, int invert, flux_field* sdf);
+
+/* constrainVelocity (grid_blocked_boundary_condition_solver3.js:389-477) on
+ * the MAC velocity `vel`: with a static collider (its SDF on the velocity
+ * grid's cells, optional; its face weights, optional) the marker/project,
+ * extrapolation, no-flux projection with friction and blocked-boundary
+ * passes; then the closed domain walls (DIRECTION_* bits: left 1, right 2,
+ * down 4, up 8, back 16, front 32; default 63) and the +-100 clamp. */
+typedef struct flux_boundary3 flux_boundary3;
+int flux_boundary3_create(const flux_grid3* g, flux_field* const* vel, const flux_field* sdf,
+                          const flux_field* const* face_weights, float friction, flux_boundary3** out);
+int flux_boundary3_set_closed(flux_boundary3* b, int flags);
+int flux_boundary3_constrain(flux_boundary3* b, int extrapolation_depth);
+int flux_boundary3_destroy(flux_boundary3* b);
+
 /* -------------------------------------------------- learned initial guess */
 /* No reference counterpart (the reference has no learned component): a 3-D
  * conv U-Net (4 levels, 16/32/64/64 channels, 3x3x3, ~0.69M parameters)

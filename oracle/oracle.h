@@ -192,4 +192,40 @@ int orc_get_threads(void);
 #ifdef __cplusplus
 }
 #endif
+
+/* ---------------- FLIP grid stages (oracle_flip.c) ---------------- */
+/* createExtrapolateToRegion3: array_utils3.js:88-228 (in may equal out) */
+void orc_extrapolate3(const int n[3], const float* in, const int32_t* valid, float* out, int iterations);
+/* clearFluidMask + markFluidCellsKernel: grid_flip_solver3.js:216-245 */
+void orc_mark_fluid3(const float* pos, int64_t count, const float origin[3], const float h[3], const int n[3],
+                     float* mask);
+/* computeU/V/WFluidAdjacent: grid_flip_solver3.js:595-625 */
+void orc_fluid_adjacent3(const float* mask, const int n[3], int axis, int32_t* out);
+/* sphere (type 0) / box (type 1, optional rotation about `axis` by `angle`):
+ * primitive_sdf3.js:21-70, 149-208 */
+typedef struct {
+    int type;
+    int rotated;
+    double center[3];
+    double radius;
+    double half[3];
+    double axis[3];
+    double angle;
+} orc_primitive3;
+double orc_primitives_distance3(const orc_primitive3* shapes, int count, double x, double y, double z);
+/* fractionInsideCpu: level_set_utils.js:89-161 */
+double orc_fraction_inside(double bl, double br, double tl, double tr);
+/* computeFaceWeights: sdf_collider3.js:131-262 */
+void orc_face_weights3(const orc_primitive3* shapes, int count, int invert, const int n[3], const double h[3],
+                       const double origin[3], float* wu, float* wv, float* ww);
+/* addShapes: the collider SDF at cell centres, sdf_collider3.js:71-110 */
+void orc_collider_sdf3(const orc_primitive3* shapes, int count, int invert, const int n[3], const double h[3],
+                       const double origin[3], float* sdf);
+/* constrainVelocity: grid_blocked_boundary_condition_solver3.js:389-477 for a
+ * static collider (sdf on the velocity grid's cells, may be NULL; face
+ * weights may be NULL) and the closed-wall flags */
+void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const float h[3], const float origin[3],
+                             int closed_flags, const float* sdf, const float* wu, const float* wv, const float* ww,
+                             float friction, int depth);
+
 #endif

@@ -509,3 +509,89 @@ def pcg_solve_mixed(shape, spacing, b, x0=None, mask=None, weights=None, precond
         "threshold": st.threshold,
         "bNorm": st.b_norm,
     }
+
+
+# ---------------------------------------------------------------- FLIP grid stages
+class Primitive3(C.Structure):
+    _fields_ = [("type", C.c_int), ("rotated", C.c_int), ("center", C.c_double * 3), ("radius", C.c_double),
+                ("half", C.c_double * 3), ("axis", C.c_double * 3), ("angle", C.c_double)]
+
+
+def _prims(shapes):
+    arr = (Primitive3 * max(len(shapes), 1))()
+    for q, s in enumerate(shapes):
+        p = arr[q]
+        p.type = 0 if s["type"] == "sphere" else 1
+        for a in range(3):
+            p.center[a] = s["center"][a]
+        if p.type == 0:
+            p.radius = s["radius"]
+        else:
+            for a in range(3):
+                p.half[a] = s["halfExtents"][a]
+            if s.get("rotation"):
+                p.rotated = 1
+                for a in range(3):
+                    p.axis[a] = s["rotation"]["axis"][a]
+                p.angle = s["rotation"]["angle"]
+    return arr, len(shapes)
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def extrapolate3(n, value, valid, iterations=5):
+    value = np.ascontiguousarray(value, np.float32)
+    valid = np.ascontiguousarray(valid, np.int32)
+    out = np.empty_like(value)
+    lib().orc_extrapolate3(_i3(n), _p(value), _p(valid), _p(out), int(iterations))
+    return out
+
+
+def mark_fluid3(pos, n, h, origin):
+    pos = np.ascontiguousarray(pos, np.float32)
+    mask = np.zeros(int(np.prod(n)), np.float32)
+    lib().orc_mark_fluid3(_p(pos), C.c_int64(pos.size // 3), _f3(origin), _f3(h), _i3(n), _p(mask))
+    return mask
+
+
+def fluid_adjacent3(mask, n, axis):
+    mask = np.ascontiguousarray(mask, np.float32)
+    out = np.zeros(int(np.prod(face_shape(n, axis))), np.int32)
+    lib().orc_fluid_adjacent3(_p(mask), _i3(n), int(axis), _p(out))
+    return out
+
+
+def primitives_distance3(shapes, x, y, z):
+    arr, cnt = _prims(shapes)
+    lib().orc_primitives_distance3.restype = C.c_double
+    return lib().orc_primitives_distance3(arr, cnt, C.c_double(x), C.c_double(y), C.c_double(z))
+
+
+def fraction_inside(bl, br, tl, tr):
+    lib().orc_fraction_inside.restype = C.c_double
+    return lib().orc_fraction_inside(C.c_double(bl), C.c_double(br), C.c_double(tl), C.c_double(tr))
+
+
+def face_weights3(shapes, n, h, origin, invert=False):
+    arr, cnt = _prims(shapes)
+    outs = [np.zeros(int(np.prod(face_shape(n, a))), np.float32) for a in range(3)]
+    lib().orc_face_weights3(arr, cnt, int(bool(invert)), _i3(n), _d3(h), _d3(origin), *[_p(o) for o in outs])
+    return outs
+
+
+def collider_sdf3(shapes, n, h, origin, invert=False):
+    arr, cnt = _prims(shapes)
+    out = np.zeros(int(np.prod(n)), np.float32)
+    lib().orc_collider_sdf3(arr, cnt, int(bool(invert)), _i3(n), _d3(h), _d3(origin), _p(out))
+    return out
+
+
+def constrain_velocity3(vel, n, h, origin, closed_flags=63, sdf=None, weights=None, friction=0.5, depth=5):
+    u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in vel]
+    sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
+    ws = [None, None, None] if weights is None else [np.ascontiguousarray(a, np.float32) for a in weights]
+    lib().orc_constrain_velocity3(_p(u), _p(v), _p(w), _i3(n), _f3(h), _f3(origin), int(closed_flags), _p(sdf),
+                                  _p(ws[0]), _p(ws[1]), _p(ws[2]), C.c_float(friction), int(depth))
+    return [u, v, w]

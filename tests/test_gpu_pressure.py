@@ -191,5 +191,10 @@ def test_mixed_precision_projection_reaches_same_tolerance(flux, storage):
     dm = pm.project(vin)
     print("plain", diag, storage, dm)
     assert dm["converged"] and dm["residual"] <= 1e-6 * dm["bNorm"]
-    assert dm["iterations"] <= int(1.5 * diag["iterations"])
+    if storage == "f16":
+        # FP16 with the h^2-scaled coarse levels preconditions as well as FP32
+        assert dm["iterations"] <= diag["iterations"] + 2
+    else:
+        # BF16's 8-bit mantissa costs iterations
+        assert dm["iterations"] <= int(1.5 * diag["iterations"])
     assert rel(pm.pressure.to_array(), ps.pressure.to_array()) <= 1e-4

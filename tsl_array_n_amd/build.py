@@ -26,6 +26,7 @@ COMMON = [
 # per-file extras: the transfer kernels must round exactly like the oracle
 EXTRA = {
     "flux_transfer.hip": ["-ffp-contract=off"],
+    "flux_flip.hip": ["-ffp-contract=off"],
 }
 
 

@@ -10,101 +10,13 @@
 #include <algorithm>
 #include <cmath>
 
+#include "flux_grid3.h"
 #include "flux_internal.h"
 #include "flux_solver.h"
 
 namespace flux {
 
 static constexpr int XTPB = 256;
-
-struct G3 {
-    int n0, n1, n2;
-    float h0, h1, h2;
-    float o0, o1, o2;
-};
-
-static G3 make_g3(const flux_grid3* g) {
-    return G3{g->n[0], g->n[1], g->n[2], g->h[0], g->h[1], g->h[2], g->origin[0], g->origin[1], g->origin[2]};
-}
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// MAC data origins (grid_data3.js:128-130)
-__device__ __forceinline__ void mac_origin(const G3& g, int comp, float o[3]) {
-    o[0] = g.o0;
-    o[1] = g.o1;
-    o[2] = g.o2;
-    if (comp != 0) o[0] = g.o0 + g.h0 * 0.5f;
-    if (comp != 1) o[1] = g.o1 + g.h1 * 0.5f;
-    if (comp != 2) o[2] = g.o2 + g.h2 * 0.5f;
-}
-
-struct Tri {
-    int i0, i1, j0, j1, k0, k1;
-    float w[8];  // 000,100,010,110,001,101,011,111
-};
-
-// trilinearCoordsAndWeights3 (grid_math3.js:61-95)
-__device__ __forceinline__ void tri_weights(const float p[3], const float o[3], const float h[3], const int s[3],
-                                            Tri& t) {
-    float f[3];
-    int i0[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float g = (p[a] - o[a]) / h[a];
-        i0[a] = (int)floorf(g);
-        f[a] = g - (float)i0[a];
-    }
-    t.i0 = clampi(i0[0], 0, s[0] - 1);
-    t.i1 = clampi(i0[0] + 1, 0, s[0] - 1);
-    t.j0 = clampi(i0[1], 0, s[1] - 1);
-    t.j1 = clampi(i0[1] + 1, 0, s[1] - 1);
-    t.k0 = clampi(i0[2], 0, s[2] - 1);
-    t.k1 = clampi(i0[2] + 1, 0, s[2] - 1);
-    float gx = 1.0f - f[0], gy = 1.0f - f[1], gz = 1.0f - f[2];
-    float fx = f[0], fy = f[1], fz = f[2];
-    t.w[0] = (gx * gy) * gz;
-    t.w[1] = (fx * gy) * gz;
-    t.w[2] = (gx * fy) * gz;
-    t.w[3] = (fx * fy) * gz;
-    t.w[4] = (gx * gy) * fz;
-    t.w[5] = (fx * gy) * fz;
-    t.w[6] = (gx * fy) * fz;
-    t.w[7] = (fx * fy) * fz;
-}
-
-// collocatedValueAtPosition3 (grid_math3.js:99-113)
-__device__ __forceinline__ float sample_tri(const float* __restrict__ d, const int s[3], const float o[3],
-                                           const float h[3], const float p[3]) {
-    Tri t;
-    tri_weights(p, o, h, s, t);
-    float acc = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        int i = (c & 1) ? t.i1 : t.i0, j = (c & 2) ? t.j1 : t.j0, k = (c & 4) ? t.k1 : t.k0;
-        float term = d[i + s[0] * (j + s[1] * k)] * t.w[c];
-        acc = (c == 0) ? term : acc + term;
-    }
-    return acc;
-}
-
-struct Mac {
-    const float* u;
-    const float* v;
-    const float* w;
-};
-
-__device__ __forceinline__ void sample_mac(const G3& g, const Mac& m, const float p[3], float out[3]) {
-    const float h[3] = {g.h0, g.h1, g.h2};
-    const float* d[3] = {m.u, m.v, m.w};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int s[3] = {g.n0 + (a == 0), g.n1 + (a == 1), g.n2 + (a == 2)};
-        float o[3];
-        mac_origin(g, a, o);
-        out[a] = sample_tri(d[a], s, o, h, p);
-    }
-}
 
 // backTrace without a collider (advection_solver3.js:20-62)
 __device__ __forceinline__ void back_trace(const G3& g, const Mac& m, const float start[3], float dt, float hmin,
