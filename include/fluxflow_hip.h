@@ -316,6 +316,51 @@ int flux_boundary3_set_closed(flux_boundary3* b, int flags);
 int flux_boundary3_constrain(flux_boundary3* b, int extrapolation_depth);
 int flux_boundary3_destroy(flux_boundary3* b);
 
+/* createGridFlipSolver3 (grid_flip_solver3.js:135-1155): free-surface FLIP
+ * with reduced pressure (air cells Dirichlet at -dt g.x), int32 fixed-point
+ * P2G, weight and fluid-adjacent extrapolation, constrainVelocity with an
+ * optional static collider, deterministic resampling (donor ranks from
+ * prefix sums, 640-793) and the pressure projection.  Particles live in the
+ * solver's own vec3 fields (all max_particles of them are simulated); the
+ * caller writes the initial positions/velocities through flux_flip3_field. */
+typedef struct {
+    int resolution[3];
+    double spacing[3];
+    double origin[3];
+    int64_t max_particles;
+    float dt;
+    float gravity[3];              /* default (0, -9.81, 0) */
+    int reduced_pressure;          /* default 1 */
+    float flip_ratio;              /* default 0.97 */
+    float velocity_damping;        /* default 0.02 */
+    float p2g_atomic_scale;        /* default 65536 */
+    float weight_epsilon;          /* default 1e-4 */
+    int closed_domain_flags;       /* default 63 (all six walls) */
+    int resample;                  /* default 1 */
+    int min_particles_per_cell;    /* default 4 */
+    int max_particles_per_cell;    /* default 8 */
+    int extrapolation_depth;       /* default 5 */
+    int max_substeps;              /* particle advection, default 32 */
+    const flux_field* collider_sdf;        /* static collider SDF on the cells, or NULL */
+    const flux_field* collider_weights[3]; /* its face weights for constrainVelocity, or NULLs */
+    float friction;                        /* default 0.5 */
+    /* the projection's options (multigrid, tolerance, ...); resolution,
+     * spacing, origin and the Dirichlet fields are set by the solver, and a
+     * max_plausible_pressure <= 0 is derived as 8 dt |g| (sum of extents)
+     * (grid_flip_solver3.js:425-432) */
+    flux_pressure3_desc pressure;
+} flux_flip3_desc;
+void flux_flip3_desc_default(flux_flip3_desc* d);
+typedef struct flux_flip3 flux_flip3;
+int flux_flip3_create(const flux_flip3_desc* d, flux_flip3** out);
+/* onAdvanceTimeStep(): advect, P2G, extrapolate, constrain, mark fluid,
+ * resample, constrain, project, constrain, extrapolate, G2P */
+int flux_flip3_step(flux_flip3* s, flux_pressure_diag* diag);
+enum { FLUX_FLIP3_POSITIONS = 0, FLUX_FLIP3_VELOCITIES = 1, FLUX_FLIP3_FLUID_MASK = 2, FLUX_FLIP3_U = 3,
+       FLUX_FLIP3_V = 4, FLUX_FLIP3_W = 5, FLUX_FLIP3_PRESSURE = 6 };
+flux_field* flux_flip3_field(flux_flip3* s, int which);
+int flux_flip3_destroy(flux_flip3* s);
+
 /* -------------------------------------------------- learned initial guess */
 /* No reference counterpart (the reference has no learned component): a 3-D
  * conv U-Net (4 levels, 16/32/64/64 channels, 3x3x3, ~0.69M parameters)

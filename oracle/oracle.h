@@ -228,4 +228,8 @@ void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const
                              int closed_flags, const float* sdf, const float* wu, const float* wv, const float* ww,
                              float friction, int depth);
 
+/* the resample pass: grid_flip_solver3.js:640-793 */
+void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const float* v, const float* w,
+                   const int n[3], const float h[3], const float origin[3], int min_pc, int max_pc);
+
 #endif

@@ -414,3 +414,61 @@ void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const
     for (int a = 0; a < 3; ++a)
         for (int64_t f = 0; f < cnt[a]; ++f) data[a][f] = clamp_component(data[a][f]);
 }
+
+/* ---- the resample pass: grid_flip_solver3.js:640-793.  Counts per cell,
+ *      donors (particles in cells holding more than max_pc) ranked by
+ *      particle index, needs of under-filled cells with at least 3 of 6
+ *      (clamped) neighbours holding min_pc, and the claim of each cell's
+ *      slice of the donor pool in cell order; a claimed donor moves to the
+ *      cell centre and takes the MAC velocity sampled there. ---- */
+void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const float* v, const float* w,
+                   const int n[3], const float h[3], const float origin[3], int min_pc, int max_pc) {
+    const int64_t cells = (int64_t)n[0] * n[1] * n[2];
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (count > 0 ? count : 1));
+    int32_t* cnt = (int32_t*)calloc(cells, sizeof(int32_t));
+    int32_t* pool = (int32_t*)malloc(sizeof(int32_t) * (count > 0 ? count : 1));
+    int32_t* need = (int32_t*)calloc(cells, sizeof(int32_t));
+    orc_cell_index3(pos, count, origin, h, n, cell);
+    for (int64_t p = 0; p < count; ++p) cnt[cell[p]]++;
+    int64_t donors = 0;
+    for (int64_t p = 0; p < count; ++p)
+        if (cnt[cell[p]] > max_pc) pool[donors++] = (int32_t)p;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+#define CNT(ii, jj, kk) cnt[(ii) + (int64_t)n[0] * ((jj) + (int64_t)n[1] * (kk))]
+                int nb = (CNT(i > 0 ? i - 1 : 0, j, k) >= min_pc) + (CNT(i < n[0] - 1 ? i + 1 : n[0] - 1, j, k) >= min_pc) +
+                         (CNT(i, j > 0 ? j - 1 : 0, k) >= min_pc) + (CNT(i, j < n[1] - 1 ? j + 1 : n[1] - 1, k) >= min_pc) +
+                         (CNT(i, j, k > 0 ? k - 1 : 0) >= min_pc) + (CNT(i, j, k < n[2] - 1 ? k + 1 : n[2] - 1) >= min_pc);
+                int c0 = CNT(i, j, k);
+#undef CNT
+                need[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = (c0 < min_pc && nb >= 3) ? min_pc - c0 : 0;
+            }
+    float ou[3], ov[3], ow[3];
+    ou[0] = origin[0]; ou[1] = origin[1] + h[1] * 0.5f; ou[2] = origin[2] + h[2] * 0.5f;
+    ov[0] = origin[0] + h[0] * 0.5f; ov[1] = origin[1]; ov[2] = origin[2] + h[2] * 0.5f;
+    ow[0] = origin[0] + h[0] * 0.5f; ow[1] = origin[1] + h[1] * 0.5f; ow[2] = origin[2];
+    int su[3] = {n[0] + 1, n[1], n[2]}, sv[3] = {n[0], n[1] + 1, n[2]}, sw[3] = {n[0], n[1], n[2] + 1};
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    int64_t base = 0;
+    for (int64_t c = 0; c < cells; ++c) {
+        int needed = need[c];
+        if (needed > 0) {
+            int i = (int)(c % n[0]), j = (int)((c / n[0]) % n[1]), k = (int)(c / ((int64_t)n[0] * n[1]));
+            float np_[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+            for (int q = 0; q < needed && q < min_pc; ++q) {
+                int64_t slot = base + q;
+                if (slot >= donors) break;
+                int32_t d = pool[slot];
+                pos[3 * d + 0] = np_[0];
+                pos[3 * d + 1] = np_[1];
+                pos[3 * d + 2] = np_[2];
+                vel[3 * d + 0] = orc_sample_trilinear3(u, su, ou, h, np_);
+                vel[3 * d + 1] = orc_sample_trilinear3(v, sv, ov, h, np_);
+                vel[3 * d + 2] = orc_sample_trilinear3(w, sw, ow, h, np_);
+            }
+        }
+        base += needed;
+    }
+    free(cell); free(cnt); free(pool); free(need);
+}

@@ -595,3 +595,74 @@ def constrain_velocity3(vel, n, h, origin, closed_flags=63, sdf=None, weights=No
     lib().orc_constrain_velocity3(_p(u), _p(v), _p(w), _i3(n), _f3(h), _f3(origin), int(closed_flags), _p(sdf),
                                   _p(ws[0]), _p(ws[1]), _p(ws[2]), C.c_float(friction), int(depth))
     return [u, v, w]
+
+
+def resample3(pos, vel, u, v, w, n, h, origin, min_pc=4, max_pc=8):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_resample3(_p(pos), _p(vel), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h),
+                        _f3(origin), int(min_pc), int(max_pc))
+    return pos, vel
+
+
+def free_surface_fields(mask, n, h, origin, gravity, dt, reduced=True):
+    """dirichlet(pos) of grid_flip_solver3.js:252-259 at cell centres (f32)."""
+    f = np.float32
+    k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    px = (f(origin[0]) + f(h[0]) * f(0.5)) + i.astype(f) * f(h[0])
+    py = (f(origin[1]) + f(h[1]) * f(0.5)) + j.astype(f) * f(h[1])
+    pz = (f(origin[2]) + f(h[2]) * f(0.5)) + k.astype(f) * f(h[2])
+    g = [f(x) for x in gravity]
+    dot = ((g[0] * px + g[1] * py) + g[2] * pz).astype(f)
+    target = (-(dot * f(dt))).astype(f).ravel() if reduced else np.zeros(int(np.prod(n)), f)
+    active = (np.asarray(mask, f) < f(0.5)).astype(np.uint8)
+    return active, target
+
+
+def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e-6, flip_ratio=0.97,
+               damping=0.02, scale=65536.0, eps=1e-4, flags=63, resample=True, min_pc=4, max_pc=8, depth=5,
+               max_substeps=32, reduced=True, max_plausible=None):
+    """One createGridFlipSolver3 time step (grid_flip_solver3.js:1083-1132)
+    composed from the oracle's stages; st holds pos, vel, u, v, w, pressure
+    and the last accepted pressure, and is updated in place."""
+    pos, vel = st["pos"], st["vel"]
+    u, v, w = st["u"], st["v"], st["w"]
+    pos = advect_particles3(pos, u, v, w, n, h, origin, dt, max_substeps)
+    vel_grid, valid = [], []
+    for a in range(3):
+        o, vd, _, _ = p2g_fixed3(pos, vel, n, h, origin, a, scale, eps)
+        vel_grid.append(o)
+        valid.append(vd)
+    old = [x.copy() for x in vel_grid]
+    vel_grid = [extrapolate3(face_shape(n, a), vel_grid[a], valid[a], depth) for a in range(3)]
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    mask = mark_fluid3(pos, n, h, origin)
+    if resample:
+        pos, vel = resample3(pos, vel, *vel_grid, n, h, origin, min_pc, max_pc)
+        mask = mark_fluid3(pos, n, h, origin)
+    adj = [fluid_adjacent3(mask, n, a) for a in range(3)]
+    if not reduced:
+        vel_grid = [(vel_grid[a] + np.float32(gravity[a]) * np.float32(dt)).astype(np.float32) for a in range(3)]
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    active, target = free_surface_fields(mask, n, h, origin, gravity, dt, reduced)
+    if max_plausible is None:
+        extent = sum(n[a] * h[a] for a in range(3))
+        gm = float(np.sqrt(sum(x * x for x in gravity)))
+        max_plausible = 8.0 * dt * gm * extent if (reduced and gm > 0) else 1e6
+    A = pressure3_assemble(n, h, vel_grid, active=active, target=target, pressure=st["pressure"], suffix="f32")
+    mg = Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], suffix="f32")
+    x, _, diag = pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], mg=mg, tol=tol,
+                                 maxiter=int(np.ceil(1.5 * max(n))))
+    bad = count_bad(x, max_plausible, suffix="f32")
+    if bad > 0:
+        x = st["p_accepted"].copy()
+    else:
+        st["p_accepted"] = x.copy()
+    diag["rejected"] = bad > 0
+    vel_grid = pressure3_correct(n, h, x, vel_grid, suffix="f32")
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    vel_grid = [extrapolate3(face_shape(n, a), vel_grid[a], adj[a], depth) for a in range(3)]
+    vel = g2p3(pos, vel, *vel_grid, *old, n, h, origin, flip_ratio, damping)
+    st.update(pos=pos, vel=vel, u=vel_grid[0], v=vel_grid[1], w=vel_grid[2], pressure=x, mask=mask)
+    return diag

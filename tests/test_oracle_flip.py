@@ -186,3 +186,84 @@ def test_constrain_no_flux_against_a_planar_wall(use_weights):
     # the open side keeps the uniform flow; the face on the wall is closed
     assert np.all(U[sl[0], sl[1], 4] == a) and np.all(V[3:13, 3:13, 4] == b)
     assert np.all(U[sl[0], sl[1], 8] == 0.0)
+
+
+def test_resample_moves_lowest_donors_to_needing_cells():
+    """A 3x3x1 slab of cells: the centre-left cell holds 9 particles (> 8, all
+    donors), its neighbours hold 4 each, the centre cell is empty with 3
+    neighbours holding >= 4: it receives the 4 lowest-index donors, placed at
+    its centre with the grid velocity there."""
+    n, h, o = (3, 3, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(0)
+    pos = []
+
+    def put(i, j, k):
+        pos.append([i + rng.uniform(0.1, 0.9), j + rng.uniform(0.1, 0.9), k + rng.uniform(0.1, 0.9)])
+
+    for _ in range(4):
+        put(1, 0, 0)
+        put(1, 2, 0)
+    donors_at = len(pos)
+    for _ in range(9):
+        put(0, 1, 0)
+    for _ in range(4):
+        put(2, 1, 0)
+    pos = np.array(pos, np.float32)
+    vel = np.zeros_like(pos)
+    u = np.full(int(np.prod(O.face_shape(n, 0))), 0.5, np.float32)
+    v = np.full(int(np.prod(O.face_shape(n, 1))), -0.25, np.float32)
+    w = np.full(int(np.prod(O.face_shape(n, 2))), 2.0, np.float32)
+    p2, v2 = O.resample3(pos.ravel(), vel.ravel(), u, v, w, n, h, o)
+    p2 = p2.reshape(-1, 3)
+    v2 = v2.reshape(-1, 3)
+    moved = [q for q in range(len(pos)) if not np.array_equal(p2[q], pos[q])]
+    assert moved == list(range(donors_at, donors_at + 4))
+    for q in moved:
+        assert p2[q].tolist() == [1.5, 1.5, 0.5] and v2[q].tolist() == [0.5, -0.25, 2.0]
+
+
+def layer_state(n, h, height_cells, ppa=2, block_x=None):
+    pos = []
+    nx = n[0] if block_x is None else block_x
+    for k in range(n[2]):
+        for j in range(height_cells):
+            for i in range(nx):
+                for sk in range(ppa):
+                    for sj in range(ppa):
+                        for si in range(ppa):
+                            pos.append([(i + (si + 0.5) / ppa) * h[0], (j + (sj + 0.5) / ppa) * h[1],
+                                        (k + (sk + 0.5) / ppa) * h[2]])
+    pos = np.array(pos, np.float32).ravel()
+    st = {"pos": pos, "vel": np.zeros_like(pos), "pressure": np.zeros(int(np.prod(n)), np.float32),
+          "p_accepted": np.zeros(int(np.prod(n)), np.float32)}
+    for a, c in enumerate("uvw"):
+        st[c] = np.zeros(int(np.prod(O.face_shape(n, a))), np.float32)
+    return st
+
+
+def test_flip_step_keeps_a_layer_at_rest():
+    """Reduced pressure: a liquid layer at rest in a closed box is in
+    hydrostatic balance; one step leaves velocities and particles still."""
+    n, h = (8, 8, 8), (1 / 8, 1 / 8, 1 / 8)
+    st = layer_state(n, h, 4)
+    p0 = st["pos"].copy()
+    diag = O.flip3_step(st, n, h, (0, 0, 0), 1 / 60, levels=2, damping=0.0)
+    assert diag["converged"] and not diag["rejected"]
+    assert np.max(np.abs(st["vel"])) < 1e-5
+    assert np.max(np.abs(st["pos"] - p0)) < 1e-6
+
+
+def test_flip_step_dam_break_starts_to_collapse():
+    n, h = (8, 8, 8), (1 / 8, 1 / 8, 1 / 8)
+    st = layer_state(n, h, 6, block_x=3)
+    count = st["pos"].size // 3
+    for _ in range(3):
+        diag = O.flip3_step(st, n, h, (0, 0, 0), 1 / 60, levels=2)
+        assert diag["converged"]
+    p = st["pos"].reshape(-1, 3)
+    v = st["vel"].reshape(-1, 3)
+    assert p.shape[0] == count and np.all(p >= 0) and np.all(p <= 1)
+    top = p[:, 1] > 0.6
+    assert np.mean(v[top, 1]) < -0.05            # the top of the column falls
+    front = p[:, 0] > 0.3
+    assert np.mean(v[front, 0]) > 0.05           # the front runs out along +x

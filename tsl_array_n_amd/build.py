@@ -27,6 +27,7 @@ COMMON = [
 EXTRA = {
     "flux_transfer.hip": ["-ffp-contract=off"],
     "flux_flip.hip": ["-ffp-contract=off"],
+    "flux_flip_solver.hip": ["-ffp-contract=off"],
 }
 
 

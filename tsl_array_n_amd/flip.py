@@ -1,0 +1,114 @@
+"""createGridFlipSolver3 on the GPU (packages/fluxflow/src/grid/grid_flip_solver3.js:135-1155).
+
+Free-surface FLIP with reduced pressure, int32 fixed-point P2G, extrapolation,
+constrainVelocity (optional static collider), deterministic resampling and
+the MGPCG pressure projection.  ``on_advance_time_step()`` runs one step;
+positions/velocities/fluid_mask/pressure and the velocity grid are fields.
+"""
+import ctypes as C
+
+from . import (F32, PREC, VEC3, Field, MGOptions, Pressure3Desc, PressureDiag, STOP_NAMES, _check, _lib,
+               face_shape, mg_options)
+
+
+class Flip3Desc(C.Structure):
+    _fields_ = [("resolution", C.c_int * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3),
+                ("max_particles", C.c_int64), ("dt", C.c_float), ("gravity", C.c_float * 3),
+                ("reduced_pressure", C.c_int), ("flip_ratio", C.c_float), ("velocity_damping", C.c_float),
+                ("p2g_atomic_scale", C.c_float), ("weight_epsilon", C.c_float), ("closed_domain_flags", C.c_int),
+                ("resample", C.c_int), ("min_particles_per_cell", C.c_int), ("max_particles_per_cell", C.c_int),
+                ("extrapolation_depth", C.c_int), ("max_substeps", C.c_int), ("collider_sdf", C.c_void_p),
+                ("collider_weights", C.c_void_p * 3), ("friction", C.c_float), ("pressure", Pressure3Desc)]
+
+
+_FIELDS = {"positions": 0, "velocities": 1, "fluid_mask": 2, "u": 3, "v": 4, "w": 5, "pressure": 6}
+
+
+class GridFlipSolver3:
+    def __init__(self, resolution, grid_spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), max_particles=None,
+                 dt=None, gravity=(0.0, -9.81, 0.0), reduced_pressure=True, flip_ratio=0.97, velocity_damping=0.02,
+                 p2g_atomic_scale=65536.0, weight_epsilon=1e-4, closed_domain_boundary_flag=63, collider_sdf=None,
+                 collider_weights=None, friction=0.5, resample=None, pressure=None, max_substeps=32,
+                 extrapolation_depth=5):
+        if not max_particles:
+            raise ValueError("createGridFlipSolver3: options.maxParticles is required.")
+        if dt is None:
+            raise ValueError("createGridFlipSolver3: dt is required.")
+        d = Flip3Desc()
+        _lib.flux_flip3_desc_default(C.byref(d))
+        for a in range(3):
+            d.resolution[a] = int(resolution[a])
+            d.spacing[a] = float(grid_spacing[a])
+            d.origin[a] = float(origin[a])
+            d.gravity[a] = float(gravity[a])
+        d.max_particles = int(max_particles)
+        d.dt = float(dt)
+        d.reduced_pressure = int(bool(reduced_pressure))
+        d.flip_ratio = flip_ratio
+        d.velocity_damping = velocity_damping
+        d.p2g_atomic_scale = p2g_atomic_scale
+        d.weight_epsilon = weight_epsilon
+        d.closed_domain_flags = int(closed_domain_boundary_flag)
+        rs = resample or {}
+        d.resample = int(bool(rs.get("enabled", True)))
+        d.min_particles_per_cell = int(rs.get("minParticlesPerCell", 4))
+        d.max_particles_per_cell = int(rs.get("maxParticlesPerCell", 8))
+        d.extrapolation_depth = int(extrapolation_depth)
+        d.max_substeps = int(max_substeps)
+        if collider_sdf is not None:
+            d.collider_sdf = collider_sdf.h
+        if collider_weights is not None:
+            for a in range(3):
+                d.collider_weights[a] = collider_weights[a].h
+        d.friction = friction
+        p = pressure or {}
+        if "multigrid" in p:
+            d.pressure.multigrid = mg_options(**p["multigrid"])
+        if "preconditioner" in p:
+            d.pressure.preconditioner = PREC[p["preconditioner"]]
+        for key, field in (("tolerance", "tolerance"), ("maxIterationsFactor", "max_iterations_factor"),
+                           ("maxPlausiblePressure", "max_plausible_pressure")):
+            if key in p:
+                setattr(d.pressure, field, float(p[key]))
+        if "maxIterations" in p:
+            d.pressure.max_iterations = int(p["maxIterations"])
+        if "relativeTolerance" in p:
+            d.pressure.relative_tolerance = int(bool(p["relativeTolerance"]))
+        if "colliderWeights" in p:
+            for a in range(3):
+                d.pressure.collider_weights[a] = p["colliderWeights"][a].h
+        self._keep = (collider_sdf, collider_weights, p)
+        h = C.c_void_p()
+        _check(_lib.flux_flip3_create(C.byref(d), C.byref(h)), "createGridFlipSolver3")
+        self.h = h.value
+        n = tuple(int(x) for x in resolution)
+        self.resolution = n
+        self.positions = self._view("positions", VEC3, (int(max_particles),))
+        self.velocities = self._view("velocities", VEC3, (int(max_particles),))
+        self.fluid_mask = self._view("fluid_mask", F32, n)
+        self.velocity_grid = [self._view(c, F32, face_shape(n, a)) for a, c in enumerate("uvw")]
+        self.pressure = self._view("pressure", F32, n)
+        self.diagnostics = {}
+
+    def _view(self, name, dtype, shape):
+        _lib.flux_flip3_field.restype = C.c_void_p
+        ptr = _lib.flux_flip3_field(C.c_void_p(self.h), _FIELDS[name])
+        return Field(dtype, shape, handle=ptr, owned=False)
+
+    def on_advance_time_step(self):
+        dg = PressureDiag()
+        _check(_lib.flux_flip3_step(C.c_void_p(self.h), C.byref(dg)), "onAdvanceTimeStep")
+        self.diagnostics = {"converged": bool(dg.converged), "rejected": bool(dg.rejected),
+                            "iterations": dg.iterations, "stoppedBy": STOP_NAMES.get(dg.stopped_by, "unknown"),
+                            "residual": dg.residual, "bNorm": dg.b_norm, "badCells": dg.bad_cells}
+        return self.diagnostics
+
+    def __del__(self):
+        try:
+            _lib.flux_flip3_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+def create_grid_flip_solver3(**kw):
+    return GridFlipSolver3(**kw)
