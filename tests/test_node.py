@@ -33,7 +33,9 @@ def test_api_surface_and_host_helpers(tmp_path):
                  "createJacobiPreconditioner", "createIdentityPreconditioner",
                  "createPreconditionedConjugateGradientSolver", "createGridPressureSolver3",
                  "createFaceCenteredGrid3", "computeFlipBoxSeed3", "createJitterRandom", "binParticles",
-                 "particleToGrid", "gridToParticle", "advectParticles", "advectFaceCentered3"]:
+                 "particleToGrid", "gridToParticle", "advectParticles", "advectFaceCentered3",
+                 "createExtrapolateToRegion3", "createSDFStaticCollider3", "sphere", "box",
+                 "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3"]:
         assert name in d["exports"], name
     assert d["strides"] == [1, 4, 12]  # TA/test/array.test.js:41-93
     assert "1D/2D/3D" in d["shapeError"]
@@ -74,3 +76,48 @@ def test_projection_through_node_matches_oracle(tmp_path):
         got = np.fromfile(tmp_path / (name + ".bin"), np.float32)
         assert np.linalg.norm(got - own[a]) <= 1e-6 * vin
         assert np.linalg.norm(got - out[a]) <= 1e-3 * vin
+
+
+@pytest.mark.gpu
+def test_flip_and_collider_through_node_match_oracle(tmp_path):
+    """createGridFlipSolver3, createSDFStaticCollider3.computeFaceWeights and
+    constrainVelocity driven from Node, against the oracle."""
+    run_node("flip_step.mjs", tmp_path, timeout=300)
+    meta = json.load(open(tmp_path / "meta.json"))
+    n, h, dt = (16, 16, 16), (1 / 16,) * 3, 1 / 60
+    pos = np.fromfile(tmp_path / "positions.bin", np.float32)
+    st = {"pos": None, "vel": None, "pressure": np.zeros(16 ** 3, np.float32), "p_accepted": np.zeros(16 ** 3, np.float32)}
+    p0 = []
+    for k in range(16):
+        for j in range(10):
+            for i in range(6):
+                for sk in range(2):
+                    for sj in range(2):
+                        for si in range(2):
+                            p0.append([(i + (si + 0.5) / 2) * h[0], (j + (sj + 0.5) / 2) * h[1], (k + (sk + 0.5) / 2) * h[2]])
+    st["pos"] = np.array(p0, np.float32).ravel()
+    st["vel"] = np.zeros_like(st["pos"])
+    for a, c in enumerate("uvw"):
+        st[c] = np.zeros(int(np.prod(O.face_shape(n, a))), np.float32)
+    for s in range(2):
+        d = O.flip3_step(st, n, h, (0, 0, 0), dt, levels=3)
+        assert meta["diags"][s]["converged"] and d["converged"]
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "fluid.bin", np.float32), st["mask"])
+    assert np.max(np.abs(pos - st["pos"])) <= 1e-5
+    vel = np.fromfile(tmp_path / "velocities.bin", np.float32)
+    assert np.linalg.norm(vel - st["vel"]) <= 1e-4 * np.linalg.norm(st["vel"])
+    # collider
+    sh = [{"type": "sphere", "center": (5.3, 6.1, 4.7), "radius": 2.6}]
+    m = (12, 12, 12)
+    sdf = O.collider_sdf3(sh, m, (1, 1, 1), (0, 0, 0))
+    np.testing.assert_allclose(np.fromfile(tmp_path / "sdf.bin", np.float32), sdf, atol=1e-6)
+    wts = O.face_weights3(sh, m, (1, 1, 1), (0, 0, 0))
+    for a, c in enumerate("uvw"):
+        assert np.max(np.abs(np.fromfile(tmp_path / ("weight_" + c + ".bin"), np.float32) - wts[a])) <= 1e-6
+    vin = [np.fromfile(tmp_path / ("vin_%s.bin" % "x".join(str(x) for x in O.face_shape(m, a))), np.float32)
+           for a in range(3)]
+    want = O.constrain_velocity3(vin, m, (1, 1, 1), (0, 0, 0), closed_flags=1 | 4 | 32, sdf=sdf, weights=wts,
+                                 friction=0.5, depth=5)
+    for a, c in enumerate("uvw"):
+        np.testing.assert_allclose(np.fromfile(tmp_path / ("vout_" + c + ".bin"), np.float32), want[a], rtol=1e-6,
+                                   atol=1e-6)

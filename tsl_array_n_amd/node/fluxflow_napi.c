@@ -32,7 +32,7 @@
         }                                                     \
     } while (0)
 
-enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3 };
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP };
 
 typedef struct {
     int kind;
@@ -50,6 +50,9 @@ static void fin_handle(napi_env env, void* data, void* hint) {
         case K_MG: flux_multigrid_destroy((flux_multigrid*)h->ptr); break;
         case K_PCG: flux_pcg_destroy((flux_pcg*)h->ptr); break;
         case K_P3: flux_pressure3_destroy((flux_pressure3*)h->ptr); break;
+        case K_EXTRAP: flux_extrapolator3_destroy((flux_extrapolator*)h->ptr); break;
+        case K_BND: flux_boundary3_destroy((flux_boundary3*)h->ptr); break;
+        case K_FLIP: flux_flip3_destroy((flux_flip3*)h->ptr); break;
         default: break;
     }
     free(h);
@@ -460,6 +463,8 @@ static napi_value js_pressure3_field(napi_env env, napi_callback_info info) {
     return wrap(env, K_FIELD_VIEW, f);
 }
 
+static napi_value diag_object(napi_env env, const flux_pressure_diag* d);
+
 static napi_value js_pressure3_project(napi_env env, napi_callback_info info) {
     napi_value a[3];
     args_n(env, info, 3, a);
@@ -472,6 +477,11 @@ static napi_value js_pressure3_project(napi_env env, napi_callback_info info) {
     flux_pressure_diag dg;
     CHECK_FLUX(flux_pressure3_project((flux_pressure3*)unwrap(env, a[0], K_P3, K_P3), in[0], in[1], in[2],
                                       (flux_field*)out[0], (flux_field*)out[1], (flux_field*)out[2], &dg));
+    return diag_object(env, &dg);
+}
+
+static napi_value diag_object(napi_env env, const flux_pressure_diag* d) {
+    const flux_pressure_diag dg = *d;
     napi_value o, s, b1, b2;
     napi_create_object(env, &o);
     napi_get_boolean(env, dg.converged != 0, &b1);
@@ -590,6 +600,229 @@ static napi_value js_two_phase_beta(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+/* ------------------------------------------------------- FLIP grid stages */
+static napi_value js_extrapolator_create(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    flux_extrapolator* e = NULL;
+    CHECK_FLUX(flux_extrapolator3_create(get_field(env, a[0]), get_field(env, a[1]), get_field(env, a[2]), &e));
+    return wrap(env, K_EXTRAP, e);
+}
+
+static napi_value js_extrapolator_run(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_extrapolator3_run((flux_extrapolator*)unwrap(env, a[0], K_EXTRAP, K_EXTRAP), get_int(env, a[1], 5)));
+    return undef(env);
+}
+
+static napi_value js_mark_fluid_cells(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_grid3 g;
+    if (get_grid(env, a[0], &g)) {
+        napi_throw_type_error(env, NULL, "markFluidCells: bad grid");
+        return NULL;
+    }
+    CHECK_FLUX(flux_mark_fluid_cells(&g, get_field(env, a[1]), (int64_t)get_double(env, a[2], 0), get_field(env, a[3])));
+    return undef(env);
+}
+
+static napi_value js_fluid_adjacent(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_grid3 g;
+    if (get_grid(env, a[0], &g)) {
+        napi_throw_type_error(env, NULL, "fluidAdjacent: bad grid");
+        return NULL;
+    }
+    CHECK_FLUX(flux_fluid_adjacent(&g, get_field(env, a[1]), get_int(env, a[2], 0), get_field(env, a[3])));
+    return undef(env);
+}
+
+static int get_grid3d(napi_env env, napi_value o, flux_grid3d* g) {
+    napi_value v;
+    if (!(v = prop(env, o, "n")) || get_int_array(env, v, g->n, 3) != 3) return -1;
+    for (int i = 0; i < 3; ++i) {
+        g->h[i] = 1.0;
+        g->origin[i] = 0.0;
+    }
+    if ((v = prop(env, o, "h"))) get_double_array(env, v, g->h, 3);
+    if ((v = prop(env, o, "origin"))) get_double_array(env, v, g->origin, 3);
+    return 0;
+}
+
+/* shapes: [{type: 'sphere'|'box', center, radius, halfExtents, rotation: {axis, angle}}] */
+static int get_shapes(napi_env env, napi_value arr, flux_primitive3** out, int* count) {
+    uint32_t n = 0;
+    if (napi_get_array_length(env, arr, &n) != napi_ok) return -1;
+    flux_primitive3* p = (flux_primitive3*)calloc(n > 0 ? n : 1, sizeof(flux_primitive3));
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value e, v;
+        napi_get_element(env, arr, i, &e);
+        char type[16] = {0};
+        size_t len = 0;
+        if ((v = prop(env, e, "type"))) napi_get_value_string_utf8(env, v, type, sizeof(type), &len);
+        p[i].type = strcmp(type, "sphere") == 0 ? 0 : (strcmp(type, "box") == 0 ? 1 : -1);
+        if ((v = prop(env, e, "center"))) get_double_array(env, v, p[i].center, 3);
+        if ((v = prop(env, e, "radius"))) p[i].radius = get_double(env, v, 0.0);
+        if ((v = prop(env, e, "halfExtents"))) get_double_array(env, v, p[i].half, 3);
+        if ((v = prop(env, e, "rotation")) && !is_nullish(env, v)) {
+            napi_value r;
+            p[i].rotated = 1;
+            if ((r = prop(env, v, "axis"))) get_double_array(env, r, p[i].axis, 3);
+            if ((r = prop(env, v, "angle"))) p[i].angle = get_double(env, r, 0.0);
+        }
+    }
+    *out = p;
+    *count = (int)n;
+    return 0;
+}
+
+static napi_value js_collider_face_weights(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_grid3d g;
+    const flux_field* w[3];
+    flux_primitive3* sh = NULL;
+    int count = 0;
+    bool inv = false;
+    napi_get_value_bool(env, a[2], &inv);
+    if (get_grid3d(env, a[0], &g) || get_field_array(env, a[3], w) != 3 || get_shapes(env, a[1], &sh, &count)) {
+        free(sh);
+        napi_throw_type_error(env, NULL, "computeFaceWeights: bad arguments");
+        return NULL;
+    }
+    int rc = flux_collider_face_weights(&g, sh, count, inv ? 1 : 0, (flux_field* const*)w);
+    free(sh);
+    CHECK_FLUX(rc);
+    return undef(env);
+}
+
+static napi_value js_collider_sdf(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_grid3d g;
+    flux_primitive3* sh = NULL;
+    int count = 0;
+    bool inv = false;
+    napi_get_value_bool(env, a[2], &inv);
+    if (get_grid3d(env, a[0], &g) || get_shapes(env, a[1], &sh, &count)) {
+        free(sh);
+        napi_throw_type_error(env, NULL, "addShapes: bad arguments");
+        return NULL;
+    }
+    int rc = flux_collider_sdf(&g, sh, count, inv ? 1 : 0, get_field(env, a[3]));
+    free(sh);
+    CHECK_FLUX(rc);
+    return undef(env);
+}
+
+static napi_value js_boundary3_create(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    flux_grid3 g;
+    const flux_field* vel[3];
+    const flux_field* w[3] = {NULL, NULL, NULL};
+    int nw = get_field_array(env, a[3], w);
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[1], vel) != 3 || (nw != 0 && nw != 3)) {
+        napi_throw_type_error(env, NULL, "createGridBlockedBoundaryConditionSolver3: bad arguments");
+        return NULL;
+    }
+    flux_boundary3* b = NULL;
+    CHECK_FLUX(flux_boundary3_create(&g, (flux_field* const*)vel, is_nullish(env, a[2]) ? NULL : get_field(env, a[2]),
+                                     nw == 3 ? w : NULL, (float)get_double(env, a[4], 0.5), &b));
+    return wrap(env, K_BND, b);
+}
+
+static napi_value js_boundary3_set_closed(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_boundary3_set_closed((flux_boundary3*)unwrap(env, a[0], K_BND, K_BND), get_int(env, a[1], 63)));
+    return undef(env);
+}
+
+static napi_value js_boundary3_constrain(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_boundary3_constrain((flux_boundary3*)unwrap(env, a[0], K_BND, K_BND), get_int(env, a[1], 5)));
+    return undef(env);
+}
+
+/* createGridFlipSolver3 options object (grid_flip_solver3.js:135-163) */
+static napi_value js_flip3_create(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    napi_value o = a[0], v;
+    flux_flip3_desc d;
+    flux_flip3_desc_default(&d);
+    int tmp[3];
+    double dv[3];
+    if ((v = prop(env, o, "resolution")) && get_int_array(env, v, tmp, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.resolution[i] = tmp[i];
+    if ((v = prop(env, o, "gridSpacing")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.spacing[i] = dv[i];
+    if ((v = prop(env, o, "origin")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.origin[i] = dv[i];
+    if ((v = prop(env, o, "gravity")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.gravity[i] = (float)dv[i];
+    if ((v = prop(env, o, "maxParticles"))) d.max_particles = (int64_t)get_double(env, v, 0);
+    if ((v = prop(env, o, "dt"))) d.dt = (float)get_double(env, v, 0);
+    if ((v = prop(env, o, "reducedPressure"))) {
+        bool b = true;
+        napi_get_value_bool(env, v, &b);
+        d.reduced_pressure = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "flipRatio"))) d.flip_ratio = (float)get_double(env, v, d.flip_ratio);
+    if ((v = prop(env, o, "velocityDamping"))) d.velocity_damping = (float)get_double(env, v, d.velocity_damping);
+    if ((v = prop(env, o, "p2gAtomicScale"))) d.p2g_atomic_scale = (float)get_double(env, v, d.p2g_atomic_scale);
+    if ((v = prop(env, o, "weightEpsilon"))) d.weight_epsilon = (float)get_double(env, v, d.weight_epsilon);
+    if ((v = prop(env, o, "closedDomainBoundaryFlag")) && !is_nullish(env, v))
+        d.closed_domain_flags = get_int(env, v, d.closed_domain_flags);
+    if ((v = prop(env, o, "resampleEnabled"))) {
+        bool b = true;
+        napi_get_value_bool(env, v, &b);
+        d.resample = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "minParticlesPerCell"))) d.min_particles_per_cell = get_int(env, v, d.min_particles_per_cell);
+    if ((v = prop(env, o, "maxParticlesPerCell"))) d.max_particles_per_cell = get_int(env, v, d.max_particles_per_cell);
+    if ((v = prop(env, o, "colliderSdf")) && !is_nullish(env, v)) d.collider_sdf = get_field(env, v);
+    if ((v = prop(env, o, "colliderWeights")) && get_field_array(env, v, d.collider_weights) < 0) {
+        napi_throw_type_error(env, NULL, "createGridFlipSolver3: colliderWeights must be three fields");
+        return NULL;
+    }
+    if ((v = prop(env, o, "friction"))) d.friction = (float)get_double(env, v, d.friction);
+    if ((v = prop(env, o, "multigrid"))) mg_opts(env, v, &d.pressure.multigrid);
+    if ((v = prop(env, o, "preconditionerKind"))) d.pressure.preconditioner = get_int(env, v, d.pressure.preconditioner);
+    if ((v = prop(env, o, "tolerance"))) d.pressure.tolerance = get_double(env, v, d.pressure.tolerance);
+    if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.pressure.max_iterations = get_int(env, v, 0);
+    if ((v = prop(env, o, "maxPlausiblePressure")) && !is_nullish(env, v))
+        d.pressure.max_plausible_pressure = get_double(env, v, 0.0);
+    flux_flip3* s = NULL;
+    CHECK_FLUX(flux_flip3_create(&d, &s));
+    return wrap(env, K_FLIP, s);
+}
+
+static napi_value js_flip3_step(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    flux_pressure_diag dg;
+    CHECK_FLUX(flux_flip3_step((flux_flip3*)unwrap(env, a[0], K_FLIP, K_FLIP), &dg));
+    return diag_object(env, &dg);
+}
+
+static napi_value js_flip3_field(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = flux_flip3_field((flux_flip3*)unwrap(env, a[0], K_FLIP, K_FLIP), get_int(env, a[1], 0));
+    if (!f) {
+        napi_value n;
+        napi_get_null(env, &n);
+        return n;
+    }
+    return wrap(env, K_FIELD_VIEW, f);
+}
+
 #define EXPORT(name, fn)                                                   \
     do {                                                                   \
         napi_value f_;                                                     \
@@ -623,6 +856,18 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("advectParticles", js_advect_particles);
     EXPORT("advectFaces", js_advect_faces);
     EXPORT("twoPhaseBeta", js_two_phase_beta);
+    EXPORT("extrapolatorCreate", js_extrapolator_create);
+    EXPORT("extrapolatorRun", js_extrapolator_run);
+    EXPORT("markFluidCells", js_mark_fluid_cells);
+    EXPORT("fluidAdjacent", js_fluid_adjacent);
+    EXPORT("colliderFaceWeights", js_collider_face_weights);
+    EXPORT("colliderSdf", js_collider_sdf);
+    EXPORT("boundary3Create", js_boundary3_create);
+    EXPORT("boundary3SetClosed", js_boundary3_set_closed);
+    EXPORT("boundary3Constrain", js_boundary3_constrain);
+    EXPORT("flip3Create", js_flip3_create);
+    EXPORT("flip3Step", js_flip3_step);
+    EXPORT("flip3Field", js_flip3_field);
     return exports;
 }
 

@@ -354,7 +354,7 @@ export function createGridPressureSolver3( {
 			diagnostics.residual = d.residual;
 			diagnostics.stoppedBy = d.stoppedBy === 'converged' ? 'none' : d.stoppedBy;
 			diagnostics.badCells = d.badCells;
-			return keep && diagnostics;
+			return diagnostics;
 
 		};
 
@@ -362,7 +362,9 @@ export function createGridPressureSolver3( {
 
 	return {
 		project, pressure, b: view( FIELD_RHS, 'float' ), diagnostics, settings,
-		dirichletMask: view( FIELD_MASK, 'u8' ), ventMask: view( FIELD_VENT, 'u8' )
+		dirichletMask: view( FIELD_MASK, 'u8' ), ventMask: view( FIELD_VENT, 'u8' ),
+		// the native solver reads these fields on every project(): hold them
+		inputs: keep
 	};
 
 }
@@ -467,5 +469,172 @@ export function twoPhaseBeta( grid, bins, gasDensity, liquidDensity, fraction ) 
 	const beta = { u: arrayN( 'float', grid.dataSizeU ), v: arrayN( 'float', grid.dataSizeV ), w: arrayN( 'float', grid.dataSizeW ) };
 	addon.twoPhaseBeta( gridArg( grid ), bins ? bins.cellStart.handle : null, gasDensity, liquidDensity, fraction.handle, [ beta.u.handle, beta.v.handle, beta.w.handle ] );
 	return beta;
+
+}
+
+// ------------------------------------------------------ FLIP grid stages
+// createExtrapolateToRegion3 (array_utils3.js:166-228): run(numberOfIterations = 5)
+export function createExtrapolateToRegion3( inputField, validField, outputField, shape = validField.shape ) {
+
+	if ( inputField.count !== validField.count || outputField.count !== validField.count ) {
+
+		throw new Error( `createExtrapolateToRegion3: shape mismatch [${ shape }].` );
+
+	}
+
+	const handle = addon.extrapolatorCreate( inputField.handle, validField.handle, outputField.handle );
+	function run( numberOfIterations = 5 ) {
+
+		addon.extrapolatorRun( handle, numberOfIterations );
+
+	}
+
+	run.inputs = { inputField, validField, outputField };
+	return run;
+
+}
+
+// primitive_sdf3.js:21-70 shape descriptors
+export function sphere( center, radius ) {
+
+	return { type: 'sphere', center, radius };
+
+}
+
+export function box( center, halfExtents, rotation = null ) {
+
+	return { type: 'box', center, halfExtents, rotation };
+
+}
+
+// createSDFStaticCollider3 (sdf_collider3.js:24-262): the SDF lives on a
+// cell-centred grid; computeFaceWeights evaluates the shapes analytically
+// (FP64) on a velocity grid's faces.
+export function createSDFStaticCollider3( resolutionX, resolutionY, resolutionZ, gridSpacingX, gridSpacingY, gridSpacingZ, originX = 0, originY = 0, originZ = 0 ) {
+
+	const resolution = [ resolutionX, resolutionY, resolutionZ ];
+	const gridSpacing = [ gridSpacingX, gridSpacingY, gridSpacingZ ];
+	const origin = [ originX, originY, originZ ];
+	const data = arrayN( 'float', resolution );
+	let currentShapes = [];
+	let currentInvert = false;
+	const collider = { grid: { resolution, gridSpacing, origin, data }, frictionCoefficient: 0.5 };
+	collider.addShapes = ( shapes, { invert = false } = {} ) => {
+
+		currentShapes = shapes;
+		currentInvert = invert;
+		addon.colliderSdf( { n: resolution, h: gridSpacing, origin }, shapes, invert, data.handle );
+
+	};
+
+	collider.addShape = ( shape, options ) => collider.addShapes( [ shape ], options );
+	collider.computeFaceWeights = ( nx, ny, nz, hx, hy, hz, ox = 0, oy = 0, oz = 0 ) => {
+
+		const u = arrayN( 'float', [ nx + 1, ny, nz ] );
+		const v = arrayN( 'float', [ nx, ny + 1, nz ] );
+		const w = arrayN( 'float', [ nx, ny, nz + 1 ] );
+		addon.colliderFaceWeights( { n: [ nx, ny, nz ], h: [ hx, hy, hz ], origin: [ ox, oy, oz ] }, currentShapes, currentInvert,
+			[ u.handle, v.handle, w.handle ] );
+		return { u, v, w, fields: { u, v, w } };
+
+	};
+
+	return collider;
+
+}
+
+// createGridBlockedBoundaryConditionSolver3 (grid_blocked_boundary_condition_solver3.js:47-490)
+// for a static collider; inflow objects are not supported by this build.
+export function createGridBlockedBoundaryConditionSolver3( velocity, resolutionX, resolutionY, resolutionZ,
+	gridSpacingX, gridSpacingY, gridSpacingZ, originX = 0, originY = 0, originZ = 0,
+	colliderSDF = null, inflows = null, colliderFaceWeights = null ) {
+
+	if ( inflows ) throw new Error( 'createGridBlockedBoundaryConditionSolver3: inflows are not supported by this build.' );
+	const grid = { n: [ resolutionX, resolutionY, resolutionZ ], h: [ gridSpacingX, gridSpacingY, gridSpacingZ ], origin: [ originX, originY, originZ ] };
+	const weights = colliderFaceWeights ? [ colliderFaceWeights.fields.u.handle, colliderFaceWeights.fields.v.handle, colliderFaceWeights.fields.w.handle ] : null;
+	const handle = addon.boundary3Create( grid, [ velocity.dataU.handle, velocity.dataV.handle, velocity.dataW.handle ],
+		colliderSDF ? colliderSDF.grid.data.handle : null, weights, colliderSDF ? colliderSDF.frictionCoefficient : 0.5 );
+	let flag = 63;
+	const solver = {
+		velocity, collider: colliderSDF, inputs: { colliderFaceWeights },
+		get closedDomainBoundaryFlag() {
+
+			return flag;
+
+		},
+		set closedDomainBoundaryFlag( f ) {
+
+			addon.boundary3SetClosed( handle, f );
+			flag = f;
+
+		},
+		constrainVelocity( extrapolationDepth = 5 ) {
+
+			addon.boundary3Constrain( handle, extrapolationDepth );
+
+		}
+	};
+	return solver;
+
+}
+
+// createGridFlipSolver3 (grid_flip_solver3.js:135-1155).  The solver owns its
+// particle and velocity fields: `velocityGrid` supplies the resolution,
+// spacing and origin, and solver.velocityGrid exposes the solver's own
+// dataU/V/W.  `collider` is a createSDFStaticCollider3 object.
+export function createGridFlipSolver3( {
+	velocityGrid, gridSpacing, origin, maxParticles, dt, gravity = [ 0, - 9.81, 0 ],
+	reducedPressure = true, flipRatio = 0.97, velocityDamping = 0.02, p2gAtomicScale = 65536,
+	weightEpsilon = 1e-4, closedDomainBoundaryFlag, collider, colliderFaceWeights, resample = {}, pressure = {}
+} = {} ) {
+
+	if ( ! velocityGrid ) throw new Error( 'createGridFlipSolver3: options.velocityGrid is required.' );
+	if ( ! maxParticles ) throw new Error( 'createGridFlipSolver3: options.maxParticles is required.' );
+	const { enabled: resampleEnabled = true, minParticlesPerCell = 4, maxParticlesPerCell = 8 } = resample;
+	const resolution = velocityGrid.resolution.slice();
+	const spacing = gridSpacing || velocityGrid.gridSpacing;
+	const org = origin || velocityGrid.origin;
+	const pre = pressure.preconditioner || 'multigrid';
+	if ( ! Object.prototype.hasOwnProperty.call( PRECONDITIONERS, pre ) ) {
+
+		throw new Error( `grid_pressure_solver3: unknown preconditioner '${ pre }' (expected ${ Object.keys( PRECONDITIONERS ).join( ', ' ) }).` );
+
+	}
+
+	const handle = addon.flip3Create( {
+		resolution, gridSpacing: spacing, origin: org, maxParticles, dt, gravity, reducedPressure, flipRatio,
+		velocityDamping, p2gAtomicScale, weightEpsilon, closedDomainBoundaryFlag, resampleEnabled,
+		minParticlesPerCell, maxParticlesPerCell,
+		colliderSdf: collider ? collider.grid.data.handle : null,
+		colliderWeights: colliderFaceWeights ? [ colliderFaceWeights.fields.u.handle, colliderFaceWeights.fields.v.handle, colliderFaceWeights.fields.w.handle ] : null,
+		friction: collider ? collider.frictionCoefficient : 0.5,
+		multigrid: pressure.multigrid || {}, preconditionerKind: PRECONDITIONERS[ pre ],
+		tolerance: pressure.tolerance === undefined ? 1e-6 : pressure.tolerance,
+		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations,
+		maxPlausiblePressure: pressure.maxPlausiblePressure === undefined ? null : pressure.maxPlausiblePressure
+	} );
+	const view = ( which, type, shape ) => wrapField( addon.flip3Field( handle, which ), type, shape );
+	const [ nx, ny, nz ] = resolution;
+	const solverGrid = {
+		resolution, gridSpacing: spacing, origin: org,
+		dataSizeU: [ nx + 1, ny, nz ], dataSizeV: [ nx, ny + 1, nz ], dataSizeW: [ nx, ny, nz + 1 ],
+		dataU: view( 3, 'float', [ nx + 1, ny, nz ] ), dataV: view( 4, 'float', [ nx, ny + 1, nz ] ),
+		dataW: view( 5, 'float', [ nx, ny, nz + 1 ] )
+	};
+	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
+	async function onAdvanceTimeStep() {
+
+		const d = addon.flip3Step( handle );
+		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
+		return diagnostics;
+
+	}
+
+	return {
+		onAdvanceTimeStep, settings: { batchStages: true },
+		positions: view( 0, 'vec3', [ maxParticles ] ), velocities: view( 1, 'vec3', [ maxParticles ] ),
+		fluidMask: view( 2, 'float', resolution ), pressure: { data: view( 6, 'float', resolution ) },
+		velocityGrid: solverGrid, diagnostics, inputs: { collider, colliderFaceWeights }
+	};
 
 }
