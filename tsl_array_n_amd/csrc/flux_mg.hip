@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
 }
 
 // r = b - A x (buildResidualKernel, multigrid.js:741-749) in the column-march
-// shape (one cell per thread and plane).
+// shape (one cell per thread and plane), as the restriction's input.
 template <int FINE, typename XT, typename RT, typename WT>
 __global__ void __launch_bounds__(MTPB) k_residual3(LevelDevT<WT> L, March M, const XT* __restrict__ x,
                                                     const XT* __restrict__ b, RT* __restrict__ r,
@@ -204,8 +204,10 @@ __global__ void __launch_bounds__(MTPB) k_residual3(LevelDevT<WT> L, March M, co
         }
         float sum, dsum;
         row_terms<float>(xc, xn, w, ih2, sum, dsum);
-        const float ax = (m && m[c]) ? -xc : sum;
-        r[c] = to_store<RT>(to_f32(b[c]) - ax);
+        // a Dirichlet cell's residual is stored as 0: its only reader, the
+        // restriction, counts masked fine cells as 0
+        const bool dir = m && m[c];
+        r[c] = to_store<RT>(dir ? 0.0f : to_f32(b[c]) - sum);
     }
 }
 
@@ -426,10 +428,8 @@ __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int 
 static constexpr int COARSE_LDS_CELLS = 4096;
 static constexpr int COARSE_TPB = 1024;
 template <typename XT, typename WT>
-__global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x,
-                                                             const XT* __restrict__ b, float omega, int iters,
-                                                             const int* __restrict__ stop) {
-    if (stop && *stop) return;
+__device__ void coarse_solve_block(const LevelDevT<WT>& L, XT* __restrict__ x, const XT* __restrict__ b, float omega,
+                                   int iters) {
     constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
     __shared__ float sx[COARSE_LDS_CELLS];
     const int n = L.n0 * L.n1 * L.n2;
@@ -479,6 +479,15 @@ __global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT
         }
     }
     for (int c = threadIdx.x; c < n; c += COARSE_TPB) x[c] = to_store<XT>(sx[c]);
+}
+
+
+template <typename XT, typename WT>
+__global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x,
+                                                             const XT* __restrict__ b, float omega, int iters,
+                                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    coarse_solve_block<XT, WT>(L, x, b, omega, iters);
 }
 
 // Coarse face weight = mean of the 2^(d-1) fine faces at index 2I along the
@@ -571,11 +580,11 @@ struct flux_multigrid {
     const flux_field* coarse_mask = nullptr;
     const flux_field* w[3] = {nullptr, nullptr, nullptr};
     bool has_w = false, has_masks = false;
-    bool fused = true;
+    bool fused = false;  // fused sweeps on every level wide enough (FLUX_FUSED_SWEEP=1)
 };
 
 static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
-static constexpr int SW_TX = 64, SW_TY = 4, SW_KZ = 32;
+static constexpr int FUSED_MIN_N = 64;  // narrower levels keep the two-launch form
 
 namespace {
 
@@ -639,18 +648,24 @@ int refresh_t(flux_multigrid* mg) {
 }
 
 bool use_fused(const flux_multigrid* mg, const MGLevel& v) {
-    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= 16 && v.n[1] >= SW_TY && v.n[2] >= 4;
+    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= FUSED_MIN_N && v.n[1] >= Sweep3::TY && v.n[2] >= 2;
 }
 
-// `iters` full sweeps (colour A then B each), ping-ponging so the result lands in v.x
+// `iters` fused sweeps (colour A then B each), ping-ponging between v.x and
+// v.alt so the result lands in v.x; with `dot`, the last sweep also stores
+// the r.z partials of the final field.
 template <typename XT, typename WT>
-void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop) {
+void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop, DotOut* dot) {
     MGLevel& v = mg->lv[l];
     LevelDevT<WT> L = level_dev<WT>(mg, v);
-    int gx = (v.n[0] + SW_TX - 1) / SW_TX, gy = (v.n[1] + SW_TY - 1) / SW_TY, gz = (v.n[2] + SW_KZ - 1) / SW_KZ;
-    unsigned nb = (unsigned)(gx * gy * gz);
-    float om = (float)mg->o.omega;
-    int colorA = reversed ? 1 : 0;
+    const int gx = (v.n[0] + Sweep3::TX - 1) / Sweep3::TX, gy = (v.n[1] + Sweep3::TY - 1) / Sweep3::TY;
+    const int gxy = gx * gy;
+    int gz = std::max(1, std::min<int>(v.n[2], (int)((g_march_blocks / 2 + gxy - 1) / gxy)));
+    const int kz = (v.n[2] + gz - 1) / gz;
+    gz = (v.n[2] + kz - 1) / kz;
+    const unsigned nb = (unsigned)(gxy * gz);
+    const float om = (float)mg->o.omega;
+    const int colorA = reversed ? 1 : 0;
     const XT* src = (const XT*)v.x;
     if (!from_zero && (iters & 1)) {
         k_copy_t<XT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>((const XT*)v.x, (XT*)v.alt,
@@ -659,23 +674,25 @@ void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
     }
     for (int it = 0; it < iters; ++it) {
         XT* dst = (((iters - 1 - it) & 1) == 0) ? (XT*)v.x : (XT*)v.alt;
-        if (l == 0) kprof_begin("sweep");
-        if (from_zero && it == 0) {
-            if (l == 0)
-                k_rbgs_fused<1, SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                    L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
-            else
-                k_rbgs_fused<0, SW_TX, SW_TY, true, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                    L, nullptr, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+        const bool zero = from_zero && it == 0;
+        const bool with_dot = dot && it == iters - 1 && nb <= dot->capacity;
+        const bool fine = (l == 0);
+        if (fine) kprof_begin("sweep");
+#define FLUX_SWEEP(FINE_, ZERO_, DOT_)                                                                       \
+    k_sweep3<FINE_, ZERO_, DOT_, XT, WT><<<nb, Sweep3::NT, 0, g_stream>>>(L, zero ? nullptr : src,            \
+                                                                          (const XT*)v.b, dst, om, colorA, kz, \
+                                                                          stop, with_dot ? dot->partials : nullptr)
+        if (fine) {
+            if (zero) FLUX_SWEEP(1, true, false);
+            else if (with_dot) FLUX_SWEEP(1, false, true);
+            else FLUX_SWEEP(1, false, false);
         } else {
-            if (l == 0)
-                k_rbgs_fused<1, SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                    L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
-            else
-                k_rbgs_fused<0, SW_TX, SW_TY, false, XT, WT><<<nb, SW_TX * SW_TY, 0, g_stream>>>(
-                    L, src, (const XT*)v.b, dst, om, colorA, SW_KZ, stop);
+            if (zero) FLUX_SWEEP(0, true, false);
+            else FLUX_SWEEP(0, false, false);
         }
-        if (l == 0) kprof_end("sweep");
+#undef FLUX_SWEEP
+        if (with_dot) dot->blocks = nb;
+        if (fine) kprof_end("sweep");
         src = dst;
     }
 }
@@ -686,7 +703,7 @@ void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
     MGLevel& v = mg->lv[l];
     if (iters <= 0) return;
     if (use_fused(mg, v)) {
-        fused_sweeps<XT, WT>(mg, l, iters, reversed, from_zero, stop);
+        fused_sweeps<XT, WT>(mg, l, iters, reversed, from_zero, stop, dot);
         return;
     }
     LevelDevT<WT> L = level_dev<WT>(mg, v);
@@ -743,8 +760,10 @@ template <int FINE, typename IT, typename OT>
 void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const IT* r, const double* scale,
                     const int* stop) {
     if (mg->ndim == 3) {
-        const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, g_march_blocks);
-        k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(M, v.n[0], v.n[1], v.n[2], r, v.mask, c.n[0], c.n[1],
+        // fewer, longer columns: consecutive coarse planes share two fine planes.
+        // The residual (k_residual3) is already 0 on masked fine cells.
+        const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, std::max(64u, g_march_blocks / 4));
+        k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(M, v.n[0], v.n[1], v.n[2], r, nullptr, c.n[0], c.n[1],
                                                                 c.n[2], (OT*)c.b, c.mask, scale, stop);
     } else {
         k_restrict<FINE, IT, OT><<<blocks_for(cells_of(c.n), MTPB), MTPB, 0, g_stream>>>(
@@ -926,8 +945,9 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     for (int a = 0; a < ndim; ++a) mg->w[a] = w ? w[a] : nullptr;
     mg->has_w = (w != nullptr);
     mg->has_masks = (mask != nullptr) || (w != nullptr);
-    // the one-pass sweep is opt-in (FLUX_FUSED_SWEEP=1) until its z-march
-    // stops being latency-bound; the two-launch form is the default
+    // the one-pass sweep measured slower than two launches (247 vs 2 x 118 us
+    // at 256^3; 15 vs 2 x 5 us at 32^3: its z-march is latency-bound), so it
+    // is opt-in (FLUX_FUSED_SWEEP=1)
     const char* fenv = getenv("FLUX_FUSED_SWEEP");
     mg->fused = (fenv && fenv[0] == '1');
     mg->lv.resize(o.levels);

@@ -4,174 +4,201 @@
 // (packages/fluxflow/src/linalg/multigrid.js:626-640), colour A then colour B
 // (A = 0 for pre-smoothing, A = 1 for the reversed post-smoothing,
 // multigrid.js:1589-1607, 1640): every colour-A cell is updated from the old
-// field, then every colour-B cell from the colour-A-updated field.
+// field, then every colour-B cell from the colour-A-updated field.  The
+// arithmetic per cell is row_terms', in the same order as the two-launch
+// kernels, so the result is bit-identical to them.
 //
 // The pass is out of place (xin -> xout), so no block ever reads a value
-// another block has already overwritten.  A block owns a TX x TY column tile
+// another block has already overwritten.  A block owns a 64 x 8 column tile
 // and a chunk of z-planes and marches along z with a one-plane lag:
-//   step s: stage old plane s+1 (tile + 2-cell ring) in LDS,
+//   step s: store the prefetched old plane s+1 (tile + 2-cell ring) and its
+//           Dirichlet bits in LDS and start loading plane s+2 into registers;
 //           colour A on plane s over the tile + 1-cell ring (the ring is
-//           recomputed redundantly so no neighbour block is needed),
+//           recomputed redundantly, so no neighbour block is needed);
 //           colour B on plane s-1 over the tile, written out with the
 //           colour-A values of plane s-1.
-// Rolling LDS buffers hold four planes of old x and mask and four planes of
-// the colour-A state.
+// Old planes rotate through 5 LDS slots and colour-A planes through 4, so a
+// step needs two barriers: a thread still finishing step s-1's colour-B
+// phase never reads a slot that step s overwrites.  Face weights, b and the
+// centre's Dirichlet bit come from global memory (L2: the ring's belong to
+// the neighbouring tiles, which the XCD-aware block order keeps on the same
+// XCD at the same time).
 #pragma once
 #include "flux_stencil.h"
+#include "flux_stencil3.h"
 
 namespace flux {
 
-template <int TX, int TY>
-struct SweepTile {
-    static constexpr int R2X = TX + 4, R2Y = TY + 4, R2 = R2X * R2Y;  // tile + 2 ring
-    static constexpr int R1X = TX + 2, R1Y = TY + 2, R1 = R1X * R1Y;  // tile + 1 ring
+struct Sweep3 {
+    static constexpr int TX = 64, TY = 8, NT = TX * TY / 2;               // 256 threads
+    static constexpr int R1X = TX + 2, R1Y = TY + 2, R1 = R1X * R1Y;     // tile + 1 ring
+    static constexpr int R2X = TX + 4, R2Y = TY + 4, R2 = R2X * R2Y;     // tile + 2 ring
+    static constexpr int Q2 = (R2 + NT - 1) / NT;                        // R2 slots per thread
+    static constexpr int Q1 = (R1 + NT - 1) / NT;                        // R1 slots per thread
+    static constexpr int QT = TX * TY / NT;                              // tile cells per thread
+    static constexpr int SX = 5, SA = 4;                                 // LDS plane slots
 };
 
-template <int FINE, int TX, int TY, bool ZERO, typename XT, typename WT>
-__global__ void __launch_bounds__(TX* TY) k_rbgs_fused(LevelDevT<WT> L, const XT* __restrict__ xin,
-                                                       const XT* __restrict__ b, XT* __restrict__ xout,
-                                                       float omega, int colorA, int kchunk,
-                                                       const int* __restrict__ stop) {
-    using T = SweepTile<TX, TY>;
+// One relaxation of a cell from its (already Dirichlet-masked, 0 where
+// absent) neighbour values.
+template <typename WT>
+__device__ __forceinline__ float relax_value(const LevelDevT<WT>& L, int i, int j, int k, int c, float xc, bool dir,
+                                             const float xn[6], float bc, float omega) {
+    float w[6];
+    face_w3(L, i, j, k, c, w);
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    float sum, dsum;
+    row_terms<float>(xc, xn, w, ih2, sum, dsum);
+    const float d = dir ? -1.0f : dsum;
+    const float ax = dir ? -xc : sum;
+    return xc + ((bc - ax) / d) * omega;
+}
+
+template <int FINE, bool ZERO, bool DOT, typename XT, typename WT>
+__global__ void __launch_bounds__(Sweep3::NT) k_sweep3(LevelDevT<WT> L, const XT* __restrict__ xin,
+                                                       const XT* __restrict__ b, XT* __restrict__ xout, float omega,
+                                                       int colorA, int kchunk, const int* __restrict__ stop,
+                                                       double* __restrict__ partials) {
+    using S = Sweep3;
     if (stop && *stop) return;
-    __shared__ float s_xo[4][T::R2];
-    __shared__ uint8_t s_m[4][T::R2];
-    __shared__ float s_xa[4][T::R1];
+    __shared__ float sx[S::SX][S::R2];
+    __shared__ uint8_t sm[S::SX][S::R2];
+    __shared__ float sa[S::SA][S::R1];
 
     const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
-    const int gx = (n0 + TX - 1) / TX, gy = (n1 + TY - 1) / TY;
-    const unsigned nb = gridDim.x;
-    const unsigned lb = xcd_remap(blockIdx.x, nb);
-    const int bx = lb % gx;
-    const int by = (lb / gx) % gy;
-    const int bz = lb / (gx * gy);
-    const int i0 = bx * TX, j0 = by * TY;
+    const int plane = n0 * n1;
+    const int gx = (n0 + S::TX - 1) / S::TX, gy = (n1 + S::TY - 1) / S::TY;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = (int)(lb % (unsigned)gx);
+    const int by = (int)((lb / (unsigned)gx) % (unsigned)gy);
+    const int bz = (int)(lb / (unsigned)(gx * gy));
+    const int i0 = bx * S::TX, j0 = by * S::TY;
     const int k0 = bz * kchunk;
     const int k1 = min(k0 + kchunk, n2);
     const int tid = threadIdx.x;
-    const int plane = n0 * n1;
     const uint8_t* __restrict__ mask = L.mask;
-    const WT* __restrict__ w0 = L.w0;
-    const WT* __restrict__ w1 = L.w1;
-    const WT* __restrict__ w2 = L.w2;
-    const float ihx = L.ih2[0], ihy = L.ih2[1], ihz = L.ih2[2];
-    const int colorB = colorA ^ 1;  // ZERO: the input field is zero (xin unused)
 
-    auto load_plane = [&](int k) {
-        const int slot = k & 3;
-        for (int t = tid; t < T::R2; t += TX * TY) {
-            int ry = t / T::R2X, rx = t - ry * T::R2X;
-            int i = i0 - 2 + rx, j = j0 - 2 + ry;
+    // this thread's slots of the 2-ring plane (old x, Dirichlet bits)
+    int g2[S::Q2];
+    bool in2[S::Q2];
+#pragma unroll
+    for (int q = 0; q < S::Q2; ++q) {
+        const int r = tid + q * S::NT;
+        const int ry = r / S::R2X, rx = r - ry * S::R2X;
+        const int i = i0 - 2 + rx, j = j0 - 2 + ry;
+        in2[q] = r < S::R2 && i >= 0 && i < n0 && j >= 0 && j < n1;
+        g2[q] = i + n0 * j;
+    }
+    float px[S::Q2];
+    uint8_t pm[S::Q2];
+    auto fetch = [&](int kk) {
+        const bool ok = kk >= 0 && kk < n2;
+#pragma unroll
+        for (int q = 0; q < S::Q2; ++q) {
             float v = 0.0f;
-            uint8_t mv = 0;
-            if (k >= 0 && k < n2 && i >= 0 && i < n0 && j >= 0 && j < n1) {
-                int c = i + n0 * (j + n1 * k);
-                if constexpr (!ZERO) v = to_f32(xin[c]);
-                if (mask) mv = mask[c];
+            uint8_t m = 0;
+            if (ok && in2[q]) {
+                const int c = g2[q] + plane * kk;
+                if (!ZERO) v = to_f32(xin[c]);
+                if (mask) m = mask[c];
             }
-            s_xo[slot][t] = v;
-            s_m[slot][t] = mv;
+            px[q] = v;
+            pm[q] = m;
+        }
+    };
+    auto xs = [](int kk) { return (kk + 2 * Sweep3::SX) % Sweep3::SX; };
+    auto as = [](int kk) { return (kk + 2 * Sweep3::SA) & (Sweep3::SA - 1); };
+    auto store = [&](int kk) {
+        const int slot = xs(kk);
+#pragma unroll
+        for (int q = 0; q < S::Q2; ++q) {
+            const int r = tid + q * S::NT;
+            if (r < S::R2) {
+                sx[slot][r] = px[q];
+                sm[slot][r] = pm[q];
+            }
         }
     };
 
-    // one relaxation of cell (i,j,k) reading neighbour values through get(dx,dy,dz)
-    // and masks through mget(dx,dy,dz) (offsets in R2 coordinates)
-    auto relax_cell = [&](int i, int j, int k, float center, uint8_t mc, auto get, auto mget) -> float {
-        int c = i + n0 * (j + n1 * k);
-        if (mc) {
-            // Dirichlet row: -x = b  ->  x += ((b + x) / -1) * omega
-            return center + ((to_f32(b[c]) + center) / -1.0f) * omega;
-        }
-        float sum = 0.0f, dsum = 0.0f;
-        // x axis
-        {
-            float dLo = 0.0f, dUp = 0.0f, hLo = 0.0f, hUp = 0.0f;
-            int fl = i + (n0 + 1) * (j + n1 * k);
-            if (i > 0) { dLo = mget(-1, 0, 0) ? center : center - get(-1, 0, 0); hLo = 1.0f; }
-            if (i < n0 - 1) { dUp = mget(1, 0, 0) ? -center : get(1, 0, 0) - center; hUp = 1.0f; }
-            if (w0) { float wl = to_f32(w0[fl]), wu = to_f32(w0[fl + 1]); dLo *= wl; dUp *= wu; hLo *= wl; hUp *= wu; }
-            sum = sum + (dUp - dLo) * ihx;
-            dsum = dsum - (hLo + hUp) * ihx;
-        }
-        {
-            float dLo = 0.0f, dUp = 0.0f, hLo = 0.0f, hUp = 0.0f;
-            int fl = i + n0 * (j + (n1 + 1) * k);
-            if (j > 0) { dLo = mget(0, -1, 0) ? center : center - get(0, -1, 0); hLo = 1.0f; }
-            if (j < n1 - 1) { dUp = mget(0, 1, 0) ? -center : get(0, 1, 0) - center; hUp = 1.0f; }
-            if (w1) { float wl = to_f32(w1[fl]), wu = to_f32(w1[fl + n0]); dLo *= wl; dUp *= wu; hLo *= wl; hUp *= wu; }
-            sum = sum + (dUp - dLo) * ihy;
-            dsum = dsum - (hLo + hUp) * ihy;
-        }
-        {
-            float dLo = 0.0f, dUp = 0.0f, hLo = 0.0f, hUp = 0.0f;
-            if (k > 0) { dLo = mget(0, 0, -1) ? center : center - get(0, 0, -1); hLo = 1.0f; }
-            if (k < n2 - 1) { dUp = mget(0, 0, 1) ? -center : get(0, 0, 1) - center; hUp = 1.0f; }
-            if (w2) { float wl = to_f32(w2[c]), wu = to_f32(w2[c + plane]); dLo *= wl; dUp *= wu; hLo *= wl; hUp *= wu; }
-            sum = sum + (dUp - dLo) * ihz;
-            dsum = dsum - (hLo + hUp) * ihz;
-        }
-        return center + ((to_f32(b[c]) - sum) / dsum) * omega;
-    };
+    // prologue: planes k0-2 and k0-1 in LDS, plane k0 in flight
+    fetch(k0 - 2);
+    store(k0 - 2);
+    fetch(k0 - 1);
+    store(k0 - 1);
+    fetch(k0);
+    double dot = 0.0;
 
-    // prologue: old planes k0-2, k0-1, k0 (out-of-range planes read as zero)
-    load_plane(k0 - 2);
-    load_plane(k0 - 1);
-    load_plane(k0);
     for (int s = k0 - 1; s <= k1; ++s) {
-        load_plane(s + 1);
+        store(s + 1);
+        fetch(s + 2);
         __syncthreads();
-        // colour A on plane s, tile + 1 ring -> s_xa[s & 3] (colour-B cells keep the old value)
+        // colour A on plane s over the tile + 1 ring
         if (s >= 0 && s < n2) {
-            const int so = s & 3;
-            for (int t = tid; t < T::R1; t += TX * TY) {
-                int ry = t / T::R1X, rx = t - ry * T::R1X;
-                int i = i0 - 1 + rx, j = j0 - 1 + ry;
-                float v = 0.0f;
-                if (i >= 0 && i < n0 && j >= 0 && j < n1) {
-                    int r2 = (rx + 1) + T::R2X * (ry + 1);
-                    float center = s_xo[so][r2];
-                    v = center;
-                    if (((i + j + s) & 1) == colorA) {
-                        auto get = [&](int dx, int dy, int dz) {
-                            return s_xo[(s + dz) & 3][r2 + dx + T::R2X * dy];
-                        };
-                        auto mget = [&](int dx, int dy, int dz) {
-                            return s_m[(s + dz) & 3][r2 + dx + T::R2X * dy];
-                        };
-                        v = relax_cell(i, j, s, center, s_m[so][r2], get, mget);
-                    }
-                }
-                s_xa[so][t] = v;
+            const int xm = xs(s - 1), x0 = xs(s), xp = xs(s + 1), a0 = as(s);
+#pragma unroll
+            for (int u = 0; u < S::Q1; ++u) {
+                const int r1 = tid + u * S::NT;
+                if (r1 >= S::R1) continue;
+                const int ry = r1 / S::R1X, rx = r1 - ry * S::R1X;
+                const int i = i0 - 1 + rx, j = j0 - 1 + ry;
+                if (i < 0 || i >= n0 || j < 0 || j >= n1 || ((i + j + s) & 1) != colorA) continue;
+                const int r2 = (rx + 1) + S::R2X * (ry + 1);
+                const int c = i + n0 * (j + n1 * s);
+                auto nb = [&](int slot, int q, bool present) {
+                    return (present && !sm[slot][q]) ? sx[slot][q] : 0.0f;
+                };
+                float xn[6];
+                xn[0] = nb(x0, r2 - 1, i > 0);
+                xn[1] = nb(x0, r2 + 1, i < n0 - 1);
+                xn[2] = nb(x0, r2 - S::R2X, j > 0);
+                xn[3] = nb(x0, r2 + S::R2X, j < n1 - 1);
+                xn[4] = nb(xm, r2, s > 0);
+                xn[5] = nb(xp, r2, s < n2 - 1);
+                const float xc = sx[x0][r2];
+                // rounded through the storage type, as the two-launch form stores it
+                sa[a0][r1] = to_f32(to_store<XT>(relax_value(L, i, j, s, c, xc, sm[x0][r2] != 0, xn, to_f32(b[c]),
+                                                             omega)));
             }
         }
         __syncthreads();
-        // colour B on plane s-1 over the tile; write the plane out
+        // colour B on plane s-1 over the tile, written out with colour A
         const int kb = s - 1;
         if (kb >= k0 && kb < k1) {
-            const int sb = kb & 3;
-            for (int t = tid; t < TX * TY; t += TX * TY) {
-                int ty = t / TX, tx = t - ty * TX;
-                int i = i0 + tx, j = j0 + ty;
-                if (i < n0 && j < n1) {
-                    int r1 = (tx + 1) + T::R1X * (ty + 1);
-                    int r2 = (tx + 2) + T::R2X * (ty + 2);
-                    float center = s_xa[sb][r1];
-                    float v = center;
-                    if (((i + j + kb) & 1) == colorB) {
-                        auto get = [&](int dx, int dy, int dz) {
-                            return s_xa[(kb + dz) & 3][r1 + dx + T::R1X * dy];
-                        };
-                        auto mget = [&](int dx, int dy, int dz) {
-                            return s_m[(kb + dz) & 3][r2 + dx + T::R2X * dy];
-                        };
-                        v = relax_cell(i, j, kb, center, s_m[sb][r2], get, mget);
-                    }
-                    xout[i + n0 * (j + n1 * kb)] = to_store<XT>(v);
+            const int xb = xs(kb), xbm = xs(kb - 1), xbp = xs(kb + 1);
+            const int am = as(kb - 1), a0 = as(kb), ap = as(kb + 1);
+#pragma unroll
+            for (int u = 0; u < S::QT; ++u) {
+                const int t = tid + u * S::NT;
+                const int ty = t / S::TX, tx = t - ty * S::TX;
+                const int i = i0 + tx, j = j0 + ty;
+                if (i >= n0 || j >= n1) continue;
+                const int r1 = (tx + 1) + S::R1X * (ty + 1);
+                const int r2 = (tx + 2) + S::R2X * (ty + 2);
+                const int c = i + n0 * (j + n1 * kb);
+                float v;
+                if (((i + j + kb) & 1) == colorA) {
+                    v = sa[a0][r1];
+                } else {
+                    auto nb = [&](int aslot, int q1, int xslot, int q2, bool present) {
+                        return (present && !sm[xslot][q2]) ? sa[aslot][q1] : 0.0f;
+                    };
+                    float xn[6];
+                    xn[0] = nb(a0, r1 - 1, xb, r2 - 1, i > 0);
+                    xn[1] = nb(a0, r1 + 1, xb, r2 + 1, i < n0 - 1);
+                    xn[2] = nb(a0, r1 - S::R1X, xb, r2 - S::R2X, j > 0);
+                    xn[3] = nb(a0, r1 + S::R1X, xb, r2 + S::R2X, j < n1 - 1);
+                    xn[4] = nb(am, r1, xbm, r2, kb > 0);
+                    xn[5] = nb(ap, r1, xbp, r2, kb < n2 - 1);
+                    const float xc = sx[xb][r2];
+                    v = relax_value(L, i, j, kb, c, xc, sm[xb][r2] != 0, xn, to_f32(b[c]), omega);
                 }
+                const XT o = to_store<XT>(v);
+                xout[c] = o;
+                if (DOT) dot += (double)to_f32(b[c]) * (double)to_f32(o);
             }
         }
-        __syncthreads();
     }
+    if (DOT) store_block_partial(dot, partials);
 }
 
 }  // namespace flux
