@@ -361,6 +361,41 @@ enum { FLUX_FLIP3_POSITIONS = 0, FLUX_FLIP3_VELOCITIES = 1, FLUX_FLIP3_FLUID_MAS
 flux_field* flux_flip3_field(flux_flip3* s, int which);
 int flux_flip3_destroy(flux_flip3* s);
 
+/* createGridSmokeSolver3 (grid_smoke_solver3.js:25-154 over createGridSolver3,
+ * grid_solver3.js:45-149): buoyancy up * (a rho + b (T - T_amb)) applied as an
+ * external force, constrainVelocity (closed walls by flag), projection,
+ * order-1 semi-Lagrangian velocity advection, then density and temperature
+ * advected through the new velocity and decayed into the other halves of
+ * their ping-pong pairs.  No collider, inflow or outflow objects. */
+typedef struct {
+    int resolution[3];
+    double spacing[3];
+    double origin[3];
+    float dt;
+    float buoyancy_smoke_density_factor;  /* default -0.000625 */
+    float buoyancy_temperature_factor;    /* default 5.0 */
+    float ambient_temperature;            /* default 0 */
+    float smoke_decay;                    /* default 0.001 */
+    float temperature_decay;              /* default 0.001 */
+    float up[3];                          /* default (0, 1, 0) */
+    int closed_domain_flags;              /* default 63 */
+    int max_substeps;                     /* default 32 */
+    int extrapolation_depth;              /* constrainVelocity's, default 5 */
+    flux_pressure3_desc pressure;         /* resolution/spacing/origin set by the solver */
+} flux_smoke3_desc;
+void flux_smoke3_desc_default(flux_smoke3_desc* d);
+typedef struct flux_smoke3 flux_smoke3;
+int flux_smoke3_create(const flux_smoke3_desc* d, flux_smoke3** out);
+int flux_smoke3_step(flux_smoke3* s, flux_pressure_diag* diag);
+/* a ball source re-injected with max into the current density/temperature
+ * (examples/34-smoke-plume-3d/main.js:146-158) */
+int flux_smoke3_source_sphere(flux_smoke3* s, const float* center, float radius, float density, float temperature);
+enum { FLUX_SMOKE3_U = 0, FLUX_SMOKE3_V = 1, FLUX_SMOKE3_W = 2, FLUX_SMOKE3_DENSITY = 3, FLUX_SMOKE3_TEMPERATURE = 4,
+       FLUX_SMOKE3_PRESSURE = 5 };
+/* DENSITY / TEMPERATURE return the current state (density.current) */
+flux_field* flux_smoke3_field(flux_smoke3* s, int which);
+int flux_smoke3_destroy(flux_smoke3* s);
+
 /* -------------------------------------------------- learned initial guess */
 /* No reference counterpart (the reference has no learned component): a 3-D
  * conv U-Net (4 levels, 16/32/64/64 channels, 3x3x3, ~0.69M parameters)

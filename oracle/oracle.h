@@ -232,4 +232,15 @@ void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const
 void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const float* v, const float* w,
                    const int n[3], const float h[3], const float origin[3], int min_pc, int max_pc);
 
+/* advectScalar3, order 1, no collider: advection_solver3.js:285-300 */
+void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
+                        const float h[3], const float origin[3], float dt, int max_sub);
+/* external force with the smoke buoyancy: external_force_solver3.js:19-47,
+ * grid_smoke_solver3.js:66-81 */
+void orc_buoyancy3(float* u, float* v, float* w, const float* dens, const float* temp, const int n[3],
+                   const float h[3], const float origin[3], float a, float b, float tamb, const float up[3], float dt);
+/* ball source with max: examples/34-smoke-plume-3d/main.js:146-158 */
+void orc_source_sphere3(float* d, const int n[3], const float h[3], const float origin[3], const float c[3], float r,
+                        float value);
+
 #endif

@@ -472,3 +472,45 @@ void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const 
     }
     free(cell); free(cnt); free(pool); free(need);
 }
+
+/* ---- createExternalForceSolver3 with the smoke buoyancy force:
+ *      external_force_solver3.js:19-47, grid_smoke_solver3.js:66-81 ---- */
+void orc_buoyancy3(float* u, float* v, float* w, const float* dens, const float* temp, const int n[3],
+                   const float h[3], const float origin[3], float a, float b, float tamb, const float up[3],
+                   float dt) {
+    float* d[3] = {u, v, w};
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    for (int ax = 0; ax < 3; ++ax) {
+        int s[3] = {n[0], n[1], n[2]};
+        s[ax] += 1;
+        float fo[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+        fo[ax] = origin[ax];
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    float pos[3] = {fo[0] + (float)i * h[0], fo[1] + (float)j * h[1], fo[2] + (float)k * h[2]};
+                    float rho = orc_sample_trilinear3(dens, n, co, h, pos);
+                    float t = orc_sample_trilinear3(temp, n, co, h, pos);
+                    float fb = a * rho + b * (t - tamb);
+                    int64_t f = i + (int64_t)s[0] * (j + (int64_t)s[1] * k);
+                    d[ax][f] = d[ax][f] + (up[ax] * fb) * dt;
+                }
+    }
+}
+
+/* ---- the ball source of examples/34-smoke-plume-3d/main.js:146-158 ---- */
+void orc_source_sphere3(float* d, const int n[3], const float h[3], const float origin[3], const float c[3], float r,
+                        float value) {
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                float px = (origin[0] + h[0] * 0.5f) + (float)i * h[0];
+                float py = (origin[1] + h[1] * 0.5f) + (float)j * h[1];
+                float pz = (origin[2] + h[2] * 0.5f) + (float)k * h[2];
+                float dx = px - c[0], dy = py - c[1], dz = pz - c[2];
+                int in = (dx * dx + dy * dy) + dz * dz < r * r;
+                int64_t q = i + (int64_t)n[0] * (j + (int64_t)n[1] * k);
+                float s = in ? value : 0.0f;
+                d[q] = d[q] > s ? d[q] : s;
+            }
+}

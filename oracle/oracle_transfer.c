@@ -478,3 +478,22 @@ void orc_two_phase_beta3(const float* frac, const int n[3], float rho_gas, float
                 }
     }
 }
+
+/* ---- advectScalar3, order 1, no collider: advection_solver3.js:285-300 ---- */
+void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
+                        const float h[3], const float origin[3], float dt, int max_sub) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    const float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    const int s[3] = {n[0], n[1], n[2]};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                float pos[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+                float tr[3];
+                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
+                dst[orc_cidx(s, i, j, k)] = sample_cubic(src, s, co, h, tr);
+            }
+}

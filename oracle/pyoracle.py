@@ -666,3 +666,57 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
     vel = g2p3(pos, vel, *vel_grid, *old, n, h, origin, flip_ratio, damping)
     st.update(pos=pos, vel=vel, u=vel_grid[0], v=vel_grid[1], w=vel_grid[2], pressure=x, mask=mask)
     return diag
+
+
+# ---------------------------------------------------------------- smoke
+def advect_scalar3(src, u, v, w, n, h, origin, dt, max_substeps=32):
+    src = np.ascontiguousarray(src, np.float32)
+    out = np.empty_like(src)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_advect_scalar3(_p(src), _p(out), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin), C.c_float(dt),
+                             int(max_substeps))
+    return out
+
+
+def buoyancy3(vel, dens, temp, n, h, origin, a, b, tamb, up, dt):
+    u, v, w = [np.ascontiguousarray(x, np.float32).copy() for x in vel]
+    dens = np.ascontiguousarray(dens, np.float32)
+    temp = np.ascontiguousarray(temp, np.float32)
+    lib().orc_buoyancy3(_p(u), _p(v), _p(w), _p(dens), _p(temp), _i3(n), _f3(h), _f3(origin), C.c_float(a),
+                        C.c_float(b), C.c_float(tamb), _f3(up), C.c_float(dt))
+    return [u, v, w]
+
+
+def source_sphere3(d, n, h, origin, center, radius, value):
+    d = np.ascontiguousarray(d, np.float32).copy()
+    lib().orc_source_sphere3(_p(d), _i3(n), _f3(h), _f3(origin), _f3(center), C.c_float(radius), C.c_float(value))
+    return d
+
+
+def smoke3_step(st, n, h, origin, dt, a=-0.000625, b=5.0, tamb=0.0, up=(0.0, 1.0, 0.0), smoke_decay=0.001,
+                temperature_decay=0.001, flags=63, levels=1, tol=1e-6, max_substeps=32, max_plausible=1e6):
+    """One createGridSmokeSolver3 step (grid_smoke_solver3.js:110-130 over
+    grid_solver3.js:128-134) composed from the oracle's stages; st holds u, v,
+    w, dens, temp, pressure and the last accepted pressure."""
+    vel = buoyancy3([st["u"], st["v"], st["w"]], st["dens"], st["temp"], n, h, origin, a, b, tamb, up, dt)
+    vel = constrain_velocity3(vel, n, h, origin, closed_flags=flags)
+    A = pressure3_assemble(n, h, vel, pressure=st["pressure"], suffix="f32")
+    mg = Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], suffix="f32")
+    x, _, diag = pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], mg=mg, tol=tol,
+                                 maxiter=int(np.ceil(1.5 * max(n))))
+    if count_bad(x, max_plausible, suffix="f32") > 0:
+        x = st["p_accepted"].copy()
+        diag["rejected"] = True
+    else:
+        st["p_accepted"] = x.copy()
+        diag["rejected"] = False
+    vel = pressure3_correct(n, h, x, vel, suffix="f32")
+    vel = constrain_velocity3(vel, n, h, origin, closed_flags=flags)
+    vel = advect_face3(*vel, n, h, origin, dt, max_substeps)
+    vel = constrain_velocity3(vel, n, h, origin, closed_flags=flags)
+    raw = advect_scalar3(st["dens"], *vel, n, h, origin, dt, max_substeps)
+    dens = (raw * np.float32(np.float32(1.0) - np.float32(smoke_decay))).astype(np.float32)
+    raw = advect_scalar3(st["temp"], *vel, n, h, origin, dt, max_substeps)
+    temp = (raw * np.float32(np.float32(1.0) - np.float32(temperature_decay))).astype(np.float32)
+    st.update(u=vel[0], v=vel[1], w=vel[2], dens=dens, temp=temp, pressure=x)
+    return diag

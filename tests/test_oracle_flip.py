@@ -267,3 +267,40 @@ def test_flip_step_dam_break_starts_to_collapse():
     assert np.mean(v[top, 1]) < -0.05            # the top of the column falls
     front = p[:, 0] > 0.3
     assert np.mean(v[front, 0]) > 0.05           # the front runs out along +x
+
+
+def smoke_state(n):
+    cells = int(np.prod(n))
+    st = {"dens": np.zeros(cells, np.float32), "temp": np.zeros(cells, np.float32),
+          "pressure": np.zeros(cells, np.float32), "p_accepted": np.zeros(cells, np.float32)}
+    for a, c in enumerate("uvw"):
+        st[c] = np.zeros(int(np.prod(O.face_shape(n, a))), np.float32)
+    return st
+
+
+def test_smoke_uniform_buoyancy_is_a_pure_gradient():
+    """A uniform temperature gives a uniform vertical force: a gradient, which
+    the projection removes entirely in a closed box; the scalars only decay."""
+    n, h = (12, 12, 12), (1.0, 1.0, 1.0)
+    st = smoke_state(n)
+    st["temp"][:] = 0.5
+    st["dens"][:] = 0.25
+    d = O.smoke3_step(st, n, h, (0, 0, 0), 0.1, levels=2)
+    assert d["converged"]
+    assert max(np.max(np.abs(st[c])) for c in "uvw") < 1e-5
+    np.testing.assert_allclose(st["temp"], 0.5 * (1 - 0.001), rtol=1e-6)
+    np.testing.assert_allclose(st["dens"], 0.25 * (1 - 0.001), rtol=1e-6)
+
+
+def test_smoke_hot_blob_rises():
+    n, h = (16, 16, 16), (1.0, 1.0, 1.0)
+    st = smoke_state(n)
+    st["temp"] = O.source_sphere3(st["temp"], n, h, (0, 0, 0), (8.0, 4.0, 8.0), 2.5, 1.0)
+    y = np.repeat(np.arange(16), 16)[None, :].repeat(16, 0).ravel() + 0.5
+    cy0 = np.sum(st["temp"] * y) / np.sum(st["temp"])
+    for _ in range(4):
+        assert O.smoke3_step(st, n, h, (0, 0, 0), 0.25, b=5.0, levels=2)["converged"]
+    cy1 = np.sum(st["temp"] * y) / np.sum(st["temp"])
+    V = st["v"].reshape(16, 17, 16)
+    assert cy1 > cy0 + 0.05            # the centre of heat moves up
+    assert V[8, 5, 8] > 0.1            # and the velocity above the source points up

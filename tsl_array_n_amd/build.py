@@ -28,6 +28,7 @@ EXTRA = {
     "flux_transfer.hip": ["-ffp-contract=off"],
     "flux_flip.hip": ["-ffp-contract=off"],
     "flux_flip_solver.hip": ["-ffp-contract=off"],
+    "flux_smoke.hip": ["-ffp-contract=off"],
 }
 
 

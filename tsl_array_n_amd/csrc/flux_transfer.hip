@@ -18,34 +18,6 @@ namespace flux {
 
 static constexpr int XTPB = 256;
 
-// backTrace without a collider (advection_solver3.js:20-62)
-__device__ __forceinline__ void back_trace(const G3& g, const Mac& m, const float start[3], float dt, float hmin,
-                                           int max_sub, float direction, float out[3]) {
-    float pt[3] = {start[0], start[1], start[2]};
-    float rem = dt;
-    for (int s = 0; s < max_sub; ++s) {
-        if (rem <= 1e-6f) break;
-        float v0[3];
-        sample_mac(g, m, pt, v0);
-        float len = sqrtf((v0[0] * v0[0] + v0[1] * v0[1]) + v0[2] * v0[2]);
-        float ns = ceilf((len * rem) / hmin);
-        if (ns < 1.0f) ns = 1.0f;
-        float sub = rem / ns;
-        float half = sub * 0.5f;
-        float mid[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) mid[a] = pt[a] - (v0[a] * half) * direction;
-        float mv[3];
-        sample_mac(g, m, mid, mv);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) pt[a] = pt[a] - (mv[a] * sub) * direction;
-        rem = rem - sub;
-    }
-    out[0] = pt[0];
-    out[1] = pt[1];
-    out[2] = pt[2];
-}
-
 // ------------------------------------------------------------------ binning
 // cellIndexOf (grid_flip_solver3.js:227-235) with an outside counter
 __global__ void k_cell_index(G3 g, const float* __restrict__ pos, int count, int* __restrict__ cell,
@@ -232,50 +204,6 @@ __global__ void k_advect_particles(G3 g, Mac m, float* __restrict__ pos, int cou
     }
 }
 
-// monotonicCubic1d3 (grid_math3.js:137-157)
-__device__ __forceinline__ float mono_cubic(float f0, float f1, float f2, float f3, float f) {
-    float D1 = f2 - f1;
-    float rawD1 = (f2 - f0) * 0.5f;
-    float rawD2 = (f3 - f1) * 0.5f;
-    bool flat = fabsf(D1) < 1e-12f;
-    float d1 = (flat || rawD1 * D1 < 0.0f) ? 0.0f : rawD1;
-    float d2 = (flat || rawD2 * D1 < 0.0f) ? 0.0f : rawD2;
-    float a3 = (d1 + d2) - D1 * 2.0f;
-    float a2 = (D1 * 3.0f - d1 * 2.0f) - d2;
-    return ((((a3 * f) * f) * f + (a2 * f) * f) + d1 * f) + f1;
-}
-
-// collocatedCubicValueAtPosition3 (grid_math3.js:183-207)
-__device__ __forceinline__ float sample_cubic(const float* __restrict__ d, const int s[3], const float o[3],
-                                              const float h[3], const float p[3]) {
-    int xi[4], yi[4], zi[4];
-    float f[3];
-    int* idx[3] = {xi, yi, zi};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float coord = (p[a] - o[a]) / h[a];
-        int i0 = (int)floorf(coord);
-        f[a] = coord - (float)i0;
-        idx[a][0] = clampi(i0 - 1, 0, s[a] - 1);
-        idx[a][1] = clampi(i0, 0, s[a] - 1);
-        idx[a][2] = clampi(i0 + 1, 0, s[a] - 1);
-        idx[a][3] = clampi(i0 + 2, 0, s[a] - 1);
-    }
-    float zv[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        float rv[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            int base = s[0] * (yi[b] + s[1] * zi[c]);
-            rv[b] = mono_cubic(d[xi[0] + base], d[xi[1] + base], d[xi[2] + base], d[xi[3] + base], f[0]);
-        }
-        zv[c] = mono_cubic(rv[0], rv[1], rv[2], rv[3], f[1]);
-    }
-    return mono_cubic(zv[0], zv[1], zv[2], zv[3], f[2]);
-}
-
-// advectFaceCentered3, order 1, no collider (advection_solver3.js:130-176)
 __global__ void k_advect_faces(G3 g, Mac m, int comp, const float* __restrict__ src, float* __restrict__ dst, float dt,
                                int max_sub) {
     const int s[3] = {g.n0 + (comp == 0), g.n1 + (comp == 1), g.n2 + (comp == 2)};
