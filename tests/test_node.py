@@ -35,7 +35,7 @@ def test_api_surface_and_host_helpers(tmp_path):
                  "createFaceCenteredGrid3", "computeFlipBoxSeed3", "createJitterRandom", "binParticles",
                  "particleToGrid", "gridToParticle", "advectParticles", "advectFaceCentered3",
                  "createExtrapolateToRegion3", "createSDFStaticCollider3", "sphere", "box",
-                 "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3"]:
+                 "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3", "createGridSmokeSolver3"]:
         assert name in d["exports"], name
     assert d["strides"] == [1, 4, 12]  # TA/test/array.test.js:41-93
     assert "1D/2D/3D" in d["shapeError"]
@@ -121,3 +121,26 @@ def test_flip_and_collider_through_node_match_oracle(tmp_path):
     for a, c in enumerate("uvw"):
         np.testing.assert_allclose(np.fromfile(tmp_path / ("vout_" + c + ".bin"), np.float32), want[a], rtol=1e-6,
                                    atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_smoke_through_node_matches_oracle(tmp_path):
+    run_node("smoke_step.mjs", tmp_path, timeout=300)
+    meta = json.load(open(tmp_path / "meta.json"))
+    n, h = (24, 24, 24), (1.0, 1.0, 1.0)
+    cells = 24 ** 3
+    st = {"dens": np.zeros(cells, np.float32), "temp": np.zeros(cells, np.float32),
+          "pressure": np.zeros(cells, np.float32), "p_accepted": np.zeros(cells, np.float32)}
+    for a, c in enumerate("uvw"):
+        st[c] = np.zeros(int(np.prod(O.face_shape(n, a))), np.float32)
+    for s in range(3):
+        st["dens"] = O.source_sphere3(st["dens"], n, h, (0, 0, 0), (12, 2.5, 12), 3.0, 1.0)
+        st["temp"] = O.source_sphere3(st["temp"], n, h, (0, 0, 0), (12, 2.5, 12), 3.0, 1.0)
+        d = O.smoke3_step(st, n, h, (0, 0, 0), 0.25, a=-0.05, b=0.1, levels=3)
+        assert d["converged"] and meta["diags"][s]["converged"]
+    vel = np.concatenate([st[c] for c in "uvw"])
+    got = np.concatenate([np.fromfile(tmp_path / (c + ".bin"), np.float32) for c in "uvw"])
+    assert np.linalg.norm(got - vel) <= 1e-4 * np.linalg.norm(vel)
+    for name, key in (("density", "dens"), ("temperature", "temp")):
+        g = np.fromfile(tmp_path / (name + ".bin"), np.float32)
+        assert np.linalg.norm(g - st[key]) <= 1e-4 * np.linalg.norm(st[key])

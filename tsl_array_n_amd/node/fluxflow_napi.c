@@ -32,7 +32,7 @@
         }                                                     \
     } while (0)
 
-enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP };
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE };
 
 typedef struct {
     int kind;
@@ -53,6 +53,7 @@ static void fin_handle(napi_env env, void* data, void* hint) {
         case K_EXTRAP: flux_extrapolator3_destroy((flux_extrapolator*)h->ptr); break;
         case K_BND: flux_boundary3_destroy((flux_boundary3*)h->ptr); break;
         case K_FLIP: flux_flip3_destroy((flux_flip3*)h->ptr); break;
+        case K_SMOKE: flux_smoke3_destroy((flux_smoke3*)h->ptr); break;
         default: break;
     }
     free(h);
@@ -823,6 +824,74 @@ static napi_value js_flip3_field(napi_env env, napi_callback_info info) {
     return wrap(env, K_FIELD_VIEW, f);
 }
 
+/* createGridSmokeSolver3 options object (grid_smoke_solver3.js:25-41) */
+static napi_value js_smoke3_create(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    napi_value o = a[0], v;
+    flux_smoke3_desc d;
+    flux_smoke3_desc_default(&d);
+    int tmp[3];
+    double dv[3];
+    if ((v = prop(env, o, "resolution")) && get_int_array(env, v, tmp, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.resolution[i] = tmp[i];
+    if ((v = prop(env, o, "gridSpacing")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.spacing[i] = dv[i];
+    if ((v = prop(env, o, "origin")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.origin[i] = dv[i];
+    if ((v = prop(env, o, "up")) && get_double_array(env, v, dv, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.up[i] = (float)dv[i];
+    if ((v = prop(env, o, "dt"))) d.dt = (float)get_double(env, v, 0);
+    if ((v = prop(env, o, "buoyancySmokeDensityFactor")))
+        d.buoyancy_smoke_density_factor = (float)get_double(env, v, d.buoyancy_smoke_density_factor);
+    if ((v = prop(env, o, "buoyancyTemperatureFactor")))
+        d.buoyancy_temperature_factor = (float)get_double(env, v, d.buoyancy_temperature_factor);
+    if ((v = prop(env, o, "ambientTemperature"))) d.ambient_temperature = (float)get_double(env, v, 0);
+    if ((v = prop(env, o, "smokeDecay"))) d.smoke_decay = (float)get_double(env, v, d.smoke_decay);
+    if ((v = prop(env, o, "temperatureDecay"))) d.temperature_decay = (float)get_double(env, v, d.temperature_decay);
+    if ((v = prop(env, o, "closedDomainBoundaryFlag")) && !is_nullish(env, v))
+        d.closed_domain_flags = get_int(env, v, d.closed_domain_flags);
+    if ((v = prop(env, o, "multigrid"))) mg_opts(env, v, &d.pressure.multigrid);
+    if ((v = prop(env, o, "preconditionerKind"))) d.pressure.preconditioner = get_int(env, v, d.pressure.preconditioner);
+    if ((v = prop(env, o, "tolerance"))) d.pressure.tolerance = get_double(env, v, d.pressure.tolerance);
+    if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.pressure.max_iterations = get_int(env, v, 0);
+    flux_smoke3* s = NULL;
+    CHECK_FLUX(flux_smoke3_create(&d, &s));
+    return wrap(env, K_SMOKE, s);
+}
+
+static napi_value js_smoke3_step(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    flux_pressure_diag dg;
+    CHECK_FLUX(flux_smoke3_step((flux_smoke3*)unwrap(env, a[0], K_SMOKE, K_SMOKE), &dg));
+    return diag_object(env, &dg);
+}
+
+static napi_value js_smoke3_field(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = flux_smoke3_field((flux_smoke3*)unwrap(env, a[0], K_SMOKE, K_SMOKE), get_int(env, a[1], 0));
+    if (!f) {
+        napi_value n;
+        napi_get_null(env, &n);
+        return n;
+    }
+    return wrap(env, K_FIELD_VIEW, f);
+}
+
+static napi_value js_smoke3_source(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    double c[3] = {0, 0, 0};
+    get_double_array(env, a[1], c, 3);
+    float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
+    CHECK_FLUX(flux_smoke3_source_sphere((flux_smoke3*)unwrap(env, a[0], K_SMOKE, K_SMOKE), cf,
+                                         (float)get_double(env, a[2], 1.0), (float)get_double(env, a[3], 1.0),
+                                         (float)get_double(env, a[4], 1.0)));
+    return undef(env);
+}
+
 #define EXPORT(name, fn)                                                   \
     do {                                                                   \
         napi_value f_;                                                     \
@@ -868,6 +937,10 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("flip3Create", js_flip3_create);
     EXPORT("flip3Step", js_flip3_step);
     EXPORT("flip3Field", js_flip3_field);
+    EXPORT("smoke3Create", js_smoke3_create);
+    EXPORT("smoke3Step", js_smoke3_step);
+    EXPORT("smoke3Field", js_smoke3_field);
+    EXPORT("smoke3Source", js_smoke3_source);
     return exports;
 }
 

@@ -638,3 +638,58 @@ export function createGridFlipSolver3( {
 	};
 
 }
+
+// createGridSmokeSolver3 (grid_smoke_solver3.js:25-154), closed walls, no
+// collider/inflow/outflow objects.  The solver owns its velocity and scalar
+// fields: velocityGrid.dataU/V/W, density.current, temperature.current.
+// sourceSphere(center, radius, density, temperature) re-injects a ball with
+// max, as examples/34-smoke-plume-3d/main.js:146-158 does.
+export function createGridSmokeSolver3( {
+	velocityGrid, gridSpacing, origin, dt, closedDomainBoundaryFlag,
+	buoyancySmokeDensityFactor = - 0.000625, buoyancyTemperatureFactor = 5.0, ambientTemperature = 0,
+	smokeDecay = 0.001, temperatureDecay = 0.001, up = [ 0, 1, 0 ], collider, inflows, outflows, pressure = {}
+} = {} ) {
+
+	if ( ! velocityGrid ) throw new Error( 'createGridSmokeSolver3: options.velocityGrid is required.' );
+	if ( collider || inflows || outflows ) throw new Error( 'createGridSmokeSolver3: collider, inflow and outflow objects are not supported by this build.' );
+	const resolution = velocityGrid.resolution.slice();
+	const spacing = gridSpacing || velocityGrid.gridSpacing;
+	const org = origin || velocityGrid.origin;
+	const pre = pressure.preconditioner || 'multigrid';
+	if ( ! Object.prototype.hasOwnProperty.call( PRECONDITIONERS, pre ) ) {
+
+		throw new Error( `grid_pressure_solver3: unknown preconditioner '${ pre }' (expected ${ Object.keys( PRECONDITIONERS ).join( ', ' ) }).` );
+
+	}
+
+	const handle = addon.smoke3Create( {
+		resolution, gridSpacing: spacing, origin: org, dt, closedDomainBoundaryFlag, up,
+		buoyancySmokeDensityFactor, buoyancyTemperatureFactor, ambientTemperature, smokeDecay, temperatureDecay,
+		multigrid: pressure.multigrid || {}, preconditionerKind: PRECONDITIONERS[ pre ],
+		tolerance: pressure.tolerance === undefined ? 1e-6 : pressure.tolerance,
+		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations
+	} );
+	const [ nx, ny, nz ] = resolution;
+	const view = ( which, shape ) => wrapField( addon.smoke3Field( handle, which ), 'float', shape );
+	const grid = {
+		resolution, gridSpacing: spacing, origin: org,
+		dataSizeU: [ nx + 1, ny, nz ], dataSizeV: [ nx, ny + 1, nz ], dataSizeW: [ nx, ny, nz + 1 ],
+		dataU: view( 0, [ nx + 1, ny, nz ] ), dataV: view( 1, [ nx, ny + 1, nz ] ), dataW: view( 2, [ nx, ny, nz + 1 ] )
+	};
+	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
+	async function onAdvanceTimeStep() {
+
+		const d = addon.smoke3Step( handle );
+		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
+		return diagnostics;
+
+	}
+
+	return {
+		onAdvanceTimeStep, velocityGrid: grid, diagnostics,
+		density: { get current() { return view( 3, resolution ); } },
+		temperature: { get current() { return view( 4, resolution ); } },
+		sourceSphere: ( center, radius, density = 1, temperature = 1 ) => addon.smoke3Source( handle, center, radius, density, temperature )
+	};
+
+}
