@@ -88,11 +88,31 @@ def measured_traffic(kernel, storage):
     return None if best is None else best[1]
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, levels, gpu_iters, iters, threads):
     """The oracle's restatement of the same projection (f32 vectors, f64
     solution accumulator: the HIP path's arithmetic) on the host cores:
     setup plus `iters` PCG iterations are timed and scaled to a full
-    projection at the GPU's iteration count."""
+    projection at the GPU's iteration count; run on `threads` OpenMP
+    threads and on one."""
+    one = cpu_sample(sc, levels, gpu_iters, 1, 1)  # single-threaded: a shorter sample
+    res = cpu_sample(sc, levels, gpu_iters, iters, threads)
+    res["single_thread_value"] = one["value"]
+    res["cpu_model"] = cpu_model()
+    res["nproc"] = os.cpu_count()
+    return res
+
+
+def cpu_sample(sc, levels, gpu_iters, iters, threads):
     from oracle import pyoracle as O
     O.set_threads(threads)
     n, h = sc["shape"], sc["spacing"]

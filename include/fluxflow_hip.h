@@ -161,6 +161,12 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
  * residual b - A x in FP64 arithmetic, dots are FP64, all scalars stay on
  * the GPU; the host reads one scalar block per chunk of iterations. */
 int flux_pcg_solve(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* state);
+/* No counterpart in the reference (support for an external initial guess,
+ * such as the learned one): x <- x + alpha g on the free cells, with
+ * alpha = g.(b - A x) / g.(A g) in FP64, the step along g that minimises the
+ * energy-norm error, so a poor direction is scaled toward zero instead of
+ * adding error.  alpha (0 when g.(A g) is not negative) is returned. */
+int flux_pcg_scaled_start(flux_pcg* s, const flux_field* g, double* alpha);
 int flux_pcg_destroy(flux_pcg* s);
 
 /* ---------------------------------------------------- pressure projection */
@@ -208,6 +214,9 @@ int flux_pressure3_project(flux_pressure3* p, const flux_field* u_in, const flux
 int flux_pressure3_assemble(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in);
 int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
                          flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag);
+/* flux_pcg_scaled_start on the assembled system (between assemble and
+ * solve); no counterpart in the reference. */
+int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, double* alpha);
 /* .pressure.data, .b, .dirichletMask, .ventMask (grid_pressure_solver3.js:572) */
 flux_field* flux_pressure3_field(flux_pressure3* p, int which);
 enum { FLUX_P3_PRESSURE = 0, FLUX_P3_RHS = 1, FLUX_P3_MASK = 2, FLUX_P3_VENT = 3, FLUX_P3_TARGET = 4,

@@ -88,3 +88,45 @@ def test_learned_guess_is_accepted_at_the_same_tolerance(flux, L):
             out.append(d["iterations"])
         results.append(out)
     print("iterations cold / learned:", results)
+
+
+def test_scaled_start_along_exact_and_random_directions(flux):
+    """pressure += alpha g with alpha = g.r / g.Ag: along the converged
+    solution alpha is 1 and the solve needs (almost) no iterations; along
+    random noise alpha is ~0 and the solve costs what the cold start costs."""
+    F = flux
+    from tsl_array_n_amd import scenes
+    n = 32
+    sc = scenes.random_scene(n, 303)
+    shape = sc["shape"]
+    beta = [F.array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+    act = F.array_n("u8", shape, sc["active"])
+    tgt = F.array_n("float", shape, sc["target"])
+
+    def solver():
+        return F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                              face_weights=beta, multigrid={"levels": 4}, tolerance=1e-6)
+
+    def vel():
+        return [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+
+    cold = solver()
+    d0 = cold.project(vel())
+    assert d0["converged"]
+    exact = F.array_n("float", shape, cold.pressure.to_array())
+    s1 = solver()
+    v1 = vel()
+    s1.assemble(v1)
+    alpha = s1.scaled_guess(exact)
+    assert abs(alpha - 1.0) < 1e-3, alpha
+    d1 = s1.solve(v1)
+    assert d1["converged"] and d1["iterations"] <= 3, d1
+    noise = F.array_n("float", shape, np.random.default_rng(9).standard_normal(int(np.prod(shape))).astype(np.float32))
+    s2 = solver()
+    v2 = vel()
+    s2.assemble(v2)
+    alpha = s2.scaled_guess(noise)
+    d2 = s2.solve(v2)
+    assert d2["converged"] and d2["residual"] <= 1e-6 * d2["bNorm"]
+    assert d2["iterations"] <= d0["iterations"] + 2, (d0, d2, alpha)
+    print("cold", d0["iterations"], "exact", d1["iterations"], "noise", d2["iterations"], "alpha", alpha)

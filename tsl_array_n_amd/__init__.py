@@ -388,6 +388,14 @@ class GridPressureSolver3:
         """first half of project(): Dirichlet fields, weights, right-hand side, coarse levels"""
         _check(_lib.flux_pressure3_assemble(C.c_void_p(self.h), *[C.c_void_p(f.h) for f in vin]), "assemble")
 
+    def scaled_guess(self, guess):
+        """between assemble() and solve(): pressure += alpha * guess on the free
+        cells, alpha minimising the energy-norm error along the guess; returns alpha"""
+        alpha = C.c_double()
+        _check(_lib.flux_pressure3_scaled_guess(C.c_void_p(self.h), C.c_void_p(guess.h), C.byref(alpha)),
+               "scaledGuess")
+        return alpha.value
+
     def solve(self, vin, vout=None):
         """second half of project(): PCG from the current pressure, breaker, correction"""
         vout = vin if vout is None else vout

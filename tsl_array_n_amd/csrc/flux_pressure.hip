@@ -353,6 +353,12 @@ int flux_pressure3_assemble(flux_pressure3* p, const flux_field* u_in, const flu
     return FLUX_OK;
 }
 
+int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, double* alpha) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p && guess, "scaledGuess: NULL argument");
+    return flux_pcg_scaled_start(p->pcg, guess, alpha);
+}
+
 int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
                          flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
     FLUX_REQUIRE_INIT();

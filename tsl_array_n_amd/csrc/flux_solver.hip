@@ -4,6 +4,7 @@
 // and linalg.js (PCG).  All arithmetic of the V-cycle and of the search
 // directions is FP32; the solution accumulator and the true residual are
 // FP64, and every dot product is reduced in FP64 in a fixed order.
+#include <vector>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -124,6 +125,35 @@ __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, 
     store_block_partial(v, partials);
 }
 
+// Terms of a Galerkin-scaled start along a direction g (free cells only):
+// partials[blk] = g.(b - A x), partials[nb + blk] = g.(A g), in FP64.
+__global__ void k_galerkin_terms(LevelDev L, const float* __restrict__ x, const float* __restrict__ b,
+                                 const float* __restrict__ g, double* partials) {
+    double gr = 0, gag = 0;
+    const int n = L.n0 * L.n1 * L.n2;
+    const uint8_t* __restrict__ m = L.mask;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_coords(L, c, i, j, k);
+        const double gc = (m && m[c]) ? 0.0 : (double)g[c];
+        const double ax = stencil_row<double>(L, i, j, k, c, [&](int q) { return (double)x[q]; }, (double*)nullptr);
+        const double ag = stencil_row<double>(
+            L, i, j, k, c, [&](int q) { return (m && m[q]) ? 0.0 : (double)g[q]; }, (double*)nullptr);
+        gr += gc * ((double)b[c] - ax);
+        gag += gc * ag;
+    }
+    store_block_partial(gr, partials);
+    store_block_partial(gag, partials + gridDim.x);
+}
+
+// x += alpha g on the free cells, rounded to FP32 once.
+__global__ void k_galerkin_step(const uint8_t* __restrict__ m, const float* __restrict__ g, float* __restrict__ x,
+                                double alpha, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || (m && m[c])) return;
+    x[c] = (float)((double)x[c] + alpha * (double)g[c]);
+}
+
 __global__ void __launch_bounds__(FIN_TPB) k_fin_setup(Scal* S, const double* partials, unsigned nb) {
     const double tot = sum_partials(partials, nb);
     if (threadIdx.x != 0) return;
@@ -194,11 +224,11 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* parti
     S->rz_old = tot;
 }
 
-// Iteration step 1: p = z + beta p (updateP, fused); Ap = A p; partials of
-// pAp.  Column-march shape: one cell per thread and plane.
+// Iteration step 1: p = z + beta p (updateP, fused); partials of p.Ap.  Ap
+// itself is not stored: the update recomputes r = b - A x from x.  Column-march shape: one cell per thread and plane.
 __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const float* __restrict__ z,
-                                                    float* __restrict__ pa, float* __restrict__ pb,
-                                                    float* __restrict__ Ap, const Scal* S, double* partials) {
+                                                    float* __restrict__ pa, float* __restrict__ pb, const Scal* S,
+                                                    double* partials) {
     if (S->stop) return;
     const float* __restrict__ pold = S->psel ? pb : pa;
     float* __restrict__ pnew = S->psel ? pa : pb;
@@ -229,7 +259,6 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
             row_terms<float>(pc, xn, w, ih2, sum, dsum);
             const float ap = (m && m[c]) ? -pc : sum;
             pnew[c] = pc;
-            Ap[c] = ap;
             v += (double)pc * (double)ap;
         }
     }
@@ -451,7 +480,7 @@ struct flux_pcg {
     flux_field* b = nullptr;
     flux_field* x = nullptr;
     int64_t n = 0;
-    float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *Ap = nullptr;
+    float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr;
     double *xa = nullptr, *xb = nullptr;
     Scal* S = nullptr;
     Scal* Sh = nullptr;  // pinned host mirror
@@ -483,7 +512,7 @@ static int pcg_precondition(flux_pcg* s, DotOut* dot) {
 static int pcg_iteration(flux_pcg* s) {
     unsigned g = red_blocks(s->n);
     kprof_begin("apply");
-    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->z, s->pa, s->pb, s->Ap, s->S, s->partials);
+    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->z, s->pa, s->pb, s->S, s->partials);
     kprof_end("apply");
     k_fin_pap<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     kprof_begin("update");
@@ -582,7 +611,6 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     s->z = (float*)dmalloc(cells * 4);
     s->pa = (float*)dmalloc(cells * 4);
     s->pb = (float*)dmalloc(cells * 4);
-    s->Ap = (float*)dmalloc(cells * 4);
     s->xa = (double*)dmalloc(cells * 8);
     s->xb = (double*)dmalloc(cells * 8);
     s->S = (Scal*)dmalloc(sizeof(Scal));
@@ -595,7 +623,7 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     void* hp = nullptr;
     if (hipHostMalloc(&hp, sizeof(Scal), hipHostMallocDefault) != hipSuccess) hp = nullptr;
     s->Sh = (Scal*)hp;
-    if (!s->r || !s->z || !s->pa || !s->pb || !s->Ap || !s->xa || !s->xb || !s->S || !s->partials || !s->Sh) {
+    if (!s->r || !s->z || !s->pa || !s->pb || !s->xa || !s->xb || !s->S || !s->partials || !s->Sh) {
         flux_pcg_destroy(s);
         return set_error(FLUX_ERR_NOMEM, "conjugateGradient: allocation failed");
     }
@@ -603,6 +631,35 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     FLUX_HIP(hipMemsetAsync(s->S, 0, sizeof(Scal), g_stream));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     *out = s;
+    return FLUX_OK;
+}
+
+int flux_pcg_scaled_start(flux_pcg* s, const flux_field* g, double* alpha_out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(s && g, "scaledStart: NULL argument");
+    FLUX_TRY(check_field(g, FLUX_F32, s->n, "scaledStart: direction"));
+    const unsigned nb = red_blocks(s->n);
+    k_galerkin_terms<<<nb, TPB, 0, g_stream>>>(s->op->L, (const float*)s->x->d, (const float*)s->b->d,
+                                               (const float*)g->d, s->partials);
+    FLUX_LAUNCH_CHECK();
+    std::vector<double> h(2 * (size_t)nb);
+    FLUX_HIP(hipMemcpyAsync(h.data(), s->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, g_stream));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    double gr = 0, gag = 0;
+    for (unsigned q = 0; q < nb; ++q) {
+        gr += h[q];
+        gag += h[nb + q];
+    }
+    // A is negative definite on the free cells: gag < 0 for any g != 0
+    double alpha = (gag < 0 && std::isfinite(gr) && std::isfinite(gag)) ? gr / gag : 0.0;
+    if (!std::isfinite(alpha)) alpha = 0.0;
+    if (alpha != 0.0) {
+        k_galerkin_step<<<blocks_for(s->n, TPB), TPB, 0, g_stream>>>(s->op->L.mask, (const float*)g->d,
+                                                                     (float*)s->x->d, alpha, (int)s->n);
+        FLUX_LAUNCH_CHECK();
+        FLUX_HIP(hipStreamSynchronize(g_stream));
+    }
+    if (alpha_out) *alpha_out = alpha;
     return FLUX_OK;
 }
 
@@ -620,7 +677,6 @@ int flux_pcg_destroy(flux_pcg* s) {
     dfree(s->z);
     dfree(s->pa);
     dfree(s->pb);
-    dfree(s->Ap);
     dfree(s->xa);
     dfree(s->xb);
     dfree(s->S);

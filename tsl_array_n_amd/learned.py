@@ -2,9 +2,11 @@
 
 Input channels (built on the GPU from an assembled system): b / rms(b), the
 Dirichlet mask, log(mean face weight) / log(density ratio).  Target: the
-pressure divided by rms(b).  The guess handed to the solver is the network
-output times rms(b); the solver then runs its usual PCG from it, so a learned
-solve is accepted only at the same FP64 true-residual tolerance.
+solution's shape, the pressure on the free cells scaled to unit RMS.  The
+solver adds the network output along its energy-optimal step
+(pressure += alpha g, alpha = g.(b - A x) / g.(A g)), which sets the
+amplitude, then runs its usual PCG, so a learned solve is accepted only at
+the same FP64 true-residual tolerance.
 """
 import ctypes as C
 
@@ -110,22 +112,34 @@ class UNet:
 
 class LearnedProjector:
     """project() with the network's guess as the PCG start:
-    assemble -> features -> guess into the free cells -> solve.  The solve is
-    the plain solver's, so it is accepted only at the same FP64 true-residual
-    tolerance; a bad guess costs iterations, never correctness."""
+    assemble -> features -> guess -> solve.  The solve is the plain solver's,
+    so it is accepted only at the same FP64 true-residual tolerance; a bad
+    guess costs iterations, never correctness.
 
-    def __init__(self, solver, net, density_ratio):
+    scaled=True (default) adds the guess to the assembled start along its
+    energy-optimal step (pressure += alpha * guess, alpha = g.r / g.Ag), so a
+    guess pointing the wrong way is scaled toward zero; scaled=False writes
+    the raw guess into the free cells."""
+
+    def __init__(self, solver, net, density_ratio, scaled=True):
         self.solver = solver
         self.net = net
         self.density_ratio = float(density_ratio)
+        self.scaled = scaled
+        self.alpha = None
+        self._g = array_n("float", solver.resolution) if scaled else None
 
     def project(self, vin, vout=None):
         s = self.solver
         s.assemble(vin)
         self.net.features(s.b, s.dirichlet_mask, s.weights, self.density_ratio)
-        _check(_lib.flux_unet_guess_into(C.c_void_p(self.net.h), C.c_void_p(self.net.input.h),
-                                         C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
-                                         C.c_void_p(s.pressure.h)), "guess")
+        if self.scaled:
+            self.net.guess(self._g)
+            self.alpha = s.scaled_guess(self._g)
+        else:
+            _check(_lib.flux_unet_guess_into(C.c_void_p(self.net.h), C.c_void_p(self.net.input.h),
+                                             C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
+                                             C.c_void_p(s.pressure.h)), "guess")
         return s.solve(vin, vout)
 
 
@@ -158,9 +172,14 @@ def training_pairs(F, n, count, seed=0, levels=None):
             continue
         net.features(ps.b, ps.dirichlet_mask, ps.weights, ratio)
         x = array_n("float", (3 * n ** 3,), net.input.to_array())
-        b = ps.b.to_array().astype(np.float64)
-        rms = np.sqrt((b * b).mean())
-        t = array_n("float", (n ** 3,), (ps.pressure.to_array() / rms).astype(np.float32))
+        # the target is the solution's shape (unit RMS over the free cells);
+        # the amplitude is set at solve time by the energy-optimal step
+        free = ps.dirichlet_mask.to_array() == 0 if ps.dirichlet_mask else np.ones(n ** 3, bool)
+        p = np.where(free, ps.pressure.to_array().astype(np.float64), 0.0)
+        rms = np.sqrt((p * p).sum() / max(int(free.sum()), 1))
+        if not rms > 0:
+            continue
+        t = array_n("float", (n ** 3,), (p / rms).astype(np.float32))
         pairs.append((x, t))
     return pairs
 
