@@ -325,6 +325,132 @@ int flux_boundary3_set_closed(flux_boundary3* b, int flags);
 int flux_boundary3_constrain(flux_boundary3* b, int extrapolation_depth);
 int flux_boundary3_destroy(flux_boundary3* b);
 
+/* ------------------------------------------------------ 2-D Eulerian step */
+/* a 2-D MAC grid: U [n0+1, n1] at (o.x, o.y + h.y/2), V [n0, n1+1] at
+ * (o.x + h.x/2, o.y), cell data at o + h/2 (createFaceCenteredGrid2,
+ * grid_data2.js:161-198) */
+typedef struct {
+    int n[2];
+    float h[2];
+    float origin[2];
+} flux_grid2;
+/* createSDFStaticCollider2 / createSDFInflow2 / createSDFOutflow2 .addPolygons
+ * (sdf_collider2.js:82-100, polygon_sdf.js:24-97): sdf (FLUX_F32 cells) =
+ * the signed distance (negative inside, even-odd) to the union of polygons,
+ * evaluated in FP64 at each cell centre; ring r spans the (x, y) points
+ * ring_start[r] .. ring_start[r+1]-1 of xy (host arrays) */
+int flux_polygon_sdf2(flux_field* sdf, const flux_grid2* g, const double* xy, const int* ring_start, int rings,
+                      int invert);
+/* advectFaceCentered2 order 1 (advection_solver2.js:81-127, 263-320): the
+ * midpoint back-trace stops at the collider (collider_sdf may be NULL); a
+ * face inside the collider keeps the output's value.  Output must not alias
+ * the input. */
+int flux_advect_faces2(const flux_grid2* g, const flux_field* u, const flux_field* v, const flux_field* collider_sdf,
+                       float dt, int max_substeps, flux_field* u_out, flux_field* v_out);
+/* createMaxAbsReducer(fields).read() (reduction.js:43-77): max |x| */
+int flux_max_abs(const flux_field* const* fields, int count, float* out);
+/* computeAdaptiveSubSteps (time/cfl.js:38-48): steps = min(max_sub,
+ * max(1, ceil(max_velocity * frame_dt / min_spacing / courant))),
+ * sub_dt = frame_dt / steps */
+int flux_adaptive_substeps(double max_velocity, double min_spacing, double frame_dt, double courant, int max_sub,
+                           int* steps, double* sub_dt);
+/* createGridBlockedBoundaryConditionSolver2(velocity, ..., collider, inflows)
+ * (grid_blocked_boundary_condition_solver2.js:98-597): bound to the two
+ * velocity fields; a static collider given by its SDF (may be NULL) with a
+ * friction coefficient; inflows (mode 0 'set', 1 'add') are SDF fields with
+ * a velocity.  constrain = markAndProject, extrapolation by `depth` Jacobi
+ * passes, no-flux projection with friction, blocked faces, closed walls
+ * (DIRECTION_* bits: left 1, right 2, down 4, up 8; default 15), inflows,
+ * the +-100 clamp. */
+typedef struct flux_boundary2 flux_boundary2;
+int flux_boundary2_create(const flux_grid2* g, flux_field* const* vel, const flux_field* collider_sdf, float friction,
+                          flux_boundary2** out);
+int flux_boundary2_set_closed(flux_boundary2* b, int flags);
+int flux_boundary2_add_inflow(flux_boundary2* b, const flux_field* sdf, float vx, float vy, int mode);
+int flux_boundary2_constrain(flux_boundary2* b, int extrapolation_depth);
+int flux_boundary2_destroy(flux_boundary2* b);
+/* createGridOutflowSolver2 (grid_outflow_solver2.js:217-396): apply =
+ * applyOutflowVelocityBC (faces inside the outflow relax toward the
+ * velocity one cell upstream by the outward Courant number, U before V);
+ * dirichlet = createOutflowPressureDirichlet2 + combineDirichlet
+ * (sdf_inflow_outflow2.js:152-199) at the cell centres: active (u8) =
+ * inside any outflow or the caller's mask, target = the caller's target
+ * where the caller's mask is set, else 0 */
+typedef struct flux_outflow2 flux_outflow2;
+int flux_outflow2_create(const flux_grid2* g, const flux_field* outflow_sdf, flux_outflow2** out);
+int flux_outflow2_apply(flux_outflow2* o, flux_field* u, flux_field* v, float dt);
+int flux_outflow2_destroy(flux_outflow2* o);
+int flux_outflow2_dirichlet(const flux_grid2* g, const flux_field* const* outflow_sdf, int count,
+                            const flux_field* user_active, const flux_field* user_target, flux_field* active,
+                            flux_field* target);
+/* createGridPressureSolver2 (grid_pressure_solver2.js:223-981): the 2-D
+ * projection -- no collider weights; optional face weights; the same PCG,
+ * multigrid, guards and circuit breaker as the 3-D solver; the iteration
+ * cap is ceil(factor * max(n) * 4) (398) */
+typedef struct {
+    int resolution[2];
+    double spacing[2];
+    double origin[2];
+    const flux_field* dirichlet_active;   /* u8 cells, or NULL */
+    const flux_field* dirichlet_target;   /* f32 cells, or NULL */
+    const flux_field* face_weights[2];    /* f32 U and V faces, or NULL */
+    flux_mg_options multigrid;
+    int preconditioner;
+    double tolerance;
+    int max_iterations;
+    double max_iterations_factor;
+    int relative_tolerance;
+    double max_plausible_pressure;
+} flux_pressure2_desc;
+void flux_pressure2_desc_default(flux_pressure2_desc* d);
+typedef struct flux_pressure2 flux_pressure2;
+int flux_pressure2_create(const flux_pressure2_desc* d, flux_pressure2** out);
+/* project(input, output) + dispatch() (663-954); output may alias input */
+int flux_pressure2_project(flux_pressure2* p, const flux_field* u_in, const flux_field* v_in, flux_field* u_out,
+                           flux_field* v_out, flux_pressure_diag* diag);
+flux_field* flux_pressure2_field(flux_pressure2* p, int which);
+enum { FLUX_P2_PRESSURE = 0, FLUX_P2_RHS = 1 };
+int flux_pressure2_destroy(flux_pressure2* p);
+/* createGridSolver2 (grid_solver2.js:131-255) without external forces:
+ * onAdvanceTimeStep(dt) = project in place, constrain; clone, advect the
+ * clone into the velocity (the collider forwarded to the advection only
+ * when advect_collider is set, as the reference's options.advection),
+ * the outflow velocity condition, constrain.  The solver owns its velocity
+ * fields (zero at creation; fill them through flux_solver2_field). */
+typedef struct {
+    flux_grid2 grid;
+    const flux_field* collider_sdf;       /* NULL: no collider */
+    float friction;                       /* default 0.5 */
+    int advect_collider;                  /* default 0 */
+    int max_substeps;                     /* advection, default 32 */
+    int n_inflows;
+    const flux_field* inflow_sdf[4];
+    float inflow_velocity[4][2];
+    int inflow_mode[4];                   /* 0 'set', 1 'add' */
+    int n_outflows;
+    const flux_field* outflow_sdf[4];
+    int outflow_velocity_bc;              /* default 1 */
+    const flux_field* dirichlet_active;   /* the caller's own Dirichlet cells, or NULL */
+    const flux_field* dirichlet_target;
+    int closed_flags;                     /* default 15 */
+    int extrapolation_depth;              /* default 5 */
+    flux_mg_options multigrid;
+    int preconditioner;
+    double tolerance;
+    int max_iterations;
+    double max_iterations_factor;
+    int relative_tolerance;
+    double max_plausible_pressure;
+} flux_solver2_desc;
+void flux_solver2_desc_default(flux_solver2_desc* d);
+typedef struct flux_solver2 flux_solver2;
+int flux_solver2_create(const flux_solver2_desc* d, flux_solver2** out);
+int flux_solver2_step(flux_solver2* s, float dt, flux_pressure_diag* diag);
+flux_field* flux_solver2_field(flux_solver2* s, int which);
+enum { FLUX_S2_U = 0, FLUX_S2_V = 1, FLUX_S2_U_PREV = 2, FLUX_S2_V_PREV = 3, FLUX_S2_PRESSURE = 4,
+       FLUX_S2_DIRICHLET = 5 };
+int flux_solver2_destroy(flux_solver2* s);
+
 /* createGridFlipSolver3 (grid_flip_solver3.js:135-1155): free-surface FLIP
  * with reduced pressure (air cells Dirichlet at -dt g.x), int32 fixed-point
  * P2G, weight and fluid-adjacent extrapolation, constrainVelocity with an

@@ -243,4 +243,38 @@ void orc_buoyancy3(float* u, float* v, float* w, const float* dens, const float*
 void orc_source_sphere3(float* d, const int n[3], const float h[3], const float origin[3], const float c[3], float r,
                         float value);
 
+/* ---------------- 2-D Eulerian step (oracle_grid2.c) ---------------- */
+/* polygonsSignedDistance: polygon_sdf.js:24-97 (rings of (x, y) pairs;
+ * ring r spans points ring_start[r] .. ring_start[r+1]-1) */
+double orc_polygons_distance2(const double* xy, const int* ring_start, int rings, double x, double y);
+/* addPolygons: sdf_collider2.js:82-100 */
+void orc_polygon_sdf2(const double* xy, const int* ring_start, int rings, int invert, const int n[2],
+                      const double h[2], const double origin[2], float* sdf);
+/* collocatedValueAtPosition2 / collocatedCubicValueAtPosition2: grid_math.js:35-76, 102-167 */
+float orc_sample_bilinear2(const float* d, const int s[2], const float o[2], const float h[2], float px, float py);
+float orc_sample_cubic2(const float* d, const int s[2], const float o[2], const float h[2], float px, float py);
+/* advectFaceCentered2, order 1, with an optional collider SDF:
+ * advection_solver2.js:81-127, 263-320 (uo/vo hold the pre-advection copy) */
+void orc_advect_face2(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
+                      const float* sdf, float dt, int max_sub, float* uo, float* vo);
+/* constrainVelocity: grid_blocked_boundary_condition_solver2.js:75-575 */
+void orc_constrain_velocity2(float* u, float* v, const int n[2], const float h[2], const float origin[2],
+                             int closed_flags, const float* sdf, float friction, int depth, int n_inflows,
+                             const float* const* inflow_sdf, const float* inflow_vel, const int* inflow_mode);
+/* applyOutflowVelocityBC: grid_outflow_solver2.js:245-365 */
+void orc_outflow_bc2(float* u, float* v, const int n[2], const float h[2], const float origin[2], float dt,
+                     const float* osdf);
+/* createGridPressureSolver2 build and correction: grid_pressure_solver2.js:677-834 */
+void orc_pressure2_assemble(const int n[2], const float h[2], const float* u, const float* v, const uint8_t* active,
+                            const float* target, float* b, float* p);
+void orc_pressure2_correct(const int n[2], const float h[2], const float* p, float* u, float* v);
+/* createOutflowPressureDirichlet2 at the cell centres: sdf_inflow_outflow2.js:152-166 */
+void orc_outflow_dirichlet2(const float* osdf, const int n[2], const float h[2], const float origin[2],
+                            uint8_t* active);
+/* createMaxAbsReducer: reduction.js:43-77 */
+float orc_max_abs(const float* const* arrays, const int64_t* counts, int narrays);
+/* computeAdaptiveSubSteps: time/cfl.js:38-48 */
+int orc_adaptive_substeps(double max_velocity, double min_spacing, double frame_dt, double courant, int max_sub,
+                          double* sub_dt);
+
 #endif

@@ -720,3 +720,169 @@ def smoke3_step(st, n, h, origin, dt, a=-0.000625, b=5.0, tamb=0.0, up=(0.0, 1.0
     temp = (raw * np.float32(np.float32(1.0) - np.float32(temperature_decay))).astype(np.float32)
     st.update(u=vel[0], v=vel[1], w=vel[2], dens=dens, temp=temp, pressure=x)
     return diag
+
+
+# ---------------------------------------------------------------- 2-D Eulerian step
+def _i2(v):
+    return (C.c_int * 2)(*[int(x) for x in v])
+
+
+def _f2(v):
+    return (C.c_float * 2)(*[float(x) for x in v])
+
+
+def _d2(v):
+    return (C.c_double * 2)(*[float(x) for x in v])
+
+
+def _rings(rings):
+    starts = [0]
+    pts = []
+    for r in rings:
+        for x, y in r:
+            pts += [float(x), float(y)]
+        starts.append(len(pts) // 2)
+    xy = np.array(pts, np.float64)
+    st = np.array(starts, np.int32)
+    return xy, st, len(rings)
+
+
+def face_shape2(n, axis):
+    return (n[0] + 1, n[1]) if axis == 0 else (n[0], n[1] + 1)
+
+
+def polygons_distance2(rings, x, y):
+    xy, st, nr = _rings(rings)
+    lib().orc_polygons_distance2.restype = C.c_double
+    return lib().orc_polygons_distance2(_p(xy), _p(st), nr, C.c_double(x), C.c_double(y))
+
+
+def polygon_sdf2(rings, n, h, origin, invert=False):
+    xy, st, nr = _rings(rings)
+    out = np.zeros(int(n[0] * n[1]), np.float32)
+    lib().orc_polygon_sdf2(_p(xy), _p(st), nr, int(bool(invert)), _i2(n), _d2(h), _d2(origin), _p(out))
+    return out
+
+
+def sample_bilinear2(d, shape, data_origin, h, x, y):
+    d = np.ascontiguousarray(d, np.float32)
+    lib().orc_sample_bilinear2.restype = C.c_float
+    return lib().orc_sample_bilinear2(_p(d), _i2(shape), _f2(data_origin), _f2(h), C.c_float(x), C.c_float(y))
+
+
+def sample_cubic2(d, shape, data_origin, h, x, y):
+    d = np.ascontiguousarray(d, np.float32)
+    lib().orc_sample_cubic2.restype = C.c_float
+    return lib().orc_sample_cubic2(_p(d), _i2(shape), _f2(data_origin), _f2(h), C.c_float(x), C.c_float(y))
+
+
+def advect_face2(u, v, n, h, origin, dt, sdf=None, max_substeps=32):
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    uo, vo = u.copy(), v.copy()
+    sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
+    lib().orc_advect_face2(_p(u), _p(v), _i2(n), _f2(h), _f2(origin), _p(sdf), C.c_float(dt), int(max_substeps),
+                           _p(uo), _p(vo))
+    return [uo, vo]
+
+
+def constrain_velocity2(vel, n, h, origin, closed_flags=15, sdf=None, friction=0.5, depth=5, inflows=()):
+    """inflows: sequence of (sdf, (vx, vy), mode) with mode 'set' or 'add'"""
+    u, v = [np.ascontiguousarray(a, np.float32).copy() for a in vel]
+    sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
+    keep = [np.ascontiguousarray(q[0], np.float32) for q in inflows]
+    arr = (C.c_void_p * max(len(keep), 1))(*[a.ctypes.data for a in keep])
+    vels = np.array([c for q in inflows for c in q[1]] or [0.0], np.float32)
+    modes = np.array([1 if q[2] == "add" else 0 for q in inflows] or [0], np.int32)
+    lib().orc_constrain_velocity2(_p(u), _p(v), _i2(n), _f2(h), _f2(origin), int(closed_flags), _p(sdf),
+                                  C.c_float(friction), int(depth), len(keep), arr, _p(vels), _p(modes))
+    return [u, v]
+
+
+def outflow_bc2(vel, n, h, origin, dt, osdf):
+    u, v = [np.ascontiguousarray(a, np.float32).copy() for a in vel]
+    osdf = np.ascontiguousarray(osdf, np.float32)
+    lib().orc_outflow_bc2(_p(u), _p(v), _i2(n), _f2(h), _f2(origin), C.c_float(dt), _p(osdf))
+    return [u, v]
+
+
+def pressure2_assemble(n, h, vel, active=None, target=None, pressure=None):
+    cells = int(n[0] * n[1])
+    u, v = [np.ascontiguousarray(a, np.float32) for a in vel]
+    active = _u8(active)
+    target = np.zeros(cells, np.float32) if target is None else np.ascontiguousarray(target, np.float32)
+    b = np.zeros(cells, np.float32)
+    p = np.zeros(cells, np.float32) if pressure is None else np.ascontiguousarray(pressure, np.float32).copy()
+    lib().orc_pressure2_assemble(_i2(n), _f2(h), _p(u), _p(v), _p(active), _p(target), _p(b), _p(p))
+    return {"b": b, "pressure": p, "mask": active}
+
+
+def pressure2_correct(n, h, p, vel):
+    u, v = [np.ascontiguousarray(a, np.float32).copy() for a in vel]
+    p = np.ascontiguousarray(p, np.float32)
+    lib().orc_pressure2_correct(_i2(n), _f2(h), _p(p), _p(u), _p(v))
+    return [u, v]
+
+
+def outflow_dirichlet2(osdf, n, h, origin):
+    osdf = np.ascontiguousarray(osdf, np.float32)
+    active = np.zeros(int(n[0] * n[1]), np.uint8)
+    lib().orc_outflow_dirichlet2(_p(osdf), _i2(n), _f2(h), _f2(origin), _p(active))
+    return active
+
+
+def max_abs(arrays):
+    keep = [np.ascontiguousarray(a, np.float32) for a in arrays]
+    ptrs = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
+    cnt = (C.c_int64 * len(keep))(*[a.size for a in keep])
+    lib().orc_max_abs.restype = C.c_float
+    return lib().orc_max_abs(ptrs, cnt, len(keep))
+
+
+def adaptive_substeps(max_velocity, min_spacing, frame_dt, courant=5.0, max_sub=32):
+    sub = C.c_double()
+    k = lib().orc_adaptive_substeps(C.c_double(max_velocity), C.c_double(min_spacing), C.c_double(frame_dt),
+                                    C.c_double(courant), int(max_sub), C.byref(sub))
+    return k, sub.value
+
+
+def solver2_step(st, n, h, origin, dt, sdf=None, inflows=(), outflow_sdf=None, flags=15, levels=1, nu=(2, 2),
+                 n_coarsest=20, tol=1e-6, max_plausible=1e6, advect_collider=True, max_substeps=32, depth=5):
+    """One createGridSolver2 step (grid_solver2.js:235-255, no external
+    force): projection, constrain, advection of a copy, outflow velocity
+    condition, constrain.  st holds u, v, pressure and the last accepted
+    pressure; the outflow SDF (cells inside it are Dirichlet p = 0,
+    sdf_inflow_outflow2.js:152-166) may be None."""
+    vel = [st["u"], st["v"]]
+    active = None if outflow_sdf is None else outflow_dirichlet2(outflow_sdf, n, h, origin)
+    A = pressure2_assemble(n, h, vel, active=active, pressure=st["pressure"])
+    mg = Multigrid(n, h, levels=levels, nu_down=nu[0], nu_up=nu[1], n_coarsest=n_coarsest, mask=A["mask"],
+                   suffix="f32")
+    x, _, diag = pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], mg=mg, tol=tol,
+                                 maxiter=int(np.ceil(1.5 * max(n) * 4)))
+    if count_bad(x, max_plausible, suffix="f32") > 0:
+        x = st["p_accepted"].copy()
+        diag["rejected"] = True
+    else:
+        st["p_accepted"] = x.copy()
+        diag["rejected"] = False
+    vel = pressure2_correct(n, h, x, vel)
+    vel = constrain_velocity2(vel, n, h, origin, flags, sdf, depth=depth, inflows=inflows)
+    vel = advect_face2(vel[0], vel[1], n, h, origin, dt, sdf if advect_collider else None, max_substeps)
+    if outflow_sdf is not None:
+        vel = outflow_bc2(vel, n, h, origin, dt, outflow_sdf)
+    vel = constrain_velocity2(vel, n, h, origin, flags, sdf, depth=depth, inflows=inflows)
+    st.update(u=vel[0], v=vel[1], pressure=x)
+    return diag
+
+
+def solver2_frame(st, n, h, origin, frame_dt, courant=5.0, max_sub=32, **kw):
+    """createGridAdaptiveTimeStep2.update() (grid_adaptive_timestep2.js) and
+    the frame's sub-steps: max |u| over both face arrays, the sub-step count
+    from it, then that many steps of sub_dt (rounded to f32 like the
+    reference's dt field)."""
+    m = max_abs([st["u"], st["v"]])
+    k, sub = adaptive_substeps(float(m), min(h), frame_dt, courant, max_sub)
+    dt = float(np.float32(sub))
+    diags = [solver2_step(st, n, h, origin, dt, **kw) for _ in range(k)]
+    return k, dt, diags

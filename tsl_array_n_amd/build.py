@@ -29,6 +29,7 @@ EXTRA = {
     "flux_flip.hip": ["-ffp-contract=off"],
     "flux_flip_solver.hip": ["-ffp-contract=off"],
     "flux_smoke.hip": ["-ffp-contract=off"],
+    "flux_grid2.hip": ["-ffp-contract=off"],
 }
 
 

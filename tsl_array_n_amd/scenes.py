@@ -16,6 +16,8 @@ The 3-D two-phase dam break at n^3 on the unit domain (h = 1/n):
     (pressurePin 'topCenter', grid_two_phase_flip_solver2.js:133-169, 770-796).
 All float arithmetic is f32, in the order of the reference expressions.
 """
+import math
+
 import numpy as np
 
 
@@ -118,3 +120,52 @@ def random_scene(n, seed):
     active[(n // 2) + n * ((n - 1) + n * (n // 2))] = 1
     return {"shape": shape, "spacing": (1.0 / n,) * 3, "fraction": frac.ravel(), "beta": beta, "vel": vel,
             "active": active, "target": np.zeros(n ** 3, np.float32), "ratio": ratio}
+
+
+def mulberry32(seed, count):
+    """createJitterRandom (grid/jitter_random.js:29-40): `count` doubles in [0, 1)"""
+    out = np.empty(count, np.float64)
+    state = seed & 0xFFFFFFFF
+    m = 0xFFFFFFFF
+    for q in range(count):
+        state = (state + 0x6D2B79F5) & m
+        t = state
+        t = ((t ^ (t >> 15)) * (t | 1)) & m
+        t ^= (t + (((t ^ (t >> 7)) * (t | 61)) & m)) & m
+        out[q] = ((t ^ (t >> 14)) & m) / 4294967296.0
+    return out
+
+
+def circle_polygon(cx, cy, radius, segments=48):
+    """makeCirclePolygon (examples/16-karman-vortex-street/main.js:217-229)"""
+    return [(cx + radius * math.cos((i / segments) * math.pi * 2), cy + radius * math.sin((i / segments) * math.pi * 2))
+            for i in range(segments)]
+
+
+def wall_strip_polygon(inner_x, outer_x, ny, margin=1000.0):
+    """makeWallStripPolygon (examples/16-karman-vortex-street/main.js:236-240)"""
+    return [(inner_x, -margin), (outer_x, -margin), (outer_x, ny + margin), (inner_x, ny + margin)]
+
+
+def karman2d(nx=256, ny=128, radius=12.0, center=(64.0, 64.0), segments=48, inflow_speed=1.0, noise=1e-3,
+             seed=0x9E3779B9, frame_dt=1.0, courant=0.5, levels=4, tolerance=1e-6):
+    """The 2-D Karman vortex street, plain method: a 48-gon cylinder collider,
+    an inflow strip (x < 2, u set to inflow_speed), an outflow slab
+    (x > nx - 2, Dirichlet p = 0 and the convective velocity condition),
+    closed top and bottom (free slip), u = inflow_speed + U(-noise, noise)
+    per U face from mulberry32(seed) in index order, v = 0; h = 1; MGPCG
+    with `levels` levels and 2/2 red-black sweeps to `tolerance`; each frame
+    of frame_dt splits into sub-steps of Courant number <= courant
+    (scene layout after examples/16-karman-vortex-street/main.js:189-400)."""
+    r = mulberry32(seed, (nx + 1) * ny)
+    u0 = (inflow_speed + (r * 2.0 - 1.0) * noise).astype(np.float32)
+    return {
+        "resolution": (nx, ny), "spacing": (1.0, 1.0), "origin": (0.0, 0.0),
+        "collider": [circle_polygon(center[0], center[1], radius, segments)],
+        "inflow": [wall_strip_polygon(2.0, -1000.0, ny)], "inflow_velocity": (inflow_speed, 0.0),
+        "outflow": [wall_strip_polygon(nx - 2.0, nx + 1000.0, ny)],
+        "closed_flags": 4 | 8, "advect_collider": True,
+        "u0": u0, "v0": np.zeros(nx * (ny + 1), np.float32),
+        "frame_dt": frame_dt, "courant": courant, "max_sub_steps": 32,
+        "multigrid": {"levels": levels, "nu_down": 2, "nu_up": 2, "n_coarsest": 20}, "tolerance": tolerance,
+    }
