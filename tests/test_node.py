@@ -10,6 +10,7 @@ import pytest
 
 from conftest import GOLDEN, ROOT
 from oracle import pyoracle as O
+from tsl_array_n_amd import scenes
 
 NODE = shutil.which("node")
 ADDON = os.path.join(ROOT, "tsl_array_n_amd", "lib", "fluxflow_napi.node")
@@ -35,7 +36,10 @@ def test_api_surface_and_host_helpers(tmp_path):
                  "createFaceCenteredGrid3", "computeFlipBoxSeed3", "createJitterRandom", "binParticles",
                  "particleToGrid", "gridToParticle", "advectParticles", "advectFaceCentered3",
                  "createExtrapolateToRegion3", "createSDFStaticCollider3", "sphere", "box",
-                 "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3", "createGridSmokeSolver3"]:
+                 "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3", "createGridSmokeSolver3",
+                 "createFaceCenteredGrid2", "createSDFStaticCollider2", "createSDFInflow2", "createSDFOutflow2",
+                 "createGridSolver2", "createGridAdaptiveTimeStep2", "computeAdaptiveSubSteps",
+                 "createMaxAbsReducer"]:
         assert name in d["exports"], name
     assert d["strides"] == [1, 4, 12]  # TA/test/array.test.js:41-93
     assert "1D/2D/3D" in d["shapeError"]
@@ -144,3 +148,28 @@ def test_smoke_through_node_matches_oracle(tmp_path):
     for name, key in (("density", "dens"), ("temperature", "temp")):
         g = np.fromfile(tmp_path / (name + ".bin"), np.float32)
         assert np.linalg.norm(g - st[key]) <= 1e-4 * np.linalg.norm(st[key])
+
+
+@pytest.mark.gpu
+def test_karman2d_through_node_matches_oracle(tmp_path):
+    """createGridSolver2 + createGridAdaptiveTimeStep2 driven from Node over
+    two frames of a reduced Karman scene, against the composed oracle fed
+    the same SDFs and initial velocity."""
+    run_node("karman_step.mjs", tmp_path, timeout=300)
+    meta = json.load(open(tmp_path / "meta.json"))
+    n, h, o = (64, 32), (1.0, 1.0), (0.0, 0.0)
+    rd = lambda name: np.fromfile(tmp_path / (name + ".bin"), np.float32)
+    col, inflow, outflow = rd("collider"), rd("inflow"), rd("outflow")
+    np.testing.assert_array_max_ulp(col, O.polygon_sdf2([scenes.circle_polygon(16, 16, 4, 48)], n, h, o), maxulp=1)
+    np.testing.assert_array_equal(rd("u0"), scenes.karman2d(nx=64, ny=32)["u0"])
+    cells = n[0] * n[1]
+    st = {"u": rd("u0"), "v": np.zeros(n[0] * (n[1] + 1), np.float32), "pressure": np.zeros(cells, np.float32),
+          "p_accepted": np.zeros(cells, np.float32)}
+    for f in range(2):
+        k, dt, diags = O.solver2_frame(st, n, h, o, 1.0, courant=0.5, sdf=col, inflows=[(inflow, (1.0, 0.0), "set")],
+                                       outflow_sdf=outflow, flags=12, levels=3, tol=1e-6)
+        assert k == meta["frames"][f]["substeps"] and all(d["converged"] for d in diags)
+        assert all(d["converged"] for d in meta["frames"][f]["diags"])
+    vel = np.concatenate([rd("u"), rd("v")])
+    want = np.concatenate([st["u"], st["v"]])
+    assert np.linalg.norm(vel - want) <= 2e-5 * np.linalg.norm(want)

@@ -32,7 +32,7 @@
         }                                                     \
     } while (0)
 
-enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE };
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE, K_S2 };
 
 typedef struct {
     int kind;
@@ -54,6 +54,7 @@ static void fin_handle(napi_env env, void* data, void* hint) {
         case K_BND: flux_boundary3_destroy((flux_boundary3*)h->ptr); break;
         case K_FLIP: flux_flip3_destroy((flux_flip3*)h->ptr); break;
         case K_SMOKE: flux_smoke3_destroy((flux_smoke3*)h->ptr); break;
+        case K_S2: flux_solver2_destroy((flux_solver2*)h->ptr); break;
         default: break;
     }
     free(h);
@@ -130,10 +131,10 @@ static int get_double_array(napi_env env, napi_value arr, double* out, int max) 
 }
 
 /* array of up to 3 fields, or null -> returns count, fills out */
-static int get_field_array(napi_env env, napi_value arr, const flux_field** out) {
+static int get_field_array_n(napi_env env, napi_value arr, const flux_field** out, uint32_t max) {
     if (is_nullish(env, arr)) return 0;
     uint32_t n = 0;
-    if (napi_get_array_length(env, arr, &n) != napi_ok || n > 3) return -1;
+    if (napi_get_array_length(env, arr, &n) != napi_ok || n > max) return -1;
     for (uint32_t i = 0; i < n; ++i) {
         napi_value e;
         napi_get_element(env, arr, i, &e);
@@ -141,6 +142,10 @@ static int get_field_array(napi_env env, napi_value arr, const flux_field** out)
         if (!out[i]) return -1;
     }
     return (int)n;
+}
+
+static int get_field_array(napi_env env, napi_value arr, const flux_field** out) {
+    return get_field_array_n(env, arr, out, 3);
 }
 
 static napi_value args_n(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
@@ -892,6 +897,184 @@ static napi_value js_smoke3_source(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+/* ---------------------------------------------------------------- 2-D */
+static int get_grid2(napi_env env, napi_value o, flux_grid2* g) {
+    napi_value v;
+    int n[2];
+    double d[2];
+    if (!(v = prop(env, o, "resolution")) || get_int_array(env, v, n, 2) != 2) return -1;
+    g->n[0] = n[0];
+    g->n[1] = n[1];
+    g->h[0] = g->h[1] = 1.0f;
+    g->origin[0] = g->origin[1] = 0.0f;
+    if ((v = prop(env, o, "gridSpacing")) && get_double_array(env, v, d, 2) == 2) {
+        g->h[0] = (float)d[0];
+        g->h[1] = (float)d[1];
+    }
+    if ((v = prop(env, o, "origin")) && get_double_array(env, v, d, 2) == 2) {
+        g->origin[0] = (float)d[0];
+        g->origin[1] = (float)d[1];
+    }
+    return 0;
+}
+
+/* addPolygons(field, grid, polygons = [[[x, y], ...], ...], invert) */
+static napi_value js_polygon_sdf2(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_grid2 g;
+    uint32_t rings = 0;
+    if (get_grid2(env, a[1], &g) || napi_get_array_length(env, a[2], &rings) != napi_ok || rings == 0) {
+        napi_throw_type_error(env, NULL, "addPolygons: bad arguments");
+        return NULL;
+    }
+    int* starts = (int*)calloc(rings + 1, sizeof(int));
+    uint32_t total = 0;
+    for (uint32_t r = 0; r < rings; ++r) {
+        napi_value ring;
+        uint32_t len = 0;
+        napi_get_element(env, a[2], r, &ring);
+        napi_get_array_length(env, ring, &len);
+        total += len;
+        starts[r + 1] = (int)total;
+    }
+    double* xy = (double*)calloc(2 * (size_t)(total > 0 ? total : 1), sizeof(double));
+    for (uint32_t r = 0; r < rings; ++r) {
+        napi_value ring;
+        napi_get_element(env, a[2], r, &ring);
+        for (int q = starts[r]; q < starts[r + 1]; ++q) {
+            napi_value pt;
+            napi_get_element(env, ring, (uint32_t)(q - starts[r]), &pt);
+            get_double_array(env, pt, xy + 2 * q, 2);
+        }
+    }
+    bool inv = false;
+    napi_get_value_bool(env, a[3], &inv);
+    int rc = flux_polygon_sdf2(get_field(env, a[0]), &g, xy, starts, (int)rings, inv ? 1 : 0);
+    free(xy);
+    free(starts);
+    CHECK_FLUX(rc);
+    return undef(env);
+}
+
+/* createMaxAbsReducer(fields).read() */
+static napi_value js_max_abs(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    const flux_field* f[8];
+    int n = get_field_array_n(env, a[0], f, 8);
+    if (n <= 0 || n > 8) {
+        napi_throw_type_error(env, NULL, "createMaxAbsReducer: expected 1 to 8 fields");
+        return NULL;
+    }
+    float m = 0.0f;
+    CHECK_FLUX(flux_max_abs(f, n, &m));
+    napi_value out;
+    napi_create_double(env, (double)m, &out);
+    return out;
+}
+
+/* computeAdaptiveSubSteps(maxVelocity, minGridSpacing, frameDt, courantNumber, maxSubSteps) */
+static napi_value js_adaptive_substeps(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    int steps = 1;
+    double sub = 0.0;
+    CHECK_FLUX(flux_adaptive_substeps(get_double(env, a[0], 0.0), get_double(env, a[1], 1.0), get_double(env, a[2], 0.0),
+                                      get_double(env, a[3], 5.0), get_int(env, a[4], 32), &steps, &sub));
+    napi_value out, v;
+    napi_create_object(env, &out);
+    napi_create_int32(env, steps, &v);
+    napi_set_named_property(env, out, "numSubSteps", v);
+    napi_create_double(env, sub, &v);
+    napi_set_named_property(env, out, "subDt", v);
+    return out;
+}
+
+/* createGridSolver2 options (grid_solver2.js:131-159) */
+static napi_value js_solver2_create(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    napi_value o = a[0], v;
+    flux_solver2_desc d;
+    flux_solver2_desc_default(&d);
+    if (get_grid2(env, o, &d.grid)) {
+        napi_throw_type_error(env, NULL, "createGridSolver2: resolution must be [nx, ny]");
+        return NULL;
+    }
+    if ((v = prop(env, o, "colliderSdf")) && !is_nullish(env, v)) d.collider_sdf = get_field(env, v);
+    if ((v = prop(env, o, "friction"))) d.friction = (float)get_double(env, v, d.friction);
+    if ((v = prop(env, o, "advectCollider"))) {
+        bool b = false;
+        napi_get_value_bool(env, v, &b);
+        d.advect_collider = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "maxSubsteps"))) d.max_substeps = get_int(env, v, d.max_substeps);
+    if ((v = prop(env, o, "inflows")) && !is_nullish(env, v)) {
+        uint32_t n = 0;
+        napi_get_array_length(env, v, &n);
+        if (n > 4) {
+            napi_throw_error(env, NULL, "createGridSolver2: at most 4 inflows");
+            return NULL;
+        }
+        d.n_inflows = (int)n;
+        for (uint32_t q = 0; q < n; ++q) {
+            napi_value e, w;
+            double vel[2] = {0.0, 0.0};
+            napi_get_element(env, v, q, &e);
+            if ((w = prop(env, e, "sdf"))) d.inflow_sdf[q] = get_field(env, w);
+            if ((w = prop(env, e, "velocity"))) get_double_array(env, w, vel, 2);
+            d.inflow_velocity[q][0] = (float)vel[0];
+            d.inflow_velocity[q][1] = (float)vel[1];
+            if ((w = prop(env, e, "mode"))) d.inflow_mode[q] = get_int(env, w, 0);
+        }
+    }
+    if ((v = prop(env, o, "outflows")) && !is_nullish(env, v)) {
+        int n = get_field_array_n(env, v, d.outflow_sdf, 4);
+        if (n < 0) {
+            napi_throw_error(env, NULL, "createGridSolver2: at most 4 outflows");
+            return NULL;
+        }
+        d.n_outflows = n;
+    }
+    if ((v = prop(env, o, "outflowVelocityBC"))) {
+        bool b = true;
+        napi_get_value_bool(env, v, &b);
+        d.outflow_velocity_bc = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "dirichletActive")) && !is_nullish(env, v)) d.dirichlet_active = get_field(env, v);
+    if ((v = prop(env, o, "dirichletTarget")) && !is_nullish(env, v)) d.dirichlet_target = get_field(env, v);
+    if ((v = prop(env, o, "closedDomainBoundaryFlag")) && !is_nullish(env, v))
+        d.closed_flags = get_int(env, v, d.closed_flags);
+    if ((v = prop(env, o, "multigrid"))) mg_opts(env, v, &d.multigrid);
+    if ((v = prop(env, o, "preconditionerKind"))) d.preconditioner = get_int(env, v, d.preconditioner);
+    if ((v = prop(env, o, "tolerance"))) d.tolerance = get_double(env, v, d.tolerance);
+    if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.max_iterations = get_int(env, v, 0);
+    flux_solver2* s = NULL;
+    CHECK_FLUX(flux_solver2_create(&d, &s));
+    return wrap(env, K_S2, s);
+}
+
+static napi_value js_solver2_step(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_pressure_diag dg;
+    CHECK_FLUX(flux_solver2_step((flux_solver2*)unwrap(env, a[0], K_S2, K_S2), (float)get_double(env, a[1], 0.0), &dg));
+    return diag_object(env, &dg);
+}
+
+static napi_value js_solver2_field(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_field* f = flux_solver2_field((flux_solver2*)unwrap(env, a[0], K_S2, K_S2), get_int(env, a[1], 0));
+    if (!f) {
+        napi_value n;
+        napi_get_null(env, &n);
+        return n;
+    }
+    return wrap(env, K_FIELD_VIEW, f);
+}
+
 #define EXPORT(name, fn)                                                   \
     do {                                                                   \
         napi_value f_;                                                     \
@@ -941,6 +1124,12 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("smoke3Step", js_smoke3_step);
     EXPORT("smoke3Field", js_smoke3_field);
     EXPORT("smoke3Source", js_smoke3_source);
+    EXPORT("polygonSdf2", js_polygon_sdf2);
+    EXPORT("maxAbs", js_max_abs);
+    EXPORT("adaptiveSubSteps", js_adaptive_substeps);
+    EXPORT("solver2Create", js_solver2_create);
+    EXPORT("solver2Step", js_solver2_step);
+    EXPORT("solver2Field", js_solver2_field);
     return exports;
 }
 

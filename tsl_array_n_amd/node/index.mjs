@@ -693,3 +693,230 @@ export function createGridSmokeSolver3( {
 	};
 
 }
+
+// ------------------------------------------------------------------- 2-D
+export const DIRECTION_NONE = 0, DIRECTION_LEFT = 1, DIRECTION_RIGHT = 2, DIRECTION_DOWN = 4, DIRECTION_UP = 8;
+export const DIRECTION_ALL = 15;
+
+// createFaceCenteredGrid2 (grid_data2.js:161-198)
+export function createFaceCenteredGrid2( nx, ny, hx, hy, ox = 0, oy = 0 ) {
+
+	const dataSizeU = [ nx + 1, ny ];
+	const dataSizeV = [ nx, ny + 1 ];
+	const grid = {
+		resolution: [ nx, ny ], gridSpacing: [ hx, hy ], origin: [ ox, oy ], dataSizeU, dataSizeV,
+		dataOriginU: [ ox, oy + hy * 0.5 ], dataOriginV: [ ox + hx * 0.5, oy ],
+		dataU: arrayN( 'float', dataSizeU ), dataV: arrayN( 'float', dataSizeV )
+	};
+	grid.dataU.fill( 0 );
+	grid.dataV.fill( 0 );
+	grid.clear = () => {
+
+		grid.dataU.fill( 0 );
+		grid.dataV.fill( 0 );
+
+	};
+
+	return grid;
+
+}
+
+// createSDFStaticCollider2 (sdf_collider2.js:24-130): the SDF lives in a
+// cell-centred field (0 until polygons are added, as the reference's zeroed
+// grid); addPolygons rasterises it on the GPU in FP64.
+export function createSDFStaticCollider2( resolutionX, resolutionY, gridSpacingX, gridSpacingY, originX = 0, originY = 0 ) {
+
+	const resolution = [ resolutionX, resolutionY ];
+	const gridSpacing = [ gridSpacingX, gridSpacingY ];
+	const origin = [ originX, originY ];
+	const data = arrayN( 'float', resolution ).fill( 0 );
+	const collider = { grid: { resolution, gridSpacing, origin, data }, frictionCoefficient: 0.5 };
+	collider.addPolygons = ( polygons, { invert = false } = {} ) => {
+
+		addon.polygonSdf2( data.handle, { resolution, gridSpacing, origin }, polygons, invert );
+
+	};
+
+	collider.addPolygon = ( points, options ) => collider.addPolygons( [ points ], options );
+	return collider;
+
+}
+
+// createSDFInflow2 / createSDFOutflow2 (sdf_inflow_outflow2.js:102-120)
+export function createSDFInflow2( resolutionX, resolutionY, gridSpacingX, gridSpacingY, originX = 0, originY = 0, options = {} ) {
+
+	const { velocity = [ 0, 0 ], mode = 'set' } = options;
+	if ( mode !== 'set' && mode !== 'add' ) throw new Error( `createSDFInflow2: mode must be 'set' or 'add'.` );
+	return { ...createSDFStaticCollider2( resolutionX, resolutionY, gridSpacingX, gridSpacingY, originX, originY ), velocity, mode };
+
+}
+
+export function createSDFOutflow2( resolutionX, resolutionY, gridSpacingX, gridSpacingY, originX = 0, originY = 0 ) {
+
+	return createSDFStaticCollider2( resolutionX, resolutionY, gridSpacingX, gridSpacingY, originX, originY );
+
+}
+
+// computeAdaptiveSubSteps (time/cfl.js:38-48)
+export function computeAdaptiveSubSteps( maxVelocityMagnitude, minGridSpacing, frameDt, options = {} ) {
+
+	const c = options.courantNumber === undefined ? 5 : options.courantNumber;
+	const m = options.maxSubSteps === undefined ? 32 : options.maxSubSteps;
+	return addon.adaptiveSubSteps( maxVelocityMagnitude, minGridSpacing, frameDt, c, m );
+
+}
+
+// createMaxAbsReducer (reduction.js:43-77)
+export function createMaxAbsReducer( fields ) {
+
+	const list = Array.isArray( fields ) ? fields : [ fields ];
+	return { read: async () => addon.maxAbs( list.map( ( f ) => f.handle ) ) };
+
+}
+
+function asList( x ) {
+
+	return ! x ? [] : ( Array.isArray( x ) ? x : [ x ] );
+
+}
+
+// createGridSolver2 (grid_solver2.js:131-255) without external forces.  The
+// solver owns its velocity fields (velocityGrid.dataU/V are the solver's;
+// the grid passed supplies resolution, spacing and origin, and its current
+// contents are copied in).  dt may be a number or an array0('float') field
+// (a live dt, re-read every step, as createGridAdaptiveTimeStep2 writes it).
+export function createGridSolver2( {
+	velocityGrid, gridSpacing, origin, force, dirichlet, collider, inflows, outflows, outflowVelocityBC = true,
+	closedDomainBoundaryFlag, dt, advection = {}, pressure = {}
+} = {} ) {
+
+	if ( ! velocityGrid ) throw new Error( 'createGridSolver2: options.velocityGrid is required.' );
+	if ( force ) throw new Error( 'createGridSolver2: external forces are not supported by this build.' );
+	if ( advection.order !== undefined && advection.order !== 1 ) throw new Error( 'createGridSolver2: only order-1 advection is built.' );
+	if ( advection.collider && advection.collider !== collider ) throw new Error( 'createGridSolver2: advection.collider must be the solver\'s collider.' );
+	const resolution = velocityGrid.resolution.slice();
+	const spacing = gridSpacing || velocityGrid.gridSpacing;
+	const org = origin || velocityGrid.origin;
+	const pre = pressure.preconditioner || 'multigrid';
+	if ( ! Object.prototype.hasOwnProperty.call( PRECONDITIONERS, pre ) ) {
+
+		throw new Error( `grid_pressure_solver2: unknown preconditioner '${ pre }' (expected ${ Object.keys( PRECONDITIONERS ).join( ', ' ) }).` );
+
+	}
+
+	let dirichletActive = null, dirichletTarget = null;
+	if ( dirichlet ) {
+
+		if ( typeof dirichlet === 'function' ) {
+
+			// dirichlet(pos) at the cell centres (grid_pressure_solver2.js:403-417)
+			const [ nx, ny ] = resolution;
+			const act = new Uint8Array( nx * ny );
+			const tgt = new Float32Array( nx * ny );
+			for ( let j = 0; j < ny; j ++ ) for ( let i = 0; i < nx; i ++ ) {
+
+				const r = dirichlet( [ org[ 0 ] + ( i + 0.5 ) * spacing[ 0 ], org[ 1 ] + ( j + 0.5 ) * spacing[ 1 ] ] );
+				act[ i + nx * j ] = r.active ? 1 : 0;
+				tgt[ i + nx * j ] = r.target || 0;
+
+			}
+
+			dirichletActive = arrayN( 'u8', resolution ).fromArray( act );
+			dirichletTarget = arrayN( 'float', resolution ).fromArray( tgt );
+
+		} else {
+
+			dirichletActive = dirichlet.active;
+			dirichletTarget = dirichlet.target;
+
+		}
+
+	}
+
+	const handle = addon.solver2Create( {
+		resolution, gridSpacing: spacing, origin: org,
+		colliderSdf: collider ? collider.grid.data.handle : null,
+		friction: collider ? collider.frictionCoefficient : 0.5,
+		advectCollider: !! advection.collider, maxSubsteps: advection.maxSubsteps === undefined ? 32 : advection.maxSubsteps,
+		inflows: asList( inflows ).map( ( q ) => ( { sdf: q.grid.data.handle, velocity: q.velocity, mode: q.mode === 'add' ? 1 : 0 } ) ),
+		outflows: asList( outflows ).map( ( q ) => q.grid.data.handle ), outflowVelocityBC,
+		dirichletActive: dirichletActive ? dirichletActive.handle : null,
+		dirichletTarget: dirichletTarget ? dirichletTarget.handle : null,
+		closedDomainBoundaryFlag,
+		multigrid: pressure.multigrid || {}, preconditionerKind: PRECONDITIONERS[ pre ],
+		tolerance: pressure.tolerance === undefined ? 1e-6 : pressure.tolerance,
+		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations
+	} );
+	const keep = [ collider, inflows, outflows, dirichletActive, dirichletTarget ];
+	const [ nx, ny ] = resolution;
+	const view = ( which, shape ) => wrapField( addon.solver2Field( handle, which ), 'float', shape );
+	const grid = {
+		resolution, gridSpacing: spacing, origin: org, dataSizeU: [ nx + 1, ny ], dataSizeV: [ nx, ny + 1 ],
+		dataU: view( 0, [ nx + 1, ny ] ), dataV: view( 1, [ nx, ny + 1 ] )
+	};
+	if ( velocityGrid.dataU && velocityGrid.dataU.handle ) {
+
+		addon.fieldCopy( grid.dataU.handle, velocityGrid.dataU.handle ); // (destination, source)
+		addon.fieldCopy( grid.dataV.handle, velocityGrid.dataV.handle );
+
+	}
+
+	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
+	const readDt = () => {
+
+		if ( typeof dt === 'number' ) return dt;
+		if ( dt && dt.handle ) {
+
+			const out = new Float32Array( 1 );
+			addon.fieldDownload( dt.handle, out );
+			return out[ 0 ];
+
+		}
+
+		throw new Error( 'createGridSolver2: options.dt must be a number or an array0(\'float\') field.' );
+
+	};
+
+	async function onAdvanceTimeStep() {
+
+		const d = addon.solver2Step( handle, readDt() );
+		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
+		return diagnostics;
+
+	}
+
+	return {
+		onAdvanceTimeStep, velocityGrid: grid, diagnostics, keep,
+		get pressure() { return { data: view( 4, resolution ) }; }
+	};
+
+}
+
+// createGridAdaptiveTimeStep2 (grid_adaptive_timestep2.js:15-44): update()
+// reads max |u| over the face arrays, writes the sub-step dt into the live
+// dt field and returns the sub-step count.
+export function createGridAdaptiveTimeStep2( { velocityGrid, gridSpacing, dt, targetDt, courantNumber, maxSubSteps } ) {
+
+	if ( typeof dt === 'number' || ! dt || ! dt.handle ) {
+
+		throw new Error( 'createGridAdaptiveTimeStep2: options.dt must be a live array0(\'float\') field.' );
+
+	}
+
+	const reducer = createMaxAbsReducer( [ velocityGrid.dataU, velocityGrid.dataV ] );
+	const minGridSpacing = Math.min( ...gridSpacing );
+	const state = { lastNumSubSteps: 1, lastSubDt: targetDt };
+	async function update() {
+
+		const maxVel = await reducer.read();
+		const { numSubSteps, subDt } = computeAdaptiveSubSteps( maxVel, minGridSpacing, targetDt, { courantNumber, maxSubSteps } );
+		dt.fromArray( new Float32Array( [ subDt ] ) );
+		state.lastNumSubSteps = numSubSteps;
+		state.lastSubDt = subDt;
+		return numSubSteps;
+
+	}
+
+	return { update, state };
+
+}
