@@ -13,6 +13,8 @@ bool g_init = false;
 int g_device = 0;
 int g_cus = 0;
 unsigned g_march_blocks = 4096;
+bool g_march_flip = true;
+unsigned g_march_toggle = 0;
 
 static thread_local std::string t_err;
 
@@ -120,6 +122,8 @@ int flux_init(int device) {
         long v = strtol(e, nullptr, 10);
         if (v >= 64 && v <= (1 << 20)) g_march_blocks = (unsigned)v;
     }
+    g_march_flip = true;
+    if (const char* e = getenv("FLUX_MARCH_FLIP")) g_march_flip = strtol(e, nullptr, 10) != 0;
     g_device = device;
     g_init = true;
     return FLUX_OK;

@@ -29,6 +29,14 @@ extern int g_cus;
 // Blocks a column-march grid aims at (FLUX_MARCH_BLOCKS, default 4096:
 // 16 blocks of 256 threads per CU on 256 CUs).
 extern unsigned g_march_blocks;
+// Consecutive finest-level column-march launches alternate the direction in
+// which their blocks walk the z-chunks (FLUX_MARCH_FLIP=0 turns it off): a
+// launch starts where the previous one ended, whose lines are still in the
+// memory-side cache.  The toggle is reset at the start of every solve and
+// V-cycle apply so the block order, and with it every partial sum, is the
+// same from run to run.
+extern bool g_march_flip;
+extern unsigned g_march_toggle;
 
 int set_error(int code, const char* fmt, ...);
 void clear_error();

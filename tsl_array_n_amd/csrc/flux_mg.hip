@@ -715,18 +715,22 @@ void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
         for (int it = 0; it < iters; ++it) {
             int c0 = reversed ? 1 : 0, c1 = reversed ? 0 : 1;
             if (from_zero && it == 0) {
-                k_relax3<FINE, 1, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop, nullptr);
+                k_relax3<FINE, 1, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, FINE ? flip_march(M) : M, x, b, om, c0,
+                                                                          stop, nullptr);
             } else {
                 if (FINE) kprof_begin("relax");
-                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c0, stop, nullptr);
+                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, FINE ? flip_march(M) : M, x, b, om, c0,
+                                                                          stop, nullptr);
                 if (FINE) kprof_end("relax");
             }
             if (FINE) kprof_begin("relax");
             if (dot && it == iters - 1 && M.nb <= dot->capacity) {
-                k_relax3<FINE, 0, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop, dot->partials);
+                k_relax3<FINE, 0, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, FINE ? flip_march(M) : M, x, b, om, c1,
+                                                                          stop, dot->partials);
                 dot->blocks = M.nb;
             } else {
-                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, M, x, b, om, c1, stop, nullptr);
+                k_relax3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, FINE ? flip_march(M) : M, x, b, om, c1,
+                                                                          stop, nullptr);
             }
             if (FINE) kprof_end("relax");
         }
@@ -763,7 +767,8 @@ void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, cons
         // fewer, longer columns: consecutive coarse planes share two fine planes.
         // The residual (k_residual3) is already 0 on masked fine cells.
         const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, std::max(64u, g_march_blocks / 4));
-        k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(M, v.n[0], v.n[1], v.n[2], r, nullptr, c.n[0], c.n[1],
+        k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(M) : M, v.n[0], v.n[1], v.n[2], r,
+                                                                nullptr, c.n[0], c.n[1],
                                                                 c.n[2], (OT*)c.b, c.mask, scale, stop);
     } else {
         k_restrict<FINE, IT, OT><<<blocks_for(cells_of(c.n), MTPB), MTPB, 0, g_stream>>>(
@@ -777,7 +782,8 @@ void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const
     const float mult = (float)(c.xscale / v.xscale);  // stored coarse x to the fine level's storage scale
     if (mg->ndim == 3) {
         const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
-        k_correct3<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(M, c.n[0], c.n[1], c.n[2], (const CT*)c.x, v.n[0],
+        k_correct3<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(M) : M, c.n[0], c.n[1], c.n[2],
+                                                               (const CT*)c.x, v.n[0],
                                                                v.n[1], v.n[2], (FT*)v.x, v.mask, scale, mult, stop);
     } else {
         k_correct<FINE, CT, FT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(
@@ -823,7 +829,8 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
         else fill_t<float>(v.x, cells, 0.0f);
         kprof_begin("residual");
         const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
-        k_residual3<1, float, float, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), M, (const float*)v.x,
+        k_residual3<1, float, float, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), flip_march(M),
+                                                                      (const float*)v.x,
                                                                       (const float*)v.b, (float*)v.buf, stop);
         kprof_end("residual");
         kprof_begin("restrict");
@@ -1038,6 +1045,7 @@ int flux_multigrid_apply(flux_multigrid* mg, const flux_field* in, flux_field* o
     FLUX_TRY(check_field(in, FLUX_F32, cells, "multigrid: input"));
     FLUX_TRY(check_field(out, FLUX_F32, cells, "multigrid: output"));
     FLUX_REQUIRE(in->d != out->d, "multigrid: input and output must differ");
+    g_march_toggle = 0;
     FLUX_TRY(mg_apply_raw(mg, (const float*)in->d, (float*)out->d, nullptr, nullptr, nullptr));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     return FLUX_OK;

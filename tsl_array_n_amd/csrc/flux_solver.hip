@@ -512,11 +512,11 @@ static int pcg_precondition(flux_pcg* s, DotOut* dot) {
 static int pcg_iteration(flux_pcg* s) {
     unsigned g = red_blocks(s->n);
     kprof_begin("apply");
-    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->z, s->pa, s->pb, s->S, s->partials);
+    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->z, s->pa, s->pb, s->S, s->partials);
     kprof_end("apply");
     k_fin_pap<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     kprof_begin("update");
-    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d,
+    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d,
                                                  s->r, s->S, s->partials);
     kprof_end("update");
     k_fin_rr<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
@@ -538,6 +538,7 @@ static int pcg_iteration(flux_pcg* s) {
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st) {
     unsigned g = red_blocks(s->n);
     unsigned gfull = blocks_for(s->n, TPB);
+    g_march_toggle = 0;
     // host-written fields of the scalar block
     memset(s->Sh, 0, sizeof(Scal));
     s->Sh->tol = tol;

@@ -16,6 +16,7 @@
 #pragma once
 #include <algorithm>
 
+#include "flux_internal.h"
 #include "flux_stencil.h"
 
 namespace flux {
@@ -32,6 +33,7 @@ struct March {
     int n2;         // planes
     unsigned bxy;   // blocks per chunk
     unsigned nb;    // blocks in the grid
+    unsigned rev;   // walk the z-chunks from the last one down
 };
 
 // `want` is the block count aimed at (several waves of the 256 CUs).
@@ -47,6 +49,13 @@ inline March march_shape(int run, int n1, int n2, int tpb, unsigned want) {
     m.kz = (n2 + (int)bz - 1) / (int)bz;
     bz = (unsigned)((n2 + m.kz - 1) / m.kz);
     m.nb = m.bxy * bz;
+    m.rev = 0;
+    return m;
+}
+
+// The next finest-level launch's march: the z-chunk order flips every call.
+inline March flip_march(March m) {
+    m.rev = g_march_flip ? (g_march_toggle ^= 1u) : 0u;
     return m;
 }
 
@@ -54,7 +63,9 @@ inline March march_shape(int run, int n1, int n2, int tpb, unsigned want) {
 // lies past the plane's end.
 __device__ __forceinline__ bool march_begin(const March& M, int& t, int& k0, int& k1) {
     const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-    const unsigned bz = lb / M.bxy, bxy = lb - bz * M.bxy;
+    unsigned bz = lb / M.bxy;
+    const unsigned bxy = lb - bz * M.bxy;
+    if (M.rev) bz = M.nb / M.bxy - 1u - bz;
     t = (int)(bxy * blockDim.x + threadIdx.x);
     k0 = (int)bz * M.kz;
     k1 = min(k0 + M.kz, M.n2);
