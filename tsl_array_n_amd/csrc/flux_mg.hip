@@ -648,7 +648,8 @@ int refresh_t(flux_multigrid* mg) {
 }
 
 bool use_fused(const flux_multigrid* mg, const MGLevel& v) {
-    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= FUSED_MIN_N && v.n[1] >= Sweep3::TY && v.n[2] >= 2;
+    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= FUSED_MIN_N && (v.n[0] & 1) == 0 && v.n[1] >= Sweep3::TY &&
+           v.n[2] >= 2;
 }
 
 // `iters` fused sweeps (colour A then B each), ping-ponging between v.x and
@@ -952,9 +953,10 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     for (int a = 0; a < ndim; ++a) mg->w[a] = w ? w[a] : nullptr;
     mg->has_w = (w != nullptr);
     mg->has_masks = (mask != nullptr) || (w != nullptr);
-    // the one-pass sweep measured slower than two launches (247 vs 2 x 118 us
-    // at 256^3; 15 vs 2 x 5 us at 32^3: its z-march is latency-bound), so it
-    // is opt-in (FLUX_FUSED_SWEEP=1)
+    // the one-pass sweep moves half the bytes of two launches but runs at
+    // ~2.1 TB/s (latency-bound z-march): 197 vs 2 x 110 us per sweep at 256^3
+    // in FP32, ~1 % per projection, and slower with FP16 weights; opt-in
+    // (FLUX_FUSED_SWEEP=1)
     const char* fenv = getenv("FLUX_FUSED_SWEEP");
     mg->fused = (fenv && fenv[0] == '1');
     mg->lv.resize(o.levels);
