@@ -112,16 +112,34 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
     const int j = t / half;
     const int p = t - j * half;
     const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
-    const uint8_t* __restrict__ m = L.mask;
+    const int plane = n0 * n1;
+    const uint8_t* __restrict__ nbits = L.nb;
     const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    // The pair of the next plane is loaded one plane ahead: in plane k the
+    // coloured cell is 2p+s, and in planes k-1 and k+1 the cell 2p+s has the
+    // other colour, so its (unchanged) value is the partner this thread read
+    // there -- the z-neighbours come from registers.
+    Pair2<XT> xnext;
+    Pair2<uint8_t> nnext;
+    const int cp0 = 2 * p + n0 * (j + n1 * k0);
+    if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp0);
+    if (nbits) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp0);
+    float xprev = 0.0f;
     for (int k = k0; k < k1; ++k) {
         const int s = (j + k + color) & 1;  // which cell of the pair has the colour
         const int i = 2 * p + s;
         const int cp = 2 * p + n0 * (j + n1 * k);
         const int c = cp + s;
+        const Pair2<XT> xp = xnext;
+        const Pair2<uint8_t> np = nnext;
+        if (k + 1 < n2) {
+            if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp + plane);
+            if (nbits && k + 1 < k1) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp + plane);
+        }
+        const unsigned bits = nbits ? (s ? np.hi : np.lo) : 0u;
         float w[6];
         face_w3(L, i, j, k, c, w);
-        const bool dir = m && m[c];
+        const bool dir = (bits & 64u) != 0;
         const Pair2<XT> bp = *reinterpret_cast<const Pair2<XT>*>(b + cp);
         const float bc = to_f32(s ? bp.hi : bp.lo);
         if (FROM_ZERO) {
@@ -137,25 +155,20 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
             o.hi = to_store<XT>(s ? v : 0.0f);
             *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
         } else {
-            const Pair2<XT> xp = *reinterpret_cast<const Pair2<XT>*>(x + cp);
             const float xc = to_f32(s ? xp.hi : xp.lo);
             const float xpart = to_f32(s ? xp.lo : xp.hi);
-            int q[6];
-            nbr_index3(n0, n1, n2, i, j, k, c, q);
             // the pair partner is the upper x neighbour of the even cell and the lower of the odd one
-            const int qo = s ? q[1] : q[0];
+            const int qo = s ? (i < n0 - 1 ? c + 1 : c) : (i > 0 ? c - 1 : c);
             const float xo = to_f32(x[qo]);
             float xn[6];
             xn[0] = s ? xpart : xo;
             xn[1] = s ? xo : xpart;
-            xn[2] = to_f32(x[q[2]]);
-            xn[3] = to_f32(x[q[3]]);
-            xn[4] = to_f32(x[q[4]]);
-            xn[5] = to_f32(x[q[5]]);
-            if (m) {
+            xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
+            xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
+            xn[4] = k > k0 ? xprev : to_f32(x[k > 0 ? c - plane : c]);
+            xn[5] = k + 1 < n2 ? to_f32(s ? xnext.hi : xnext.lo) : xc;
 #pragma unroll
-                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
-            }
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
             float sum, dsum;
             row_terms<float>(xc, xn, w, ih2, sum, dsum);
             const float d = dir ? -1.0f : dsum;
@@ -165,6 +178,7 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
             if (s) o.hi = nv;
             else o.lo = nv;
             *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+            xprev = xpart;
             if (DOT) {
                 dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
                 dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
@@ -184,30 +198,37 @@ __global__ void __launch_bounds__(MTPB) k_residual3(LevelDevT<WT> L, March M, co
     int t, k0, k1;
     if (!march_begin(M, t, k0, k1)) return;
     const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const int plane = n0 * n1;
     const int j = t / n0;
     const int i = t - j * n0;
-    const uint8_t* __restrict__ m = L.mask;
+    const uint8_t* __restrict__ nbits = L.nb;
     const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    // the z-neighbours are the previous and the next plane's centres
+    float xnext = to_f32(x[i + n0 * (j + n1 * k0)]);
+    float xprev = 0.0f;
     for (int k = k0; k < k1; ++k) {
         const int c = i + n0 * (j + n1 * k);
+        const float xc = xnext;
+        if (k + 1 < n2) xnext = to_f32(x[c + plane]);
+        const unsigned bits = nbits ? nbits[c] : 0u;
         float w[6];
         face_w3(L, i, j, k, c, w);
-        int q[6];
-        nbr_index3(n0, n1, n2, i, j, k, c, q);
-        const float xc = to_f32(x[c]);
         float xn[6];
+        xn[0] = to_f32(x[i > 0 ? c - 1 : c]);
+        xn[1] = to_f32(x[i < n0 - 1 ? c + 1 : c]);
+        xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
+        xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
+        xn[4] = k > k0 ? xprev : to_f32(x[k > 0 ? c - plane : c]);
+        xn[5] = k + 1 < n2 ? xnext : xc;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) xn[a] = to_f32(x[q[a]]);
-        if (m) {
-#pragma unroll
-            for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
-        }
+        for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
         float sum, dsum;
         row_terms<float>(xc, xn, w, ih2, sum, dsum);
         // a Dirichlet cell's residual is stored as 0: its only reader, the
         // restriction, counts masked fine cells as 0
-        const bool dir = m && m[c];
+        const bool dir = (bits & 64u) != 0;
         r[c] = to_store<RT>(dir ? 0.0f : to_f32(b[c]) - sum);
+        xprev = xc;
     }
 }
 
@@ -564,6 +585,7 @@ struct MGLevel {
     void* buf = nullptr;  // residual (FP32 on level 0)
     void* alt = nullptr;  // ping-pong partner of x for the fused sweep
     uint8_t* mask = nullptr;              // level 0: the caller's (not owned)
+    uint8_t* nb = nullptr;                // neighbour-mask bits of the column-march kernels
     void* w[3] = {nullptr, nullptr, nullptr};  // level 0: caller's FP32 or own converted copy
     bool own = false;
     bool own_w = false;
@@ -606,6 +628,7 @@ LevelDevT<WT> level_dev(const flux_multigrid* mg, const MGLevel& v) {
     L.w0 = (const WT*)v.w[0];
     L.w1 = (const WT*)v.w[1];
     L.w2 = mg->ndim == 3 ? (const WT*)v.w[2] : nullptr;
+    L.nb = v.mask ? v.nb : nullptr;
     return L;
 }
 
@@ -643,6 +666,9 @@ int refresh_t(flux_multigrid* mg) {
             mg->ndim, f.n[0], f.n[1], src, c.n[0], c.n[1], c.n[2], mg->has_w ? (const ST*)c.w[0] : nullptr,
             mg->has_w ? (const ST*)c.w[1] : nullptr, mg->has_w ? (const ST*)c.w[2] : nullptr, c.mask);
     }
+    for (MGLevel& v : mg->lv)
+        if (v.nb && v.mask)
+            k_nbits3<ST><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), v.nb);
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
@@ -908,6 +934,7 @@ int flux_multigrid_destroy(flux_multigrid* mg) {
         MGLevel& v = mg->lv[l];
         dfree(v.buf);
         dfree(v.alt);
+        dfree(v.nb);
         if (v.own) {
             dfree(v.x);
             dfree(v.b);
@@ -978,6 +1005,10 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
         if (ndim == 3 && mg->fused) {
             v.alt = dmalloc(cells * vs);
             ok = ok && v.alt;
+        }
+        if (mg->has_masks || mask) {
+            v.nb = (uint8_t*)dmalloc(cells);
+            ok = ok && v.nb;
         }
         if (l == 0) {
             v.mask = mask ? (uint8_t*)mask->d : nullptr;

@@ -237,29 +237,37 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
     int t, k0, k1;
     if (march_begin(M, t, k0, k1)) {
         const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
         const int j = t / n0;
         const int i = t - j * n0;
-        const uint8_t* __restrict__ m = L.mask;
+        const uint8_t* __restrict__ nbits = L.nb;
         const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        auto pval = [&](int q) { return z[q] + pold[q] * beta; };
+        // the z-neighbours are the previous and the next plane's centres
+        float pnext = pval(i + n0 * (j + n1 * k0));
+        float pprev = 0.0f;
         for (int k = k0; k < k1; ++k) {
             const int c = i + n0 * (j + n1 * k);
+            const float pc = pnext;
+            if (k + 1 < n2) pnext = pval(c + plane);
+            const unsigned bits = nbits ? nbits[c] : 0u;
             float w[6];
             face_w3(L, i, j, k, c, w);
-            int q[6];
-            nbr_index3(n0, n1, n2, i, j, k, c, q);
-            const float pc = z[c] + pold[c] * beta;
             float xn[6];
+            xn[0] = pval(i > 0 ? c - 1 : c);
+            xn[1] = pval(i < n0 - 1 ? c + 1 : c);
+            xn[2] = pval(j > 0 ? c - n0 : c);
+            xn[3] = pval(j < n1 - 1 ? c + n0 : c);
+            xn[4] = k > k0 ? pprev : pval(k > 0 ? c - plane : c);
+            xn[5] = k + 1 < n2 ? pnext : pc;
 #pragma unroll
-            for (int a = 0; a < 6; ++a) xn[a] = z[q[a]] + pold[q[a]] * beta;
-            if (m) {
-#pragma unroll
-                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0f : xn[a];
-            }
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
             float sum, dsum;
             row_terms<float>(pc, xn, w, ih2, sum, dsum);
-            const float ap = (m && m[c]) ? -pc : sum;
+            const float ap = (bits & 64u) ? -pc : sum;
             pnew[c] = pc;
             v += (double)pc * (double)ap;
+            pprev = pc;
         }
     }
     store_block_partial(v, partials);
@@ -298,31 +306,39 @@ __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double
     int t, k0, k1;
     if (march_begin(M, t, k0, k1)) {
         const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
         const int j = t / n0;
         const int i = t - j * n0;
-        const uint8_t* __restrict__ m = L.mask;
+        const uint8_t* __restrict__ nbits = L.nb;
         const double ih2[3] = {L.ih2d[0], L.ih2d[1], L.ih2d[2]};
+        auto xval = [&](int q) { return xold[q] + alpha * (double)p[q]; };
+        // the z-neighbours are the previous and the next plane's centres
+        double xnext = xval(i + n0 * (j + n1 * k0));
+        double xprev = 0.0;
         for (int k = k0; k < k1; ++k) {
             const int c = i + n0 * (j + n1 * k);
+            const double xc = xnext;
+            if (k + 1 < n2) xnext = xval(c + plane);
+            const unsigned bits = nbits ? nbits[c] : 0u;
             float w[6];
             face_w3(L, i, j, k, c, w);
-            int q[6];
-            nbr_index3(n0, n1, n2, i, j, k, c, q);
-            const double xc = xold[c] + alpha * (double)p[c];
             double xn[6];
+            xn[0] = xval(i > 0 ? c - 1 : c);
+            xn[1] = xval(i < n0 - 1 ? c + 1 : c);
+            xn[2] = xval(j > 0 ? c - n0 : c);
+            xn[3] = xval(j < n1 - 1 ? c + n0 : c);
+            xn[4] = k > k0 ? xprev : xval(k > 0 ? c - plane : c);
+            xn[5] = k + 1 < n2 ? xnext : xc;
 #pragma unroll
-            for (int a = 0; a < 6; ++a) xn[a] = xold[q[a]] + alpha * (double)p[q[a]];
-            if (m) {
-#pragma unroll
-                for (int a = 0; a < 6; ++a) xn[a] = m[q[a]] ? 0.0 : xn[a];
-            }
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0 : xn[a];
             double sum, dsum;
             row_terms<double>(xc, xn, w, ih2, sum, dsum);
-            const double ax = (m && m[c]) ? -xc : sum;
+            const double ax = (bits & 64u) ? -xc : sum;
             xnew[c] = xc;
             const double rv = (double)b[c] - ax;
             r[c] = (float)rv;
             v += rv * rv;
+            xprev = xc;
         }
     }
     store_block_partial(v, partials);
@@ -397,7 +413,16 @@ struct flux_operator {
     const flux_field* mask = nullptr;
     const flux_field* w[3] = {nullptr, nullptr, nullptr};
     LevelDev L;
+    uint8_t* nb = nullptr;  // neighbour-mask bits of the column-march kernels (refreshed per solve)
 };
+
+namespace flux {
+// The operator's neighbour-mask bits from its (caller-owned, possibly
+// changed) mask.
+static void operator_refresh_bits(flux_operator* op) {
+    if (op->nb) k_nbits3<float><<<blocks_for(cells_of(op->n), TPB), TPB, 0, g_stream>>>(op->L, op->nb);
+}
+}  // namespace flux
 
 extern "C" {
 
@@ -436,6 +461,14 @@ int flux_operator_create(int ndim, const int* shape, const double* spacing, cons
     }
     op->mask = mask;
     op->L = make_level(ndim, op->n, op->h, mask ? (const uint8_t*)mask->d : nullptr, w ? wp : nullptr);
+    if (mask) {
+        op->nb = (uint8_t*)dmalloc(cells_of(op->n));
+        if (!op->nb) {
+            delete op;
+            return set_error(FLUX_ERR_NOMEM, "createLaplacianOperator: allocation failed");
+        }
+        op->L.nb = op->nb;
+    }
     *out = op;
     return FLUX_OK;
 }
@@ -466,6 +499,9 @@ int flux_operator_diagonal(flux_operator* op, flux_field* out) {
 }
 
 int flux_operator_destroy(flux_operator* op) {
+    if (!op) return FLUX_OK;
+    if (g_init) (void)hipStreamSynchronize(g_stream);
+    dfree(op->nb);
     delete op;
     return FLUX_OK;
 }
@@ -539,6 +575,7 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     unsigned g = red_blocks(s->n);
     unsigned gfull = blocks_for(s->n, TPB);
     g_march_toggle = 0;
+    operator_refresh_bits(s->op);
     // host-written fields of the scalar block
     memset(s->Sh, 0, sizeof(Scal));
     s->Sh->tol = tol;

@@ -41,6 +41,10 @@ struct LevelDevT {
     const WT* w0;        // face weights per axis; all null = unit weights
     const WT* w1;
     const WT* w2;
+    // column-march kernels: per cell, bit a (0..5) = mask of the clamped
+    // neighbour a (-x, +x, -y, +y, -z, +z; the cell itself where the
+    // neighbour is outside), bit 6 = the cell's own mask; null when mask is
+    const uint8_t* nb = nullptr;
 };
 using LevelDev = LevelDevT<float>;
 

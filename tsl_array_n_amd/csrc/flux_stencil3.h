@@ -113,6 +113,25 @@ __device__ __forceinline__ void nbr_index3(int n0, int n1, int n2, int i, int j,
     q[5] = k < n2 - 1 ? c + plane : c;
 }
 
+// The neighbour-mask bits of every cell (LevelDevT::nb) from L.mask.
+template <typename WT>
+__global__ void k_nbits3(LevelDevT<WT> L, uint8_t* __restrict__ nb) {
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n0 * n1 * n2) return;
+    const int row = c / n0;
+    const int i = c - row * n0;
+    const int k = row / n1;
+    const int j = row - k * n1;
+    int q[6];
+    nbr_index3(n0, n1, n2, i, j, k, c, q);
+    unsigned v = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) v |= L.mask[q[a]] ? (1u << a) : 0u;
+    v |= L.mask[c] ? 64u : 0u;
+    nb[c] = (uint8_t)v;
+}
+
 // Row sum and diagonal from the centre value, the six Dirichlet-masked
 // neighbour values and the six face weights (outside faces zero).
 template <typename T>
