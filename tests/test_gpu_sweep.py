@@ -108,3 +108,28 @@ def test_fused_projection_matches_two_launch(flux):
     assert abs(res[0][0]["iterations"] - res[1][0]["iterations"]) <= 1
     p0, p1 = res[0][1], res[1][1]
     assert np.linalg.norm(p1 - p0) <= 1e-5 * np.linalg.norm(p0)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((64, 32, 48), 4), ((96, 64, 64), 5)])
+def test_vcycle_tail_is_bit_identical(flux, storage, shape, levels):
+    """The last two levels in one workgroup (k_vcycle_tail) against their
+    separate launches: one V-cycle agrees bit for bit."""
+    F = flux
+    sc = scenes.dam_break_system(64) if shape == (64, 64, 64) else box_scene(shape, 7)
+    outs = []
+    for tail in ("0", "1"):
+        old = os.environ.get("FLUX_VCYCLE_TAIL")
+        os.environ["FLUX_VCYCLE_TAIL"] = tail
+        try:
+            ps, vel, mg = build_mg(F, sc, levels, storage, False)
+        finally:
+            if old is None:
+                del os.environ["FLUX_VCYCLE_TAIL"]
+            else:
+                os.environ["FLUX_VCYCLE_TAIL"] = old
+        z = F.array_n("float", shape)
+        mg(ps.b, z)
+        outs.append(z.to_array())
+    assert np.all(np.isfinite(outs[0])) and np.any(outs[0] != 0)
+    np.testing.assert_array_equal(outs[1], outs[0])

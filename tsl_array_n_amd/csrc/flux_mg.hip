@@ -421,7 +421,10 @@ __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int 
     const int cplane = c0 * c1;
     for (int k = k0; k < k1; ++k) {
         const int c = i + f0 * (j + f1 * k);
-        if (mf && mf[c]) continue;
+        // every load is issued before the mask is known (one memory round
+        // trip per plane); a masked cell is not written
+        const bool masked = mf && mf[c];
+        const float old = to_f32(xf[c]);
         int zk[2];
         float wz[2];
         taps(k, c2, zk, wz);
@@ -437,7 +440,7 @@ __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int 
                 }
         if (rr) sum = sum / scale;
         if (mult != 1.0f) sum = sum * mult;
-        xf[c] = to_store<FT>(to_f32(xf[c]) + sum);
+        if (!masked) xf[c] = to_store<FT>(old + sum);
     }
 }
 
@@ -509,6 +512,198 @@ __global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT
                                                              const int* __restrict__ stop) {
     if (stop && *stop) return;
     coarse_solve_block<XT, WT>(L, x, b, omega, iters);
+}
+
+// The V-cycle's tail in one workgroup: a 3-D level `lt` of at most
+// COARSE_LDS_CELLS cells and the coarsest level below it.  Exactly the
+// launches it replaces -- nu_down colour pairs from x = 0, the residual,
+// the full-weighting restriction, the coarsest solve, the trilinear
+// correction and nu_up reversed colour pairs -- with the same per-cell
+// arithmetic and the same rounding through the storage type, but with every
+// vector in LDS and one barrier between phases.  x of level lt is written out
+// for the next finer level's correction.
+template <typename ST>
+__global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, LevelDevT<ST> L1,
+                                                            const ST* __restrict__ b0, ST* __restrict__ x0,
+                                                            float omega, int nu_down, int nu_up, int n_coarsest,
+                                                            float mult, const int* __restrict__ stop) {
+    constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
+    __shared__ float sx0[COARSE_LDS_CELLS];
+    __shared__ float sr0[COARSE_LDS_CELLS];
+    __shared__ float sx1[COARSE_LDS_CELLS];
+    if (stop && *stop) return;
+    const int n0 = L0.n0 * L0.n1 * L0.n2;
+    const int n1 = L1.n0 * L1.n1 * L1.n2;
+    const float ih0[3] = {L0.ih2[0], L0.ih2[1], L0.ih2[2]};
+    const float ih1[3] = {L1.ih2[0], L1.ih2[1], L1.ih2[2]};
+    // this thread's level-lt cells: weights, neighbour slots, mask/colour bits, b
+    float bc[PER], w[PER][6];
+    int q[PER][6];
+    unsigned bits[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int c = threadIdx.x + u * COARSE_TPB;
+        bits[u] = 0;
+        bc[u] = 0.0f;
+        if (c < n0) {
+            int i, j, k;
+            cell_coords(L0, c, i, j, k);
+            face_w3(L0, i, j, k, c, w[u]);
+            nbr_index3(L0.n0, L0.n1, L0.n2, i, j, k, c, q[u]);
+            if (L0.mask) {
+                for (int a = 0; a < 6; ++a) bits[u] |= L0.mask[q[u][a]] ? (1u << a) : 0u;
+                bits[u] |= L0.mask[c] ? 64u : 0u;
+            }
+            bits[u] |= ((i + j + k) & 1) ? 128u : 0u;
+            bc[u] = to_f32(b0[c]);
+        }
+    }
+    for (int c = threadIdx.x; c < n0; c += COARSE_TPB) sx0[c] = 0.0f;
+    __syncthreads();
+    auto sweep0 = [&](int color) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int c = threadIdx.x + u * COARSE_TPB;
+            if (c >= n0 || (int)((bits[u] >> 7) & 1u) != color) continue;
+            const float xc = sx0[c];
+            float xn[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits[u] >> a) & 1u) ? 0.0f : sx0[q[u][a]];
+            float sum, dsum;
+            row_terms<float>(xc, xn, w[u], ih0, sum, dsum);
+            const bool dir = (bits[u] >> 6) & 1u;
+            const float d = dir ? -1.0f : dsum;
+            const float ax = dir ? -xc : sum;
+            sx0[c] = to_f32(to_store<ST>(xc + ((bc[u] - ax) / d) * omega));
+        }
+        __syncthreads();
+    };
+    for (int it = 0; it < nu_down; ++it) {
+        sweep0(0);
+        sweep0(1);
+    }
+    // residual (k_residual3): a Dirichlet cell's residual is 0
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int c = threadIdx.x + u * COARSE_TPB;
+        if (c >= n0) continue;
+        const float xc = sx0[c];
+        float xn[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) xn[a] = ((bits[u] >> a) & 1u) ? 0.0f : sx0[q[u][a]];
+        float sum, dsum;
+        row_terms<float>(xc, xn, w[u], ih0, sum, dsum);
+        const bool dir = (bits[u] >> 6) & 1u;
+        sr0[c] = to_f32(to_store<ST>(dir ? 0.0f : bc[u] - sum));
+    }
+    __syncthreads();
+    // restriction (k_restrict3) into the coarsest level's b; its cells' data
+    float bc1 = 0.0f, w1[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int q1[6] = {0, 0, 0, 0, 0, 0};
+    unsigned bits1 = 0;
+    const int f0 = L0.n0, f1 = L0.n1, f2 = L0.n2;
+    const int c0 = L1.n0, c1 = L1.n1, c2 = L1.n2;
+    for (int t = threadIdx.x; t < n1; t += COARSE_TPB) {
+        int I, J, K;
+        cell_coords(L1, t, I, J, K);
+        const float tw[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+        int ti[4], tj[4];
+        ti[0] = I > 0 ? 2 * I - 1 : 2 * I;
+        ti[1] = 2 * I;
+        ti[2] = 2 * I + 1;
+        ti[3] = (I + 1 < c0) ? 2 * I + 2 : 2 * I + 1;
+        tj[0] = J > 0 ? 2 * J - 1 : 2 * J;
+        tj[1] = 2 * J;
+        tj[2] = 2 * J + 1;
+        tj[3] = (J + 1 < c1) ? 2 * J + 2 : 2 * J + 1;
+        auto reduce_plane = [&](int fk) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int bq = 0; bq < 4; ++bq) sum += sr0[ti[a] + f0 * tj[bq] + f0 * f1 * fk] * (tw[a] * tw[bq]);
+            return sum;
+        };
+        const int tk0 = K > 0 ? 2 * K - 1 : 2 * K;
+        const int tk3 = (K + 1 < c2) ? 2 * K + 2 : 2 * K + 1;
+        const float P0 = reduce_plane(tk0);
+        const float P1 = (tk0 == 2 * K) ? P0 : reduce_plane(2 * K);
+        const float P2 = reduce_plane(2 * K + 1);
+        const float P3 = (tk3 == 2 * K + 1) ? P2 : reduce_plane(tk3);
+        const float v = P0 * tw[0] + P1 * tw[1] + P2 * tw[2] + P3 * tw[3];
+        bc1 = to_f32(to_store<ST>((L1.mask && L1.mask[t]) ? 0.0f : v * 1.0f));
+        face_w3(L1, I, J, K, t, w1);
+        nbr_index3(c0, c1, c2, I, J, K, t, q1);
+        if (L1.mask) {
+            for (int a = 0; a < 6; ++a) bits1 |= L1.mask[q1[a]] ? (1u << a) : 0u;
+            bits1 |= L1.mask[t] ? 64u : 0u;
+        }
+        bits1 |= ((I + J + K) & 1) ? 128u : 0u;
+        sx1[t] = 0.0f;
+    }
+    (void)f2;
+    __syncthreads();
+    // the coarsest solve (coarse_solve_block), one cell per thread
+    const bool own1 = (int)threadIdx.x < n1;
+    for (int it = 0; it < n_coarsest; ++it) {
+        for (int color = 0; color < 2; ++color) {
+            if (own1 && (int)((bits1 >> 7) & 1u) == color) {
+                const int c = threadIdx.x;
+                const float xc = sx1[c];
+                float xn[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits1 >> a) & 1u) ? 0.0f : sx1[q1[a]];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w1, ih1, sum, dsum);
+                const bool dir = (bits1 >> 6) & 1u;
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                sx1[c] = to_f32(to_store<ST>(xc + ((bc1 - ax) / d) * omega));
+            }
+            __syncthreads();
+        }
+    }
+    // trilinear correction (k_correct3) of level lt
+    auto taps = [](int F, int nc, int* ti, float* tw) {
+        const int ci = F >> 1;
+        const bool even = (F & 1) == 0;
+        const int lower = ci > 0 ? ci - 1 : ci;
+        const int upper = ci < nc - 1 ? ci + 1 : ci;
+        ti[0] = even ? lower : ci;
+        tw[0] = even ? 0.25f : 0.75f;
+        ti[1] = even ? ci : upper;
+        tw[1] = even ? 0.75f : 0.25f;
+    };
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int c = threadIdx.x + u * COARSE_TPB;
+        if (c >= n0 || ((bits[u] >> 6) & 1u)) continue;
+        int i, j, k;
+        cell_coords(L0, c, i, j, k);
+        int xi[2], yj[2], zk[2];
+        float wx[2], wy[2], wz[2];
+        taps(i, c0, xi, wx);
+        taps(j, c1, yj, wy);
+        taps(k, c2, zk, wz);
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+                for (int cq = 0; cq < 2; ++cq) {
+                    const float wgt = wx[a] * wy[bq] * wz[cq];
+                    sum += sx1[xi[a] + c0 * yj[bq] + c0 * c1 * zk[cq]] * wgt;
+                }
+        if (mult != 1.0f) sum = sum * mult;
+        sx0[c] = to_f32(to_store<ST>(sx0[c] + sum));
+    }
+    __syncthreads();
+    for (int it = 0; it < nu_up; ++it) {
+        sweep0(1);
+        sweep0(0);
+    }
+    for (int c = threadIdx.x; c < n0; c += COARSE_TPB) x0[c] = to_store<ST>(sx0[c]);
 }
 
 // Coarse face weight = mean of the 2^(d-1) fine faces at index 2I along the
@@ -603,6 +798,7 @@ struct flux_multigrid {
     const flux_field* w[3] = {nullptr, nullptr, nullptr};
     bool has_w = false, has_masks = false;
     bool fused = false;  // fused sweeps on every level wide enough (FLUX_FUSED_SWEEP=1)
+    bool tail = true;    // the last two levels in one workgroup (FLUX_VCYCLE_TAIL=0 turns it off)
 };
 
 static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
@@ -793,7 +989,12 @@ void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, cons
     if (mg->ndim == 3) {
         // fewer, longer columns: consecutive coarse planes share two fine planes.
         // The residual (k_residual3) is already 0 on masked fine cells.
-        const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, std::max(64u, g_march_blocks / 4));
+        static const unsigned rdiv = [] {
+            const char* e = getenv("FLUX_RESTRICT_DIV");
+            const long v = e ? strtol(e, nullptr, 10) : 4;
+            return (unsigned)(v >= 1 && v <= 64 ? v : 4);
+        }();
+        const March M = march_shape(c.n[0], c.n[1], c.n[2], MTPB, std::max(64u, g_march_blocks / rdiv));
         k_restrict3<FINE, IT, OT><<<M.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(M) : M, v.n[0], v.n[1], v.n[2], r,
                                                                 nullptr, c.n[0], c.n[1],
                                                                 c.n[2], (OT*)c.b, c.mask, scale, stop);
@@ -851,6 +1052,14 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
     MGLevel& c = mg->lv[l + 1];
     int64_t cells = cells_of(v.n), cc = cells_of(c.n);
     const double* scale = fine ? rr : nullptr;
+    if (!fine && l + 2 == nl && mg->ndim == 3 && cells <= COARSE_LDS_CELLS && cc <= COARSE_TPB &&
+        !use_fused(mg, v) && mg->tail) {
+        k_vcycle_tail<ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), level_dev<ST>(mg, c), (const ST*)v.b,
+                                                          (ST*)v.x, (float)mg->o.omega, mg->o.nu_down,
+                                                          mg->o.nu_up, mg->o.n_coarsest,
+                                                          (float)(c.xscale / v.xscale), stop);
+        return FLUX_OK;
+    }
     if (fine) {
         if (mg->o.nu_down > 0) level_smooth<1, float, ST>(mg, l, mg->o.nu_down, false, true, stop);
         else fill_t<float>(v.x, cells, 0.0f);
@@ -986,6 +1195,8 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     // (FLUX_FUSED_SWEEP=1)
     const char* fenv = getenv("FLUX_FUSED_SWEEP");
     mg->fused = (fenv && fenv[0] == '1');
+    const char* tenv = getenv("FLUX_VCYCLE_TAIL");
+    mg->tail = !(tenv && tenv[0] == '0');
     mg->lv.resize(o.levels);
     const size_t es = store_size(o.storage);
     bool ok = true;
