@@ -258,6 +258,23 @@ int flux_advect_particles(const flux_grid3* g, flux_field* positions, int64_t co
 /* advectFaceCentered3, order 1, no collider (advection_solver3.js:130-176) */
 int flux_advect_faces(const flux_grid3* g, const flux_field* const* vel_in, flux_field* const* vel_out,
                       float dt, int max_substeps);
+/* createSemiLagrangianAdvectionSolver3({velocityGrid, dt, maxSubsteps, order})
+ * (advection_solver3.js:67-362), no collider: the velocity fields are bound
+ * at creation (their contents are read at each call); order 1 is the
+ * monotone tricubic semi-Lagrangian step, order 2 MacCormack with the
+ * min/max clamp over the forward trace's trilinear corners (90-128).  Order 2
+ * owns 12 bytes of device scratch per face. */
+typedef struct flux_advection3 flux_advection3;
+int flux_advection3_create(const flux_grid3* g, const flux_field* const* velocity, float dt, int max_substeps,
+                           int order, flux_advection3** out);
+/* the time step of later calls (dt as a live node) */
+int flux_advection3_set_dt(flux_advection3* a, float dt);
+/* advectFaceCentered3(input, output) (130-282); output must alias neither the
+ * input nor the velocity */
+int flux_advection3_faces(flux_advection3* a, const flux_field* const* in, flux_field* const* out);
+/* advectScalar3(input, output) (285-357): cell-centred fields */
+int flux_advection3_scalar(flux_advection3* a, const flux_field* in, flux_field* out);
+int flux_advection3_destroy(flux_advection3* a);
 /* liquid fraction per cell from sorted particles (count > 0 -> 1), then
  * two-phase face beta = rho_ref / max(rho_face, 1e-4)
  * (grid_two_phase_flip_solver2.js:693-762, in 3-D) */
@@ -347,6 +364,22 @@ int flux_polygon_sdf2(flux_field* sdf, const flux_grid2* g, const double* xy, co
  * the input. */
 int flux_advect_faces2(const flux_grid2* g, const flux_field* u, const flux_field* v, const flux_field* collider_sdf,
                        float dt, int max_substeps, flux_field* u_out, flux_field* v_out);
+/* createSemiLagrangianAdvectionSolver2({velocityGrid, collider, dt,
+ * maxSubsteps, order}) (advection_solver2.js:174-526), face grids: the
+ * velocity fields and the collider SDF (may be NULL) are bound at creation;
+ * order 1, or 2 = MacCormack with the clamp over the forward trace's bilinear
+ * corners that lie outside the collider (221-258).  Order 2 owns 12 bytes of
+ * device scratch per face. */
+typedef struct flux_advection2 flux_advection2;
+int flux_advection2_create(const flux_grid2* g, const flux_field* u, const flux_field* v,
+                           const flux_field* collider_sdf, float dt, int max_substeps, int order,
+                           flux_advection2** out);
+int flux_advection2_set_dt(flux_advection2* a, float dt);
+/* advectFaceCentered2(input, output): a face inside the collider keeps the
+ * output's value; the output must alias neither the input nor the velocity */
+int flux_advection2_faces(flux_advection2* a, const flux_field* in_u, const flux_field* in_v, flux_field* out_u,
+                          flux_field* out_v);
+int flux_advection2_destroy(flux_advection2* a);
 /* createMaxAbsReducer(fields).read() (reduction.js:43-77): max |x| */
 int flux_max_abs(const flux_field* const* fields, int count, float* out);
 /* computeAdaptiveSubSteps (time/cfl.js:38-48): steps = min(max_sub,
@@ -441,6 +474,7 @@ typedef struct {
     double max_iterations_factor;
     int relative_tolerance;
     double max_plausible_pressure;
+    int advection_order;                  /* options.advection.order: 1 (default) or 2 (MacCormack) */
 } flux_solver2_desc;
 void flux_solver2_desc_default(flux_solver2_desc* d);
 typedef struct flux_solver2 flux_solver2;
@@ -499,7 +533,7 @@ int flux_flip3_destroy(flux_flip3* s);
 /* createGridSmokeSolver3 (grid_smoke_solver3.js:25-154 over createGridSolver3,
  * grid_solver3.js:45-149): buoyancy up * (a rho + b (T - T_amb)) applied as an
  * external force, constrainVelocity (closed walls by flag), projection,
- * order-1 semi-Lagrangian velocity advection, then density and temperature
+ * semi-Lagrangian velocity advection (order 1 or 2), then density and temperature
  * advected through the new velocity and decayed into the other halves of
  * their ping-pong pairs.  No collider, inflow or outflow objects. */
 typedef struct {
@@ -517,6 +551,7 @@ typedef struct {
     int max_substeps;                     /* default 32 */
     int extrapolation_depth;              /* constrainVelocity's, default 5 */
     flux_pressure3_desc pressure;         /* resolution/spacing/origin set by the solver */
+    int advection_order;                  /* options.advection.order: 1 (default) or 2, velocity and scalars */
 } flux_smoke3_desc;
 void flux_smoke3_desc_default(flux_smoke3_desc* d);
 typedef struct flux_smoke3 flux_smoke3;

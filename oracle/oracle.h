@@ -235,6 +235,19 @@ void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const 
 /* advectScalar3, order 1, no collider: advection_solver3.js:285-300 */
 void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
                         const float h[3], const float origin[3], float dt, int max_sub);
+/* order 2 (MacCormack), no collider: advection_solver3.js:90-128 (clamp),
+ * 180-282 (faces), 302-357 (scalar) */
+void orc_advect_faces3_vd(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
+                          const float h[3], const float origin[3], float dt, int max_sub);
+void orc_advect_faces3_mc_vd(const float* const vel[3], const float* const data[3], float* const out[3],
+                             const int n[3], const float h[3], const float origin[3], float dt, int max_sub);
+/* trace (advection_solver3.js:84-88) of count points, in place */
+void orc_trace3(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
+                const float h[3], const float origin[3], float dt, int max_sub, float direction);
+void orc_advect_face3_mc(const float* u, const float* v, const float* w, float* u_out, float* v_out, float* w_out,
+                         const int n[3], const float h[3], const float origin[3], float dt, int max_sub);
+void orc_advect_scalar3_mc(const float* src, float* dst, const float* u, const float* v, const float* w,
+                           const int n[3], const float h[3], const float origin[3], float dt, int max_sub);
 /* external force with the smoke buoyancy: external_force_solver3.js:19-47,
  * grid_smoke_solver3.js:66-81 */
 void orc_buoyancy3(float* u, float* v, float* w, const float* dens, const float* temp, const int n[3],
@@ -257,6 +270,17 @@ float orc_sample_cubic2(const float* d, const int s[2], const float o[2], const 
  * advection_solver2.js:81-127, 263-320 (uo/vo hold the pre-advection copy) */
 void orc_advect_face2(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
                       const float* sdf, float dt, int max_sub, float* uo, float* vo);
+/* advectFaceCentered2, order 2 (MacCormack) with the collider:
+ * advection_solver2.js:221-258, 310-430; the _vd form advects data (du, dv)
+ * through the velocity (u, v), order 1 or 2 */
+void orc_advect_faces2_vd(const float* u, const float* v, const float* du, const float* dv, const int n[2],
+                          const float h[2], const float origin[2], const float* sdf, float dt, int max_sub, int order,
+                          float* uo, float* vo);
+/* trace with the collider (advection_solver2.js:197-201), in place */
+void orc_trace2(float* pos, int64_t count, const float* u, const float* v, const int n[2], const float h[2],
+                const float origin[2], const float* sdf, float dt, int max_sub, float direction);
+void orc_advect_face2_mc(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
+                         const float* sdf, float dt, int max_sub, float* uo, float* vo);
 /* constrainVelocity: grid_blocked_boundary_condition_solver2.js:75-575 */
 void orc_constrain_velocity2(float* u, float* v, const int n[2], const float h[2], const float origin[2],
                              int closed_flags, const float* sdf, float friction, int depth, int n_inflows,

@@ -190,28 +190,6 @@ static void back_trace2(const float* u, const float* v, const int n[2], const fl
     out[1] = py;
 }
 
-/* ---- advectFaceCentered2, order 1: advection_solver2.js:263-320.  A face
- *      whose position lies inside the collider keeps the output's value
- *      (uo/vo hold the pre-advection copy). ---- */
-void orc_advect_face2(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
-                      const float* sdf, float dt, int max_sub, float* uo, float* vo) {
-    const int sc[2] = {n[0], n[1]};
-    const float so[2] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f};
-    const int sz[2][2] = {{n[0] + 1, n[1]}, {n[0], n[1] + 1}};
-    const float oo[2][2] = {{origin[0] + 0.0f, origin[1] + h[1] * 0.5f}, {origin[0] + h[0] * 0.5f, origin[1] + 0.0f}};
-    const float* src[2] = {u, v};
-    float* dst[2] = {uo, vo};
-    for (int a = 0; a < 2; ++a)
-        for (int j = 0; j < sz[a][1]; ++j)
-            for (int i = 0; i < sz[a][0]; ++i) {
-                const float px = oo[a][0] + h[0] * (float)i, py = oo[a][1] + h[1] * (float)j;
-                if (sdf && !(orc_sample_bilinear2(sdf, sc, so, h, px, py) > 0.0f)) continue;
-                float t[2];
-                back_trace2(u, v, n, h, origin, sdf, px, py, dt, max_sub, 1.0f, t);
-                dst[a][i + (int64_t)sz[a][0] * j] = orc_sample_cubic2(src[a], sz[a], oo[a], h, t[0], t[1]);
-            }
-}
-
 /* ---- constrainVelocity: grid_blocked_boundary_condition_solver2.js:75-96
  *      (friction), 251-402 (collider kernels), 520-575 (the plan), for a
  *      static collider (velocity 0) given by its cell-centred SDF, the
@@ -481,4 +459,97 @@ void orc_outflow_dirichlet2(const float* osdf, const int n[2], const float h[2],
             const float px = so[0] + h[0] * (float)i, py = so[1] + h[1] * (float)j;
             active[i + (int64_t)n[0] * j] = orc_sample_bilinear2(osdf, sc, so, h, px, py) < 0;
         }
+}
+
+/* ---- advectFaceCentered2: advection_solver2.js:263-320 (order 1),
+ *      221-258 and 310-430 (order 2, MacCormack).  The trace follows the
+ *      solver's velocity grid (u, v) and stops at the collider; the value is
+ *      the bicubic sample of the advected grid (du, dv).  A face whose
+ *      position lies inside the collider keeps the output's value (uo/vo
+ *      hold its previous contents).  Order 2: forward = order 1 (the input
+ *      inside the collider), backward = the forward result sampled at the
+ *      trace with direction -1, then fwd + (in - bwd) / 2 where it lies in
+ *      the range of the 4 bilinear corners of the forward trace whose face
+ *      positions lie outside the collider, else fwd. ---- */
+void orc_advect_faces2_vd(const float* u, const float* v, const float* du, const float* dv, const int n[2],
+                          const float h[2], const float origin[2], const float* sdf, float dt, int max_sub, int order,
+                          float* uo, float* vo) {
+    const int sc[2] = {n[0], n[1]};
+    const float so[2] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f};
+    const int sz[2][2] = {{n[0] + 1, n[1]}, {n[0], n[1] + 1}};
+    const float oo[2][2] = {{origin[0] + 0.0f, origin[1] + h[1] * 0.5f}, {origin[0] + h[0] * 0.5f, origin[1] + 0.0f}};
+    const float* src[2] = {du, dv};
+    float* dst[2] = {uo, vo};
+    float* fwd[2] = {uo, vo};
+    if (order == 2)
+        for (int a = 0; a < 2; ++a) fwd[a] = (float*)malloc(sizeof(float) * (size_t)sz[a][0] * sz[a][1]);
+    for (int a = 0; a < 2; ++a)
+        for (int j = 0; j < sz[a][1]; ++j)
+            for (int i = 0; i < sz[a][0]; ++i) {
+                const int64_t f = i + (int64_t)sz[a][0] * j;
+                const float px = oo[a][0] + h[0] * (float)i, py = oo[a][1] + h[1] * (float)j;
+                if (sdf && !(orc_sample_bilinear2(sdf, sc, so, h, px, py) > 0.0f)) {
+                    if (order == 2) fwd[a][f] = src[a][f];
+                    continue;
+                }
+                float t[2];
+                back_trace2(u, v, n, h, origin, sdf, px, py, dt, max_sub, 1.0f, t);
+                fwd[a][f] = orc_sample_cubic2(src[a], sz[a], oo[a], h, t[0], t[1]);
+            }
+    if (order != 2) return;
+    for (int a = 0; a < 2; ++a)
+        for (int j = 0; j < sz[a][1]; ++j)
+            for (int i = 0; i < sz[a][0]; ++i) {
+                const int64_t f = i + (int64_t)sz[a][0] * j;
+                const float px = oo[a][0] + h[0] * (float)i, py = oo[a][1] + h[1] * (float)j;
+                if (sdf && !(orc_sample_bilinear2(sdf, sc, so, h, px, py) > 0.0f)) continue;
+                float tb[2], tf[2];
+                back_trace2(u, v, n, h, origin, sdf, px, py, dt, max_sub, -1.0f, tb);
+                const float bwd = orc_sample_cubic2(fwd[a], sz[a], oo[a], h, tb[0], tb[1]);
+                back_trace2(u, v, n, h, origin, sdf, px, py, dt, max_sub, 1.0f, tf);
+                const float fv = fwd[a][f];
+                const float corrected = fv + (src[a][f] - bwd) * 0.5f;
+                const float gx = (tf[0] - oo[a][0]) / h[0], gy = (tf[1] - oo[a][1]) / h[1];
+                const int i0 = (int)floorf(gx), j0 = (int)floorf(gy);
+                const int ci[2] = {clampi2(i0, 0, sz[a][0] - 1), clampi2(i0 + 1, 0, sz[a][0] - 1)};
+                const int cj[2] = {clampi2(j0, 0, sz[a][1] - 1), clampi2(j0 + 1, 0, sz[a][1] - 1)};
+                float mn = 1e30f, mx = -1e30f;
+                int have = 0;
+                for (int c = 0; c < 4; ++c) {
+                    const int ni = ci[c & 1], nj = cj[c >> 1];
+                    const float qx = oo[a][0] + h[0] * (float)ni, qy = oo[a][1] + h[1] * (float)nj;
+                    const int valid = sdf ? orc_sample_bilinear2(sdf, sc, so, h, qx, qy) > 0.0f : 1;
+                    const float val = src[a][ni + (int64_t)sz[a][0] * nj];
+                    if (valid) {
+                        mn = fminf(mn, val);
+                        mx = fmaxf(mx, val);
+                        have = 1;
+                    }
+                }
+                dst[a][f] = (have && corrected >= mn && corrected <= mx) ? corrected : fv;
+            }
+    for (int a = 0; a < 2; ++a) free(fwd[a]);
+}
+
+void orc_advect_face2(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
+                      const float* sdf, float dt, int max_sub, float* uo, float* vo) {
+    orc_advect_faces2_vd(u, v, u, v, n, h, origin, sdf, dt, max_sub, 1, uo, vo);
+}
+
+void orc_advect_face2_mc(const float* u, const float* v, const int n[2], const float h[2], const float origin[2],
+                         const float* sdf, float dt, int max_sub, float* uo, float* vo) {
+    orc_advect_faces2_vd(u, v, u, v, n, h, origin, sdf, dt, max_sub, 2, uo, vo);
+}
+
+/* ---- trace (advection_solver2.js:197-201) of each point, with the
+ *      collider's crossing clamp; exposed for the same reason as
+ *      orc_trace3. ---- */
+void orc_trace2(float* pos, int64_t count, const float* u, const float* v, const int n[2], const float h[2],
+                const float origin[2], const float* sdf, float dt, int max_sub, float direction) {
+    for (int64_t p = 0; p < count; ++p) {
+        float t[2];
+        back_trace2(u, v, n, h, origin, sdf, pos[2 * p], pos[2 * p + 1], dt, max_sub, direction, t);
+        pos[2 * p] = t[0];
+        pos[2 * p + 1] = t[1];
+    }
 }

@@ -431,14 +431,14 @@ static float sample_cubic(const float* data, const int s[3], const float o[3], c
     return mono_cubic(zv[0], zv[1], zv[2], zv[3], fz);
 }
 
-/* ---- advectFaceCentered3, order 1, no collider: advection_solver3.js:130-176 ---- */
-void orc_advect_face3(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
-                      const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+/* ---- advectFaceCentered3, order 1, no collider: advection_solver3.js:130-176.
+ *      The trace follows the solver's velocity grid (vel); the face value is
+ *      the tricubic sample of the advected grid (data) at the trace. ---- */
+void orc_advect_faces3_vd(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
+                          const float h[3], const float origin[3], float dt, int max_sub) {
     float ou[3], ov[3], ow[3];
     mac_origins(origin, h, ou, ov, ow);
     float hmin = fminf(h[0], fminf(h[1], h[2]));
-    const float* src[3] = {u, v, w};
-    float* dst[3] = {uo_, vo_, wo_};
     const float* org[3] = {ou, ov, ow};
     for (int a = 0; a < 3; ++a) {
         int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
@@ -449,10 +449,17 @@ void orc_advect_face3(const float* u, const float* v, const float* w, float* uo_
                     float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
                                     org[a][2] + (float)k * h[2]};
                     float tr[3];
-                    back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
-                    dst[a][orc_cidx(s, i, j, k)] = sample_cubic(src[a], s, org[a], h, tr);
+                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
+                    out[a][orc_cidx(s, i, j, k)] = sample_cubic(data[a], s, org[a], h, tr);
                 }
     }
+}
+
+void orc_advect_face3(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
+                      const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+    const float* vel[3] = {u, v, w};
+    float* out[3] = {uo_, vo_, wo_};
+    orc_advect_faces3_vd(vel, vel, out, n, h, origin, dt, max_sub);
 }
 
 /* ---- two-phase face beta (3-D form of grid_two_phase_flip_solver2.js:693-762) ---- */
@@ -496,4 +503,105 @@ void orc_advect_scalar3(const float* src, float* dst, const float* u, const floa
                 back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
                 dst[orc_cidx(s, i, j, k)] = sample_cubic(src, s, co, h, tr);
             }
+}
+
+/* ---- order 2 (MacCormack) without a collider: advection_solver3.js:90-128
+ *      (min/max clamp over the 8 trilinear corners of the forward trace),
+ *      180-282 (faces), 302-357 (scalar).  fwd = order 1; bwd = fwd sampled
+ *      at the trace with direction -1; out = fwd + (in - bwd) / 2 where it
+ *      lies within the corners' range, else fwd. ---- */
+static float mc_clamp(const float* data, const int s[3], const float o[3], const float h[3], const float tr[3],
+                      float corrected, float fwd) {
+    orc_tri t;
+    tri_weights(tr, o, h, s, &t);
+    float mn = 1e30f, mx = -1e30f;
+    for (int c = 0; c < 8; ++c) {
+        int i, j, k;
+        tri_corner(&t, c, &i, &j, &k);
+        const float v = data[orc_cidx(s, i, j, k)];
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    return (corrected >= mn && corrected <= mx) ? corrected : fwd;
+}
+
+void orc_advect_faces3_mc_vd(const float* const vel[3], const float* const data[3], float* const out[3],
+                             const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    const float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float* org[3] = {ou, ov, ow};
+    float* fwd[3];
+    for (int a = 0; a < 3; ++a) fwd[a] = (float*)malloc(sizeof(float) * (size_t)orc_faces(n, a));
+    orc_advect_faces3_vd(vel, data, fwd, n, h, origin, dt, max_sub);
+    for (int a = 0; a < 3; ++a) {
+        int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    const float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
+                                          org[a][2] + (float)k * h[2]};
+                    float tb[3], tf[3];
+                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, -1.0f, tb);
+                    const float bwd = sample_cubic(fwd[a], s, org[a], h, tb);
+                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tf);
+                    const int64_t f = orc_cidx(s, i, j, k);
+                    const float fv = fwd[a][f];
+                    const float corrected = fv + (data[a][f] - bwd) * 0.5f;
+                    out[a][f] = mc_clamp(data[a], s, org[a], h, tf, corrected, fv);
+                }
+    }
+    for (int a = 0; a < 3; ++a) free(fwd[a]);
+}
+
+void orc_advect_face3_mc(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
+                         const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+    const float* vel[3] = {u, v, w};
+    float* out[3] = {uo_, vo_, wo_};
+    orc_advect_faces3_mc_vd(vel, vel, out, n, h, origin, dt, max_sub);
+}
+
+void orc_advect_scalar3_mc(const float* src, float* dst, const float* u, const float* v, const float* w,
+                           const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    const float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    const int s[3] = {n[0], n[1], n[2]};
+    float* fwd = (float*)malloc(sizeof(float) * (size_t)orc_cells(n));
+    orc_advect_scalar3(src, fwd, u, v, w, n, h, origin, dt, max_sub);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const float pos[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+                float tb[3], tf[3];
+                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, -1.0f, tb);
+                const float bwd = sample_cubic(fwd, s, co, h, tb);
+                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tf);
+                const int64_t c = orc_cidx(s, i, j, k);
+                const float fv = fwd[c];
+                const float corrected = fv + (src[c] - bwd) * 0.5f;
+                dst[c] = mc_clamp(src, s, co, h, tf, corrected, fv);
+            }
+    free(fwd);
+}
+
+/* ---- trace (advection_solver3.js:84-88): backTrace of each point through
+ *      the MAC velocity with the given direction, no collider.  Exposed so
+ *      the MacCormack clamp's corners can be recomputed outside the
+ *      oracle. ---- */
+void orc_trace3(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
+                const float h[3], const float origin[3], float dt, int max_sub, float direction) {
+    float ou[3], ov[3], ow[3];
+    mac_origins(origin, h, ou, ov, ow);
+    const float hmin = fminf(h[0], fminf(h[1], h[2]));
+    for (int64_t p = 0; p < count; ++p) {
+        float tr[3];
+        back_trace(u, v, w, n, h, ou, ov, ow, &pos[3 * p], dt, hmin, max_sub, direction, tr);
+        pos[3 * p] = tr[0];
+        pos[3 * p + 1] = tr[1];
+        pos[3 * p + 2] = tr[2];
+    }
 }

@@ -471,12 +471,33 @@ def advect_particles3(pos, u, v, w, n, h, origin, dt, max_substeps=32):
     return pos
 
 
-def advect_face3(u, v, w, n, h, origin, dt, max_substeps=32):
+def advect_face3(u, v, w, n, h, origin, dt, max_substeps=32, order=1):
     arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
     outs = [np.zeros_like(a) for a in arrs]
-    lib().orc_advect_face3(*[_p(a) for a in arrs], *[_p(a) for a in outs], _i3(n), _f3(h), _f3(origin),
-                           C.c_float(dt), int(max_substeps))
+    fn = lib().orc_advect_face3 if order == 1 else lib().orc_advect_face3_mc
+    fn(*[_p(a) for a in arrs], *[_p(a) for a in outs], _i3(n), _f3(h), _f3(origin), C.c_float(dt), int(max_substeps))
     return outs
+
+
+def advect_faces3_vd(vel, data, n, h, origin, dt, max_substeps=32, order=1):
+    """advectFaceCentered3 of the face grid `data` through the velocity grid
+    `vel` (advection_solver3.js:130-282), order 1 or 2."""
+    keep = [np.ascontiguousarray(a, np.float32) for a in vel] + [np.ascontiguousarray(a, np.float32) for a in data]
+    V = (C.c_void_p * 3)(*[_p(a) for a in keep[:3]])
+    D = (C.c_void_p * 3)(*[_p(a) for a in keep[3:]])
+    outs = [np.zeros_like(a) for a in keep[3:]]
+    Oa = (C.c_void_p * 3)(*[_p(a) for a in outs])
+    fn = lib().orc_advect_faces3_vd if order == 1 else lib().orc_advect_faces3_mc_vd
+    fn(V, D, Oa, _i3(n), _f3(h), _f3(origin), C.c_float(dt), int(max_substeps))
+    return outs
+
+
+def trace3(pos, u, v, w, n, h, origin, dt, direction=1.0, max_substeps=32):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_trace3(_p(pos), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin),
+                     C.c_float(dt), int(max_substeps), C.c_float(direction))
+    return pos
 
 
 def two_phase_beta3(frac, n, rho_gas, rho_liquid):
@@ -669,12 +690,12 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
 
 
 # ---------------------------------------------------------------- smoke
-def advect_scalar3(src, u, v, w, n, h, origin, dt, max_substeps=32):
+def advect_scalar3(src, u, v, w, n, h, origin, dt, max_substeps=32, order=1):
     src = np.ascontiguousarray(src, np.float32)
     out = np.empty_like(src)
     arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
-    lib().orc_advect_scalar3(_p(src), _p(out), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin), C.c_float(dt),
-                             int(max_substeps))
+    fn = lib().orc_advect_scalar3 if order == 1 else lib().orc_advect_scalar3_mc
+    fn(_p(src), _p(out), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin), C.c_float(dt), int(max_substeps))
     return out
 
 
@@ -694,7 +715,7 @@ def source_sphere3(d, n, h, origin, center, radius, value):
 
 
 def smoke3_step(st, n, h, origin, dt, a=-0.000625, b=5.0, tamb=0.0, up=(0.0, 1.0, 0.0), smoke_decay=0.001,
-                temperature_decay=0.001, flags=63, levels=1, tol=1e-6, max_substeps=32, max_plausible=1e6):
+                temperature_decay=0.001, flags=63, levels=1, tol=1e-6, max_substeps=32, max_plausible=1e6, order=1):
     """One createGridSmokeSolver3 step (grid_smoke_solver3.js:110-130 over
     grid_solver3.js:128-134) composed from the oracle's stages; st holds u, v,
     w, dens, temp, pressure and the last accepted pressure."""
@@ -712,11 +733,11 @@ def smoke3_step(st, n, h, origin, dt, a=-0.000625, b=5.0, tamb=0.0, up=(0.0, 1.0
         diag["rejected"] = False
     vel = pressure3_correct(n, h, x, vel, suffix="f32")
     vel = constrain_velocity3(vel, n, h, origin, closed_flags=flags)
-    vel = advect_face3(*vel, n, h, origin, dt, max_substeps)
+    vel = advect_face3(*vel, n, h, origin, dt, max_substeps, order=order)
     vel = constrain_velocity3(vel, n, h, origin, closed_flags=flags)
-    raw = advect_scalar3(st["dens"], *vel, n, h, origin, dt, max_substeps)
+    raw = advect_scalar3(st["dens"], *vel, n, h, origin, dt, max_substeps, order=order)
     dens = (raw * np.float32(np.float32(1.0) - np.float32(smoke_decay))).astype(np.float32)
-    raw = advect_scalar3(st["temp"], *vel, n, h, origin, dt, max_substeps)
+    raw = advect_scalar3(st["temp"], *vel, n, h, origin, dt, max_substeps, order=order)
     temp = (raw * np.float32(np.float32(1.0) - np.float32(temperature_decay))).astype(np.float32)
     st.update(u=vel[0], v=vel[1], w=vel[2], dens=dens, temp=temp, pressure=x)
     return diag
@@ -776,14 +797,37 @@ def sample_cubic2(d, shape, data_origin, h, x, y):
     return lib().orc_sample_cubic2(_p(d), _i2(shape), _f2(data_origin), _f2(h), C.c_float(x), C.c_float(y))
 
 
-def advect_face2(u, v, n, h, origin, dt, sdf=None, max_substeps=32):
+def advect_face2(u, v, n, h, origin, dt, sdf=None, max_substeps=32, order=1, out=None):
+    """out: the output's previous contents (kept on faces inside the
+    collider); defaults to the input."""
     u = np.ascontiguousarray(u, np.float32)
     v = np.ascontiguousarray(v, np.float32)
-    uo, vo = u.copy(), v.copy()
+    uo, vo = (u.copy(), v.copy()) if out is None else [np.ascontiguousarray(a, np.float32).copy() for a in out]
     sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
-    lib().orc_advect_face2(_p(u), _p(v), _i2(n), _f2(h), _f2(origin), _p(sdf), C.c_float(dt), int(max_substeps),
-                           _p(uo), _p(vo))
+    fn = lib().orc_advect_face2 if order == 1 else lib().orc_advect_face2_mc
+    fn(_p(u), _p(v), _i2(n), _f2(h), _f2(origin), _p(sdf), C.c_float(dt), int(max_substeps), _p(uo), _p(vo))
     return [uo, vo]
+
+
+def advect_faces2_vd(vel, data, n, h, origin, dt, sdf=None, max_substeps=32, order=1, out=None):
+    """advectFaceCentered2 of the face grid `data` through `vel` with the
+    collider; out: the output's previous contents (default: data)."""
+    u, v = [np.ascontiguousarray(a, np.float32) for a in vel]
+    du, dv = [np.ascontiguousarray(a, np.float32) for a in data]
+    uo, vo = (du.copy(), dv.copy()) if out is None else [np.ascontiguousarray(a, np.float32).copy() for a in out]
+    sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
+    lib().orc_advect_faces2_vd(_p(u), _p(v), _p(du), _p(dv), _i2(n), _f2(h), _f2(origin), _p(sdf), C.c_float(dt),
+                               int(max_substeps), int(order), _p(uo), _p(vo))
+    return [uo, vo]
+
+
+def trace2(pos, u, v, n, h, origin, dt, sdf=None, direction=1.0, max_substeps=32):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    u, v = [np.ascontiguousarray(a, np.float32) for a in (u, v)]
+    sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
+    lib().orc_trace2(_p(pos), C.c_int64(pos.size // 2), _p(u), _p(v), _i2(n), _f2(h), _f2(origin), _p(sdf),
+                     C.c_float(dt), int(max_substeps), C.c_float(direction))
+    return pos
 
 
 def constrain_velocity2(vel, n, h, origin, closed_flags=15, sdf=None, friction=0.5, depth=5, inflows=()):
@@ -847,7 +891,8 @@ def adaptive_substeps(max_velocity, min_spacing, frame_dt, courant=5.0, max_sub=
 
 
 def solver2_step(st, n, h, origin, dt, sdf=None, inflows=(), outflow_sdf=None, flags=15, levels=1, nu=(2, 2),
-                 n_coarsest=20, tol=1e-6, max_plausible=1e6, advect_collider=True, max_substeps=32, depth=5):
+                 n_coarsest=20, tol=1e-6, max_plausible=1e6, advect_collider=True, max_substeps=32, depth=5,
+                 order=1):
     """One createGridSolver2 step (grid_solver2.js:235-255, no external
     force): projection, constrain, advection of a copy, outflow velocity
     condition, constrain.  st holds u, v, pressure and the last accepted
@@ -868,7 +913,8 @@ def solver2_step(st, n, h, origin, dt, sdf=None, inflows=(), outflow_sdf=None, f
         diag["rejected"] = False
     vel = pressure2_correct(n, h, x, vel)
     vel = constrain_velocity2(vel, n, h, origin, flags, sdf, depth=depth, inflows=inflows)
-    vel = advect_face2(vel[0], vel[1], n, h, origin, dt, sdf if advect_collider else None, max_substeps)
+    vel = advect_face2(vel[0], vel[1], n, h, origin, dt, sdf if advect_collider else None, max_substeps,
+                       order=order)
     if outflow_sdf is not None:
         vel = outflow_bc2(vel, n, h, origin, dt, outflow_sdf)
     vel = constrain_velocity2(vel, n, h, origin, flags, sdf, depth=depth, inflows=inflows)

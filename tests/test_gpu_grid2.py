@@ -63,6 +63,35 @@ def test_advect_faces2_matches_oracle(flux, G):
         np.testing.assert_array_equal(fout[1].to_array(), vo)
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("with_collider", [False, True])
+def test_advection_solver2_matches_oracle(flux, G, order, with_collider):
+    """createSemiLagrangianAdvectionSolver2 with a separate advected grid,
+    order 1 and MacCormack, with and without the collider; faces inside the
+    collider keep the output's previous value"""
+    F = flux
+    n = (48, 32)
+    u, v = smooth_velocity(n, 7)
+    rng = np.random.default_rng(8)
+    du = (np.sin(0.4 * np.arange(u.size)) + 2.0 * (np.arange(u.size) % 49 > 24) + rng.uniform(-0.05, 0.05, u.size))
+    dv = (np.cos(0.3 * np.arange(v.size)) + rng.uniform(-0.05, 0.05, v.size))
+    du, dv = du.astype(np.float32), dv.astype(np.float32)
+    col = sdf_field(F, G, n, [scenes.circle_polygon(20.0, 16.0, 5.0, 48)]) if with_collider else None
+    vel = (F.array_n("float", G.face_shape2(n, 0), u), F.array_n("float", G.face_shape2(n, 1), v))
+    din = (F.array_n("float", G.face_shape2(n, 0), du), F.array_n("float", G.face_shape2(n, 1), dv))
+    prev = [np.full(u.size, 7.25, np.float32), np.full(v.size, -3.5, np.float32)]
+    out = (F.array_n("float", G.face_shape2(n, 0), prev[0]), F.array_n("float", G.face_shape2(n, 1), prev[1]))
+    adv = G.SemiLagrangianAdvectionSolver2(G.grid2(n), vel, 0.9, collider=col, order=order)
+    for dt in (0.9, 2.6):
+        adv.set_dt(dt)
+        adv.advect_face_centered2(din, out)
+        want = O.advect_faces2_vd([u, v], [du, dv], n, (1.0, 1.0), (0.0, 0.0), dt,
+                                  sdf=col.sdf.to_array() if col is not None else None, order=order, out=prev)
+        np.testing.assert_array_equal(out[0].to_array(), want[0])
+        np.testing.assert_array_equal(out[1].to_array(), want[1])
+        prev = want
+
+
 def test_constrain2_matches_oracle(flux, G):
     F = flux
     n = (48, 32)
@@ -139,8 +168,8 @@ def test_pressure2_matches_oracle(flux, G):
     np.testing.assert_array_equal(vel[1].to_array(), want[1])
 
 
-def run_karman(F, G, scene, frames):
-    solver, stepper, keep = G.create_karman2d(scene)
+def run_karman(F, G, scene, frames, order=1):
+    solver, stepper, keep = G.create_karman2d(scene, order=order)
     st = {"u": scene["u0"].copy(), "v": scene["v0"].copy(),
           "pressure": np.zeros(int(np.prod(scene["resolution"])), np.float32)}
     st["p_accepted"] = st["pressure"].copy()
@@ -152,7 +181,7 @@ def run_karman(F, G, scene, frames):
                                         courant=scene["courant"], sdf=col,
                                         inflows=[(inflow, scene["inflow_velocity"], "set")], outflow_sdf=outflow,
                                         flags=scene["closed_flags"], levels=scene["multigrid"]["levels"],
-                                        tol=scene["tolerance"])
+                                        tol=scene["tolerance"], order=order)
         assert k == len(diags) and np.float32(dt) == np.float32(stepper.state["lastSubDt"])
         for d, od in zip(diags, odiags):
             assert d["converged"] and od["converged"] and not d["rejected"]
@@ -168,6 +197,11 @@ def run_karman(F, G, scene, frames):
 def test_karman_reduced_frames_match_oracle(flux, G):
     scene = scenes.karman2d(nx=64, ny=32, radius=4.0, center=(16.0, 16.0), levels=3)
     run_karman(flux, G, scene, 4)
+
+
+def test_karman_maccormack_frames_match_oracle(flux, G):
+    scene = scenes.karman2d(nx=64, ny=32, radius=4.0, center=(16.0, 16.0), levels=3)
+    run_karman(flux, G, scene, 3, order=2)
 
 
 def test_karman_full_size_frames_match_oracle(flux, G):

@@ -1,7 +1,8 @@
 """createGridSmokeSolver3 on the GPU against the oracle's composed step (GPU tests).
 
 A 24^3 closed box with a hot, dense ball re-injected every step (the smoke
-plume scene at reduced size), four steps: velocity, density and
+plume scene at reduced size), four steps with order-1 and with MacCormack
+advection: velocity, density and
 temperature agree to 1e-4 of their norms (the projections agree to the PCG
 tolerance; everything else is the same f32 arithmetic in the same order).
 """
@@ -14,11 +15,13 @@ from tsl_array_n_amd.smoke import GridSmokeSolver3
 pytestmark = pytest.mark.gpu
 
 
-def test_smoke_steps_match_oracle(flux):
+@pytest.mark.parametrize("order", [1, 2])
+def test_smoke_steps_match_oracle(flux, order):
     n, h, dt = (24, 24, 24), (1.0, 1.0, 1.0), 0.25
-    kw = dict(a=-0.05, b=0.1)
+    kw = dict(a=-0.05, b=0.1, order=order)
     s = GridSmokeSolver3(resolution=n, grid_spacing=h, dt=dt, buoyancy_smoke_density_factor=-0.05,
-                         buoyancy_temperature_factor=0.1, pressure={"multigrid": {"levels": 3}})
+                         buoyancy_temperature_factor=0.1, pressure={"multigrid": {"levels": 3}},
+                         advection={"order": order})
     cells = int(np.prod(n))
     st = {"dens": np.zeros(cells, np.float32), "temp": np.zeros(cells, np.float32),
           "pressure": np.zeros(cells, np.float32), "p_accepted": np.zeros(cells, np.float32)}

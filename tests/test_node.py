@@ -28,7 +28,7 @@ def run_node(script, *args, timeout=120):
 def test_api_surface_and_host_helpers(tmp_path):
     assert os.path.exists(ADDON), "build the addon with __graft_entry__.build()"
     out = tmp_path / "api.json"
-    run_node("check_api.mjs", out)
+    run_node("check_api.mjs", out, os.path.join(GOLDEN, "stalled_system", "frame-110"))
     d = json.load(open(out))
     for name in ["init", "arrayN", "array0", "createLaplacianOperator", "createMultigridPreconditioner",
                  "createJacobiPreconditioner", "createIdentityPreconditioner",
@@ -39,11 +39,16 @@ def test_api_surface_and_host_helpers(tmp_path):
                  "createGridBlockedBoundaryConditionSolver3", "createGridFlipSolver3", "createGridSmokeSolver3",
                  "createFaceCenteredGrid2", "createSDFStaticCollider2", "createSDFInflow2", "createSDFOutflow2",
                  "createGridSolver2", "createGridAdaptiveTimeStep2", "computeAdaptiveSubSteps",
-                 "createMaxAbsReducer"]:
+                 "createMaxAbsReducer", "exportSystem", "loadSystem", "createSemiLagrangianAdvectionSolver3",
+                 "createCellCenteredScalarGrid3"]:
         assert name in d["exports"], name
     assert d["strides"] == [1, 4, 12]  # TA/test/array.test.js:41-93
     assert "1D/2D/3D" in d["shapeError"]
     assert "flux_init" in d["initError"]
+    fx = d["fixture"]
+    assert fx["resolution"] == [48, 24, 24] and fx["bLength"] == 48 * 24 * 24 and fx["weightU"] == 49 * 24 * 24
+    assert abs(fx["bNorm"] - 3.888) < 5e-3 and fx["iterations"] == 15
+    assert d["substeps"][0] == 4  # cfl.test.js:31-39
     golden = json.load(open(os.path.join(GOLDEN, "mulberry32.json")))
     np.testing.assert_array_equal(d["random"], golden["vectors"][0]["values"][:16])
     ref = O.flip_box_seed3([0, 0, 0], [0.4, 0.6, 1.0], [1 / 16] * 3)
@@ -148,6 +153,21 @@ def test_smoke_through_node_matches_oracle(tmp_path):
     for name, key in (("density", "dens"), ("temperature", "temp")):
         g = np.fromfile(tmp_path / (name + ".bin"), np.float32)
         assert np.linalg.norm(g - st[key]) <= 1e-4 * np.linalg.norm(st[key])
+
+
+@pytest.mark.gpu
+def test_maccormack_advection_through_node_matches_oracle(tmp_path):
+    """createSemiLagrangianAdvectionSolver3 (order 2) from Node, dt a live
+    array0 field changed between the two dispatches: bit for bit the oracle."""
+    run_node("advect_step.mjs", tmp_path, timeout=300)
+    n, h, o = (20, 16, 12), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    rd = lambda name: np.fromfile(tmp_path / (name + ".bin"), np.float32)
+    V = [rd("vel" + c) for c in "UVW"]
+    D = [rd("data" + c) for c in "UVW"]
+    want = O.advect_faces3_vd(V, D, n, h, o, 1.3, order=2)
+    for a, c in enumerate("UVW"):
+        np.testing.assert_array_equal(rd("out" + c), want[a])
+    np.testing.assert_array_equal(rd("outScalar"), O.advect_scalar3(rd("scalar"), *V, n, h, o, 2.2, order=2))
 
 
 @pytest.mark.gpu

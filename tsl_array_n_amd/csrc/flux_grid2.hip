@@ -108,7 +108,7 @@ __device__ __forceinline__ float cubic2(const float* __restrict__ d, int s0, int
 // backTrace with the collider (advection_solver2.js:81-127)
 __device__ void back_trace2(const G2& g, const float* __restrict__ u, const float* __restrict__ v,
                             const float* __restrict__ sdf, float sx, float sy, float dt, int max_sub, float& ox,
-                            float& oy) {
+                            float& oy, float direction = 1.0f) {
     const float hmin = fminf(g.h0, g.h1);
     float px = sx, py = sy, rem = dt;
     for (int s = 0; s < max_sub; ++s) {
@@ -120,9 +120,9 @@ __device__ void back_trace2(const G2& g, const float* __restrict__ u, const floa
         if (ns < 1.0f) ns = 1.0f;
         const float sub = rem / ns;
         const float half = sub * 0.5f;
-        const float mx = px - (v0[0] * half) * 1.0f, my = py - (v0[1] * half) * 1.0f;
+        const float mx = px - (v0[0] * half) * direction, my = py - (v0[1] * half) * direction;
         mac2(g, u, v, mx, my, mv);
-        const float nx = px - (mv[0] * sub) * 1.0f, ny = py - (mv[1] * sub) * 1.0f;
+        const float nx = px - (mv[0] * sub) * direction, ny = py - (mv[1] * sub) * direction;
         const float phi0 = sdf ? sdf_at(g, sdf, px, py) : 1.0f;
         const float phi1 = sdf ? sdf_at(g, sdf, nx, ny) : 1.0f;
         if (phi1 <= 0.0f) {
@@ -182,8 +182,11 @@ __global__ void k_polygon_sdf2(int n0, int n1, double h0, double h1, double o0, 
 }
 
 // advectFaceCentered2, order 1 (advection_solver2.js:263-320): U faces then
-// V faces in one grid; a face inside the collider keeps the output's value.
+// V faces in one grid; the trace follows the velocity (u, v), the value is
+// the bicubic sample of the advected grid (du, dv); a face inside the
+// collider keeps the output's value.
 __global__ void k_advect_face2(G2 g, const float* __restrict__ u, const float* __restrict__ v,
+                               const float* __restrict__ du, const float* __restrict__ dv,
                                const float* __restrict__ sdf, float dt, int max_sub, float* __restrict__ uo,
                                float* __restrict__ vo) {
     const int nu = (g.n0 + 1) * g.n1;
@@ -199,9 +202,94 @@ __global__ void k_advect_face2(G2 g, const float* __restrict__ u, const float* _
     float tx, ty;
     back_trace2(g, u, v, sdf, px, py, dt, max_sub, tx, ty);
     if (axis == 0)
-        uo[f] = cubic2(u, g.n0 + 1, g.n1, g.o0 + 0.0f, g.o1 + g.h1 * 0.5f, g.h0, g.h1, tx, ty);
+        uo[f] = cubic2(du, g.n0 + 1, g.n1, g.o0 + 0.0f, g.o1 + g.h1 * 0.5f, g.h0, g.h1, tx, ty);
     else
-        vo[f] = cubic2(v, g.n0, g.n1 + 1, g.o0 + g.h0 * 0.5f, g.o1 + 0.0f, g.h0, g.h1, tx, ty);
+        vo[f] = cubic2(dv, g.n0, g.n1 + 1, g.o0 + g.h0 * 0.5f, g.o1 + 0.0f, g.h0, g.h1, tx, ty);
+}
+
+// the data grid of face component `axis`: shape and data origin
+__device__ __forceinline__ void face_grid2(const G2& g, int axis, int& s0, int& s1, float& ox, float& oy) {
+    s0 = g.n0 + (axis == 0);
+    s1 = g.n1 + (axis == 1);
+    ox = axis == 0 ? g.o0 + 0.0f : g.o0 + g.h0 * 0.5f;
+    oy = axis == 0 ? g.o1 + g.h1 * 0.5f : g.o1 + 0.0f;
+}
+
+// advectFaceCentered2, order 2 (MacCormack), forward (advection_solver2.js:
+// 322-352): order 1, the input inside the collider; with the clamp range of
+// the same trace (gatherLocalMinMax, 221-248): min / max of the input over
+// the 4 bilinear corners whose face positions lie outside the collider
+// (lo > hi when none does, so the correction is never kept)
+__global__ void k_mc_forward2(G2 g, const float* __restrict__ u, const float* __restrict__ v,
+                              const float* __restrict__ du, const float* __restrict__ dv,
+                              const float* __restrict__ sdf, float dt, int max_sub, float* __restrict__ fu,
+                              float* __restrict__ fv, float2* __restrict__ ru, float2* __restrict__ rv) {
+    const int nu = (g.n0 + 1) * g.n1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nu + g.n0 * (g.n1 + 1)) return;
+    const int axis = t < nu ? 0 : 1;
+    const int f = axis == 0 ? t : t - nu;
+    int s0, s1;
+    float ox, oy;
+    face_grid2(g, axis, s0, s1, ox, oy);
+    const float* __restrict__ d = axis == 0 ? du : dv;
+    float* __restrict__ fo = axis == 0 ? fu : fv;
+    const int i = f % s0, j = f / s0;
+    float px, py;
+    face_pos2(g, axis, i, j, px, py);
+    if (sdf && !(sdf_at(g, sdf, px, py) > 0.0f)) {
+        fo[f] = d[f];
+        return;
+    }
+    float tx, ty;
+    back_trace2(g, u, v, sdf, px, py, dt, max_sub, tx, ty);
+    fo[f] = cubic2(d, s0, s1, ox, oy, g.h0, g.h1, tx, ty);
+    const float gx = (tx - ox) / g.h0, gy = (ty - oy) / g.h1;
+    const int i0 = (int)floorf(gx), j0 = (int)floorf(gy);
+    const int ci[2] = {clampi(i0, 0, s0 - 1), clampi(i0 + 1, 0, s0 - 1)};
+    const int cj[2] = {clampi(j0, 0, s1 - 1), clampi(j0 + 1, 0, s1 - 1)};
+    float lo = 1e30f, hi = -1e30f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int ni = ci[c & 1], nj = cj[c >> 1];
+        const bool valid = sdf ? sdf_at(g, sdf, ox + g.h0 * (float)ni, oy + g.h1 * (float)nj) > 0.0f : true;
+        const float val = d[ni + s0 * nj];
+        if (valid) {
+            lo = fminf(lo, val);
+            hi = fmaxf(hi, val);
+        }
+    }
+    (axis == 0 ? ru : rv)[f] = make_float2(lo, hi);
+}
+
+// order 2, backward + correct + clamp (354-428): the forward result sampled
+// at the trace with direction -1; fwd + (in - bwd) / 2 where it lies in the
+// range, else fwd; a face inside the collider keeps the output's value
+__global__ void k_mc_correct2(G2 g, const float* __restrict__ u, const float* __restrict__ v,
+                              const float* __restrict__ du, const float* __restrict__ dv,
+                              const float* __restrict__ sdf, float dt, int max_sub, const float* __restrict__ fu,
+                              const float* __restrict__ fv, const float2* __restrict__ ru,
+                              const float2* __restrict__ rv, float* __restrict__ uo, float* __restrict__ vo) {
+    const int nu = (g.n0 + 1) * g.n1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nu + g.n0 * (g.n1 + 1)) return;
+    const int axis = t < nu ? 0 : 1;
+    const int f = axis == 0 ? t : t - nu;
+    int s0, s1;
+    float ox, oy;
+    face_grid2(g, axis, s0, s1, ox, oy);
+    const int i = f % s0, j = f / s0;
+    float px, py;
+    face_pos2(g, axis, i, j, px, py);
+    if (sdf && !(sdf_at(g, sdf, px, py) > 0.0f)) return;
+    const float* __restrict__ fw = axis == 0 ? fu : fv;
+    float tx, ty;
+    back_trace2(g, u, v, sdf, px, py, dt, max_sub, tx, ty, -1.0f);
+    const float bwd = cubic2(fw, s0, s1, ox, oy, g.h0, g.h1, tx, ty);
+    const float fval = fw[f];
+    const float corrected = fval + ((axis == 0 ? du : dv)[f] - bwd) * 0.5f;
+    const float2 r = (axis == 0 ? ru : rv)[f];
+    (axis == 0 ? uo : vo)[f] = (corrected >= r.x && corrected <= r.y) ? corrected : fval;
 }
 
 // fillUMarker/fillVMarker + markAndProjectU/V
@@ -547,6 +635,18 @@ struct flux_pressure2 {
     int maxiter = 0;
 };
 
+struct flux_advection2 {
+    flux_grid2 g{};
+    const flux_field* u = nullptr;  // the velocity grid bound at creation
+    const flux_field* v = nullptr;
+    const flux_field* sdf = nullptr;
+    float dt = 0.0f;
+    int max_sub = 32;
+    int order = 1;
+    float* fwd = nullptr;     // order 2: forward result of the U then V faces
+    float2* range = nullptr;  // order 2: clamp ranges
+};
+
 struct flux_solver2 {
     flux_solver2_desc d;
     flux_field* vel[2] = {nullptr, nullptr};
@@ -556,6 +656,7 @@ struct flux_solver2 {
     flux_boundary2* bnd = nullptr;
     flux_outflow2* out[4] = {nullptr, nullptr, nullptr, nullptr};
     flux_pressure2* pres = nullptr;
+    flux_advection2* adv = nullptr;  // through velocityPrev (grid_solver2.js:198-199)
 };
 
 extern "C" {
@@ -605,9 +706,96 @@ int flux_advect_faces2(const flux_grid2* g, const flux_field* u, const flux_fiel
     if (collider_sdf) FLUX_TRY(check_field(collider_sdf, FLUX_F32, (int64_t)g->n[0] * g->n[1], "collider sdf"));
     const int64_t nf = all_faces2(g->n);
     k_advect_face2<<<blocks_for(nf, GTPB), GTPB, 0, g_stream>>>(
-        make_g2(g), (const float*)u->d, (const float*)v->d, collider_sdf ? (const float*)collider_sdf->d : nullptr,
-        dt, max_substeps, (float*)u_out->d, (float*)v_out->d);
+        make_g2(g), (const float*)u->d, (const float*)v->d, (const float*)u->d, (const float*)v->d,
+        collider_sdf ? (const float*)collider_sdf->d : nullptr, dt, max_substeps, (float*)u_out->d,
+        (float*)v_out->d);
     FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int flux_advection2_create(const flux_grid2* g, const flux_field* u, const flux_field* v,
+                           const flux_field* collider_sdf, float dt, int max_substeps, int order,
+                           flux_advection2** out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(out, "createSemiLagrangianAdvectionSolver2: NULL argument");
+    FLUX_TRY(check_grid2(g, "createSemiLagrangianAdvectionSolver2"));
+    FLUX_TRY(check_faces2(g, u, v, "createSemiLagrangianAdvectionSolver2: velocity"));
+    FLUX_REQUIRE(order == 1 || order == 2, "createSemiLagrangianAdvectionSolver2: order must be 1 or 2");
+    FLUX_REQUIRE(max_substeps >= 1 && max_substeps <= 1024,
+                 "createSemiLagrangianAdvectionSolver2: maxSubsteps out of range");
+    if (collider_sdf) FLUX_TRY(check_field(collider_sdf, FLUX_F32, (int64_t)g->n[0] * g->n[1], "collider sdf"));
+    const int64_t nf = all_faces2(g->n);
+    flux_advection2* a = new flux_advection2();
+    a->g = *g;
+    a->u = u;
+    a->v = v;
+    a->sdf = collider_sdf;
+    a->dt = dt;
+    a->max_sub = max_substeps;
+    a->order = order;
+    if (order == 2) {
+        a->fwd = (float*)dmalloc(sizeof(float) * nf);
+        a->range = (float2*)dmalloc(sizeof(float2) * nf);
+        if (!a->fwd || !a->range) {
+            flux_advection2_destroy(a);
+            return set_error(FLUX_ERR_HIP, "createSemiLagrangianAdvectionSolver2: out of device memory");
+        }
+    }
+    *out = a;
+    return FLUX_OK;
+}
+
+int flux_advection2_set_dt(flux_advection2* a, float dt) {
+    FLUX_REQUIRE(a, "advection: NULL solver");
+    a->dt = dt;
+    return FLUX_OK;
+}
+
+// advectFaceCentered2(input, output) of the solver (advection_solver2.js:
+// 263-430) on the stream, no synchronisation
+static int advection2_faces_async(flux_advection2* a, const flux_field* in_u, const flux_field* in_v,
+                                  flux_field* out_u, flux_field* out_v) {
+    FLUX_REQUIRE(a, "advectFaceCentered2: NULL solver");
+    FLUX_TRY(check_faces2(&a->g, in_u, in_v, "advectFaceCentered2: input"));
+    FLUX_TRY(check_faces2(&a->g, out_u, out_v, "advectFaceCentered2: output"));
+    for (const flux_field* o : {out_u, out_v})
+        FLUX_REQUIRE(o->d != in_u->d && o->d != in_v->d && o->d != a->u->d && o->d != a->v->d,
+                     "advectFaceCentered2: output must not alias the input or velocity");
+    const int64_t nf = all_faces2(a->g.n);
+    const int nu = (int)faces2(a->g.n, 0);
+    const G2 g = make_g2(&a->g);
+    const float* sdf = a->sdf ? (const float*)a->sdf->d : nullptr;
+    const float *u = (const float*)a->u->d, *v = (const float*)a->v->d;
+    const float *du = (const float*)in_u->d, *dv = (const float*)in_v->d;
+    if (a->order == 1) {
+        k_advect_face2<<<blocks_for(nf, GTPB), GTPB, 0, g_stream>>>(g, u, v, du, dv, sdf, a->dt, a->max_sub,
+                                                                     (float*)out_u->d, (float*)out_v->d);
+    } else {
+        float *fu = a->fwd, *fv = a->fwd + nu;
+        float2 *ru = a->range, *rv = a->range + nu;
+        k_mc_forward2<<<blocks_for(nf, GTPB), GTPB, 0, g_stream>>>(g, u, v, du, dv, sdf, a->dt, a->max_sub, fu, fv,
+                                                                    ru, rv);
+        k_mc_correct2<<<blocks_for(nf, GTPB), GTPB, 0, g_stream>>>(g, u, v, du, dv, sdf, a->dt, a->max_sub, fu, fv,
+                                                                    ru, rv, (float*)out_u->d, (float*)out_v->d);
+    }
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int flux_advection2_faces(flux_advection2* a, const flux_field* in_u, const flux_field* in_v, flux_field* out_u,
+                          flux_field* out_v) {
+    FLUX_REQUIRE_INIT();
+    FLUX_TRY(advection2_faces_async(a, in_u, in_v, out_u, out_v));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    return FLUX_OK;
+}
+
+int flux_advection2_destroy(flux_advection2* a) {
+    if (!a) return FLUX_OK;
+    if (g_init) (void)hipStreamSynchronize(g_stream);
+    if (a->fwd) dfree(a->fwd);
+    if (a->range) dfree(a->range);
+    delete a;
     return FLUX_OK;
 }
 
@@ -951,6 +1139,7 @@ void flux_solver2_desc_default(flux_solver2_desc* d) {
     d->outflow_velocity_bc = 1;
     d->max_substeps = 32;
     d->extrapolation_depth = 5;
+    d->advection_order = 1;
     flux_pressure2_desc pd;
     flux_pressure2_desc_default(&pd);
     d->multigrid = pd.multigrid;
@@ -965,6 +1154,7 @@ void flux_solver2_desc_default(flux_solver2_desc* d) {
 int flux_solver2_destroy(flux_solver2* s) {
     if (!s) return FLUX_OK;
     flux_pressure2_destroy(s->pres);
+    flux_advection2_destroy(s->adv);
     for (int q = 0; q < 4; ++q) flux_outflow2_destroy(s->out[q]);
     flux_boundary2_destroy(s->bnd);
     for (int a = 0; a < 2; ++a) {
@@ -1005,6 +1195,8 @@ int flux_solver2_create(const flux_solver2_desc* d, flux_solver2** out) {
         S2_CHECK(flux_field_upload(s->vel[a], std::vector<float>(shape[0] * shape[1], 0.0f).data(),
                                    sizeof(float) * shape[0] * shape[1]));
     }
+    S2_CHECK(flux_advection2_create(&g, s->prev[0], s->prev[1], d->advect_collider ? d->collider_sdf : nullptr, 0.0f,
+                                    d->max_substeps, d->advection_order == 0 ? 1 : d->advection_order, &s->adv));
     S2_CHECK(flux_boundary2_create(&g, s->vel, d->collider_sdf, d->friction, &s->bnd));
     S2_CHECK(flux_boundary2_set_closed(s->bnd, d->closed_flags));
     for (int q = 0; q < d->n_inflows; ++q)
@@ -1058,8 +1250,8 @@ int flux_solver2_step(flux_solver2* s, float dt, flux_pressure_diag* diag) {
                                                                (const float*)s->vel[1]->d, (float*)s->prev[0]->d,
                                                                (float*)s->prev[1]->d);
     FLUX_LAUNCH_CHECK();
-    FLUX_TRY(flux_advect_faces2(&d.grid, s->prev[0], s->prev[1], d.advect_collider ? d.collider_sdf : nullptr, dt,
-                                d.max_substeps, s->vel[0], s->vel[1]));
+    FLUX_TRY(flux_advection2_set_dt(s->adv, dt));
+    FLUX_TRY(advection2_faces_async(s->adv, s->prev[0], s->prev[1], s->vel[0], s->vel[1]));
     if (d.outflow_velocity_bc)
         for (int q = 0; q < d.n_outflows; ++q) FLUX_TRY(flux_outflow2_apply(s->out[q], s->vel[0], s->vel[1], dt));
     FLUX_TRY(flux_boundary2_constrain(s->bnd, d.extrapolation_depth));

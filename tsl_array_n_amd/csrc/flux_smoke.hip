@@ -2,9 +2,10 @@
 // pressure projection (packages/fluxflow/src/grid/grid_smoke_solver3.js:25-154
 // over createGridSolver3, grid_solver3.js:45-149): buoyancy as an external
 // force (external_force_solver3.js:19-47), constrainVelocity, projection,
-// semi-Lagrangian velocity advection (order 1), then density and temperature
-// advected through the new velocity (advectScalar3, advection_solver3.js:
-// 285-300) and decayed into the other half of their ping-pong pairs.
+// semi-Lagrangian velocity advection (order 1 or 2, through velocityPrev), then
+// density and temperature advected through the new velocity (advectScalar3,
+// advection_solver3.js:285-357) and decayed into the other half of their
+// ping-pong pairs.
 // Built with -ffp-contract=off.
 #include <cmath>
 #include <cstring>
@@ -41,23 +42,6 @@ __global__ void k_buoyancy(G3 g, int axis, const float* __restrict__ dens, const
     vel[f] = vel[f] + (up * fb) * dt;
 }
 
-// advectScalar3, order 1, no collider: the cell-centred value at the back
-// trace of the cell centre, monotone tricubic
-__global__ void k_advect_scalar(G3 g, Mac m, const float* __restrict__ src, float* __restrict__ dst, float dt,
-                                int max_sub) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.n0 * g.n1 * g.n2) return;
-    const int i = c % g.n0, j = (c / g.n0) % g.n1, k = c / (g.n0 * g.n1);
-    const int s[3] = {g.n0, g.n1, g.n2};
-    const float h[3] = {g.h0, g.h1, g.h2};
-    const float o[3] = {g.o0 + g.h0 * 0.5f, g.o1 + g.h1 * 0.5f, g.o2 + g.h2 * 0.5f};
-    const float pos[3] = {o[0] + (float)i * h[0], o[1] + (float)j * h[1], o[2] + (float)k * h[2]};
-    const float hmin = fminf(g.h0, fminf(g.h1, g.h2));
-    float tr[3];
-    back_trace(g, m, pos, dt, hmin, max_sub, 1.0f, tr);
-    dst[c] = sample_cubic(src, s, o, h, tr);
-}
-
 // buildDecayKernel (grid_smoke_solver3.js:100-108): state = raw * (1 - decay)
 __global__ void k_decay(const float* __restrict__ raw, float* __restrict__ state, int n, float decay) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -92,6 +76,8 @@ struct flux_smoke3 {
     flux_field* raw = nullptr;
     flux_boundary3* boundary = nullptr;
     flux_pressure3* pressure = nullptr;
+    flux_advection3* adv_vel = nullptr;     // through velocityPrev (grid_solver3.js:90-91)
+    flux_advection3* adv_scalar = nullptr;  // through the new velocity (grid_smoke_solver3.js:92)
     int active = 0;  // index of the current density/temperature state
 };
 
@@ -111,6 +97,7 @@ void flux_smoke3_desc_default(flux_smoke3_desc* d) {
     d->max_substeps = 32;
     d->extrapolation_depth = 5;
     flux_pressure3_desc_default(&d->pressure);
+    d->advection_order = 1;
 }
 
 int flux_smoke3_create(const flux_smoke3_desc* d, flux_smoke3** out) {
@@ -142,6 +129,11 @@ int flux_smoke3_create(const flux_smoke3_desc* d, flux_smoke3** out) {
         ok(flux_field_create(FLUX_F32, 3, cs, &s->temp[q]));
     }
     ok(flux_field_create(FLUX_F32, 3, cs, &s->raw));
+    if (rc == FLUX_OK) {
+        const int order = d->advection_order == 0 ? 1 : d->advection_order;
+        ok(advection3_create(&s->g, s->prev, d->dt, d->max_substeps, order, &s->adv_vel));
+        if (rc == FLUX_OK) ok(advection3_create(&s->g, s->vel, d->dt, d->max_substeps, order, &s->adv_scalar));
+    }
     if (rc == FLUX_OK) {
         ok(flux_boundary3_create(&s->g, s->vel, nullptr, nullptr, 0.5f, &s->boundary));
         if (rc == FLUX_OK) ok(flux_boundary3_set_closed(s->boundary, d->closed_domain_flags));
@@ -188,16 +180,13 @@ int flux_smoke3_step(flux_smoke3* s, flux_pressure_diag* diag) {
     FLUX_TRY(flux_boundary3_constrain(s->boundary, d.extrapolation_depth));
     // advection: velocityPrev = velocity; velocity = advect(velocityPrev)
     for (int a = 0; a < 3; ++a) FLUX_TRY(flux_field_copy(s->prev[a], s->vel[a]));
-    FLUX_TRY(flux_advect_faces(&s->g, s->prev, s->vel, d.dt, d.max_substeps));
+    FLUX_TRY(advection3_faces_async(s->adv_vel, s->prev, s->vel));
     FLUX_TRY(flux_boundary3_constrain(s->boundary, d.extrapolation_depth));
     // scalars through the new velocity, decayed into the other state
-    Mac m{(const float*)s->vel[0]->d, (const float*)s->vel[1]->d, (const float*)s->vel[2]->d};
     const unsigned gc = blocks_for(s->cells, KTPB);
-    k_advect_scalar<<<gc, KTPB, 0, g_stream>>>(g, m, (const float*)s->dens[cur]->d, (float*)s->raw->d, d.dt,
-                                               d.max_substeps);
+    FLUX_TRY(advection3_scalar_async(s->adv_scalar, s->dens[cur], s->raw));
     k_decay<<<gc, KTPB, 0, g_stream>>>((const float*)s->raw->d, (float*)s->dens[nxt]->d, (int)s->cells, d.smoke_decay);
-    k_advect_scalar<<<gc, KTPB, 0, g_stream>>>(g, m, (const float*)s->temp[cur]->d, (float*)s->raw->d, d.dt,
-                                               d.max_substeps);
+    FLUX_TRY(advection3_scalar_async(s->adv_scalar, s->temp[cur], s->raw));
     k_decay<<<gc, KTPB, 0, g_stream>>>((const float*)s->raw->d, (float*)s->temp[nxt]->d, (int)s->cells,
                                        d.temperature_decay);
     FLUX_LAUNCH_CHECK();
@@ -236,6 +225,8 @@ int flux_smoke3_destroy(flux_smoke3* s) {
     if (g_init) (void)hipStreamSynchronize(g_stream);
     if (s->pressure) flux_pressure3_destroy(s->pressure);
     if (s->boundary) flux_boundary3_destroy(s->boundary);
+    advection3_free(s->adv_vel);
+    advection3_free(s->adv_scalar);
     for (int a = 0; a < 3; ++a) {
         if (s->vel[a]) flux_field_destroy(s->vel[a]);
         if (s->prev[a]) flux_field_destroy(s->prev[a]);

@@ -17,4 +17,10 @@ struct DotOut {
 int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr, DotOut* dot);
 int64_t mg_cells(const flux_multigrid* mg);
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
+// semi-Lagrangian advection without the final stream synchronisation
+int advection3_create(const flux_grid3* g, const flux_field* const* vel, float dt, int max_substeps, int order,
+                      flux_advection3** out);
+int advection3_faces_async(flux_advection3* a, const flux_field* const* in, flux_field* const* out);
+int advection3_scalar_async(flux_advection3* a, const flux_field* in, flux_field* out);
+void advection3_free(flux_advection3* a);
 }  // namespace flux

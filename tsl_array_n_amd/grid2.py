@@ -85,6 +85,37 @@ def advect_face_centered2(g, vin, vout, dt, collider=None, max_substeps=32):
            "advectFaceCentered2")
 
 
+class SemiLagrangianAdvectionSolver2:
+    """createSemiLagrangianAdvectionSolver2({velocityGrid, collider, dt,
+    maxSubsteps, order}) (advection_solver2.js:174-526) for face grids: the
+    velocity pair and the collider are bound here; order 1 or 2 (MacCormack)."""
+
+    def __init__(self, g, velocity_grid, dt, collider=None, max_substeps=32, order=1):
+        if order not in (1, 2):
+            raise ValueError("createSemiLagrangianAdvectionSolver2: order must be 1 or 2")
+        self.velocity_grid = tuple(velocity_grid)
+        self.collider = collider
+        h = C.c_void_p()
+        _check(_lib.flux_advection2_create(C.byref(g), C.c_void_p(velocity_grid[0].h), C.c_void_p(velocity_grid[1].h),
+                                           C.c_void_p(collider.sdf.h if collider is not None else None),
+                                           C.c_float(dt), int(max_substeps), int(order), C.byref(h)),
+               "createSemiLagrangianAdvectionSolver2")
+        self.h = h.value
+
+    def set_dt(self, dt):
+        _check(_lib.flux_advection2_set_dt(C.c_void_p(self.h), C.c_float(dt)), "advection dt")
+
+    def advect_face_centered2(self, vin, vout):
+        _check(_lib.flux_advection2_faces(C.c_void_p(self.h), C.c_void_p(vin[0].h), C.c_void_p(vin[1].h),
+                                          C.c_void_p(vout[0].h), C.c_void_p(vout[1].h)), "advectFaceCentered2")
+
+    def __del__(self):
+        try:
+            _lib.flux_advection2_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
 def max_abs(fields):
     arr = (C.c_void_p * len(fields))(*[f.h for f in fields])
     out = C.c_float()
@@ -256,7 +287,8 @@ class Solver2Desc(C.Structure):
                 ("dirichlet_active", C.c_void_p), ("dirichlet_target", C.c_void_p), ("closed_flags", C.c_int),
                 ("extrapolation_depth", C.c_int), ("multigrid", MGOptions), ("preconditioner", C.c_int),
                 ("tolerance", C.c_double), ("max_iterations", C.c_int), ("max_iterations_factor", C.c_double),
-                ("relative_tolerance", C.c_int), ("max_plausible_pressure", C.c_double)]
+                ("relative_tolerance", C.c_int), ("max_plausible_pressure", C.c_double),
+                ("advection_order", C.c_int)]
 
 
 _S2 = {"u": 0, "v": 1, "u_prev": 2, "v_prev": 3, "pressure": 4, "dirichlet": 5}
@@ -264,7 +296,7 @@ _S2 = {"u": 0, "v": 1, "u_prev": 2, "v_prev": 3, "pressure": 4, "dirichlet": 5}
 
 class GridSolver2:
     """createGridSolver2({velocityGrid resolution, gridSpacing, origin, collider, inflows, outflows,
-    closedDomainBoundaryFlag, advection: {collider, maxSubsteps}, pressure: {...}}) with no external
+    closedDomainBoundaryFlag, advection: {collider, maxSubsteps, order}, pressure: {...}}) with no external
     force.  The solver owns its velocity (``velocity_grid`` = (U, V) fields, zero at creation);
     ``on_advance_time_step(dt)`` runs one step and returns the pressure diagnostics."""
 
@@ -280,8 +312,9 @@ class GridSolver2:
             d.collider_sdf = collider.sdf.h
             d.friction = collider.friction_coefficient
         adv = advection or {}
-        if adv.get("order", 1) != 1:
-            raise ValueError("createGridSolver2: only order-1 advection is built")
+        if adv.get("order", 1) not in (1, 2):
+            raise ValueError("createGridSolver2: advection.order must be 1 or 2")
+        d.advection_order = int(adv.get("order", 1))
         d.advect_collider = int(adv.get("collider") is not None)
         if adv.get("collider") is not None and adv["collider"] is not collider:
             raise ValueError("createGridSolver2: advection.collider must be the solver's collider")
@@ -377,10 +410,11 @@ class GridAdaptiveTimeStep2:
         return k, sub32
 
 
-def create_karman2d(scene):
+def create_karman2d(scene, order=1):
     """The 2-D Karman scene (scenes.karman2d) as a GridSolver2 with its
     collider, inflow strip and outflow slab, the initial velocity uploaded,
-    and its adaptive time step; returns (solver, stepper, keep)."""
+    and its adaptive time step (order: the advection's, 1 or MacCormack 2);
+    returns (solver, stepper, keep)."""
     nx, ny = scene["resolution"]
     h, o = scene["spacing"], scene["origin"]
     collider = SDFStaticCollider2(nx, ny, h[0], h[1], o[0], o[1])
@@ -391,7 +425,7 @@ def create_karman2d(scene):
     outflow.add_polygons(scene["outflow"])
     solver = GridSolver2((nx, ny), h, o, collider=collider, inflows=inflow, outflows=outflow,
                          closed_domain_boundary_flag=scene["closed_flags"],
-                         advection={"collider": collider if scene["advect_collider"] else None},
+                         advection={"collider": collider if scene["advect_collider"] else None, "order": order},
                          pressure={"multigrid": scene["multigrid"], "tolerance": scene["tolerance"]})
     u, v = solver.velocity_grid
     u.from_array(scene["u0"])

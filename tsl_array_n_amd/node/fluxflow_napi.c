@@ -32,7 +32,7 @@
         }                                                     \
     } while (0)
 
-enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE, K_S2 };
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE, K_S2, K_ADV3 };
 
 typedef struct {
     int kind;
@@ -55,6 +55,7 @@ static void fin_handle(napi_env env, void* data, void* hint) {
         case K_FLIP: flux_flip3_destroy((flux_flip3*)h->ptr); break;
         case K_SMOKE: flux_smoke3_destroy((flux_smoke3*)h->ptr); break;
         case K_S2: flux_solver2_destroy((flux_solver2*)h->ptr); break;
+        case K_ADV3: flux_advection3_destroy((flux_advection3*)h->ptr); break;
         default: break;
     }
     free(h);
@@ -591,6 +592,52 @@ static napi_value js_advect_faces(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+/* createSemiLagrangianAdvectionSolver3: (grid, [u, v, w], dt, maxSubsteps, order) */
+static napi_value js_advection3_create(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    args_n(env, info, 5, a);
+    flux_grid3 g;
+    const flux_field* v[3];
+    if (get_grid(env, a[0], &g) || get_field_array(env, a[1], v) != 3) {
+        napi_throw_type_error(env, NULL, "createSemiLagrangianAdvectionSolver3: bad arguments");
+        return NULL;
+    }
+    flux_advection3* adv = NULL;
+    CHECK_FLUX(flux_advection3_create(&g, v, (float)get_double(env, a[2], 0.0), get_int(env, a[3], 32),
+                                      get_int(env, a[4], 1), &adv));
+    return wrap(env, K_ADV3, adv);
+}
+
+static napi_value js_advection3_set_dt(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_advection3_set_dt((flux_advection3*)unwrap(env, a[0], K_ADV3, K_ADV3),
+                                      (float)get_double(env, a[1], 0.0)));
+    return undef(env);
+}
+
+static napi_value js_advection3_faces(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    const flux_field* vi[3];
+    const flux_field* vo[3];
+    if (get_field_array(env, a[1], vi) != 3 || get_field_array(env, a[2], vo) != 3) {
+        napi_throw_type_error(env, NULL, "advectFaceCentered3: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_advection3_faces((flux_advection3*)unwrap(env, a[0], K_ADV3, K_ADV3), vi,
+                                     (flux_field* const*)vo));
+    return undef(env);
+}
+
+static napi_value js_advection3_scalar(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    CHECK_FLUX(flux_advection3_scalar((flux_advection3*)unwrap(env, a[0], K_ADV3, K_ADV3), get_field(env, a[1]),
+                                      get_field(env, a[2])));
+    return undef(env);
+}
+
 static napi_value js_two_phase_beta(napi_env env, napi_callback_info info) {
     napi_value a[6];
     args_n(env, info, 6, a);
@@ -860,6 +907,8 @@ static napi_value js_smoke3_create(napi_env env, napi_callback_info info) {
     if ((v = prop(env, o, "preconditionerKind"))) d.pressure.preconditioner = get_int(env, v, d.pressure.preconditioner);
     if ((v = prop(env, o, "tolerance"))) d.pressure.tolerance = get_double(env, v, d.pressure.tolerance);
     if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.pressure.max_iterations = get_int(env, v, 0);
+    if ((v = prop(env, o, "advectionOrder"))) d.advection_order = get_int(env, v, 1);
+    if ((v = prop(env, o, "maxSubsteps"))) d.max_substeps = get_int(env, v, d.max_substeps);
     flux_smoke3* s = NULL;
     CHECK_FLUX(flux_smoke3_create(&d, &s));
     return wrap(env, K_SMOKE, s);
@@ -1010,6 +1059,7 @@ static napi_value js_solver2_create(napi_env env, napi_callback_info info) {
         d.advect_collider = b ? 1 : 0;
     }
     if ((v = prop(env, o, "maxSubsteps"))) d.max_substeps = get_int(env, v, d.max_substeps);
+    if ((v = prop(env, o, "advectionOrder"))) d.advection_order = get_int(env, v, 1);
     if ((v = prop(env, o, "inflows")) && !is_nullish(env, v)) {
         uint32_t n = 0;
         napi_get_array_length(env, v, &n);
@@ -1107,6 +1157,10 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("g2p", js_g2p);
     EXPORT("advectParticles", js_advect_particles);
     EXPORT("advectFaces", js_advect_faces);
+    EXPORT("advection3Create", js_advection3_create);
+    EXPORT("advection3SetDt", js_advection3_set_dt);
+    EXPORT("advection3Faces", js_advection3_faces);
+    EXPORT("advection3Scalar", js_advection3_scalar);
     EXPORT("twoPhaseBeta", js_two_phase_beta);
     EXPORT("extrapolatorCreate", js_extrapolator_create);
     EXPORT("extrapolatorRun", js_extrapolator_run);

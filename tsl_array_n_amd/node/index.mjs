@@ -464,6 +464,81 @@ export function advectFaceCentered3( input, output, dt, maxSubsteps = 32 ) {
 
 }
 
+// createCellCenteredScalarGrid3 (grid_data3.js:181-218): cell data at
+// origin + h/2
+export function createCellCenteredScalarGrid3( nx, ny, nz, hx, hy, hz, ox = 0, oy = 0, oz = 0 ) {
+
+	const grid = {
+		resolution: [ nx, ny, nz ], gridSpacing: [ hx, hy, hz ], origin: [ ox, oy, oz ],
+		dataSize: [ nx, ny, nz ], dataOrigin: [ ox + hx * 0.5, oy + hy * 0.5, oz + hz * 0.5 ],
+		data: arrayN( 'float', [ nx, ny, nz ] ), cellCentered: true
+	};
+	grid.clear = () => grid.data.fill( 0 );
+	return grid;
+
+}
+
+// createSemiLagrangianAdvectionSolver3 (advection_solver3.js:67-362), no
+// collider: velocityGrid is bound here (its contents are read at each
+// dispatch); dt is a number or an array0('float') field read at dispatch;
+// order 1 or 2 (MacCormack).  advectFaceCentered3 / advectScalar3 return
+// 0-arg dispatchers, as the reference's do.
+export function createSemiLagrangianAdvectionSolver3( { velocityGrid, collider, dt, maxSubsteps = 32, order = 1 } = {} ) {
+
+	if ( ! velocityGrid ) throw new Error( 'createSemiLagrangianAdvectionSolver3: options.velocityGrid is required.' );
+	if ( collider ) throw new Error( 'createSemiLagrangianAdvectionSolver3: a 3-D collider is not supported by this build.' );
+	if ( order !== 1 && order !== 2 ) throw new Error( 'createSemiLagrangianAdvectionSolver3: order must be 1 or 2.' );
+	const readDt = () => {
+
+		if ( typeof dt === 'number' ) return dt;
+		if ( dt && dt.handle ) {
+
+			const out = new Float32Array( 1 );
+			addon.fieldDownload( dt.handle, out );
+			return out[ 0 ];
+
+		}
+
+		throw new Error( 'createSemiLagrangianAdvectionSolver3: options.dt must be a number or an array0(\'float\') field.' );
+
+	};
+
+	const handle = addon.advection3Create( gridArg( velocityGrid ),
+		[ velocityGrid.dataU.handle, velocityGrid.dataV.handle, velocityGrid.dataW.handle ], readDt(), maxSubsteps, order );
+	const live = typeof dt !== 'number';
+	function advectFaceCentered3( input, output ) {
+
+		return () => {
+
+			if ( live ) addon.advection3SetDt( handle, readDt() );
+			addon.advection3Faces( handle, [ input.dataU.handle, input.dataV.handle, input.dataW.handle ],
+				[ output.dataU.handle, output.dataV.handle, output.dataW.handle ] );
+
+		};
+
+	}
+
+	function advectScalar3( input, output ) {
+
+		for ( const g of [ input, output ] ) {
+
+			if ( ! g.cellCentered ) throw new Error( 'advectScalar3: input and output must be cell-centred scalar grids (createCellCenteredScalarGrid3).' );
+
+		}
+
+		return () => {
+
+			if ( live ) addon.advection3SetDt( handle, readDt() );
+			addon.advection3Scalar( handle, input.data.handle, output.data.handle );
+
+		};
+
+	}
+
+	return { advectFaceCentered3, advectScalar3 };
+
+}
+
 export function twoPhaseBeta( grid, bins, gasDensity, liquidDensity, fraction ) {
 
 	const beta = { u: arrayN( 'float', grid.dataSizeU ), v: arrayN( 'float', grid.dataSizeV ), w: arrayN( 'float', grid.dataSizeW ) };
@@ -647,7 +722,8 @@ export function createGridFlipSolver3( {
 export function createGridSmokeSolver3( {
 	velocityGrid, gridSpacing, origin, dt, closedDomainBoundaryFlag,
 	buoyancySmokeDensityFactor = - 0.000625, buoyancyTemperatureFactor = 5.0, ambientTemperature = 0,
-	smokeDecay = 0.001, temperatureDecay = 0.001, up = [ 0, 1, 0 ], collider, inflows, outflows, pressure = {}
+	smokeDecay = 0.001, temperatureDecay = 0.001, up = [ 0, 1, 0 ], collider, inflows, outflows, pressure = {},
+	advection = {}
 } = {} ) {
 
 	if ( ! velocityGrid ) throw new Error( 'createGridSmokeSolver3: options.velocityGrid is required.' );
@@ -667,7 +743,9 @@ export function createGridSmokeSolver3( {
 		buoyancySmokeDensityFactor, buoyancyTemperatureFactor, ambientTemperature, smokeDecay, temperatureDecay,
 		multigrid: pressure.multigrid || {}, preconditionerKind: PRECONDITIONERS[ pre ],
 		tolerance: pressure.tolerance === undefined ? 1e-6 : pressure.tolerance,
-		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations
+		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations,
+		advectionOrder: advection.order === undefined ? 1 : advection.order,
+		maxSubsteps: advection.maxSubsteps === undefined ? 32 : advection.maxSubsteps
 	} );
 	const [ nx, ny, nz ] = resolution;
 	const view = ( which, shape ) => wrapField( addon.smoke3Field( handle, which ), 'float', shape );
@@ -792,7 +870,7 @@ export function createGridSolver2( {
 
 	if ( ! velocityGrid ) throw new Error( 'createGridSolver2: options.velocityGrid is required.' );
 	if ( force ) throw new Error( 'createGridSolver2: external forces are not supported by this build.' );
-	if ( advection.order !== undefined && advection.order !== 1 ) throw new Error( 'createGridSolver2: only order-1 advection is built.' );
+	if ( advection.order !== undefined && advection.order !== 1 && advection.order !== 2 ) throw new Error( 'createGridSolver2: advection.order must be 1 or 2.' );
 	if ( advection.collider && advection.collider !== collider ) throw new Error( 'createGridSolver2: advection.collider must be the solver\'s collider.' );
 	const resolution = velocityGrid.resolution.slice();
 	const spacing = gridSpacing || velocityGrid.gridSpacing;
@@ -838,6 +916,7 @@ export function createGridSolver2( {
 		colliderSdf: collider ? collider.grid.data.handle : null,
 		friction: collider ? collider.frictionCoefficient : 0.5,
 		advectCollider: !! advection.collider, maxSubsteps: advection.maxSubsteps === undefined ? 32 : advection.maxSubsteps,
+		advectionOrder: advection.order === undefined ? 1 : advection.order,
 		inflows: asList( inflows ).map( ( q ) => ( { sdf: q.grid.data.handle, velocity: q.velocity, mode: q.mode === 'add' ? 1 : 0 } ) ),
 		outflows: asList( outflows ).map( ( q ) => q.grid.data.handle ), outflowVelocityBC,
 		dirichletActive: dirichletActive ? dirichletActive.handle : null,
@@ -918,5 +997,64 @@ export function createGridAdaptiveTimeStep2( { velocityGrid, gridSpacing, dt, ta
 	}
 
 	return { update, state };
+
+}
+
+// ------------------------------------------------------------- fixtures
+// The pressure-system fixture format of export_system.mjs:65-126: meta.json
+// and one little-endian float32 .bin per array (b, pressure, dirichletMask,
+// ventMask, weightU/V/W), first index fastest.
+export async function exportSystem( directory, solver, { source = 'tsl_array_n_amd', frame = 0, faceWeights = null } = {} ) {
+
+	const fs = require( 'fs' );
+	const path = require( 'path' );
+	fs.mkdirSync( directory, { recursive: true } );
+	const res = solver.pressure.resolution.slice();
+	const asFloat = async ( field ) => Float32Array.from( await field.toArray() );
+	const arrays = {
+		b: await asFloat( solver.b ), pressure: await asFloat( solver.pressure.data ),
+		dirichletMask: solver.dirichletMask ? await asFloat( solver.dirichletMask ) : null,
+		ventMask: solver.ventMask ? await asFloat( solver.ventMask ) : null,
+		weightU: faceWeights ? await asFloat( faceWeights.u ) : null,
+		weightV: faceWeights ? await asFloat( faceWeights.v ) : null,
+		weightW: faceWeights ? await asFloat( faceWeights.w ) : null
+	};
+	const written = [];
+	for ( const [ name, values ] of Object.entries( arrays ) ) {
+
+		if ( values === null ) continue;
+		fs.writeFileSync( path.join( directory, `${ name }.bin` ), Buffer.from( values.buffer ) );
+		written.push( name );
+
+	}
+
+	const [ nx, ny, nz ] = res;
+	const meta = {
+		source, frame, resolution: res, gridSpacing: solver.pressure.gridSpacing,
+		dataSizeU: [ nx + 1, ny, nz ], dataSizeV: [ nx, ny + 1, nz ], dataSizeW: [ nx, ny, nz + 1 ],
+		diagnostics: { ...solver.diagnostics }, arrays: written
+	};
+	fs.writeFileSync( path.join( directory, 'meta.json' ), JSON.stringify( meta, null, '\t' ) + '\n' );
+	return meta;
+
+}
+
+export function loadSystem( directory ) {
+
+	const fs = require( 'fs' );
+	const path = require( 'path' );
+	const meta = JSON.parse( fs.readFileSync( path.join( directory, 'meta.json' ), 'utf8' ) );
+	const read = ( name ) => {
+
+		const file = path.join( directory, `${ name }.bin` );
+		if ( ! fs.existsSync( file ) ) return null;
+		const buf = fs.readFileSync( file );
+		return new Float32Array( buf.buffer.slice( buf.byteOffset, buf.byteOffset + buf.byteLength ) );
+
+	};
+
+	const arrays = {};
+	for ( const name of [ 'b', 'pressure', 'dirichletMask', 'ventMask', 'weightU', 'weightV', 'weightW' ] ) arrays[ name ] = read( name );
+	return { meta, resolution: meta.resolution, gridSpacing: meta.gridSpacing, arrays };
 
 }

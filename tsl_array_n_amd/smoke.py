@@ -1,7 +1,8 @@
 """createGridSmokeSolver3 on the GPU (packages/fluxflow/src/grid/grid_smoke_solver3.js:25-154).
 
-Buoyancy, constrainVelocity, projection, order-1 semi-Lagrangian advection,
-then density and temperature advected and decayed.  ``on_advance_time_step()``
+Buoyancy, constrainVelocity, projection, semi-Lagrangian advection (order 1,
+or MacCormack with ``advection={"order": 2}``), then density and temperature
+advected and decayed.  ``on_advance_time_step()``
 runs one step; ``source_sphere`` is the example scenes' re-injected ball.
 """
 import ctypes as C
@@ -15,7 +16,7 @@ class Smoke3Desc(C.Structure):
                 ("buoyancy_temperature_factor", C.c_float), ("ambient_temperature", C.c_float),
                 ("smoke_decay", C.c_float), ("temperature_decay", C.c_float), ("up", C.c_float * 3),
                 ("closed_domain_flags", C.c_int), ("max_substeps", C.c_int), ("extrapolation_depth", C.c_int),
-                ("pressure", Pressure3Desc)]
+                ("pressure", Pressure3Desc), ("advection_order", C.c_int)]
 
 
 _FIELDS = {"u": 0, "v": 1, "w": 2, "density": 3, "temperature": 4, "pressure": 5}
@@ -25,7 +26,7 @@ class GridSmokeSolver3:
     def __init__(self, resolution, grid_spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), dt=None,
                  buoyancy_smoke_density_factor=-0.000625, buoyancy_temperature_factor=5.0, ambient_temperature=0.0,
                  smoke_decay=0.001, temperature_decay=0.001, up=(0.0, 1.0, 0.0), closed_domain_boundary_flag=63,
-                 pressure=None):
+                 pressure=None, advection=None):
         if dt is None:
             raise ValueError("createGridSmokeSolver3: dt is required.")
         d = Smoke3Desc()
@@ -42,6 +43,10 @@ class GridSmokeSolver3:
         d.smoke_decay = smoke_decay
         d.temperature_decay = temperature_decay
         d.closed_domain_flags = int(closed_domain_boundary_flag)
+        adv = advection or {}
+        d.advection_order = int(adv.get("order", 1))
+        if "maxSubsteps" in adv:
+            d.max_substeps = int(adv["maxSubsteps"])
         p = pressure or {}
         if "multigrid" in p:
             d.pressure.multigrid = mg_options(**p["multigrid"])

@@ -84,6 +84,42 @@ def advect_faces(grid, vel_in, vel_out, dt, max_substeps=32):
            "advectFaceCentered3")
 
 
+class SemiLagrangianAdvectionSolver3:
+    """createSemiLagrangianAdvectionSolver3({velocityGrid, dt, maxSubsteps,
+    order}) (advection_solver3.js:67-362), no collider: the velocity fields
+    are bound here and read at each call; order 1 or 2 (MacCormack)."""
+
+    def __init__(self, grid, velocity_grid, dt, max_substeps=32, order=1):
+        if order not in (1, 2):
+            raise ValueError("createSemiLagrangianAdvectionSolver3: order must be 1 or 2")
+        self.grid, self.velocity_grid, self.order = grid, list(velocity_grid), order
+        h = C.c_void_p()
+        _check(_lib.flux_advection3_create(C.byref(grid.c), _mac(self.velocity_grid), C.c_float(dt),
+                                           int(max_substeps), int(order), C.byref(h)),
+               "createSemiLagrangianAdvectionSolver3")
+        self.h = h.value
+
+    def set_dt(self, dt):
+        _check(_lib.flux_advection3_set_dt(C.c_void_p(self.h), C.c_float(dt)), "advection dt")
+
+    def advect_face_centered3(self, input, output):
+        _check(_lib.flux_advection3_faces(C.c_void_p(self.h), _mac(input), _mac(output)), "advectFaceCentered3")
+
+    def advect_scalar3(self, input, output):
+        _check(_lib.flux_advection3_scalar(C.c_void_p(self.h), C.c_void_p(input.h), C.c_void_p(output.h)),
+               "advectScalar3")
+
+    def __del__(self):
+        try:
+            _lib.flux_advection3_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+def create_semi_lagrangian_advection_solver3(grid, velocity_grid, dt, max_substeps=32, order=1):
+    return SemiLagrangianAdvectionSolver3(grid, velocity_grid, dt, max_substeps, order)
+
+
 def two_phase_beta(grid, bins, rho_gas, rho_liquid, fraction, beta):
     _check(_lib.flux_two_phase_beta(C.byref(grid.c), C.c_void_p(bins.cell_start.h if bins is not None else None),
                                     C.c_float(rho_gas), C.c_float(rho_liquid), C.c_void_p(fraction.h), _mac(beta)),
