@@ -232,6 +232,26 @@ void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const
 void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const float* v, const float* w,
                    const int n[3], const float h[3], const float origin[3], int min_pc, int max_pc);
 
+/* carried concentration and density coupling of createGridFlipSolver3
+ * (grid_flip_solver3.js:265-357, 493-531, 848-880, 967-1049) */
+void orc_conc_accumulate3(const float* pos, const float* conc, int64_t count, const int n[3], const float h[3],
+                          const float origin[3], float scale, int32_t* accum, int32_t* cnt);
+void orc_conc_resolve3(const int32_t* accum, const int32_t* cnt, int64_t cells, float scale, int coupling, float d0,
+                       float d1, float* cell_conc, float* cell_density);
+void orc_density_beta3(const float* density, const float* fluid_mask, const int n[3], float d0, float d1,
+                       float* bu, float* bv, float* bw);
+void orc_reduced_gravity3(float* u, float* v, float* w, const float* const beta[3], const float* density,
+                          const int n[3], const float h[3], const float origin[3], const float g[3], float dt, float d0,
+                          float d1);
+void orc_mix_fade3(const float* pos, float* conc, int64_t count, const float* cell_conc, const int n[3],
+                   const float h[3], const float origin[3], float mixing, float fade);
+void orc_p2g_fixed_mass3(const float* pos, const float* vel, const float* conc, int64_t count, int comp,
+                         const int face_n[3], const float data_origin[3], const float h[3], float scale, float d0,
+                         float d1, int32_t* numer, int32_t* denom);
+/* phase-aware resampling (grid_two_phase_flip_solver2.js:1131-1503, in 3-D) */
+void orc_resample_phase3(float* pos, float* vel, const float* conc, int64_t count, const float* u, const float* v,
+                         const float* w, const int n[3], const float h[3], const float origin[3], int min_pc,
+                         int max_pc, const int32_t* cnt, const int32_t* accum, float scale);
 /* advectScalar3, order 1, no collider: advection_solver3.js:285-300 */
 void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
                         const float h[3], const float origin[3], float dt, int max_sub);

@@ -514,3 +514,89 @@ void orc_source_sphere3(float* d, const int n[3], const float h[3], const float 
                 d[q] = d[q] > s ? d[q] : s;
             }
 }
+
+/* ---- phase-aware resampling (grid_two_phase_flip_solver2.js:1131-1503, in
+ *      3-D): over/under detection on the total count, donors split by phase
+ *      (concentration > 0.5) into a liquid pool (bottom half, ranked by
+ *      particle index) and a gas pool (top half, ranked downward), each capped
+ *      at max(1, count / 2); an under-filled cell with at least 3 of its 6
+ *      clamped neighbours holding min_pc takes donors of its own majority
+ *      phase (its neighbours' when it is empty).  cnt / accum are the
+ *      per-cell count and fixed-point concentration sum at `scale`. ---- */
+void orc_resample_phase3(float* pos, float* vel, const float* conc, int64_t count, const float* u, const float* v,
+                         const float* w, const int n[3], const float h[3], const float origin[3], int min_pc,
+                         int max_pc, const int32_t* cnt, const int32_t* accum, float scale) {
+    const int64_t cells = (int64_t)n[0] * n[1] * n[2];
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (count > 0 ? count : 1));
+    int32_t* pool = (int32_t*)malloc(sizeof(int32_t) * (count > 0 ? count : 1));
+    int32_t* need = (int32_t*)calloc(cells, sizeof(int32_t));
+    orc_cell_index3(pos, count, origin, h, n, cell);
+    const int64_t half = count / 2 > 1 ? count / 2 : 1;
+    const int64_t top = count - 1;
+    int64_t nl = 0, ng = 0;
+    for (int64_t p = 0; p < count; ++p) {
+        if (cnt[cell[p]] <= max_pc) continue;
+        if (conc[p] > 0.5f) {
+            if (nl < half) pool[nl] = (int32_t)p;
+            ++nl;
+        } else {
+            if (ng < half) pool[top - ng] = (int32_t)p;
+            ++ng;
+        }
+    }
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int64_t nbr[6] = {orc_cidx(n, i > 0 ? i - 1 : 0, j, k), orc_cidx(n, i < n[0] - 1 ? i + 1 : n[0] - 1, j, k),
+                                        orc_cidx(n, i, j > 0 ? j - 1 : 0, k), orc_cidx(n, i, j < n[1] - 1 ? j + 1 : n[1] - 1, k),
+                                        orc_cidx(n, i, j, k > 0 ? k - 1 : 0), orc_cidx(n, i, j, k < n[2] - 1 ? k + 1 : n[2] - 1)};
+                int well = 0;
+                int32_t ntot = 0, nsum = 0;
+                for (int q = 0; q < 6; ++q) {
+                    well += cnt[nbr[q]] >= min_pc;
+                    ntot = (int32_t)((uint32_t)ntot + (uint32_t)cnt[nbr[q]]);
+                    nsum = (int32_t)((uint32_t)nsum + (uint32_t)accum[nbr[q]]);
+                }
+                const int64_t c = orc_cidx(n, i, j, k);
+                const float total = (float)cnt[c];
+                const float own = ((float)accum[c] / scale) / (total > 1.0f ? total : 1.0f);
+                const float ntf = (float)ntot;
+                const float nmean = ((float)nsum / scale) / (ntf > 1.0f ? ntf : 1.0f);
+                const int liquid = total > 0.5f ? own > 0.5f : nmean > 0.5f;
+                const int eligible = cnt[c] < min_pc && well >= 3;
+                const int needed = min_pc - cnt[c];
+                need[c] = eligible ? (liquid ? needed : -needed) : 0;
+            }
+    float ou[3], ov[3], ow[3];
+    ou[0] = origin[0]; ou[1] = origin[1] + h[1] * 0.5f; ou[2] = origin[2] + h[2] * 0.5f;
+    ov[0] = origin[0] + h[0] * 0.5f; ov[1] = origin[1]; ov[2] = origin[2] + h[2] * 0.5f;
+    ow[0] = origin[0] + h[0] * 0.5f; ow[1] = origin[1] + h[1] * 0.5f; ow[2] = origin[2];
+    int su[3] = {n[0] + 1, n[1], n[2]}, sv[3] = {n[0], n[1] + 1, n[2]}, sw[3] = {n[0], n[1], n[2] + 1};
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    const int64_t avail_l = nl < half ? nl : half, avail_g = ng < half ? ng : half;
+    int64_t base_l = 0, base_g = 0;
+    for (int64_t c = 0; c < cells; ++c) {
+        const int nd = need[c];
+        if (nd == 0) continue;
+        const int liquid = nd > 0;
+        const int needed = liquid ? nd : -nd;
+        const int64_t base = liquid ? base_l : base_g;
+        const int64_t avail = liquid ? avail_l : avail_g;
+        const int i = (int)(c % n[0]), j = (int)((c / n[0]) % n[1]), k = (int)(c / ((int64_t)n[0] * n[1]));
+        const float np_[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+        for (int q = 0; q < needed && q < min_pc; ++q) {
+            const int64_t slot = base + q;
+            if (slot >= avail) break;
+            const int32_t d = pool[liquid ? slot : top - slot];
+            pos[3 * d + 0] = np_[0];
+            pos[3 * d + 1] = np_[1];
+            pos[3 * d + 2] = np_[2];
+            vel[3 * d + 0] = orc_sample_trilinear3(u, su, ou, h, np_);
+            vel[3 * d + 1] = orc_sample_trilinear3(v, sv, ov, h, np_);
+            vel[3 * d + 2] = orc_sample_trilinear3(w, sw, ow, h, np_);
+        }
+        if (liquid) base_l += needed;
+        else base_g += needed;
+    }
+    free(cell); free(pool); free(need);
+}

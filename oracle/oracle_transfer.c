@@ -605,3 +605,131 @@ void orc_trace3(float* pos, int64_t count, const float* u, const float* v, const
         pos[3 * p + 2] = tr[2];
     }
 }
+
+/* ==== carried concentration and density coupling of createGridFlipSolver3
+ *      (grid_flip_solver3.js:265-357, 493-531, 848-880, 967-1049) ==== */
+#define ORC_MIN_DENSITY 1e-4f
+
+static inline float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+/* accumulateConcentration (992-998): per cell the particle count and the sum
+ * of round(clamp(c, 0, 1) * scale) as wrapping int32 atomics */
+void orc_conc_accumulate3(const float* pos, const float* conc, int64_t count, const int n[3], const float h[3],
+                          const float origin[3], float scale, int32_t* accum, int32_t* cnt) {
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (count > 0 ? count : 1));
+    orc_cell_index3(pos, count, origin, h, n, cell);
+    for (int64_t p = 0; p < count; ++p) {
+        cnt[cell[p]] = (int32_t)((uint32_t)cnt[cell[p]] + 1u);
+        accum[cell[p]] = (int32_t)((uint32_t)accum[cell[p]] + (uint32_t)f32_to_i32_sat(rintf(clamp01(conc[p]) * scale)));
+    }
+    free(cell);
+}
+
+/* resolveCellConcentration (1000-1012): mean concentration of a cell's
+ * particles (0 for an empty cell), and with density coupling the cell density
+ * d0 + (d1 - d0) c with d0, d1 floored at MIN_DENSITY */
+void orc_conc_resolve3(const int32_t* accum, const int32_t* cnt, int64_t cells, float scale, int coupling, float d0,
+                       float d1, float* cell_conc, float* cell_density) {
+    const float a = d0 > ORC_MIN_DENSITY ? d0 : ORC_MIN_DENSITY, b = d1 > ORC_MIN_DENSITY ? d1 : ORC_MIN_DENSITY;
+    for (int64_t c = 0; c < cells; ++c) {
+        const float total = (float)cnt[c];
+        const float sum = (float)accum[c] / scale;
+        const float mean = sum / (total > 1.0f ? total : 1.0f);
+        const float cc = total > 0.5f ? clamp01(mean) : 0.0f;
+        cell_conc[c] = cc;
+        if (coupling) cell_density[c] = a + (b - a) * cc;
+    }
+}
+
+/* computeBetaU/V/W with faceDensity (313-357): beta = rho_ref / max(rho_face,
+ * MIN_DENSITY), rho_face the mean of two fluid cells, the fluid one's own
+ * density when one side is fluid, else rho_ref; edge faces clamp indices */
+void orc_density_beta3(const float* density, const float* fluid_mask, const int n[3], float d0, float d1,
+                       float* bu, float* bv, float* bw) {
+    const float a = d0 > ORC_MIN_DENSITY ? d0 : ORC_MIN_DENSITY, b = d1 > ORC_MIN_DENSITY ? d1 : ORC_MIN_DENSITY;
+    const float ref = a > b ? a : b;
+    float* out[3] = {bu, bv, bw};
+    for (int ax = 0; ax < 3; ++ax) {
+        const int s[3] = {n[0] + (ax == 0), n[1] + (ax == 1), n[2] + (ax == 2)};
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    int A[3] = {i, j, k}, B[3] = {i, j, k};
+                    A[ax] = clampi(A[ax] - 1, 0, n[ax] - 1);
+                    B[ax] = clampi(B[ax], 0, n[ax] - 1);
+                    const int64_t ca = orc_cidx(n, A[0], A[1], A[2]), cb = orc_cidx(n, B[0], B[1], B[2]);
+                    const float ra = density[ca], rb = density[cb];
+                    const int fa = fluid_mask[ca] > 0.5f, fb = fluid_mask[cb] > 0.5f;
+                    const float rf = (fa && fb) ? (ra + rb) * 0.5f : ((fa || fb) ? (fa ? ra : rb) : ref);
+                    out[ax][orc_cidx(s, i, j, k)] = ref / (rf > ORC_MIN_DENSITY ? rf : ORC_MIN_DENSITY);
+                }
+    }
+}
+
+/* applyReducedGravityU/V/W (848-880): u -= beta dt / rho_ref (g . x_face)
+ * d(rho)/dx across the face, indices clamped at the walls */
+void orc_reduced_gravity3(float* u, float* v, float* w, const float* const beta[3], const float* density,
+                          const int n[3], const float h[3], const float origin[3], const float g[3], float dt, float d0,
+                          float d1) {
+    const float a = d0 > ORC_MIN_DENSITY ? d0 : ORC_MIN_DENSITY, b = d1 > ORC_MIN_DENSITY ? d1 : ORC_MIN_DENSITY;
+    const float ref = a > b ? a : b;
+    float* vel[3] = {u, v, w};
+    for (int ax = 0; ax < 3; ++ax) {
+        const int s[3] = {n[0] + (ax == 0), n[1] + (ax == 1), n[2] + (ax == 2)};
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    const int I[3] = {i, j, k};
+                    float pos[3];
+                    for (int q = 0; q < 3; ++q)
+                        pos[q] = origin[q] + (q == ax ? (float)I[q] * h[q] : ((float)I[q] + 0.5f) * h[q]);
+                    int A[3] = {i, j, k}, B[3] = {i, j, k};
+                    A[ax] = clampi(A[ax] - 1, 0, n[ax] - 1);
+                    B[ax] = clampi(B[ax], 0, n[ax] - 1);
+                    const float drho = (density[orc_cidx(n, B[0], B[1], B[2])] - density[orc_cidx(n, A[0], A[1], A[2])]) / h[ax];
+                    const float gdot = (g[0] * pos[0] + g[1] * pos[1]) + g[2] * pos[2];
+                    const int64_t f = orc_cidx(s, i, j, k);
+                    const float term = (((beta[ax][f] * dt) / ref) * gdot) * drho;
+                    vel[ax][f] = vel[ax][f] - term;
+                }
+    }
+}
+
+/* mixAndFade (1018-1027): c <- clamp((c + (local - c) mixing) (1 - fade)),
+ * local = the cell concentration sampled trilinearly at the particle (c
+ * itself where that is not finite); mixing and fade clamped to [0, 1] */
+void orc_mix_fade3(const float* pos, float* conc, int64_t count, const float* cell_conc, const int n[3],
+                   const float h[3], const float origin[3], float mixing, float fade) {
+    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    const float mx = clamp01(mixing), fd = clamp01(fade);
+    for (int64_t p = 0; p < count; ++p) {
+        const float sampled = orc_sample_trilinear3(cell_conc, n, co, h, &pos[3 * p]);
+        const float c = clamp01(conc[p]);
+        const float local = isfinite(sampled) ? sampled : c;
+        const float blended = c + (local - c) * mx;
+        conc[p] = clamp01(blended * (1.0f - fd));
+    }
+}
+
+/* buildScatter with massWeight (493-531): each corner weight times
+ * rho(c) / rho_ref, rho(c) = d0 + (d1 - d0) clamp(c, 0, 1) */
+void orc_p2g_fixed_mass3(const float* pos, const float* vel, const float* conc, int64_t count, int comp,
+                         const int face_n[3], const float data_origin[3], const float h[3], float scale, float d0,
+                         float d1, int32_t* numer, int32_t* denom) {
+    const float a = d0 > ORC_MIN_DENSITY ? d0 : ORC_MIN_DENSITY, b = d1 > ORC_MIN_DENSITY ? d1 : ORC_MIN_DENSITY;
+    const float ref = a > b ? a : b;
+    for (int64_t p = 0; p < count; ++p) {
+        orc_tri t;
+        tri_weights(&pos[3 * p], data_origin, h, face_n, &t);
+        const float value = vel[3 * p + comp];
+        const float mass = (a + (b - a) * clamp01(conc[p])) / ref;
+        for (int c = 0; c < 8; ++c) {
+            int i, j, k;
+            tri_corner(&t, c, &i, &j, &k);
+            const int64_t f = orc_cidx(face_n, i, j, k);
+            const float w = t.w[c] * mass;
+            numer[f] = (int32_t)((uint32_t)numer[f] + (uint32_t)f32_to_i32_sat(rintf((value * w) * scale)));
+            denom[f] = (int32_t)((uint32_t)denom[f] + (uint32_t)f32_to_i32_sat(rintf(w * scale)));
+        }
+    }
+}

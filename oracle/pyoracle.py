@@ -409,7 +409,8 @@ def mac_origin(origin, h, comp):
     return o
 
 
-def p2g_fixed3(pos, vel, n, h, origin, comp, scale=65536.0, eps=1e-4):
+def p2g_fixed3(pos, vel, n, h, origin, comp, scale=65536.0, eps=1e-4, mass=None):
+    """mass: None, or (concentration, d0, d1) for the mass-weighted scatter"""
     pos = np.ascontiguousarray(pos, np.float32)
     vel = np.ascontiguousarray(vel, np.float32)
     fn = list(n)
@@ -417,8 +418,14 @@ def p2g_fixed3(pos, vel, n, h, origin, comp, scale=65536.0, eps=1e-4):
     nf = int(np.prod(fn))
     numer = np.zeros(nf, np.int32)
     denom = np.zeros(nf, np.int32)
-    lib().orc_p2g_fixed3(_p(pos), _p(vel), C.c_int64(pos.size // 3), comp, _i3(fn), _f3(mac_origin(origin, h, comp)),
-                         _f3(h), C.c_float(scale), _p(numer), _p(denom))
+    if mass is None:
+        lib().orc_p2g_fixed3(_p(pos), _p(vel), C.c_int64(pos.size // 3), comp, _i3(fn),
+                             _f3(mac_origin(origin, h, comp)), _f3(h), C.c_float(scale), _p(numer), _p(denom))
+    else:
+        conc = np.ascontiguousarray(mass[0], np.float32)
+        lib().orc_p2g_fixed_mass3(_p(pos), _p(vel), _p(conc), C.c_int64(pos.size // 3), comp, _i3(fn),
+                                  _f3(mac_origin(origin, h, comp)), _f3(h), C.c_float(scale), C.c_float(mass[1]),
+                                  C.c_float(mass[2]), _p(numer), _p(denom))
     out = np.zeros(nf, np.float32)
     valid = np.zeros(nf, np.int32)
     lib().orc_p2g_finalize_fixed(_p(numer), _p(denom), C.c_int64(nf), C.c_float(scale), C.c_float(eps), _p(out), _p(valid))
@@ -627,6 +634,68 @@ def resample3(pos, vel, u, v, w, n, h, origin, min_pc=4, max_pc=8):
     return pos, vel
 
 
+# ------------------------------------------------ concentration and density
+def conc_accumulate3(pos, conc, n, h, origin, scale=4096.0):
+    pos = np.ascontiguousarray(pos, np.float32)
+    conc = np.ascontiguousarray(conc, np.float32)
+    cells = int(np.prod(n))
+    accum, cnt = np.zeros(cells, np.int32), np.zeros(cells, np.int32)
+    lib().orc_conc_accumulate3(_p(pos), _p(conc), C.c_int64(conc.size), _i3(n), _f3(h), _f3(origin),
+                               C.c_float(scale), _p(accum), _p(cnt))
+    return accum, cnt
+
+
+def conc_resolve3(accum, cnt, scale=4096.0, density=None):
+    """density: None or (d0, d1); returns (cell concentration, cell density or None)"""
+    accum = np.ascontiguousarray(accum, np.int32)
+    cnt = np.ascontiguousarray(cnt, np.int32)
+    cc = np.zeros(accum.size, np.float32)
+    rho = np.zeros(accum.size, np.float32)
+    d0, d1 = density if density is not None else (0.0, 0.0)
+    lib().orc_conc_resolve3(_p(accum), _p(cnt), C.c_int64(accum.size), C.c_float(scale), int(density is not None),
+                            C.c_float(d0), C.c_float(d1), _p(cc), _p(rho))
+    return cc, (rho if density is not None else None)
+
+
+def density_beta3(density, fluid_mask, n, d0, d1):
+    density = np.ascontiguousarray(density, np.float32)
+    fluid_mask = np.ascontiguousarray(fluid_mask, np.float32)
+    outs = [np.zeros(int(np.prod(face_shape(n, a))), np.float32) for a in range(3)]
+    lib().orc_density_beta3(_p(density), _p(fluid_mask), _i3(n), C.c_float(d0), C.c_float(d1), *[_p(o) for o in outs])
+    return outs
+
+
+def reduced_gravity3(vel, beta, density, n, h, origin, gravity, dt, d0, d1):
+    u, v, w = [np.ascontiguousarray(x, np.float32).copy() for x in vel]
+    bk = [np.ascontiguousarray(b, np.float32) for b in beta]
+    B = (C.c_void_p * 3)(*[_p(b) for b in bk])
+    density = np.ascontiguousarray(density, np.float32)
+    lib().orc_reduced_gravity3(_p(u), _p(v), _p(w), B, _p(density), _i3(n), _f3(h), _f3(origin), _f3(gravity),
+                               C.c_float(dt), C.c_float(d0), C.c_float(d1))
+    return [u, v, w]
+
+
+def mix_fade3(pos, conc, cell_conc, n, h, origin, mixing, fade):
+    pos = np.ascontiguousarray(pos, np.float32)
+    conc = np.ascontiguousarray(conc, np.float32).copy()
+    cell_conc = np.ascontiguousarray(cell_conc, np.float32)
+    lib().orc_mix_fade3(_p(pos), _p(conc), C.c_int64(conc.size), _p(cell_conc), _i3(n), _f3(h), _f3(origin),
+                        C.c_float(mixing), C.c_float(fade))
+    return conc
+
+
+def resample_phase3(pos, vel, conc, u, v, w, n, h, origin, min_pc, max_pc, cnt, accum, scale=4096.0):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    conc = np.ascontiguousarray(conc, np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    cnt = np.ascontiguousarray(cnt, np.int32)
+    accum = np.ascontiguousarray(accum, np.int32)
+    lib().orc_resample_phase3(_p(pos), _p(vel), _p(conc), C.c_int64(conc.size), *[_p(a) for a in arrs], _i3(n),
+                              _f3(h), _f3(origin), int(min_pc), int(max_pc), _p(cnt), _p(accum), C.c_float(scale))
+    return pos, vel
+
+
 def free_surface_fields(mask, n, h, origin, gravity, dt, reduced=True):
     """dirichlet(pos) of grid_flip_solver3.js:252-259 at cell centres (f32)."""
     f = np.float32
@@ -641,18 +710,36 @@ def free_surface_fields(mask, n, h, origin, gravity, dt, reduced=True):
     return active, target
 
 
+def pin_cell3(n, pin):
+    """pressurePin: 'topCenter' (floor(nx/2), ny-1, floor(nz/2)), an (i, j, k) cell, or None"""
+    if pin is None:
+        return None
+    if pin == "topCenter":
+        return (n[0] // 2, n[1] - 1, n[2] // 2)
+    return tuple(int(x) for x in pin)
+
+
 def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e-6, flip_ratio=0.97,
                damping=0.02, scale=65536.0, eps=1e-4, flags=63, resample=True, min_pc=4, max_pc=8, depth=5,
-               max_substeps=32, reduced=True, max_plausible=None):
+               max_substeps=32, reduced=True, max_plausible=None, carry_concentration=False, mixing=0.0, fade=0.0,
+               conc_scale=4096.0, density=None, mass_weighted=False, pressure_pin=None, phase_resample=False):
     """One createGridFlipSolver3 time step (grid_flip_solver3.js:1083-1132)
     composed from the oracle's stages; st holds pos, vel, u, v, w, pressure
-    and the last accepted pressure, and is updated in place."""
+    and the last accepted pressure (and conc with carry_concentration), and is
+    updated in place.  density = (ambient, component) couples the
+    concentration to the pressure's face weights (265-357) and, with reduced
+    pressure, to the gravity term (848-880); mass_weighted weights the
+    scatter by rho / rho_ref (493-531); pressure_pin pins one cell to 0 when
+    the free surface leaves no Dirichlet cell; phase_resample splits the
+    resampling's donors by phase."""
     pos, vel = st["pos"], st["vel"]
     u, v, w = st["u"], st["v"], st["w"]
+    conc = st.get("conc")
     pos = advect_particles3(pos, u, v, w, n, h, origin, dt, max_substeps)
+    mass = (conc, density[0], density[1]) if (mass_weighted and density is not None and carry_concentration) else None
     vel_grid, valid = [], []
     for a in range(3):
-        o, vd, _, _ = p2g_fixed3(pos, vel, n, h, origin, a, scale, eps)
+        o, vd, _, _ = p2g_fixed3(pos, vel, n, h, origin, a, scale, eps, mass=mass)
         vel_grid.append(o)
         valid.append(vd)
     old = [x.copy() for x in vel_grid]
@@ -660,20 +747,38 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
     vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
     mask = mark_fluid3(pos, n, h, origin)
     if resample:
-        pos, vel = resample3(pos, vel, *vel_grid, n, h, origin, min_pc, max_pc)
+        if phase_resample and carry_concentration:
+            accum, cnt = conc_accumulate3(pos, conc, n, h, origin, conc_scale)
+            pos, vel = resample_phase3(pos, vel, conc, *vel_grid, n, h, origin, min_pc, max_pc, cnt, accum, conc_scale)
+        else:
+            pos, vel = resample3(pos, vel, *vel_grid, n, h, origin, min_pc, max_pc)
         mask = mark_fluid3(pos, n, h, origin)
     adj = [fluid_adjacent3(mask, n, a) for a in range(3)]
+    beta, cell_conc, cell_density = None, None, None
+    if carry_concentration:
+        accum, cnt = conc_accumulate3(pos, conc, n, h, origin, conc_scale)
+        cell_conc, cell_density = conc_resolve3(accum, cnt, conc_scale, density)
+        if density is not None:
+            beta = density_beta3(cell_density, mask, n, density[0], density[1])
     if not reduced:
         vel_grid = [(vel_grid[a] + np.float32(gravity[a]) * np.float32(dt)).astype(np.float32) for a in range(3)]
+    elif density is not None:
+        vel_grid = reduced_gravity3(vel_grid, beta, cell_density, n, h, origin, gravity, dt, density[0], density[1])
     vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
     active, target = free_surface_fields(mask, n, h, origin, gravity, dt, reduced)
+    pin = pin_cell3(n, pressure_pin)
+    if pin is not None and not active.any():
+        c = pin[0] + n[0] * (pin[1] + n[1] * pin[2])
+        active[c] = 1
+        target[c] = 0.0
     if max_plausible is None:
         extent = sum(n[a] * h[a] for a in range(3))
         gm = float(np.sqrt(sum(x * x for x in gravity)))
         max_plausible = 8.0 * dt * gm * extent if (reduced and gm > 0) else 1e6
-    A = pressure3_assemble(n, h, vel_grid, active=active, target=target, pressure=st["pressure"], suffix="f32")
-    mg = Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], suffix="f32")
-    x, _, diag = pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], mg=mg, tol=tol,
+    A = pressure3_assemble(n, h, vel_grid, active=active, target=target, beta=beta, pressure=st["pressure"],
+                           suffix="f32")
+    mg = Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f32")
+    x, _, diag = pcg_solve_mixed(n, h, A["b"], x0=A["pressure"], mask=A["mask"], weights=A["weights"], mg=mg, tol=tol,
                                  maxiter=int(np.ceil(1.5 * max(n))))
     bad = count_bad(x, max_plausible, suffix="f32")
     if bad > 0:
@@ -681,10 +786,13 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
     else:
         st["p_accepted"] = x.copy()
     diag["rejected"] = bad > 0
-    vel_grid = pressure3_correct(n, h, x, vel_grid, suffix="f32")
+    vel_grid = pressure3_correct(n, h, x, vel_grid, beta=beta, suffix="f32")
     vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
     vel_grid = [extrapolate3(face_shape(n, a), vel_grid[a], adj[a], depth) for a in range(3)]
     vel = g2p3(pos, vel, *vel_grid, *old, n, h, origin, flip_ratio, damping)
+    if carry_concentration:
+        conc = mix_fade3(pos, conc, cell_conc, n, h, origin, mixing, fade)
+        st.update(conc=conc, cell_conc=cell_conc, cell_density=cell_density)
     st.update(pos=pos, vel=vel, u=vel_grid[0], v=vel_grid[1], w=vel_grid[2], pressure=x, mask=mask)
     return diag
 
