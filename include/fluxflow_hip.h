@@ -518,6 +518,30 @@ typedef struct {
      * max_plausible_pressure <= 0 is derived as 8 dt |g| (sum of extents)
      * (grid_flip_solver3.js:425-432) */
     flux_pressure3_desc pressure;
+    /* carried concentration (carryConcentration, mixing, fade,
+     * concentrationAtomicScale; grid_flip_solver3.js:967-1049): a per-particle
+     * value in [0, 1], averaged per cell by fixed-point atomics each step and
+     * blended back by mixing / faded by fade after G2P */
+    int carry_concentration;           /* default 0 */
+    float mixing;                      /* default 0 */
+    float fade;                        /* default 0 */
+    float concentration_atomic_scale;  /* default 4096 */
+    /* ambientDensity / componentDensity (265-357, 848-880): both > 0 couple
+     * the cell concentration to the density, the pressure's face weights
+     * rho_ref / rho_face and, with reduced pressure, the gravity term */
+    float ambient_density;             /* default 0 (off) */
+    float component_density;           /* default 0 (off) */
+    int mass_weighted_transfer;        /* massWeightedTransfer (493-531), default 0 */
+    /* pressurePin (grid_two_phase_flip_solver2.js:133-169): when the free
+     * surface leaves no Dirichlet cell (a closed domain full of fluid), this
+     * cell is pinned to 0: 0 none (default), 1 top centre (nx/2, ny-1, nz/2),
+     * 2 the cell pin_cell */
+    int pressure_pin;
+    int pin_cell[3];
+    /* phase-aware resampling (grid_two_phase_flip_solver2.js:1131-1503, in
+     * 3-D): donors split by phase, a recipient takes its majority phase;
+     * needs carry_concentration.  Default 0 */
+    int phase_resample;
 } flux_flip3_desc;
 void flux_flip3_desc_default(flux_flip3_desc* d);
 typedef struct flux_flip3 flux_flip3;
@@ -526,7 +550,8 @@ int flux_flip3_create(const flux_flip3_desc* d, flux_flip3** out);
  * resample, constrain, project, constrain, extrapolate, G2P */
 int flux_flip3_step(flux_flip3* s, flux_pressure_diag* diag);
 enum { FLUX_FLIP3_POSITIONS = 0, FLUX_FLIP3_VELOCITIES = 1, FLUX_FLIP3_FLUID_MASK = 2, FLUX_FLIP3_U = 3,
-       FLUX_FLIP3_V = 4, FLUX_FLIP3_W = 5, FLUX_FLIP3_PRESSURE = 6 };
+       FLUX_FLIP3_V = 4, FLUX_FLIP3_W = 5, FLUX_FLIP3_PRESSURE = 6, FLUX_FLIP3_CONCENTRATION = 7,
+       FLUX_FLIP3_CELL_CONCENTRATION = 8, FLUX_FLIP3_CELL_DENSITY = 9 };
 flux_field* flux_flip3_field(flux_flip3* s, int which);
 int flux_flip3_destroy(flux_flip3* s);
 

@@ -198,3 +198,39 @@ def test_mixed_precision_projection_reaches_same_tolerance(flux, storage):
         # BF16's 8-bit mantissa costs iterations
         assert dm["iterations"] <= int(1.5 * diag["iterations"])
     assert rel(pm.pressure.to_array(), ps.pressure.to_array()) <= 1e-4
+
+
+@pytest.mark.parametrize("shape", [(12, 12, 8), (12, 16, 8), (16, 12, 8), (10, 6, 6), (16, 16, 8)])
+def test_projection_on_planes_that_do_not_fill_wavefronts(flux, shape):
+    """The PCG's r.z is fused into the V-cycle's last relaxation as a block
+    reduction; a plane of (n0/2) x n1 cell pairs that is not a multiple of
+    the 64-lane wavefront leaves part of a wavefront idle, and every thread
+    must still pass the reduction once (a free-surface system, 2 levels)."""
+    F = flux
+    n = int(np.prod(shape))
+    j = (np.arange(n) // shape[0]) % shape[1]
+    active = (j >= shape[1] // 2).astype(np.uint8)
+    target = (active * (0.05 * 9.81 * (j + 0.5))).astype(np.float32)
+    rng = np.random.default_rng(2)
+    vel = []
+    for a in range(3):
+        fs = list(shape)
+        fs[a] += 1
+        v = rng.standard_normal(fs[::-1]).astype(np.float32)
+        lo, hi = [slice(None)] * 3, [slice(None)] * 3
+        lo[2 - a], hi[2 - a] = 0, -1
+        v[tuple(lo)] = 0
+        v[tuple(hi)] = 0
+        vel.append(v.ravel())
+    ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=(1.0, 1.0, 1.0),
+                                        dirichlet=(F.array_n("u8", shape, active), F.array_n("float", shape, target)),
+                                        multigrid={"levels": 2}, tolerance=1e-6)
+    d = ps.project([F.array_n("float", F.face_shape(shape, a), vel[a]) for a in range(3)])
+    A = O.pressure3_assemble(shape, (1.0, 1.0, 1.0), vel, active=active, target=target, suffix="f32")
+    mg = O.Multigrid(shape, (1.0, 1.0, 1.0), levels=2, mask=A["mask"], coarse_mask=A["vent"], suffix="f32")
+    x, _, od = O.pcg_solve_mixed(shape, (1.0, 1.0, 1.0), A["b"], x0=A["pressure"], mask=A["mask"], mg=mg, tol=1e-6,
+                                 maxiter=100)
+    assert d["converged"] and od["converged"], (d, od)
+    assert abs(d["iterations"] - od["iterations"]) <= 1
+    p = ps.pressure.to_array()
+    assert np.linalg.norm(p - x) <= 1e-5 * np.linalg.norm(x)

@@ -112,22 +112,24 @@ def test_fused_projection_matches_two_launch(flux):
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((64, 32, 48), 4), ((96, 64, 64), 5)])
-def test_vcycle_tail_is_bit_identical(flux, storage, shape, levels):
+@pytest.mark.parametrize("env", ["FLUX_VCYCLE_TAIL", "FLUX_CORRECT_PAIRS"])
+def test_vcycle_variants_are_bit_identical(flux, storage, shape, levels, env):
     """The last two levels in one workgroup (k_vcycle_tail) against their
-    separate launches: one V-cycle agrees bit for bit."""
+    separate launches, and the paired prolongation-correction against the
+    one-cell-per-thread kernel: one V-cycle agrees bit for bit."""
     F = flux
     sc = scenes.dam_break_system(64) if shape == (64, 64, 64) else box_scene(shape, 7)
     outs = []
-    for tail in ("0", "1"):
-        old = os.environ.get("FLUX_VCYCLE_TAIL")
-        os.environ["FLUX_VCYCLE_TAIL"] = tail
+    for flag in ("0", "1"):
+        old = os.environ.get(env)
+        os.environ[env] = flag
         try:
             ps, vel, mg = build_mg(F, sc, levels, storage, False)
         finally:
             if old is None:
-                del os.environ["FLUX_VCYCLE_TAIL"]
+                del os.environ[env]
             else:
-                os.environ["FLUX_VCYCLE_TAIL"] = old
+                os.environ[env] = old
         z = F.array_n("float", shape)
         mg(ps.b, z)
         outs.append(z.to_array())

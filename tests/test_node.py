@@ -156,6 +156,35 @@ def test_smoke_through_node_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_flip_concentration_through_node_matches_oracle(tmp_path):
+    """createGridFlipSolver3 with carried concentration, density coupling and
+    the top-centre pin from Node: two steps of a stably stratified closed
+    box, against the oracle's composed step"""
+    n = (8, 8, 8)
+    g = np.stack(np.meshgrid(*[(np.arange(2 * n[a]) + 0.5) / 2 for a in range(3)], indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + np.random.default_rng(3).uniform(-0.1, 0.1, g.shape)).astype(np.float32).ravel()
+    conc = (pos.reshape(-1, 3)[:, 1] < 4).astype(np.float32)
+    pos.tofile(tmp_path / "pos.bin")
+    conc.tofile(tmp_path / "conc.bin")
+    run_node("flip_conc_step.mjs", tmp_path, *n, timeout=300)
+    meta = json.load(open(tmp_path / "meta.json"))
+    cells = int(np.prod(n))
+    st = {"pos": pos.copy(), "vel": np.zeros_like(pos), "conc": conc.copy(), "pressure": np.zeros(cells, np.float32),
+          "p_accepted": np.zeros(cells, np.float32)}
+    for a, c in enumerate("uvw"):
+        st[c] = np.zeros(int(np.prod(O.face_shape(n, a))), np.float32)
+    for s in range(2):
+        d = O.flip3_step(st, n, (1.0, 1.0, 1.0), (0, 0, 0), 0.05, levels=2, carry_concentration=True, mixing=0.3,
+                         fade=0.01, density=(1.0, 3.0), pressure_pin="topCenter", resample=False)
+        assert d["converged"] and meta["diags"][s]["converged"]
+    rd = lambda name: np.fromfile(tmp_path / (name + ".bin"), np.float32)
+    assert np.max(np.abs(rd("out_pos") - st["pos"])) <= 1e-5
+    assert np.max(np.abs(rd("out_vel") - st["vel"])) <= 1e-4 * max(np.abs(st["vel"]).max(), 1e-3)
+    assert np.max(np.abs(rd("out_conc") - st["conc"])) <= 1e-5
+    np.testing.assert_array_equal(rd("out_cell_density"), st["cell_density"])
+
+
+@pytest.mark.gpu
 def test_maccormack_advection_through_node_matches_oracle(tmp_path):
     """createSemiLagrangianAdvectionSolver3 (order 2) from Node, dt a live
     array0 field changed between the two dispatches: bit for bit the oracle."""

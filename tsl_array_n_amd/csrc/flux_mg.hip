@@ -104,84 +104,87 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
     if (stop && *stop) return;
     double dot = 0.0;
     int t, k0, k1;
-    if (!march_begin(M, t, k0, k1)) {
-        if (DOT) store_block_partial(dot, partials);
-        return;
-    }
-    const int half = M.run;
-    const int j = t / half;
-    const int p = t - j * half;
-    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
-    const int plane = n0 * n1;
-    const uint8_t* __restrict__ nbits = L.nb;
-    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
-    // The pair of the next plane is loaded one plane ahead: in plane k the
-    // coloured cell is 2p+s, and in planes k-1 and k+1 the cell 2p+s has the
-    // other colour, so its (unchanged) value is the partner this thread read
-    // there -- the z-neighbours come from registers.
-    Pair2<XT> xnext;
-    Pair2<uint8_t> nnext;
-    const int cp0 = 2 * p + n0 * (j + n1 * k0);
-    if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp0);
-    if (nbits) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp0);
-    float xprev = 0.0f;
-    for (int k = k0; k < k1; ++k) {
-        const int s = (j + k + color) & 1;  // which cell of the pair has the colour
-        const int i = 2 * p + s;
-        const int cp = 2 * p + n0 * (j + n1 * k);
-        const int c = cp + s;
-        const Pair2<XT> xp = xnext;
-        const Pair2<uint8_t> np = nnext;
-        if (k + 1 < n2) {
-            if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp + plane);
-            if (nbits && k + 1 < k1) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp + plane);
-        }
-        const unsigned bits = nbits ? (s ? np.hi : np.lo) : 0u;
-        float w[6];
-        face_w3(L, i, j, k, c, w);
-        const bool dir = (bits & 64u) != 0;
-        const Pair2<XT> bp = *reinterpret_cast<const Pair2<XT>*>(b + cp);
-        const float bc = to_f32(s ? bp.hi : bp.lo);
-        if (FROM_ZERO) {
-            // x = 0 on entry: A x = 0 on a free row and -0 on a Dirichlet row
-            float sum, dsum;
-            const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            row_terms<float>(0.0f, zn, w, ih2, sum, dsum);
-            const float d = dir ? -1.0f : dsum;
-            const float ax = dir ? -0.0f : sum;
-            const float v = 0.0f + ((bc - ax) / d) * omega;
-            Pair2<XT> o;
-            o.lo = to_store<XT>(s ? 0.0f : v);
-            o.hi = to_store<XT>(s ? v : 0.0f);
-            *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
-        } else {
-            const float xc = to_f32(s ? xp.hi : xp.lo);
-            const float xpart = to_f32(s ? xp.lo : xp.hi);
-            // the pair partner is the upper x neighbour of the even cell and the lower of the odd one
-            const int qo = s ? (i < n0 - 1 ? c + 1 : c) : (i > 0 ? c - 1 : c);
-            const float xo = to_f32(x[qo]);
-            float xn[6];
-            xn[0] = s ? xpart : xo;
-            xn[1] = s ? xo : xpart;
-            xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
-            xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
-            xn[4] = k > k0 ? xprev : to_f32(x[k > 0 ? c - plane : c]);
-            xn[5] = k + 1 < n2 ? to_f32(s ? xnext.hi : xnext.lo) : xc;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
-            float sum, dsum;
-            row_terms<float>(xc, xn, w, ih2, sum, dsum);
-            const float d = dir ? -1.0f : dsum;
-            const float ax = dir ? -xc : sum;
-            const XT nv = to_store<XT>(xc + ((bc - ax) / d) * omega);
-            Pair2<XT> o = xp;
-            if (s) o.hi = nv;
-            else o.lo = nv;
-            *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
-            xprev = xpart;
-            if (DOT) {
-                dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
-                dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
+    // Every thread of a DOT launch reaches the block reduction exactly once
+    // (it holds a barrier): a thread past the plane skips the march, it does
+    // not return early, since a plane need not fill whole wavefronts.
+    const bool active = march_begin(M, t, k0, k1);
+    if (!DOT && !active) return;
+    if (active) {
+        const int half = M.run;
+        const int j = t / half;
+        const int p = t - j * half;
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        // The pair of the next plane is loaded one plane ahead: in plane k the
+        // coloured cell is 2p+s, and in planes k-1 and k+1 the cell 2p+s has the
+        // other colour, so its (unchanged) value is the partner this thread read
+        // there -- the z-neighbours come from registers.
+        Pair2<XT> xnext;
+        Pair2<uint8_t> nnext;
+        const int cp0 = 2 * p + n0 * (j + n1 * k0);
+        if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp0);
+        if (nbits) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp0);
+        float xprev = 0.0f;
+        for (int k = k0; k < k1; ++k) {
+            const int s = (j + k + color) & 1;  // which cell of the pair has the colour
+            const int i = 2 * p + s;
+            const int cp = 2 * p + n0 * (j + n1 * k);
+            const int c = cp + s;
+            const Pair2<XT> xp = xnext;
+            const Pair2<uint8_t> np = nnext;
+            if (k + 1 < n2) {
+                if (!FROM_ZERO) xnext = *reinterpret_cast<const Pair2<XT>*>(x + cp + plane);
+                if (nbits && k + 1 < k1) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp + plane);
+            }
+            const unsigned bits = nbits ? (s ? np.hi : np.lo) : 0u;
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            const bool dir = (bits & 64u) != 0;
+            const Pair2<XT> bp = *reinterpret_cast<const Pair2<XT>*>(b + cp);
+            const float bc = to_f32(s ? bp.hi : bp.lo);
+            if (FROM_ZERO) {
+                // x = 0 on entry: A x = 0 on a free row and -0 on a Dirichlet row
+                float sum, dsum;
+                const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                row_terms<float>(0.0f, zn, w, ih2, sum, dsum);
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -0.0f : sum;
+                const float v = 0.0f + ((bc - ax) / d) * omega;
+                Pair2<XT> o;
+                o.lo = to_store<XT>(s ? 0.0f : v);
+                o.hi = to_store<XT>(s ? v : 0.0f);
+                *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+            } else {
+                const float xc = to_f32(s ? xp.hi : xp.lo);
+                const float xpart = to_f32(s ? xp.lo : xp.hi);
+                // the pair partner is the upper x neighbour of the even cell and the lower of the odd one
+                const int qo = s ? (i < n0 - 1 ? c + 1 : c) : (i > 0 ? c - 1 : c);
+                const float xo = to_f32(x[qo]);
+                float xn[6];
+                xn[0] = s ? xpart : xo;
+                xn[1] = s ? xo : xpart;
+                xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
+                xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
+                xn[4] = k > k0 ? xprev : to_f32(x[k > 0 ? c - plane : c]);
+                xn[5] = k + 1 < n2 ? to_f32(s ? xnext.hi : xnext.lo) : xc;
+    #pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w, ih2, sum, dsum);
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                const XT nv = to_store<XT>(xc + ((bc - ax) / d) * omega);
+                Pair2<XT> o = xp;
+                if (s) o.hi = nv;
+                else o.lo = nv;
+                *reinterpret_cast<Pair2<XT>*>(x + cp) = o;
+                xprev = xpart;
+                if (DOT) {
+                    dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
+                    dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
+                }
             }
         }
     }
@@ -441,6 +444,83 @@ __global__ void __launch_bounds__(MTPB) k_correct3(March M, int c0, int c1, int 
         if (rr) sum = sum / scale;
         if (mult != 1.0f) sum = sum * mult;
         if (!masked) xf[c] = to_store<FT>(old + sum);
+    }
+}
+
+// k_correct3 for an even fine row length: each thread owns the cell pair
+// (2p, 2p+1), whose x-taps are (p-1, p) and (p, p+1) (clamped), so the pair
+// reads 3 coarse columns instead of 4 and the fine x / mask as one vector.
+// Every cell's sum runs over the same taps in the same order as k_correct3.
+template <int FINE, typename CT, typename FT>
+__global__ void __launch_bounds__(MTPB) k_correct3p(March M, int c0, int c1, int c2, const CT* __restrict__ xc,
+                                                    int f0, int f1, int f2, FT* __restrict__ xf,
+                                                    const uint8_t* __restrict__ mf, const double* __restrict__ rr,
+                                                    float mult, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t, k0, k1;
+    if (!march_begin(M, t, k0, k1)) return;
+    const int h0 = f0 >> 1;
+    const int j = t / h0;
+    const int p = t - j * h0;
+    auto taps = [](int F, int nc, int* ti, float* tw) {
+        const int ci = F >> 1;
+        const bool even = (F & 1) == 0;
+        const int lower = ci > 0 ? ci - 1 : ci;
+        const int upper = ci < nc - 1 ? ci + 1 : ci;
+        ti[0] = even ? lower : ci;
+        tw[0] = even ? 0.25f : 0.75f;
+        ti[1] = even ? ci : upper;
+        tw[1] = even ? 0.75f : 0.25f;
+    };
+    // x-columns: lower = (p-1 clamped), p, upper = (p+1 clamped)
+    const int xl = p > 0 ? p - 1 : p, xu = p < c0 - 1 ? p + 1 : p;
+    int yj[2];
+    float wy[2];
+    taps(j, c1, yj, wy);
+    const float scale = rr ? residual_scale(rr, f0 * f1 * f2) : 1.0f;
+    const int cplane = c0 * c1;
+    for (int k = k0; k < k1; ++k) {
+        const int c = 2 * p + f0 * (j + f1 * k);
+        Pair2<uint8_t> mp{0, 0};
+        if (mf) mp = *reinterpret_cast<const Pair2<uint8_t>*>(mf + c);
+        const Pair2<FT> old = *reinterpret_cast<const Pair2<FT>*>(xf + c);
+        int zk[2];
+        float wz[2];
+        taps(k, c2, zk, wz);
+        float v[3][2][2];  // [x column][y tap][z tap]
+#pragma unroll
+        for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+            for (int cq = 0; cq < 2; ++cq) {
+                const int base = c0 * yj[bq] + cplane * zk[cq];
+                v[0][bq][cq] = to_f32(xc[xl + base]);
+                v[1][bq][cq] = to_f32(xc[p + base]);
+                v[2][bq][cq] = to_f32(xc[xu + base]);
+            }
+        // even cell: taps (lower 0.25, p 0.75); odd cell: (p 0.75, upper 0.25)
+        const float wxe[2] = {0.25f, 0.75f}, wxo[2] = {0.75f, 0.25f};
+        float se = 0.0f, so = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+                for (int cq = 0; cq < 2; ++cq) {
+                    se += v[a][bq][cq] * (wxe[a] * wy[bq] * wz[cq]);
+                    so += v[a + 1][bq][cq] * (wxo[a] * wy[bq] * wz[cq]);
+                }
+        if (rr) {
+            se = se / scale;
+            so = so / scale;
+        }
+        if (mult != 1.0f) {
+            se = se * mult;
+            so = so * mult;
+        }
+        Pair2<FT> out;
+        out.lo = mp.lo ? old.lo : to_store<FT>(to_f32(old.lo) + se);
+        out.hi = mp.hi ? old.hi : to_store<FT>(to_f32(old.hi) + so);
+        *reinterpret_cast<Pair2<FT>*>(xf + c) = out;
     }
 }
 
@@ -799,6 +879,7 @@ struct flux_multigrid {
     bool has_w = false, has_masks = false;
     bool fused = false;  // fused sweeps on every level wide enough (FLUX_FUSED_SWEEP=1)
     bool tail = true;    // the last two levels in one workgroup (FLUX_VCYCLE_TAIL=0 turns it off)
+    bool pairs = true;   // paired prolongation-correction on even rows (FLUX_CORRECT_PAIRS=0 turns it off)
 };
 
 static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
@@ -1008,7 +1089,12 @@ void restrict_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, cons
 template <int FINE, typename CT, typename FT>
 void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const double* scale, const int* stop) {
     const float mult = (float)(c.xscale / v.xscale);  // stored coarse x to the fine level's storage scale
-    if (mg->ndim == 3) {
+    if (mg->ndim == 3 && mg->pairs && (v.n[0] & 1) == 0) {
+        const March M = march_shape(v.n[0] / 2, v.n[1], v.n[2], MTPB, g_march_blocks);
+        k_correct3p<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(M) : M, c.n[0], c.n[1], c.n[2],
+                                                                (const CT*)c.x, v.n[0], v.n[1], v.n[2], (FT*)v.x,
+                                                                v.mask, scale, mult, stop);
+    } else if (mg->ndim == 3) {
         const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
         k_correct3<FINE, CT, FT><<<M.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(M) : M, c.n[0], c.n[1], c.n[2],
                                                                (const CT*)c.x, v.n[0],
@@ -1197,6 +1283,8 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     mg->fused = (fenv && fenv[0] == '1');
     const char* tenv = getenv("FLUX_VCYCLE_TAIL");
     mg->tail = !(tenv && tenv[0] == '0');
+    const char* penv = getenv("FLUX_CORRECT_PAIRS");
+    mg->pairs = !(penv && penv[0] == '0');
     mg->lv.resize(o.levels);
     const size_t es = store_size(o.storage);
     bool ok = true;

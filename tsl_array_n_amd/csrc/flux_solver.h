@@ -17,6 +17,11 @@ struct DotOut {
 int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr, DotOut* dot);
 int64_t mg_cells(const flux_multigrid* mg);
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
+// P2G with the reference's fixed-point atomics on the stream; with `conc`
+// each particle's weights are scaled by its density over the heavier one
+int p2g_fixed_async(const flux_grid3* g, const flux_field* positions, const flux_field* velocities, int64_t count,
+                    int component, float scale, float eps, flux_field* numer, flux_field* denom, flux_field* out,
+                    flux_field* valid, const flux_field* conc, float d0, float d1);
 // semi-Lagrangian advection without the final stream synchronisation
 int advection3_create(const flux_grid3* g, const flux_field* const* vel, float dt, int max_substeps, int order,
                       flux_advection3** out);

@@ -117,10 +117,18 @@ __device__ __forceinline__ int f32_to_i32_sat(float v) {
 }
 
 // buildScatter (grid_flip_solver3.js:508-535): int32 fixed-point atomics
+// With `conc` (massWeightedTransfer, grid_flip_solver3.js:493-531) every
+// corner weight is scaled by rho(c) / rho_ref, rho(c) = ma + (mb - ma) c.
 __global__ void k_p2g_scatter_fixed(G3 g, int comp, const float* __restrict__ pos, const float* __restrict__ vel,
-                                    int count, float scale, int* __restrict__ numer, int* __restrict__ denom) {
+                                    int count, float scale, int* __restrict__ numer, int* __restrict__ denom,
+                                    const float* __restrict__ conc, float ma, float mb, float mref) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= count) return;
+    float mass = 1.0f;
+    if (conc) {
+        const float c = fminf(fmaxf(conc[p], 0.0f), 1.0f);
+        mass = (ma + (mb - ma) * c) / mref;
+    }
     const int s[3] = {g.n0 + (comp == 0), g.n1 + (comp == 1), g.n2 + (comp == 2)};
     float o[3];
     mac_origin(g, comp, o);
@@ -133,7 +141,7 @@ __global__ void k_p2g_scatter_fixed(G3 g, int comp, const float* __restrict__ po
     for (int cc = 0; cc < 8; ++cc) {
         int i = (cc & 1) ? tr.i1 : tr.i0, j = (cc & 2) ? tr.j1 : tr.j0, k = (cc & 4) ? tr.k1 : tr.k0;
         int f = i + s[0] * (j + s[1] * k);
-        float w = tr.w[cc];
+        float w = conc ? tr.w[cc] * mass : tr.w[cc];
         atomicAdd(&numer[f], f32_to_i32_sat(rintf((value * w) * scale)));
         atomicAdd(&denom[f], f32_to_i32_sat(rintf(w * scale)));
     }
@@ -384,16 +392,20 @@ int flux_p2g_gather(const flux_grid3* g, const flux_field* positions, const flux
     return FLUX_OK;
 }
 
-int flux_p2g_fixed(const flux_grid3* g, const flux_field* positions, const flux_field* velocities, int64_t count,
-                   int component, float scale, float eps, flux_field* numer, flux_field* denom, flux_field* out,
-                   flux_field* valid) {
-    FLUX_REQUIRE_INIT();
+}  // extern "C"
+
+namespace flux {
+
+int p2g_fixed_async(const flux_grid3* g, const flux_field* positions, const flux_field* velocities, int64_t count,
+                    int component, float scale, float eps, flux_field* numer, flux_field* denom, flux_field* out,
+                    flux_field* valid, const flux_field* conc, float d0, float d1) {
     FLUX_TRY(check_grid(g));
     FLUX_REQUIRE(component >= 0 && component < 3, "p2g: component must be 0, 1 or 2");
     FLUX_REQUIRE(count >= 0 && count < (1ll << 31), "p2g: count out of range");
     FLUX_REQUIRE(positions && velocities, "p2g: NULL field");
     FLUX_REQUIRE((int64_t)(positions->bytes / 12) >= count && (int64_t)(velocities->bytes / 12) >= count,
                  "p2g: particle fields hold fewer than %lld particles", (long long)count);
+    if (conc) FLUX_TRY(check_field(conc, FLUX_F32, count, "p2g: concentration"));
     int64_t nf = faces_of(g->n, component);
     FLUX_TRY(check_field(numer, FLUX_I32, nf, "p2g: numer"));
     FLUX_TRY(check_field(denom, FLUX_I32, nf, "p2g: denom"));
@@ -401,15 +413,28 @@ int flux_p2g_fixed(const flux_grid3* g, const flux_field* positions, const flux_
     FLUX_TRY(check_field(valid, FLUX_I32, nf, "p2g: valid"));
     FLUX_TRY(fill_device(numer->d, FLUX_I32, nf, 0));
     FLUX_TRY(fill_device(denom->d, FLUX_I32, nf, 0));
+    const float ma = std::max(d0, 1e-4f), mb = std::max(d1, 1e-4f);
     if (count > 0)
-        k_p2g_scatter_fixed<<<blocks_for(count, XTPB), XTPB, 0, g_stream>>>(make_g3(g), component,
-                                                                             (const float*)positions->d,
-                                                                             (const float*)velocities->d, (int)count,
-                                                                             scale, (int*)numer->d, (int*)denom->d);
+        k_p2g_scatter_fixed<<<blocks_for(count, XTPB), XTPB, 0, g_stream>>>(
+            make_g3(g), component, (const float*)positions->d, (const float*)velocities->d, (int)count, scale,
+            (int*)numer->d, (int*)denom->d, conc ? (const float*)conc->d : nullptr, ma, mb, std::max(ma, mb));
     k_p2g_finalize_fixed<<<blocks_for(nf, XTPB), XTPB, 0, g_stream>>>((int)nf, (const int*)numer->d,
                                                                         (const int*)denom->d, scale, eps,
                                                                         (float*)out->d, (int*)valid->d);
     FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+}  // namespace flux
+
+extern "C" {
+
+int flux_p2g_fixed(const flux_grid3* g, const flux_field* positions, const flux_field* velocities, int64_t count,
+                   int component, float scale, float eps, flux_field* numer, flux_field* denom, flux_field* out,
+                   flux_field* valid) {
+    FLUX_REQUIRE_INIT();
+    FLUX_TRY(p2g_fixed_async(g, positions, velocities, count, component, scale, eps, numer, denom, out, valid,
+                             nullptr, 0.0f, 0.0f));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     return FLUX_OK;
 }

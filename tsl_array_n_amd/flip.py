@@ -2,8 +2,12 @@
 
 Free-surface FLIP with reduced pressure, int32 fixed-point P2G, extrapolation,
 constrainVelocity (optional static collider), deterministic resampling and
-the MGPCG pressure projection.  ``on_advance_time_step()`` runs one step;
-positions/velocities/fluid_mask/pressure and the velocity grid are fields.
+the MGPCG pressure projection; optionally a carried concentration (mixing,
+fade) coupled to the density (face weights and the reduced-pressure gravity
+term), a mass-weighted scatter, a pressure pin for closed all-fluid domains
+and phase-aware resampling.  ``on_advance_time_step()`` runs one step;
+positions/velocities/fluid_mask/pressure/concentration and the velocity grid
+are fields.
 """
 import ctypes as C
 
@@ -18,10 +22,15 @@ class Flip3Desc(C.Structure):
                 ("p2g_atomic_scale", C.c_float), ("weight_epsilon", C.c_float), ("closed_domain_flags", C.c_int),
                 ("resample", C.c_int), ("min_particles_per_cell", C.c_int), ("max_particles_per_cell", C.c_int),
                 ("extrapolation_depth", C.c_int), ("max_substeps", C.c_int), ("collider_sdf", C.c_void_p),
-                ("collider_weights", C.c_void_p * 3), ("friction", C.c_float), ("pressure", Pressure3Desc)]
+                ("collider_weights", C.c_void_p * 3), ("friction", C.c_float), ("pressure", Pressure3Desc),
+                ("carry_concentration", C.c_int), ("mixing", C.c_float), ("fade", C.c_float),
+                ("concentration_atomic_scale", C.c_float), ("ambient_density", C.c_float),
+                ("component_density", C.c_float), ("mass_weighted_transfer", C.c_int), ("pressure_pin", C.c_int),
+                ("pin_cell", C.c_int * 3), ("phase_resample", C.c_int)]
 
 
-_FIELDS = {"positions": 0, "velocities": 1, "fluid_mask": 2, "u": 3, "v": 4, "w": 5, "pressure": 6}
+_FIELDS = {"positions": 0, "velocities": 1, "fluid_mask": 2, "u": 3, "v": 4, "w": 5, "pressure": 6,
+           "concentration": 7, "cell_concentration": 8, "cell_density": 9}
 
 
 class GridFlipSolver3:
@@ -29,7 +38,9 @@ class GridFlipSolver3:
                  dt=None, gravity=(0.0, -9.81, 0.0), reduced_pressure=True, flip_ratio=0.97, velocity_damping=0.02,
                  p2g_atomic_scale=65536.0, weight_epsilon=1e-4, closed_domain_boundary_flag=63, collider_sdf=None,
                  collider_weights=None, friction=0.5, resample=None, pressure=None, max_substeps=32,
-                 extrapolation_depth=5):
+                 extrapolation_depth=5, carry_concentration=False, mixing=0.0, fade=0.0,
+                 concentration_atomic_scale=4096.0, ambient_density=None, component_density=None,
+                 mass_weighted_transfer=False, pressure_pin=None):
         if not max_particles:
             raise ValueError("createGridFlipSolver3: options.maxParticles is required.")
         if dt is None:
@@ -53,6 +64,21 @@ class GridFlipSolver3:
         d.resample = int(bool(rs.get("enabled", True)))
         d.min_particles_per_cell = int(rs.get("minParticlesPerCell", 4))
         d.max_particles_per_cell = int(rs.get("maxParticlesPerCell", 8))
+        d.phase_resample = int(bool(rs.get("phaseAware", False)))
+        d.carry_concentration = int(bool(carry_concentration))
+        d.mixing, d.fade = float(mixing), float(fade)
+        d.concentration_atomic_scale = float(concentration_atomic_scale)
+        if (ambient_density is None) != (component_density is None):
+            raise ValueError("createGridFlipSolver3: ambientDensity and componentDensity go together")
+        if ambient_density is not None:
+            d.ambient_density, d.component_density = float(ambient_density), float(component_density)
+        d.mass_weighted_transfer = int(bool(mass_weighted_transfer))
+        if pressure_pin == "topCenter":
+            d.pressure_pin = 1
+        elif pressure_pin is not None:
+            d.pressure_pin = 2
+            for a in range(3):
+                d.pin_cell[a] = int(pressure_pin[a])
         d.extrapolation_depth = int(extrapolation_depth)
         d.max_substeps = int(max_substeps)
         if collider_sdf is not None:
@@ -88,6 +114,9 @@ class GridFlipSolver3:
         self.fluid_mask = self._view("fluid_mask", F32, n)
         self.velocity_grid = [self._view(c, F32, face_shape(n, a)) for a, c in enumerate("uvw")]
         self.pressure = self._view("pressure", F32, n)
+        self.concentration = self._view("concentration", F32, (int(max_particles),)) if carry_concentration else None
+        self.cell_concentration = self._view("cell_concentration", F32, n) if carry_concentration else None
+        self.cell_density = self._view("cell_density", F32, n) if ambient_density is not None else None
         self.diagnostics = {}
 
     def _view(self, name, dtype, shape):

@@ -851,6 +851,31 @@ static napi_value js_flip3_create(napi_env env, napi_callback_info info) {
     if ((v = prop(env, o, "maxIterations")) && !is_nullish(env, v)) d.pressure.max_iterations = get_int(env, v, 0);
     if ((v = prop(env, o, "maxPlausiblePressure")) && !is_nullish(env, v))
         d.pressure.max_plausible_pressure = get_double(env, v, 0.0);
+    if ((v = prop(env, o, "carryConcentration"))) {
+        bool b = false;
+        napi_get_value_bool(env, v, &b);
+        d.carry_concentration = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "mixing"))) d.mixing = (float)get_double(env, v, 0.0);
+    if ((v = prop(env, o, "fade"))) d.fade = (float)get_double(env, v, 0.0);
+    if ((v = prop(env, o, "concentrationAtomicScale")))
+        d.concentration_atomic_scale = (float)get_double(env, v, d.concentration_atomic_scale);
+    if ((v = prop(env, o, "ambientDensity")) && !is_nullish(env, v)) d.ambient_density = (float)get_double(env, v, 0.0);
+    if ((v = prop(env, o, "componentDensity")) && !is_nullish(env, v))
+        d.component_density = (float)get_double(env, v, 0.0);
+    if ((v = prop(env, o, "massWeightedTransfer"))) {
+        bool b = false;
+        napi_get_value_bool(env, v, &b);
+        d.mass_weighted_transfer = b ? 1 : 0;
+    }
+    if ((v = prop(env, o, "pressurePinMode"))) d.pressure_pin = get_int(env, v, 0);
+    if ((v = prop(env, o, "pinCell")) && !is_nullish(env, v) && get_int_array(env, v, tmp, 3) == 3)
+        for (int i = 0; i < 3; ++i) d.pin_cell[i] = tmp[i];
+    if ((v = prop(env, o, "phaseAware"))) {
+        bool b = false;
+        napi_get_value_bool(env, v, &b);
+        d.phase_resample = b ? 1 : 0;
+    }
     flux_flip3* s = NULL;
     CHECK_FLUX(flux_flip3_create(&d, &s));
     return wrap(env, K_FLIP, s);

@@ -660,12 +660,29 @@ export function createGridBlockedBoundaryConditionSolver3( velocity, resolutionX
 export function createGridFlipSolver3( {
 	velocityGrid, gridSpacing, origin, maxParticles, dt, gravity = [ 0, - 9.81, 0 ],
 	reducedPressure = true, flipRatio = 0.97, velocityDamping = 0.02, p2gAtomicScale = 65536,
-	weightEpsilon = 1e-4, closedDomainBoundaryFlag, collider, colliderFaceWeights, resample = {}, pressure = {}
+	weightEpsilon = 1e-4, closedDomainBoundaryFlag, collider, colliderFaceWeights, resample = {}, pressure = {},
+	carryConcentration = false, mixing = 0, fade = 0, concentrationAtomicScale = 4096, ambientDensity,
+	componentDensity, massWeightedTransfer = false, pressurePin = null
 } = {} ) {
 
 	if ( ! velocityGrid ) throw new Error( 'createGridFlipSolver3: options.velocityGrid is required.' );
 	if ( ! maxParticles ) throw new Error( 'createGridFlipSolver3: options.maxParticles is required.' );
-	const { enabled: resampleEnabled = true, minParticlesPerCell = 4, maxParticlesPerCell = 8 } = resample;
+	const { enabled: resampleEnabled = true, minParticlesPerCell = 4, maxParticlesPerCell = 8, phaseAware = false } = resample;
+	let pressurePinMode = 0;
+	let pinCell = null;
+	if ( pressurePin === 'topCenter' ) pressurePinMode = 1;
+	else if ( Array.isArray( pressurePin ) && pressurePin.length === 3 ) {
+
+		pressurePinMode = 2;
+		pinCell = pressurePin;
+
+	} else if ( pressurePin !== null && pressurePin !== undefined ) {
+
+		throw new Error( `createGridFlipSolver3: pressurePin must be 'topCenter', null, or an [i, j, k] cell index, got ${ JSON.stringify( pressurePin ) }.` );
+
+	}
+
+	const coupled = ambientDensity !== undefined && componentDensity !== undefined;
 	const resolution = velocityGrid.resolution.slice();
 	const spacing = gridSpacing || velocityGrid.gridSpacing;
 	const org = origin || velocityGrid.origin;
@@ -686,7 +703,10 @@ export function createGridFlipSolver3( {
 		multigrid: pressure.multigrid || {}, preconditionerKind: PRECONDITIONERS[ pre ],
 		tolerance: pressure.tolerance === undefined ? 1e-6 : pressure.tolerance,
 		maxIterations: pressure.maxIterations === undefined ? null : pressure.maxIterations,
-		maxPlausiblePressure: pressure.maxPlausiblePressure === undefined ? null : pressure.maxPlausiblePressure
+		maxPlausiblePressure: pressure.maxPlausiblePressure === undefined ? null : pressure.maxPlausiblePressure,
+		carryConcentration, mixing, fade, concentrationAtomicScale,
+		ambientDensity: coupled ? ambientDensity : null, componentDensity: coupled ? componentDensity : null,
+		massWeightedTransfer, pressurePinMode, pinCell, phaseAware
 	} );
 	const view = ( which, type, shape ) => wrapField( addon.flip3Field( handle, which ), type, shape );
 	const [ nx, ny, nz ] = resolution;
@@ -709,7 +729,10 @@ export function createGridFlipSolver3( {
 		onAdvanceTimeStep, settings: { batchStages: true },
 		positions: view( 0, 'vec3', [ maxParticles ] ), velocities: view( 1, 'vec3', [ maxParticles ] ),
 		fluidMask: view( 2, 'float', resolution ), pressure: { data: view( 6, 'float', resolution ) },
-		velocityGrid: solverGrid, diagnostics, inputs: { collider, colliderFaceWeights }
+		velocityGrid: solverGrid, diagnostics, inputs: { collider, colliderFaceWeights },
+		concentration: carryConcentration ? view( 7, 'float', [ maxParticles ] ) : null,
+		cellConcentration: carryConcentration ? view( 8, 'float', resolution ) : null,
+		cellDensity: coupled ? view( 9, 'float', resolution ) : null
 	};
 
 }
