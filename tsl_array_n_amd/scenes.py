@@ -122,6 +122,51 @@ def random_scene(n, seed):
             "active": active, "target": np.zeros(n ** 3, np.float32), "ratio": ratio}
 
 
+def mixed_precision_scene(n, seed, band=0.25):
+    """A scene of the mixed-precision configuration: a random union of 1-6
+    spheres / boxes (radius 6-30 cells at 128^3) of the denser phase, the
+    density ratio log-uniform in [1, 1000], a band-limited Gaussian-noise
+    velocity (spectrum cut at `band` of the Nyquist wavenumber) whose
+    divergence is the RHS, closed walls and an open top layer of cells
+    (Dirichlet 0, which keeps the system well posed)."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    frac = np.zeros((n, n, n), np.float32)
+    for _ in range(rng.integers(1, 7)):
+        c = rng.uniform(0.2, 0.8, 3) * n
+        r = rng.uniform(6, 30) * n / 128.0
+        if rng.random() < 0.5:
+            inside = (i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2 < r * r
+        else:
+            inside = (np.abs(i - c[0]) < r) & (np.abs(j - c[1]) < r) & (np.abs(k - c[2]) < r)
+        frac[inside] = 1.0
+    ratio = float(np.exp(rng.uniform(0, np.log(1000.0))))
+    beta = two_phase_beta(frac.ravel(), shape, 1.0, ratio)
+    vel = []
+    for a in range(3):
+        fs = [n, n, n]
+        fs[a] += 1
+        noise = rng.standard_normal(fs[::-1])
+        spec = np.fft.rfftn(noise)
+        kz = np.abs(np.fft.fftfreq(fs[2]))[:, None, None]
+        ky = np.abs(np.fft.fftfreq(fs[1]))[None, :, None]
+        kx = np.abs(np.fft.rfftfreq(fs[0]))[None, None, :]
+        spec[(kx > 0.5 * band) | (ky > 0.5 * band) | (kz > 0.5 * band)] = 0
+        arr = np.fft.irfftn(spec, s=fs[::-1], axes=(0, 1, 2)).astype(np.float32)
+        arr /= max(float(np.abs(arr).max()), 1e-30)
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[2 - a] = 0
+        sl1[2 - a] = -1
+        arr[tuple(sl0)] = 0
+        arr[tuple(sl1)] = 0
+        vel.append(arr.ravel())
+    active = (j.ravel() == n - 1).astype(np.uint8)
+    return {"shape": shape, "spacing": (1.0 / n,) * 3, "fraction": frac.ravel(), "beta": beta, "vel": vel,
+            "active": active, "target": np.zeros(n ** 3, np.float32), "ratio": ratio}
+
+
 def mulberry32(seed, count):
     """createJitterRandom (grid/jitter_random.js:29-40): `count` doubles in [0, 1)"""
     out = np.empty(count, np.float64)
