@@ -605,6 +605,13 @@ int flux_unet_destroy(flux_unet* u);
 int flux_unet_set_params(flux_unet* u, const float* host, int64_t count);
 /* which = 0: parameters, 1: gradients of the last train step */
 int flux_unet_get_params(flux_unet* u, float* host, int64_t count, int which);
+/* Arithmetic of the inference entry points (forward, guess, guess_into):
+ * FLUX_UNET_F32 direct FP32 convolutions (the default); FLUX_UNET_BF16
+ * implicit-GEMM convolutions on the matrix cores with BF16 weights and
+ * channels-last BF16 activations, FP32 accumulation and an FP32 output.
+ * Training always runs in FP32. */
+enum { FLUX_UNET_F32 = 0, FLUX_UNET_BF16 = 1 };
+int flux_unet_set_precision(flux_unet* u, int precision);
 /* input FLUX_F32 [3][nz][ny][nx] -> out FLUX_F32 [nz][ny][nx] */
 int flux_unet_forward(flux_unet* u, const flux_field* input, flux_field* out);
 /* forward, MSE against target, backward, and an Adam step when lr > 0 */

@@ -146,3 +146,39 @@ def backward(p, cache, gy):
 def mse_loss(y, t):
     d = y - t
     return float((d * d).mean()), 2.0 * d / d.size
+
+
+def to_bf16(a):
+    """round to the nearest BF16 value, ties to even (finite inputs), returned
+    as float64: the upper 16 bits of the float32 after adding 0x7fff plus
+    the lowest kept bit."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, x):
+    """the BF16 inference form: input and weights rounded to BF16, exact
+    products summed at high precision, bias added and ReLU applied before
+    each layer's output is rounded to BF16; pooling averages BF16 values and
+    rounds the mean; the last layer's output stays unrounded.  Returns the
+    prediction [1, z, y, x]."""
+    q = {k: (to_bf16(v) if k.endswith(".w") else np.asarray(v, np.float32).astype(np.float64)) for k, v in p.items()}
+    relu = dict((n, r) for n, _, _, r in LAYERS)
+
+    def layer(name, inp):
+        z = conv3d(inp, q[name + ".w"], q[name + ".b"])
+        z = np.maximum(z, 0) if relu[name] else z
+        return to_bf16(z) if name != "out" else z
+
+    def bpool(t):
+        return to_bf16(pool(t))
+
+    s0 = layer("enc0b", layer("enc0a", to_bf16(x)))
+    s1 = layer("enc1b", layer("enc1a", bpool(s0)))
+    s2 = layer("enc2b", layer("enc2a", bpool(s1)))
+    m = layer("mid", bpool(s2))
+    d2 = layer("dec2", np.concatenate([up(m), s2], 0))
+    d1 = layer("dec1b", layer("dec1a", np.concatenate([up(d2), s1], 0)))
+    d0 = layer("dec0b", layer("dec0a", np.concatenate([up(d1), s0], 0)))
+    return layer("out", d0)

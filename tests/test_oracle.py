@@ -458,6 +458,36 @@ def test_unet_conv_matches_scipy_correlate():
         np.testing.assert_allclose(got[co], ref, rtol=1e-12, atol=1e-12)
 
 
+def test_bf16_rounding_matches_torch():
+    """the oracle's round-to-nearest-even BF16 against torch's CPU cast, on
+    random values over many binades and on exact ties (low half 0x8000)"""
+    import torch
+    from oracle import unet_ref as U
+    rng = np.random.default_rng(4)
+    v = (rng.standard_normal(20000) * np.exp2(rng.integers(-30, 30, 20000))).astype(np.float32)
+    bits = v.view(np.uint32)
+    ties = ((bits[:2000] & 0xFFFF0000) | 0x8000).view(np.float32)
+    for a in (v, ties, np.float32([0.0, -0.0, 1.0, -2.5, 65504.0])):
+        ref = torch.from_numpy(a).to(torch.bfloat16).to(torch.float64).numpy()
+        np.testing.assert_array_equal(U.to_bf16(a), ref)
+
+
+def test_unet_bf16_forward_is_the_fp32_forward_rounded():
+    """the BF16 restatement stays within BF16 rounding of the float64
+    forward and differs from it (the rounding happens)"""
+    from oracle import unet_ref as U
+    rng = np.random.default_rng(6)
+    p = U.init_params(2)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.1
+    x = rng.standard_normal((3, 8, 8, 16))
+    y, _ = U.forward(p, x)
+    yb = U.forward_bf16(p, x)
+    r = np.linalg.norm(yb - y) / np.linalg.norm(y)
+    assert 1e-5 < r < 2e-2, r
+
+
 def test_unet_gradients_match_finite_differences():
     from oracle import unet_ref as U
     assert 600_000 < U.param_count() < 720_000

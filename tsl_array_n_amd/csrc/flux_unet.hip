@@ -311,6 +311,196 @@ int64_t unet_param_count() {
     return n;
 }
 
+// ------------------------------------------------ BF16 inference on MFMA
+// Implicit GEMM per layer: M = voxels, N = output channels (padded to 16),
+// K = 27 taps x input channels (padded to 8), ordered tap-major in groups of
+// 8 channels: group G = tap * CB + cb holds channels cb*8 .. cb*8+7 of tap
+// (dz*9 + dy*3 + dx).  One v_mfma_f32_16x16x32_bf16 consumes 4 groups.
+// Operand layout of that instruction (lane l, r = l % 16, q = l / 16):
+//   A[row r][k 8q+j], B[k 8q+j][col r], D[row 4q+i][col r], j < 8, i < 4.
+// Activations are channels-last BF16 ([voxel][C], 16-byte channel groups);
+// the accumulation is FP32; bias, ReLU and the rounding to BF16 are the
+// epilogue.  The last layer (one channel) is stored as FP32.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// round to nearest even (activations are finite)
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+struct MfmaIn {
+    const uint16_t* a;  // c1 channels per voxel, half resolution when up != 0
+    const uint16_t* b;  // c2 channels per voxel, full resolution
+    int c1, c2, up;
+};
+
+// weights packed as wq[G][npad][8]: B fragment of group G, column co = 8 bf16
+__global__ void k_pack_weights(const float* __restrict__ w, uint16_t* __restrict__ wq, int cout, int cin, int cb_per_tap,
+                               int groups, int npad) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)groups * npad * 8;
+    if (t >= total) return;
+    int j = (int)(t % 8);
+    int co = (int)((t / 8) % npad);
+    int g = (int)(t / (8 * (int64_t)npad));
+    int tap = g / cb_per_tap, ci = (g % cb_per_tap) * 8 + j;
+    float v = (tap < 27 && co < cout && ci < cin) ? w[((int64_t)co * cin + ci) * 27 + tap] : 0.0f;
+    wq[t] = f32_to_bf16(v);
+}
+
+// [3][nz][ny][nx] FP32 -> [voxel][8] BF16 (channels 3..7 zero)
+__global__ void k_input_to_cl(const float* __restrict__ x, uint16_t* __restrict__ out, int64_t nv) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    uint4 o;
+    o.x = f32_to_bf16(x[v]) | ((uint32_t)f32_to_bf16(x[nv + v]) << 16);
+    o.y = f32_to_bf16(x[2 * nv + v]);
+    o.z = 0;
+    o.w = 0;
+    reinterpret_cast<uint4*>(out)[v] = o;
+}
+
+// 2x2x2 average pooling of channels-last BF16, one thread per (coarse voxel,
+// group of 8 channels); the mean of 8 BF16 values is exact in FP32
+__global__ void k_pool_cl(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int c, int nz, int ny, int nx) {
+    const int hz = nz >> 1, hy = ny >> 1, hx = nx >> 1, cg = c / 8;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)hz * hy * hx * cg;
+    if (t >= total) return;
+    int g = (int)(t % cg);
+    int64_t cv = t / cg;
+    int x = (int)(cv % hx), y = (int)((cv / hx) % hy), z = (int)(cv / ((int64_t)hx * hy));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int d = 0; d < 8; ++d) {
+        int64_t fv = ((int64_t)(2 * z + (d >> 2)) * ny + (2 * y + ((d >> 1) & 1))) * nx + (2 * x + (d & 1));
+        uint4 r = reinterpret_cast<const uint4*>(in + fv * c)[g];
+        uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            s[2 * k] += bf16_to_f32(w4[k] & 0xFFFFu);
+            s[2 * k + 1] += bf16_to_f32(w4[k] >> 16);
+        }
+    }
+    uint4 o;
+    uint32_t* ow = &o.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        ow[k] = f32_to_bf16(s[2 * k] * 0.125f) | ((uint32_t)f32_to_bf16(s[2 * k + 1] * 0.125f) << 16);
+    reinterpret_cast<uint4*>(out + cv * c)[g] = o;
+}
+
+// one wavefront computes MT x NT tiles of 16 voxels x 16 channels; the block
+// holds 4 independent wavefronts.  The fragments of the next K step are
+// loaded before the MFMAs of the current one.
+template <int MT, int NT>
+__global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* __restrict__ wq,
+                                                    const float* __restrict__ bias, void* __restrict__ out, int cout,
+                                                    int npad, int groups, int nz, int ny, int nx, int relu,
+                                                    int out_f32) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int n_groups_n = npad / (16 * NT);
+    const int64_t mgrp = wave / n_groups_n;
+    const int ngrp = (int)(wave % n_groups_n);
+    const int64_t nv = (int64_t)nz * ny * nx;
+    const int64_t vbase = mgrp * 16 * MT;
+    if (vbase >= nv) return;  // wavefront-uniform
+    const int r = lane & 15, q = lane >> 4;
+    const int cb_per_tap = (in.c1 + in.c2) >> 3, cb1 = in.c1 >> 3;
+    const int hz = nz >> 1, hy = ny >> 1, hx = nx >> 1;
+    (void)hz;
+    int vx[MT], vy[MT], vz[MT];
+    bool vok[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        int64_t v = vbase + 16 * m + r;
+        vok[m] = v < nv;
+        vx[m] = (int)(v % nx);
+        vy[m] = (int)((v / nx) % ny);
+        vz[m] = (int)(v / ((int64_t)nx * ny));
+    }
+    int tap = q / cb_per_tap, cb = q % cb_per_tap;  // this lane's group G = 4c + q
+    const uint16_t* wlane = wq + ((int64_t)q * npad + ngrp * NT * 16 + r) * 8;
+    const int64_t wstep = (int64_t)4 * npad * 8;
+
+    auto load_a = [&](int tp, int cbk, bf16x8* a) {
+        const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int x = vx[m] + dx - 1, y = vy[m] + dy - 1, z = vz[m] + dz - 1;
+            uint4 raw = make_uint4(0, 0, 0, 0);
+            if (vok[m] && tp < 27 && x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz) {
+                const uint16_t* p;
+                if (cbk < cb1) {
+                    p = in.up ? in.a + ((((int64_t)(z >> 1) * hy + (y >> 1)) * hx + (x >> 1)) * in.c1 + cbk * 8)
+                              : in.a + ((((int64_t)z * ny + y) * nx + x) * in.c1 + cbk * 8);
+                } else {
+                    p = in.b + ((((int64_t)z * ny + y) * nx + x) * in.c2 + (cbk - cb1) * 8);
+                }
+                raw = *reinterpret_cast<const uint4*>(p);
+            }
+            a[m] = __builtin_bit_cast(bf16x8, raw);
+        }
+    };
+    auto load_b = [&](const uint16_t* wp, bf16x8* b) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) b[n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wp + n * 128));
+    };
+
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a[MT], b[NT], a2[MT], b2[NT];
+    load_a(tap, cb, a);
+    load_b(wlane, b);
+    const int nchunks = groups >> 2;
+    for (int c = 0; c < nchunks; ++c) {
+        const bool more = c + 1 < nchunks;
+        if (more) {
+            cb += 4;
+            while (cb >= cb_per_tap) {
+                cb -= cb_per_tap;
+                ++tap;
+            }
+            load_a(tap, cb, a2);
+            load_b(wlane + (c + 1) * wstep, b2);
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) a[m] = a2[m];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) b[n] = b2[n];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = (ngrp * NT + n) * 16 + r;
+        if (co >= cout) continue;
+        const float bco = bias[co];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t v = vbase + 16 * m + 4 * q + i;
+                if (v >= nv) continue;
+                float s = acc[m][n][i] + bco;
+                if (relu) s = fmaxf(s, 0.0f);
+                if (out_f32) reinterpret_cast<float*>(out)[v * cout + co] = s;
+                else reinterpret_cast<uint16_t*>(out)[v * cout + co] = f32_to_bf16(s);
+            }
+    }
+}
+
 }  // namespace flux
 
 using namespace flux;
@@ -333,6 +523,15 @@ struct flux_unet {
     double* scal = nullptr;       // [0] loss, [1] b.b
     int64_t nvox[4];
     int steps = 0;
+    // BF16 inference (flux_unet_set_precision)
+    int precision = FLUX_UNET_F32;
+    uint16_t* wq = nullptr;        // packed weights of every layer
+    int64_t off_wq[kNumLayers];
+    int groups[kNumLayers], npad[kNumLayers];
+    uint16_t* y16[kNumLayers] = {};  // channels-last layer outputs (the last layer writes y[L_OUT])
+    uint16_t* pooled16[3] = {};
+    uint16_t* x16 = nullptr;       // [voxel][8]
+    int64_t version = 1, packed_version = 0;
 };
 
 namespace {
@@ -407,6 +606,121 @@ int forward(flux_unet* u, const float* x) {
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
+
+// ---- BF16 inference path
+MfmaIn mfma_input(const flux_unet* u, int l) {
+    switch (l) {
+        case L_E0A: return MfmaIn{u->x16, nullptr, 8, 0, 0};
+        case L_E0B: return MfmaIn{u->y16[L_E0A], nullptr, 16, 0, 0};
+        case L_E1A: return MfmaIn{u->pooled16[0], nullptr, 16, 0, 0};
+        case L_E1B: return MfmaIn{u->y16[L_E1A], nullptr, 32, 0, 0};
+        case L_E2A: return MfmaIn{u->pooled16[1], nullptr, 32, 0, 0};
+        case L_E2B: return MfmaIn{u->y16[L_E2A], nullptr, 64, 0, 0};
+        case L_MID: return MfmaIn{u->pooled16[2], nullptr, 64, 0, 0};
+        case L_D2: return MfmaIn{u->y16[L_MID], u->y16[L_E2B], 64, 64, 1};
+        case L_D1A: return MfmaIn{u->y16[L_D2], u->y16[L_E1B], 64, 32, 1};
+        case L_D1B: return MfmaIn{u->y16[L_D1A], nullptr, 32, 0, 0};
+        case L_D0A: return MfmaIn{u->y16[L_D1B], u->y16[L_E0B], 32, 16, 1};
+        case L_D0B: return MfmaIn{u->y16[L_D0A], nullptr, 16, 0, 0};
+        default: return MfmaIn{u->y16[L_D0B], nullptr, 16, 0, 0};
+    }
+}
+
+int mfma_channels(int l) { return l == L_E0A ? 8 : kLayers[l].cin; }
+
+// allocate the packed weights and the channels-last activations
+int bf16_setup(flux_unet* u) {
+    if (u->wq) return FLUX_OK;
+    int64_t off = 0;
+    for (int l = 0; l < kNumLayers; ++l) {
+        const int cbt = mfma_channels(l) / 8;
+        u->groups[l] = (27 * cbt + 3) / 4 * 4;
+        u->npad[l] = (kLayers[l].cout + 15) / 16 * 16;
+        u->off_wq[l] = off;
+        off += (int64_t)u->groups[l] * u->npad[l] * 8;
+    }
+    bool ok = true;
+    auto alloc = [&](uint16_t*& p, int64_t n) {
+        p = (uint16_t*)dmalloc(n * 2);
+        ok = ok && p;
+    };
+    alloc(u->wq, off);
+    alloc(u->x16, 8 * u->nvox[0]);
+    for (int l = 0; l < L_OUT; ++l) alloc(u->y16[l], (int64_t)kLayers[l].cout * u->nvox[kLayers[l].level]);
+    alloc(u->pooled16[0], 16 * u->nvox[1]);
+    alloc(u->pooled16[1], 32 * u->nvox[2]);
+    alloc(u->pooled16[2], 64 * u->nvox[3]);
+    if (!ok) return set_error(FLUX_ERR_NOMEM, "unet: BF16 buffers: allocation failed");
+    u->packed_version = 0;
+    return FLUX_OK;
+}
+
+template <int MT, int NT>
+void launch_mfma(const MfmaIn& in, const uint16_t* wq, const float* b, void* out, int cout, int npad, int groups,
+                 int nz, int ny, int nx, int relu, int out_f32) {
+    const int64_t nv = (int64_t)nz * ny * nx;
+    const int64_t waves = (nv + 16 * MT - 1) / (16 * MT) * (npad / (16 * NT));
+    k_conv3_mfma<MT, NT><<<(unsigned)((waves + 3) / 4), 256, 0, g_stream>>>(in, wq, b, out, cout, npad, groups, nz, ny,
+                                                                          nx, relu, out_f32);
+}
+
+// the widest tile (4 x N/16) that still gives >= 4096 wavefronts; smaller
+// levels trade tile reuse for parallelism
+void conv_mfma(flux_unet* u, int l, void* out, int out_f32) {
+    const LayerSpec& s = kLayers[l];
+    const int nx = u->n[0] >> s.level, ny = u->n[1] >> s.level, nz = u->n[2] >> s.level;
+    const int64_t nv = (int64_t)nz * ny * nx;
+    const int ntn = u->npad[l] / 16;
+    int mt = 4, nt = ntn;
+    auto waves = [&]() { return (nv + 16 * mt - 1) / (16 * mt) * (ntn / nt); };
+    while (waves() < 4096 && (mt > 1 || nt > 1)) {
+        if (mt > 1) mt >>= 1;
+        else nt >>= 1;
+    }
+    MfmaIn in = mfma_input(u, l);
+    const uint16_t* wq = u->wq + u->off_wq[l];
+    const float* b = u->params + u->off_b[l];
+    const int cp = s.cout, np = u->npad[l], g = u->groups[l];
+#define FLUX_MFMA_CASE(M, N)                                                             \
+    if (mt == M && nt == N) {                                                            \
+        launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);       \
+        return;                                                                          \
+    }
+    FLUX_MFMA_CASE(4, 4) FLUX_MFMA_CASE(4, 2) FLUX_MFMA_CASE(4, 1)
+    FLUX_MFMA_CASE(2, 4) FLUX_MFMA_CASE(2, 2) FLUX_MFMA_CASE(2, 1)
+    FLUX_MFMA_CASE(1, 4) FLUX_MFMA_CASE(1, 2) FLUX_MFMA_CASE(1, 1)
+#undef FLUX_MFMA_CASE
+}
+
+int forward_bf16(flux_unet* u, const float* x) {
+    FLUX_TRY(bf16_setup(u));
+    if (u->packed_version != u->version) {
+        for (int l = 0; l < kNumLayers; ++l) {
+            const int64_t total = (int64_t)u->groups[l] * u->npad[l] * 8;
+            k_pack_weights<<<blocks_for(total, UTPB), UTPB, 0, g_stream>>>(
+                u->params + u->off_w[l], u->wq + u->off_wq[l], kLayers[l].cout, kLayers[l].cin,
+                mfma_channels(l) / 8, u->groups[l], u->npad[l]);
+        }
+        u->packed_version = u->version;
+    }
+    k_input_to_cl<<<blocks_for(u->nvox[0], UTPB), UTPB, 0, g_stream>>>(x, u->x16, u->nvox[0]);
+    auto pool = [&](int src, int dst, int c, int level) {
+        const int nx = u->n[0] >> level, ny = u->n[1] >> level, nz = u->n[2] >> level;
+        const int64_t total = (int64_t)(nz >> 1) * (ny >> 1) * (nx >> 1) * (c / 8);
+        k_pool_cl<<<blocks_for(total, UTPB), UTPB, 0, g_stream>>>(u->y16[src], u->pooled16[dst], c, nz, ny, nx);
+    };
+    for (int l = 0; l < kNumLayers; ++l) {
+        if (l == L_OUT) conv_mfma(u, l, u->y[L_OUT], 1);
+        else conv_mfma(u, l, u->y16[l], 0);
+        if (l == L_E0B) pool(L_E0B, 0, 16, 0);
+        if (l == L_E1B) pool(L_E1B, 1, 32, 1);
+        if (l == L_E2B) pool(L_E2B, 2, 64, 2);
+    }
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int infer(flux_unet* u, const float* x) { return u->precision == FLUX_UNET_BF16 ? forward_bf16(u, x) : forward(u, x); }
 
 // gradient of layer l's input from gy[l]: returns into dst (full-res, cin
 // channels laid out [up part | skip part]) -- computed as a conv of the masked
@@ -509,6 +823,10 @@ int flux_unet_destroy(flux_unet* u) {
         dfree(u->pooled[q]);
         dfree(u->gpool[q]);
     }
+    dfree(u->wq);
+    dfree(u->x16);
+    for (int l = 0; l < kNumLayers; ++l) dfree(u->y16[l]);
+    for (int q = 0; q < 3; ++q) dfree(u->pooled16[q]);
     dfree(u->part);
     dfree(u->scal);
     delete u;
@@ -578,6 +896,7 @@ int flux_unet_set_params(flux_unet* u, const float* host, int64_t count) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(u && host && count == u->nparams, "unet: expected %lld parameters", (long long)(u ? u->nparams : 0));
     FLUX_HIP(hipMemcpy(u->params, host, count * 4, hipMemcpyHostToDevice));
+    u->version += 1;
     return FLUX_OK;
 }
 
@@ -591,13 +910,22 @@ int flux_unet_get_params(flux_unet* u, float* host, int64_t count, int which) {
     return FLUX_OK;
 }
 
+int flux_unet_set_precision(flux_unet* u, int precision) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(u, "unet: NULL");
+    FLUX_REQUIRE(precision == FLUX_UNET_F32 || precision == FLUX_UNET_BF16, "unet: unknown precision %d", precision);
+    if (precision == FLUX_UNET_BF16) FLUX_TRY(bf16_setup(u));
+    u->precision = precision;
+    return FLUX_OK;
+}
+
 // forward of a 3-channel input [3][nz][ny][nx]; out [nz][ny][nx]
 int flux_unet_forward(flux_unet* u, const flux_field* input, flux_field* out) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(u, "unet: NULL");
     FLUX_TRY(check_field(input, FLUX_F32, 3 * u->nvox[0], "unet: input"));
     FLUX_TRY(check_field(out, FLUX_F32, u->nvox[0], "unet: output"));
-    FLUX_TRY(forward(u, (const float*)input->d));
+    FLUX_TRY(infer(u, (const float*)input->d));
     k_ucopy<<<blocks_for(u->nvox[0], UTPB), UTPB, 0, g_stream>>>(u->y[L_OUT], (float*)out->d, u->nvox[0]);
     FLUX_LAUNCH_CHECK();
     FLUX_HIP(hipStreamSynchronize(g_stream));
@@ -623,6 +951,7 @@ int flux_unet_train_step(flux_unet* u, const flux_field* input, const flux_field
         float c1 = 1.0f - powf(beta1, (float)u->steps), c2 = 1.0f - powf(beta2, (float)u->steps);
         k_adam<<<blocks_for(u->nparams, UTPB), UTPB, 0, g_stream>>>(u->params, u->grads, u->adam_m, u->adam_v,
                                                                      u->nparams, lr, beta1, beta2, eps, c1, c2);
+        u->version += 1;
     }
     FLUX_LAUNCH_CHECK();
     double l = 0;
@@ -662,7 +991,7 @@ int flux_unet_guess(flux_unet* u, const flux_field* input, flux_field* guess) {
     int64_t nv = u->nvox[0];
     FLUX_TRY(check_field(input, FLUX_F32, 3 * nv, "unet: input"));
     FLUX_TRY(check_field(guess, FLUX_F32, nv, "unet: guess"));
-    FLUX_TRY(forward(u, (const float*)input->d));
+    FLUX_TRY(infer(u, (const float*)input->d));
     k_scale_out<<<blocks_for(nv, UTPB), UTPB, 0, g_stream>>>(u->y[L_OUT], u->scal + 1, nv, (float*)guess->d);
     FLUX_LAUNCH_CHECK();
     FLUX_HIP(hipStreamSynchronize(g_stream));
@@ -676,7 +1005,7 @@ int flux_unet_guess_into(flux_unet* u, const flux_field* input, const flux_field
     FLUX_TRY(check_field(input, FLUX_F32, 3 * nv, "unet: input"));
     FLUX_TRY(check_field(pressure, FLUX_F32, nv, "unet: pressure"));
     if (mask) FLUX_TRY(check_field(mask, FLUX_U8, nv, "unet: mask"));
-    FLUX_TRY(forward(u, (const float*)input->d));
+    FLUX_TRY(infer(u, (const float*)input->d));
     k_guess_into<<<blocks_for(nv, UTPB), UTPB, 0, g_stream>>>(u->y[L_OUT], u->scal + 1,
                                                                mask ? (const uint8_t*)mask->d : nullptr, nv,
                                                                (float*)pressure->d);

@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _check, _lib, array_n
+from . import FluxError, _check, _lib, array_n
 
 _lib.flux_unet_param_count.restype = C.c_int64
 
@@ -55,16 +55,26 @@ def init_params(seed=0):
 
 
 class UNet:
-    def __init__(self, n, training=False, params=None):
+    def __init__(self, n, training=False, params=None, precision="f32"):
+        """precision "f32" (direct FP32 convolutions) or "bf16" (implicit-GEMM
+        convolutions on the matrix cores) for forward/guess; training is FP32"""
         self.n = tuple(n)
         h = C.c_void_p()
         _check(_lib.flux_unet_create(int(n[0]), int(n[1]), int(n[2]), int(bool(training)), C.byref(h)), "unet")
         self.h = h.value
+        self.set_precision(precision)
         self.count = param_count()
         self.set_params(init_params() if params is None else params)
         cells = int(np.prod(n))
         self.input = array_n("float", (3 * cells,))
         self.output = array_n("float", (cells,))
+
+    def set_precision(self, precision):
+        code = {"f32": 0, "bf16": 1}.get(precision)
+        if code is None:
+            raise FluxError("unet: precision must be 'f32' or 'bf16', got %r" % (precision,))
+        _check(_lib.flux_unet_set_precision(C.c_void_p(self.h), code))
+        self.precision = precision
 
     def set_params(self, params):
         vec = flatten(params) if isinstance(params, dict) else np.ascontiguousarray(params, np.float32)
