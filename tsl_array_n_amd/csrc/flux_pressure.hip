@@ -359,6 +359,12 @@ int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, doub
     return flux_pcg_scaled_start(p->pcg, guess, alpha);
 }
 
+int flux_pressure3_block_guess(flux_pressure3* p, const flux_field* guess, int block, int* coarse_iterations) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p && guess, "blockGuess: NULL argument");
+    return flux_pcg_block_start(p->pcg, guess, block, coarse_iterations);
+}
+
 int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
                          flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
     FLUX_REQUIRE_INIT();

@@ -129,14 +129,18 @@ class LearnedProjector:
     scaled=True (default) adds the guess to the assembled start along its
     energy-optimal step (pressure += alpha * guess, alpha = g.r / g.Ag), so a
     guess pointing the wrong way is scaled toward zero; scaled=False writes
-    the raw guess into the free cells."""
+    the raw guess into the free cells.  block=B (with scaled=True) replaces
+    the single step by the coarse correction with one alpha per block of B^3
+    cells (the Galerkin solve in the span of the block-restricted guesses)."""
 
-    def __init__(self, solver, net, density_ratio, scaled=True):
+    def __init__(self, solver, net, density_ratio, scaled=True, block=None):
         self.solver = solver
         self.net = net
         self.density_ratio = float(density_ratio)
         self.scaled = scaled
+        self.block = block
         self.alpha = None
+        self.coarse_iterations = None
         self._g = array_n("float", solver.resolution) if scaled else None
 
     def project(self, vin, vout=None):
@@ -145,7 +149,10 @@ class LearnedProjector:
         self.net.features(s.b, s.dirichlet_mask, s.weights, self.density_ratio)
         if self.scaled:
             self.net.guess(self._g)
-            self.alpha = s.scaled_guess(self._g)
+            if self.block:
+                self.coarse_iterations = s.block_guess(self._g, self.block)
+            else:
+                self.alpha = s.scaled_guess(self._g)
         else:
             _check(_lib.flux_unet_guess_into(C.c_void_p(self.net.h), C.c_void_p(self.net.input.h),
                                              C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
@@ -204,3 +211,33 @@ def train(F, n, pairs, steps, lr=1e-3, seed=0, log=None):
         if log and (s % 50 == 0 or s == steps - 1):
             log("step %d loss %.4e" % (s, losses[-1]))
     return net, losses
+
+
+def main(argv=None):
+    """time the network's forward: python -m tsl_array_n_amd.learned
+    --size 128 --precision bf16 --reps 20 (random He-uniform weights, random
+    input); prints one line with the time per forward and the FLOP rate"""
+    import argparse
+    from . import init, timer_start, timer_stop
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--precision", default="bf16", choices=["f32", "bf16"])
+    ap.add_argument("--reps", type=int, default=20)
+    a = ap.parse_args(argv)
+    init(0)
+    n = (a.size,) * 3
+    net = UNet(n, precision=a.precision)
+    rng = np.random.default_rng(0)
+    net.forward(rng.standard_normal(3 * a.size ** 3).astype(np.float32))
+    timer_start()
+    for _ in range(a.reps):
+        net.forward()
+    ms = timer_stop() / a.reps
+    levels = {"enc0a": 0, "enc0b": 0, "enc1a": 1, "enc1b": 1, "enc2a": 2, "enc2b": 2, "mid": 3, "dec2": 2,
+              "dec1a": 1, "dec1b": 1, "dec0a": 0, "dec0b": 0, "out": 0}
+    flops = sum(2 * 27 * ci * co * (a.size >> levels[name]) ** 3 for name, ci, co in LAYERS)
+    print("unet forward %d^3 %s: %.3f ms, %.1f TFLOP/s" % (a.size, a.precision, ms, flops / ms * 1e-9))
+
+
+if __name__ == "__main__":
+    main()
