@@ -396,6 +396,8 @@ __global__ void k_pool_cl(const uint16_t* __restrict__ in, uint16_t* __restrict_
 // one wavefront computes MT x NT tiles of 16 voxels x 16 channels; the block
 // holds 4 independent wavefronts.  The fragments of the next K step are
 // loaded before the MFMAs of the current one.
+__device__ __forceinline__ uint32_t bit_of(bool b) { return b ? 1u : 0u; }
+
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* __restrict__ wq,
                                                     const float* __restrict__ bias, void* __restrict__ out, int cout,
@@ -413,15 +415,26 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
     const int cb_per_tap = (in.c1 + in.c2) >> 3, cb1 = in.c1 >> 3;
     const int hz = nz >> 1, hy = ny >> 1, hx = nx >> 1;
     (void)hz;
-    int vx[MT], vy[MT], vz[MT];
-    bool vok[MT];
+    // per tile row: flat voxel index, coordinates and the 27-bit mask of the
+    // taps whose neighbour lies inside the grid (0 past the last voxel)
+    int vx[MT], vy[MT], vz[MT], vidx[MT];
+    uint32_t vmask[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-        int64_t v = vbase + 16 * m + r;
-        vok[m] = v < nv;
-        vx[m] = (int)(v % nx);
-        vy[m] = (int)((v / nx) % ny);
-        vz[m] = (int)(v / ((int64_t)nx * ny));
+        // voxel counts stay below 2^31 (checked on the host): 32-bit division
+        const uint32_t v = (uint32_t)vbase + 16u * m + r;
+        const uint32_t row = v / (uint32_t)nx;
+        vidx[m] = (int)v;
+        vx[m] = (int)(v - row * (uint32_t)nx);
+        vz[m] = (int)(row / (uint32_t)ny);
+        vy[m] = (int)(row - (uint32_t)vz[m] * (uint32_t)ny);
+        // tap t = dz*9 + dy*3 + dx is valid when each axis' neighbour is:
+        // x repeats its 3-bit pattern 9 times, y fills 3-bit groups, z 9-bit ones
+        const uint32_t mx = (vx[m] > 0 ? 1u : 0u) | 2u | (vx[m] < nx - 1 ? 4u : 0u);
+        const uint32_t rx = mx * 0x1249249u;
+        const uint32_t ry = ((bit_of(vy[m] > 0) * 7u) | (7u << 3) | (bit_of(vy[m] < ny - 1) * (7u << 6))) * 0x40201u;
+        const uint32_t rz = (bit_of(vz[m] > 0) * 0x1FFu) | (0x1FFu << 9) | (bit_of(vz[m] < nz - 1) * (0x1FFu << 18));
+        vmask[m] = (int64_t)v < nv ? (rx & ry & rz) : 0u;
     }
     int tap = q / cb_per_tap, cb = q % cb_per_tap;  // this lane's group G = 4c + q
     const uint16_t* wlane = wq + ((int64_t)q * npad + ngrp * NT * 16 + r) * 8;
@@ -429,19 +442,23 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
 
     auto load_a = [&](int tp, int cbk, bf16x8* a) {
         const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+        const bool from_a = cbk < cb1;
+        const bool up = in.up && from_a;
+        const uint16_t* base = from_a ? in.a : in.b;
+        const int64_t cs = from_a ? in.c1 : in.c2;
+        const int coff = (from_a ? cbk : cbk - cb1) * 8;
+        const int delta = ((dz - 1) * ny + (dy - 1)) * nx + (dx - 1);
+        const uint32_t bit = tp < 27 ? 1u << tp : 0u;
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
-            const int x = vx[m] + dx - 1, y = vy[m] + dy - 1, z = vz[m] + dz - 1;
             uint4 raw = make_uint4(0, 0, 0, 0);
-            if (vok[m] && tp < 27 && x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz) {
-                const uint16_t* p;
-                if (cbk < cb1) {
-                    p = in.up ? in.a + ((((int64_t)(z >> 1) * hy + (y >> 1)) * hx + (x >> 1)) * in.c1 + cbk * 8)
-                              : in.a + ((((int64_t)z * ny + y) * nx + x) * in.c1 + cbk * 8);
-                } else {
-                    p = in.b + ((((int64_t)z * ny + y) * nx + x) * in.c2 + (cbk - cb1) * 8);
-                }
-                raw = *reinterpret_cast<const uint4*>(p);
+            if (vmask[m] & bit) {
+                int64_t qv;
+                if (up)
+                    qv = ((int64_t)((vz[m] + dz - 1) >> 1) * hy + ((vy[m] + dy - 1) >> 1)) * hx + ((vx[m] + dx - 1) >> 1);
+                else
+                    qv = (int64_t)(vidx[m] + delta);
+                raw = *reinterpret_cast<const uint4*>(base + qv * cs + coff);
             }
             a[m] = __builtin_bit_cast(bf16x8, raw);
         }
@@ -631,6 +648,8 @@ int mfma_channels(int l) { return l == L_E0A ? 8 : kLayers[l].cin; }
 // allocate the packed weights and the channels-last activations
 int bf16_setup(flux_unet* u) {
     if (u->wq) return FLUX_OK;
+    // the MFMA loader keeps flat voxel indices in 32 bits
+    FLUX_REQUIRE(u->nvox[0] < ((int64_t)1 << 31), "unet: BF16 inference needs fewer than 2^31 voxels");
     int64_t off = 0;
     for (int l = 0; l < kNumLayers; ++l) {
         const int cbt = mfma_channels(l) / 8;
