@@ -94,3 +94,28 @@ def test_bf16_inference_time(flux, size):
         size, ms["f32"], flops / ms["f32"] * 1e-9, ms["bf16"], flops / ms["bf16"] * 1e-9, rel(ys["bf16"], ys["f32"])))
     assert rel(ys["bf16"], ys["f32"]) <= 2e-2
     assert ms["bf16"] < ms["f32"]
+
+
+@pytest.mark.parametrize("n,mode", [((32, 16, 8), "2"), ((64, 32, 32), "2"), ((128, 64, 32), "1")])
+def test_halo_tiled_convolutions_are_bit_identical(flux, n, mode):
+    """the halo-tiled kernel (levels of whole 16x4x4 bricks, input brick in
+    LDS) does the flat kernel's arithmetic in the same order: forced
+    wherever it fits (mode 2) and by the default rule (mode 1)"""
+    import os
+    from tsl_array_n_amd import learned as L
+    p = {k: v.astype(np.float32) for k, v in random_params(9).items()}
+    x = np.random.default_rng(10).standard_normal(3 * int(np.prod(n))).astype(np.float32)
+    outs = []
+    for flag in ("0", mode):
+        old = os.environ.get("FLUX_UNET_HALO")
+        os.environ["FLUX_UNET_HALO"] = flag
+        try:
+            net = L.UNet(n, params=p, precision="bf16")
+        finally:
+            if old is None:
+                del os.environ["FLUX_UNET_HALO"]
+            else:
+                os.environ["FLUX_UNET_HALO"] = old
+        outs.append(net.forward(x).to_array())
+    assert np.all(np.isfinite(outs[0])) and np.any(outs[0] != 0)
+    np.testing.assert_array_equal(outs[1], outs[0])

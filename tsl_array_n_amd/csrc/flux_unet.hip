@@ -518,6 +518,113 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
     }
 }
 
+// Halo-tiled form for levels made of whole 16 x 4 x 4 bricks: the block
+// copies its brick's input with a one-voxel halo (18 x 6 x 6 voxels, all
+// channels, zero outside the grid; upsampling and concat resolved while
+// copying) to LDS once, then each wavefront computes one z-slice of the
+// brick (tile row m = y offset, lane row = x offset) with every N tile,
+// reading A fragments from LDS.  Same K order and fragments as
+// k_conv3_mfma, so the results are bit-identical.
+static constexpr int kHaloVox = 18 * 6 * 6;
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_conv3_mfma_halo(MfmaIn in, const uint16_t* __restrict__ wq,
+                                                         const float* __restrict__ bias, void* __restrict__ out,
+                                                         int cout, int npad, int groups, int nz, int ny, int nx,
+                                                         int relu, int out_f32) {
+    extern __shared__ uint4 halo[];  // [halo voxel][channel group]
+    const int cg = (in.c1 + in.c2) >> 3, cg1 = in.c1 >> 3;
+    const int bx = nx >> 4, by = ny >> 2;
+    const int brick = blockIdx.x;
+    const int x0 = (brick % bx) * 16, y0 = ((brick / bx) % by) * 4, z0 = (brick / (bx * by)) * 4;
+    const int hz = nz >> 1, hy = ny >> 1, hx = nx >> 1;
+    (void)hz;
+    const int total = kHaloVox * cg;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int g = t % cg, hv = t / cg;
+        const int x = x0 - 1 + hv % 18, y = y0 - 1 + (hv / 18) % 6, z = z0 - 1 + hv / 108;
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if (x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz) {
+            if (g < cg1) {
+                const int64_t qv = in.up ? (((int64_t)(z >> 1) * hy + (y >> 1)) * hx + (x >> 1))
+                                         : (((int64_t)z * ny + y) * nx + x);
+                val = *reinterpret_cast<const uint4*>(in.a + qv * in.c1 + g * 8);
+            } else {
+                val = *reinterpret_cast<const uint4*>(in.b + (((int64_t)z * ny + y) * nx + x) * in.c2 + (g - cg1) * 8);
+            }
+        }
+        halo[t] = val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    int tap = q / cg, cb = q % cg;
+    const uint16_t* wlane = wq + ((int64_t)q * npad + r) * 8;
+    const int64_t wstep = (int64_t)4 * npad * 8;
+    auto load_a = [&](int tp, int cbk, bf16x8* a) {
+        const bool ok = tp < 27;
+        const int tt = ok ? tp : 0;
+        const int dz = tt / 9, dy = (tt / 3) % 3, dx = tt % 3;
+        const int base = ((w + dz) * 6 + dy) * 18 + r + dx;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint4 raw = halo[(base + m * 18) * cg + cbk];
+            a[m] = __builtin_bit_cast(bf16x8, ok ? raw : make_uint4(0, 0, 0, 0));
+        }
+    };
+    auto load_b = [&](const uint16_t* wp, bf16x8* b) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) b[n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wp + n * 128));
+    };
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a[4], b[NT], a2[4], b2[NT];
+    load_a(tap, cb, a);
+    load_b(wlane, b);
+    const int nchunks = groups >> 2;
+    for (int c = 0; c < nchunks; ++c) {
+        const bool more = c + 1 < nchunks;
+        if (more) {
+            cb += 4;
+            while (cb >= cg) {
+                cb -= cg;
+                ++tap;
+            }
+            load_a(tap, cb, a2);
+            load_b(wlane + (c + 1) * wstep, b2);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = a2[m];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) b[n] = b2[n];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + r;
+        if (co >= cout) continue;
+        const float bco = bias[co];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t v = ((int64_t)(z0 + w) * ny + (y0 + m)) * nx + (x0 + 4 * q + i);
+                float s2 = acc[m][n][i] + bco;
+                if (relu) s2 = fmaxf(s2, 0.0f);
+                if (out_f32) reinterpret_cast<float*>(out)[v * cout + co] = s2;
+                else reinterpret_cast<uint16_t*>(out)[v * cout + co] = f32_to_bf16(s2);
+            }
+    }
+}
+
 }  // namespace flux
 
 using namespace flux;
@@ -549,6 +656,7 @@ struct flux_unet {
     uint16_t* pooled16[3] = {};
     uint16_t* x16 = nullptr;       // [voxel][8]
     int64_t version = 1, packed_version = 0;
+    int halo = 1;  // halo-tiled convolutions: 0 never, 1 where measured faster, 2 wherever they fit
 };
 
 namespace {
@@ -700,6 +808,24 @@ void conv_mfma(flux_unet* u, int l, void* out, int out_f32) {
     const uint16_t* wq = u->wq + u->off_wq[l];
     const float* b = u->params + u->off_b[l];
     const int cp = s.cout, np = u->npad[l], g = u->groups[l];
+    const int64_t halo_bytes = (int64_t)kHaloVox * (mfma_channels(l) / 8) * 16;
+    const int64_t bricks = (int64_t)(nx / 16) * (ny / 4) * (nz / 4);
+    // measured: the halo form pays where the brick is small (<= 32 channels,
+    // <= 41 KiB of LDS, several blocks per CU) and the level has >= 1024 bricks
+    const bool fits = nx % 16 == 0 && ny % 4 == 0 && nz % 4 == 0 && halo_bytes <= 64 * 1024;
+    const bool pays = mfma_channels(l) <= 32 && bricks >= 1024;
+    if (fits && (u->halo == 2 || (u->halo == 1 && pays))) {
+        if (ntn == 1)
+            k_conv3_mfma_halo<1><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
+                                                                              s.relu, out_f32);
+        else if (ntn == 2)
+            k_conv3_mfma_halo<2><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
+                                                                              s.relu, out_f32);
+        else
+            k_conv3_mfma_halo<4><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
+                                                                              s.relu, out_f32);
+        return;
+    }
 #define FLUX_MFMA_CASE(M, N)                                                             \
     if (mt == M && nt == N) {                                                            \
         launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);       \
@@ -858,6 +984,13 @@ int flux_unet_create(int nx, int ny, int nz, int training, flux_unet** out) {
     FLUX_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx % 8 == 0 && ny % 8 == 0 && nz % 8 == 0,
                  "unet: grid [%d, %d, %d] must be divisible by 8", nx, ny, nz);
     flux_unet* u = new flux_unet();
+    {
+        // FLUX_UNET_HALO=0 keeps every layer on the flat-tile kernel, =2 uses
+        // the halo-tiled one wherever it fits (the same arithmetic; for
+        // comparison)
+        const char* e = getenv("FLUX_UNET_HALO");
+        u->halo = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
+    }
     u->n[0] = nx;
     u->n[1] = ny;
     u->n[2] = nz;
