@@ -398,11 +398,15 @@ __global__ void k_pool_cl(const uint16_t* __restrict__ in, uint16_t* __restrict_
 // loaded before the MFMAs of the current one.
 __device__ __forceinline__ uint32_t bit_of(bool b) { return b ? 1u : 0u; }
 
+// Split K (gridDim.y > 1, `part` non-null): split blockIdx.y runs K steps
+// [y * steps, (y + 1) * steps) and stores its raw FP32 sums to
+// part[(split * nv + v) * cout + co]; k_split_finish adds the splits in a
+// fixed order and applies bias, ReLU and the rounding.
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* __restrict__ wq,
                                                     const float* __restrict__ bias, void* __restrict__ out, int cout,
                                                     int npad, int groups, int nz, int ny, int nx, int relu,
-                                                    int out_f32) {
+                                                    int out_f32, float* __restrict__ part, int steps) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int n_groups_n = npad / (16 * NT);
@@ -436,7 +440,9 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
         const uint32_t rz = (bit_of(vz[m] > 0) * 0x1FFu) | (0x1FFu << 9) | (bit_of(vz[m] < nz - 1) * (0x1FFu << 18));
         vmask[m] = (int64_t)v < nv ? (rx & ry & rz) : 0u;
     }
-    int tap = q / cb_per_tap, cb = q % cb_per_tap;  // this lane's group G = 4c + q
+    const int c_begin = (int)blockIdx.y * steps;
+    const int c_end = min(groups >> 2, c_begin + steps);
+    int tap = (4 * c_begin + q) / cb_per_tap, cb = (4 * c_begin + q) % cb_per_tap;  // this lane's group 4c + q
     const uint16_t* wlane = wq + ((int64_t)q * npad + ngrp * NT * 16 + r) * 8;
     const int64_t wstep = (int64_t)4 * npad * 8;
 
@@ -475,10 +481,9 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
         for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     bf16x8 a[MT], b[NT], a2[MT], b2[NT];
     load_a(tap, cb, a);
-    load_b(wlane, b);
-    const int nchunks = groups >> 2;
-    for (int c = 0; c < nchunks; ++c) {
-        const bool more = c + 1 < nchunks;
+    load_b(wlane + c_begin * wstep, b);
+    for (int c = c_begin; c < c_end; ++c) {
+        const bool more = c + 1 < c_end;
         if (more) {
             cb += 4;
             while (cb >= cb_per_tap) {
@@ -510,12 +515,30 @@ __global__ void __launch_bounds__(256) k_conv3_mfma(MfmaIn in, const uint16_t* _
             for (int i = 0; i < 4; ++i) {
                 const int64_t v = vbase + 16 * m + 4 * q + i;
                 if (v >= nv) continue;
+                if (part) {
+                    part[((int64_t)blockIdx.y * nv + v) * cout + co] = acc[m][n][i];
+                    continue;
+                }
                 float s = acc[m][n][i] + bco;
                 if (relu) s = fmaxf(s, 0.0f);
                 if (out_f32) reinterpret_cast<float*>(out)[v * cout + co] = s;
                 else reinterpret_cast<uint16_t*>(out)[v * cout + co] = f32_to_bf16(s);
             }
     }
+}
+
+// sum of the K splits in split order, then bias, ReLU, rounding
+__global__ void k_split_finish(const float* __restrict__ part, int splits, const float* __restrict__ bias,
+                               void* __restrict__ out, int64_t nv, int cout, int relu, int out_f32) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = nv * cout;
+    if (t >= n) return;
+    float acc = part[t];
+    for (int k = 1; k < splits; ++k) acc += part[(int64_t)k * n + t];
+    float s = acc + bias[t % cout];
+    if (relu) s = fmaxf(s, 0.0f);
+    if (out_f32) reinterpret_cast<float*>(out)[t] = s;
+    else reinterpret_cast<uint16_t*>(out)[t] = f32_to_bf16(s);
 }
 
 // Halo-tiled form for levels made of whole 16 x 4 x 4 bricks: the block
@@ -657,6 +680,8 @@ struct flux_unet {
     uint16_t* x16 = nullptr;       // [voxel][8]
     int64_t version = 1, packed_version = 0;
     int halo = 1;  // halo-tiled convolutions: 0 never, 1 where measured faster, 2 wherever they fit
+    int split = 1;                 // split K on small levels (FLUX_UNET_SPLIT=0 disables)
+    float* split_part = nullptr;   // FP32 partial sums of the K splits (kSplitPartBytes)
 };
 
 namespace {
@@ -733,6 +758,7 @@ int forward(flux_unet* u, const float* x) {
 }
 
 // ---- BF16 inference path
+static constexpr int64_t kSplitPartBytes = (int64_t)32 << 20;
 MfmaIn mfma_input(const flux_unet* u, int l) {
     switch (l) {
         case L_E0A: return MfmaIn{u->x16, nullptr, 8, 0, 0};
@@ -772,6 +798,8 @@ int bf16_setup(flux_unet* u) {
         ok = ok && p;
     };
     alloc(u->wq, off);
+    u->split_part = (float*)dmalloc(kSplitPartBytes);
+    ok = ok && u->split_part;
     alloc(u->x16, 8 * u->nvox[0]);
     for (int l = 0; l < L_OUT; ++l) alloc(u->y16[l], (int64_t)kLayers[l].cout * u->nvox[kLayers[l].level]);
     alloc(u->pooled16[0], 16 * u->nvox[1]);
@@ -784,11 +812,16 @@ int bf16_setup(flux_unet* u) {
 
 template <int MT, int NT>
 void launch_mfma(const MfmaIn& in, const uint16_t* wq, const float* b, void* out, int cout, int npad, int groups,
-                 int nz, int ny, int nx, int relu, int out_f32) {
+                 int nz, int ny, int nx, int relu, int out_f32, float* part, int splits) {
     const int64_t nv = (int64_t)nz * ny * nx;
     const int64_t waves = (nv + 16 * MT - 1) / (16 * MT) * (npad / (16 * NT));
-    k_conv3_mfma<MT, NT><<<(unsigned)((waves + 3) / 4), 256, 0, g_stream>>>(in, wq, b, out, cout, npad, groups, nz, ny,
-                                                                          nx, relu, out_f32);
+    const int chunks = groups / 4, steps = (chunks + splits - 1) / splits;
+    dim3 grid((unsigned)((waves + 3) / 4), (unsigned)splits);
+    k_conv3_mfma<MT, NT><<<grid, 256, 0, g_stream>>>(in, wq, b, out, cout, npad, groups, nz, ny, nx, relu, out_f32,
+                                                    splits > 1 ? part : nullptr, steps);
+    if (splits > 1)
+        k_split_finish<<<blocks_for(nv * cout, UTPB), UTPB, 0, g_stream>>>(part, splits, b, out, nv, cout, relu,
+                                                                           out_f32);
 }
 
 // the widest tile (4 x N/16) that still gives >= 4096 wavefronts; smaller
@@ -826,10 +859,21 @@ void conv_mfma(flux_unet* u, int l, void* out, int out_f32) {
                                                                               s.relu, out_f32);
         return;
     }
-#define FLUX_MFMA_CASE(M, N)                                                             \
-    if (mt == M && nt == N) {                                                            \
-        launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);       \
-        return;                                                                          \
+    // levels too small to fill the GPU (< 4096 wavefronts even with 1x1
+    // tiles): split K until 4096 run, each split keeping >= 8 K steps, while
+    // the FP32 partial sums stay within 32 MiB (measured: splitting levels
+    // that already have 4096 wavefronts only adds the finishing pass)
+    int splits = 1;
+    const int chunks = g / 4;
+    while (u->split && waves() * splits < 4096 && splits < 8 && chunks / (2 * splits) >= 8 &&
+           (int64_t)2 * splits * nv * cp * 4 <= kSplitPartBytes)
+        splits *= 2;
+    // every split must own at least one K step
+    while (splits > 1 && (int64_t)(splits - 1) * ((chunks + splits - 1) / splits) >= chunks) splits >>= 1;
+#define FLUX_MFMA_CASE(M, N)                                                                             \
+    if (mt == M && nt == N) {                                                                            \
+        launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32, u->split_part, splits); \
+        return;                                                                                          \
     }
     FLUX_MFMA_CASE(4, 4) FLUX_MFMA_CASE(4, 2) FLUX_MFMA_CASE(4, 1)
     FLUX_MFMA_CASE(2, 4) FLUX_MFMA_CASE(2, 2) FLUX_MFMA_CASE(2, 1)
@@ -969,6 +1013,7 @@ int flux_unet_destroy(flux_unet* u) {
         dfree(u->gpool[q]);
     }
     dfree(u->wq);
+    dfree(u->split_part);
     dfree(u->x16);
     for (int l = 0; l < kNumLayers; ++l) dfree(u->y16[l]);
     for (int q = 0; q < 3; ++q) dfree(u->pooled16[q]);
@@ -990,6 +1035,8 @@ int flux_unet_create(int nx, int ny, int nz, int training, flux_unet** out) {
         // comparison)
         const char* e = getenv("FLUX_UNET_HALO");
         u->halo = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
+        const char* sp = getenv("FLUX_UNET_SPLIT");
+        u->split = !(sp && sp[0] == '0');
     }
     u->n[0] = nx;
     u->n[1] = ny;
