@@ -190,3 +190,36 @@ def test_divergence_matches_oracle(flux, T):
     T.divergence(g, T.mac_fields(g, vel), out)
     ref = O.divergence3(n, h, vel, suffix="f32")
     np.testing.assert_array_equal(out.to_array(), ref)
+
+
+def test_p2g_fixed_point_ties_round_half_to_even(flux, T):
+    """Particles whose value * w * 65536 and w * 65536 are exact half
+    integers (grid_flip_solver3.js:508-535, WGSL round = half to even): the
+    GPU's integer sums equal the closed form and the oracle's."""
+    F = flux
+    halves = [0.5, 1.5, 2.5, 3.5, -0.5, -1.5, -2.5, 7.5]
+    pos, vel = [], []
+    for q, k in enumerate(halves):
+        pos.append((float(1 + q % 4), 2 + q // 4 + 0.5, 3.5))
+        vel.append((k / 65536.0, 0.0, 0.0))
+    pos.append((1.0 + 2.5 / 65536.0, 1.5, 1.5))
+    vel.append((0.0, 0.0, 0.0))
+    pos = np.array(pos, np.float32).ravel()
+    vel = np.array(vel, np.float32).ravel()
+    n, h = (6, 6, 6), (1.0, 1.0, 1.0)
+    cnt = len(pos) // 3
+    g = T.Grid(n, h)
+    P = F.array_n("vec3", cnt, pos)
+    V = F.array_n("vec3", cnt, vel)
+    fs = g.face_shape(0)
+    numer, denom = F.array_n("int", fs), F.array_n("int", fs)
+    out, valid = F.array_n("float", fs), F.array_n("int", fs)
+    T.p2g_fixed(g, P, V, cnt, 0, numer, denom, out, valid)
+    nu, de = numer.to_array(), denom.to_array()
+    for q, k in enumerate(halves):
+        f = (1 + q % 4) + fs[0] * ((2 + q // 4) + fs[1] * 3)
+        assert nu[f] == round(k) and de[f] == 65536, (k, nu[f])
+    assert de[1 + fs[0] * (1 + fs[1])] == 65534 and de[2 + fs[0] * (1 + fs[1])] == 2
+    _, _, rn, rd = O.p2g_fixed3(pos, vel, n, h, [0, 0, 0], 0)
+    np.testing.assert_array_equal(nu, rn)
+    np.testing.assert_array_equal(de, rd)
