@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--kernel", default="relax", help="kernel family timed for the roofline")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="V-cycle storage: f32 is the reference's plain method, f16/bf16 the mixed form")
+    ap.add_argument("--smoother", default="rbgs", choices=["rbgs", "jacobi"],
+                    help="V-cycle smoother: rbgs is the reference's plain method, jacobi its damped-Jacobi form")
+    ap.add_argument("--nu", type=int, default=2, help="smoothing iterations down / up / final")
+    ap.add_argument("--jacobi-omega", type=float, default=2.0 / 3.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=2, help="PCG iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -60,13 +64,15 @@ ALGO_BYTES_PER_CELL = {
     "update": 41.0,
     # x 4 + b 4 + mask 1 + weights 12 read, r 4 written
     "residual": 25.0,
+    # one damped-Jacobi sweep: x read 4, b 4, mask 1, weights 12, x' written 4
+    "jacobi": 25.0,
     # coarse x (1/8 of 4) + mask 1 read, fine x read+write 8
     "correct": 9.5,
 }
 
 
 # fine-level kernel of each family, as rocprofv3 names it
-KERNEL_SYMBOL = {"relax": "k_relax3<1, 0,", "sweep": "k_rbgs_fused<1,", "apply": "k_pcg_apply3",
+KERNEL_SYMBOL = {"relax": "k_relax3<1, 0,", "sweep": "k_rbgs_fused<1,", "jacobi": "k_jacobi3<1,", "apply": "k_pcg_apply3",
                  "update": "k_pcg_update3", "residual": "k_residual3<1,", "correct": "k_correct<1,"}
 
 
@@ -170,7 +176,10 @@ def main():
     act = F.array_n("u8", shape, sc["active"])
     tgt = F.array_n("float", shape, sc["target"])
     ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
-                                        face_weights=beta, multigrid={"levels": args.levels, "storage": args.storage},
+                                        face_weights=beta,
+                                        multigrid={"levels": args.levels, "storage": args.storage,
+                                                   "smoother": args.smoother, "nu_down": args.nu, "nu_up": args.nu,
+                                                   "n_final": args.nu, "jacobi_omega": args.jacobi_omega},
                                         tolerance=1e-6,
                                         relative_tolerance=True)
 

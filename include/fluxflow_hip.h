@@ -107,7 +107,15 @@ typedef struct {
     int n_final;       /* numberOfFinalIterations, default 2 */
     double omega;      /* sorFactor, default 1.0 */
     int storage;       /* FLUX_STORE_*, default FLUX_STORE_F32 */
+    int smoother;      /* FLUX_SMOOTH_*, default red-black Gauss-Seidel */
+    double jacobi_omega; /* damped-Jacobi factor (multigrid_jacobi.js:1106), default 2/3 */
 } flux_mg_options;
+/* Smoother: red-black Gauss-Seidel, colour pairs in place (buildRelaxKernel,
+ * multigrid.js:626-640), or damped Jacobi, single sweeps ping-ponging through
+ * a second buffer (buildJacobiRelaxKernel and relax(),
+ * sandbox/jacobi-smoother/multigrid_jacobi.js:632-642, 1271-1285); the
+ * Jacobi form requires even sweep counts ("sweep counts must be even"). */
+enum { FLUX_SMOOTH_RBGS = 0, FLUX_SMOOTH_JACOBI = 1 };
 /* fills the reference defaults */
 void flux_mg_options_default(flux_mg_options* o);
 typedef struct flux_multigrid flux_multigrid;
@@ -178,6 +186,40 @@ int flux_pcg_scaled_start(flux_pcg* s, const flux_field* g, double* alpha);
  * iteration count is returned. */
 int flux_pcg_block_start(flux_pcg* s, const flux_field* g, int block, int* coarse_iterations);
 int flux_pcg_destroy(flux_pcg* s);
+
+/* ------------------------------------------------------------ dot reducer */
+/* createDotReducer(shape, fieldA, fieldB) (linalg.js:374-512): lane i of a
+ * 2-D/3-D shape [n0, ...] sums every element whose first index is i (n0
+ * lanes); a 1-D shape of n elements has min(n, 64) lanes striding by the
+ * lane count.  The lane sums and their total are FP64 and deterministic (the
+ * reference sums lanes in FP32 and the lanes in FP64 on the host).  Both
+ * fields are FLUX_F32 with `shape`'s element count; they are bound for the
+ * reducer's lifetime. */
+typedef struct flux_dot flux_dot;
+int flux_dot_create(int ndim, const int* shape, const flux_field* a, const flux_field* b, flux_dot** out);
+/* `dispatch` (linalg.js:506): computes the lanes into the `partial` field
+ * (FP32 per lane) without reading anything back */
+int flux_dot_dispatch(flux_dot* d);
+/* read() (linalg.js:490-502): dispatch, then the total */
+int flux_dot_read(flux_dot* d, double* total);
+/* .lanes and .partial (linalg.js:506) */
+int flux_dot_info(const flux_dot* d, int* lanes, flux_field** partial);
+int flux_dot_destroy(flux_dot* d);
+
+/* ------------------------------------------- plain conjugate gradient */
+/* createConjugateGradientSolver(applyOperator, b, x) (linalg.js:563-728):
+ * A is the Laplacian `op`, or, with op NULL, the diagonal operator
+ * out = diag * in (the operator of examples/04-conjugate-gradient).  b and x
+ * are FLUX_F32 fields of one shape; x is the initial guess and the result. */
+typedef struct flux_cg flux_cg;
+int flux_cg_create(flux_operator* op, const flux_field* diag, flux_field* b, flux_field* x, flux_cg** out);
+/* solve(tol, maxiter) (linalg.js:641-724): absolute tolerance on
+ * sqrt(r.r), the residual recomputed from x every iteration after the first,
+ * the reference's guards; *converged is its final check r.r < tol. */
+int flux_cg_solve(flux_cg* s, double tol, int maxiter, int* converged, int* iterations);
+/* the solver's p, r, Ap, Ax fields (linalg.js:727) */
+int flux_cg_fields(flux_cg* s, flux_field** p, flux_field** r, flux_field** ap, flux_field** ax);
+int flux_cg_destroy(flux_cg* s);
 
 /* ---------------------------------------------------- pressure projection */
 /* createGridPressureSolver3 options (grid_pressure_solver3.js:66-86).

@@ -53,8 +53,12 @@ typedef struct {
     int nu_up;           /* numberOfSmoothingIterationsUp (2) */
     int n_coarsest;      /* numberOfCoarsestIterations (20) */
     int n_final;         /* numberOfFinalIterations (2) */
-    double omega;        /* sorFactor (1.0) */
+    double omega;        /* sorFactor (1.0); the Jacobi smoother's damping (2/3) */
+    int smoother;        /* ORC_SMOOTH_RBGS (multigrid.js) or ORC_SMOOTH_JACOBI
+                            (sandbox/jacobi-smoother/multigrid_jacobi.js) */
 } orc_mg_options;
+
+enum { ORC_SMOOTH_RBGS = 0, ORC_SMOOTH_JACOBI = 1 };
 
 typedef struct {
     int iterations;
@@ -145,6 +149,8 @@ void orc_two_phase_beta3(const float* liquid_fraction, const int n[3],
     void orc_diag_##S(const orc_grid* g, real* out, const uint8_t* mask, const real* const* w);   \
     void orc_relax_##S(const orc_grid* g, real* x, const real* b, const uint8_t* mask,            \
                        const real* const* w, double omega, int color);                            \
+    void orc_jacobi_##S(const orc_grid* g, const real* xr, real* xw, const real* b,                \
+                        const uint8_t* mask, const real* const* w, double omega);                 \
     void orc_residual_##S(const orc_grid* g, const real* x, const real* b, real* r,               \
                           const uint8_t* mask, const real* const* w);                             \
     void orc_restrict_##S(const orc_grid* fine, const real* r_fine, const uint8_t* fine_mask,     \

@@ -45,7 +45,11 @@ class MGOptions(C.Structure):
         ("n_coarsest", C.c_int),
         ("n_final", C.c_int),
         ("omega", C.c_double),
+        ("smoother", C.c_int),
     ]
+
+
+SMOOTHERS = {"rbgs": 0, "jacobi": 1}
 
 
 class PCGState(C.Structure):
@@ -207,12 +211,15 @@ class Multigrid:
     """createMultigridPreconditioner restated (multigrid.js:1213-1735)."""
 
     def __init__(self, shape, spacing, levels=1, nu_down=2, nu_up=2, n_coarsest=20, n_final=2,
-                 omega=1.0, mask=None, coarse_mask=None, weights=None, suffix="f64"):
+                 omega=None, mask=None, coarse_mask=None, weights=None, suffix="f64", smoother="rbgs"):
+        if omega is None:
+            omega = 2.0 / 3.0 if smoother == "jacobi" else 1.0
         self.suffix = suffix
         self.shape = tuple(shape)
         self.g = make_grid(shape, spacing)
         o = MGOptions()
         o.levels, o.nu_down, o.nu_up, o.n_coarsest, o.n_final, o.omega = levels, nu_down, nu_up, n_coarsest, n_final, omega
+        o.smoother = SMOOTHERS[smoother]
         self.o = o
         self.mask = _u8(mask)
         self.coarse_mask = _u8(coarse_mask)
@@ -220,6 +227,8 @@ class Multigrid:
         self._wkeep, wp = _w(self.weights)
         self.h = getattr(lib(), "orc_mg_create_" + suffix)(C.byref(self.g), C.byref(o), _p(self.mask), _p(self.coarse_mask), wp)
         if not self.h:
+            if smoother == "jacobi" and any(v % 2 for v in (nu_down, nu_up, n_coarsest, n_final)):
+                raise ValueError("multigrid (jacobi): sweep counts must be even so the ping-pong lands back in x")
             raise ValueError("multigrid: shape is not divisible by 2^(numberOfLevels-1)")
 
     def apply(self, r):

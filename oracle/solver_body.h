@@ -331,6 +331,9 @@ typedef struct {
 void* FN(orc_mg_create)(const orc_grid* g, const orc_mg_options* o, const uint8_t* mask,
                         const uint8_t* coarse_src_mask, const REAL* const* w) {
     if (o->levels < 1 || o->levels > ORC_MAX_LEVELS) return NULL;
+    if (o->smoother == ORC_SMOOTH_JACOBI &&
+        ((o->nu_down & 1) || (o->nu_up & 1) || (o->n_coarsest & 1) || (o->n_final & 1)))
+        return NULL; /* multigrid_jacobi.js:1273-1276: sweep counts must be even */
     int div = 1 << (o->levels - 1);
     for (int a = 0; a < g->ndim; ++a)
         if (g->n[a] % div) return NULL; /* computeLevelShapes 176-202 */
@@ -402,7 +405,40 @@ int FN(orc_mg_level_info)(void* p, int level, int* n_out, uint8_t** mask, REAL**
     return 0;
 }
 
+/* One damped-Jacobi sweep (buildJacobiRelaxKernel,
+ * sandbox/jacobi-smoother/multigrid_jacobi.js:632-642): every cell reads the
+ * old iterate `xr` and writes xr_c + ((b - A xr)_c / diag_c) * omega to `xw`. */
+void FN(orc_jacobi)(const orc_grid* g, const REAL* xr, REAL* xw, const REAL* b, const uint8_t* mask,
+                    const REAL* const* w, double omega) {
+    const int* n = g->n;
+    REAL om = (REAL)omega;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                int64_t c = orc_cidx(n, i, j, k);
+                REAL ax = FN(row_apply)(g, xr, mask, w, i, j, k);
+                REAL d = FN(row_diag)(g, mask, w, i, j, k);
+                REAL upd = (REAL)((REAL)(b[c] - ax) / d) * om;
+                xw[c] = (REAL)(xr[c] + upd);
+            }
+}
+
+/* relax (multigrid.js:1589-1607; the Jacobi form multigrid_jacobi.js:1271-1285):
+ * red-black takes `iters` colour pairs, post-smoothing colours reversed; the
+ * Jacobi form takes `iters` single sweeps ping-ponging through buf (an even
+ * count lands back in x; the reference rejects odd counts) and ignores the
+ * order, being symmetric by construction. */
 static void FN(mg_relax_n)(FN(orc_mg)* mg, int l, int iters, int reversed) {
+    if (mg->o.smoother == ORC_SMOOTH_JACOBI) {
+        const REAL* const* w = mg->w[l][0] ? (const REAL* const*)mg->w[l] : NULL;
+        for (int it = 0; it < iters; ++it) {
+            const REAL* src = (it & 1) ? mg->buf[l] : mg->x[l];
+            REAL* dst = (it & 1) ? mg->x[l] : mg->buf[l];
+            FN(orc_jacobi)(&mg->g[l], src, dst, mg->b[l], mg->mask[l], w, mg->o.omega);
+        }
+        return;
+    }
     for (int it = 0; it < iters; ++it) {
         const REAL* const* w = mg->w[l][0] ? (const REAL* const*)mg->w[l] : NULL;
         FN(orc_relax)(&mg->g[l], mg->x[l], mg->b[l], mg->mask[l], w, mg->o.omega, reversed ? 1 : 0);

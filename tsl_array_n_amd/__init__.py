@@ -61,7 +61,8 @@ def lib():
 
 class MGOptions(C.Structure):
     _fields_ = [("levels", C.c_int), ("nu_down", C.c_int), ("nu_up", C.c_int), ("n_coarsest", C.c_int),
-                ("n_final", C.c_int), ("omega", C.c_double), ("storage", C.c_int)]
+                ("n_final", C.c_int), ("omega", C.c_double), ("storage", C.c_int), ("smoother", C.c_int),
+                ("jacobi_omega", C.c_double)]
 
 
 class PCGState(C.Structure):
@@ -196,10 +197,16 @@ def _fields3(fields, n):
     return arr
 
 
-def mg_options(levels=1, nu_down=2, nu_up=2, n_coarsest=20, n_final=2, omega=1.0, storage="f32", **_):
+SMOOTHER = {"rbgs": 0, "jacobi": 1}
+
+
+def mg_options(levels=1, nu_down=2, nu_up=2, n_coarsest=20, n_final=2, omega=1.0, storage="f32", smoother="rbgs",
+               jacobi_omega=2.0 / 3.0, **_):
     o = MGOptions()
     o.levels, o.nu_down, o.nu_up, o.n_coarsest, o.n_final, o.omega = levels, nu_down, nu_up, n_coarsest, n_final, omega
     o.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
+    o.smoother = SMOOTHER[smoother] if isinstance(smoother, str) else int(smoother)
+    o.jacobi_omega = jacobi_omega
     return o
 
 
@@ -327,6 +334,81 @@ class PCGSolver:
 
 def create_pcg_solver(operator, preconditioner, b, x, multigrid=None):
     return PCGSolver(operator, b, x, preconditioner, multigrid)
+
+
+class DotReducer:
+    """createDotReducer(shape, fieldA, fieldB) (linalg.js:374-512): ``read()``
+    returns sum(a * b); ``lanes`` and the per-lane ``partial`` field as in
+    the reference."""
+
+    def __init__(self, shape, a, b):
+        shape = tuple(int(s) for s in shape)
+        self._keep = (a, b)
+        h = C.c_void_p()
+        sh = (C.c_int * max(1, len(shape)))(*shape)
+        _check(_lib.flux_dot_create(len(shape), sh, C.c_void_p(a.h), C.c_void_p(b.h), C.byref(h)), "createDotReducer")
+        self.h = h.value
+        lanes = C.c_int()
+        part = C.c_void_p()
+        _check(_lib.flux_dot_info(C.c_void_p(self.h), C.byref(lanes), C.byref(part)))
+        self.lanes = lanes.value
+        self.partial = Field(F32, (self.lanes,), handle=part.value, owned=False)
+
+    def dispatch(self):
+        _check(_lib.flux_dot_dispatch(C.c_void_p(self.h)), "createDotReducer")
+
+    def read(self):
+        t = C.c_double()
+        _check(_lib.flux_dot_read(C.c_void_p(self.h), C.byref(t)), "createDotReducer")
+        return t.value
+
+    def __del__(self):
+        try:
+            if _initialised and self.h:
+                _lib.flux_dot_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+def create_dot_reducer(shape, a, b):
+    return DotReducer(shape, a, b)
+
+
+class CGSolver:
+    """createConjugateGradientSolver(applyOperator, b, x) (linalg.js:563-728):
+    ``operator`` is a LaplacianOperator, or a Field holding the diagonal of a
+    diagonal operator."""
+
+    def __init__(self, operator, b, x):
+        if b.dtype != x.dtype:
+            raise FluxError("conjugateGradient: element type mismatch, b.type(%s) != x.type(%s)." % (b.dtype, x.dtype))
+        if b.shape != x.shape:
+            raise FluxError("conjugateGradient: shape mismatch, b.shape(%s) != x.shape(%s)." % (b.shape, x.shape))
+        self._keep = (operator, b, x)
+        diag = isinstance(operator, Field)
+        h = C.c_void_p()
+        _check(_lib.flux_cg_create(C.c_void_p(None if diag else operator.h), C.c_void_p(operator.h if diag else None),
+                                   C.c_void_p(b.h), C.c_void_p(x.h), C.byref(h)), "conjugateGradient")
+        self.h = h.value
+        self.iterations = 0
+
+    def solve(self, tol, maxiter):
+        ok = C.c_int()
+        it = C.c_int()
+        _check(_lib.flux_cg_solve(C.c_void_p(self.h), C.c_double(tol), int(maxiter), C.byref(ok), C.byref(it)), "solve")
+        self.iterations = it.value
+        return bool(ok.value)
+
+    def __del__(self):
+        try:
+            if _initialised and self.h:
+                _lib.flux_cg_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+def create_cg_solver(operator, b, x):
+    return CGSolver(operator, b, x)
 
 
 class GridPressureSolver3:

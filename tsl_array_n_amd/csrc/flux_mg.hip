@@ -191,6 +191,168 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
     if (DOT) store_block_partial(dot, partials);
 }
 
+// One damped-Jacobi sweep (buildJacobiRelaxKernel,
+// sandbox/jacobi-smoother/multigrid_jacobi.js:632-642): every cell reads the
+// old iterate xr and writes xr_c + ((b - A xr)_c / diag_c) * omega to xw, so
+// the sweep is out of place (ping-pong).  Generic shape: one cell per thread.
+template <int FINE, int FROM_ZERO, typename XT, typename WT>
+__global__ void k_jacobi(LevelDevT<WT> L, const XT* __restrict__ xr, XT* __restrict__ xw, const XT* __restrict__ b,
+                         float omega, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n0 * L.n1 * L.n2) return;
+    int i, j, k;
+    cell_coords(L, c, i, j, k);
+    float d;
+    float ax;
+    if (FROM_ZERO) {
+        (void)stencil_row<float>(L, i, j, k, c, [&](int) { return 0.0f; }, &d);
+        ax = 0.0f;
+    } else {
+        ax = stencil_row<float>(L, i, j, k, c, [&](int q) { return to_f32(xr[q]); }, &d);
+    }
+    const float cur = FROM_ZERO ? 0.0f : to_f32(xr[c]);
+    xw[c] = to_store<XT>(cur + ((to_f32(b[c]) - ax) / d) * omega);
+}
+
+// The damped-Jacobi sweep for an even row length in the column-march shape:
+// thread t owns the cell pair (2p, 2p+1) of row j and marches over a chunk of
+// planes; both cells are updated from the old iterate and stored as one
+// aligned vector.  The z-neighbour pairs come from registers (the previous
+// plane's centre pair and the next plane's, loaded one plane ahead), and so
+// does the lower z-face weight pair (the previous plane's upper one).
+// FROM_ZERO takes xr as zero (xr is not read); DOT stores each block's
+// partial of b.xw (the PCG's r.z with the final z).
+template <int FINE, int FROM_ZERO, int DOT, typename XT, typename WT>
+__global__ void __launch_bounds__(MTPB) k_jacobi3(LevelDevT<WT> L, March M, const XT* __restrict__ xr,
+                                                  XT* __restrict__ xw, const XT* __restrict__ b, float omega,
+                                                  const int* __restrict__ stop, double* __restrict__ partials) {
+    if (stop && *stop) return;
+    double dot = 0.0;
+    int t, k0, k1;
+    const bool active = march_begin(M, t, k0, k1);
+    if (!DOT && !active) return;
+    if (active) {
+        const int half = M.run;
+        const int j = t / half;
+        const int p = t - j * half;
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const int i0 = 2 * p;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        const bool has_w = L.w0 != nullptr;
+        Pair2<XT> xc2, xnext, xprev;
+        const int cp0 = i0 + n0 * (j + n1 * k0);
+        if (!FROM_ZERO) {
+            xnext = *reinterpret_cast<const Pair2<XT>*>(xr + cp0);
+            if (k0 > 0) xprev = *reinterpret_cast<const Pair2<XT>*>(xr + cp0 - plane);
+        }
+        Pair2<WT> wz_lo;
+        if (has_w) wz_lo = *reinterpret_cast<const Pair2<WT>*>(L.w2 + cp0);
+        for (int k = k0; k < k1; ++k) {
+            const int cp = i0 + n0 * (j + n1 * k);
+            if (!FROM_ZERO) {
+                xc2 = xnext;
+                if (k + 1 < n2) xnext = *reinterpret_cast<const Pair2<XT>*>(xr + cp + plane);
+            }
+            const Pair2<uint8_t> np = nbits ? *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp) : Pair2<uint8_t>{0, 0};
+            const Pair2<XT> bp = *reinterpret_cast<const Pair2<XT>*>(b + cp);
+            // face weights of both cells: x faces i0, i0+1, i0+2; y faces of rows j and j+1; z faces k and k+1
+            float wx0 = 1.0f, wx1 = 1.0f, wx2 = 1.0f, wyl[2] = {1.0f, 1.0f}, wyu[2] = {1.0f, 1.0f};
+            float wzl[2] = {1.0f, 1.0f}, wzu[2] = {1.0f, 1.0f};
+            Pair2<WT> wz_up;
+            if (has_w) {
+                const int fx = i0 + (n0 + 1) * (j + n1 * k);
+                wx0 = to_f32(L.w0[fx]);
+                wx1 = to_f32(L.w0[fx + 1]);
+                wx2 = to_f32(L.w0[fx + 2]);
+                const int fy = i0 + n0 * (j + (n1 + 1) * k);
+                const Pair2<WT> yl = *reinterpret_cast<const Pair2<WT>*>(L.w1 + fy);
+                const Pair2<WT> yu = *reinterpret_cast<const Pair2<WT>*>(L.w1 + fy + n0);
+                wyl[0] = to_f32(yl.lo);
+                wyl[1] = to_f32(yl.hi);
+                wyu[0] = to_f32(yu.lo);
+                wyu[1] = to_f32(yu.hi);
+                wz_up = *reinterpret_cast<const Pair2<WT>*>(L.w2 + cp + plane);
+                wzl[0] = to_f32(wz_lo.lo);
+                wzl[1] = to_f32(wz_lo.hi);
+                wzu[0] = to_f32(wz_up.lo);
+                wzu[1] = to_f32(wz_up.hi);
+            }
+            float w[2][6];
+            w[0][0] = i0 > 0 ? wx0 : 0.0f;
+            w[0][1] = wx1;
+            w[1][0] = wx1;
+            w[1][1] = i0 + 1 < n0 - 1 ? wx2 : 0.0f;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                w[s][2] = j > 0 ? wyl[s] : 0.0f;
+                w[s][3] = j < n1 - 1 ? wyu[s] : 0.0f;
+                w[s][4] = k > 0 ? wzl[s] : 0.0f;
+                w[s][5] = k < n2 - 1 ? wzu[s] : 0.0f;
+            }
+            float out[2];
+            if (FROM_ZERO) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const unsigned bits = s ? np.hi : np.lo;
+                    const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    float sum, dsum;
+                    row_terms<float>(0.0f, zn, w[s], ih2, sum, dsum);
+                    const bool dir = (bits & 64u) != 0;
+                    const float d = dir ? -1.0f : dsum;
+                    const float ax = dir ? -0.0f : sum;
+                    out[s] = 0.0f + ((to_f32(s ? bp.hi : bp.lo) - ax) / d) * omega;
+                }
+            } else {
+                const float xl = to_f32(xr[i0 > 0 ? cp - 1 : cp]);
+                const float xu = to_f32(xr[i0 + 1 < n0 - 1 ? cp + 2 : cp + 1]);
+                const Pair2<XT> ys = *reinterpret_cast<const Pair2<XT>*>(xr + (j > 0 ? cp - n0 : cp));
+                const Pair2<XT> yn = *reinterpret_cast<const Pair2<XT>*>(xr + (j < n1 - 1 ? cp + n0 : cp));
+                const float c0 = to_f32(xc2.lo), c1 = to_f32(xc2.hi);
+                float xn[2][6];
+                xn[0][0] = xl;
+                xn[0][1] = c1;
+                xn[1][0] = c0;
+                xn[1][1] = xu;
+                xn[0][2] = to_f32(ys.lo);
+                xn[1][2] = to_f32(ys.hi);
+                xn[0][3] = to_f32(yn.lo);
+                xn[1][3] = to_f32(yn.hi);
+                xn[0][4] = k > 0 ? to_f32(xprev.lo) : c0;
+                xn[1][4] = k > 0 ? to_f32(xprev.hi) : c1;
+                xn[0][5] = k + 1 < n2 ? to_f32(xnext.lo) : c0;
+                xn[1][5] = k + 1 < n2 ? to_f32(xnext.hi) : c1;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const unsigned bits = s ? np.hi : np.lo;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) xn[s][a] = ((bits >> a) & 1u) ? 0.0f : xn[s][a];
+                    const float xc = s ? c1 : c0;
+                    float sum, dsum;
+                    row_terms<float>(xc, xn[s], w[s], ih2, sum, dsum);
+                    const bool dir = (bits & 64u) != 0;
+                    const float d = dir ? -1.0f : dsum;
+                    const float ax = dir ? -xc : sum;
+                    out[s] = xc + ((to_f32(s ? bp.hi : bp.lo) - ax) / d) * omega;
+                }
+                xprev = xc2;
+            }
+            Pair2<XT> o;
+            o.lo = to_store<XT>(out[0]);
+            o.hi = to_store<XT>(out[1]);
+            *reinterpret_cast<Pair2<XT>*>(xw + cp) = o;
+            if (has_w) wz_lo = wz_up;
+            if (DOT) {
+                dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
+                dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
+            }
+        }
+    }
+    if (DOT) store_block_partial(dot, partials);
+}
+
 // r = b - A x (buildResidualKernel, multigrid.js:741-749) in the column-march
 // shape (one cell per thread and plane), as the restriction's input.
 template <int FINE, typename XT, typename RT, typename WT>
@@ -531,9 +693,11 @@ __global__ void __launch_bounds__(MTPB) k_correct3p(March M, int c0, int c1, int
 // the storage type, as a store to the level's x would be.
 static constexpr int COARSE_LDS_CELLS = 4096;
 static constexpr int COARSE_TPB = 1024;
+// With `jacobi` the sweeps are damped-Jacobi single sweeps (`iters` of them,
+// even), each computed from the previous iterate before any value changes.
 template <typename XT, typename WT>
 __device__ void coarse_solve_block(const LevelDevT<WT>& L, XT* __restrict__ x, const XT* __restrict__ b, float omega,
-                                   int iters) {
+                                   int iters, int jacobi) {
     constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
     __shared__ float sx[COARSE_LDS_CELLS];
     const int n = L.n0 * L.n1 * L.n2;
@@ -562,6 +726,35 @@ __device__ void coarse_solve_block(const LevelDevT<WT>& L, XT* __restrict__ x, c
     }
     for (int c = threadIdx.x; c < n; c += COARSE_TPB) sx[c] = 0.0f;
     __syncthreads();
+    if (jacobi) {
+        for (int it = 0; it < iters; ++it) {
+            float nv[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int c = threadIdx.x + u * COARSE_TPB;
+                nv[u] = 0.0f;
+                if (c >= n) continue;
+                const float xc = sx[c];
+                float xn[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits[u] >> a) & 1u) ? 0.0f : sx[q[u][a]];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w[u], ih2, sum, dsum);
+                const bool dir = (bits[u] >> 6) & 1u;
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                nv[u] = to_f32(to_store<XT>(xc + ((bc[u] - ax) / d) * omega));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int c = threadIdx.x + u * COARSE_TPB;
+                if (c < n) sx[c] = nv[u];
+            }
+            __syncthreads();
+        }
+        iters = 0;
+    }
     for (int it = 0; it < iters; ++it) {
         for (int color = 0; color < 2; ++color) {
 #pragma unroll
@@ -589,9 +782,9 @@ __device__ void coarse_solve_block(const LevelDevT<WT>& L, XT* __restrict__ x, c
 template <typename XT, typename WT>
 __global__ void __launch_bounds__(COARSE_TPB) k_coarse_solve(LevelDevT<WT> L, XT* __restrict__ x,
                                                              const XT* __restrict__ b, float omega, int iters,
-                                                             const int* __restrict__ stop) {
+                                                             int jacobi, const int* __restrict__ stop) {
     if (stop && *stop) return;
-    coarse_solve_block<XT, WT>(L, x, b, omega, iters);
+    coarse_solve_block<XT, WT>(L, x, b, omega, iters, jacobi);
 }
 
 // The V-cycle's tail in one workgroup: a 3-D level `lt` of at most
@@ -606,7 +799,7 @@ template <typename ST>
 __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, LevelDevT<ST> L1,
                                                             const ST* __restrict__ b0, ST* __restrict__ x0,
                                                             float omega, int nu_down, int nu_up, int n_coarsest,
-                                                            float mult, const int* __restrict__ stop) {
+                                                            float mult, int jacobi, const int* __restrict__ stop) {
     constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
     __shared__ float sx0[COARSE_LDS_CELLS];
     __shared__ float sr0[COARSE_LDS_CELLS];
@@ -658,7 +851,38 @@ __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, Le
         }
         __syncthreads();
     };
+    // damped-Jacobi single sweep of level lt (k_jacobi3's arithmetic)
+    auto jsweep0 = [&]() {
+        float nv[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int c = threadIdx.x + u * COARSE_TPB;
+            nv[u] = 0.0f;
+            if (c >= n0) continue;
+            const float xc = sx0[c];
+            float xn[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits[u] >> a) & 1u) ? 0.0f : sx0[q[u][a]];
+            float sum, dsum;
+            row_terms<float>(xc, xn, w[u], ih0, sum, dsum);
+            const bool dir = (bits[u] >> 6) & 1u;
+            const float d = dir ? -1.0f : dsum;
+            const float ax = dir ? -xc : sum;
+            nv[u] = to_f32(to_store<ST>(xc + ((bc[u] - ax) / d) * omega));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int c = threadIdx.x + u * COARSE_TPB;
+            if (c < n0) sx0[c] = nv[u];
+        }
+        __syncthreads();
+    };
     for (int it = 0; it < nu_down; ++it) {
+        if (jacobi) {
+            jsweep0();
+            continue;
+        }
         sweep0(0);
         sweep0(1);
     }
@@ -725,7 +949,28 @@ __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, Le
     __syncthreads();
     // the coarsest solve (coarse_solve_block), one cell per thread
     const bool own1 = (int)threadIdx.x < n1;
-    for (int it = 0; it < n_coarsest; ++it) {
+    if (jacobi) {
+        for (int it = 0; it < n_coarsest; ++it) {
+            float nv = 0.0f;
+            if (own1) {
+                const int c = threadIdx.x;
+                const float xc = sx1[c];
+                float xn[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits1 >> a) & 1u) ? 0.0f : sx1[q1[a]];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w1, ih1, sum, dsum);
+                const bool dir = (bits1 >> 6) & 1u;
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                nv = to_f32(to_store<ST>(xc + ((bc1 - ax) / d) * omega));
+            }
+            __syncthreads();
+            if (own1) sx1[threadIdx.x] = nv;
+            __syncthreads();
+        }
+    }
+    for (int it = 0; it < (jacobi ? 0 : n_coarsest); ++it) {
         for (int color = 0; color < 2; ++color) {
             if (own1 && (int)((bits1 >> 7) & 1u) == color) {
                 const int c = threadIdx.x;
@@ -780,6 +1025,10 @@ __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, Le
     }
     __syncthreads();
     for (int it = 0; it < nu_up; ++it) {
+        if (jacobi) {
+            jsweep0();
+            continue;
+        }
         sweep0(1);
         sweep0(0);
     }
@@ -1001,11 +1250,54 @@ void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
     }
 }
 
+float smooth_omega(const flux_multigrid* mg) {
+    return (float)(mg->o.smoother == FLUX_SMOOTH_JACOBI ? mg->o.jacobi_omega : mg->o.omega);
+}
+
+// `iters` damped-Jacobi sweeps (an even count): x -> buf -> x ...; the
+// residual buffer is free while the level smooths.  With `dot`, the last
+// sweep also stores the r.z partials of the final field.
+template <int FINE, typename XT, typename WT>
+void jacobi_sweeps(flux_multigrid* mg, int l, int iters, bool from_zero, const int* stop, DotOut* dot) {
+    MGLevel& v = mg->lv[l];
+    LevelDevT<WT> L = level_dev<WT>(mg, v);
+    const float om = smooth_omega(mg);
+    const bool pairs = mg->ndim == 3 && (v.n[0] & 1) == 0;
+    const March M = pairs ? march_shape(v.n[0] / 2, v.n[1], v.n[2], MTPB, g_march_blocks) : March{};
+    const unsigned g = blocks_for(cells_of(v.n), MTPB);
+    for (int it = 0; it < iters; ++it) {
+        const XT* src = (it & 1) ? (const XT*)v.buf : (const XT*)v.x;
+        XT* dst = (it & 1) ? (XT*)v.x : (XT*)v.buf;
+        const bool zero = from_zero && it == 0;
+        const bool with_dot = dot && pairs && it == iters - 1 && M.nb <= dot->capacity;
+        if (FINE) kprof_begin("jacobi");
+        if (pairs) {
+            const March Mf = FINE ? flip_march(M) : M;
+            if (zero) k_jacobi3<FINE, 1, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, Mf, src, dst, (const XT*)v.b, om,
+                                                                               stop, nullptr);
+            else if (with_dot) k_jacobi3<FINE, 0, 1, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, Mf, src, dst,
+                                                                                         (const XT*)v.b, om, stop,
+                                                                                         dot->partials);
+            else k_jacobi3<FINE, 0, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, Mf, src, dst, (const XT*)v.b, om,
+                                                                          stop, nullptr);
+        } else {
+            if (zero) k_jacobi<FINE, 1, XT, WT><<<g, MTPB, 0, g_stream>>>(L, src, dst, (const XT*)v.b, om, stop);
+            else k_jacobi<FINE, 0, XT, WT><<<g, MTPB, 0, g_stream>>>(L, src, dst, (const XT*)v.b, om, stop);
+        }
+        if (FINE) kprof_end("jacobi");
+        if (with_dot) dot->blocks = M.nb;
+    }
+}
+
 template <int FINE, typename XT, typename WT>
 void relax_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from_zero, const int* stop,
                   DotOut* dot = nullptr) {
     MGLevel& v = mg->lv[l];
     if (iters <= 0) return;
+    if (mg->o.smoother == FLUX_SMOOTH_JACOBI) {
+        jacobi_sweeps<FINE, XT, WT>(mg, l, iters, from_zero, stop, dot);
+        return;
+    }
     if (use_fused(mg, v)) {
         fused_sweeps<XT, WT>(mg, l, iters, reversed, from_zero, stop, dot);
         return;
@@ -1113,13 +1405,14 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
     MGLevel& v = mg->lv[l];
     int nl = (int)mg->lv.size();
     const bool fine = (l == 0);
+    const int jac = mg->o.smoother == FLUX_SMOOTH_JACOBI ? 1 : 0;
     if (l == nl - 1) {
         int64_t cells = cells_of(v.n);
         if (fine) {
             if (cells <= COARSE_SINGLE_BLOCK_MAX)
                 k_coarse_solve<float, ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), (float*)v.x,
-                                                                    (const float*)v.b, (float)mg->o.omega,
-                                                                    mg->o.n_coarsest, stop);
+                                                                    (const float*)v.b, smooth_omega(mg),
+                                                                    mg->o.n_coarsest, jac, stop);
             else if (mg->o.n_coarsest > 0)
                 level_smooth<1, float, ST>(mg, l, mg->o.n_coarsest, false, true, stop);
             else
@@ -1127,7 +1420,7 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
         } else {
             if (cells <= COARSE_SINGLE_BLOCK_MAX)
                 k_coarse_solve<ST, ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), (ST*)v.x, (const ST*)v.b,
-                                                                 (float)mg->o.omega, mg->o.n_coarsest, stop);
+                                                                 smooth_omega(mg), mg->o.n_coarsest, jac, stop);
             else if (mg->o.n_coarsest > 0)
                 level_smooth<0, ST, ST>(mg, l, mg->o.n_coarsest, false, true, stop);
             else
@@ -1141,9 +1434,9 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
     if (!fine && l + 2 == nl && mg->ndim == 3 && cells <= COARSE_LDS_CELLS && cc <= COARSE_TPB &&
         !use_fused(mg, v) && mg->tail) {
         k_vcycle_tail<ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), level_dev<ST>(mg, c), (const ST*)v.b,
-                                                          (ST*)v.x, (float)mg->o.omega, mg->o.nu_down,
+                                                          (ST*)v.x, smooth_omega(mg), mg->o.nu_down,
                                                           mg->o.nu_up, mg->o.n_coarsest,
-                                                          (float)(c.xscale / v.xscale), stop);
+                                                          (float)(c.xscale / v.xscale), jac, stop);
         return FLUX_OK;
     }
     if (fine) {
@@ -1220,6 +1513,8 @@ void flux_mg_options_default(flux_mg_options* o) {
     o->n_final = 2;
     o->omega = 1.0;
     o->storage = FLUX_STORE_F32;
+    o->smoother = FLUX_SMOOTH_RBGS;
+    o->jacobi_omega = 2.0 / 3.0;
 }
 
 int flux_multigrid_destroy(flux_multigrid* mg) {
@@ -1256,6 +1551,12 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
                  "multigrid: unknown storage %d", o.storage);
     FLUX_REQUIRE(o.nu_down >= 0 && o.nu_up >= 0 && o.n_coarsest >= 0 && o.n_final >= 0,
                  "multigrid: iteration counts must be >= 0");
+    FLUX_REQUIRE(o.smoother == FLUX_SMOOTH_RBGS || o.smoother == FLUX_SMOOTH_JACOBI, "multigrid: unknown smoother %d",
+                 o.smoother);
+    FLUX_REQUIRE(o.smoother != FLUX_SMOOTH_JACOBI ||
+                     !((o.nu_down | o.nu_up | o.n_coarsest | o.n_final) & 1),
+                 "multigrid (jacobi): sweep counts must be even so the ping-pong lands back in x, got %d/%d/%d/%d.",
+                 o.nu_down, o.nu_up, o.n_coarsest, o.n_final);
     int div = 1 << (o.levels - 1);
     for (int a = 0; a < ndim; ++a)
         if (shape[a] % div)

@@ -327,6 +327,19 @@ static void mg_opts(napi_env env, napi_value o, flux_mg_options* m) {
     if ((v = prop(env, o, "numberOfFinalIterations"))) m->n_final = get_int(env, v, m->n_final);
     if ((v = prop(env, o, "sorFactor"))) m->omega = get_double(env, v, m->omega);
     if ((v = prop(env, o, "storage"))) m->storage = get_int(env, v, m->storage);
+    if ((v = prop(env, o, "smoother"))) {
+        /* 'rbgs' | 'jacobi' (or the FLUX_SMOOTH_* number) */
+        napi_valuetype t;
+        char name[16] = {0};
+        size_t len = 0;
+        if (napi_typeof(env, v, &t) == napi_ok && t == napi_string) {
+            napi_get_value_string_utf8(env, v, name, sizeof(name), &len);
+            m->smoother = strcmp(name, "jacobi") == 0 ? FLUX_SMOOTH_JACOBI : FLUX_SMOOTH_RBGS;
+        } else {
+            m->smoother = get_int(env, v, m->smoother);
+        }
+    }
+    if ((v = prop(env, o, "omega"))) m->jacobi_omega = get_double(env, v, m->jacobi_omega);
 }
 
 static napi_value js_multigrid_create(napi_env env, napi_callback_info info) {
