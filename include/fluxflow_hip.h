@@ -205,6 +205,20 @@ int flux_dot_read(flux_dot* d, double* total);
 /* .lanes and .partial (linalg.js:506) */
 int flux_dot_info(const flux_dot* d, int* lanes, flux_field** partial);
 int flux_dot_destroy(flux_dot* d);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the reducer's sums for a, b of `shape` without a
+ * reducer object and without any allocation.  The per-block partials live in
+ * a static device array of 2 MiB (each GPU the library runs on holds its own
+ * until the process ends), so a shape may have at most 131071 lanes (the
+ * first axis of a 2-D/3-D shape); larger shapes are rejected with
+ * FLUX_ERR_ARG.  Writes the lane sums (FP32, `lanes` elements; may be NULL)
+ * and the FP64 total (a FLUX_F64 field, element 0) -- the same values, bit
+ * for bit, as flux_dot_read and the `partial` field.  Asynchronous on the
+ * library stream (flux_sync waits).  Every call on one GPU shares its
+ * scratch: calls that overlap on one GPU (from several host threads) race. */
+int flux_dot_noalloc(int ndim, const int* shape, const flux_field* a, const flux_field* b, flux_field* lanes_out,
+                     flux_field* total_out);
 
 /* ------------------------------------------- plain conjugate gradient */
 /* createConjugateGradientSolver(applyOperator, b, x) (linalg.js:563-728):

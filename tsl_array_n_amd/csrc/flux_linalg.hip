@@ -38,9 +38,11 @@ __global__ void __launch_bounds__(DOT_TPB) k_dot_lanes(const float* __restrict__
 }
 
 // lane64[i] = sum_x part[x * lanes + i] (x in order); lane32[i] its FP32
-// value (the reference's `partial` field); lane64[lanes] = the lane total
+// value (the reference's `partial` field); lane64[lanes] = the lane total,
+// also stored at *total when given
 __global__ void __launch_bounds__(DOT_FIN_TPB) k_dot_finish(const double* __restrict__ part, unsigned gx, int lanes,
-                                                            float* __restrict__ lane32, double* __restrict__ lane64) {
+                                                            float* __restrict__ lane32, double* __restrict__ lane64,
+                                                            double* __restrict__ total) {
     double mine = 0.0;
     for (int i = threadIdx.x; i < lanes; i += DOT_FIN_TPB) {
         double s = 0.0;
@@ -50,7 +52,10 @@ __global__ void __launch_bounds__(DOT_FIN_TPB) k_dot_finish(const double* __rest
         mine += s;
     }
     const double tot = block_sum(mine);
-    if (threadIdx.x == 0) lane64[lanes] = tot;
+    if (threadIdx.x == 0) {
+        lane64[lanes] = tot;
+        if (total) *total = tot;
+    }
 }
 
 struct DotGeom {
@@ -79,17 +84,26 @@ static int dot_geometry(int ndim, const int* shape, DotGeom* g, const char* who)
     return FLUX_OK;
 }
 
-// Row blocks for a partials buffer of `cap` doubles.
-static unsigned dot_row_blocks(const DotGeom& g, size_t cap) {
-    size_t by_cap = cap / (size_t)g.lanes;
+// Scratch of the allocation-free entry point: the per-block partials and the
+// lane sums fit in DOT_SCRATCH_DOUBLES (2 MiB), so a shape has at most
+// (DOT_SCRATCH_DOUBLES - 1) / 2 lanes there.  Both entry points use the same
+// row-block count, so their sums run in the same order.
+static constexpr size_t DOT_SCRATCH_DOUBLES = (size_t)1 << 18;
+
+static bool dot_fits_scratch(const DotGeom& g) { return 2 * (size_t)g.lanes + 1 <= DOT_SCRATCH_DOUBLES; }
+
+// Row blocks: at most DOT_MAX_ROW_BLOCKS, and within the scratch when the
+// shape fits it.
+static unsigned dot_row_blocks(const DotGeom& g) {
+    size_t by_cap = dot_fits_scratch(g) ? (DOT_SCRATCH_DOUBLES - (size_t)g.lanes - 1) / (size_t)g.lanes : 1;
     return (unsigned)std::max<size_t>(1, std::min<size_t>({(size_t)g.rows, (size_t)DOT_MAX_ROW_BLOCKS, by_cap}));
 }
 
 static void dot_launch(const DotGeom& g, const float* a, const float* b, double* part, unsigned gx, float* lane32,
-                       double* lane64) {
+                       double* lane64, double* total) {
     dim3 grid(gx, (unsigned)((g.lanes + DOT_TPB - 1) / DOT_TPB));
     k_dot_lanes<<<grid, DOT_TPB, 0, g_stream>>>(a, b, g.count, g.lanes, g.rows, part);
-    k_dot_finish<<<1, DOT_FIN_TPB, 0, g_stream>>>(part, gx, g.lanes, lane32, lane64);
+    k_dot_finish<<<1, DOT_FIN_TPB, 0, g_stream>>>(part, gx, g.lanes, lane32, lane64, total);
 }
 
 // ---------------------------------------------------------------- CG kernels
@@ -131,6 +145,31 @@ __global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ r
 
 using namespace flux;
 
+// ------------------------------------------------ allocation-free reducer
+// The partials and lane sums of flux_dot_noalloc live in this array; each
+// device holds its own copy until the process ends.  In host code `scratch`
+// names a host-side stand-in: kernels receive the device address obtained
+// once per device with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[DOT_SCRATCH_DOUBLES];
+
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static double* s_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+static int scratch_for_current_device(double** out) {
+    int dev = 0;
+    FLUX_HIP(hipGetDevice(&dev));
+    FLUX_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "dotNoAlloc: device %d out of range", dev);
+    if (!s_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(FLUX_ERR_HIP, "dotNoAlloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        s_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = s_scratch_ptr[dev];
+    return FLUX_OK;
+}
+
 struct flux_dot {
     const flux_field* a = nullptr;
     const flux_field* b = nullptr;
@@ -165,7 +204,7 @@ int flux_dot_create(int ndim, const int* shape, const flux_field* a, const flux_
     d->a = a;
     d->b = b;
     d->g = g;
-    d->gx = dot_row_blocks(g, (size_t)DOT_MAX_ROW_BLOCKS * (size_t)g.lanes);
+    d->gx = dot_row_blocks(g);
     const int64_t lanes64 = g.lanes;
     int rc = flux_field_create(FLUX_F32, 1, &lanes64, &d->partial);
     d->part = (double*)dmalloc(sizeof(double) * (size_t)d->gx * (size_t)g.lanes);
@@ -181,7 +220,8 @@ int flux_dot_create(int ndim, const int* shape, const flux_field* a, const flux_
 int flux_dot_dispatch(flux_dot* d) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(d, "createDotReducer: NULL reducer");
-    dot_launch(d->g, (const float*)d->a->d, (const float*)d->b->d, d->part, d->gx, (float*)d->partial->d, d->lane64);
+    dot_launch(d->g, (const float*)d->a->d, (const float*)d->b->d, d->part, d->gx, (float*)d->partial->d, d->lane64,
+               nullptr);
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
@@ -199,6 +239,30 @@ int flux_dot_info(const flux_dot* d, int* lanes, flux_field** partial) {
     FLUX_REQUIRE(d, "createDotReducer: NULL reducer");
     if (lanes) *lanes = d->g.lanes;
     if (partial) *partial = d->partial;
+    return FLUX_OK;
+}
+
+int flux_dot_noalloc(int ndim, const int* shape, const flux_field* a, const flux_field* b, flux_field* lanes_out,
+                     flux_field* total_out) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(shape && a && b && total_out, "dotNoAlloc: NULL argument");
+    DotGeom g;
+    FLUX_TRY(dot_geometry(ndim, shape, &g, "dotNoAlloc"));
+    FLUX_TRY(check_field(a, FLUX_F32, g.count, "dotNoAlloc: fieldA"));
+    FLUX_TRY(check_field(b, FLUX_F32, g.count, "dotNoAlloc: fieldB"));
+    if (lanes_out) FLUX_TRY(check_field(lanes_out, FLUX_F32, g.lanes, "dotNoAlloc: lanes"));
+    FLUX_REQUIRE(total_out->dtype == FLUX_F64 && total_out->count >= 1, "dotNoAlloc: total must be an f64 field");
+    FLUX_REQUIRE(dot_fits_scratch(g),
+                 "dotNoAlloc: %d lanes exceed the %zu-double scratch (at most %zu lanes); use createDotReducer", g.lanes,
+                 DOT_SCRATCH_DOUBLES, (DOT_SCRATCH_DOUBLES - 1) / 2);
+    double* base = nullptr;
+    FLUX_TRY(scratch_for_current_device(&base));
+    const unsigned gx = dot_row_blocks(g);
+    double* part = base;
+    double* lane64 = base + (size_t)gx * (size_t)g.lanes;
+    dot_launch(g, (const float*)a->d, (const float*)b->d, part, gx, lanes_out ? (float*)lanes_out->d : nullptr, lane64,
+               (double*)total_out->d);
+    FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }
 
