@@ -9,6 +9,20 @@ velocity correction (createGridPressureSolver3.project, grid_pressure_solver3.js
 The pressure is reset to zero before each step so every step solves the same
 system from the same start; inputs are resident in HBM before timing starts.
 
+Every form of the solver is run on the same system, each timed on its own:
+  plain_rbgs    FP32 V-cycle, red-black Gauss-Seidel 2/2 (multigrid.js defaults)
+  mixed_f16     the same V-cycle stored in FP16 (finest level included),
+                FP32 arithmetic, FP32 outer CG, FP64 dots and true residual
+  mixed_bf16    the same in BF16
+  plain_jacobi  (on request) FP32 V-cycle, damped Jacobi 4/4, omega 6/7
+                (sandbox/jacobi-smoother/multigrid_jacobi.js)
+The headline form (`--headline`, default mixed_f16) gives `value`,
+`ms_per_step` and the roofline; `forms` carries every form's time,
+iterations and FP64 relative residual, and `speedup_vs_plain` is each form's
+time against plain_rbgs.  The CPU leg times the oracle's restatement of the
+plain form on a bounded sample (extrapolated, and marked so) and solves the
+same system in FP64 to report each form's error against it.
+
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
 runs on one GPU.
 """
@@ -25,22 +39,36 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+FORMS = {
+    "plain_rbgs": {"storage": "f32", "smoother": "rbgs", "nu": 2, "omega": 1.0},
+    "mixed_f16": {"storage": "f16", "smoother": "rbgs", "nu": 2, "omega": 1.0},
+    "mixed_bf16": {"storage": "bf16", "smoother": "rbgs", "nu": 2, "omega": 1.0},
+    "plain_jacobi": {"storage": "f32", "smoother": "jacobi", "nu": 4, "omega": 6.0 / 7.0},
+}
+DEFAULT_FORMS = "plain_rbgs,mixed_f16,mixed_bf16"
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed projections of the headline form")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form-steps", type=int, default=5, help="timed projections of each other form")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--kernel", default="relax", help="kernel family timed for the roofline")
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
-                    help="V-cycle storage: f32 is the reference's plain method, f16/bf16 the mixed form")
-    ap.add_argument("--smoother", default="rbgs", choices=["rbgs", "jacobi"],
-                    help="V-cycle smoother: rbgs is the reference's plain method, jacobi its damped-Jacobi form")
-    ap.add_argument("--nu", type=int, default=2, help="smoothing iterations down / up / final")
-    ap.add_argument("--jacobi-omega", type=float, default=2.0 / 3.0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline", default="mixed_f16", choices=sorted(FORMS))
+    ap.add_argument("--forms", default=DEFAULT_FORMS,
+                    help="comma-separated forms to run (the headline always runs); plain_jacobi is not in the default "
+                         "set: at 256^3 its V-cycle is not a usable preconditioner for this system (the residual-growth "
+                         "guard stops it near iteration 29, on the oracle too)")
+    ap.add_argument("--kernel", default=None, help="kernel family timed for the roofline (default: the headline "
+                                                    "form's smoother)")
+    # a single custom form (overrides --headline) for sweeps
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--smoother", choices=["rbgs", "jacobi"])
+    ap.add_argument("--nu", type=int)
+    ap.add_argument("--jacobi-omega", type=float)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs (baseline and FP64 reference)")
     ap.add_argument("--cpu-iters", type=int, default=2, help="PCG iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -49,31 +77,41 @@ def parse():
     return ap.parse_args()
 
 
-# algorithmic bytes per finest-level cell and launch, per kernel family
-# (FP32 fields, u8 mask, three FP32 face-weight arrays ~ 12 B/cell)
-ALGO_BYTES_PER_CELL = {
-    # one colour of red-black GS: x read (4) + b on the colour (2) + mask (1)
-    # + face weights (12) + x written on the colour (2)
-    "relax": 21.0,
-    # one fused red-black sweep (both colours): x in (4) + b (4) + mask (1)
-    # + face weights (12) + x out (4)
-    "sweep": 25.0,
-    # p = z + beta p (z 4 + p 4 read, p 4 written), Ap written (4), mask 1, weights 12
-    "apply": 29.0,
-    # x64 read+write (16), p 4, b 4, r written 4, mask 1, weights 12
-    "update": 41.0,
-    # x 4 + b 4 + mask 1 + weights 12 read, r 4 written
-    "residual": 25.0,
-    # one damped-Jacobi sweep: x read 4, b 4, mask 1, weights 12, x' written 4
-    "jacobi": 25.0,
-    # coarse x (1/8 of 4) + mask 1 read, fine x read+write 8
-    "correct": 9.5,
-}
+def store_bytes(storage):
+    return 4 if storage == "f32" else 2
 
 
-# fine-level kernel of each family, as rocprofv3 names it
-KERNEL_SYMBOL = {"relax": "k_relax3<1, 0,", "sweep": "k_rbgs_fused<1,", "jacobi": "k_jacobi3<1,", "apply": "k_pcg_apply3",
-                 "update": "k_pcg_update3", "residual": "k_residual3<1,", "correct": "k_correct<1,"}
+def algo_bytes_per_cell(kernel, storage):
+    """Algorithmic bytes per finest-level cell and launch: the unique bytes
+    the launch must move (u8 neighbour-mask byte; face weights one value per
+    face, three per cell)."""
+    s = store_bytes(storage)
+    if kernel == "relax":
+        # one red-black colour launch: x read, b on the colour (half), x
+        # written on the colour (half) in the fine level's storage, mask 1,
+        # weights 3 x storage (21 B in FP32, 13 B in FP16/BF16)
+        return 2 * s + 1 + 3 * s
+    if kernel == "jacobi":
+        # one steady-state damped-Jacobi sweep: x read, b read, x' written in
+        # the fine level's storage (FP32 for f32; FP16/BF16 in the mixed
+        # form), mask 1, weights 3 x storage
+        return 3 * s + 1 + 3 * s
+    if kernel == "update":
+        # x64 read + write (16), p 4, b 4, r written 4, mask 1, weights 12
+        return 41.0
+    if kernel == "apply":
+        # z 4 + p 4 read, p 4 written, mask 1, weights 12
+        return 25.0
+    if kernel == "residual":
+        return 3 * s + 1 + 3 * s if storage != "f32" else 25.0
+    raise KeyError(kernel)
+
+
+# fine-level kernel of each family as rocprofv3 names it
+KERNEL_SYMBOL = {("relax", "f32"): "k_relax3<1, 0, 0,", ("relax", "f16"): "k_relax3m<0, 0, __half>",
+                 ("relax", "bf16"): "k_relax3m<0, 0, __hip_bfloat16>", ("jacobi", "f32"): "k_jacobi3<1, 0, 0,",
+                 ("jacobi", "f16"): "k_jacobi3m<0, 0, __half>", ("jacobi", "bf16"): "k_jacobi3m<0, 0, __hip_bfloat16>",
+                 ("update", "f32"): "k_pcg_update3", ("apply", "f32"): "k_pcg_apply3"}
 
 
 def measured_traffic(kernel, storage):
@@ -81,7 +119,7 @@ def measured_traffic(kernel, storage):
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
     None when no summary covers it."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_dambreak256_%s.json" % storage)
-    sym = KERNEL_SYMBOL.get(kernel)
+    sym = KERNEL_SYMBOL.get((kernel, storage)) or KERNEL_SYMBOL.get((kernel, "f32"))
     if not sym or not os.path.exists(path):
         return None
     best = None
@@ -104,21 +142,11 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(sc, levels, gpu_iters, iters, threads):
-    """The oracle's restatement of the same projection (f32 vectors, f64
-    solution accumulator: the HIP path's arithmetic) on the host cores:
-    setup plus `iters` PCG iterations are timed and scaled to a full
-    projection at the GPU's iteration count; run on `threads` OpenMP
-    threads and on one."""
-    one = cpu_sample(sc, levels, gpu_iters, 1, 1)  # single-threaded: a shorter sample
-    res = cpu_sample(sc, levels, gpu_iters, iters, threads)
-    res["single_thread_value"] = one["value"]
-    res["cpu_model"] = cpu_model()
-    res["nproc"] = os.cpu_count()
-    return res
-
-
 def cpu_sample(sc, levels, gpu_iters, iters, threads):
+    """The oracle's restatement of the plain projection (f32 vectors, f64
+    solution accumulator: the HIP plain path's arithmetic) on `threads` host
+    threads: assembly, V-cycle setup and 1 + `iters` PCG iterations are
+    timed; the projection time is extrapolated to the GPU's iteration count."""
     from oracle import pyoracle as O
     O.set_threads(threads)
     n, h = sc["shape"], sc["spacing"]
@@ -140,10 +168,28 @@ def cpu_sample(sc, levels, gpu_iters, iters, threads):
         "unit": "projections/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d^3 system: assembly + V-cycle setup + %d PCG iterations timed (%.1f s), scaled to %d iterations"
+        "extrapolated": True,
+        "sample": "plain form (RBGS 2/2, FP32 V-cycle) on the %d^3 system: assembly + V-cycle setup + %d PCG iterations "
+                  "timed (%.1f s); the projection time is extrapolated to the GPU's %d iterations"
                   % (n[0], 1 + iters, t3 - t0, gpu_iters),
         "s_per_iteration": per_iter,
     }
+
+
+def fp64_reference(sc, levels, x0, threads):
+    """The same system solved on the host in FP64 (the oracle's f64 PCG with
+    its FP64 V-cycle) to a relative residual of 1e-10, warm-started from a
+    GPU answer: the pinned system has one solution, so the start only
+    shortens the solve."""
+    from oracle import pyoracle as O
+    O.set_threads(threads)
+    n, h = sc["shape"], sc["spacing"]
+    t0 = time.perf_counter()
+    A = O.pressure3_assemble(n, h, sc["vel"], active=sc["active"], target=sc["target"], beta=sc["beta"], suffix="f64")
+    mg = O.Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f64")
+    x, st = O.pcg_solve(n, h, A["b"], x0=x0.astype(np.float64), mask=A["mask"], weights=A["weights"], mg=mg,
+                        tol=1e-10, maxiter=400)
+    return x, st, time.perf_counter() - t0
 
 
 def main():
@@ -153,6 +199,23 @@ def main():
         sys.exit(2)
     import tsl_array_n_amd as F
     from tsl_array_n_amd import scenes
+
+    forms = dict(FORMS)
+    headline = args.headline
+    if args.storage or args.smoother or args.nu or args.jacobi_omega:
+        base = dict(FORMS[headline])
+        for k, v in (("storage", args.storage), ("smoother", args.smoother), ("nu", args.nu),
+                     ("omega", args.jacobi_omega)):
+            if v is not None:
+                base[k] = v
+        forms = {"custom": base}
+        headline = "custom"
+        run = ["custom"]
+    else:
+        run = [f for f in args.forms.split(",") if f in FORMS and f != headline]
+        run = [headline] + run
+    hform = forms[headline]
+    kernel = args.kernel or ("jacobi" if hform["smoother"] == "jacobi" else "relax")
 
     F.init(int(os.environ.get("LOCAL_RANK", "0")))
     if args.copy_calibration:
@@ -170,78 +233,118 @@ def main():
     print("bench: building the %d^3 dam-break system on %s" % (n, dev), file=sys.stderr, flush=True)
     sc = scenes.dam_break_system(n)
     shape = sc["shape"]
+    cells = float(np.prod(shape))
     vin = [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
     vout = [F.array_n("float", F.face_shape(shape, a)) for a in range(3)]
     beta = [F.array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
     act = F.array_n("u8", shape, sc["active"])
     tgt = F.array_n("float", shape, sc["target"])
-    ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
-                                        face_weights=beta,
-                                        multigrid={"levels": args.levels, "storage": args.storage,
-                                                   "smoother": args.smoother, "nu_down": args.nu, "nu_up": args.nu,
-                                                   "n_final": args.nu, "jacobi_omega": args.jacobi_omega},
-                                        tolerance=1e-6,
-                                        relative_tolerance=True)
 
-    def step():
-        ps.pressure.fill(0.0)
-        return ps.project(vin, vout)
-
-    for w in range(args.warmup):
-        d = step()
-        print("bench: warmup %d: %s" % (w, d), file=sys.stderr, flush=True)
-
-    F.kprof_enable(args.kernel)
-    F.sync()
-    t0 = time.perf_counter()
-    iters = []
-    for s in range(args.steps):
-        d = step()
-        iters.append(d["iterations"])
-        if not d["converged"]:
-            print("bench: step %d did not converge: %s" % (s, d), file=sys.stderr, flush=True)
-    F.sync()
-    t1 = time.perf_counter()
-    k_ms, k_launches = F.kprof_read()
-    F.kprof_enable("")
-    elapsed = t1 - t0
-    print("bench: %d steps in %.3f s, iterations %s, last %s" % (args.steps, elapsed, iters, d), file=sys.stderr,
-          flush=True)
-
-    cells = float(np.prod(shape))
+    results, pressures = {}, {}
+    line_time = None
     roof = None
-    if k_launches:
-        avg_s = k_ms / 1e3 / k_launches
-        algo = ALGO_BYTES_PER_CELL.get(args.kernel, 0.0) * cells
-        achieved = algo / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.kernel, args.storage),
-                "kernel": args.kernel, "avg_us": round(avg_s * 1e6, 2), "launches": k_launches,
-                "algorithmic_bytes": algo}
+    for name in run:
+        fm = forms[name]
+        ps = F.create_grid_pressure_solver3(
+            resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt), face_weights=beta,
+            multigrid={"levels": args.levels, "storage": fm["storage"], "smoother": fm["smoother"],
+                       "nu_down": fm["nu"], "nu_up": fm["nu"], "n_final": fm["nu"], "jacobi_omega": fm["omega"],
+                       "omega": 1.0},
+            tolerance=1e-6, relative_tolerance=True)
+
+        def step():
+            ps.pressure.fill(0.0)
+            return ps.project(vin, vout)
+
+        for w in range(args.warmup):
+            d = step()
+            print("bench: %s warmup %d: %s" % (name, w, d), file=sys.stderr, flush=True)
+        steps = args.steps if name == headline else min(args.steps, args.form_steps)
+        if name == headline:
+            F.kprof_enable(kernel)
+        F.sync()
+        t0 = time.perf_counter()
+        iters, last = [], None
+        for s in range(steps):
+            last = step()
+            iters.append(last["iterations"])
+            if not last["converged"]:
+                print("bench: %s step %d did not converge: %s" % (name, s, last), file=sys.stderr, flush=True)
+        F.sync()
+        t1 = time.perf_counter()
+        if name == headline:
+            k_ms, k_launches = F.kprof_read()
+            F.kprof_enable("")
+            line_time = (t1 - t0, steps)
+            if k_launches:
+                avg_s = k_ms / 1e3 / k_launches
+                algo = algo_bytes_per_cell(kernel, fm["storage"]) * cells
+                achieved = algo / avg_s / 1e9
+                roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(kernel, fm["storage"]),
+                        "kernel": "%s (%s)" % (KERNEL_SYMBOL.get((kernel, fm["storage"]), kernel), fm["storage"]),
+                        "avg_us": round(avg_s * 1e6, 2), "launches": k_launches, "algorithmic_bytes": algo,
+                        "algorithmic_bytes_per_cell": algo / cells}
+        ms = (t1 - t0) / steps * 1e3
+        results[name] = {"ms_per_projection": round(ms, 3), "projections_per_s": round(1e3 / ms, 3),
+                         "iterations": int(np.median(iters)), "converged": bool(last["converged"]),
+                         "fp64_relative_residual": last["residual"] / last["bNorm"] if last["bNorm"] else None,
+                         "storage": fm["storage"], "smoother": fm["smoother"], "nu": fm["nu"],
+                         "omega": round(fm["omega"], 4)}
+        pressures[name] = ps.pressure.to_array()
+        print("bench: %s: %d steps, %.3f ms/projection, iterations %s, last %s" % (name, steps, ms, iters, last),
+              file=sys.stderr, flush=True)
+        del ps
+
+    plain = results.get("plain_rbgs")
+    if plain:
+        for r in results.values():
+            r["speedup_vs_plain"] = round(plain["ms_per_projection"] / r["ms_per_projection"], 3)
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        gpu_iters = (plain or results[headline])["iterations"]
         print("bench: CPU baseline on %d threads" % threads, file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sc, args.levels, int(np.median(iters)), args.cpu_iters, threads)
+        cpu = cpu_sample(sc, args.levels, gpu_iters, args.cpu_iters, threads)
+        one = cpu_sample(sc, args.levels, gpu_iters, 1, 1)
+        cpu["single_thread_value"] = one["value"]
+        cpu["cpu_model"] = cpu_model()
+        cpu["nproc"] = os.cpu_count()
+        print("bench: FP64 reference solve on %d threads" % threads, file=sys.stderr, flush=True)
+        x64, st64, t64 = fp64_reference(sc, args.levels, pressures[run[0]], threads)
+        nx = np.linalg.norm(x64)
+        for name, p in pressures.items():
+            results[name]["error_vs_fp64_cpu_solve"] = float(np.linalg.norm(p.astype(np.float64) - x64) / nx)
+        cpu["fp64_reference"] = {"tolerance": 1e-10, "iterations": st64["iterations"],
+                                 "converged": st64["converged"], "seconds": round(t64, 1),
+                                 "warm_start": run[0]}
 
+    elapsed, steps = line_time
+    hf = results[headline]
     line = {
         "metric": "projections/sec to 1e-6 rel residual at 256^3",
-        "value": args.steps / elapsed,
+        "value": steps / elapsed,
         "unit": "projections/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": elapsed / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 accumulator, residual and dots)",
+        "dtype": {"f32": "f32", "f16": "f16 storage / f32 arithmetic", "bf16": "bf16 storage / f32 arithmetic"}[
+            hform["storage"]] + " V-cycle; f32 CG vectors; f64 accumulator, residual and dots",
         "data": "synthetic: seeded 3-D two-phase dam break, frame-0 system",
-        "config": {"workload": "3D FLIP two-phase dam break %d^3, liquid:air 1000:1, pinned top-centre, "
-                               "MGPCG %d levels 2/2 RBGS, V-cycle storage %s, cold start" % (n, args.levels, args.storage),
-                   "cells": int(cells), "particles": sc["particles"], "iterations": int(np.median(iters)),
+        "config": {"workload": "3D FLIP two-phase dam break %d^3, liquid:air 1000:1, pinned top-centre, MGPCG %d levels, "
+                               "form %s (%s V-cycle, %s %d/%d), cold start"
+                               % (n, args.levels, headline, hform["storage"], hform["smoother"], hform["nu"],
+                                  hform["nu"]),
+                   "cells": int(cells), "particles": sc["particles"], "iterations": hf["iterations"],
                    "parallelism": "single GPU", "device": dev},
+        "speedup_vs_plain": hf.get("speedup_vs_plain"),
+        "forms": results,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -48,6 +48,10 @@ int flux_init(int device);
 int flux_shutdown(void);
 /* waits for all queued work (the reference's awaited readbacks) */
 int flux_sync(void);
+/* makes the library's device current on the calling host thread, for
+ * bindings that run library calls on worker threads (the N-API addon's
+ * async work; the reference has one JS thread, context.js:74-144) */
+int flux_thread_bind(void);
 /* device name and compute-unit count */
 int flux_device_info(char* name, size_t name_len, int* compute_units);
 
@@ -169,6 +173,21 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
  * residual b - A x in FP64 arithmetic, dots are FP64, all scalars stay on
  * the GPU; the host reads one scalar block per chunk of iterations. */
 int flux_pcg_solve(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* state);
+/* No counterpart in the reference (whose beta is r'z'/rz, linalg.js:1198-1226):
+ * the flexible (Polak-Ribiere) beta r'.(z' - z)/rz for an inexact
+ * preconditioner.  Off by default. */
+int flux_pcg_set_flexible(flux_pcg* s, int flexible);
+/* No counterpart in the reference: an additive coarse-space correction of
+ * the preconditioner, M^-1 r = V(r) + Phi (Phi^T A Phi)^-1 Phi^T r, where
+ * the columns of Phi are the free-cell indicator (free_indicator != 0; the
+ * near-null mode of a system pinned at one cell) and up to three FLUX_F32
+ * fields of the operator's shape (such as a learned guess), each restricted
+ * to the free cells.  Phi^T A Phi is assembled in FP64 at every solve's
+ * setup and inverted on the host (degenerate directions dropped); Phi^T r
+ * is fused into the residual update and Phi a into the direction update, so
+ * an iteration reads each field twice more.  The fields stay bound; pass
+ * (0, 0, NULL) to remove the correction. */
+int flux_pcg_set_coarse_space(flux_pcg* s, int free_indicator, int nfields, const flux_field* const* fields);
 /* No counterpart in the reference (support for an external initial guess,
  * such as the learned one): x <- x + alpha g on the free cells, with
  * alpha = g.(b - A x) / g.(A g) in FP64, the step along g that minimises the
@@ -285,6 +304,11 @@ int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_f
 int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, double* alpha);
 /* flux_pcg_block_start on the assembled system; no counterpart in the reference */
 int flux_pressure3_block_guess(flux_pressure3* p, const flux_field* guess, int block, int* coarse_iterations);
+/* flux_pcg_set_coarse_space / flux_pcg_set_flexible on the projection's
+ * solver; no counterpart in the reference */
+int flux_pressure3_set_coarse_space(flux_pressure3* p, int free_indicator, int nfields,
+                                    const flux_field* const* fields);
+int flux_pressure3_set_flexible(flux_pressure3* p, int flexible);
 /* .pressure.data, .b, .dirichletMask, .ventMask (grid_pressure_solver3.js:572) */
 flux_field* flux_pressure3_field(flux_pressure3* p, int which);
 enum { FLUX_P3_PRESSURE = 0, FLUX_P3_RHS = 1, FLUX_P3_MASK = 2, FLUX_P3_VENT = 3, FLUX_P3_TARGET = 4,
@@ -685,6 +709,16 @@ int flux_unet_forward(flux_unet* u, const flux_field* input, flux_field* out);
 /* forward, MSE against target, backward, and an Adam step when lr > 0 */
 int flux_unet_train_step(flux_unet* u, const flux_field* input, const flux_field* target, float lr, float beta1,
                          float beta2, float eps, double* loss);
+/* No counterpart in the reference: a training step on the energy-norm loss
+ * L = 1 - (g.r)^2 / ((g.B g) e_star) of the guess g = network output on the
+ * free cells (mask: the system's Dirichlet mask), B = -A the system's
+ * operator `op`, r the residual of the start (b on the free cells when the
+ * start is 0 there) and e_star = x*.(-r) > 0 for the exact correction x*:
+ * the relative energy error after the solver's optimal step along g.  The
+ * gradient uses one application of `op`; Adam when lr > 0.  *loss = L. */
+int flux_unet_train_step_energy(flux_unet* u, const flux_field* input, flux_operator* op, const flux_field* r,
+                                const flux_field* mask, double e_star, float lr, float beta1, float beta2, float eps,
+                                double* loss);
 /* input channels from an assembled system (b, Dirichlet mask, face weights) */
 int flux_unet_features(flux_unet* u, const flux_field* b, const flux_field* mask, const flux_field* const* weights,
                        float density_ratio, flux_field* input);

@@ -187,6 +187,7 @@ ORC_DECL(float, f32)
 
 /* the HIP plain path's arithmetic: f32 vectors, double solution accumulator,
  * double true residual, double dots (see oracle_f32.c) */
+void orc_set_flexible(int on);
 int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec,
                         void* mg, const float* b, float* x, double* x64_out, double tol, int maxiter,
                         int relative, orc_pcg_state* st);

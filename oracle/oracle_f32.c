@@ -56,6 +56,13 @@ static void apply_mixed(const orc_grid* g, const double* x, float* r, const floa
     *rr_out = s;
 }
 
+/* Flexible (Polak-Ribiere) beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k instead of
+ * the reference's r_{k+1}.z_{k+1} / r_k.z_k (linalg.js:1198-1226): the
+ * library's choice for mixed-precision V-cycles, restated here so the GPU
+ * path can be compared with it (orc_set_flexible). */
+static int g_flexible = 0;
+void orc_set_flexible(int on) { g_flexible = on ? 1 : 0; }
+
 int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec, void* mg,
                         const float* b, float* x, double* x64_out, double tol, int maxiter, int relative,
                         orc_pcg_state* st) {
@@ -112,6 +119,7 @@ int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* con
             for (int64_t q = 0; q < n; ++q) xd[q] = xd[q] + alpha * (double)p[q];
             apply_mixed(g, xd, r, b, mask, w, &rr);
             newRTr = rr;
+            const double rzo = g_flexible ? dot_f32(r, z, n) : 0.0; /* r_{k+1}.z_k */
             double res = sqrt(fabs(rr));
             if (res <= thr) { st->stopped_by = ORC_STOP_CONVERGED; break; }
             if (best < 0 || fabs(rr) < best) best = fabs(rr);
@@ -120,7 +128,7 @@ int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* con
             double newRZ = dot_f32(r, z, n);
             if (degenerate_f32(oldRZ)) { st->stopped_by = ORC_STOP_DEGENERATE_OLD_RZ; break; }
             int flip = (oldRZ > 0) != (newRZ > 0);
-            double braw = newRZ / oldRZ;
+            double braw = (g_flexible ? newRZ - rzo : newRZ) / oldRZ;
             beta = (flip || !(fabs(braw) <= 10)) ? 0.0 : braw;
             oldRZ = newRZ;
         }

@@ -523,6 +523,11 @@ def two_phase_beta3(frac, n, rho_gas, rho_liquid):
     return outs
 
 
+def set_flexible(on):
+    """the flexible (Polak-Ribiere) beta in pcg_solve_mixed"""
+    lib().orc_set_flexible(int(bool(on)))
+
+
 def pcg_solve_mixed(shape, spacing, b, x0=None, mask=None, weights=None, precond="multigrid", mg=None,
                     tol=1e-6, maxiter=100, relative=True):
     """The HIP plain path's arithmetic: f32 vectors, f64 accumulator and residual."""
@@ -1080,3 +1085,85 @@ def block_galerkin_start(shape, spacing, b, x, g, block, mask=None, weights=None
         alpha[keep] = np.linalg.solve(M, f)
     xn = x64 + np.where(free, alpha[bid] * g64, 0.0)
     return xn, alpha
+
+
+def pcg_coarse_space(shape, spacing, b, mask, weights, mg, vectors, free_indicator=True, tol=1e-6, maxiter=200):
+    """PCG with the additive coarse-space correction of the preconditioner,
+    M^-1 r = V(r) + Phi (Phi^T A Phi)^-1 Phi^T r (Phi = [free-cell indicator]
+    + vectors, each restricted to the free cells), in the GPU plain path's
+    arithmetic: f32 r, z, p; f64 solution accumulator and true residual;
+    f64 dots; the reference's guards (linalg.js:1147-1311) and beta.  A
+    restatement in numpy around the oracle's f32 V-cycle and operator."""
+    n = int(np.prod(shape))
+    free = (np.asarray(mask) == 0) if mask is not None else np.ones(n, bool)
+    cols = ([np.where(free, 1.0, 0.0)] if free_indicator else []) + [np.where(free, np.asarray(v, np.float64), 0.0)
+                                                                        for v in vectors]
+    Phi = np.stack(cols, 1) if cols else np.zeros((n, 0))
+    AP = np.stack([apply(shape, spacing, Phi[:, q], mask, weights, suffix="f64") for q in range(Phi.shape[1])], 1) \
+        if cols else np.zeros((n, 0))
+    E = Phi.T @ AP
+    if cols:
+        lam, V = np.linalg.eigh(-0.5 * (E + E.T))
+        keep = lam > 1e-12 * lam.max()
+        Einv = -(V[:, keep] / lam[keep]) @ V[:, keep].T
+    phi32 = Phi.astype(np.float32)
+    b32 = np.asarray(b, np.float32)
+    x = np.zeros(n)
+    r64 = b32.astype(np.float64) - apply(shape, spacing, x, mask, weights, suffix="f64")
+    r = r64.astype(np.float32)
+    bb = float(np.dot(b32.astype(np.float64), b32.astype(np.float64)))
+    thr = max(tol * np.sqrt(bb), tol)
+    rr0 = float(np.dot(r64, r64))
+    st = {"iterations": 0, "stoppedBy": "none", "converged": False}
+    if np.sqrt(rr0) < thr:
+        st.update(stoppedBy="converged", converged=True)
+        return x, st
+
+    def prec(r):
+        z = mg.apply(r).astype(np.float32)
+        if cols:
+            c = Phi.T @ r.astype(np.float64)
+            a = (Einv @ c).astype(np.float32)
+            z = (z + (phi32 @ a)).astype(np.float32)
+        return z
+
+    z = prec(r)
+    rz_old = float(np.dot(r.astype(np.float64), z.astype(np.float64)))
+    base = max(abs(rr0), 1e-12)
+    best, beta = -1.0, 0.0
+    p = np.zeros(n, np.float32)
+    for it in range(maxiter):
+        st["iterations"] = it + 1
+        p = (z + p * np.float32(beta)).astype(np.float32)
+        Ap = apply(shape, spacing, p, mask, weights, suffix="f64")
+        pAp = float(np.dot(p.astype(np.float64), Ap))
+        alpha = rz_old / pAp
+        if not np.isfinite(pAp) or pAp == 0:
+            st["stoppedBy"] = "degenerate-pAp"
+            break
+        if abs(pAp) > base * 1e8:
+            st["stoppedBy"] = "pAp-growth"
+            break
+        if not abs(alpha) <= 1e6:
+            st["stoppedBy"] = "alpha-magnitude"
+            break
+        x = x + alpha * p.astype(np.float64)
+        r64 = b32.astype(np.float64) - apply(shape, spacing, x, mask, weights, suffix="f64")
+        r = r64.astype(np.float32)
+        rr = float(np.dot(r64, r64))
+        if np.sqrt(rr) <= thr:
+            st.update(stoppedBy="converged", converged=True)
+            break
+        if best < 0 or rr < best:
+            best = rr
+        elif rr > best * 100:
+            st["stoppedBy"] = "residual-growing"
+            break
+        z = prec(r)
+        rz = float(np.dot(r.astype(np.float64), z.astype(np.float64)))
+        flip = (rz_old > 0) != (rz > 0)
+        braw = rz / rz_old
+        beta = 0.0 if (flip or not abs(braw) <= 10) else braw
+        rz_old = rz
+    st["residual"] = float(np.sqrt(rr)) if st["iterations"] else float(np.sqrt(rr0))
+    return x, st

@@ -97,3 +97,4 @@ def test_jacobi_odd_sweeps_rejected(flux):
     F = flux
     with pytest.raises(F.FluxError, match="must be even"):
         F.create_multigrid_preconditioner((8, 8, 8), (1, 1, 1), levels=2, nu_down=3, smoother="jacobi")
+

@@ -222,3 +222,24 @@ def test_karman2d_through_node_matches_oracle(tmp_path):
     vel = np.concatenate([rd("u"), rd("v")])
     want = np.concatenate([st["u"], st["v"]])
     assert np.linalg.norm(vel - want) <= 2e-5 * np.linalg.norm(want)
+
+
+@pytest.mark.gpu
+def test_async_projection_dot_reducer_and_cg_through_node(tmp_path):
+    """project() resolves without blocking the event loop (napi_async_work),
+    concurrent calls are serialised; createDotReducer's single-cell probes and
+    ramp (examples/31-conjugate-gradient-3d/main.js:168-238); CG on
+    A = diag(1..8) (examples/04-conjugate-gradient)."""
+    out = tmp_path / "async.json"
+    r = run_node("async_linalg.mjs", out, timeout=300)
+    print(r.stdout)
+    d = json.load(open(out))
+    assert d["project"]["diag"]["converged"]
+    assert d["project"]["ticks"] > 0  # the event loop ran while the GPU solved
+    assert d["both"] == [True, True]
+    assert d["lanes"] == 6
+    for p in d["probes"]:
+        assert p["total"] == 1 and p["lit"] == [[p["cell"][0], 1]], p
+    assert d["ramp"] == 7140
+    assert d["cgSucceeded"]
+    np.testing.assert_allclose(d["cgX"], 1.0 / np.arange(1, 9), atol=1e-3)

@@ -315,6 +315,16 @@ class PCGSolver:
         self.h = h.value
         self.state = {}
 
+    def set_coarse_space(self, free_indicator=True, fields=()):
+        fields = list(fields)
+        self._cs_keep = fields
+        arr = (C.c_void_p * max(1, len(fields)))(*[f.h for f in fields]) if fields else None
+        _check(_lib.flux_pcg_set_coarse_space(C.c_void_p(self.h), int(bool(free_indicator)), len(fields), arr),
+               "coarseSpace")
+
+    def set_flexible(self, on=True):
+        _check(_lib.flux_pcg_set_flexible(C.c_void_p(self.h), int(bool(on))), "flexible")
+
     def solve(self, tol, maxiter, relative_tolerance=False):
         st = PCGState()
         _check(_lib.flux_pcg_solve(C.c_void_p(self.h), C.c_double(tol), int(maxiter), int(bool(relative_tolerance)),
@@ -477,6 +487,21 @@ class GridPressureSolver3:
         _check(_lib.flux_pressure3_scaled_guess(C.c_void_p(self.h), C.c_void_p(guess.h), C.byref(alpha)),
                "scaledGuess")
         return alpha.value
+
+    def set_coarse_space(self, free_indicator=True, fields=()):
+        """the additive coarse-space correction of the preconditioner:
+        M^-1 r = V(r) + Phi (Phi^T A Phi)^-1 Phi^T r, Phi = [free-cell
+        indicator] + fields (each restricted to the free cells); the fields
+        stay bound to the solver"""
+        fields = list(fields)
+        self._cs_keep = fields
+        arr = (C.c_void_p * max(1, len(fields)))(*[f.h for f in fields]) if fields else None
+        _check(_lib.flux_pressure3_set_coarse_space(C.c_void_p(self.h), int(bool(free_indicator)), len(fields), arr),
+               "coarseSpace")
+
+    def set_flexible(self, on=True):
+        """the flexible (Polak-Ribiere) beta in the projection's PCG"""
+        _check(_lib.flux_pressure3_set_flexible(C.c_void_p(self.h), int(bool(on))), "flexible")
 
     def block_guess(self, guess, block=8):
         """between assemble() and solve(): pressure += alpha_K * guess on the

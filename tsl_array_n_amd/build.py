@@ -85,7 +85,7 @@ def build_addon(verbose=False):
     cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-Wno-unused-parameter",
            "-DNODE_GYP_MODULE_NAME=fluxflow_napi", "-I" + NODE_INCLUDE, "-include", "stdio.h",
            os.path.join(HERE, "node", "fluxflow_napi.c"), "-o", ADDON + ".tmp",
-           "-L" + OUT, "-lfluxhip", "-Wl,-rpath,$ORIGIN"]
+           "-L" + OUT, "-lfluxhip", "-lpthread", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("addon build failed: %s\n%s" % (r.stdout, r.stderr))

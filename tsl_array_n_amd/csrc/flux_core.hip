@@ -143,6 +143,12 @@ int flux_shutdown(void) {
     return FLUX_OK;
 }
 
+int flux_thread_bind(void) {
+    FLUX_REQUIRE_INIT();
+    FLUX_HIP(hipSetDevice(g_device));
+    return FLUX_OK;
+}
+
 int flux_sync(void) {
     FLUX_REQUIRE_INIT();
     FLUX_HIP(hipStreamSynchronize(g_stream));

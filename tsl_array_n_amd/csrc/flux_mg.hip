@@ -191,6 +191,126 @@ __global__ void __launch_bounds__(MTPB) k_relax3(LevelDevT<WT> L, March M, XT* _
     if (DOT) store_block_partial(dot, partials);
 }
 
+// One red-black colour of the finest level in the mixed form, where the
+// level's x and b are stored in ST (FP16/BF16) in scaled units (b~ = r s,
+// x~ = x s / xs; the operator carries xs / h^2), in k_relax3's pair shape:
+//   MODE 1 (first launch, colour 0, from x~ = 0): reads r (FP32), stores
+//          b~ = r s for the later launches and the residual, and x~;
+//   MODE 0: one colour in place on x~ with b~;
+//   MODE 2 (the V-cycle's last launch): the colour's update with b from r
+//          (FP32, times s); both cells of the pair are written to the FP32
+//          output z = x~ xs / s (the PCG's z), with DOT the r.z partials.
+// In MODE 2 the pair is read from x~ and written to z, so no x~ value is
+// overwritten while another thread may still read it.
+template <int MODE, int DOT, typename ST>
+__global__ void __launch_bounds__(MTPB) k_relax3m(LevelDevT<ST> L, March M, ST* __restrict__ x,
+                                                  const ST* __restrict__ b16, ST* __restrict__ b16out,
+                                                  const float* __restrict__ r, float* __restrict__ z,
+                                                  const double* __restrict__ rr, float xs, float omega, int color,
+                                                  const int* __restrict__ stop, double* __restrict__ partials) {
+    if (stop && *stop) return;
+    double dot = 0.0;
+    int t, k0, k1;
+    const bool active = march_begin(M, t, k0, k1);
+    if (!DOT && !active) return;
+    if (active) {
+        const int half = M.run;
+        const int j = t / half;
+        const int p = t - j * half;
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        const float s = residual_scale(rr, n0 * n1 * n2);
+        const float zscale = xs / s;
+        Pair2<ST> xnext;
+        Pair2<uint8_t> nnext;
+        const int cp0 = 2 * p + n0 * (j + n1 * k0);
+        if (MODE != 1) xnext = *reinterpret_cast<const Pair2<ST>*>(x + cp0);
+        if (nbits) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp0);
+        float xprev = 0.0f;
+        for (int k = k0; k < k1; ++k) {
+            const int sel = (j + k + color) & 1;  // which cell of the pair has the colour
+            const int i = 2 * p + sel;
+            const int cp = 2 * p + n0 * (j + n1 * k);
+            const int c = cp + sel;
+            const Pair2<ST> xp = xnext;
+            const Pair2<uint8_t> np = nnext;
+            if (k + 1 < n2) {
+                if (MODE != 1) xnext = *reinterpret_cast<const Pair2<ST>*>(x + cp + plane);
+                if (nbits && k + 1 < k1) nnext = *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp + plane);
+            }
+            const unsigned bits = nbits ? (sel ? np.hi : np.lo) : 0u;
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            const bool dir = (bits & 64u) != 0;
+            float bc, rlo = 0.0f, rhi = 0.0f;
+            if (MODE == 0) {
+                const Pair2<ST> bp = *reinterpret_cast<const Pair2<ST>*>(b16 + cp);
+                bc = to_f32(sel ? bp.hi : bp.lo);
+            } else {
+                const Pair2<float> rp = *reinterpret_cast<const Pair2<float>*>(r + cp);
+                rlo = rp.lo;
+                rhi = rp.hi;
+                if (MODE == 1) {
+                    Pair2<ST> bo;
+                    bo.lo = to_store<ST>(rp.lo * s);
+                    bo.hi = to_store<ST>(rp.hi * s);
+                    *reinterpret_cast<Pair2<ST>*>(b16out + cp) = bo;
+                    bc = to_f32(sel ? bo.hi : bo.lo);
+                } else {
+                    bc = (sel ? rp.hi : rp.lo) * s;
+                }
+            }
+            if (MODE == 1) {
+                float sum, dsum;
+                const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                row_terms<float>(0.0f, zn, w, ih2, sum, dsum);
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -0.0f : sum;
+                const float v = 0.0f + ((bc - ax) / d) * omega;
+                Pair2<ST> o;
+                o.lo = to_store<ST>(sel ? 0.0f : v);
+                o.hi = to_store<ST>(sel ? v : 0.0f);
+                *reinterpret_cast<Pair2<ST>*>(x + cp) = o;
+            } else {
+                const float xc = to_f32(sel ? xp.hi : xp.lo);
+                const float xpart = to_f32(sel ? xp.lo : xp.hi);
+                const int qo = sel ? (i < n0 - 1 ? c + 1 : c) : (i > 0 ? c - 1 : c);
+                const float xo = to_f32(x[qo]);
+                float xn[6];
+                xn[0] = sel ? xpart : xo;
+                xn[1] = sel ? xo : xpart;
+                xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
+                xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
+                xn[4] = k > k0 ? xprev : to_f32(x[k > 0 ? c - plane : c]);
+                xn[5] = k + 1 < n2 ? to_f32(sel ? xnext.hi : xnext.lo) : xc;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w, ih2, sum, dsum);
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                const ST nv = to_store<ST>(xc + ((bc - ax) / d) * omega);
+                Pair2<ST> o = xp;
+                if (sel) o.hi = nv;
+                else o.lo = nv;
+                if (MODE == 2) {
+                    Pair2<float> zo;
+                    zo.lo = to_f32(o.lo) * zscale;
+                    zo.hi = to_f32(o.hi) * zscale;
+                    *reinterpret_cast<Pair2<float>*>(z + cp) = zo;
+                    if (DOT) dot += (double)rlo * (double)zo.lo + (double)rhi * (double)zo.hi;
+                } else {
+                    *reinterpret_cast<Pair2<ST>*>(x + cp) = o;
+                }
+                xprev = xpart;
+            }
+        }
+    }
+    if (DOT) store_block_partial(dot, partials);
+}
+
 // One damped-Jacobi sweep (buildJacobiRelaxKernel,
 // sandbox/jacobi-smoother/multigrid_jacobi.js:632-642): every cell reads the
 // old iterate xr and writes xr_c + ((b - A xr)_c / diag_c) * omega to xw, so
@@ -348,6 +468,178 @@ __global__ void __launch_bounds__(MTPB) k_jacobi3(LevelDevT<WT> L, March M, cons
                 dot += (double)to_f32(bp.lo) * (double)to_f32(o.lo);
                 dot += (double)to_f32(bp.hi) * (double)to_f32(o.hi);
             }
+        }
+    }
+    if (DOT) store_block_partial(dot, partials);
+}
+
+// The damped-Jacobi sweep of the finest level in the mixed form, where the
+// level's x and b are stored in ST (FP16/BF16) in scaled units: b~ = r s
+// (s the residual scale, unit RMS for FP16, 1 for BF16) and x~ = x s / xs
+// (xs the level's storage scale, the operator carrying xs / h^2).  The
+// PCG's vectors stay FP32:
+//   MODE 1 (first sweep, from x~ = 0): reads r (FP32), stores b~ = r s (ST)
+//          for the later sweeps and the residual, and x~' (ST);
+//   MODE 0 (middle sweeps): x~ -> x~' with b~;
+//   MODE 2 (last sweep): reads b from r (FP32, times s), writes the FP32
+//          output z = x~' xs / s and, with DOT, the r.z partials.
+// Same column-march pair shape and per-cell arithmetic as k_jacobi3.
+template <int MODE, int DOT, typename ST>
+__global__ void __launch_bounds__(MTPB) k_jacobi3m(LevelDevT<ST> L, March M, const ST* __restrict__ xr,
+                                                   ST* __restrict__ xw, const ST* __restrict__ b16,
+                                                   ST* __restrict__ b16out, const float* __restrict__ r,
+                                                   float* __restrict__ z, const double* __restrict__ rr, float xs,
+                                                   float omega, const int* __restrict__ stop,
+                                                   double* __restrict__ partials) {
+    if (stop && *stop) return;
+    double dot = 0.0;
+    int t, k0, k1;
+    const bool active = march_begin(M, t, k0, k1);
+    if (!DOT && !active) return;
+    if (active) {
+        const int half = M.run;
+        const int j = t / half;
+        const int p = t - j * half;
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const int i0 = 2 * p;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+        const bool has_w = L.w0 != nullptr;
+        const float s = residual_scale(rr, n0 * n1 * n2);
+        const float zscale = xs / s;
+        Pair2<ST> xc2, xnext, xprev;
+        const int cp0 = i0 + n0 * (j + n1 * k0);
+        if (MODE != 1) {
+            xnext = *reinterpret_cast<const Pair2<ST>*>(xr + cp0);
+            if (k0 > 0) xprev = *reinterpret_cast<const Pair2<ST>*>(xr + cp0 - plane);
+        }
+        Pair2<ST> wz_lo;
+        if (has_w) wz_lo = *reinterpret_cast<const Pair2<ST>*>(L.w2 + cp0);
+        for (int k = k0; k < k1; ++k) {
+            const int cp = i0 + n0 * (j + n1 * k);
+            if (MODE != 1) {
+                xc2 = xnext;
+                if (k + 1 < n2) xnext = *reinterpret_cast<const Pair2<ST>*>(xr + cp + plane);
+            }
+            const Pair2<uint8_t> np = nbits ? *reinterpret_cast<const Pair2<uint8_t>*>(nbits + cp) : Pair2<uint8_t>{0, 0};
+            float bv[2], rv[2] = {0.0f, 0.0f};
+            if (MODE == 0) {
+                const Pair2<ST> bp = *reinterpret_cast<const Pair2<ST>*>(b16 + cp);
+                bv[0] = to_f32(bp.lo);
+                bv[1] = to_f32(bp.hi);
+            } else {
+                const Pair2<float> rp = *reinterpret_cast<const Pair2<float>*>(r + cp);
+                rv[0] = rp.lo;
+                rv[1] = rp.hi;
+                bv[0] = rp.lo * s;
+                bv[1] = rp.hi * s;
+                if (MODE == 1) {
+                    Pair2<ST> bo;
+                    bo.lo = to_store<ST>(bv[0]);
+                    bo.hi = to_store<ST>(bv[1]);
+                    *reinterpret_cast<Pair2<ST>*>(b16out + cp) = bo;
+                    // the later sweeps see b~ rounded to storage; so does this one
+                    bv[0] = to_f32(bo.lo);
+                    bv[1] = to_f32(bo.hi);
+                }
+            }
+            float wx0 = 1.0f, wx1 = 1.0f, wx2 = 1.0f, wyl[2] = {1.0f, 1.0f}, wyu[2] = {1.0f, 1.0f};
+            float wzl[2] = {1.0f, 1.0f}, wzu[2] = {1.0f, 1.0f};
+            Pair2<ST> wz_up;
+            if (has_w) {
+                const int fx = i0 + (n0 + 1) * (j + n1 * k);
+                wx0 = to_f32(L.w0[fx]);
+                wx1 = to_f32(L.w0[fx + 1]);
+                wx2 = to_f32(L.w0[fx + 2]);
+                const int fy = i0 + n0 * (j + (n1 + 1) * k);
+                const Pair2<ST> yl = *reinterpret_cast<const Pair2<ST>*>(L.w1 + fy);
+                const Pair2<ST> yu = *reinterpret_cast<const Pair2<ST>*>(L.w1 + fy + n0);
+                wyl[0] = to_f32(yl.lo);
+                wyl[1] = to_f32(yl.hi);
+                wyu[0] = to_f32(yu.lo);
+                wyu[1] = to_f32(yu.hi);
+                wz_up = *reinterpret_cast<const Pair2<ST>*>(L.w2 + cp + plane);
+                wzl[0] = to_f32(wz_lo.lo);
+                wzl[1] = to_f32(wz_lo.hi);
+                wzu[0] = to_f32(wz_up.lo);
+                wzu[1] = to_f32(wz_up.hi);
+            }
+            float w[2][6];
+            w[0][0] = i0 > 0 ? wx0 : 0.0f;
+            w[0][1] = wx1;
+            w[1][0] = wx1;
+            w[1][1] = i0 + 1 < n0 - 1 ? wx2 : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                w[q][2] = j > 0 ? wyl[q] : 0.0f;
+                w[q][3] = j < n1 - 1 ? wyu[q] : 0.0f;
+                w[q][4] = k > 0 ? wzl[q] : 0.0f;
+                w[q][5] = k < n2 - 1 ? wzu[q] : 0.0f;
+            }
+            float out[2];
+            if (MODE == 1) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const unsigned bits = q ? np.hi : np.lo;
+                    const float zn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    float sum, dsum;
+                    row_terms<float>(0.0f, zn, w[q], ih2, sum, dsum);
+                    const bool dir = (bits & 64u) != 0;
+                    const float d = dir ? -1.0f : dsum;
+                    const float ax = dir ? -0.0f : sum;
+                    out[q] = 0.0f + ((bv[q] - ax) / d) * omega;
+                }
+            } else {
+                const float xl = to_f32(xr[i0 > 0 ? cp - 1 : cp]);
+                const float xu = to_f32(xr[i0 + 1 < n0 - 1 ? cp + 2 : cp + 1]);
+                const Pair2<ST> ys = *reinterpret_cast<const Pair2<ST>*>(xr + (j > 0 ? cp - n0 : cp));
+                const Pair2<ST> yn = *reinterpret_cast<const Pair2<ST>*>(xr + (j < n1 - 1 ? cp + n0 : cp));
+                const float c0 = to_f32(xc2.lo), c1 = to_f32(xc2.hi);
+                float xn[2][6];
+                xn[0][0] = xl;
+                xn[0][1] = c1;
+                xn[1][0] = c0;
+                xn[1][1] = xu;
+                xn[0][2] = to_f32(ys.lo);
+                xn[1][2] = to_f32(ys.hi);
+                xn[0][3] = to_f32(yn.lo);
+                xn[1][3] = to_f32(yn.hi);
+                xn[0][4] = k > 0 ? to_f32(xprev.lo) : c0;
+                xn[1][4] = k > 0 ? to_f32(xprev.hi) : c1;
+                xn[0][5] = k + 1 < n2 ? to_f32(xnext.lo) : c0;
+                xn[1][5] = k + 1 < n2 ? to_f32(xnext.hi) : c1;
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const unsigned bits = q ? np.hi : np.lo;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) xn[q][a] = ((bits >> a) & 1u) ? 0.0f : xn[q][a];
+                    const float xc = q ? c1 : c0;
+                    float sum, dsum;
+                    row_terms<float>(xc, xn[q], w[q], ih2, sum, dsum);
+                    const bool dir = (bits & 64u) != 0;
+                    const float d = dir ? -1.0f : dsum;
+                    const float ax = dir ? -xc : sum;
+                    out[q] = xc + ((bv[q] - ax) / d) * omega;
+                }
+                xprev = xc2;
+            }
+            if (MODE == 2) {
+                Pair2<float> o;
+                o.lo = to_f32(to_store<ST>(out[0])) * zscale;
+                o.hi = to_f32(to_store<ST>(out[1])) * zscale;
+                *reinterpret_cast<Pair2<float>*>(z + cp) = o;
+                if (DOT) {
+                    dot += (double)rv[0] * (double)o.lo;
+                    dot += (double)rv[1] * (double)o.hi;
+                }
+            } else {
+                Pair2<ST> o;
+                o.lo = to_store<ST>(out[0]);
+                o.hi = to_store<ST>(out[1]);
+                *reinterpret_cast<Pair2<ST>*>(xw + cp) = o;
+            }
+            if (has_w) wz_lo = wz_up;
         }
     }
     if (DOT) store_block_partial(dot, partials);
@@ -1108,6 +1400,9 @@ struct MGLevel {
     void* b = nullptr;    // level 0: the caller's input (FP32)
     void* buf = nullptr;  // residual (FP32 on level 0)
     void* alt = nullptr;  // ping-pong partner of x for the fused sweep
+    // the finest level's storage-type copies in the mixed Jacobi form
+    // (x~, b~ and the ping-pong / residual buffer), scaled units
+    void *x16 = nullptr, *b16 = nullptr, *buf16 = nullptr;
     uint8_t* mask = nullptr;              // level 0: the caller's (not owned)
     uint8_t* nb = nullptr;                // neighbour-mask bits of the column-march kernels
     void* w[3] = {nullptr, nullptr, nullptr};  // level 0: caller's FP32 or own converted copy
@@ -1129,6 +1424,13 @@ struct flux_multigrid {
     bool fused = false;  // fused sweeps on every level wide enough (FLUX_FUSED_SWEEP=1)
     bool tail = true;    // the last two levels in one workgroup (FLUX_VCYCLE_TAIL=0 turns it off)
     bool pairs = true;   // paired prolongation-correction on even rows (FLUX_CORRECT_PAIRS=0 turns it off)
+    // mixed form: the finest level stores x and b in the storage type too
+    // (opt-in, FLUX_FINE_STORAGE=1).  Off by default: the rounding of z at
+    // every smoothing update makes the V-cycle a slightly nonlinear operator
+    // and PCG needs more iterations (64^3 dam break 50 -> 51 with FP16 and
+    // red-black, 256^3 62 -> 82), more than the halved fine-level traffic
+    // saves.
+    bool fine16 = false;
 };
 
 static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
@@ -1270,7 +1572,8 @@ void jacobi_sweeps(flux_multigrid* mg, int l, int iters, bool from_zero, const i
         XT* dst = (it & 1) ? (XT*)v.x : (XT*)v.buf;
         const bool zero = from_zero && it == 0;
         const bool with_dot = dot && pairs && it == iters - 1 && M.nb <= dot->capacity;
-        if (FINE) kprof_begin("jacobi");
+        const char* tag = zero ? "jacobi_zero" : "jacobi";
+        if (FINE) kprof_begin(tag);
         if (pairs) {
             const March Mf = FINE ? flip_march(M) : M;
             if (zero) k_jacobi3<FINE, 1, 0, XT, WT><<<M.nb, MTPB, 0, g_stream>>>(L, Mf, src, dst, (const XT*)v.b, om,
@@ -1284,7 +1587,7 @@ void jacobi_sweeps(flux_multigrid* mg, int l, int iters, bool from_zero, const i
             if (zero) k_jacobi<FINE, 1, XT, WT><<<g, MTPB, 0, g_stream>>>(L, src, dst, (const XT*)v.b, om, stop);
             else k_jacobi<FINE, 0, XT, WT><<<g, MTPB, 0, g_stream>>>(L, src, dst, (const XT*)v.b, om, stop);
         }
-        if (FINE) kprof_end("jacobi");
+        if (FINE) kprof_end(tag);
         if (with_dot) dot->blocks = M.nb;
     }
 }
@@ -1398,10 +1701,121 @@ void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const
     }
 }
 
+template <typename ST>
+int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot);
+
+// The finest level of the mixed form: r (FP32) is scaled into b~ by the
+// first pre-smoothing launch, the level smooths, computes its residual and
+// corrects in ST, and the last post-smoothing launch writes z in FP32 (and
+// the r.z partials).  Red-black runs its colour pairs in place on x~;
+// Jacobi's sweep counts are even, so the ping-pong between x~ and buf~ ends
+// in x~ before the residual and before the last sweep.
+template <typename ST>
+int vcycle_fine16(flux_multigrid* mg, const int* stop, const double* rr, DotOut* dot) {
+    MGLevel& v = mg->lv[0];
+    MGLevel& c = mg->lv[1];
+    LevelDevT<ST> L = level_dev<ST>(mg, v);
+    const float om = smooth_omega(mg);
+    const float xs = (float)v.xscale;
+    const double* srr = (mg->o.storage == FLUX_STORE_F16) ? rr : nullptr;
+    const float* r = (const float*)v.b;
+    float* z = (float*)v.x;
+    const March M = march_shape(v.n[0] / 2, v.n[1], v.n[2], MTPB, g_march_blocks);
+    ST* x16 = (ST*)v.x16;
+    ST* b16 = (ST*)v.b16;
+    ST* alt = (ST*)v.buf16;
+    const int nu = mg->o.nu_down;
+    const bool rbgs = mg->o.smoother == FLUX_SMOOTH_RBGS;
+    // red-black: colour pairs in place, post-smoothing reversed
+    for (int it = 0; rbgs && it < nu; ++it) {
+        kprof_begin(it == 0 ? "relax_io" : "relax");
+        if (it == 0)
+            k_relax3m<1, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, nullptr, b16, r, nullptr, srr, xs,
+                                                             om, 0, stop, nullptr);
+        else
+            k_relax3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, b16, nullptr, nullptr, nullptr,
+                                                             srr, xs, om, 0, stop, nullptr);
+        kprof_end(it == 0 ? "relax_io" : "relax");
+        kprof_begin("relax");
+        k_relax3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, b16, nullptr, nullptr, nullptr, srr,
+                                                         xs, om, 1, stop, nullptr);
+        kprof_end("relax");
+    }
+    for (int it = 0; !rbgs && it < nu; ++it) {
+        const ST* src = (it & 1) ? alt : x16;
+        ST* dst = (it & 1) ? x16 : alt;
+        // "jacobi" times the steady-state sweeps, "jacobi_io" the first and last
+        const char* tag = it == 0 ? "jacobi_io" : "jacobi";
+        kprof_begin(tag);
+        if (it == 0)
+            k_jacobi3m<1, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), nullptr, dst, nullptr, b16, r, nullptr,
+                                                              srr, xs, om, stop, nullptr);
+        else
+            k_jacobi3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), src, dst, b16, nullptr, nullptr,
+                                                              nullptr, srr, xs, om, stop, nullptr);
+        kprof_end(tag);
+    }
+    kprof_begin("residual");
+    const March Mr = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+    k_residual3<1, ST, ST, ST><<<Mr.nb, MTPB, 0, g_stream>>>(L, flip_march(Mr), x16, b16, alt, stop);
+    kprof_end("residual");
+    kprof_begin("restrict");
+    restrict_level<1, ST, ST>(mg, v, c, alt, nullptr, stop);
+    kprof_end("restrict");
+    FLUX_TRY(vcycle_t<ST>(mg, 1, stop, rr, nullptr));
+    // the correction adds into x~ (correct_level works on v.x)
+    void* zsave = v.x;
+    v.x = x16;
+    kprof_begin("correct");
+    correct_level<1, ST, ST>(mg, v, c, nullptr, stop);
+    kprof_end("correct");
+    v.x = zsave;
+    const int nf = mg->o.n_final;
+    for (int it = 0; rbgs && it < nf; ++it) {
+        kprof_begin("relax");
+        k_relax3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, b16, nullptr, nullptr, nullptr, srr,
+                                                         xs, om, 1, stop, nullptr);
+        kprof_end("relax");
+        const bool last = it + 1 == nf;
+        kprof_begin(last ? "relax_io" : "relax");
+        if (!last)
+            k_relax3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, b16, nullptr, nullptr, nullptr,
+                                                             srr, xs, om, 0, stop, nullptr);
+        else if (dot && M.nb <= dot->capacity) {
+            k_relax3m<2, 1, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, nullptr, nullptr, r, z, srr, xs,
+                                                             om, 0, stop, dot->partials);
+            dot->blocks = M.nb;
+        } else
+            k_relax3m<2, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), x16, nullptr, nullptr, r, z, srr, xs,
+                                                             om, 0, stop, nullptr);
+        kprof_end(last ? "relax_io" : "relax");
+    }
+    for (int it = 0; !rbgs && it < nf; ++it) {
+        const ST* src = (it & 1) ? alt : x16;
+        ST* dst = (it & 1) ? x16 : alt;
+        const char* tag = it + 1 < nf ? "jacobi" : "jacobi_io";
+        kprof_begin(tag);
+        if (it + 1 < nf) {
+            k_jacobi3m<0, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), src, dst, b16, nullptr, nullptr,
+                                                              nullptr, srr, xs, om, stop, nullptr);
+        } else if (dot && M.nb <= dot->capacity) {
+            k_jacobi3m<2, 1, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), src, nullptr, nullptr, nullptr, r, z,
+                                                              srr, xs, om, stop, dot->partials);
+            dot->blocks = M.nb;
+        } else {
+            k_jacobi3m<2, 0, ST><<<M.nb, MTPB, 0, g_stream>>>(L, flip_march(M), src, nullptr, nullptr, nullptr, r, z,
+                                                              srr, xs, om, stop, nullptr);
+        }
+        kprof_end(tag);
+    }
+    return FLUX_OK;
+}
+
 // vCycle (multigrid.js:1609-1642).  Level 0 has FP32 vectors and weights of
 // type ST; coarse levels store everything in ST.
 template <typename ST>
 int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot) {
+    if (l == 0 && mg->fine16) return vcycle_fine16<ST>(mg, stop, rr, dot);
     MGLevel& v = mg->lv[l];
     int nl = (int)mg->lv.size();
     const bool fine = (l == 0);
@@ -1500,6 +1914,7 @@ int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, 
 }
 
 int64_t mg_cells(const flux_multigrid* mg) { return cells_of(mg->lv[0].n); }
+int mg_storage(const flux_multigrid* mg) { return mg ? mg->o.storage : FLUX_STORE_F32; }
 
 }  // namespace flux
 
@@ -1525,6 +1940,9 @@ int flux_multigrid_destroy(flux_multigrid* mg) {
         dfree(v.buf);
         dfree(v.alt);
         dfree(v.nb);
+        dfree(v.x16);
+        dfree(v.b16);
+        dfree(v.buf16);
         if (v.own) {
             dfree(v.x);
             dfree(v.b);
@@ -1586,6 +2004,9 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     mg->tail = !(tenv && tenv[0] == '0');
     const char* penv = getenv("FLUX_CORRECT_PAIRS");
     mg->pairs = !(penv && penv[0] == '0');
+    const char* f16env = getenv("FLUX_FINE_STORAGE");
+    mg->fine16 = o.storage != FLUX_STORE_F32 && ndim == 3 && o.levels > 1 && (n0[0] % 2) == 0 && o.nu_down >= 1 &&
+                 o.n_final >= 1 && (f16env && f16env[0] == '1');
     mg->lv.resize(o.levels);
     const size_t es = store_size(o.storage);
     bool ok = true;
@@ -1597,8 +2018,14 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
         }
         // FP16 coarse levels hold x / h_l^2 (operator scaled by h_l^2) so the
         // solution, of size h^2 |b|, stays clear of FP16's subnormal range
-        v.xscale = (o.storage == FLUX_STORE_F16 && l > 0) ? v.h[0] * v.h[0] : 1.0;
+        v.xscale = (o.storage == FLUX_STORE_F16 && (l > 0 || mg->fine16)) ? v.h[0] * v.h[0] : 1.0;
         int64_t cells = cells_of(v.n);
+        if (l == 0 && mg->fine16) {
+            v.x16 = dmalloc(cells * es);
+            v.b16 = dmalloc(cells * es);
+            v.buf16 = dmalloc(cells * es);
+            ok = ok && v.x16 && v.b16 && v.buf16;
+        }
         size_t vs = (l == 0) ? 4 : es;  // finest vectors stay FP32
         v.buf = dmalloc(cells * vs);
         ok = ok && v.buf;

@@ -359,6 +359,19 @@ int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, doub
     return flux_pcg_scaled_start(p->pcg, guess, alpha);
 }
 
+int flux_pressure3_set_coarse_space(flux_pressure3* p, int free_indicator, int nfields,
+                                    const flux_field* const* fields) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p, "coarseSpace: NULL solver");
+    return flux_pcg_set_coarse_space(p->pcg, free_indicator, nfields, fields);
+}
+
+int flux_pressure3_set_flexible(flux_pressure3* p, int flexible) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(p, "flexible: NULL solver");
+    return flux_pcg_set_flexible(p->pcg, flexible);
+}
+
 int flux_pressure3_block_guess(flux_pressure3* p, const flux_field* guess, int block, int* coarse_iterations) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(p && guess, "blockGuess: NULL argument");

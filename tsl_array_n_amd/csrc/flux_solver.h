@@ -16,6 +16,7 @@ struct DotOut {
 };
 int mg_apply_raw(flux_multigrid* mg, const float* r, float* z, const int* stop, const double* rr, DotOut* dot);
 int64_t mg_cells(const flux_multigrid* mg);
+int mg_storage(const flux_multigrid* mg);
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st);
 // P2G with the reference's fixed-point atomics on the stream; with `conc`
 // each particle's weights are scaled by its density over the heavier one

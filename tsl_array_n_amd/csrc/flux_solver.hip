@@ -97,8 +97,45 @@ __global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, i
 struct Scal {
     double pAp, rr, rz, rz_old, bb, base, alpha, beta, best, thr, tol, rr0;
     int stop, iter, stop_iter, relative;
-    int xsel, psel, pad0, pad1;
+    int xsel, psel, flexible, cs_m;
+    double rzo;  // r_{k+1} . z_k of the flexible beta
+    // coarse-space correction (cs_m vectors): Einv = (Phi^T A Phi)^-1,
+    // c = Phi^T r of the current residual, a = Einv c the coarse amplitudes
+    double cs_einv[16], cs_c[4], cs_a[4];
 };
+
+// The coarse space of the additive correction: column 0 the free-cell
+// indicator when `free0`, then up to three caller fields, each restricted to
+// the free cells.
+static constexpr int CS_MAX = 4;
+struct CoarseSpace {
+    const float* f[3] = {nullptr, nullptr, nullptr};
+    int nf = 0;
+    int free0 = 0;
+    __host__ __device__ int m() const { return free0 + nf; }
+};
+
+// phi_q at cell c (0 on Dirichlet cells)
+__device__ __forceinline__ float cs_phi(const CoarseSpace& cs, int q, int c, bool dir) {
+    if (dir) return 0.0f;
+    if (cs.free0) {
+        if (q == 0) return 1.0f;
+        return cs.f[q - 1][c];
+    }
+    return cs.f[q][c];
+}
+
+// a = Einv c, and the flexible beta's r_{k+1}.(z_k + Phi a_k) = rzo + a_k . c
+__device__ __forceinline__ void cs_solve(Scal* S, bool fold_rzo) {
+    const int m = S->cs_m;
+    if (fold_rzo)
+        for (int q = 0; q < m; ++q) S->rzo += S->cs_a[q] * S->cs_c[q];
+    for (int q = 0; q < m; ++q) {
+        double v = 0;
+        for (int t = 0; t < m; ++t) v += S->cs_einv[q * CS_MAX + t] * S->cs_c[t];
+        S->cs_a[q] = v;
+    }
+}
 
 __device__ __forceinline__ bool degenerate(double v) { return !(fabs(v) <= 1.7976931348623157e308) || v == 0.0; }
 
@@ -261,8 +298,10 @@ __global__ void k_pcg_dot_rz(const float* __restrict__ r, const float* __restric
 // (betaKernel, linalg.js:1198-1226).
 __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* partials, unsigned nb, int first) {
     if (S->stop) return;
-    const double tot = sum_partials(partials, nb);
+    double tot = sum_partials(partials, nb);
     if (threadIdx.x != 0) return;
+    // r.(z + Phi a) = r.z + a.(Phi^T r)
+    for (int q = 0; q < S->cs_m; ++q) tot += S->cs_a[q] * S->cs_c[q];
     S->rz = tot;
     if (first) {
         S->rz_old = tot;
@@ -278,7 +317,10 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* parti
         return;
     }
     bool flip = (rzOld > 0) != (tot > 0);
-    double braw = tot / rzOld;
+    // flexible (Polak-Ribiere) beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k when
+    // the preconditioner is inexact (mixed-precision storage); otherwise the
+    // reference's r_{k+1}.z_{k+1} / r_k.z_k (linalg.js:1198-1226)
+    double braw = (S->flexible ? tot - S->rzo : tot) / rzOld;
     bool restart = flip || !(fabs(braw) <= 10.0);  // MAX_BETA_MAGNITUDE
     S->beta = restart ? 0.0 : braw;
     S->rz_old = tot;
@@ -288,11 +330,14 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* parti
 // itself is not stored: the update recomputes r = b - A x from x.  Column-march shape: one cell per thread and plane.
 __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const float* __restrict__ z,
                                                     float* __restrict__ pa, float* __restrict__ pb, const Scal* S,
-                                                    double* partials) {
+                                                    CoarseSpace cs, double* partials) {
     if (S->stop) return;
     const float* __restrict__ pold = S->psel ? pb : pa;
     float* __restrict__ pnew = S->psel ? pa : pb;
     const float beta = (float)S->beta;
+    const int csm = cs.m();
+    float ca[CS_MAX] = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < csm; ++q) ca[q] = (float)S->cs_a[q];
     double v = 0;
     int t, k0, k1;
     if (march_begin(M, t, k0, k1)) {
@@ -302,7 +347,16 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
         const int i = t - j * n0;
         const uint8_t* __restrict__ nbits = L.nb;
         const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
-        auto pval = [&](int q) { return z[q] + pold[q] * beta; };
+        // the preconditioned residual with the coarse correction: z + Phi a
+        auto zval = [&](int q) {
+            float v = z[q];
+            if (csm) {
+                const bool dir = nbits && ((nbits[q] >> 6) & 1u);
+                for (int t = 0; t < csm; ++t) v = v + ca[t] * cs_phi(cs, t, q, dir);
+            }
+            return v;
+        };
+        auto pval = [&](int q) { return zval(q) + pold[q] * beta; };
         // the z-neighbours are the previous and the next plane's centres
         float pnext = pval(i + n0 * (j + n1 * k0));
         float pprev = 0.0f;
@@ -353,16 +407,23 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_pap(Scal* S, const double* part
 // Iteration step 2: x += alpha p (FP64 accumulator, ping-pong buffers);
 // r = b - A x in FP64 arithmetic (the true residual, recomputed every
 // iteration); partials of rr.
+// With `zold` (flexible beta) the partials of r_{k+1}.z_k follow at
+// partials[gridDim.x ...].
+// With a coarse space, the partials of Phi^T r follow (one array of
+// gridDim.x per column, after the rr and r.z_k arrays).
 __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double* __restrict__ xa,
                                                      double* __restrict__ xb, const float* __restrict__ pa,
                                                      const float* __restrict__ pb, const float* __restrict__ b,
-                                                     float* __restrict__ r, const Scal* S, double* partials) {
+                                                     float* __restrict__ r, const float* __restrict__ zold,
+                                                     const Scal* S, CoarseSpace cs, double* partials) {
     if (S->stop) return;
     const double* __restrict__ xold = S->xsel ? xb : xa;
     double* __restrict__ xnew = S->xsel ? xa : xb;
     const float* __restrict__ p = S->psel ? pb : pa;  // the p written by k_pcg_apply3
     const double alpha = S->alpha;
-    double v = 0;
+    double v = 0, vz = 0;
+    double vc[CS_MAX] = {0, 0, 0, 0};
+    const int csm = cs.m();
     int t, k0, k1;
     if (march_begin(M, t, k0, k1)) {
         const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
@@ -396,19 +457,30 @@ __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double
             const double ax = (bits & 64u) ? -xc : sum;
             xnew[c] = xc;
             const double rv = (double)b[c] - ax;
-            r[c] = (float)rv;
+            const float rf = (float)rv;
+            r[c] = rf;
             v += rv * rv;
+            if (zold) vz += (double)rf * (double)zold[c];
+            for (int q = 0; q < csm; ++q) vc[q] += (double)rf * (double)cs_phi(cs, q, c, (bits & 64u) != 0);
             xprev = xc;
         }
     }
     store_block_partial(v, partials);
+    if (zold) store_block_partial(vz, partials + gridDim.x);
+    for (int q = 0; q < csm; ++q) store_block_partial(vc[q], partials + (size_t)(2 + q) * gridDim.x);
 }
 
 // rr; convergence and residual-growth tests.
 __global__ void __launch_bounds__(FIN_TPB) k_fin_rr(Scal* S, const double* partials, unsigned nb) {
     if (S->stop) return;
     const double rr = sum_partials(partials, nb);
+    const double rzo = S->flexible ? sum_partials(partials + nb, nb) : 0.0;
+    double c[CS_MAX];
+    for (int q = 0; q < S->cs_m; ++q) c[q] = sum_partials(partials + (size_t)(2 + q) * nb, nb);
     if (threadIdx.x != 0) return;
+    S->rzo = rzo;
+    for (int q = 0; q < S->cs_m; ++q) S->cs_c[q] = c[q];
+    cs_solve(S, S->flexible != 0);
     S->rr = rr;
     S->xsel ^= 1;
     double res = sqrt(fabs(rr));
@@ -420,6 +492,47 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rr(Scal* S, const double* parti
     if (!(res == res)) return;
     if (S->best < 0 || fabs(rr) < S->best) S->best = fabs(rr);
     else if (S->best > 0 && fabs(rr) > S->best * 100.0) S->stop = 6;  // MAX_RESIDUAL_GROWTH
+}
+
+// Coarse-space setup: partials of E_qt = phi_q . A phi_t (FP64, q, t < m)
+// at partials[(q * m + t) * gridDim.x + blk].
+__global__ void __launch_bounds__(TPB) k_cs_gram(LevelDev L, CoarseSpace cs, double* __restrict__ partials) {
+    const int m = cs.m();
+    double e[CS_MAX * CS_MAX];
+    for (int q = 0; q < m * m; ++q) e[q] = 0.0;
+    const int n = L.n0 * L.n1 * L.n2;
+    const uint8_t* __restrict__ msk = L.mask;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        if (msk && msk[c]) continue;
+        int i, j, k;
+        cell_coords(L, c, i, j, k);
+        for (int t = 0; t < m; ++t) {
+            const double at = stencil_row<double>(
+                L, i, j, k, c, [&](int qq) { return (double)cs_phi(cs, t, qq, msk && msk[qq]); }, (double*)nullptr);
+            for (int q = 0; q < m; ++q) e[q * m + t] += (double)cs_phi(cs, q, c, false) * at;
+        }
+    }
+    for (int q = 0; q < m * m; ++q) store_block_partial(e[q], partials + (size_t)q * gridDim.x);
+}
+
+// c0 = Phi^T r0 partials (the setup residual), then a0 in k_fin_cs
+__global__ void k_cs_dots(const uint8_t* __restrict__ msk, CoarseSpace cs, const float* __restrict__ r, int n,
+                          double* __restrict__ partials) {
+    const int m = cs.m();
+    double vc[CS_MAX] = {0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const bool dir = msk && msk[c];
+        for (int q = 0; q < m; ++q) vc[q] += (double)r[c] * (double)cs_phi(cs, q, c, dir);
+    }
+    for (int q = 0; q < m; ++q) store_block_partial(vc[q], partials + (size_t)q * gridDim.x);
+}
+
+__global__ void __launch_bounds__(FIN_TPB) k_fin_cs(Scal* S, const double* partials, unsigned nb) {
+    double c[CS_MAX];
+    for (int q = 0; q < S->cs_m; ++q) c[q] = sum_partials(partials + (size_t)q * nb, nb);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < S->cs_m; ++q) S->cs_c[q] = c[q];
+    cs_solve(S, false);
 }
 
 __global__ void k_pcg_copy_out(const double* __restrict__ xa, const double* __restrict__ xb, float* __restrict__ x32,
@@ -584,6 +697,8 @@ struct flux_pcg {
     unsigned n_partials = 0;
     March M;  // column-march grid of the iteration kernels
     int last_iterations = 4;
+    int flexible = 0;  // Polak-Ribiere beta
+    CoarseSpace cs;    // additive coarse-space correction of the preconditioner
 };
 
 namespace flux {
@@ -608,12 +723,14 @@ static int pcg_precondition(flux_pcg* s, DotOut* dot) {
 static int pcg_iteration(flux_pcg* s) {
     unsigned g = red_blocks(s->n);
     kprof_begin("apply");
-    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->z, s->pa, s->pb, s->S, s->partials);
+    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->z, s->pa, s->pb, s->S, s->cs,
+                                                s->partials);
     kprof_end("apply");
     k_fin_pap<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     kprof_begin("update");
-    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d,
-                                                 s->r, s->S, s->partials);
+    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->xa, s->xb, s->pa, s->pb,
+                                                 (const float*)s->b->d, s->r, s->flexible ? s->z : nullptr, s->S,
+                                                 s->cs, s->partials);
     kprof_end("update");
     k_fin_rr<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     DotOut dot;
@@ -631,6 +748,78 @@ static int pcg_iteration(flux_pcg* s) {
     return FLUX_OK;
 }
 
+// (-E)^+ for the symmetric m x m matrix -E (m <= 4) by cyclic Jacobi
+// rotations: eigenvalues below 1e-12 of the largest are dropped, so a
+// degenerate or repeated column contributes nothing.  Returns -(-E)^+.
+static void coarse_inverse(const double* E, int m, double* out) {
+    double a[CS_MAX][CS_MAX], v[CS_MAX][CS_MAX];
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            a[i][j] = -0.5 * (E[i * m + j] + E[j * m + i]);
+            v[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0;
+        for (int i = 0; i < m; ++i)
+            for (int j = i + 1; j < m; ++j) off += a[i][j] * a[i][j];
+        if (off < 1e-30) break;
+        for (int pq = 0; pq < m; ++pq)
+            for (int q = pq + 1; q < m; ++q) {
+                if (a[pq][q] == 0.0) continue;
+                const double th = (a[q][q] - a[pq][pq]) / (2.0 * a[pq][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double akp = a[k][pq], akq = a[k][q];
+                    a[k][pq] = c * akp - sn * akq;
+                    a[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double apk = a[pq][k], aqk = a[q][k];
+                    a[pq][k] = c * apk - sn * aqk;
+                    a[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double vkp = v[k][pq], vkq = v[k][q];
+                    v[k][pq] = c * vkp - sn * vkq;
+                    v[k][q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    double lmax = 0;
+    for (int i = 0; i < m; ++i) lmax = std::max(lmax, a[i][i]);
+    for (int i = 0; i < CS_MAX * CS_MAX; ++i) out[i] = 0.0;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double sum = 0;
+            for (int k = 0; k < m; ++k)
+                if (a[k][k] > 1e-12 * lmax && std::isfinite(a[k][k])) sum += v[i][k] * v[j][k] / a[k][k];
+            out[i * CS_MAX + j] = -sum;
+        }
+}
+
+// The Gram matrix E = Phi^T A Phi of the coarse space, reduced on the host
+// in a fixed order, and its inverse into the scalar block.
+int coarse_space_setup(flux_pcg* s, Scal* Sh) {
+    const int m = s->cs.m();
+    Sh->cs_m = m;
+    if (!m) return FLUX_OK;
+    const unsigned g = red_blocks(s->n);
+    k_cs_gram<<<g, TPB, 0, g_stream>>>(s->op->L, s->cs, s->partials);
+    FLUX_LAUNCH_CHECK();
+    std::vector<double> h((size_t)m * m * g);
+    FLUX_HIP(hipMemcpyAsync(h.data(), s->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, g_stream));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    double E[CS_MAX * CS_MAX];
+    for (int q = 0; q < m * m; ++q) {
+        double acc = 0;
+        for (unsigned b = 0; b < g; ++b) acc += h[(size_t)q * g + b];
+        E[q] = acc;
+    }
+    coarse_inverse(E, m, Sh->cs_einv);
+    return FLUX_OK;
+}
+
 int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_state* st) {
     unsigned g = red_blocks(s->n);
     unsigned gfull = blocks_for(s->n, TPB);
@@ -640,6 +829,8 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     memset(s->Sh, 0, sizeof(Scal));
     s->Sh->tol = tol;
     s->Sh->relative = relative ? 1 : 0;
+    s->Sh->flexible = s->flexible;
+    FLUX_TRY(coarse_space_setup(s, s->Sh));
     FLUX_HIP(hipMemcpyAsync(s->S, s->Sh, sizeof(Scal), hipMemcpyHostToDevice, g_stream));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     // setup (runGpuResidentSetup, linalg.js:1613-1637)
@@ -648,6 +839,10 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     k_fin_setup<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
     k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->partials);
     k_fin_bb<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
+    if (s->cs.m()) {
+        k_cs_dots<<<g, TPB, 0, g_stream>>>(s->op->L.mask, s->cs, s->r, (int)s->n, s->partials);
+        k_fin_cs<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
+    }
     FLUX_TRY(pcg_precondition(s, nullptr));
     k_pcg_dot_rz<<<g, TPB, 0, g_stream>>>(s->r, s->z, (int)s->n, s->S, s->partials);
     k_fin_rz<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g, 1);
@@ -716,7 +911,16 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     // partials of every reduction: the capped grid-stride grids, the column-march
     // grids of the iteration kernels and of the V-cycle's fused r.z (a march
     // grid over cell pairs; larger grids fall back to the separate dot kernel)
-    s->n_partials = (unsigned)std::max<int64_t>(std::max<int64_t>(RED_MAX_BLOCKS, s->M.nb), 4096);
+    // (2 + CS_MAX) arrays of the update's partials; CS_MAX^2 of the coarse
+    // space's Gram matrix
+    s->n_partials = (unsigned)std::max<int64_t>(
+        std::max<int64_t>((int64_t)CS_MAX * CS_MAX * RED_MAX_BLOCKS, (2 + CS_MAX) * (int64_t)s->M.nb), 4096);
+    // the reference's beta unless asked otherwise (FLUX_FLEXIBLE_PCG=1 or
+    // flux_pcg_set_flexible): measured on the 64^3 dam break with FP16
+    // finest-level storage, the flexible beta took more iterations (63) than
+    // the reference's (51)
+    s->flexible = 0;
+    if (const char* e = getenv("FLUX_FLEXIBLE_PCG")) s->flexible = e[0] == '1' ? 1 : 0;
     s->partials = (double*)dmalloc(sizeof(double) * s->n_partials);
     void* hp = nullptr;
     if (hipHostMalloc(&hp, sizeof(Scal), hipHostMallocDefault) != hipSuccess) hp = nullptr;
@@ -849,6 +1053,27 @@ int flux_pcg_block_start(flux_pcg* s, const flux_field* g, int block, int* coars
     if (e == hipSuccess) e = e2;
     if (e != hipSuccess) return set_error(FLUX_ERR_HIP, "blockStart: %s", hipGetErrorString(e));
     if (coarse_iterations) *coarse_iterations = it;
+    return FLUX_OK;
+}
+
+int flux_pcg_set_coarse_space(flux_pcg* s, int free_indicator, int nfields, const flux_field* const* fields) {
+    FLUX_REQUIRE(s, "conjugateGradient: NULL solver");
+    FLUX_REQUIRE(nfields >= 0 && nfields <= 3 && (nfields == 0 || fields),
+                 "coarseSpace: 0 to 3 fields, got %d", nfields);
+    CoarseSpace cs;
+    cs.free0 = free_indicator ? 1 : 0;
+    cs.nf = nfields;
+    for (int q = 0; q < nfields; ++q) {
+        FLUX_TRY(check_field(fields[q], FLUX_F32, s->n, "coarseSpace: vector"));
+        cs.f[q] = (const float*)fields[q]->d;
+    }
+    s->cs = cs;
+    return FLUX_OK;
+}
+
+int flux_pcg_set_flexible(flux_pcg* s, int flexible) {
+    FLUX_REQUIRE(s, "conjugateGradient: NULL solver");
+    s->flexible = flexible ? 1 : 0;
     return FLUX_OK;
 }
 

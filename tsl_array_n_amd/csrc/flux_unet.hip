@@ -652,6 +652,47 @@ __global__ void __launch_bounds__(256) k_conv3_mfma_halo(MfmaIn in, const uint16
 
 using namespace flux;
 
+// Energy-norm loss: the guess g = y on the free cells
+__global__ void k_mask_out(const float* __restrict__ y, const uint8_t* __restrict__ mask, float* __restrict__ g,
+                           int64_t n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) g[q] = (mask && mask[q]) ? 0.0f : y[q];
+}
+
+// partials of a.b (part[0..gridDim)) and c.d (part[gridDim..2 gridDim))
+__global__ void k_dot2(const float* __restrict__ a, const float* __restrict__ b, const float* __restrict__ c,
+                       const float* __restrict__ d, int64_t n, double* __restrict__ part) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (q < n) {
+        s0 = (double)a[q] * b[q];
+        s1 = (double)c[q] * d[q];
+    }
+    __shared__ double red[2][UTPB];
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    __syncthreads();
+    for (int st = UTPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + st];
+            red[1][threadIdx.x] += red[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = red[0][0];
+        part[gridDim.x + blockIdx.x] = red[1][0];
+    }
+}
+
+// dL/dy = c1 r + c2 (A g) on the free cells, 0 on the Dirichlet cells
+__global__ void k_energy_grad(const float* __restrict__ r, const float* __restrict__ ag,
+                              const uint8_t* __restrict__ mask, float c1, float c2, float* __restrict__ gy,
+                              int64_t n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) gy[q] = (mask && mask[q]) ? 0.0f : c1 * r[q] + c2 * ag[q];
+}
+
 struct flux_unet {
     int n[3];  // nx, ny, nz of the finest level
     int64_t nparams = 0;
@@ -1157,6 +1198,77 @@ int flux_unet_train_step(flux_unet* u, const flux_field* input, const flux_field
     FLUX_HIP(hipMemcpyAsync(&l, u->scal, 8, hipMemcpyDeviceToHost, g_stream));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     if (loss) *loss = l / (double)nv;
+    return FLUX_OK;
+}
+
+// forward + energy-norm loss + backward (+ Adam when lr > 0).  With the
+// guess g = y on the free cells, A the system's operator (negative definite
+// on the free cells), r the residual of the start, B = -A and e* = x*.(-r)
+// the energy of the exact correction x*, the loss is the relative energy
+// error left after the optimal step along g:
+//   L = 1 - (g.r)^2 / ((g.B g) e*),   dL/dg = -2 (g.r) r / (q e*) - 2 (g.r)^2 A g / (q^2 e*)
+// with q = g.B g = -g.A g (so a larger step along g never helps: the loss
+// is scale-free, as the solver rescales g).  *loss = L.
+int flux_unet_train_step_energy(flux_unet* u, const flux_field* input, flux_operator* op, const flux_field* r,
+                                const flux_field* mask, double e_star, float lr, float beta1, float beta2, float eps,
+                                double* loss) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(u && u->grads && op && r, "unet: energy training needs a training network, an operator and r");
+    FLUX_REQUIRE(e_star > 0.0, "unet: e* must be positive, got %g", e_star);
+    FLUX_TRY(check_field(input, FLUX_F32, 3 * u->nvox[0], "unet: input"));
+    const int64_t nv = u->nvox[0];
+    FLUX_TRY(check_field(r, FLUX_F32, nv, "unet: residual"));
+    if (mask) FLUX_TRY(check_field(mask, FLUX_U8, nv, "unet: mask"));
+    const uint8_t* m = mask ? (const uint8_t*)mask->d : nullptr;
+    const float* x = (const float*)input->d;
+    FLUX_TRY(forward(u, x));
+    const unsigned nb = blocks_for(nv, UTPB);
+    // g into gy[L_OUT] (scratch until the gradient overwrites it), A g into
+    // gin_up[L_D0A] (48 channels at level 0; backward writes it only after
+    // the gradient below has consumed A g)
+    float* g = u->gy[L_OUT];
+    float* ag = u->gin_up[L_D0A];
+    k_mask_out<<<nb, UTPB, 0, g_stream>>>(u->y[L_OUT], m, g, nv);
+    flux_field gf, af;
+    gf.d = g;
+    af.d = ag;
+    for (flux_field* f : {&gf, &af}) {
+        f->dtype = FLUX_F32;
+        f->ndim = 1;
+        f->shape[0] = nv;
+        f->count = nv;
+        f->bytes = (size_t)nv * 4;
+    }
+    FLUX_TRY(flux_operator_apply(op, &gf, &af));
+    k_dot2<<<nb, UTPB, 0, g_stream>>>(g, (const float*)r->d, g, ag, nv, u->part);
+    std::vector<double> h(2 * (size_t)nb);
+    FLUX_HIP(hipMemcpyAsync(h.data(), u->part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, g_stream));
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    double gr = 0, gag = 0;
+    for (unsigned q = 0; q < nb; ++q) {
+        gr += h[q];
+        gag += h[nb + q];
+    }
+    const double qv = -gag;
+    double l = 1.0;
+    float c1 = 0.0f, c2 = 0.0f;
+    if (qv > 0 && std::isfinite(qv) && std::isfinite(gr)) {
+        l = 1.0 - gr * gr / (qv * e_star);
+        c1 = (float)(-2.0 * gr / (qv * e_star));
+        c2 = (float)(-2.0 * gr * gr / (qv * qv * e_star));
+    }
+    k_energy_grad<<<nb, UTPB, 0, g_stream>>>((const float*)r->d, ag, m, c1, c2, u->gy[L_OUT], nv);
+    FLUX_TRY(backward(u, x));
+    if (lr > 0.0f) {
+        u->steps += 1;
+        float b1c = 1.0f - powf(beta1, (float)u->steps), b2c = 1.0f - powf(beta2, (float)u->steps);
+        k_adam<<<blocks_for(u->nparams, UTPB), UTPB, 0, g_stream>>>(u->params, u->grads, u->adam_m, u->adam_v,
+                                                                     u->nparams, lr, beta1, beta2, eps, b1c, b2c);
+        u->version += 1;
+    }
+    FLUX_LAUNCH_CHECK();
+    FLUX_HIP(hipStreamSynchronize(g_stream));
+    if (loss) *loss = l;
     return FLUX_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <node_api.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
 
 #include "../../include/fluxflow_hip.h"
@@ -32,7 +33,7 @@
         }                                                     \
     } while (0)
 
-enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE, K_S2, K_ADV3 };
+enum { K_FIELD = 1, K_FIELD_VIEW, K_OP, K_MG, K_PCG, K_P3, K_EXTRAP, K_BND, K_FLIP, K_SMOKE, K_S2, K_ADV3, K_DOT, K_CG };
 
 typedef struct {
     int kind;
@@ -56,6 +57,8 @@ static void fin_handle(napi_env env, void* data, void* hint) {
         case K_SMOKE: flux_smoke3_destroy((flux_smoke3*)h->ptr); break;
         case K_S2: flux_solver2_destroy((flux_solver2*)h->ptr); break;
         case K_ADV3: flux_advection3_destroy((flux_advection3*)h->ptr); break;
+        case K_DOT: flux_dot_destroy((flux_dot*)h->ptr); break;
+        case K_CG: flux_cg_destroy((flux_cg*)h->ptr); break;
         default: break;
     }
     free(h);
@@ -407,6 +410,8 @@ static napi_value js_pcg_create(napi_env env, napi_callback_info info) {
     return wrap(env, K_PCG, s);
 }
 
+static napi_value pcg_state_object(napi_env env, const flux_pcg_state* st);
+
 static napi_value js_pcg_solve(napi_env env, napi_callback_info info) {
     napi_value a[4];
     args_n(env, info, 4, a);
@@ -415,6 +420,11 @@ static napi_value js_pcg_solve(napi_env env, napi_callback_info info) {
     napi_get_value_bool(env, a[3], &rel);
     CHECK_FLUX(flux_pcg_solve((flux_pcg*)unwrap(env, a[0], K_PCG, K_PCG), get_double(env, a[1], 1e-6),
                               get_int(env, a[2], 100), rel ? 1 : 0, &st));
+    return pcg_state_object(env, &st);
+}
+
+static napi_value pcg_state_object(napi_env env, const flux_pcg_state* stp) {
+    const flux_pcg_state st = *stp;
     napi_value o, s, b;
     napi_create_object(env, &o);
     napi_set_named_property(env, o, "iterations", num(env, st.iterations));
@@ -1163,6 +1173,255 @@ static napi_value js_solver2_field(napi_env env, napi_callback_info info) {
     return wrap(env, K_FIELD_VIEW, f);
 }
 
+
+/* ------------------------------------------------------------ async work */
+/* The reference's solver calls are async (project() returns an async
+ * dispatch, solve() and onAdvanceTimeStep() are awaited,
+ * grid_pressure_solver3.js:502-556): here the blocking library call runs on
+ * a libuv worker thread (napi_async_work) and the returned Promise settles
+ * on the JS thread, so the event loop keeps running meanwhile.  Library
+ * calls must not interleave (one stream, linalg.js:1793-1797): one mutex
+ * serialises every worker job. */
+enum { JOB_PCG, JOB_P3, JOB_FLIP, JOB_SMOKE, JOB_S2, JOB_DOT, JOB_CG };
+typedef struct {
+    napi_async_work work;
+    napi_deferred deferred;
+    napi_ref keep; /* the JS values the job reads, alive until it settles */
+    int kind;
+    void* obj;
+    double tol;
+    int maxiter, rel, converged, iterations;
+    float dt;
+    const flux_field* in[3];
+    flux_field* out[3];
+    int rc;
+    char err[512];
+    flux_pcg_state st;
+    flux_pressure_diag dg;
+    double value;
+} job_t;
+
+static pthread_mutex_t g_job_mutex = PTHREAD_MUTEX_INITIALIZER;
+
+static void job_execute(napi_env env, void* data) {
+    (void)env;
+    job_t* j = (job_t*)data;
+    pthread_mutex_lock(&g_job_mutex);
+    j->rc = flux_thread_bind();
+    if (j->rc == FLUX_OK) {
+        switch (j->kind) {
+            case JOB_PCG: j->rc = flux_pcg_solve((flux_pcg*)j->obj, j->tol, j->maxiter, j->rel, &j->st); break;
+            case JOB_P3:
+                j->rc = flux_pressure3_project((flux_pressure3*)j->obj, j->in[0], j->in[1], j->in[2], j->out[0],
+                                               j->out[1], j->out[2], &j->dg);
+                break;
+            case JOB_FLIP: j->rc = flux_flip3_step((flux_flip3*)j->obj, &j->dg); break;
+            case JOB_SMOKE: j->rc = flux_smoke3_step((flux_smoke3*)j->obj, &j->dg); break;
+            case JOB_S2: j->rc = flux_solver2_step((flux_solver2*)j->obj, j->dt, &j->dg); break;
+            case JOB_DOT: j->rc = flux_dot_read((flux_dot*)j->obj, &j->value); break;
+            case JOB_CG: j->rc = flux_cg_solve((flux_cg*)j->obj, j->tol, j->maxiter, &j->converged, &j->iterations); break;
+            default: j->rc = FLUX_ERR_ARG; break;
+        }
+    }
+    if (j->rc != FLUX_OK) snprintf(j->err, sizeof(j->err), "%s", flux_last_error());
+    pthread_mutex_unlock(&g_job_mutex);
+}
+
+static napi_value pcg_state_object(napi_env env, const flux_pcg_state* st);
+
+static void job_complete(napi_env env, napi_status status, void* data) {
+    job_t* j = (job_t*)data;
+    if (status != napi_ok || j->rc != FLUX_OK) {
+        napi_value msg, err;
+        napi_create_string_utf8(env, status != napi_ok ? "fluxflow addon: async work failed" : j->err,
+                                NAPI_AUTO_LENGTH, &msg);
+        napi_create_error(env, NULL, msg, &err);
+        napi_reject_deferred(env, j->deferred, err);
+    } else {
+        napi_value v = NULL;
+        switch (j->kind) {
+            case JOB_PCG: v = pcg_state_object(env, &j->st); break;
+            case JOB_DOT: napi_create_double(env, j->value, &v); break;
+            case JOB_CG: {
+                napi_value b;
+                napi_create_object(env, &v);
+                napi_get_boolean(env, j->converged != 0, &b);
+                napi_set_named_property(env, v, "converged", b);
+                napi_set_named_property(env, v, "iterations", num(env, j->iterations));
+                break;
+            }
+            default: v = diag_object(env, &j->dg); break;
+        }
+        napi_resolve_deferred(env, j->deferred, v);
+    }
+    if (j->keep) napi_delete_reference(env, j->keep);
+    napi_delete_async_work(env, j->work);
+    free(j);
+}
+
+/* queues `j` and returns its Promise; `keep` (may be NULL) stays referenced
+ * until the job settles */
+static napi_value job_start(napi_env env, job_t* j, napi_value keep) {
+    napi_value promise, name;
+    if (napi_create_promise(env, &j->deferred, &promise) != napi_ok) {
+        free(j);
+        napi_throw_error(env, NULL, "fluxflow addon: napi_create_promise failed");
+        return NULL;
+    }
+    if (keep) napi_create_reference(env, keep, 1, &j->keep);
+    napi_create_string_utf8(env, "fluxflow", NAPI_AUTO_LENGTH, &name);
+    napi_create_async_work(env, NULL, name, job_execute, job_complete, j, &j->work);
+    napi_queue_async_work(env, j->work);
+    return promise;
+}
+
+static job_t* job_new(int kind, void* obj) {
+    job_t* j = (job_t*)calloc(1, sizeof(job_t));
+    j->kind = kind;
+    j->obj = obj;
+    return j;
+}
+
+static napi_value js_pcg_solve_async(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_pcg* s = (flux_pcg*)unwrap(env, a[0], K_PCG, K_PCG);
+    if (!s) {
+        napi_throw_type_error(env, NULL, "solve: not a solver");
+        return NULL;
+    }
+    job_t* j = job_new(JOB_PCG, s);
+    j->tol = get_double(env, a[1], 1e-6);
+    j->maxiter = get_int(env, a[2], 100);
+    bool rel = false;
+    napi_get_value_bool(env, a[3], &rel);
+    j->rel = rel ? 1 : 0;
+    return job_start(env, j, a[0]);
+}
+
+static napi_value js_pressure3_project_async(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    flux_pressure3* p = (flux_pressure3*)unwrap(env, a[0], K_P3, K_P3);
+    job_t* j = job_new(JOB_P3, p);
+    const flux_field* out[3] = {NULL, NULL, NULL};
+    if (!p || get_field_array(env, a[1], j->in) != 3 || get_field_array(env, a[2], out) != 3) {
+        free(j);
+        napi_throw_type_error(env, NULL, "project: expected (solver, [u,v,w], [u,v,w])");
+        return NULL;
+    }
+    for (int i = 0; i < 3; ++i) j->out[i] = (flux_field*)out[i];
+    napi_value keep;
+    napi_create_array_with_length(env, 3, &keep);
+    napi_set_element(env, keep, 0, a[0]);
+    napi_set_element(env, keep, 1, a[1]);
+    napi_set_element(env, keep, 2, a[2]);
+    return job_start(env, j, keep);
+}
+
+static napi_value step_async(napi_env env, napi_callback_info info, int kind, int hkind) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    void* h = unwrap(env, a[0], hkind, hkind);
+    if (!h) {
+        napi_throw_type_error(env, NULL, "onAdvanceTimeStep: not a solver");
+        return NULL;
+    }
+    job_t* j = job_new(kind, h);
+    if (kind == JOB_S2) j->dt = (float)get_double(env, a[1], 0.0);
+    return job_start(env, j, a[0]);
+}
+
+static napi_value js_flip3_step_async(napi_env env, napi_callback_info info) {
+    return step_async(env, info, JOB_FLIP, K_FLIP);
+}
+static napi_value js_smoke3_step_async(napi_env env, napi_callback_info info) {
+    return step_async(env, info, JOB_SMOKE, K_SMOKE);
+}
+static napi_value js_solver2_step_async(napi_env env, napi_callback_info info) {
+    return step_async(env, info, JOB_S2, K_S2);
+}
+
+/* ------------------------------------------------- dot reducer and CG */
+static napi_value js_dot_create(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    int shp[3];
+    int nd = get_int_array(env, a[0], shp, 3);
+    flux_dot* d = NULL;
+    CHECK_FLUX(flux_dot_create(nd, shp, get_field(env, a[1]), get_field(env, a[2]), &d));
+    return wrap(env, K_DOT, d);
+}
+
+static napi_value js_dot_read_async(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    flux_dot* d = (flux_dot*)unwrap(env, a[0], K_DOT, K_DOT);
+    if (!d) {
+        napi_throw_type_error(env, NULL, "read: not a dot reducer");
+        return NULL;
+    }
+    return job_start(env, job_new(JOB_DOT, d), a[0]);
+}
+
+static napi_value js_dot_dispatch(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    CHECK_FLUX(flux_dot_dispatch((flux_dot*)unwrap(env, a[0], K_DOT, K_DOT)));
+    return undef(env);
+}
+
+static napi_value js_dot_info(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    int lanes = 0;
+    flux_field* part = NULL;
+    CHECK_FLUX(flux_dot_info((flux_dot*)unwrap(env, a[0], K_DOT, K_DOT), &lanes, &part));
+    napi_value o;
+    napi_create_object(env, &o);
+    napi_set_named_property(env, o, "lanes", num(env, lanes));
+    napi_set_named_property(env, o, "partial", wrap(env, K_FIELD_VIEW, part));
+    return o;
+}
+
+static napi_value js_cg_create(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    flux_cg* s = NULL;
+    flux_operator* op = is_nullish(env, a[0]) ? NULL : (flux_operator*)unwrap(env, a[0], K_OP, K_OP);
+    const flux_field* diag = is_nullish(env, a[1]) ? NULL : get_field(env, a[1]);
+    CHECK_FLUX(flux_cg_create(op, diag, get_field(env, a[2]), get_field(env, a[3]), &s));
+    return wrap(env, K_CG, s);
+}
+
+static napi_value js_cg_solve_async(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    flux_cg* s = (flux_cg*)unwrap(env, a[0], K_CG, K_CG);
+    if (!s) {
+        napi_throw_type_error(env, NULL, "solve: not a CG solver");
+        return NULL;
+    }
+    job_t* j = job_new(JOB_CG, s);
+    j->tol = get_double(env, a[1], 1e-6);
+    j->maxiter = get_int(env, a[2], 100);
+    return job_start(env, j, a[0]);
+}
+
+static napi_value js_cg_fields(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    args_n(env, info, 1, a);
+    flux_field *p = NULL, *r = NULL, *ap = NULL, *ax = NULL;
+    CHECK_FLUX(flux_cg_fields((flux_cg*)unwrap(env, a[0], K_CG, K_CG), &p, &r, &ap, &ax));
+    napi_value o;
+    napi_create_object(env, &o);
+    napi_set_named_property(env, o, "p", wrap(env, K_FIELD_VIEW, p));
+    napi_set_named_property(env, o, "r", wrap(env, K_FIELD_VIEW, r));
+    napi_set_named_property(env, o, "Ap", wrap(env, K_FIELD_VIEW, ap));
+    napi_set_named_property(env, o, "Ax", wrap(env, K_FIELD_VIEW, ax));
+    return o;
+}
+
 #define EXPORT(name, fn)                                                   \
     do {                                                                   \
         napi_value f_;                                                     \
@@ -1187,6 +1446,18 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("multigridRefresh", js_multigrid_refresh);
     EXPORT("pcgCreate", js_pcg_create);
     EXPORT("pcgSolve", js_pcg_solve);
+    EXPORT("pcgSolveAsync", js_pcg_solve_async);
+    EXPORT("pressure3ProjectAsync", js_pressure3_project_async);
+    EXPORT("flip3StepAsync", js_flip3_step_async);
+    EXPORT("smoke3StepAsync", js_smoke3_step_async);
+    EXPORT("solver2StepAsync", js_solver2_step_async);
+    EXPORT("dotCreate", js_dot_create);
+    EXPORT("dotReadAsync", js_dot_read_async);
+    EXPORT("dotDispatch", js_dot_dispatch);
+    EXPORT("dotInfo", js_dot_info);
+    EXPORT("cgCreate", js_cg_create);
+    EXPORT("cgSolveAsync", js_cg_solve_async);
+    EXPORT("cgFields", js_cg_fields);
     EXPORT("pressure3Create", js_pressure3_create);
     EXPORT("pressure3Field", js_pressure3_field);
     EXPORT("pressure3Project", js_pressure3_project);

@@ -220,7 +220,7 @@ export function createPreconditionedConjugateGradientSolver( applyOperator, appl
 	async function solve( tol, maxiter, residualCheckInterval = 1, gpuResidentScalars = false, batchIterations = true,
 		gpuResidentSetup = false, relativeTolerance = false ) {
 
-		const st = addon.pcgSolve( solver, tol, maxiter, relativeTolerance === true );
+		const st = await addon.pcgSolveAsync( solver, tol, maxiter, relativeTolerance === true );
 		state.residualSquared = st.residualSquared;
 		state.iterations = st.iterations;
 		state.stoppedBy = st.stoppedBy === 'converged' ? 'none' : st.stoppedBy;
@@ -229,6 +229,62 @@ export function createPreconditionedConjugateGradientSolver( applyOperator, appl
 	}
 
 	return { solve, state, settings };
+
+}
+
+// createDotReducer(shape, fieldA, fieldB) -- linalg.js:374-512: read()
+// resolves to sum(a * b); lanes and the per-lane partial field as in the
+// reference; dispatch() computes the lanes without reading back.
+export function createDotReducer( shape, fieldA, fieldB ) {
+
+	if ( shape.length < 1 || shape.length > 3 ) throw new Error( `createDotReducer: only 1D/2D/3D shapes are supported, got ${ shape.length }D.` );
+	const reducer = addon.dotCreate( shape, fieldA.handle, fieldB.handle );
+	const info = addon.dotInfo( reducer );
+	const partial = wrapField( info.partial, 'float', [ info.lanes ] );
+	partial.keepAlive = reducer;
+	return {
+		read: () => addon.dotReadAsync( reducer ),
+		lanes: info.lanes,
+		partial,
+		dispatch: () => addon.dotDispatch( reducer )
+	};
+
+}
+
+// createConjugateGradientSolver(applyOperator, b, x) -- linalg.js:563-728.
+// applyOperator is a createLaplacianOperator result, or { diagonal: field }
+// for the diagonal operator out = diag * in (examples/04-conjugate-gradient).
+export function createConjugateGradientSolver( applyOperator, b, x, options = {} ) {
+
+	if ( b.type !== x.type ) throw new Error( `conjugateGradient: element type mismatch, b.type(${ b.type }) != x.type(${ x.type }).` );
+	if ( b.shape.length !== x.shape.length || b.shape.some( ( v, i ) => v !== x.shape[ i ] ) ) {
+
+		throw new Error( `conjugateGradient: shape mismatch, b.shape(${ b.shape }) != x.shape(${ x.shape }).` );
+
+	}
+
+	if ( b.type !== 'float' ) throw new Error( `conjugateGradient: GPU atomic dot product only supports type "float", got "${ b.type }".` );
+	const op = applyOperator && applyOperator.native ? applyOperator.native.op : null;
+	const diag = applyOperator && applyOperator.diagonal ? applyOperator.diagonal.handle : null;
+	if ( ! op && ! diag ) throw new Error( 'conjugateGradient: the operator must come from createLaplacianOperator or be { diagonal: field }' );
+	const solver = addon.cgCreate( op, diag, b.handle, x.handle );
+	const f = addon.cgFields( solver );
+	const field = ( h ) => {
+
+		const w = wrapField( h, 'float', b.shape );
+		w.keepAlive = solver;
+		return w;
+
+	};
+
+	async function solve( tol, maxiter ) {
+
+		const r = await addon.cgSolveAsync( solver, tol, maxiter );
+		return r.converged;
+
+	}
+
+	return { solve, p: field( f.p ), r: field( f.r ), Ap: field( f.Ap ), Ax: field( f.Ax ) };
 
 }
 
@@ -347,7 +403,7 @@ export function createGridPressureSolver3( {
 		const outFields = [ output.dataU.handle, output.dataV.handle, output.dataW.handle ];
 		return async function dispatch() {
 
-			const d = addon.pressure3Project( solver, inFields, outFields );
+			const d = await addon.pressure3ProjectAsync( solver, inFields, outFields );
 			diagnostics.converged = d.converged;
 			diagnostics.rejected = d.rejected;
 			diagnostics.iterations = d.iterations;
@@ -719,7 +775,7 @@ export function createGridFlipSolver3( {
 	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
 	async function onAdvanceTimeStep() {
 
-		const d = addon.flip3Step( handle );
+		const d = await addon.flip3StepAsync( handle );
 		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
 		return diagnostics;
 
@@ -780,7 +836,7 @@ export function createGridSmokeSolver3( {
 	const diagnostics = { converged: null, rejected: false, iterations: null, stoppedBy: null };
 	async function onAdvanceTimeStep() {
 
-		const d = addon.smoke3Step( handle );
+		const d = await addon.smoke3StepAsync( handle );
 		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
 		return diagnostics;
 
@@ -981,7 +1037,7 @@ export function createGridSolver2( {
 
 	async function onAdvanceTimeStep() {
 
-		const d = addon.solver2Step( handle, readDt() );
+		const d = await addon.solver2StepAsync( handle, readDt() );
 		Object.assign( diagnostics, d, { stoppedBy: d.stoppedBy === 'converged' ? 'none' : d.stoppedBy } );
 		return diagnostics;
 
