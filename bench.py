@@ -10,13 +10,21 @@ The pressure is reset to zero before each step so every step solves the same
 system from the same start; inputs are resident in HBM before timing starts.
 
 Every form of the solver is run on the same system, each timed on its own:
-  plain_rbgs    FP32 V-cycle, red-black Gauss-Seidel 2/2 (multigrid.js defaults)
-  mixed_f16     the same V-cycle stored in FP16 (finest level included),
-                FP32 arithmetic, FP32 outer CG, FP64 dots and true residual
-  mixed_bf16    the same in BF16
-  plain_jacobi  (on request) FP32 V-cycle, damped Jacobi 4/4, omega 6/7
-                (sandbox/jacobi-smoother/multigrid_jacobi.js)
-The headline form (`--headline`, default mixed_f16) gives `value`,
+  plain_rbgs          FP32 V-cycle, red-black Gauss-Seidel 2/2 (multigrid.js
+                      defaults): the reference's method
+  mixed_f16           coarse levels and every level's face weights stored in
+                      FP16 (finest-level x, b FP32), FP32 arithmetic, FP32
+                      outer CG, FP64 dots and true residual
+  mixed_bf16          the same in BF16
+  mixed_f16_cs        mixed_f16 plus the additive coarse-space correction of
+                      the preconditioner along the free-cell indicator (the
+                      near-null mode of the system pinned at one cell)
+  mixed_f16_cs_exact  mixed_f16_cs with the exact solution as a second coarse
+                      vector: the bound of what a learned coarse vector could
+                      add
+  plain_jacobi        (on request) FP32 V-cycle, damped Jacobi 4/4, omega 6/7
+                      (sandbox/jacobi-smoother/multigrid_jacobi.js)
+The headline form (`--headline`, default mixed_f16_cs) gives `value`,
 `ms_per_step` and the roofline; `forms` carries every form's time,
 iterations and FP64 relative residual, and `speedup_vs_plain` is each form's
 time against plain_rbgs.  The CPU leg times the oracle's restatement of the
@@ -43,9 +51,15 @@ FORMS = {
     "plain_rbgs": {"storage": "f32", "smoother": "rbgs", "nu": 2, "omega": 1.0},
     "mixed_f16": {"storage": "f16", "smoother": "rbgs", "nu": 2, "omega": 1.0},
     "mixed_bf16": {"storage": "bf16", "smoother": "rbgs", "nu": 2, "omega": 1.0},
+    # + the additive coarse-space correction along the free-cell indicator
+    "mixed_f16_cs": {"storage": "f16", "smoother": "rbgs", "nu": 2, "omega": 1.0, "cs": "free"},
+    # the bound of any learned coarse vector: the exact solution as the
+    # second coarse vector (taken from a plain solve before timing)
+    "mixed_f16_cs_exact": {"storage": "f16", "smoother": "rbgs", "nu": 2, "omega": 1.0, "cs": "free+exact"},
     "plain_jacobi": {"storage": "f32", "smoother": "jacobi", "nu": 4, "omega": 6.0 / 7.0},
 }
-DEFAULT_FORMS = "plain_rbgs,mixed_f16,mixed_bf16"
+DEFAULT_FORMS = "plain_rbgs,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_exact"
+# the single custom form of --storage/--smoother/--nu/--coarse-space sweeps
 
 
 def parse():
@@ -56,7 +70,7 @@ def parse():
     ap.add_argument("--form-steps", type=int, default=5, help="timed projections of each other form")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--headline", default="mixed_f16", choices=sorted(FORMS))
+    ap.add_argument("--headline", default="mixed_f16_cs", choices=sorted(FORMS))
     ap.add_argument("--forms", default=DEFAULT_FORMS,
                     help="comma-separated forms to run (the headline always runs); plain_jacobi is not in the default "
                          "set: at 256^3 its V-cycle is not a usable preconditioner for this system (the residual-growth "
@@ -68,6 +82,8 @@ def parse():
     ap.add_argument("--smoother", choices=["rbgs", "jacobi"])
     ap.add_argument("--nu", type=int)
     ap.add_argument("--jacobi-omega", type=float)
+    ap.add_argument("--coarse-space", choices=["none", "free"])
+    ap.add_argument("--levels-custom", type=int, dest="levels_custom")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs (baseline and FP64 reference)")
     ap.add_argument("--cpu-iters", type=int, default=2, help="PCG iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -81,21 +97,28 @@ def store_bytes(storage):
     return 4 if storage == "f32" else 2
 
 
+def fine_storage():
+    """FLUX_FINE_STORAGE=1 stores the finest level's x and b in the V-cycle's
+    storage type too (opt-in); otherwise they are FP32"""
+    return os.environ.get("FLUX_FINE_STORAGE") == "1"
+
+
 def algo_bytes_per_cell(kernel, storage):
     """Algorithmic bytes per finest-level cell and launch: the unique bytes
     the launch must move (u8 neighbour-mask byte; face weights one value per
     face, three per cell)."""
     s = store_bytes(storage)
+    x = s if fine_storage() else 4
     if kernel == "relax":
         # one red-black colour launch: x read, b on the colour (half), x
-        # written on the colour (half) in the fine level's storage, mask 1,
-        # weights 3 x storage (21 B in FP32, 13 B in FP16/BF16)
-        return 2 * s + 1 + 3 * s
+        # written on the colour (half), mask 1, weights 3 x storage
+        # (21 B in FP32; 15 B with FP16/BF16 weights; 11 B with FP16 x, b too)
+        return 2 * x + 1 + 3 * s
     if kernel == "jacobi":
         # one steady-state damped-Jacobi sweep: x read, b read, x' written in
         # the fine level's storage (FP32 for f32; FP16/BF16 in the mixed
         # form), mask 1, weights 3 x storage
-        return 3 * s + 1 + 3 * s
+        return 3 * x + 1 + 3 * s
     if kernel == "update":
         # x64 read + write (16), p 4, b 4, r written 4, mask 1, weights 12
         return 41.0
@@ -108,10 +131,17 @@ def algo_bytes_per_cell(kernel, storage):
 
 
 # fine-level kernel of each family as rocprofv3 names it
-KERNEL_SYMBOL = {("relax", "f32"): "k_relax3<1, 0, 0,", ("relax", "f16"): "k_relax3m<0, 0, __half>",
-                 ("relax", "bf16"): "k_relax3m<0, 0, __hip_bfloat16>", ("jacobi", "f32"): "k_jacobi3<1, 0, 0,",
-                 ("jacobi", "f16"): "k_jacobi3m<0, 0, __half>", ("jacobi", "bf16"): "k_jacobi3m<0, 0, __hip_bfloat16>",
+KERNEL_SYMBOL = {("relax", "f32"): "k_relax3<1, 0, 0, float, float>",
+                 ("relax", "f16"): "k_relax3<1, 0, 0, float, __half>",
+                 ("relax", "bf16"): "k_relax3<1, 0, 0, float, __hip_bfloat16>",
+                 ("jacobi", "f32"): "k_jacobi3<1, 0, 0, float, float>",
+                 ("jacobi", "f16"): "k_jacobi3<1, 0, 0, float, __half>",
                  ("update", "f32"): "k_pcg_update3", ("apply", "f32"): "k_pcg_apply3"}
+if fine_storage():
+    KERNEL_SYMBOL.update({("relax", "f16"): "k_relax3m<0, 0, __half>",
+                          ("relax", "bf16"): "k_relax3m<0, 0, __hip_bfloat16>",
+                          ("jacobi", "f16"): "k_jacobi3m<0, 0, __half>",
+                          ("jacobi", "bf16"): "k_jacobi3m<0, 0, __hip_bfloat16>"})
 
 
 def measured_traffic(kernel, storage):
@@ -176,19 +206,22 @@ def cpu_sample(sc, levels, gpu_iters, iters, threads):
     }
 
 
-def fp64_reference(sc, levels, x0, threads):
+FP64_TOL = 1e-9
+
+
+def fp64_reference(sc, levels, threads):
     """The same system solved on the host in FP64 (the oracle's f64 PCG with
-    its FP64 V-cycle) to a relative residual of 1e-10, warm-started from a
-    GPU answer: the pinned system has one solution, so the start only
-    shortens the solve."""
+    its FP64 V-cycle) from the cold start to a relative residual of 1e-9.
+    (A warm start from a GPU answer trips the reference's pAp-growth guard,
+    whose base is the initial residual.)"""
     from oracle import pyoracle as O
     O.set_threads(threads)
     n, h = sc["shape"], sc["spacing"]
     t0 = time.perf_counter()
     A = O.pressure3_assemble(n, h, sc["vel"], active=sc["active"], target=sc["target"], beta=sc["beta"], suffix="f64")
     mg = O.Multigrid(n, h, levels=levels, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f64")
-    x, st = O.pcg_solve(n, h, A["b"], x0=x0.astype(np.float64), mask=A["mask"], weights=A["weights"], mg=mg,
-                        tol=1e-10, maxiter=400)
+    x, st = O.pcg_solve(n, h, A["b"], x0=A["pressure"], mask=A["mask"], weights=A["weights"], mg=mg, tol=FP64_TOL,
+                        maxiter=400)
     return x, st, time.perf_counter() - t0
 
 
@@ -202,12 +235,14 @@ def main():
 
     forms = dict(FORMS)
     headline = args.headline
-    if args.storage or args.smoother or args.nu or args.jacobi_omega:
+    if args.storage or args.smoother or args.nu or args.jacobi_omega or args.coarse_space:
         base = dict(FORMS[headline])
         for k, v in (("storage", args.storage), ("smoother", args.smoother), ("nu", args.nu),
                      ("omega", args.jacobi_omega)):
             if v is not None:
                 base[k] = v
+        if args.coarse_space:
+            base["cs"] = None if args.coarse_space == "none" else args.coarse_space
         forms = {"custom": base}
         headline = "custom"
         run = ["custom"]
@@ -251,6 +286,17 @@ def main():
                        "nu_down": fm["nu"], "nu_up": fm["nu"], "n_final": fm["nu"], "jacobi_omega": fm["omega"],
                        "omega": 1.0},
             tolerance=1e-6, relative_tolerance=True)
+        cs = fm.get("cs")
+        if cs == "free":
+            ps.set_coarse_space(True, [])
+        elif cs == "free+exact":
+            exact = pressures.get("plain_rbgs")
+            if exact is None:
+                ps.pressure.fill(0.0)
+                ps.project(vin, vout)
+                exact = ps.pressure.to_array()
+            exact_field = F.array_n("float", shape, exact)
+            ps.set_coarse_space(True, [exact_field])
 
         def step():
             ps.pressure.fill(0.0)
@@ -291,7 +337,7 @@ def main():
                          "iterations": int(np.median(iters)), "converged": bool(last["converged"]),
                          "fp64_relative_residual": last["residual"] / last["bNorm"] if last["bNorm"] else None,
                          "storage": fm["storage"], "smoother": fm["smoother"], "nu": fm["nu"],
-                         "omega": round(fm["omega"], 4)}
+                         "omega": round(fm["omega"], 4), "coarse_space": fm.get("cs")}
         pressures[name] = ps.pressure.to_array()
         print("bench: %s: %d steps, %.3f ms/projection, iterations %s, last %s" % (name, steps, ms, iters, last),
               file=sys.stderr, flush=True)
@@ -313,13 +359,14 @@ def main():
         cpu["cpu_model"] = cpu_model()
         cpu["nproc"] = os.cpu_count()
         print("bench: FP64 reference solve on %d threads" % threads, file=sys.stderr, flush=True)
-        x64, st64, t64 = fp64_reference(sc, args.levels, pressures[run[0]], threads)
+        x64, st64, t64 = fp64_reference(sc, args.levels, threads)
         nx = np.linalg.norm(x64)
         for name, p in pressures.items():
-            results[name]["error_vs_fp64_cpu_solve"] = float(np.linalg.norm(p.astype(np.float64) - x64) / nx)
-        cpu["fp64_reference"] = {"tolerance": 1e-10, "iterations": st64["iterations"],
-                                 "converged": st64["converged"], "seconds": round(t64, 1),
-                                 "warm_start": run[0]}
+            results[name]["error_vs_fp64_cpu_solve"] = (float(np.linalg.norm(p.astype(np.float64) - x64) / nx)
+                                                        if st64["converged"] else None)
+        cpu["fp64_reference"] = {"tolerance": FP64_TOL, "iterations": st64["iterations"],
+                                 "converged": st64["converged"], "stopped_by": st64["stoppedBy"],
+                                 "seconds": round(t64, 1), "threads": threads}
 
     elapsed, steps = line_time
     hf = results[headline]

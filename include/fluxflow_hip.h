@@ -160,6 +160,7 @@ typedef struct {
     double residual;         /* ||b - A x||_2 of the last true residual (FP64) */
     double threshold;        /* max(tol*||b||, tol) when relative, else tol */
     double b_norm;
+    int restarts;            /* beta reset to 0 on a sign flip or |beta| > 10 (linalg.js:2436-2442) */
 } flux_pcg_state;
 typedef struct flux_pcg flux_pcg;
 /* createPreconditionedConjugateGradientSolver(applyOperator, applyPreconditioner, b, x)

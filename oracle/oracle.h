@@ -67,6 +67,7 @@ typedef struct {
     double residual_sq;  /* r.r of the last true residual */
     double threshold;    /* absolute stop threshold used */
     double b_norm;
+    int restarts;        /* beta resets: sign flip or |beta| > 10 (linalg.js:2436-2442) */
 } orc_pcg_state;
 
 /* ---------------- deterministic inputs ---------------- */
@@ -187,6 +188,11 @@ ORC_DECL(float, f32)
 
 /* the HIP plain path's arithmetic: f32 vectors, double solution accumulator,
  * double true residual, double dots (see oracle_f32.c) */
+/* Post-smoothing with the colours in the pre-smoothing order (not reversed):
+ * the non-symmetric V-cycle that the reference's captured frame-112 stalled
+ * on, a host-side switch of the stop-code tests. */
+void orc_set_post_forward(int on);
+int orc_post_forward_flag(void);
 void orc_set_flexible(int on);
 int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec,
                         void* mg, const float* b, float* x, double* x64_out, double tol, int maxiter,

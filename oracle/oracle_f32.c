@@ -63,6 +63,10 @@ static void apply_mixed(const orc_grid* g, const double* x, float* r, const floa
 static int g_flexible = 0;
 void orc_set_flexible(int on) { g_flexible = on ? 1 : 0; }
 
+static int g_post_forward = 0;
+void orc_set_post_forward(int on) { g_post_forward = on ? 1 : 0; }
+int orc_post_forward_flag(void) { return g_post_forward; }
+
 int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec, void* mg,
                         const float* b, float* x, double* x64_out, double tol, int maxiter, int relative,
                         orc_pcg_state* st) {
@@ -130,6 +134,7 @@ int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* con
             int flip = (oldRZ > 0) != (newRZ > 0);
             double braw = (g_flexible ? newRZ - rzo : newRZ) / oldRZ;
             beta = (flip || !(fabs(braw) <= 10)) ? 0.0 : braw;
+            st->restarts += (flip || !(fabs(braw) <= 10)) ? 1 : 0;
             oldRZ = newRZ;
         }
         st->converged = st->stopped_by == ORC_STOP_CONVERGED;

@@ -60,6 +60,7 @@ class PCGState(C.Structure):
         ("residual_sq", C.c_double),
         ("threshold", C.c_double),
         ("b_norm", C.c_double),
+        ("restarts", C.c_int),
     ]
 
 
@@ -284,6 +285,7 @@ def pcg_solve(shape, spacing, b, x0=None, mask=None, weights=None, precond="mult
         "residual": float(np.sqrt(abs(st.residual_sq))),
         "threshold": st.threshold,
         "bNorm": st.b_norm,
+        "restarts": st.restarts,
     }
 
 
@@ -523,6 +525,12 @@ def two_phase_beta3(frac, n, rho_gas, rho_liquid):
     return outs
 
 
+def set_post_forward(on):
+    """post-smoothing in the pre-smoothing colour order (a non-symmetric
+    V-cycle), for the stop-code tests"""
+    lib().orc_set_post_forward(int(bool(on)))
+
+
 def set_flexible(on):
     """the flexible (Polak-Ribiere) beta in pcg_solve_mixed"""
     lib().orc_set_flexible(int(bool(on)))
@@ -550,6 +558,7 @@ def pcg_solve_mixed(shape, spacing, b, x0=None, mask=None, weights=None, precond
         "residual": float(np.sqrt(abs(st.residual_sq))),
         "threshold": st.threshold,
         "bNorm": st.b_norm,
+        "restarts": st.restarts,
     }
 
 

@@ -430,6 +430,7 @@ void FN(orc_jacobi)(const orc_grid* g, const REAL* xr, REAL* xw, const REAL* b, 
  * count lands back in x; the reference rejects odd counts) and ignores the
  * order, being symmetric by construction. */
 static void FN(mg_relax_n)(FN(orc_mg)* mg, int l, int iters, int reversed) {
+    if (orc_post_forward_flag()) reversed = 0; /* the non-symmetric V-cycle of the stop-code tests */
     if (mg->o.smoother == ORC_SMOOTH_JACOBI) {
         const REAL* const* w = mg->w[l][0] ? (const REAL* const*)mg->w[l] : NULL;
         for (int it = 0; it < iters; ++it) {
@@ -602,6 +603,7 @@ int FN(orc_pcg_solve)(const orc_grid* g, const uint8_t* mask, const REAL* const*
             int flip = (oldRZ > 0) != (newRZ > 0);
             double betaRaw = newRZ / oldRZ;
             double beta = (flip || fabs(betaRaw) > 10) ? 0.0 : betaRaw;
+            st->restarts += (flip || fabs(betaRaw) > 10) ? 1 : 0;
             REAL be = (REAL)beta;
             for (int64_t q = 0; q < n; ++q) p[q] = (REAL)(z[q] + (REAL)(p[q] * be));
             oldRZ = newRZ;

@@ -133,3 +133,89 @@ def test_scaled_start_along_exact_and_random_directions(flux):
     assert d2["converged"] and d2["residual"] <= 1e-6 * d2["bNorm"]
     assert d2["iterations"] <= d0["iterations"] + 2, (d0, d2, alpha)
     print("cold", d0["iterations"], "exact", d1["iterations"], "noise", d2["iterations"], "alpha", alpha)
+
+
+def test_energy_loss_gradients_match_oracle(flux, L):
+    """flux_unet_train_step_energy: the loss and every parameter gradient
+    against the float64 oracle -- dL/dy = -2 (g.r) r / (q e*) - 2 (g.r)^2 A g
+    / (q^2 e*) on the free cells (g the output there, q = -g.A g) computed
+    in numpy with the oracle's operator, then the oracle's backward."""
+    from oracle import pyoracle as O
+    F = flux
+    n = (16, 8, 8)
+    N = int(np.prod(n))
+    rng = np.random.default_rng(4)
+    p = U.init_params(3)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.1
+    x = rng.standard_normal((3, n[2], n[1], n[0]))
+    mask = (rng.random(N) < 0.1).astype(np.uint8)
+    w = [rng.uniform(0.5, 2.0, int(np.prod(F.face_shape(n, a)))).astype(np.float32) for a in range(3)]
+    h = (0.5, 0.5, 0.5)
+    r = np.where(mask == 0, rng.standard_normal(N), 0.0).astype(np.float32)
+    e_star = 3.7
+    net = L.UNet(n, training=True, params={k: v.astype(np.float32) for k, v in p.items()})
+    mf = F.array_n("u8", n, mask)
+    wf = [F.array_n("float", F.face_shape(n, a), w[a]) for a in range(3)]
+    pair = {"x": F.array_n("float", (3 * N,), x.astype(np.float32).ravel()), "r": F.array_n("float", n, r),
+            "mask": mf, "op": F.create_laplacian_operator(n, h, mf, wf), "e_star": e_star}
+    loss = net.train_step_energy(pair, lr=0.0)
+    yr, cache = U.forward(p, x)
+    free = mask == 0
+    g = np.where(free, yr.ravel(), 0.0)
+    ag = O.apply(n, h, g, mask, [a.astype(np.float64) for a in w], suffix="f64")
+    gr = float(g @ r)
+    q = -float(g @ ag)
+    want = 1.0 - gr * gr / (q * e_star)
+    assert abs(loss - want) <= 1e-4 * max(abs(want), 1e-3), (loss, want)
+    gy = np.where(free, -2 * gr / (q * e_star) * r - 2 * gr * gr / (q * q * e_star) * ag, 0.0)
+    grads, _ = U.backward(p, cache, gy.reshape(1, n[2], n[1], n[0]))
+    got = L.unflatten(net.get(which=1))
+    for name in grads:
+        assert rel(got[name], grads[name]) <= 1e-3, name
+
+
+def test_energy_training_and_learned_coarse_vector(flux, L):
+    """Trained on MGPCG-produced pairs with the energy-norm loss, the network
+    supplies a coarse vector of the preconditioner (coarse space {free
+    cells, guess}); on held-out scenes every solve reaches the same FP64
+    true-residual tolerance, and the iterations and times of the cold
+    start, of the free-cell correction alone and with the learned vector
+    are recorded."""
+    import time
+    F = flux
+    from tsl_array_n_amd import scenes
+    n = 32
+    pairs = L.energy_pairs(F, n, 9, seed=5, levels=4)
+    assert len(pairs) >= 6
+    net, losses = L.train_energy(F, n, pairs, 300, lr=1e-3)
+    print("energy loss first %.3f last %.3f" % (np.mean(losses[:9]), np.mean(losses[-9:])))
+    assert np.mean(losses[-9:]) < np.mean(losses[:9])
+    rows = []
+    for q, seed in enumerate((101, 202, 303)):
+        sc = scenes.dam_break_system(n, box_max=(0.35, 0.55, 1.0)) if q == 2 else scenes.random_scene(n, seed)
+        ratio = 1000.0 if q == 2 else sc["ratio"]
+        shape = sc["shape"]
+        beta = [F.array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+        act = F.array_n("u8", shape, sc["active"])
+        tgt = F.array_n("float", shape, sc["target"])
+        row = {}
+        for mode in ("cold", "free", "learned"):
+            vel = [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+            ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                                face_weights=beta, multigrid={"levels": 4}, tolerance=1e-6)
+            if mode == "free":
+                ps.set_coarse_space(True, [])
+            proj = L.CoarseProjector(ps, net, ratio) if mode == "learned" else ps
+            d = proj.project(vel)  # warm-up of the same system
+            ps.pressure.fill(0.0)
+            F.sync()
+            t0 = time.perf_counter()
+            d = proj.project(vel)
+            F.sync()
+            ms = (time.perf_counter() - t0) * 1e3
+            assert d["converged"] and d["residual"] <= 1e-6 * d["bNorm"], (mode, d)
+            row[mode] = (d["iterations"], round(ms, 2))
+        rows.append(row)
+    print("iterations, ms (cold / free-cell correction / + learned vector):", rows)

@@ -67,7 +67,8 @@ class MGOptions(C.Structure):
 
 class PCGState(C.Structure):
     _fields_ = [("iterations", C.c_int), ("stopped_by", C.c_int), ("converged", C.c_int),
-                ("residual", C.c_double), ("threshold", C.c_double), ("b_norm", C.c_double)]
+                ("residual", C.c_double), ("threshold", C.c_double), ("b_norm", C.c_double),
+                ("restarts", C.c_int)]
 
 
 class Pressure3Desc(C.Structure):
@@ -331,7 +332,7 @@ class PCGSolver:
                                    C.byref(st)), "solve")
         self.state = {"iterations": st.iterations, "stoppedBy": STOP_NAMES.get(st.stopped_by, "unknown"),
                       "converged": bool(st.converged), "residual": st.residual, "threshold": st.threshold,
-                      "bNorm": st.b_norm}
+                      "bNorm": st.b_norm, "restarts": st.restarts}
         return bool(st.converged)
 
     def __del__(self):

@@ -1091,7 +1091,8 @@ template <typename ST>
 __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, LevelDevT<ST> L1,
                                                             const ST* __restrict__ b0, ST* __restrict__ x0,
                                                             float omega, int nu_down, int nu_up, int n_coarsest,
-                                                            float mult, int jacobi, const int* __restrict__ stop) {
+                                                            float mult, int jacobi, int post_forward,
+                                                            const int* __restrict__ stop) {
     constexpr int PER = COARSE_LDS_CELLS / COARSE_TPB;
     __shared__ float sx0[COARSE_LDS_CELLS];
     __shared__ float sr0[COARSE_LDS_CELLS];
@@ -1321,8 +1322,8 @@ __global__ void __launch_bounds__(COARSE_TPB) k_vcycle_tail(LevelDevT<ST> L0, Le
             jsweep0();
             continue;
         }
-        sweep0(1);
-        sweep0(0);
+        sweep0(post_forward ? 0 : 1);
+        sweep0(post_forward ? 1 : 0);
     }
     for (int c = threadIdx.x; c < n0; c += COARSE_TPB) x0[c] = to_store<ST>(sx0[c]);
 }
@@ -1431,6 +1432,10 @@ struct flux_multigrid {
     // red-black, 256^3 62 -> 82), more than the halved fine-level traffic
     // saves.
     bool fine16 = false;
+    // post-smoothing in the pre-smoothing colour order, a non-symmetric
+    // V-cycle (FLUX_MG_POST_FORWARD=1): a host-side switch that forces the
+    // PCG guards in tests
+    bool post_forward = false;
 };
 
 static constexpr int COARSE_SINGLE_BLOCK_MAX = COARSE_LDS_CELLS;
@@ -1850,7 +1855,8 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
         k_vcycle_tail<ST><<<1, COARSE_TPB, 0, g_stream>>>(level_dev<ST>(mg, v), level_dev<ST>(mg, c), (const ST*)v.b,
                                                           (ST*)v.x, smooth_omega(mg), mg->o.nu_down,
                                                           mg->o.nu_up, mg->o.n_coarsest,
-                                                          (float)(c.xscale / v.xscale), jac, stop);
+                                                          (float)(c.xscale / v.xscale), jac, mg->post_forward ? 1 : 0,
+                                                          stop);
         return FLUX_OK;
     }
     if (fine) {
@@ -1878,10 +1884,10 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
         kprof_begin("correct");
         correct_level<1, ST, float>(mg, v, c, scale, stop);
         kprof_end("correct");
-        level_smooth<1, float, ST>(mg, l, mg->o.n_final, true, false, stop, dot);
+        level_smooth<1, float, ST>(mg, l, mg->o.n_final, !mg->post_forward, false, stop, dot);
     } else {
         correct_level<0, ST, ST>(mg, v, c, nullptr, stop);
-        level_smooth<0, ST, ST>(mg, l, mg->o.nu_up, true, false, stop);
+        level_smooth<0, ST, ST>(mg, l, mg->o.nu_up, !mg->post_forward, false, stop);
     }
     return FLUX_OK;
 }
@@ -2004,6 +2010,8 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     mg->tail = !(tenv && tenv[0] == '0');
     const char* penv = getenv("FLUX_CORRECT_PAIRS");
     mg->pairs = !(penv && penv[0] == '0');
+    const char* pfenv = getenv("FLUX_MG_POST_FORWARD");
+    mg->post_forward = pfenv && pfenv[0] == '1';
     const char* f16env = getenv("FLUX_FINE_STORAGE");
     mg->fine16 = o.storage != FLUX_STORE_F32 && ndim == 3 && o.levels > 1 && (n0[0] % 2) == 0 && o.nu_down >= 1 &&
                  o.n_final >= 1 && (f16env && f16env[0] == '1');

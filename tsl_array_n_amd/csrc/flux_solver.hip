@@ -98,6 +98,7 @@ struct Scal {
     double pAp, rr, rz, rz_old, bb, base, alpha, beta, best, thr, tol, rr0;
     int stop, iter, stop_iter, relative;
     int xsel, psel, flexible, cs_m;
+    int restarts, pad2;
     double rzo;  // r_{k+1} . z_k of the flexible beta
     // coarse-space correction (cs_m vectors): Einv = (Phi^T A Phi)^-1,
     // c = Phi^T r of the current residual, a = Einv c the coarse amplitudes
@@ -323,6 +324,7 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* parti
     double braw = (S->flexible ? tot - S->rzo : tot) / rzOld;
     bool restart = flip || !(fabs(braw) <= 10.0);  // MAX_BETA_MAGNITUDE
     S->beta = restart ? 0.0 : braw;
+    S->restarts += restart ? 1 : 0;
     S->rz_old = tot;
 }
 
@@ -347,32 +349,33 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
         const int i = t - j * n0;
         const uint8_t* __restrict__ nbits = L.nb;
         const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
-        // the preconditioned residual with the coarse correction: z + Phi a
-        auto zval = [&](int q) {
+        // the preconditioned residual with the coarse correction, z + Phi a
+        // (Phi is 0 on Dirichlet cells; a Dirichlet neighbour's value is
+        // masked to 0 below, so only centre values need the cell's own bit)
+        auto zval = [&](int q, bool dir) {
             float v = z[q];
-            if (csm) {
-                const bool dir = nbits && ((nbits[q] >> 6) & 1u);
-                for (int t = 0; t < csm; ++t) v = v + ca[t] * cs_phi(cs, t, q, dir);
-            }
+            for (int t = 0; t < csm; ++t) v = v + ca[t] * cs_phi(cs, t, q, dir);
             return v;
         };
-        auto pval = [&](int q) { return zval(q) + pold[q] * beta; };
+        auto pval = [&](int q, bool dir) { return zval(q, dir) + pold[q] * beta; };
         // the z-neighbours are the previous and the next plane's centres
-        float pnext = pval(i + n0 * (j + n1 * k0));
+        const int c0 = i + n0 * (j + n1 * k0);
+        float pnext = pval(c0, csm && nbits && ((nbits[c0] >> 6) & 1u));
         float pprev = 0.0f;
         for (int k = k0; k < k1; ++k) {
             const int c = i + n0 * (j + n1 * k);
             const float pc = pnext;
-            if (k + 1 < n2) pnext = pval(c + plane);
             const unsigned bits = nbits ? nbits[c] : 0u;
+            // bit 5: the +z neighbour (the next plane's centre) is Dirichlet
+            if (k + 1 < n2) pnext = pval(c + plane, (bits >> 5) & 1u);
             float w[6];
             face_w3(L, i, j, k, c, w);
             float xn[6];
-            xn[0] = pval(i > 0 ? c - 1 : c);
-            xn[1] = pval(i < n0 - 1 ? c + 1 : c);
-            xn[2] = pval(j > 0 ? c - n0 : c);
-            xn[3] = pval(j < n1 - 1 ? c + n0 : c);
-            xn[4] = k > k0 ? pprev : pval(k > 0 ? c - plane : c);
+            xn[0] = pval(i > 0 ? c - 1 : c, false);
+            xn[1] = pval(i < n0 - 1 ? c + 1 : c, false);
+            xn[2] = pval(j > 0 ? c - n0 : c, false);
+            xn[3] = pval(j < n1 - 1 ? c + n0 : c, false);
+            xn[4] = k > k0 ? pprev : pval(k > 0 ? c - plane : c, false);
             xn[5] = k + 1 < n2 ? pnext : pc;
 #pragma unroll
             for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
@@ -871,6 +874,7 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     st->residual = sqrt(fabs(h.rr));
     st->threshold = h.thr;
     st->b_norm = sqrt(fabs(h.bb));
+    st->restarts = h.restarts;
     s->last_iterations = st->iterations;
     return FLUX_OK;
 }

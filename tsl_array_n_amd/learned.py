@@ -101,6 +101,19 @@ class UNet:
                                          C.byref(loss)))
         return loss.value
 
+    def train_step_energy(self, pair, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        """one Adam step on the energy-norm loss of an energy_pairs() pair:
+        L = 1 - (g.r)^2 / ((g.B g) e*) with g the output on the free cells,
+        B = -A (flux_unet_train_step_energy); returns L"""
+        self.input.copy_from(pair["x"])
+        loss = C.c_double()
+        _check(_lib.flux_unet_train_step_energy(C.c_void_p(self.h), C.c_void_p(self.input.h),
+                                                C.c_void_p(pair["op"].h), C.c_void_p(pair["r"].h),
+                                                C.c_void_p(pair["mask"].h if pair["mask"] else None),
+                                                C.c_double(pair["e_star"]), C.c_float(lr), C.c_float(beta1),
+                                                C.c_float(beta2), C.c_float(eps), C.byref(loss)), "train_step_energy")
+        return loss.value
+
     def features(self, b, mask, weights, density_ratio):
         w = (C.c_void_p * 3)(*[f.h for f in weights]) if weights is not None else None
         _check(_lib.flux_unet_features(C.c_void_p(self.h), C.c_void_p(b.h), C.c_void_p(mask.h if mask else None), w,
@@ -158,6 +171,92 @@ class LearnedProjector:
                                              C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
                                              C.c_void_p(s.pressure.h)), "guess")
         return s.solve(vin, vout)
+
+
+class CoarseProjector:
+    """project() with the network's guess as a coarse vector of the
+    preconditioner: assemble -> features -> guess g -> coarse space {free
+    cells, g} (flux_pressure3_set_coarse_space) -> the plain solve, accepted
+    at the same FP64 true-residual tolerance.  With start=True the guess is
+    also the PCG start along its energy-optimal step."""
+
+    def __init__(self, solver, net, density_ratio, start=False):
+        self.solver = solver
+        self.net = net
+        self.density_ratio = float(density_ratio)
+        self.start = start
+        self.g = array_n("float", solver.resolution)
+        solver.set_coarse_space(True, [self.g])
+
+    def project(self, vin, vout=None):
+        s = self.solver
+        s.assemble(vin)
+        self.net.features(s.b, s.dirichlet_mask, s.weights, self.density_ratio)
+        self.net.guess(self.g)
+        if self.start:
+            s.scaled_guess(self.g)
+        return s.solve(vin, vout)
+
+
+def scene_for(n, q, rng):
+    """training / evaluation scenes: dam-break variants (every third) and
+    random two-phase scenes"""
+    from . import scenes
+    if q % 3 == 2:
+        return scenes.dam_break_system(n, box_max=(rng.uniform(0.2, 0.7), rng.uniform(0.3, 0.8), 1.0)), 1000.0
+    sc = scenes.random_scene(n, int(rng.integers(0, 2 ** 31)))
+    return sc, sc["ratio"]
+
+
+def energy_pairs(F, n, count, seed=0, levels=None):
+    """Training pairs for the energy-norm loss, produced by the MGPCG solver
+    itself: the input features, the start's residual r (b on the free
+    cells), the Dirichlet mask, the system's operator and e* = x*.(-r) of
+    the converged solution x*; everything stays on the GPU."""
+    levels = levels or max(2, int(np.log2(n)) - 2)
+    rng = np.random.default_rng(seed)
+    pairs = []
+    net = UNet((n, n, n), training=False)
+    q = 0
+    while len(pairs) < count and q < 4 * count:
+        sc, ratio = scene_for(n, q, rng)
+        q += 1
+        shape = sc["shape"]
+        vel = [array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+        beta = [array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+        act = array_n("u8", shape, sc["active"])
+        tgt = array_n("float", shape, sc["target"])
+        ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                            face_weights=beta, multigrid={"levels": levels}, tolerance=1e-6)
+        d = ps.project(vel)
+        if not d["converged"]:
+            continue
+        mask = ps.dirichlet_mask.to_array() if ps.dirichlet_mask else np.zeros(n ** 3, np.uint8)
+        free = mask == 0
+        b = ps.b.to_array().astype(np.float64)
+        x = ps.pressure.to_array().astype(np.float64)
+        e_star = -float(np.dot(x[free], b[free]))
+        if not e_star > 0:
+            continue
+        net.features(ps.b, ps.dirichlet_mask, ps.weights, ratio)
+        feats = array_n("float", (3 * n ** 3,), net.input.to_array())
+        mfield = array_n("u8", shape, mask)
+        r = array_n("float", shape, np.where(free, b, 0.0).astype(np.float32))
+        wf = [array_n("float", F.face_shape(shape, a), ps.weights[a].to_array()) for a in range(3)]
+        op = F.create_laplacian_operator(shape, sc["spacing"], mfield, wf)
+        pairs.append({"x": feats, "r": r, "mask": mfield, "op": op, "e_star": e_star, "keep": (wf,),
+                      "ratio": ratio})
+    return pairs
+
+
+def train_energy(F, n, pairs, steps, lr=1e-3, seed=0, log=None):
+    net = UNet((n, n, n), training=True, params=init_params(seed))
+    losses = []
+    for s in range(steps):
+        losses.append(net.train_step_energy(pairs[s % len(pairs)], lr=lr))
+        if log and (s % 100 == 0 or s == steps - 1):
+            log("step %d energy loss %.4e" % (s, losses[-1]))
+    return net, losses
 
 
 def training_pairs(F, n, count, seed=0, levels=None):

@@ -214,3 +214,36 @@ def karman2d(nx=256, ny=128, radius=12.0, center=(64.0, 64.0), segments=48, infl
         "frame_dt": frame_dt, "courant": courant, "max_sub_steps": 32,
         "multigrid": {"levels": levels, "nu_down": 2, "nu_up": 2, "n_coarsest": 20}, "tolerance": tolerance,
     }
+
+
+def flip_dam_break(n, ppa=2, box_max=(0.4, 0.6, 1.0), jitter=0.2, seed=0):
+    """Particles of the two-phase FLIP dam break on the unit domain at n^3
+    (h = 1/n): ppa^3 particles per cell in EVERY cell (the gas is simulated
+    too, as in grid_two_phase_flip_solver2.js), on the sub-cell lattice of
+    computeFlipBoxSeed3 (grid_flip_solver3.js:70-130) with a jitter of
+    +-jitter sub-spacings per axis (numpy's generator, not mulberry32: the
+    full-size scene only needs a deterministic seed); concentration 1 in the
+    liquid block (cells with centre x < 0.4, y < 0.6) and 0 in the gas.
+    Returns float32 positions (x, y, z per particle) and concentrations,
+    generated one sub-plane at a time."""
+    h = 1.0 / n
+    m = n * ppa
+    sub = h / ppa
+    cx, cy, _ = liquid_block_cells(n, box_max)
+    rng = np.random.default_rng(seed)
+    count = m ** 3
+    pos = np.empty((count, 3), np.float32)
+    conc = np.empty(count, np.float32)
+    ii, jj = np.meshgrid(np.arange(m, dtype=np.float32), np.arange(m, dtype=np.float32), indexing="xy")
+    ii, jj = ii.ravel(), jj.ravel()
+    liquid_xy = ((ii // ppa) < cx) & ((jj // ppa) < cy)
+    per = m * m
+    for kk in range(m):
+        sl = slice(kk * per, (kk + 1) * per)
+        jit = rng.uniform(-jitter, jitter, (per, 3)).astype(np.float32)
+        pos[sl, 0] = (ii + np.float32(0.5) + jit[:, 0]) * np.float32(sub)
+        pos[sl, 1] = (jj + np.float32(0.5) + jit[:, 1]) * np.float32(sub)
+        pos[sl, 2] = (np.float32(kk) + np.float32(0.5) + jit[:, 2]) * np.float32(sub)
+        conc[sl] = liquid_xy.astype(np.float32)
+    return {"n": (n, n, n), "spacing": (h, h, h), "positions": pos.ravel(), "concentration": conc, "count": count,
+            "liquid": int(conc.sum()), "ratio": 1000.0}
