@@ -678,6 +678,11 @@ int flux_smoke3_step(flux_smoke3* s, flux_pressure_diag* diag);
 /* a ball source re-injected with max into the current density/temperature
  * (examples/34-smoke-plume-3d/main.js:146-158) */
 int flux_smoke3_source_sphere(flux_smoke3* s, const float* center, float radius, float density, float temperature);
+/* a per-cell source re-injected with max: state = max(state, source) for the
+ * current density and/or temperature (NULL leaves that state alone); the
+ * example's createSourceKernel (examples/34-smoke-plume-3d/main.js:146-158)
+ * with the source value read from a field, e.g. a seeded noisy ball */
+int flux_smoke3_source_field(flux_smoke3* s, const flux_field* density, const flux_field* temperature);
 enum { FLUX_SMOKE3_U = 0, FLUX_SMOKE3_V = 1, FLUX_SMOKE3_W = 2, FLUX_SMOKE3_DENSITY = 3, FLUX_SMOKE3_TEMPERATURE = 4,
        FLUX_SMOKE3_PRESSURE = 5 };
 /* DENSITY / TEMPERATURE return the current state (density.current) */

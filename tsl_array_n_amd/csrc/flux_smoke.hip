@@ -61,6 +61,12 @@ __global__ void k_source_sphere(G3 g, float* __restrict__ d, float cx, float cy,
     d[c] = fmaxf(d[c], in ? value : 0.0f);
 }
 
+// the same max re-injection with a per-cell source value
+__global__ void k_source_max(float* __restrict__ d, const float* __restrict__ src, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) d[c] = fmaxf(d[c], src[c]);
+}
+
 }  // namespace flux
 
 using namespace flux;
@@ -203,6 +209,24 @@ int flux_smoke3_source_sphere(flux_smoke3* s, const float* center, float radius,
                                                density);
     k_source_sphere<<<gc, KTPB, 0, g_stream>>>(g, (float*)s->temp[s->active]->d, center[0], center[1], center[2], radius,
                                                temperature);
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int flux_smoke3_source_field(flux_smoke3* s, const flux_field* density, const flux_field* temperature) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(s, "smoke source: NULL solver");
+    const flux_field* src[2] = {density, temperature};
+    flux_field* dst[2] = {s->dens[s->active], s->temp[s->active]};
+    for (int q = 0; q < 2; ++q) {
+        if (!src[q]) continue;
+        FLUX_REQUIRE(src[q]->dtype == FLUX_F32 && src[q]->count == s->cells,
+                     "smoke source: the %s source must be a float field of %lld cells",
+                     q ? "temperature" : "density", (long long)s->cells);
+    }
+    const unsigned gc = blocks_for(s->cells, KTPB);
+    for (int q = 0; q < 2; ++q)
+        if (src[q]) k_source_max<<<gc, KTPB, 0, g_stream>>>((float*)dst[q]->d, (const float*)src[q]->d, (int)s->cells);
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
 }

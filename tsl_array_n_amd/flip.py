@@ -181,8 +181,19 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--storage", default="f32")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy a 512 MiB field three times (the known-size kernel that scales FETCH_SIZE / "
+                         "WRITE_SIZE counter readings to bytes)")
     a = ap.parse_args(argv)
     init(0)
+    if a.copy_calibration:
+        from . import array_n
+        src = array_n("float", (128 << 20,)).fill(1.0)
+        dst = array_n("float", (128 << 20,))
+        for _ in range(3):
+            dst.copy_from(src)
+        sync()
+        del src, dst
     t0 = time.perf_counter()
     s, p = two_phase_dam_break(a.n, storage=a.storage)
     print("flip: %d^3, %d particles (%d liquid), setup %.1f s" % (a.n, p["count"], p["liquid"],

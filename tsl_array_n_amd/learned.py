@@ -312,18 +312,80 @@ def train(F, n, pairs, steps, lr=1e-3, seed=0, log=None):
     return net, losses
 
 
+def evaluate(F, train_size=64, pairs=24, steps=600, size=256, reps=3, levels=6, storage="f16", log=print):
+    """Train on MGPCG pairs at train_size with the energy-norm loss, then
+    solve the size^3 two-phase dam break (the benchmark's system) with the
+    network's guess as a coarse vector (BF16 inference at size^3) against the
+    cold start and the free-cell correction alone; returns a table of
+    iterations and ms per projection (inference included)."""
+    import time
+    from . import scenes
+    t0 = time.perf_counter()
+    data = energy_pairs(F, train_size, pairs, seed=11)
+    net, losses = train_energy(F, train_size, data, steps, lr=1e-3, log=log)
+    log("trained on %d pairs at %d^3, %d steps, %.1f s; energy loss %.3f -> %.3f"
+        % (len(data), train_size, steps, time.perf_counter() - t0, float(np.mean(losses[:len(data)])),
+           float(np.mean(losses[-len(data):]))))
+    params = net.get()
+    del data
+    big = UNet((size,) * 3, precision="bf16", params=params)
+    sc = scenes.dam_break_system(size)
+    shape = sc["shape"]
+    beta = [array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)]
+    act = array_n("u8", shape, sc["active"])
+    tgt = array_n("float", shape, sc["target"])
+    vin = [array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+    vout = [array_n("float", F.face_shape(shape, a)) for a in range(3)]
+    table = {}
+    for mode in ("cold", "free", "learned", "learned+start"):
+        ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
+                                            face_weights=beta, multigrid={"levels": levels, "storage": storage},
+                                            tolerance=1e-6)
+        if mode == "free":
+            ps.set_coarse_space(True, [])
+        proj = CoarseProjector(ps, big, 1000.0, start=mode.endswith("start")) if mode.startswith("learned") else ps
+        ps.pressure.fill(0.0)
+        d = proj.project(vin, vout)
+        times, its = [], []
+        for _ in range(reps):
+            ps.pressure.fill(0.0)
+            F.sync()
+            t = time.perf_counter()
+            d = proj.project(vin, vout)
+            F.sync()
+            times.append((time.perf_counter() - t) * 1e3)
+            its.append(d["iterations"])
+        table[mode] = {"iterations": int(np.median(its)), "ms": round(float(np.median(times)), 2),
+                       "converged": d["converged"], "fp64_relative_residual": d["residual"] / d["bNorm"]}
+        log("%s: %s" % (mode, table[mode]))
+        del ps, proj
+    return table
+
+
 def main(argv=None):
     """time the network's forward: python -m tsl_array_n_amd.learned
     --size 128 --precision bf16 --reps 20 (random He-uniform weights, random
-    input); prints one line with the time per forward and the FLOP rate"""
+    input); prints one line with the time per forward and the FLOP rate.
+    With --evaluate: train with the energy-norm loss at --train-size and
+    report the learned coarse vector at --size (evaluate())."""
     import argparse
+    import json
     from . import init, timer_start, timer_stop
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--precision", default="bf16", choices=["f32", "bf16"])
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--evaluate", action="store_true")
+    ap.add_argument("--train-size", type=int, default=64)
+    ap.add_argument("--pairs", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=600)
     a = ap.parse_args(argv)
     init(0)
+    if a.evaluate:
+        import tsl_array_n_amd as F
+        table = evaluate(F, a.train_size, a.pairs, a.steps, a.size, reps=3, levels=max(2, int(np.log2(a.size)) - 2))
+        print(json.dumps(table), flush=True)
+        return
     n = (a.size,) * 3
     net = UNet(n, precision=a.precision)
     rng = np.random.default_rng(0)

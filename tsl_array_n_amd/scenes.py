@@ -247,3 +247,22 @@ def flip_dam_break(n, ppa=2, box_max=(0.4, 0.6, 1.0), jitter=0.2, seed=0):
         conc[sl] = liquid_xy.astype(np.float32)
     return {"n": (n, n, n), "spacing": (h, h, h), "positions": pos.ravel(), "concentration": conc, "count": count,
             "liquid": int(conc.sum()), "ratio": 1000.0}
+
+
+def smoke_plume_sources(n, radius, center=None, noise=0.01, seed=0):
+    """The 3-D smoke plume's per-cell sources on an n^3 grid of unit cells:
+    a ball of `radius` cells (cell centres (i + 0.5, j + 0.5, k + 0.5),
+    strictly inside, the example's test in f32) at the bottom centre
+    (n/2, radius + 1.5, n/2) by default, density 1 and temperature
+    1 + U(-noise, noise) per cell from `seed`; 0 outside."""
+    c = np.array(center if center is not None else (n / 2.0, radius + 1.5, n / 2.0), np.float32)
+    x = np.arange(n, dtype=np.float32) + np.float32(0.5)
+    dx = (x - c[0])[None, None, :]
+    dy = (x - c[1])[None, :, None]
+    dz = (x - c[2])[:, None, None]
+    inside = ((dx * dx + dy * dy) + dz * dz) < np.float32(radius) * np.float32(radius)
+    rng = np.random.default_rng(seed)
+    jitter = rng.uniform(-noise, noise, (n, n, n)).astype(np.float32)
+    dens = np.where(inside, np.float32(1.0), np.float32(0.0)).astype(np.float32)
+    temp = np.where(inside, np.float32(1.0) + jitter, np.float32(0.0)).astype(np.float32)
+    return dens.ravel(), temp.ravel()

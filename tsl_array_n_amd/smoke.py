@@ -87,6 +87,12 @@ class GridSmokeSolver3:
         _check(_lib.flux_smoke3_source_sphere(C.c_void_p(self.h), c, C.c_float(radius), C.c_float(density),
                                               C.c_float(temperature)), "source")
 
+    def source_field(self, density=None, temperature=None):
+        """state = max(state, source) per cell for the current density and/or
+        temperature (float Fields of the grid's cells, None leaves one alone)"""
+        _check(_lib.flux_smoke3_source_field(C.c_void_p(self.h), C.c_void_p(density.h if density else None),
+                                             C.c_void_p(temperature.h if temperature else None)), "source")
+
     def on_advance_time_step(self):
         dg = PressureDiag()
         _check(_lib.flux_smoke3_step(C.c_void_p(self.h), C.byref(dg)), "onAdvanceTimeStep")
