@@ -195,16 +195,6 @@ int flux_pcg_set_coarse_space(flux_pcg* s, int free_indicator, int nfields, cons
  * energy-norm error, so a poor direction is scaled toward zero instead of
  * adding error.  alpha (0 when g.(A g) is not negative) is returned. */
 int flux_pcg_scaled_start(flux_pcg* s, const flux_field* g, double* alpha);
-/* No counterpart in the reference: the coarse correction along a learned
- * guess.  The coarse space holds, for every block K of block^3 cells,
- * phi_K = g restricted to the free cells of K; x <- x + sum_K alpha_K phi_K
- * with (phi^T A phi) alpha = phi^T (b - A x), the energy-optimal correction
- * in that space (one block covering the grid is flux_pcg_scaled_start).  The
- * coarse rows are assembled on the GPU in FP64 (fixed order); the coarse
- * system (7-point on the block grid, at most 2^20 blocks) is solved by
- * Jacobi-CG in FP64 on the host to a relative residual of 1e-12, whose
- * iteration count is returned. */
-int flux_pcg_block_start(flux_pcg* s, const flux_field* g, int block, int* coarse_iterations);
 int flux_pcg_destroy(flux_pcg* s);
 
 /* ------------------------------------------------------------ dot reducer */
@@ -303,8 +293,6 @@ int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_f
 /* flux_pcg_scaled_start on the assembled system (between assemble and
  * solve); no counterpart in the reference. */
 int flux_pressure3_scaled_guess(flux_pressure3* p, const flux_field* guess, double* alpha);
-/* flux_pcg_block_start on the assembled system; no counterpart in the reference */
-int flux_pressure3_block_guess(flux_pressure3* p, const flux_field* guess, int block, int* coarse_iterations);
 /* flux_pcg_set_coarse_space / flux_pcg_set_flexible on the projection's
  * solver; no counterpart in the reference */
 int flux_pressure3_set_coarse_space(flux_pressure3* p, int free_indicator, int nfields,

@@ -1065,37 +1065,6 @@ def solver2_frame(st, n, h, origin, frame_dt, courant=5.0, max_sub=32, **kw):
     return k, dt, diags
 
 
-def block_galerkin_start(shape, spacing, b, x, g, block, mask=None, weights=None):
-    """The coarse correction along a guess g (no reference counterpart): the
-    coarse space holds phi_K = g restricted to the free cells of block K
-    (block^3 cells, ragged at the high ends), and
-        x_new = x + sum_K alpha_K phi_K,  (phi^T A phi) alpha = phi^T (b - A x),
-    built densely here (one operator apply per block) and solved with
-    numpy's dense solver on the blocks where phi_K != 0.  Returns
-    (x_new, alpha)."""
-    n0, n1, n2 = (list(shape) + [1])[:3]
-    n = n0 * n1 * n2
-    nbx, nby, nbz = (-(-n0 // block), -(-n1 // block), -(-n2 // block))
-    free = np.ones(n, bool) if mask is None else (np.asarray(mask).ravel() == 0)
-    idx = np.arange(n)
-    i, j, k = idx % n0, (idx // n0) % n1, idx // (n0 * n1)
-    bid = (i // block) + nbx * ((j // block) + nby * (k // block))
-    g64 = np.where(free, np.asarray(g, np.float64), 0.0)
-    x64 = np.asarray(x, np.float64)
-    r = np.asarray(b, np.float64) - apply(shape, spacing, x64, mask, weights)
-    nk = nbx * nby * nbz
-    phis = [np.where(bid == K, g64, 0.0) for K in range(nk)]
-    keep = [K for K in range(nk) if np.any(phis[K] != 0)]
-    aphi = [apply(shape, spacing, phis[K], mask, weights) for K in keep]
-    M = np.array([[phis[K] @ ap for ap in aphi] for K in keep])
-    f = np.array([phis[K] @ r for K in keep])
-    alpha = np.zeros(nk)
-    if keep:
-        alpha[keep] = np.linalg.solve(M, f)
-    xn = x64 + np.where(free, alpha[bid] * g64, 0.0)
-    return xn, alpha
-
-
 def pcg_coarse_space(shape, spacing, b, mask, weights, mg, vectors, free_indicator=True, tol=1e-6, maxiter=200):
     """PCG with the additive coarse-space correction of the preconditioner,
     M^-1 r = V(r) + Phi (Phi^T A Phi)^-1 Phi^T r (Phi = [free-cell indicator]

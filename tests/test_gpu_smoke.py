@@ -97,10 +97,11 @@ def test_plume_matches_oracle(flux, n, steps):
 
 def test_plume_128_200_steps(flux):
     """The plume at 128^3 for its 200 steps on the GPU: every projection
-    converges to the reference's stop (|r| <= 1e-6 max(|b|, 1)); the
-    first-order semi-Lagrangian transport keeps density in [0, 1] and
-    temperature in [0, 1.01] (trilinear interpolation is a convex
-    combination and decay only shrinks); the plume rises; and the 200th step,
+    converges to the reference's stop (|r| <= 1e-6 max(|b|, 1)); density and
+    temperature stay within a few percent of [0, 1] and [0, 1.01] (the
+    scalar sampler is the reference's cubic whose slopes are limited in sign
+    only, grid_math3.js:137-207, so it overshoots a little where the plume's
+    edge is steep; it is not a convex combination); the plume rises; and the 200th step,
     taken by the oracle from the GPU's state after step 199, agrees with the
     GPU's own to the same 1e-4 as the early steps."""
     import time
@@ -123,12 +124,14 @@ def test_plume_128_200_steps(flux):
     ms = (time.perf_counter() - t0) * 1e3 / 199
     dens = s.density.to_array()
     temp = s.temperature.to_array()
-    assert np.all(np.isfinite(dens)) and np.all(np.isfinite(temp))
-    assert dens.min() >= 0.0 and dens.max() <= 1.0
-    assert temp.min() >= 0.0 and temp.max() <= 1.01
     cy = float(np.sum(dens.reshape(n, n, n) * (np.arange(n) + 0.5)[None, :, None]) / dens.sum())
-    print("plume 128^3: %.2f ms/step, iterations min %d median %d max %d, density centroid y %.1f (source %.1f)"
-          % (ms, min(its), int(np.median(its)), max(its), cy, src_y), flush=True)
+    print("plume 128^3: %.2f ms/step, iterations min %d median %d max %d, density centroid y %.1f (source %.1f), "
+          "density [%.4f, %.4f], temperature [%.4f, %.4f]"
+          % (ms, min(its), int(np.median(its)), max(its), cy, src_y, dens.min(), dens.max(), temp.min(), temp.max()),
+          flush=True)
+    assert np.all(np.isfinite(dens)) and np.all(np.isfinite(temp))
+    assert dens.min() >= -0.1 and dens.max() <= 1.1
+    assert temp.min() >= -0.1 and temp.max() <= 1.11
     assert cy > src_y + 4.0
     # the last step from the GPU's state, by the oracle
     st = {"u": s.field("u").to_array(), "v": s.field("v").to_array(), "w": s.field("w").to_array(),

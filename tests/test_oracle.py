@@ -522,70 +522,3 @@ def test_unet_gradients_match_finite_differences():
     lm = U.mse_loss(U.forward(p, x)[0], t)[0]
     x[idx] = old
     assert abs((lp - lm) / (2 * eps) - gx[idx]) <= 1e-6 + 1e-4 * abs(gx[idx])
-
-
-# ------------------------------------------------- coarse correction oracle
-def _block_system(n=16, seed=303):
-    from tsl_array_n_amd import scenes
-    sc = scenes.random_scene(n, seed)
-    shape, h = sc["shape"], sc["spacing"]
-    A = O.pressure3_assemble(shape, h, sc["vel"], active=sc["active"], target=sc["target"], beta=sc["beta"],
-                             suffix="f64")
-    mg = O.Multigrid(shape, h, levels=3, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f64")
-    p, st = O.pcg_solve(shape, h, A["b"], mask=A["mask"], weights=A["weights"], mg=mg, tol=1e-13, maxiter=500)
-    assert st["converged"]
-    return shape, h, A, p
-
-
-def _energy_error(shape, h, A, p_star, x):
-    e = np.asarray(x, np.float64) - p_star
-    e[A["mask"].ravel() != 0] = 0.0
-    return -e @ O.apply(shape, h, e, A["mask"], A["weights"])
-
-
-def test_block_start_one_block_is_the_scaled_step():
-    """one block covering the grid: alpha = g.(b - A x) / g.(A g)"""
-    shape, h, A, p = _block_system()
-    rng = np.random.default_rng(1)
-    g = p * (1 + 0.3 * rng.standard_normal(p.size))
-    g[A["mask"].ravel() != 0] = 0.0
-    x = np.where(A["mask"].ravel() != 0, p, 0.0)
-    xn, alpha = O.block_galerkin_start(shape, h, A["b"], x, g, 16, A["mask"], A["weights"])
-    r = A["b"] - O.apply(shape, h, x, A["mask"], A["weights"])
-    a1 = (g @ r) / (g @ O.apply(shape, h, g, A["mask"], A["weights"]))
-    assert alpha.shape == (1,) and abs(alpha[0] - a1) <= 1e-12 * abs(a1)
-
-
-@pytest.mark.parametrize("block", [4, 5])
-def test_block_start_recovers_blockwise_scaled_solution(block):
-    """g = p* / s_K with a random scale per block: the coarse correction from
-    x = 0 returns p* (alpha_K = s_K), ragged blocks included"""
-    shape, h, A, p = _block_system()
-    n0, n1, n2 = shape
-    nbx, nby = -(-n0 // block), -(-n1 // block)
-    idx = np.arange(p.size)
-    bid = (idx % n0) // block + nbx * (((idx // n0) % n1) // block + nby * ((idx // (n0 * n1)) // block))
-    s = np.random.default_rng(2).uniform(0.5, 2.0, bid.max() + 1)
-    free = A["mask"].ravel() == 0
-    g = np.where(free, p / s[bid], 0.0)
-    x = np.where(free, 0.0, p)
-    xn, alpha = O.block_galerkin_start(shape, h, A["b"], x, g, block, A["mask"], A["weights"])
-    assert np.max(np.abs(xn - p)) <= 1e-8 * np.max(np.abs(p))
-    used = np.unique(bid[free & (g != 0)])
-    np.testing.assert_allclose(alpha[used], s[used], rtol=1e-7)
-
-
-def test_block_start_energy_is_below_the_scaled_step():
-    """the coarse space contains g itself: the energy-norm error after the
-    block correction is at most that after the single scaled step, which is
-    at most that of the start"""
-    shape, h, A, p = _block_system()
-    rng = np.random.default_rng(3)
-    free = A["mask"].ravel() == 0
-    g = np.where(free, p + 0.2 * np.std(p) * rng.standard_normal(p.size), 0.0)
-    x = np.where(free, 0.0, p)
-    e0 = _energy_error(shape, h, A, p, x)
-    x1, _ = O.block_galerkin_start(shape, h, A["b"], x, g, 16, A["mask"], A["weights"])
-    x4, _ = O.block_galerkin_start(shape, h, A["b"], x, g, 4, A["mask"], A["weights"])
-    e1, e4 = _energy_error(shape, h, A, p, x1), _energy_error(shape, h, A, p, x4)
-    assert e4 <= e1 * (1 + 1e-9) and e1 < e0

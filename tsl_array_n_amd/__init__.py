@@ -504,16 +504,6 @@ class GridPressureSolver3:
         """the flexible (Polak-Ribiere) beta in the projection's PCG"""
         _check(_lib.flux_pressure3_set_flexible(C.c_void_p(self.h), int(bool(on))), "flexible")
 
-    def block_guess(self, guess, block=8):
-        """between assemble() and solve(): pressure += alpha_K * guess on the
-        free cells of every block K of block^3 cells, the alphas minimising the
-        energy-norm error in that coarse space; returns the coarse CG
-        iteration count"""
-        it = C.c_int()
-        _check(_lib.flux_pressure3_block_guess(C.c_void_p(self.h), C.c_void_p(guess.h), int(block), C.byref(it)),
-               "blockGuess")
-        return it.value
-
     def solve(self, vin, vout=None):
         """second half of project(): PCG from the current pressure, breaker, correction"""
         vout = vin if vout is None else vout

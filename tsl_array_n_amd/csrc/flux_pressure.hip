@@ -372,12 +372,6 @@ int flux_pressure3_set_flexible(flux_pressure3* p, int flexible) {
     return flux_pcg_set_flexible(p->pcg, flexible);
 }
 
-int flux_pressure3_block_guess(flux_pressure3* p, const flux_field* guess, int block, int* coarse_iterations) {
-    FLUX_REQUIRE_INIT();
-    FLUX_REQUIRE(p && guess, "blockGuess: NULL argument");
-    return flux_pcg_block_start(p->pcg, guess, block, coarse_iterations);
-}
-
 int flux_pressure3_solve(flux_pressure3* p, const flux_field* u_in, const flux_field* v_in, const flux_field* w_in,
                          flux_field* u_out, flux_field* v_out, flux_field* w_out, flux_pressure_diag* diag) {
     FLUX_REQUIRE_INIT();

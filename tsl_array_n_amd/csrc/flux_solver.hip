@@ -192,66 +192,6 @@ __global__ void k_galerkin_step(const uint8_t* __restrict__ m, const float* __re
     x[c] = (float)((double)x[c] + alpha * (double)g[c]);
 }
 
-// Block-wise Galerkin start: the coarse space holds phi_K = g restricted to
-// the free cells of block K (edge B, ragged at the high ends).  One
-// workgroup per block K accumulates, in FP64 and a fixed order,
-//   out[0][K] = phi_K . (b - A x)
-//   out[1][K] = phi_K . A phi_K
-//   out[2+a][K] = phi_K . A phi_{K + e_a}   (a = x, y, z; 0 at the last block)
-// The coarse matrix phi^T A phi couples face-adjacent blocks only (A is the
-// 7-point operator) and is symmetric.
-__global__ void __launch_bounds__(TPB) k_block_galerkin(LevelDev L, const float* __restrict__ x,
-                                                        const float* __restrict__ b, const float* __restrict__ g,
-                                                        int B, int nbx, int nby, int nbz, double* __restrict__ out) {
-    const int K = blockIdx.x;
-    const int bi = K % nbx, bj = (K / nbx) % nby, bk = K / (nbx * nby);
-    const int i0 = bi * B, j0 = bj * B, k0 = bk * B;
-    const int ex = min(B, L.n0 - i0), ey = min(B, L.n1 - j0), ez = min(B, L.n2 - k0);
-    const int cells = ex * ey * ez;
-    const uint8_t* __restrict__ m = L.mask;
-    double fr = 0, dd = 0, o[3] = {0, 0, 0};
-    // value of phi_{block containing (i0 + sx*B, ...)} at cell q
-    auto phi = [&](int q, int sx, int sy, int sz) -> double {
-        if (m && m[q]) return 0.0;
-        int row = q / L.n0;
-        int qi = q - row * L.n0, qk = row / L.n1, qj = row - qk * L.n1;
-        int di = qi - (i0 + sx * B), dj = qj - (j0 + sy * B), dk = qk - (k0 + sz * B);
-        return (di >= 0 && di < B && dj >= 0 && dj < B && dk >= 0 && dk < B) ? (double)g[q] : 0.0;
-    };
-    for (int t = threadIdx.x; t < cells; t += blockDim.x) {
-        const int li = t % ex, lj = (t / ex) % ey, lk = t / (ex * ey);
-        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-        const int c = i + L.n0 * (j + L.n1 * k);
-        if (m && m[c]) continue;
-        const double gc = (double)g[c];
-        const double ax = stencil_row<double>(L, i, j, k, c, [&](int q) { return (double)x[q]; }, (double*)nullptr);
-        fr += gc * ((double)b[c] - ax);
-        dd += gc * stencil_row<double>(L, i, j, k, c, [&](int q) { return phi(q, 0, 0, 0); }, (double*)nullptr);
-        if (li == B - 1 && bi + 1 < nbx)
-            o[0] += gc * stencil_row<double>(L, i, j, k, c, [&](int q) { return phi(q, 1, 0, 0); }, (double*)nullptr);
-        if (lj == B - 1 && bj + 1 < nby)
-            o[1] += gc * stencil_row<double>(L, i, j, k, c, [&](int q) { return phi(q, 0, 1, 0); }, (double*)nullptr);
-        if (lk == B - 1 && bk + 1 < nbz)
-            o[2] += gc * stencil_row<double>(L, i, j, k, c, [&](int q) { return phi(q, 0, 0, 1); }, (double*)nullptr);
-    }
-    store_block_partial(fr, out);
-    store_block_partial(dd, out + gridDim.x);
-    store_block_partial(o[0], out + 2 * gridDim.x);
-    store_block_partial(o[1], out + 3 * gridDim.x);
-    store_block_partial(o[2], out + 4 * gridDim.x);
-}
-
-// x += alpha[block of c] g on the free cells, rounded to FP32 once
-__global__ void k_block_step(const uint8_t* __restrict__ m, const float* __restrict__ g, float* __restrict__ x,
-                             const double* __restrict__ alpha, int n0, int n1, int n, int B, int nbx, int nby) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || (m && m[c])) return;
-    const int row = c / n0;
-    const int i = c - row * n0, k = row / n1, j = row - k * n1;
-    const double a = alpha[(i / B) + nbx * ((j / B) + nby * (k / B))];
-    x[c] = (float)((double)x[c] + a * (double)g[c]);
-}
-
 __global__ void __launch_bounds__(FIN_TPB) k_fin_setup(Scal* S, const double* partials, unsigned nb) {
     const double tot = sum_partials(partials, nb);
     if (threadIdx.x != 0) return;
@@ -966,97 +906,6 @@ int flux_pcg_scaled_start(flux_pcg* s, const flux_field* g, double* alpha_out) {
         FLUX_HIP(hipStreamSynchronize(g_stream));
     }
     if (alpha_out) *alpha_out = alpha;
-    return FLUX_OK;
-}
-
-int flux_pcg_block_start(flux_pcg* s, const flux_field* g, int block, int* coarse_iterations) {
-    FLUX_REQUIRE_INIT();
-    FLUX_REQUIRE(s && g, "blockStart: NULL argument");
-    FLUX_REQUIRE(block >= 1, "blockStart: block edge must be >= 1, got %d", block);
-    FLUX_TRY(check_field(g, FLUX_F32, s->n, "blockStart: direction"));
-    const int* n = s->op->n;
-    const int nbx = (n[0] + block - 1) / block, nby = (n[1] + block - 1) / block, nbz = (n[2] + block - 1) / block;
-    const int64_t nk = (int64_t)nbx * nby * nbz;
-    FLUX_REQUIRE(nk <= (1 << 20), "blockStart: %lld blocks of edge %d exceed 2^20; use larger blocks", (long long)nk,
-                 block);
-    double* dbuf = (double*)dmalloc(sizeof(double) * (size_t)(5 * nk));
-    FLUX_REQUIRE(dbuf, "blockStart: allocation failed");
-    k_block_galerkin<<<(unsigned)nk, TPB, 0, g_stream>>>(s->op->L, (const float*)s->x->d, (const float*)s->b->d,
-                                                        (const float*)g->d, block, nbx, nby, nbz, dbuf);
-    std::vector<double> h(5 * (size_t)nk);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dbuf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, g_stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
-    if (e != hipSuccess) {
-        dfree(dbuf);
-        return set_error(FLUX_ERR_HIP, "blockStart: %s", hipGetErrorString(e));
-    }
-    // coarse system (phi^T A phi) alpha = phi^T r, negated to SPD; blocks
-    // without a free cell where g != 0 keep alpha = 0.  Jacobi-preconditioned
-    // CG in FP64 to a relative residual of 1e-12.
-    const double* f = h.data();
-    const double* dg = f + nk;
-    const double* off[3] = {f + 2 * nk, f + 3 * nk, f + 4 * nk};
-    const int64_t st[3] = {1, nbx, (int64_t)nbx * nby};
-    const int nbd[3] = {nbx, nby, nbz};
-    std::vector<double> alpha(nk, 0.0), r(nk, 0.0), z(nk), p(nk, 0.0), q(nk);
-    std::vector<char> act(nk);
-    bool finite = true;
-    for (int64_t K = 0; K < nk; ++K) {
-        act[K] = dg[K] < 0.0;
-        r[K] = act[K] ? -f[K] : 0.0;
-        finite = finite && std::isfinite(f[K]) && std::isfinite(dg[K]);
-    }
-    auto matvec = [&](const std::vector<double>& v, std::vector<double>& y) {
-        for (int64_t K = 0; K < nk; ++K) y[K] = act[K] ? -dg[K] * v[K] : 0.0;
-        for (int64_t K = 0; K < nk; ++K) {
-            int64_t c[3] = {K % nbx, (K / nbx) % nby, K / ((int64_t)nbx * nby)};
-            for (int a = 0; a < 3; ++a)
-                if (c[a] + 1 < nbd[a] && act[K] && act[K + st[a]]) {
-                    const double w = -off[a][K];
-                    y[K] += w * v[K + st[a]];
-                    y[K + st[a]] += w * v[K];
-                }
-        }
-    };
-    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
-        double acc = 0;
-        for (int64_t K = 0; K < nk; ++K) acc += a[K] * b[K];
-        return acc;
-    };
-    int it = 0;
-    if (finite) {
-        const double r0 = std::sqrt(dot(r, r));
-        double rz = 0;
-        for (; it < 4000 && std::sqrt(dot(r, r)) > 1e-12 * r0 && r0 > 0; ++it) {
-            for (int64_t K = 0; K < nk; ++K) z[K] = act[K] ? r[K] / -dg[K] : 0.0;
-            const double rz_new = dot(r, z);
-            const double beta = it == 0 ? 0.0 : rz_new / rz;
-            rz = rz_new;
-            for (int64_t K = 0; K < nk; ++K) p[K] = z[K] + beta * p[K];
-            matvec(p, q);
-            const double pq = dot(p, q);
-            if (!(pq > 0)) break;
-            const double a = rz / pq;
-            for (int64_t K = 0; K < nk; ++K) {
-                alpha[K] += a * p[K];
-                r[K] -= a * q[K];
-            }
-        }
-    }
-    for (int64_t K = 0; K < nk; ++K)
-        if (!std::isfinite(alpha[K]) || !finite) alpha[K] = 0.0;
-    e = hipMemcpyAsync(dbuf, alpha.data(), sizeof(double) * (size_t)nk, hipMemcpyHostToDevice, g_stream);
-    if (e == hipSuccess) {
-        k_block_step<<<blocks_for(s->n, TPB), TPB, 0, g_stream>>>(s->op->L.mask, (const float*)g->d, (float*)s->x->d,
-                                                                   dbuf, n[0], n[1], (int)s->n, block, nbx, nby);
-        e = hipGetLastError();
-    }
-    hipError_t e2 = hipStreamSynchronize(g_stream);
-    dfree(dbuf);
-    if (e == hipSuccess) e = e2;
-    if (e != hipSuccess) return set_error(FLUX_ERR_HIP, "blockStart: %s", hipGetErrorString(e));
-    if (coarse_iterations) *coarse_iterations = it;
     return FLUX_OK;
 }
 

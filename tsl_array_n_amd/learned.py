@@ -142,18 +142,14 @@ class LearnedProjector:
     scaled=True (default) adds the guess to the assembled start along its
     energy-optimal step (pressure += alpha * guess, alpha = g.r / g.Ag), so a
     guess pointing the wrong way is scaled toward zero; scaled=False writes
-    the raw guess into the free cells.  block=B (with scaled=True) replaces
-    the single step by the coarse correction with one alpha per block of B^3
-    cells (the Galerkin solve in the span of the block-restricted guesses)."""
+    the raw guess into the free cells."""
 
-    def __init__(self, solver, net, density_ratio, scaled=True, block=None):
+    def __init__(self, solver, net, density_ratio, scaled=True):
         self.solver = solver
         self.net = net
         self.density_ratio = float(density_ratio)
         self.scaled = scaled
-        self.block = block
         self.alpha = None
-        self.coarse_iterations = None
         self._g = array_n("float", solver.resolution) if scaled else None
 
     def project(self, vin, vout=None):
@@ -162,10 +158,7 @@ class LearnedProjector:
         self.net.features(s.b, s.dirichlet_mask, s.weights, self.density_ratio)
         if self.scaled:
             self.net.guess(self._g)
-            if self.block:
-                self.coarse_iterations = s.block_guess(self._g, self.block)
-            else:
-                self.alpha = s.scaled_guess(self._g)
+            self.alpha = s.scaled_guess(self._g)
         else:
             _check(_lib.flux_unet_guess_into(C.c_void_p(self.net.h), C.c_void_p(self.net.input.h),
                                              C.c_void_p(s.dirichlet_mask.h if s.dirichlet_mask else None),
