@@ -126,6 +126,38 @@ def test_p2g_fixed_point_bit_exact(flux, T, comp):
     np.testing.assert_array_equal(out.to_array(), ro)
 
 
+@pytest.mark.parametrize("comp", [0, 1])
+def test_p2g_block_box_and_direct_atomics_are_bit_identical(flux, T, comp, monkeypatch):
+    """The scatter sums a block's corners in LDS when they fit a small box
+    (particles in seeded order) and adds them to the grid directly otherwise
+    (the same particles shuffled, or FLUX_P2G_BOX=0): integer sums in any
+    order, so all three give the oracle's bits."""
+    F = flux
+    n = (48, 40, 32)
+    h = (1.0 / 48,) * 3
+    pos = O.flip_box_seed3([0, 0, 0], [1.0, 1.0, 0.5], list(h), ppa=2, jitter=0.2)
+    cnt = pos.size // 3
+    rng = np.random.default_rng(20 + comp)
+    vel = (rng.standard_normal(3 * cnt) * 2).astype(np.float32)
+    perm = rng.permutation(cnt)
+    shuffled_pos = pos.reshape(-1, 3)[perm].ravel()
+    shuffled_vel = vel.reshape(-1, 3)[perm].ravel()
+    g = T.Grid(n, h)
+    fs = g.face_shape(comp)
+    _, rv, rn, rd = O.p2g_fixed3(pos, vel, n, h, [0, 0, 0], comp)
+    for name, p_, v_, box in (("ordered", pos, vel, "1"), ("shuffled", shuffled_pos, shuffled_vel, "1"),
+                              ("direct", pos, vel, "0")):
+        monkeypatch.setenv("FLUX_P2G_BOX", box)
+        P = F.array_n("vec3", cnt, p_)
+        V = F.array_n("vec3", cnt, v_)
+        numer, denom = F.array_n("int", fs), F.array_n("int", fs)
+        out, valid = F.array_n("float", fs), F.array_n("int", fs)
+        T.p2g_fixed(g, P, V, cnt, comp, numer, denom, out, valid)
+        np.testing.assert_array_equal(numer.to_array(), rn, err_msg=name)
+        np.testing.assert_array_equal(denom.to_array(), rd, err_msg=name)
+        np.testing.assert_array_equal(valid.to_array(), rv, err_msg=name)
+
+
 def _random_mac(n, rng, scale=1.0):
     return [(rng.standard_normal(int(np.prod(O.face_shape(n, a)))) * scale).astype(np.float32) for a in range(3)]
 

@@ -149,3 +149,33 @@ def test_split_k_matches_unsplit(flux):
     print("split vs unsplit rel %.2e" % rel(outs[1], outs[0]))
     assert rel(outs[1], outs[0]) <= 2e-3
     np.testing.assert_array_equal(outs[2], outs[1])
+
+
+@pytest.mark.parametrize("n", [(64, 32, 32), (40, 24, 16)])
+def test_forced_tiles_are_bit_identical(flux, n, monkeypatch):
+    """the flat kernel's wide tiles (MT rows of 16 voxels x NT columns of 16
+    channels) sum each output's K terms in the order of the 1x1 tile, so
+    every forced tile gives the 1x1 bits, ragged voxel tails (40x24x16:
+    30 voxels at the coarsest level) and capped widths included; split K
+    and the halo kernel are off on every side"""
+    from tsl_array_n_amd import learned as L
+    p = {k: v.astype(np.float32) for k, v in random_params(13).items()}
+    x = np.random.default_rng(14).standard_normal(3 * int(np.prod(n))).astype(np.float32)
+    monkeypatch.setenv("FLUX_UNET_SPLIT", "0")
+    monkeypatch.setenv("FLUX_UNET_HALO", "0")
+    outs = {}
+    for tile in ("1x1", "4x4", "4x2", "2x1", "2x4", "1x2"):
+        monkeypatch.setenv("FLUX_UNET_TILE", tile)
+        net = L.UNet(n, params=p, precision="bf16")
+        outs[tile] = net.forward(x).to_array()
+        del net
+    assert np.all(np.isfinite(outs["1x1"])) and np.any(outs["1x1"] != 0)
+    for tile, y in outs.items():
+        np.testing.assert_array_equal(y, outs["1x1"], err_msg=tile)
+
+
+def test_bad_forced_tile_is_rejected(flux, monkeypatch):
+    from tsl_array_n_amd import learned as L
+    monkeypatch.setenv("FLUX_UNET_TILE", "3x1")
+    with pytest.raises(Exception, match="tile sides"):
+        L.UNet((16, 16, 16), precision="bf16")

@@ -131,14 +131,102 @@ __device__ __forceinline__ int f2i_sat(float v) {
 }
 
 // accumulateConcentration (grid_flip_solver3.js:992-998): count and the
-// fixed-point sum of clamp(c, 0, 1) per cell (integer atomics, order-free)
-__global__ void k_conc_accumulate(G3 g, const float* __restrict__ pos, const float* __restrict__ conc, int count,
-                                  float scale, int* __restrict__ accum, int* __restrict__ cnt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
-    const int c = cell_of(g, pos, p);
-    atomicAdd(&cnt[c], 1);
-    atomicAdd(&accum[c], f2i_sat(rintf(clamp01(conc[p]) * scale)));
+// fixed-point sum of clamp(c, 0, 1) per cell (integer atomics, order-free).
+// A block takes kCellPPT * STPB consecutive particles; when their cells lie in
+// a box of at most kCellBox cells (particles in spatial order) both sums are
+// formed in LDS and added to the grid once per touched cell, otherwise every
+// particle adds to the grid directly.  Integer sums in any order give the
+// same bits.
+static constexpr int kCellPPT = 4;
+static constexpr int kCellBox = 4096;
+
+__device__ __forceinline__ void cell_coords(const G3& g, const float* __restrict__ pos, int64_t p, int c[3]) {
+    const float x = (pos[3 * p + 0] - g.o0) / g.h0;
+    const float y = (pos[3 * p + 1] - g.o1) / g.h1;
+    const float z = (pos[3 * p + 2] - g.o2) / g.h2;
+    c[0] = clampi((int)floorf(x), 0, g.n0 - 1);
+    c[1] = clampi((int)floorf(y), 0, g.n1 - 1);
+    c[2] = clampi((int)floorf(z), 0, g.n2 - 1);
+}
+
+__global__ void __launch_bounds__(STPB) k_conc_accumulate(G3 g, const float* __restrict__ pos,
+                                                         const float* __restrict__ conc, int count, float scale,
+                                                         int* __restrict__ accum, int* __restrict__ cnt) {
+    __shared__ int s_acc[kCellBox];
+    __shared__ int s_cnt[kCellBox];
+    __shared__ int s_lo[3], s_hi[3];
+    const int64_t first = (int64_t)blockIdx.x * (STPB * kCellPPT);
+    if (threadIdx.x < 3) {
+        s_lo[threadIdx.x] = 0x7fffffff;
+        s_hi[threadIdx.x] = -1;
+    }
+    int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+#pragma unroll
+    for (int q = 0; q < kCellPPT; ++q) {
+        const int64_t p = first + q * STPB + threadIdx.x;
+        if (p < count) {
+            int c[3];
+            cell_coords(g, pos, p, c);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], c[a]);
+                hi[a] = max(hi[a], c[a]);
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], d));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], d));
+        }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&s_lo[a], lo[a]);
+            atomicMax(&s_hi[a], hi[a]);
+        }
+    __syncthreads();
+    const int b0 = s_lo[0], b1 = s_lo[1], b2 = s_lo[2];
+    const int bx = s_hi[0] - b0 + 1, by = s_hi[1] - b1 + 1, bz = s_hi[2] - b2 + 1;
+    if (bx <= 0) return;  // no particle in this block (uniform)
+    const int64_t vol = (int64_t)bx * by * bz;
+    const bool lds = vol <= kCellBox;  // uniform over the block
+    if (lds) {
+        for (int e = threadIdx.x; e < (int)vol; e += STPB) {
+            s_acc[e] = 0;
+            s_cnt[e] = 0;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kCellPPT; ++q) {
+        const int64_t p = first + q * STPB + threadIdx.x;
+        if (p >= count) continue;
+        int c[3];
+        cell_coords(g, pos, p, c);
+        const int v = f2i_sat(rintf(clamp01(conc[p]) * scale));
+        if (lds) {
+            const int e = (c[0] - b0) + bx * ((c[1] - b1) + by * (c[2] - b2));
+            atomicAdd(&s_cnt[e], 1);
+            atomicAdd(&s_acc[e], v);
+        } else {
+            const int cell = c[0] + g.n0 * (c[1] + g.n1 * c[2]);
+            atomicAdd(&cnt[cell], 1);
+            atomicAdd(&accum[cell], v);
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (int e = threadIdx.x; e < (int)vol; e += STPB) {
+        const int nc = s_cnt[e], v = s_acc[e];
+        if (nc == 0) continue;  // no particle: both sums stay 0
+        const int cell = (b0 + e % bx) + g.n0 * ((b1 + (e / bx) % by) + g.n1 * (b2 + e / (bx * by)));
+        atomicAdd(&cnt[cell], nc);
+        if (v) atomicAdd(&accum[cell], v);
+    }
 }
 
 // resolveCellConcentration (1000-1012)
@@ -415,7 +503,7 @@ int accumulate_concentration(flux_flip3* s) {
     FLUX_TRY(fill_device(s->conc_accum, FLUX_I32, nc, 0));
     FLUX_TRY(fill_device(s->conc_count, FLUX_I32, nc, 0));
     if (np > 0)
-        k_conc_accumulate<<<blocks_for(np, STPB), STPB, 0, g_stream>>>(
+        k_conc_accumulate<<<blocks_for(np, STPB * kCellPPT), STPB, 0, g_stream>>>(
             make_g3(&s->g), (const float*)s->positions->d, (const float*)s->conc->d, np,
             s->d.concentration_atomic_scale, s->conc_accum, s->conc_count);
     FLUX_LAUNCH_CHECK();

@@ -116,34 +116,125 @@ __device__ __forceinline__ int f32_to_i32_sat(float v) {
     return (int)v;
 }
 
-// buildScatter (grid_flip_solver3.js:508-535): int32 fixed-point atomics
-// With `conc` (massWeightedTransfer, grid_flip_solver3.js:493-531) every
-// corner weight is scaled by rho(c) / rho_ref, rho(c) = ma + (mb - ma) c.
-__global__ void k_p2g_scatter_fixed(G3 g, int comp, const float* __restrict__ pos, const float* __restrict__ vel,
-                                    int count, float scale, int* __restrict__ numer, int* __restrict__ denom,
-                                    const float* __restrict__ conc, float ma, float mb, float mref) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
-    float mass = 1.0f;
-    if (conc) {
-        const float c = fminf(fmaxf(conc[p], 0.0f), 1.0f);
-        mass = (ma + (mb - ma) * c) / mref;
-    }
+// buildScatter (grid_flip_solver3.js:508-535): every particle adds its 8
+// corner weights w and value * w, in int32 fixed point (rint(x * scale),
+// saturated), to the face accumulators; with `conc` (massWeightedTransfer,
+// grid_flip_solver3.js:493-531) every corner weight is scaled by
+// rho(c) / rho_ref, rho(c) = ma + (mb - ma) c.
+// Aggregated per block: a block takes kP2GPPT * XTPB
+// consecutive particles; when every corner they reach lies in a box of at
+// most `box_limit` faces (particles stored in spatial order: seeded, or
+// sorted, as they are in a FLIP scene), the fixed-point contributions are
+// first summed in LDS and the box is then added to the grid with one atomic
+// per touched face and array, instead of 16 atomics per particle.  Integer
+// addition is associative and commutative (wrapping like the atomics), so the
+// grid ends with the same bits as the per-corner atomics in any order.  A
+// block whose box is larger adds its corners to the grid directly.
+static constexpr int kP2GPPT = 4;
+static constexpr int kP2GBox = 4096;
+
+__global__ void __launch_bounds__(XTPB) k_p2g_scatter_box(G3 g, int comp, const float* __restrict__ pos,
+                                                          const float* __restrict__ vel, int count, float scale,
+                                                          int* __restrict__ numer, int* __restrict__ denom,
+                                                          const float* __restrict__ conc, float ma, float mb,
+                                                          float mref, int box_limit) {
+    __shared__ int s_num[kP2GBox];
+    __shared__ int s_den[kP2GBox];
+    __shared__ int s_lo[3], s_hi[3];
     const int s[3] = {g.n0 + (comp == 0), g.n1 + (comp == 1), g.n2 + (comp == 2)};
     float o[3];
     mac_origin(g, comp, o);
     const float h[3] = {g.h0, g.h1, g.h2};
-    const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-    Tri tr;
-    tri_weights(pp, o, h, s, tr);
-    float value = vel[3 * p + comp];
+    const int64_t first = (int64_t)blockIdx.x * (XTPB * kP2GPPT);
+    if (threadIdx.x < 3) {
+        s_lo[threadIdx.x] = 0x7fffffff;
+        s_hi[threadIdx.x] = -1;
+    }
+    // pass 1: the box of corners the block's particles reach
+    int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
 #pragma unroll
-    for (int cc = 0; cc < 8; ++cc) {
-        int i = (cc & 1) ? tr.i1 : tr.i0, j = (cc & 2) ? tr.j1 : tr.j0, k = (cc & 4) ? tr.k1 : tr.k0;
-        int f = i + s[0] * (j + s[1] * k);
-        float w = conc ? tr.w[cc] * mass : tr.w[cc];
-        atomicAdd(&numer[f], f32_to_i32_sat(rintf((value * w) * scale)));
-        atomicAdd(&denom[f], f32_to_i32_sat(rintf(w * scale)));
+    for (int q = 0; q < kP2GPPT; ++q) {
+        const int64_t p = first + q * XTPB + threadIdx.x;
+        if (p < count) {
+            const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+            Tri tr;
+            tri_weights(pp, o, h, s, tr);
+            lo[0] = min(lo[0], tr.i0);
+            lo[1] = min(lo[1], tr.j0);
+            lo[2] = min(lo[2], tr.k0);
+            hi[0] = max(hi[0], tr.i1);
+            hi[1] = max(hi[1], tr.j1);
+            hi[2] = max(hi[2], tr.k1);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], d));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], d));
+        }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&s_lo[a], lo[a]);
+            atomicMax(&s_hi[a], hi[a]);
+        }
+    __syncthreads();
+    const int b0 = s_lo[0], b1 = s_lo[1], b2 = s_lo[2];
+    const int bx = s_hi[0] - b0 + 1, by = s_hi[1] - b1 + 1, bz = s_hi[2] - b2 + 1;
+    if (bx <= 0) return;  // no particle in this block (uniform)
+    const int64_t vol = (int64_t)bx * by * bz;
+    const bool lds = vol <= (int64_t)min(box_limit, kP2GBox);  // uniform over the block
+    if (lds) {
+        for (int e = threadIdx.x; e < (int)vol; e += XTPB) {
+            s_num[e] = 0;
+            s_den[e] = 0;
+        }
+        __syncthreads();
+    }
+    // pass 2: the corners, into LDS or straight to the grid
+#pragma unroll
+    for (int q = 0; q < kP2GPPT; ++q) {
+        const int64_t p = first + q * XTPB + threadIdx.x;
+        if (p >= count) continue;
+        float mass = 1.0f;
+        if (conc) {
+            const float c = fminf(fmaxf(conc[p], 0.0f), 1.0f);
+            mass = (ma + (mb - ma) * c) / mref;
+        }
+        const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        Tri tr;
+        tri_weights(pp, o, h, s, tr);
+        const float value = vel[3 * p + comp];
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) {
+            const int i = (cc & 1) ? tr.i1 : tr.i0, j = (cc & 2) ? tr.j1 : tr.j0, k = (cc & 4) ? tr.k1 : tr.k0;
+            const float w = conc ? tr.w[cc] * mass : tr.w[cc];
+            const int nv = f32_to_i32_sat(rintf((value * w) * scale));
+            const int dv = f32_to_i32_sat(rintf(w * scale));
+            if (lds) {
+                const int e = (i - b0) + bx * ((j - b1) + by * (k - b2));
+                atomicAdd(&s_num[e], nv);
+                atomicAdd(&s_den[e], dv);
+            } else {
+                const int f = i + s[0] * (j + s[1] * k);
+                atomicAdd(&numer[f], nv);
+                atomicAdd(&denom[f], dv);
+            }
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    // adding 0 changes nothing: untouched and cancelled faces are skipped
+    for (int e = threadIdx.x; e < (int)vol; e += XTPB) {
+        const int nv = s_num[e], dv = s_den[e];
+        if ((nv | dv) == 0) continue;
+        const int i = b0 + e % bx, j = b1 + (e / bx) % by, k = b2 + e / (bx * by);
+        const int f = i + s[0] * (j + s[1] * k);
+        if (nv) atomicAdd(&numer[f], nv);
+        if (dv) atomicAdd(&denom[f], dv);
     }
 }
 
@@ -396,6 +487,13 @@ int flux_p2g_gather(const flux_grid3* g, const flux_field* positions, const flux
 
 namespace flux {
 
+// FLUX_P2G_BOX=0 sends every block's corners straight to the grid (the
+// per-corner atomics; the same bits, for comparison)
+static int p2g_box_limit() {
+    const char* e = getenv("FLUX_P2G_BOX");
+    return (e && e[0] == '0') ? 0 : kP2GBox;
+}
+
 int p2g_fixed_async(const flux_grid3* g, const flux_field* positions, const flux_field* velocities, int64_t count,
                     int component, float scale, float eps, flux_field* numer, flux_field* denom, flux_field* out,
                     flux_field* valid, const flux_field* conc, float d0, float d1) {
@@ -415,9 +513,10 @@ int p2g_fixed_async(const flux_grid3* g, const flux_field* positions, const flux
     FLUX_TRY(fill_device(denom->d, FLUX_I32, nf, 0));
     const float ma = std::max(d0, 1e-4f), mb = std::max(d1, 1e-4f);
     if (count > 0)
-        k_p2g_scatter_fixed<<<blocks_for(count, XTPB), XTPB, 0, g_stream>>>(
+        k_p2g_scatter_box<<<blocks_for(count, XTPB * kP2GPPT), XTPB, 0, g_stream>>>(
             make_g3(g), component, (const float*)positions->d, (const float*)velocities->d, (int)count, scale,
-            (int*)numer->d, (int*)denom->d, conc ? (const float*)conc->d : nullptr, ma, mb, std::max(ma, mb));
+            (int*)numer->d, (int*)denom->d, conc ? (const float*)conc->d : nullptr, ma, mb, std::max(ma, mb),
+            p2g_box_limit());
     k_p2g_finalize_fixed<<<blocks_for(nf, XTPB), XTPB, 0, g_stream>>>((int)nf, (const int*)numer->d,
                                                                         (const int*)denom->d, scale, eps,
                                                                         (float*)out->d, (int*)valid->d);

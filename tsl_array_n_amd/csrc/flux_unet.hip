@@ -13,6 +13,7 @@
 // tensor at (z/2, y/2, x/2)).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -296,7 +297,7 @@ struct LayerSpec {
     int cin, cout, relu, level;
 };
 
-static const LayerSpec kLayers[] = {
+static constexpr LayerSpec kLayers[] = {
     {"enc0a", 3, 16, 1, 0},  {"enc0b", 16, 16, 1, 0}, {"enc1a", 16, 32, 1, 1}, {"enc1b", 32, 32, 1, 1},
     {"enc2a", 32, 64, 1, 2}, {"enc2b", 64, 64, 1, 2}, {"mid", 64, 64, 1, 3},   {"dec2", 128, 64, 1, 2},
     {"dec1a", 96, 32, 1, 1}, {"dec1b", 32, 32, 1, 1}, {"dec0a", 48, 16, 1, 0}, {"dec0b", 16, 16, 1, 0},
@@ -304,6 +305,17 @@ static const LayerSpec kLayers[] = {
 };
 static constexpr int kNumLayers = 13;
 enum { L_E0A, L_E0B, L_E1A, L_E1B, L_E2A, L_E2B, L_MID, L_D2, L_D1A, L_D1B, L_D0A, L_D0B, L_OUT };
+
+// the BF16 convolution kernels are instantiated for output tiles of 1, 2 or
+// 4 column blocks of 16 channels: every layer's padded width must be one
+constexpr bool mfma_widths_supported() {
+    for (int l = 0; l < kNumLayers; ++l) {
+        const int ntn = (kLayers[l].cout + 15) / 16;
+        if (ntn != 1 && ntn != 2 && ntn != 4) return false;
+    }
+    return true;
+}
+static_assert(mfma_widths_supported(), "a layer's output channels pad to a width the MFMA kernels do not cover");
 
 int64_t unet_param_count() {
     int64_t n = 0;
@@ -722,7 +734,8 @@ struct flux_unet {
     int64_t version = 1, packed_version = 0;
     int halo = 1;  // halo-tiled convolutions: 0 never, 1 where measured faster, 2 wherever they fit
     int split = 1;                 // split K on small levels (FLUX_UNET_SPLIT=0 disables)
-    float* split_part = nullptr;   // FP32 partial sums of the K splits (kSplitPartBytes)
+    int tile_mt = 0, tile_nt = 0;  // forced flat-kernel tile (FLUX_UNET_TILE=MxN), 0 = chosen per layer
+    float* split_part = nullptr;   // FP32 partial sums of the K splits, sized for the layers that split
 };
 
 namespace {
@@ -820,6 +833,56 @@ MfmaIn mfma_input(const flux_unet* u, int l) {
 
 int mfma_channels(int l) { return l == L_E0A ? 8 : kLayers[l].cin; }
 
+// How one layer's BF16 convolution runs: the halo-tiled kernel, or the flat
+// kernel with an MT x NT tile (rows of 16 voxels x columns of 16 channels)
+// and K split `splits` ways.
+struct MfmaPlan {
+    bool halo;
+    int mt, nt, splits;
+    int64_t nv, halo_bytes, bricks;
+};
+
+// the widest tile (4 x N/16) that still gives >= 4096 wavefronts; smaller
+// levels trade tile reuse for parallelism
+MfmaPlan plan_mfma(const flux_unet* u, int l) {
+    const LayerSpec& s = kLayers[l];
+    const int nx = u->n[0] >> s.level, ny = u->n[1] >> s.level, nz = u->n[2] >> s.level;
+    MfmaPlan p{};
+    p.nv = (int64_t)nz * ny * nx;
+    const int ntn = u->npad[l] / 16;
+    p.halo_bytes = (int64_t)kHaloVox * (mfma_channels(l) / 8) * 16;
+    p.bricks = (int64_t)(nx / 16) * (ny / 4) * (nz / 4);
+    // measured: the halo form pays where the brick is small (<= 32 channels,
+    // <= 41 KiB of LDS, several blocks per CU) and the level has >= 1024 bricks
+    const bool fits = nx % 16 == 0 && ny % 4 == 0 && nz % 4 == 0 && p.halo_bytes <= 64 * 1024;
+    const bool pays = mfma_channels(l) <= 32 && p.bricks >= 1024;
+    p.halo = u->tile_mt == 0 && fits && (u->halo == 2 || (u->halo == 1 && pays));
+    p.mt = 4;
+    p.nt = ntn;
+    auto waves = [&]() { return (p.nv + 16 * p.mt - 1) / (16 * p.mt) * (ntn / p.nt); };
+    if (u->tile_mt) {
+        p.mt = u->tile_mt;
+        p.nt = std::min(u->tile_nt, ntn);
+    } else {
+        while (waves() < 4096 && (p.mt > 1 || p.nt > 1)) {
+            if (p.mt > 1) p.mt >>= 1;
+            else p.nt >>= 1;
+        }
+    }
+    // levels too small to fill the GPU (< 4096 wavefronts even with 1x1
+    // tiles): split K until 4096 run, each split keeping >= 8 K steps, while
+    // the FP32 partial sums stay within 32 MiB (measured: splitting levels
+    // that already have 4096 wavefronts only adds the finishing pass)
+    p.splits = 1;
+    const int chunks = u->groups[l] / 4;
+    while (!p.halo && u->split && waves() * p.splits < 4096 && p.splits < 8 && chunks / (2 * p.splits) >= 8 &&
+           (int64_t)2 * p.splits * p.nv * s.cout * 4 <= kSplitPartBytes)
+        p.splits *= 2;
+    // every split must own at least one K step
+    while (p.splits > 1 && (int64_t)(p.splits - 1) * ((chunks + p.splits - 1) / p.splits) >= chunks) p.splits >>= 1;
+    return p;
+}
+
 // allocate the packed weights and the channels-last activations
 int bf16_setup(flux_unet* u) {
     if (u->wq) return FLUX_OK;
@@ -839,8 +902,15 @@ int bf16_setup(flux_unet* u) {
         ok = ok && p;
     };
     alloc(u->wq, off);
-    u->split_part = (float*)dmalloc(kSplitPartBytes);
-    ok = ok && u->split_part;
+    int64_t split_bytes = 0;
+    for (int l = 0; l < kNumLayers; ++l) {
+        const MfmaPlan p = plan_mfma(u, l);
+        if (p.splits > 1) split_bytes = std::max<int64_t>(split_bytes, (int64_t)p.splits * p.nv * kLayers[l].cout * 4);
+    }
+    if (split_bytes > 0) {
+        u->split_part = (float*)dmalloc((size_t)split_bytes);
+        ok = ok && u->split_part;
+    }
     alloc(u->x16, 8 * u->nvox[0]);
     for (int l = 0; l < L_OUT; ++l) alloc(u->y16[l], (int64_t)kLayers[l].cout * u->nvox[kLayers[l].level]);
     alloc(u->pooled16[0], 16 * u->nvox[1]);
@@ -865,61 +935,40 @@ void launch_mfma(const MfmaIn& in, const uint16_t* wq, const float* b, void* out
                                                                            out_f32);
 }
 
-// the widest tile (4 x N/16) that still gives >= 4096 wavefronts; smaller
-// levels trade tile reuse for parallelism
-void conv_mfma(flux_unet* u, int l, void* out, int out_f32) {
+int conv_mfma(flux_unet* u, int l, void* out, int out_f32) {
     const LayerSpec& s = kLayers[l];
     const int nx = u->n[0] >> s.level, ny = u->n[1] >> s.level, nz = u->n[2] >> s.level;
-    const int64_t nv = (int64_t)nz * ny * nx;
+    const MfmaPlan p = plan_mfma(u, l);
     const int ntn = u->npad[l] / 16;
-    int mt = 4, nt = ntn;
-    auto waves = [&]() { return (nv + 16 * mt - 1) / (16 * mt) * (ntn / nt); };
-    while (waves() < 4096 && (mt > 1 || nt > 1)) {
-        if (mt > 1) mt >>= 1;
-        else nt >>= 1;
-    }
     MfmaIn in = mfma_input(u, l);
     const uint16_t* wq = u->wq + u->off_wq[l];
     const float* b = u->params + u->off_b[l];
     const int cp = s.cout, np = u->npad[l], g = u->groups[l];
-    const int64_t halo_bytes = (int64_t)kHaloVox * (mfma_channels(l) / 8) * 16;
-    const int64_t bricks = (int64_t)(nx / 16) * (ny / 4) * (nz / 4);
-    // measured: the halo form pays where the brick is small (<= 32 channels,
-    // <= 41 KiB of LDS, several blocks per CU) and the level has >= 1024 bricks
-    const bool fits = nx % 16 == 0 && ny % 4 == 0 && nz % 4 == 0 && halo_bytes <= 64 * 1024;
-    const bool pays = mfma_channels(l) <= 32 && bricks >= 1024;
-    if (fits && (u->halo == 2 || (u->halo == 1 && pays))) {
+    if (p.halo) {
+        const dim3 grid((unsigned)p.bricks);
+        const size_t lds = (size_t)p.halo_bytes;
         if (ntn == 1)
-            k_conv3_mfma_halo<1><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
-                                                                              s.relu, out_f32);
+            k_conv3_mfma_halo<1><<<grid, 256, lds, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);
         else if (ntn == 2)
-            k_conv3_mfma_halo<2><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
-                                                                              s.relu, out_f32);
+            k_conv3_mfma_halo<2><<<grid, 256, lds, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);
+        else if (ntn == 4)
+            k_conv3_mfma_halo<4><<<grid, 256, lds, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32);
         else
-            k_conv3_mfma_halo<4><<<(unsigned)bricks, 256, (size_t)halo_bytes, g_stream>>>(in, wq, b, out, cp, np, g, nz, ny, nx,
-                                                                              s.relu, out_f32);
-        return;
+            return set_error(FLUX_ERR_STATE, "unet: layer %s: no halo kernel for %d output channels", s.name, np);
+        return FLUX_OK;
     }
-    // levels too small to fill the GPU (< 4096 wavefronts even with 1x1
-    // tiles): split K until 4096 run, each split keeping >= 8 K steps, while
-    // the FP32 partial sums stay within 32 MiB (measured: splitting levels
-    // that already have 4096 wavefronts only adds the finishing pass)
-    int splits = 1;
-    const int chunks = g / 4;
-    while (u->split && waves() * splits < 4096 && splits < 8 && chunks / (2 * splits) >= 8 &&
-           (int64_t)2 * splits * nv * cp * 4 <= kSplitPartBytes)
-        splits *= 2;
-    // every split must own at least one K step
-    while (splits > 1 && (int64_t)(splits - 1) * ((chunks + splits - 1) / splits) >= chunks) splits >>= 1;
-#define FLUX_MFMA_CASE(M, N)                                                                             \
-    if (mt == M && nt == N) {                                                                            \
-        launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32, u->split_part, splits); \
-        return;                                                                                          \
+    FLUX_REQUIRE(p.splits == 1 || u->split_part, "unet: layer %s splits K without a partial-sum buffer", s.name);
+#define FLUX_MFMA_CASE(M, N)                                                                                    \
+    if (p.mt == M && p.nt == N) {                                                                               \
+        launch_mfma<M, N>(in, wq, b, out, cp, np, g, nz, ny, nx, s.relu, out_f32, u->split_part, p.splits);     \
+        return FLUX_OK;                                                                                         \
     }
     FLUX_MFMA_CASE(4, 4) FLUX_MFMA_CASE(4, 2) FLUX_MFMA_CASE(4, 1)
     FLUX_MFMA_CASE(2, 4) FLUX_MFMA_CASE(2, 2) FLUX_MFMA_CASE(2, 1)
     FLUX_MFMA_CASE(1, 4) FLUX_MFMA_CASE(1, 2) FLUX_MFMA_CASE(1, 1)
 #undef FLUX_MFMA_CASE
+    return set_error(FLUX_ERR_STATE, "unet: layer %s: no MFMA kernel for a %dx%d tile over %d channels", s.name, p.mt,
+                     p.nt, np);
 }
 
 int forward_bf16(flux_unet* u, const float* x) {
@@ -940,8 +989,8 @@ int forward_bf16(flux_unet* u, const float* x) {
         k_pool_cl<<<blocks_for(total, UTPB), UTPB, 0, g_stream>>>(u->y16[src], u->pooled16[dst], c, nz, ny, nx);
     };
     for (int l = 0; l < kNumLayers; ++l) {
-        if (l == L_OUT) conv_mfma(u, l, u->y[L_OUT], 1);
-        else conv_mfma(u, l, u->y16[l], 0);
+        if (l == L_OUT) FLUX_TRY(conv_mfma(u, l, u->y[L_OUT], 1));
+        else FLUX_TRY(conv_mfma(u, l, u->y16[l], 0));
         if (l == L_E0B) pool(L_E0B, 0, 16, 0);
         if (l == L_E1B) pool(L_E1B, 1, 32, 1);
         if (l == L_E2B) pool(L_E2B, 2, 64, 2);
@@ -1078,6 +1127,21 @@ int flux_unet_create(int nx, int ny, int nz, int training, flux_unet** out) {
         u->halo = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
         const char* sp = getenv("FLUX_UNET_SPLIT");
         u->split = !(sp && sp[0] == '0');
+        // FLUX_UNET_TILE=MxN (M, N in 1, 2, 4) forces the flat kernel's tile
+        // on every layer (N capped at the layer's width); each output sums
+        // its K terms in the same order whatever the tile, so any tile gives
+        // the same bits (split K off): a check of the tile indexing
+        const char* t = getenv("FLUX_UNET_TILE");
+        int tm = 0, tn = 0;
+        if (t && sscanf(t, "%dx%d", &tm, &tn) == 2) {
+            auto valid = [](int v) { return v == 1 || v == 2 || v == 4; };
+            if (!valid(tm) || !valid(tn)) {
+                delete u;
+                return set_error(FLUX_ERR_ARG, "unet: FLUX_UNET_TILE=%s: tile sides must be 1, 2 or 4", t);
+            }
+            u->tile_mt = tm;
+            u->tile_nt = tn;
+        }
     }
     u->n[0] = nx;
     u->n[1] = ny;
