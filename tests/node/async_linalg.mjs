@@ -55,6 +55,22 @@ async function main() {
 	const both = await Promise.all( [ solver.project( grid, grid )(), solver.project( grid, grid )() ] );
 	res.both = both.map( ( d ) => d.converged );
 
+	// the free-cell coarse-space correction cuts the iterations; the
+	// projection is accepted at the same tolerance
+	grid.dataV.fromArray( v );
+	grid.dataU.fromArray( new Float32Array( ( n + 1 ) * n * n ) );
+	grid.dataW.fromArray( new Float32Array( n * n * ( n + 1 ) ) );
+	solver.pressure.data.fromArray( new Float32Array( n * n * n ) );
+	const cold = await solver.project( grid, grid )();
+	res.coldIterations = cold.iterations;
+	solver.setCoarseSpace( true, [] );
+	grid.dataV.fromArray( v );
+	grid.dataU.fromArray( new Float32Array( ( n + 1 ) * n * n ) );
+	grid.dataW.fromArray( new Float32Array( n * n * ( n + 1 ) ) );
+	solver.pressure.data.fromArray( new Float32Array( n * n * n ) );
+	const cs = await solver.project( grid, grid )();
+	res.coarseSpace = { converged: cs.converged, iterations: cs.iterations };
+
 	// --- dot reducer probes
 	const shape = [ 6, 4, 5 ];
 	const ones = flux.arrayN( 'float', shape ).fromArray( new Float32Array( 120 ).fill( 1 ) );

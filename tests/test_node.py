@@ -237,6 +237,8 @@ def test_async_projection_dot_reducer_and_cg_through_node(tmp_path):
     assert d["project"]["diag"]["converged"]
     assert d["project"]["ticks"] > 0  # the event loop ran while the GPU solved
     assert d["both"] == [True, True]
+    print("cold", d["coldIterations"], "free-cell coarse space", d["coarseSpace"])
+    assert d["coarseSpace"]["converged"] and d["coarseSpace"]["iterations"] < d["coldIterations"]
     assert d["lanes"] == 6
     for p in d["probes"]:
         assert p["total"] == 1 and p["lit"] == [[p["cell"][0], 1]], p

@@ -493,6 +493,39 @@ static napi_value js_pressure3_field(napi_env env, napi_callback_info info) {
     return wrap(env, K_FIELD_VIEW, f);
 }
 
+// pressure3SetCoarseSpace(p, freeIndicator, [fields]): the coarse-space
+// correction of the projection's preconditioner (no reference counterpart)
+static napi_value js_pressure3_set_coarse_space(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    args_n(env, info, 3, a);
+    flux_pressure3* p = (flux_pressure3*)unwrap(env, a[0], K_P3, K_P3);
+    bool is_array = false;
+    uint32_t n = 0;
+    const flux_field* f[3] = {NULL, NULL, NULL};
+    if (napi_is_array(env, a[2], &is_array) == napi_ok && is_array) {
+        napi_get_array_length(env, a[2], &n);
+        if (n > 3) {
+            napi_throw_error(env, NULL, "coarseSpace: at most 3 fields");
+            return NULL;
+        }
+        for (uint32_t q = 0; q < n; ++q) {
+            napi_value e;
+            napi_get_element(env, a[2], q, &e);
+            f[q] = get_field(env, e);
+        }
+    }
+    CHECK_FLUX(flux_pressure3_set_coarse_space(p, get_int(env, a[1], 1), (int)n, f));
+    return NULL;
+}
+
+// pressure3SetFlexible(p, on): the flexible (Polak-Ribiere) beta
+static napi_value js_pressure3_set_flexible(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_pressure3_set_flexible((flux_pressure3*)unwrap(env, a[0], K_P3, K_P3), get_int(env, a[1], 1)));
+    return NULL;
+}
+
 static napi_value diag_object(napi_env env, const flux_pressure_diag* d);
 
 static napi_value js_pressure3_project(napi_env env, napi_callback_info info) {
@@ -1460,6 +1493,8 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("cgFields", js_cg_fields);
     EXPORT("pressure3Create", js_pressure3_create);
     EXPORT("pressure3Field", js_pressure3_field);
+    EXPORT("pressure3SetCoarseSpace", js_pressure3_set_coarse_space);
+    EXPORT("pressure3SetFlexible", js_pressure3_set_flexible);
     EXPORT("pressure3Project", js_pressure3_project);
     EXPORT("binParticles", js_bin_particles);
     EXPORT("p2gGather", js_p2g_gather);

@@ -416,11 +416,31 @@ export function createGridPressureSolver3( {
 
 	}
 
+	// no reference counterpart: the coarse-space correction of the
+	// preconditioner, M^-1 r = V(r) + Phi (Phi^T A Phi)^-1 Phi^T r with Phi the
+	// free-cell indicator and up to three fields, and the flexible beta
+	const coarseFields = [];
+	function setCoarseSpace( freeIndicator = true, fields = [] ) {
+
+		const handles = fields.map( ( f ) => f.handle );
+		addon.pressure3SetCoarseSpace( solver, freeIndicator ? 1 : 0, handles );
+		coarseFields.length = 0;
+		coarseFields.push( ...fields );
+
+	}
+
+	function setFlexible( on = true ) {
+
+		addon.pressure3SetFlexible( solver, on ? 1 : 0 );
+
+	}
+
 	return {
 		project, pressure, b: view( FIELD_RHS, 'float' ), diagnostics, settings,
 		dirichletMask: view( FIELD_MASK, 'u8' ), ventMask: view( FIELD_VENT, 'u8' ),
+		setCoarseSpace, setFlexible,
 		// the native solver reads these fields on every project(): hold them
-		inputs: keep
+		inputs: keep, coarseFields
 	};
 
 }
