@@ -336,6 +336,14 @@ int flux_g2p(const flux_grid3* g, const flux_field* positions, flux_field* veloc
 /* advectParticles (grid_flip_solver3.js:933-940), no collider */
 int flux_advect_particles(const flux_grid3* g, flux_field* positions, int64_t count,
                           const flux_field* const* vel, float dt, int max_substeps);
+/* the same with the FLIP solver's static collider (advectionSolver.trace
+ * with `collider`, grid_flip_solver3.js:927-940; advection_solver3.js:20-62):
+ * a sub-step that ends inside the collider (its SDF, cell-centred on the
+ * grid, sampled <= 0) stops at the SDF-weighted point between its ends.
+ * collider_sdf may be NULL (then identical to flux_advect_particles). */
+int flux_advect_particles_collider(const flux_grid3* g, flux_field* positions, int64_t count,
+                                   const flux_field* const* vel, const flux_field* collider_sdf, float dt,
+                                   int max_substeps);
 /* advectFaceCentered3, order 1, no collider (advection_solver3.js:130-176) */
 int flux_advect_faces(const flux_grid3* g, const flux_field* const* vel_in, flux_field* const* vel_out,
                       float dt, int max_substeps);
@@ -350,6 +358,12 @@ int flux_advection3_create(const flux_grid3* g, const flux_field* const* velocit
                            int order, flux_advection3** out);
 /* the time step of later calls (dt as a live node) */
 int flux_advection3_set_dt(flux_advection3* a, float dt);
+/* the options' `collider` (advection_solver3.js:67-82): a static collider's
+ * SDF cell-centred on the velocity grid (NULL removes it).  The back-trace
+ * then stops at the collider (advection_solver3.js:41-52), sample points
+ * inside it keep the output's value, and order 2's clamp ranges over the
+ * corners outside it (95-128).  The field is read at every call. */
+int flux_advection3_set_collider(flux_advection3* a, const flux_field* collider_sdf);
 /* advectFaceCentered3(input, output) (130-282); output must alias neither the
  * input nor the velocity */
 int flux_advection3_faces(flux_advection3* a, const flux_field* const* in, flux_field* const* out);
@@ -407,6 +421,17 @@ typedef struct {
  * the surface; weights[3] are FLUX_F32 face fields. */
 int flux_collider_face_weights(const flux_grid3d* g, const flux_primitive3* shapes, int count, int invert,
                                flux_field* const* weights);
+/* createOutflowPressureDirichlet3 over `count` (<= 4) createSDFOutflow3
+ * objects combined with the caller's own Dirichlet fields by
+ * combineDirichlet3(outflows, user) (sdf_inflow_outflow3.js:44-94),
+ * evaluated at the cell centres: active (u8) = inside any outflow (its SDF,
+ * cell-centred on this grid, sampled < 0) or the caller's mask; target =
+ * the caller's target where the caller's mask is set, else 0.  user_active
+ * and user_target may be NULL.  The outputs feed flux_pressure3_desc's
+ * dirichlet_active / dirichlet_target. */
+int flux_outflow3_dirichlet(const flux_grid3* g, const flux_field* const* outflow_sdf, int count,
+                            const flux_field* user_active, const flux_field* user_target, flux_field* active,
+                            flux_field* target);
 /* addShapes (sdf_collider3.js:71-110): the signed distance at cell centres */
 int flux_collider_sdf(const flux_grid3d* g, const flux_primitive3* shapes, int count, int invert, flux_field* sdf);
 
@@ -420,6 +445,11 @@ typedef struct flux_boundary3 flux_boundary3;
 int flux_boundary3_create(const flux_grid3* g, flux_field* const* vel, const flux_field* sdf,
                           const flux_field* const* face_weights, float friction, flux_boundary3** out);
 int flux_boundary3_set_closed(flux_boundary3* b, int flags);
+/* setInflows (grid_blocked_boundary_condition_solver3.js:128-190) with
+ * createSDFInflow3 objects (sdf_inflow_outflow3.js:15-24): the inflow's SDF
+ * on the velocity grid's cells, its velocity and mode (0 'set', 1 'add');
+ * applied after the walls, before the clamp, in the order added (at most 4) */
+int flux_boundary3_add_inflow(flux_boundary3* b, const flux_field* sdf, float vx, float vy, float vz, int mode);
 int flux_boundary3_constrain(flux_boundary3* b, int extrapolation_depth);
 int flux_boundary3_destroy(flux_boundary3* b);
 
@@ -623,6 +653,13 @@ typedef struct {
      * 3-D): donors split by phase, a recipient takes its majority phase;
      * needs carry_concentration.  Default 0 */
     int phase_resample;
+    /* colliderPushThresh / colliderPushShift (pushOutOfCollider,
+     * grid_flip_solver3.js:945-964): with a collider, after the particle
+     * advection a particle where the collider's SDF is below the threshold
+     * moves along the SDF's normalised gradient by (thresh - sdf + shift).
+     * Defaults 0 and 0 */
+    float collider_push_thresh;
+    float collider_push_shift;
 } flux_flip3_desc;
 void flux_flip3_desc_default(flux_flip3_desc* d);
 typedef struct flux_flip3 flux_flip3;

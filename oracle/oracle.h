@@ -240,6 +240,15 @@ void orc_collider_sdf3(const orc_primitive3* shapes, int count, int invert, cons
 void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const float h[3], const float origin[3],
                              int closed_flags, const float* sdf, const float* wu, const float* wv, const float* ww,
                              float friction, int depth);
+void orc_constrain_velocity3_inflows(float* u, float* v, float* w, const int n[3], const float h[3],
+                                     const float origin[3], int closed_flags, const float* sdf, const float* wu,
+                                     const float* wv, const float* ww, float friction, int depth, int n_inflows,
+                                     const float* const* inflow_sdf, const float* inflow_vel,
+                                     const int* inflow_mode);
+void orc_outflow_dirichlet3(const float* osdf, const int n[3], const float h[3], const float origin[3],
+                            uint8_t* active);
+void orc_push_out3(float* pos, int64_t count, const float* sdf, const int n[3], const float h[3],
+                   const float origin[3], float thresh, float shift);
 
 /* the resample pass: grid_flip_solver3.js:640-793 */
 void orc_resample3(float* pos, float* vel, int64_t count, const float* u, const float* v, const float* w,
@@ -277,6 +286,22 @@ void orc_advect_faces3_mc_vd(const float* const vel[3], const float* const data[
 /* trace (advection_solver3.js:84-88) of count points, in place */
 void orc_trace3(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
                 const float h[3], const float origin[3], float dt, int max_sub, float direction);
+/* advection through a static collider (csdf: its SDF cell-centred on the
+ * velocity grid, or NULL): advection_solver3.js:20-62 (the trace stops at
+ * the collider), 130-357 (points inside keep the output's value; order 2
+ * clamps over the corners outside the collider) */
+void orc_advect_faces3_c(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
+                         const float h[3], const float origin[3], const float* csdf, float dt, int max_sub,
+                         int order);
+void orc_advect_scalar3_c(const float* src, float* dst, const float* u, const float* v, const float* w,
+                          const int n[3], const float h[3], const float origin[3], const float* csdf, float dt,
+                          int max_sub, int order);
+void orc_advect_particles3_c(float* pos, int64_t count, const float* u, const float* v, const float* w,
+                             const int n[3], const float h[3], const float origin[3], const float* csdf, float dt,
+                             int max_sub);
+void orc_trace3_c(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
+                  const float h[3], const float origin[3], const float* csdf, float dt, int max_sub,
+                  float direction);
 void orc_advect_face3_mc(const float* u, const float* v, const float* w, float* u_out, float* v_out, float* w_out,
                          const int n[3], const float h[3], const float origin[3], float dt, int max_sub);
 void orc_advect_scalar3_mc(const float* src, float* dst, const float* u, const float* v, const float* w,

@@ -282,6 +282,22 @@ static float clamp_component(float v) {
 void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const float h[3], const float origin[3],
                              int closed_flags, const float* sdf, const float* wu, const float* wv, const float* ww,
                              float friction, int depth) {
+    orc_constrain_velocity3_inflows(u, v, w, n, h, origin, closed_flags, sdf, wu, wv, ww, friction, depth, 0, NULL,
+                                    NULL, NULL);
+}
+
+/* The same with inflow objects (createSDFInflow3, sdf_inflow_outflow3.js:15-24;
+ * buildInflowKernels, grid_blocked_boundary_condition_solver3.js:130-176):
+ * after the domain walls, inflow q in order sets (mode 0) or adds (mode 1)
+ * its velocity component on every face whose position lies inside its SDF
+ * (trilinear sample < 0, isInsideSdf), then the +-100 clamp.  Each inflow
+ * SDF is cell-centred on the velocity grid; inflow_vel holds 3 floats per
+ * inflow. */
+void orc_constrain_velocity3_inflows(float* u, float* v, float* w, const int n[3], const float h[3],
+                                     const float origin[3], int closed_flags, const float* sdf, const float* wu,
+                                     const float* wv, const float* ww, float friction, int depth, int n_inflows,
+                                     const float* const* inflow_sdf, const float* inflow_vel,
+                                     const int* inflow_mode) {
     int sz[3][3] = {{n[0] + 1, n[1], n[2]}, {n[0], n[1] + 1, n[2]}, {n[0], n[1], n[2] + 1}};
     float* data[3] = {u, v, w};
     const float* wts[3] = {wu, wv, ww};
@@ -411,6 +427,23 @@ void orc_constrain_velocity3(float* u, float* v, float* w, const int n[3], const
             if (closed_flags & 16) w[i + (int64_t)sz[2][0] * (j + (int64_t)sz[2][1] * 0)] = 0;
             if (closed_flags & 32) w[i + (int64_t)sz[2][0] * (j + (int64_t)sz[2][1] * n[2])] = 0;
         }
+    for (int q = 0; q < n_inflows; ++q) {
+        const float so[3] = {origin[0] + 0.5f * h[0], origin[1] + 0.5f * h[1], origin[2] + 0.5f * h[2]};
+        for (int a = 0; a < 3; ++a) {
+            float fo[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+            fo[a] = origin[a];
+            const float value = inflow_vel[3 * q + a];
+            for (int k = 0; k < sz[a][2]; ++k)
+                for (int j = 0; j < sz[a][1]; ++j)
+                    for (int i = 0; i < sz[a][0]; ++i) {
+                        const int64_t f = i + (int64_t)sz[a][0] * (j + (int64_t)sz[a][1] * k);
+                        const float pt[3] = {fo[0] + (float)i * h[0], fo[1] + (float)j * h[1],
+                                             fo[2] + (float)k * h[2]};
+                        if (!(orc_sample_trilinear3(inflow_sdf[q], n, so, h, pt) < 0)) continue;
+                        data[a][f] = inflow_mode[q] == 1 ? data[a][f] + value : value;
+                    }
+        }
+    }
     for (int a = 0; a < 3; ++a)
         for (int64_t f = 0; f < cnt[a]; ++f) data[a][f] = clamp_component(data[a][f]);
 }
@@ -599,4 +632,38 @@ void orc_resample_phase3(float* pos, float* vel, const float* conc, int64_t coun
         else base_g += needed;
     }
     free(cell); free(pool); free(need);
+}
+
+/* ---- createOutflowPressureDirichlet3 (sdf_inflow_outflow3.js:44-62) for one
+ *      outflow whose SDF is cell-centred on the pressure grid: active where
+ *      the SDF sampled at the cell centre (dataPosition) is < 0 ---- */
+void orc_outflow_dirichlet3(const float* osdf, const int n[3], const float h[3], const float origin[3],
+                            uint8_t* active) {
+    const float so[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const float p[3] = {so[0] + h[0] * (float)i, so[1] + h[1] * (float)j, so[2] + h[2] * (float)k};
+                active[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = orc_sample_trilinear3(osdf, n, so, h, p) < 0;
+            }
+}
+
+/* ---- pushOutOfCollider (grid_flip_solver3.js:945-964): a particle where
+ *      the collider's SDF (cell-centred on the grid) is below thresh moves
+ *      along the SDF's normalised trilinear gradient by (thresh - sdf) +
+ *      shift ---- */
+void orc_push_out3(float* pos, int64_t count, const float* sdf, const int n[3], const float h[3],
+                   const float origin[3], float thresh, float shift) {
+    const float so[3] = {origin[0] + 0.5f * h[0], origin[1] + 0.5f * h[1], origin[2] + 0.5f * h[2]};
+    for (int64_t p = 0; p < count; ++p) {
+        const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        const float v = orc_sample_trilinear3(sdf, n, so, h, pp);
+        if (!(v < thresh)) continue;
+        float g[3];
+        sdf_gradient3(sdf, n, so, h, pp, g);
+        const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+        if (!(len > 0.0f)) continue;
+        const float dist = (thresh - v) + shift;
+        for (int a = 0; a < 3; ++a) pos[3 * p + a] = pp[a] + (g[a] / len) * dist;
+    }
 }

@@ -337,10 +337,23 @@ void orc_g2p3(const float* pos, float* vel, int64_t count, const float* u, const
     }
 }
 
-/* ---- backTrace (no collider): advection_solver3.js:20-62 ---- */
-static void back_trace(const float* u, const float* v, const float* w, const int n[3], const float h[3],
-                       const float ou[3], const float ov[3], const float ow[3], const float start[3],
-                       float dt, float hmin, int max_sub, float direction, float out[3]) {
+/* ---- a static collider's SDF, cell-centred on the velocity grid
+ *      (createSDFStaticCollider3's sample, sdf_collider3.js:33-37:
+ *      collocatedValueAtPosition3); no collider samples as 1
+ *      (advection_solver3.js:78-82) ---- */
+static float collider_sample(const float* csdf, const int n[3], const float h[3], const float so[3],
+                             const float p[3]) {
+    return csdf ? orc_sample_trilinear3(csdf, n, so, h, p) : 1.0f;
+}
+
+/* ---- backTrace: advection_solver3.js:20-62.  With a collider a sub-step
+ *      whose end point lies in it (phi1 <= 0) stops at the SDF-weighted
+ *      blend of its two ends, w = |phi1| / (|phi0| + |phi1|), and ends the
+ *      trace. ---- */
+static void back_trace_c(const float* u, const float* v, const float* w, const int n[3], const float h[3],
+                         const float ou[3], const float ov[3], const float ow[3], const float* csdf,
+                         const float start[3], float dt, float hmin, int max_sub, float direction, float out[3]) {
+    const float so[3] = {ov[0], ou[1], ou[2]};
     float pt[3] = {start[0], start[1], start[2]};
     float rem = dt;
     for (int s = 0; s < max_sub; ++s) {
@@ -356,7 +369,18 @@ static void back_trace(const float* u, const float* v, const float* w, const int
         for (int a = 0; a < 3; ++a) mid[a] = pt[a] - (v0[a] * half) * direction;
         float mv[3];
         sample_mac(u, v, w, n, h, ou, ov, ow, mid, mv);
-        for (int a = 0; a < 3; ++a) pt[a] = pt[a] - (mv[a] * sub) * direction;
+        float next[3];
+        for (int a = 0; a < 3; ++a) next[a] = pt[a] - (mv[a] * sub) * direction;
+        if (csdf) {
+            const float phi0 = collider_sample(csdf, n, h, so, pt);
+            const float phi1 = collider_sample(csdf, n, h, so, next);
+            if (phi1 <= 0.0f) {
+                const float wt = fabsf(phi1) / (fabsf(phi0) + fabsf(phi1));
+                for (int a = 0; a < 3; ++a) pt[a] = pt[a] * wt + next[a] * (1.0f - wt);
+                break;
+            }
+        }
+        for (int a = 0; a < 3; ++a) pt[a] = next[a];
         rem = rem - sub;
     }
     out[0] = pt[0];
@@ -364,10 +388,18 @@ static void back_trace(const float* u, const float* v, const float* w, const int
     out[2] = pt[2];
 }
 
+
 /* ---- advectParticles: grid_flip_solver3.js:933-940 ---- */
 void orc_advect_particles3(float* pos, int64_t count, const float* u, const float* v, const float* w,
                            const int n[3], const float h[3], const float origin[3], float dt,
                            int max_sub) {
+    orc_advect_particles3_c(pos, count, u, v, w, n, h, origin, NULL, dt, max_sub);
+}
+
+/* the same through the collider (its SDF cell-centred on the grid, or NULL) */
+void orc_advect_particles3_c(float* pos, int64_t count, const float* u, const float* v, const float* w,
+                             const int n[3], const float h[3], const float origin[3], const float* csdf, float dt,
+                             int max_sub) {
     float ou[3], ov[3], ow[3];
     mac_origins(origin, h, ou, ov, ow);
     float hmin = fminf(h[0], fminf(h[1], h[2]));
@@ -379,7 +411,7 @@ void orc_advect_particles3(float* pos, int64_t count, const float* u, const floa
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < count; ++p) {
         float tr[3];
-        back_trace(u, v, w, n, h, ou, ov, ow, &pos[3 * p], dt, hmin, max_sub, -1.0f, tr);
+        back_trace_c(u, v, w, n, h, ou, ov, ow, csdf, &pos[3 * p], dt, hmin, max_sub, -1.0f, tr);
         int bad = !isfinite(tr[0]) || !isfinite(tr[1]) || !isfinite(tr[2]);
         for (int a = 0; a < 3; ++a) {
             float s = bad ? pos[3 * p + a] : tr[a];
@@ -431,28 +463,91 @@ static float sample_cubic(const float* data, const int s[3], const float o[3], c
     return mono_cubic(zv[0], zv[1], zv[2], zv[3], fz);
 }
 
-/* ---- advectFaceCentered3, order 1, no collider: advection_solver3.js:130-176.
- *      The trace follows the solver's velocity grid (vel); the face value is
- *      the tricubic sample of the advected grid (data) at the trace. ---- */
-void orc_advect_faces3_vd(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
-                          const float h[3], const float origin[3], float dt, int max_sub) {
+/* ---- advectFaceCentered3 (advection_solver3.js:130-282) through the
+ *      collider (csdf: its SDF cell-centred on the grid, or NULL).  The trace
+ *      follows the solver's velocity grid (vel); the face value is the
+ *      tricubic sample of the advected grid (data) at the trace.  A face
+ *      whose position lies in the collider (sample <= 0) keeps out's value.
+ *      Order 2 (MacCormack): fwd = order 1 (the input where the face lies in
+ *      the collider); bwd = fwd sampled at the trace with direction -1;
+ *      out = fwd + (in - bwd) / 2 where that lies within the min/max of the
+ *      input over the 8 trilinear corners of the forward trace whose own
+ *      face positions lie outside the collider (gatherLocalMinMax,
+ *      clampCorrection 95-128), else fwd. ---- */
+static float mc_clamp_c(const float* data, const int s[3], const float o[3], const float h[3], const float tr[3],
+                        float corrected, float fwd, const float* csdf, const int n[3], const float so[3]) {
+    orc_tri t;
+    tri_weights(tr, o, h, s, &t);
+    float mn = 1e30f, mx = -1e30f;
+    int have = 0;
+    for (int c = 0; c < 8; ++c) {
+        int i, j, k;
+        tri_corner(&t, c, &i, &j, &k);
+        const float cp[3] = {o[0] + (float)i * h[0], o[1] + (float)j * h[1], o[2] + (float)k * h[2]};
+        if (!(collider_sample(csdf, n, h, so, cp) > 0.0f)) continue;
+        const float v = data[orc_cidx(s, i, j, k)];
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+        have = 1;
+    }
+    return (have && corrected >= mn && corrected <= mx) ? corrected : fwd;
+}
+
+void orc_advect_faces3_c(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
+                         const float h[3], const float origin[3], const float* csdf, float dt, int max_sub,
+                         int order) {
     float ou[3], ov[3], ow[3];
     mac_origins(origin, h, ou, ov, ow);
-    float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float hmin = fminf(h[0], fminf(h[1], h[2]));
+    const float so[3] = {ov[0], ou[1], ou[2]};
     const float* org[3] = {ou, ov, ow};
+    float* fwd[3] = {NULL, NULL, NULL};
     for (int a = 0; a < 3; ++a) {
         int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
+        float* dst = out[a];
+        if (order == 2) dst = fwd[a] = (float*)malloc(sizeof(float) * (size_t)orc_faces(n, a));
 #pragma omp parallel for schedule(static)
         for (int k = 0; k < s[2]; ++k)
             for (int j = 0; j < s[1]; ++j)
                 for (int i = 0; i < s[0]; ++i) {
                     float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
                                     org[a][2] + (float)k * h[2]};
+                    const int64_t f = orc_cidx(s, i, j, k);
+                    if (!(collider_sample(csdf, n, h, so, pos) > 0.0f)) {
+                        if (order == 2) dst[f] = data[a][f];
+                        continue;
+                    }
                     float tr[3];
-                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
-                    out[a][orc_cidx(s, i, j, k)] = sample_cubic(data[a], s, org[a], h, tr);
+                    back_trace_c(vel[0], vel[1], vel[2], n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, 1.0f, tr);
+                    dst[f] = sample_cubic(data[a], s, org[a], h, tr);
                 }
     }
+    if (order != 2) return;
+    for (int a = 0; a < 3; ++a) {
+        int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < s[2]; ++k)
+            for (int j = 0; j < s[1]; ++j)
+                for (int i = 0; i < s[0]; ++i) {
+                    const float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
+                                          org[a][2] + (float)k * h[2]};
+                    if (!(collider_sample(csdf, n, h, so, pos) > 0.0f)) continue;
+                    float tb[3], tf[3];
+                    back_trace_c(vel[0], vel[1], vel[2], n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, -1.0f, tb);
+                    const float bwd = sample_cubic(fwd[a], s, org[a], h, tb);
+                    back_trace_c(vel[0], vel[1], vel[2], n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, 1.0f, tf);
+                    const int64_t f = orc_cidx(s, i, j, k);
+                    const float fv = fwd[a][f];
+                    const float corrected = fv + (data[a][f] - bwd) * 0.5f;
+                    out[a][f] = mc_clamp_c(data[a], s, org[a], h, tf, corrected, fv, csdf, n, so);
+                }
+    }
+    for (int a = 0; a < 3; ++a) free(fwd[a]);
+}
+
+void orc_advect_faces3_vd(const float* const vel[3], const float* const data[3], float* const out[3], const int n[3],
+                          const float h[3], const float origin[3], float dt, int max_sub) {
+    orc_advect_faces3_c(vel, data, out, n, h, origin, NULL, dt, max_sub, 1);
 }
 
 void orc_advect_face3(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
@@ -486,73 +581,59 @@ void orc_two_phase_beta3(const float* frac, const int n[3], float rho_gas, float
     }
 }
 
-/* ---- advectScalar3, order 1, no collider: advection_solver3.js:285-300 ---- */
-void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
-                        const float h[3], const float origin[3], float dt, int max_sub) {
+/* ---- advectScalar3 (advection_solver3.js:285-357) through the collider
+ *      (csdf or NULL): a cell whose centre lies in the collider keeps dst's
+ *      value; order 2 as for the faces, corners at the cell centres ---- */
+void orc_advect_scalar3_c(const float* src, float* dst, const float* u, const float* v, const float* w,
+                          const int n[3], const float h[3], const float origin[3], const float* csdf, float dt,
+                          int max_sub, int order) {
     float ou[3], ov[3], ow[3];
     mac_origins(origin, h, ou, ov, ow);
     const float hmin = fminf(h[0], fminf(h[1], h[2]));
     const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
     const int s[3] = {n[0], n[1], n[2]};
+    float* fwd = order == 2 ? (float*)malloc(sizeof(float) * (size_t)orc_cells(n)) : dst;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i) {
                 float pos[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+                const int64_t c = orc_cidx(s, i, j, k);
+                if (!(collider_sample(csdf, n, h, co, pos) > 0.0f)) {
+                    if (order == 2) fwd[c] = src[c];
+                    continue;
+                }
                 float tr[3];
-                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tr);
-                dst[orc_cidx(s, i, j, k)] = sample_cubic(src, s, co, h, tr);
+                back_trace_c(u, v, w, n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, 1.0f, tr);
+                fwd[c] = sample_cubic(src, s, co, h, tr);
             }
+    if (order != 2) return;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const float pos[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
+                if (!(collider_sample(csdf, n, h, co, pos) > 0.0f)) continue;
+                float tb[3], tf[3];
+                back_trace_c(u, v, w, n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, -1.0f, tb);
+                const float bwd = sample_cubic(fwd, s, co, h, tb);
+                back_trace_c(u, v, w, n, h, ou, ov, ow, csdf, pos, dt, hmin, max_sub, 1.0f, tf);
+                const int64_t c = orc_cidx(s, i, j, k);
+                const float fv = fwd[c];
+                const float corrected = fv + (src[c] - bwd) * 0.5f;
+                dst[c] = mc_clamp_c(src, s, co, h, tf, corrected, fv, csdf, n, co);
+            }
+    free(fwd);
 }
 
-/* ---- order 2 (MacCormack) without a collider: advection_solver3.js:90-128
- *      (min/max clamp over the 8 trilinear corners of the forward trace),
- *      180-282 (faces), 302-357 (scalar).  fwd = order 1; bwd = fwd sampled
- *      at the trace with direction -1; out = fwd + (in - bwd) / 2 where it
- *      lies within the corners' range, else fwd. ---- */
-static float mc_clamp(const float* data, const int s[3], const float o[3], const float h[3], const float tr[3],
-                      float corrected, float fwd) {
-    orc_tri t;
-    tri_weights(tr, o, h, s, &t);
-    float mn = 1e30f, mx = -1e30f;
-    for (int c = 0; c < 8; ++c) {
-        int i, j, k;
-        tri_corner(&t, c, &i, &j, &k);
-        const float v = data[orc_cidx(s, i, j, k)];
-        mn = fminf(mn, v);
-        mx = fmaxf(mx, v);
-    }
-    return (corrected >= mn && corrected <= mx) ? corrected : fwd;
+void orc_advect_scalar3(const float* src, float* dst, const float* u, const float* v, const float* w, const int n[3],
+                        const float h[3], const float origin[3], float dt, int max_sub) {
+    orc_advect_scalar3_c(src, dst, u, v, w, n, h, origin, NULL, dt, max_sub, 1);
 }
 
 void orc_advect_faces3_mc_vd(const float* const vel[3], const float* const data[3], float* const out[3],
                              const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
-    float ou[3], ov[3], ow[3];
-    mac_origins(origin, h, ou, ov, ow);
-    const float hmin = fminf(h[0], fminf(h[1], h[2]));
-    const float* org[3] = {ou, ov, ow};
-    float* fwd[3];
-    for (int a = 0; a < 3; ++a) fwd[a] = (float*)malloc(sizeof(float) * (size_t)orc_faces(n, a));
-    orc_advect_faces3_vd(vel, data, fwd, n, h, origin, dt, max_sub);
-    for (int a = 0; a < 3; ++a) {
-        int s[3] = {n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)};
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < s[2]; ++k)
-            for (int j = 0; j < s[1]; ++j)
-                for (int i = 0; i < s[0]; ++i) {
-                    const float pos[3] = {org[a][0] + (float)i * h[0], org[a][1] + (float)j * h[1],
-                                          org[a][2] + (float)k * h[2]};
-                    float tb[3], tf[3];
-                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, -1.0f, tb);
-                    const float bwd = sample_cubic(fwd[a], s, org[a], h, tb);
-                    back_trace(vel[0], vel[1], vel[2], n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tf);
-                    const int64_t f = orc_cidx(s, i, j, k);
-                    const float fv = fwd[a][f];
-                    const float corrected = fv + (data[a][f] - bwd) * 0.5f;
-                    out[a][f] = mc_clamp(data[a], s, org[a], h, tf, corrected, fv);
-                }
-    }
-    for (int a = 0; a < 3; ++a) free(fwd[a]);
+    orc_advect_faces3_c(vel, data, out, n, h, origin, NULL, dt, max_sub, 2);
 }
 
 void orc_advect_face3_mc(const float* u, const float* v, const float* w, float* uo_, float* vo_, float* wo_,
@@ -564,28 +645,7 @@ void orc_advect_face3_mc(const float* u, const float* v, const float* w, float* 
 
 void orc_advect_scalar3_mc(const float* src, float* dst, const float* u, const float* v, const float* w,
                            const int n[3], const float h[3], const float origin[3], float dt, int max_sub) {
-    float ou[3], ov[3], ow[3];
-    mac_origins(origin, h, ou, ov, ow);
-    const float hmin = fminf(h[0], fminf(h[1], h[2]));
-    const float co[3] = {origin[0] + h[0] * 0.5f, origin[1] + h[1] * 0.5f, origin[2] + h[2] * 0.5f};
-    const int s[3] = {n[0], n[1], n[2]};
-    float* fwd = (float*)malloc(sizeof(float) * (size_t)orc_cells(n));
-    orc_advect_scalar3(src, fwd, u, v, w, n, h, origin, dt, max_sub);
-#pragma omp parallel for schedule(static)
-    for (int k = 0; k < n[2]; ++k)
-        for (int j = 0; j < n[1]; ++j)
-            for (int i = 0; i < n[0]; ++i) {
-                const float pos[3] = {co[0] + (float)i * h[0], co[1] + (float)j * h[1], co[2] + (float)k * h[2]};
-                float tb[3], tf[3];
-                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, -1.0f, tb);
-                const float bwd = sample_cubic(fwd, s, co, h, tb);
-                back_trace(u, v, w, n, h, ou, ov, ow, pos, dt, hmin, max_sub, 1.0f, tf);
-                const int64_t c = orc_cidx(s, i, j, k);
-                const float fv = fwd[c];
-                const float corrected = fv + (src[c] - bwd) * 0.5f;
-                dst[c] = mc_clamp(src, s, co, h, tf, corrected, fv);
-            }
-    free(fwd);
+    orc_advect_scalar3_c(src, dst, u, v, w, n, h, origin, NULL, dt, max_sub, 2);
 }
 
 /* ---- trace (advection_solver3.js:84-88): backTrace of each point through
@@ -594,12 +654,19 @@ void orc_advect_scalar3_mc(const float* src, float* dst, const float* u, const f
  *      oracle. ---- */
 void orc_trace3(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
                 const float h[3], const float origin[3], float dt, int max_sub, float direction) {
+    orc_trace3_c(pos, count, u, v, w, n, h, origin, NULL, dt, max_sub, direction);
+}
+
+/* the same through the collider (csdf or NULL) */
+void orc_trace3_c(float* pos, int64_t count, const float* u, const float* v, const float* w, const int n[3],
+                  const float h[3], const float origin[3], const float* csdf, float dt, int max_sub,
+                  float direction) {
     float ou[3], ov[3], ow[3];
     mac_origins(origin, h, ou, ov, ow);
     const float hmin = fminf(h[0], fminf(h[1], h[2]));
     for (int64_t p = 0; p < count; ++p) {
         float tr[3];
-        back_trace(u, v, w, n, h, ou, ov, ow, &pos[3 * p], dt, hmin, max_sub, direction, tr);
+        back_trace_c(u, v, w, n, h, ou, ov, ow, csdf, &pos[3 * p], dt, hmin, max_sub, direction, tr);
         pos[3 * p] = tr[0];
         pos[3 * p + 1] = tr[1];
         pos[3 * p + 2] = tr[2];

@@ -518,6 +518,60 @@ def trace3(pos, u, v, w, n, h, origin, dt, direction=1.0, max_substeps=32):
     return pos
 
 
+def advect_faces3_c(vel, data, n, h, origin, csdf, dt, max_substeps=32, order=1, out=None):
+    """advectFaceCentered3 through a static collider (csdf: its SDF
+    cell-centred on the grid, or None), advection_solver3.js:20-282; faces
+    inside the collider keep `out`'s values (zeros when out is None)"""
+    keep = [np.ascontiguousarray(a, np.float32) for a in vel] + [np.ascontiguousarray(a, np.float32) for a in data]
+    csdf = None if csdf is None else np.ascontiguousarray(csdf, np.float32)
+    V = (C.c_void_p * 3)(*[_p(a) for a in keep[:3]])
+    D = (C.c_void_p * 3)(*[_p(a) for a in keep[3:]])
+    outs = [np.zeros_like(a) for a in keep[3:]] if out is None else \
+        [np.ascontiguousarray(a, np.float32).copy() for a in out]
+    Oa = (C.c_void_p * 3)(*[_p(a) for a in outs])
+    lib().orc_advect_faces3_c(V, D, Oa, _i3(n), _f3(h), _f3(origin), _p(csdf), C.c_float(dt), int(max_substeps),
+                              int(order))
+    return outs
+
+
+def advect_scalar3_c(src, u, v, w, n, h, origin, csdf, dt, max_substeps=32, order=1, out=None):
+    """advectScalar3 through a static collider (advection_solver3.js:285-357);
+    cells inside keep `out`'s values"""
+    src = np.ascontiguousarray(src, np.float32)
+    dst = np.zeros_like(src) if out is None else np.ascontiguousarray(out, np.float32).copy()
+    csdf = None if csdf is None else np.ascontiguousarray(csdf, np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_advect_scalar3_c(_p(src), _p(dst), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin), _p(csdf),
+                               C.c_float(dt), int(max_substeps), int(order))
+    return dst
+
+
+def push_out3(pos, sdf, n, h, origin, thresh=0.0, shift=0.0):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    sdf = np.ascontiguousarray(sdf, np.float32)
+    lib().orc_push_out3(_p(pos), C.c_int64(pos.size // 3), _p(sdf), _i3(n), _f3(h), _f3(origin), C.c_float(thresh),
+                        C.c_float(shift))
+    return pos
+
+
+def advect_particles3_c(pos, u, v, w, n, h, origin, csdf, dt, max_substeps=32):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    csdf = None if csdf is None else np.ascontiguousarray(csdf, np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_advect_particles3_c(_p(pos), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h),
+                                  _f3(origin), _p(csdf), C.c_float(dt), int(max_substeps))
+    return pos
+
+
+def trace3_c(pos, u, v, w, n, h, origin, csdf, dt, direction=1.0, max_substeps=32):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    csdf = None if csdf is None else np.ascontiguousarray(csdf, np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_trace3_c(_p(pos), C.c_int64(pos.size // 3), *[_p(a) for a in arrs], _i3(n), _f3(h), _f3(origin),
+                       _p(csdf), C.c_float(dt), int(max_substeps), C.c_float(direction))
+    return pos
+
+
 def two_phase_beta3(frac, n, rho_gas, rho_liquid):
     frac = np.ascontiguousarray(frac, np.float32)
     outs = [np.zeros(int(np.prod(face_shape(n, a))), np.float32) for a in range(3)]
@@ -639,12 +693,21 @@ def collider_sdf3(shapes, n, h, origin, invert=False):
     return out
 
 
-def constrain_velocity3(vel, n, h, origin, closed_flags=63, sdf=None, weights=None, friction=0.5, depth=5):
+def constrain_velocity3(vel, n, h, origin, closed_flags=63, sdf=None, weights=None, friction=0.5, depth=5,
+                        inflows=()):
+    """constrainVelocity (grid_blocked_boundary_condition_solver3.js:389-477);
+    inflows: (sdf, (vx, vy, vz), mode) triples, mode 'set' or 'add', each SDF
+    cell-centred on the velocity grid"""
     u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in vel]
     sdf = None if sdf is None else np.ascontiguousarray(sdf, np.float32)
     ws = [None, None, None] if weights is None else [np.ascontiguousarray(a, np.float32) for a in weights]
-    lib().orc_constrain_velocity3(_p(u), _p(v), _p(w), _i3(n), _f3(h), _f3(origin), int(closed_flags), _p(sdf),
-                                  _p(ws[0]), _p(ws[1]), _p(ws[2]), C.c_float(friction), int(depth))
+    isdf = [np.ascontiguousarray(q[0], np.float32) for q in inflows]
+    ptrs = (C.c_void_p * max(1, len(isdf)))(*[a.ctypes.data for a in isdf])
+    ivel = np.array([list(q[1]) for q in inflows] or [[0, 0, 0]], np.float32).ravel()
+    imode = np.array([1 if q[2] == "add" else 0 for q in inflows] or [0], np.int32)
+    lib().orc_constrain_velocity3_inflows(_p(u), _p(v), _p(w), _i3(n), _f3(h), _f3(origin), int(closed_flags), _p(sdf),
+                                          _p(ws[0]), _p(ws[1]), _p(ws[2]), C.c_float(friction), int(depth),
+                                          len(isdf), ptrs, _p(ivel), _p(imode))
     return [u, v, w]
 
 
@@ -745,7 +808,8 @@ def pin_cell3(n, pin):
 def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e-6, flip_ratio=0.97,
                damping=0.02, scale=65536.0, eps=1e-4, flags=63, resample=True, min_pc=4, max_pc=8, depth=5,
                max_substeps=32, reduced=True, max_plausible=None, carry_concentration=False, mixing=0.0, fade=0.0,
-               conc_scale=4096.0, density=None, mass_weighted=False, pressure_pin=None, phase_resample=False):
+               conc_scale=4096.0, density=None, mass_weighted=False, pressure_pin=None, phase_resample=False,
+               collider_sdf=None, collider_weights=None, friction=0.5, push_thresh=0.0, push_shift=0.0):
     """One createGridFlipSolver3 time step (grid_flip_solver3.js:1083-1132)
     composed from the oracle's stages; st holds pos, vel, u, v, w, pressure
     and the last accepted pressure (and conc with carry_concentration), and is
@@ -754,11 +818,17 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
     pressure, to the gravity term (848-880); mass_weighted weights the
     scatter by rho / rho_ref (493-531); pressure_pin pins one cell to 0 when
     the free surface leaves no Dirichlet cell; phase_resample splits the
-    resampling's donors by phase."""
+    resampling's donors by phase.  collider_sdf (cell-centred on the grid)
+    is the static collider: the particle trace stops at it, particles inside
+    are pushed out (push_thresh, push_shift; 945-964) and constrainVelocity
+    projects against it (collider_weights: its face weights, friction)."""
     pos, vel = st["pos"], st["vel"]
     u, v, w = st["u"], st["v"], st["w"]
     conc = st.get("conc")
-    pos = advect_particles3(pos, u, v, w, n, h, origin, dt, max_substeps)
+    pos = advect_particles3_c(pos, u, v, w, n, h, origin, collider_sdf, dt, max_substeps)
+    if collider_sdf is not None:
+        pos = push_out3(pos, collider_sdf, n, h, origin, push_thresh, push_shift)
+    cons = dict(sdf=collider_sdf, weights=collider_weights, friction=friction)
     mass = (conc, density[0], density[1]) if (mass_weighted and density is not None and carry_concentration) else None
     vel_grid, valid = [], []
     for a in range(3):
@@ -767,7 +837,7 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
         valid.append(vd)
     old = [x.copy() for x in vel_grid]
     vel_grid = [extrapolate3(face_shape(n, a), vel_grid[a], valid[a], depth) for a in range(3)]
-    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth, **cons)
     mask = mark_fluid3(pos, n, h, origin)
     if resample:
         if phase_resample and carry_concentration:
@@ -787,7 +857,7 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
         vel_grid = [(vel_grid[a] + np.float32(gravity[a]) * np.float32(dt)).astype(np.float32) for a in range(3)]
     elif density is not None:
         vel_grid = reduced_gravity3(vel_grid, beta, cell_density, n, h, origin, gravity, dt, density[0], density[1])
-    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth, **cons)
     active, target = free_surface_fields(mask, n, h, origin, gravity, dt, reduced)
     pin = pin_cell3(n, pressure_pin)
     if pin is not None and not active.any():
@@ -810,7 +880,7 @@ def flip3_step(st, n, h, origin, dt, gravity=(0.0, -9.81, 0.0), levels=1, tol=1e
         st["p_accepted"] = x.copy()
     diag["rejected"] = bad > 0
     vel_grid = pressure3_correct(n, h, x, vel_grid, beta=beta, suffix="f32")
-    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth)
+    vel_grid = constrain_velocity3(vel_grid, n, h, origin, closed_flags=flags, depth=depth, **cons)
     vel_grid = [extrapolate3(face_shape(n, a), vel_grid[a], adj[a], depth) for a in range(3)]
     vel = g2p3(pos, vel, *vel_grid, *old, n, h, origin, flip_ratio, damping)
     if carry_concentration:
@@ -1004,6 +1074,26 @@ def outflow_dirichlet2(osdf, n, h, origin):
     active = np.zeros(int(n[0] * n[1]), np.uint8)
     lib().orc_outflow_dirichlet2(_p(osdf), _i2(n), _f2(h), _f2(origin), _p(active))
     return active
+
+
+def outflow_dirichlet3(outflows, n, h, origin, user_active=None, user_target=None):
+    """createOutflowPressureDirichlet3 over a list of outflow SDFs (active =
+    inside any, target 0), combined by combineDirichlet3(outflow, user)
+    (sdf_inflow_outflow3.js:44-94: active = either; the user's target where
+    the user's mask is set, the outflow's 0 elsewhere)"""
+    cells = int(np.prod(n))
+    act = np.zeros(cells, bool)
+    for osdf in outflows:
+        a = np.zeros(cells, np.uint8)
+        lib().orc_outflow_dirichlet3(_p(np.ascontiguousarray(osdf, np.float32)), _i3(n), _f3(h), _f3(origin), _p(a))
+        act |= a != 0
+    target = np.zeros(cells, np.float32)
+    if user_active is not None:
+        ua = np.asarray(user_active).ravel() != 0
+        act |= ua
+        if user_target is not None:
+            target = np.where(ua, np.asarray(user_target, np.float32).ravel(), np.float32(0)).astype(np.float32)
+    return act.astype(np.uint8), target
 
 
 def max_abs(arrays):

@@ -95,3 +95,67 @@ def test_constrain_velocity_matches_oracle(flux, collider):
                                  depth=5)
     for a in range(3):
         np.testing.assert_allclose(fields[a].to_array(), want[a], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("collider", ["none", "sdf"])
+def test_constrain_velocity_with_inflows_matches_oracle(flux, collider):
+    """createSDFInflow3 objects in constrainVelocity: a 'set' ball and an
+    'add' box, applied after the walls and before the clamp, in order (the
+    ball's set region overlaps the box's add region)"""
+    n, h, o = (16, 12, 14), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(19)
+    vel = [rng.uniform(-3, 3, int(np.prod(F.face_shape(n, a)))).astype(np.float32) for a in range(3)]
+    g = Grid(n, h, o)
+    fields = [F.array_n("float", F.face_shape(n, a), vel[a]) for a in range(3)]
+    ball = O.collider_sdf3([B.sphere((3.5, 6.0, 7.0), 2.5)], n, h, o)
+    box = O.collider_sdf3([B.box((4.5, 6.0, 7.0), (2.0, 2.0, 3.0))], n, h, o)
+    inflows = [(F.array_n("float", n, ball), (1.5, -0.25, 0.5), "set"),
+               (F.array_n("float", n, box), (0.75, 0.0, -2.0), "add")]
+    sdf_h = sdf = None
+    if collider == "sdf":
+        sdf_h = O.collider_sdf3([B.sphere((12.0, 6.0, 7.0), 2.2)], n, h, o)
+        sdf = F.array_n("float", n, sdf_h)
+    bs = B.BoundarySolver3(g, fields, sdf=sdf, friction=0.5, inflows=inflows)
+    bs.closed_domain_boundary_flag = 63
+    bs.constrain_velocity(5)
+    want = O.constrain_velocity3(vel, n, h, o, closed_flags=63, sdf=sdf_h, friction=0.5, depth=5,
+                                 inflows=[(ball, (1.5, -0.25, 0.5), "set"), (box, (0.75, 0.0, -2.0), "add")])
+    for a in range(3):
+        np.testing.assert_allclose(fields[a].to_array(), want[a], rtol=1e-6, atol=1e-6)
+    # the ball's faces hold its velocity (or it plus the box's where both cover them)
+    assert np.count_nonzero(fields[0].to_array() == np.float32(1.5)) > 0
+    with pytest.raises(Exception, match="mode"):
+        B.BoundarySolver3(g, fields, inflows=[(inflows[0][0], (0, 0, 0), "replace")])
+
+
+def test_outflow_dirichlet_matches_oracle_and_drives_a_projection(flux):
+    """createOutflowPressureDirichlet3 over two outflow objects combined with
+    a user pin (combineDirichlet3): the fields equal the oracle's; a
+    projection with them has zero pressure inside the outflows, the pin's
+    target at the pin, and converges"""
+    n, h, o = (20, 16, 12), (0.5, 0.5, 0.5), (0.0, 0.0, 0.0)
+    g = Grid(n, h, o)
+    o1 = O.collider_sdf3([B.sphere((8.0, 7.5, 3.0), 1.6)], n, h, o)
+    o2 = O.collider_sdf3([B.box((1.5, 4.0, 3.0), (1.0, 4.0, 3.0))], n, h, o)
+    cells = int(np.prod(n))
+    ua = np.zeros(cells, np.uint8)
+    ut = np.zeros(cells, np.float32)
+    pin = 3 + n[0] * (2 + n[1] * 4)
+    ua[pin] = 1
+    ut[pin] = 0.75
+    act, tgt = B.outflow_dirichlet3(g, [F.array_n("float", n, o1), F.array_n("float", n, o2)],
+                                    F.array_n("u8", n, ua), F.array_n("float", n, ut))
+    wa, wt = O.outflow_dirichlet3([o1, o2], n, h, o, ua, ut)
+    np.testing.assert_array_equal(act.to_array(), wa)
+    np.testing.assert_array_equal(tgt.to_array(), wt)
+    assert wa.sum() > 1 + 10 and wa[pin] == 1
+    rng = np.random.default_rng(3)
+    vel = [F.array_n("float", F.face_shape(n, a), rng.standard_normal(int(np.prod(F.face_shape(n, a))))
+                     .astype(np.float32)) for a in range(3)]
+    ps = F.create_grid_pressure_solver3(resolution=n, grid_spacing=h, dirichlet=(act, tgt),
+                                        multigrid={"levels": 2}, tolerance=1e-6)
+    d = ps.project(vel)
+    assert d["converged"], d
+    p = ps.pressure.to_array()
+    np.testing.assert_allclose(p[(wa != 0) & (np.arange(cells) != pin)], 0.0, atol=1e-6)
+    assert abs(p[pin] - 0.75) <= 1e-6

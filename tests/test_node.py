@@ -245,3 +245,41 @@ def test_async_projection_dot_reducer_and_cg_through_node(tmp_path):
     assert d["ramp"] == 7140
     assert d["cgSucceeded"]
     np.testing.assert_allclose(d["cgX"], 1.0 / np.arange(1, 9), atol=1e-3)
+
+
+@pytest.mark.gpu
+def test_collider_inflow_outflow_through_node_match_oracle(tmp_path):
+    """createSDFInflow3 in constrainVelocity, createOutflowPressureDirichlet3
+    combined with a user pin by combineDirichlet3 (evaluated on the GPU), and
+    MacCormack advection through a collider, from Node, against the oracle"""
+    run_node("collider3.mjs", tmp_path, timeout=300)
+    meta = json.load(open(tmp_path / "meta.json"))
+    n, h, o = (12, 12, 12), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    rd = lambda name: np.fromfile(tmp_path / (name + ".bin"), np.float32)
+    vin = [rd("vin_" + c) for c in "UVW"]
+    want = O.constrain_velocity3(vin, n, h, o, closed_flags=63, depth=5,
+                                 inflows=[(rd("inflow_sdf"), (1.5, -0.25, 0.5), "set")])
+    for a, c in enumerate("UVW"):
+        np.testing.assert_allclose(rd("vout_" + c), want[a], rtol=1e-6, atol=1e-6)
+    assert np.count_nonzero(rd("vout_U") == np.float32(1.5)) > 0
+    # outflow Dirichlet + pin
+    cells = 12 ** 3
+    ua = np.zeros(cells, np.uint8)
+    ut = np.zeros(cells, np.float32)
+    p = meta["pin"][0] + 12 * (meta["pin"][1] + 12 * meta["pin"][2])
+    ua[p], ut[p] = 1, 0.75
+    wa, wt = O.outflow_dirichlet3([rd("outflow_sdf")], n, h, o, ua, ut)
+    mask = np.fromfile(tmp_path / "mask.bin", np.uint8)
+    np.testing.assert_array_equal(mask, wa)
+    assert meta["diag"]["converged"]
+    pr = rd("pressure")
+    np.testing.assert_allclose(pr[(wa != 0) & (np.arange(cells) != p)], 0.0, atol=1e-6)
+    assert abs(pr[p] - 0.75) <= 1e-6
+    assert meta["hostInside"]["active"] and meta["hostPin"]["active"] and meta["hostPin"]["target"] == 0.75
+    assert not meta["hostFree"]["active"]
+    # advection through the collider
+    V = [rd("avel_" + c) for c in "UVW"]
+    prev = [np.full(int(np.prod(O.face_shape(n, a))), -3.0, np.float32) for a in range(3)]
+    want = O.advect_faces3_c(V, V, n, h, o, rd("collider_sdf"), 1.7, order=2, out=prev)
+    for a, c in enumerate("UVW"):
+        np.testing.assert_array_equal(rd("aout_" + c), want[a])

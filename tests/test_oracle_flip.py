@@ -159,6 +159,41 @@ def test_constrain_walls_and_clamp():
     assert np.all(np.abs(np.concatenate(out)) <= 100)
 
 
+def test_constrain_inflows_set_add_and_order():
+    """inflows after the walls, in order, by the sign of the SDF sampled at
+    each face: half-spaces phi = x - 3.25 ('set') and phi = 1.75 - y
+    ('add'), linear so the trilinear samples are exact; faces at x < 3.25
+    take the set velocity, faces at y > 1.75 add theirs, and where both
+    hold the later inflow wins ('add' after 'set' adds to the set value,
+    'set' after 'add' overwrites it)"""
+    n, h, o = (6, 5, 4), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(4)
+    vel = [rng.uniform(-5, 5, int(np.prod(O.face_shape(n, a)))).astype(np.float32) for a in range(3)]
+    k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    phi_set = ((i + 0.5) - 3.25).astype(np.float32).ravel()
+    phi_add = (1.75 - (j + 0.5)).astype(np.float32).ravel()
+    vs, va = (2.0, -1.0, 0.5), (0.25, 3.0, -0.75)
+    for order in ("set-add", "add-set"):
+        inflows = [(phi_set, vs, "set"), (phi_add, va, "add")]
+        if order == "add-set":
+            inflows = inflows[::-1]
+        out = O.constrain_velocity3(vel, n, h, o, closed_flags=0, inflows=inflows)
+        for a in range(3):
+            fs = O.face_shape(n, a)
+            kk, jj, ii = np.meshgrid(np.arange(fs[2]), np.arange(fs[1]), np.arange(fs[0]), indexing="ij")
+            x = ii + (0.0 if a == 0 else 0.5)
+            y = jj + (0.0 if a == 1 else 0.5)
+            in_set = (x < 3.25).ravel()
+            in_add = (y > 1.75).ravel()
+            want = vel[a].copy()
+            for name in order.split("-"):
+                if name == "set":
+                    want = np.where(in_set, np.float32(vs[a]), want)
+                else:
+                    want = np.where(in_add, want + np.float32(va[a]), want)
+            np.testing.assert_array_equal(out[a], want.astype(np.float32), err_msg="%s axis %d" % (order, a))
+
+
 @pytest.mark.parametrize("use_weights", [False, True])
 def test_constrain_no_flux_against_a_planar_wall(use_weights):
     """A solid half-space x >= 8 and a uniform velocity (a, b, c): blocked faces

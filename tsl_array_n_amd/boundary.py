@@ -6,7 +6,7 @@ level_set_utils.js) and constrainVelocity
 """
 import ctypes as C
 
-from . import _check, _lib
+from . import _check, _lib, array_n
 from .transfer import _mac
 
 # closedDomainBoundaryFlag bits (constant.js:24-37)
@@ -114,18 +114,43 @@ def collider_sdf(n, h, origin, shapes, sdf, invert=False):
     _check(_lib.flux_collider_sdf(C.byref(g), arr, cnt, int(bool(invert)), C.c_void_p(sdf.h)), "addShapes")
 
 
+def outflow_dirichlet3(grid, outflows, user_active=None, user_target=None, active=None, target=None):
+    """createOutflowPressureDirichlet3(outflows) combined with the caller's
+    Dirichlet fields by combineDirichlet3 (sdf_inflow_outflow3.js:44-94), at
+    the cell centres of `grid`: returns (active u8, target float) fields for
+    the pressure solver's `dirichlet`.  Each outflow is its SDF field,
+    cell-centred on the same grid."""
+    n = grid.n
+    active = active if active is not None else array_n("u8", n)
+    target = target if target is not None else array_n("float", n)
+    arr = (C.c_void_p * max(1, len(outflows)))(*[o.h for o in outflows])
+    _check(_lib.flux_outflow3_dirichlet(C.byref(grid.c), arr, len(outflows),
+                                        C.c_void_p(user_active.h) if user_active is not None else None,
+                                        C.c_void_p(user_target.h) if user_target is not None else None,
+                                        C.c_void_p(active.h), C.c_void_p(target.h)), "createOutflowPressureDirichlet3")
+    return active, target
+
+
 class BoundarySolver3:
     """createGridBlockedBoundaryConditionSolver3 for a static collider:
     constrain_velocity(extrapolation_depth=5)."""
 
-    def __init__(self, grid, vel, sdf=None, face_weights=None, friction=0.5):
-        self._keep = (vel, sdf, face_weights)
+    def __init__(self, grid, vel, sdf=None, face_weights=None, friction=0.5, inflows=()):
+        """inflows: (sdf field, (vx, vy, vz), mode 'set' | 'add') triples
+        (createSDFInflow3 objects, their SDF on the velocity grid's cells)"""
+        self._keep = (vel, sdf, face_weights, tuple(q[0] for q in inflows))
         h = C.c_void_p()
         fw = _mac(face_weights) if face_weights is not None else None
         _check(_lib.flux_boundary3_create(C.byref(grid.c), _mac(vel), C.c_void_p(sdf.h) if sdf is not None else None,
                                           fw, C.c_float(friction), C.byref(h)), "boundary")
         self.h = h.value
         self._flags = DIRECTION_ALL_3D
+        for isdf, v, mode in inflows:
+            if mode not in ("set", "add"):
+                raise ValueError("setInflows: mode must be 'set' or 'add', got %r" % (mode,))
+            _check(_lib.flux_boundary3_add_inflow(C.c_void_p(self.h), C.c_void_p(isdf.h), C.c_float(v[0]),
+                                                  C.c_float(v[1]), C.c_float(v[2]), 1 if mode == "add" else 0),
+                   "setInflows")
 
     @property
     def closed_domain_boundary_flag(self):

@@ -1,8 +1,11 @@
 // createSemiLagrangianAdvectionSolver3 for gfx950 (packages/fluxflow/src/grid/
-// advection_solver3.js:67-362), no collider: a velocity grid bound at
-// creation, advectFaceCentered3 and advectScalar3 of any face / cell grid
-// through it, order 1 (semi-Lagrangian, monotone tricubic sample at the
-// midpoint back-trace) or order 2 (MacCormack with the min/max clamp).
+// advection_solver3.js:67-362): a velocity grid bound at creation,
+// advectFaceCentered3 and advectScalar3 of any face / cell grid through it,
+// order 1 (semi-Lagrangian, monotone tricubic sample at the midpoint
+// back-trace) or order 2 (MacCormack with the min/max clamp).  With a static
+// collider (its SDF cell-centred on the velocity grid) the back-trace stops
+// at the collider, points inside it keep the output's value, and the order-2
+// clamp ranges over the corners outside it.
 //
 // Order 2 runs in two launches instead of the reference's three: the forward
 // launch also gathers the clamp range (min/max of the input over the 8
@@ -24,6 +27,7 @@ struct flux_advection3 {
     float dt = 0.0f;
     int max_sub = 32;
     int order = 1;
+    const flux_field* collider = nullptr;  // the collider's SDF on the cells, or none
     float* fwd = nullptr;      // order 2: forward result, one slot per sample point
     float2* range = nullptr;   // order 2: clamp range per sample point
     int64_t scratch = 0;       // slots in fwd / range
@@ -77,21 +81,27 @@ __device__ __forceinline__ int locate(const SampleSet& S, int t, int s[3], float
 
 // order 1 (advection_solver3.js:142-176, 287-298): the monotone tricubic
 // sample of the data at the back-trace through the velocity grid
-__global__ void __launch_bounds__(ATPB) k_adv_order1(G3 g, Mac vel, SampleSet S, float dt, int max_sub) {
+__global__ void __launch_bounds__(ATPB) k_adv_order1(G3 g, Mac vel, Sdf3 col, SampleSet S, float dt, int max_sub) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.start[S.count]) return;
     int s[3], f;
     float o[3], pos[3], tr[3];
     const int c = locate(S, t, s, o, g, pos, f);
+    if (!(sdf_sample(col, pos) > 0.0f)) return;  // inside the collider: the output keeps its value
     const float h[3] = {g.h0, g.h1, g.h2};
-    back_trace(g, vel, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, 1.0f, tr);
+    back_trace_c(g, vel, col, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, 1.0f, tr);
     pick(S.out, c)[f] = sample_cubic(pick(S.data, c), s, o, h, tr);
 }
 
 // order 2, forward (makeForward, advection_solver3.js:195-222; the scalar
 // form 320-334) and the clamp range of the same trace (gatherLocalMinMax,
 // 95-120): min / max of the data over the trilinear corners
-__global__ void __launch_bounds__(ATPB) k_mc_forward(G3 g, Mac vel, SampleSet S, float dt, int max_sub) {
+// With the collider a point inside it forwards its own input value (and has
+// no range: its correction is skipped), and only corners whose own positions
+// lie outside the collider enter the range; with none the range stays
+// empty (1e30, -1e30), so the correction falls back to fwd, as
+// clampCorrection's haveValid does.
+__global__ void __launch_bounds__(ATPB) k_mc_forward(G3 g, Mac vel, Sdf3 col, SampleSet S, float dt, int max_sub) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.start[S.count]) return;
     int s[3], f;
@@ -99,7 +109,11 @@ __global__ void __launch_bounds__(ATPB) k_mc_forward(G3 g, Mac vel, SampleSet S,
     const int c = locate(S, t, s, o, g, pos, f);
     const float h[3] = {g.h0, g.h1, g.h2};
     const float* __restrict__ d = pick(S.data, c);
-    back_trace(g, vel, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, 1.0f, tr);
+    if (!(sdf_sample(col, pos) > 0.0f)) {
+        pick(S.out, c)[f] = d[f];
+        return;
+    }
+    back_trace_c(g, vel, col, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, 1.0f, tr);
     pick(S.out, c)[f] = sample_cubic(d, s, o, h, tr);
     Tri T;
     tri_weights(tr, o, h, s, T);
@@ -107,6 +121,10 @@ __global__ void __launch_bounds__(ATPB) k_mc_forward(G3 g, Mac vel, SampleSet S,
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int i = (q & 1) ? T.i1 : T.i0, j = (q & 2) ? T.j1 : T.j0, k = (q & 4) ? T.k1 : T.k0;
+        if (col.d) {
+            const float cp[3] = {o[0] + (float)i * h[0], o[1] + (float)j * h[1], o[2] + (float)k * h[2]};
+            if (!(sdf_sample(col, cp) > 0.0f)) continue;
+        }
         const float v = d[i + s[0] * (j + s[1] * k)];
         lo = fminf(lo, v);
         hi = fmaxf(hi, v);
@@ -118,15 +136,16 @@ __global__ void __launch_bounds__(ATPB) k_mc_forward(G3 g, Mac vel, SampleSet S,
 // makeCorrectAndClamp 256-274; clampCorrection 122-128): the forward result
 // sampled at the trace with direction -1, fwd + (in - bwd) / 2 kept where it
 // lies within the range, else fwd
-__global__ void __launch_bounds__(ATPB) k_mc_correct(G3 g, Mac vel, SampleSet S, float dt, int max_sub) {
+__global__ void __launch_bounds__(ATPB) k_mc_correct(G3 g, Mac vel, Sdf3 col, SampleSet S, float dt, int max_sub) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.start[S.count]) return;
     int s[3], f;
     float o[3], pos[3], tr[3];
     const int c = locate(S, t, s, o, g, pos, f);
+    if (!(sdf_sample(col, pos) > 0.0f)) return;  // inside the collider: the output keeps its value
     const float h[3] = {g.h0, g.h1, g.h2};
     const float* __restrict__ fw = pick(S.fwd, c);
-    back_trace(g, vel, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, -1.0f, tr);
+    back_trace_c(g, vel, col, pos, dt, fminf(g.h0, fminf(g.h1, g.h2)), max_sub, -1.0f, tr);
     const float bwd = sample_cubic(fw, s, o, h, tr);
     const float fv = fw[f];
     const float corrected = fv + (pick(S.data, c)[f] - bwd) * 0.5f;
@@ -151,9 +170,10 @@ static int run(flux_advection3* a, SampleSet S) {
     if (total == 0) return FLUX_OK;
     const G3 g = make_g3(&a->g);
     const Mac m = vel_of(a);
+    const Sdf3 col = sdf3_on(g, a->collider ? (const float*)a->collider->d : nullptr);
     const unsigned nb = blocks_for(total, ATPB);
     if (a->order == 1) {
-        k_adv_order1<<<nb, ATPB, 0, g_stream>>>(g, m, S, a->dt, a->max_sub);
+        k_adv_order1<<<nb, ATPB, 0, g_stream>>>(g, m, col, S, a->dt, a->max_sub);
     } else {
         FLUX_REQUIRE(total <= a->scratch, "advection: %d points exceed the scratch of %lld", total,
                      (long long)a->scratch);
@@ -163,9 +183,9 @@ static int run(flux_advection3* a, SampleSet S) {
             S.fwd[c] = a->fwd + S.start[c];
             S.range[c] = a->range + S.start[c];
         }
-        k_mc_forward<<<nb, ATPB, 0, g_stream>>>(g, m, S, a->dt, a->max_sub);
+        k_mc_forward<<<nb, ATPB, 0, g_stream>>>(g, m, col, S, a->dt, a->max_sub);
         for (int c = 0; c < S.count; ++c) S.out[c] = out[c];
-        k_mc_correct<<<nb, ATPB, 0, g_stream>>>(g, m, S, a->dt, a->max_sub);
+        k_mc_correct<<<nb, ATPB, 0, g_stream>>>(g, m, col, S, a->dt, a->max_sub);
     }
     FLUX_LAUNCH_CHECK();
     return FLUX_OK;
@@ -273,6 +293,14 @@ int flux_advection3_create(const flux_grid3* g, const flux_field* const* velocit
 int flux_advection3_set_dt(flux_advection3* a, float dt) {
     FLUX_REQUIRE(a, "advection: NULL solver");
     a->dt = dt;
+    return FLUX_OK;
+}
+
+int flux_advection3_set_collider(flux_advection3* a, const flux_field* collider_sdf) {
+    FLUX_REQUIRE(a, "advection: NULL solver");
+    if (collider_sdf)
+        FLUX_TRY(check_field(collider_sdf, FLUX_F32, cells_of(a->g.n), "createSemiLagrangianAdvectionSolver3: collider"));
+    a->collider = collider_sdf;
     return FLUX_OK;
 }
 

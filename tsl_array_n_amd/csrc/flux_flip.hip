@@ -244,35 +244,6 @@ __global__ void k_collider_sdf(G3D g, const flux_primitive3* __restrict__ shapes
 }
 
 // ------------------------------------------------------ velocity constraint
-// trilinearGradientAtPosition3 (grid_math3.js:295-356)
-__device__ __forceinline__ void grad_axis(float coord, int n, int& lo, int& hi, float& t) {
-    const int idx = (int)floorf(coord);
-    int l = idx < n - 2 ? idx : n - 2;
-    l = l > 0 ? l : 0;
-    lo = l;
-    hi = l + 1 < n - 1 ? l + 1 : n - 1;
-    const float tt = coord - (float)l;
-    t = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
-}
-
-__device__ void sdf_gradient(const float* __restrict__ d, const int n[3], const float o[3], const float h[3],
-                             const float p[3], float g[3]) {
-    int xl, xh, yl, yh, zl, zh;
-    float tx, ty, tz;
-    grad_axis((p[0] - o[0]) / h[0], n[0], xl, xh, tx);
-    grad_axis((p[1] - o[1]) / h[1], n[1], yl, yh, ty);
-    grad_axis((p[2] - o[2]) / h[2], n[2], zl, zh, tz);
-    auto D = [&](int i, int j, int k) { return d[i + n[0] * (j + n[1] * k)]; };
-    const float v000 = D(xl, yl, zl), v100 = D(xh, yl, zl), v010 = D(xl, yh, zl), v110 = D(xh, yh, zl);
-    const float v001 = D(xl, yl, zh), v101 = D(xh, yl, zh), v011 = D(xl, yh, zh), v111 = D(xh, yh, zh);
-    g[0] = ((((1.0f - ty) * (1.0f - tz)) * (v100 - v000) + (ty * (1.0f - tz)) * (v110 - v010)) +
-            ((1.0f - ty) * tz) * (v101 - v001) + (ty * tz) * (v111 - v011)) / h[0];
-    g[1] = ((((1.0f - tx) * (1.0f - tz)) * (v010 - v000) + (tx * (1.0f - tz)) * (v110 - v100)) +
-            ((1.0f - tx) * tz) * (v011 - v001) + (tx * tz) * (v111 - v101)) / h[1];
-    g[2] = ((((1.0f - tx) * (1.0f - ty)) * (v001 - v000) + (tx * (1.0f - ty)) * (v101 - v100)) +
-            ((1.0f - tx) * ty) * (v011 - v010) + (tx * ty) * (v111 - v110)) / h[2];
-}
-
 // fillMarker + markAndProject (…solver3.js:200-262) for a static collider:
 // an open face is fluid; a closed one takes the collider's velocity (0).
 __global__ void k_mark_and_project(G3 g, int axis, const float* __restrict__ sdf, const float* __restrict__ weight,
@@ -374,7 +345,22 @@ __global__ void k_blocked_boundary(G3 g, const float* __restrict__ sdf, float* _
 
 // zeroULeft ... zeroWFront (…solver3.js:118-125) by closedDomainBoundaryFlag,
 // then clampVelocityU/V/W (±100, non-finite -> 0, 74-96), one face field per launch.
-__global__ void k_walls_and_clamp(G3 g, int axis, int flags, float* __restrict__ d) {
+// inflow objects (createSDFInflow3): SDF cell-centred on the velocity grid,
+// velocity, mode 0 'set' / 1 'add'
+static constexpr int kMaxInflows3 = 4;
+struct Inflows3 {
+    const float* sdf[kMaxInflows3];
+    float vel[kMaxInflows3][3];
+    int mode[kMaxInflows3];
+    int count;
+};
+
+// The walls by flag (zeroULeft ... zeroWFront), then every inflow in order
+// (buildInflowKernels, grid_blocked_boundary_condition_solver3.js:130-176:
+// a face whose position is inside the inflow's SDF takes or adds its
+// velocity component), then the +-100 clamp: each is a per-face update, so
+// one thread applies them to its face in the reference's order.
+__global__ void k_walls_and_clamp(G3 g, int axis, int flags, Inflows3 inflows, float* __restrict__ d) {
     const int s[3] = {g.n0 + (axis == 0), g.n1 + (axis == 1), g.n2 + (axis == 2)};
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= s[0] * s[1] * s[2]) return;
@@ -383,7 +369,52 @@ __global__ void k_walls_and_clamp(G3 g, int axis, int flags, float* __restrict__
     const int lo_bit = 1 << (2 * axis), hi_bit = 1 << (2 * axis + 1);
     if ((flags & lo_bit) && I[axis] == 0) v = 0.0f;
     if ((flags & hi_bit) && I[axis] == s[axis] - 1) v = 0.0f;
+    if (inflows.count > 0) {
+        const int n[3] = {g.n0, g.n1, g.n2};
+        const float h[3] = {g.h0, g.h1, g.h2};
+        const float so[3] = {g.o0 + 0.5f * g.h0, g.o1 + 0.5f * g.h1, g.o2 + 0.5f * g.h2};
+        float fo[3];
+        mac_origin(g, axis, fo);
+        const float pt[3] = {fo[0] + (float)I[0] * h[0], fo[1] + (float)I[1] * h[1], fo[2] + (float)I[2] * h[2]};
+        for (int q = 0; q < inflows.count; ++q) {
+            if (!(sample_tri(inflows.sdf[q], n, so, h, pt) < 0.0f)) continue;
+            const float value = inflows.vel[q][axis];
+            v = inflows.mode[q] == 1 ? v + value : value;
+        }
+    }
     d[f] = !isfinite(v) ? 0.0f : fminf(fmaxf(v, -100.0f), 100.0f);
+}
+
+// createOutflowPressureDirichlet3 + combineDirichlet3(outflow, user)
+// (sdf_inflow_outflow3.js:44-94) at the cell centres: active = inside any
+// outflow (its SDF, cell-centred on this grid, sampled < 0) or the user's
+// mask; target = the user's target where the user's mask is set, else 0
+static constexpr int kMaxOutflows3 = 4;
+struct Outflows3 {
+    const float* sdf[kMaxOutflows3];
+    int count;
+};
+
+__global__ void k_outflow_dirichlet3(G3 g, Outflows3 o, const uint8_t* __restrict__ user_active,
+                                     const float* __restrict__ user_target, uint8_t* __restrict__ active,
+                                     float* __restrict__ target) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.n0 * g.n1 * g.n2) return;
+    const int i = c % g.n0, j = (c / g.n0) % g.n1, k = c / (g.n0 * g.n1);
+    const int n[3] = {g.n0, g.n1, g.n2};
+    const float h[3] = {g.h0, g.h1, g.h2};
+    const float so[3] = {g.o0 + g.h0 * 0.5f, g.o1 + g.h1 * 0.5f, g.o2 + g.h2 * 0.5f};
+    const float p[3] = {so[0] + h[0] * (float)i, so[1] + h[1] * (float)j, so[2] + h[2] * (float)k};
+    bool act = false;
+    for (int q = 0; q < o.count; ++q) act = act || sample_tri(o.sdf[q], n, so, h, p) < 0.0f;
+    float tg = 0.0f;
+    if (user_active) {
+        const bool ua = user_active[c] != 0;
+        act = act || ua;
+        tg = ua ? (user_target ? user_target[c] : 0.0f) : 0.0f;
+    }
+    active[c] = (uint8_t)act;
+    target[c] = tg;
 }
 
 }  // namespace flux
@@ -469,6 +500,7 @@ struct flux_boundary3 {
     float* tmp[3] = {nullptr, nullptr, nullptr};
     flux_field* marker_field[3] = {nullptr, nullptr, nullptr};
     flux_extrapolator* ex[3] = {nullptr, nullptr, nullptr};
+    Inflows3 inflows{};
 };
 
 extern "C" {
@@ -573,6 +605,32 @@ int flux_collider_face_weights(const flux_grid3d* g, const flux_primitive3* shap
     return FLUX_OK;
 }
 
+int flux_outflow3_dirichlet(const flux_grid3* g, const flux_field* const* outflow_sdf, int count,
+                            const flux_field* user_active, const flux_field* user_target, flux_field* active,
+                            flux_field* target) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(g && active && target && (count == 0 || outflow_sdf), "createOutflowPressureDirichlet3: NULL argument");
+    FLUX_REQUIRE(count >= 0 && count <= kMaxOutflows3, "createOutflowPressureDirichlet3: at most %d outflows, got %d",
+                 kMaxOutflows3, count);
+    for (int a = 0; a < 3; ++a) FLUX_REQUIRE(g->n[a] > 0 && g->h[a] > 0, "createOutflowPressureDirichlet3: bad grid");
+    const int64_t cells = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    FLUX_TRY(check_field(active, FLUX_U8, cells, "dirichlet active"));
+    FLUX_TRY(check_field(target, FLUX_F32, cells, "dirichlet target"));
+    Outflows3 o{};
+    o.count = count;
+    for (int q = 0; q < count; ++q) {
+        FLUX_TRY(check_field(outflow_sdf[q], FLUX_F32, cells, "outflow sdf"));
+        o.sdf[q] = (const float*)outflow_sdf[q]->d;
+    }
+    if (user_active) FLUX_TRY(check_field(user_active, FLUX_U8, cells, "dirichlet active"));
+    if (user_target) FLUX_TRY(check_field(user_target, FLUX_F32, cells, "dirichlet target"));
+    k_outflow_dirichlet3<<<blocks_for(cells, FTPB), FTPB, 0, g_stream>>>(
+        make_g3(g), o, user_active ? (const uint8_t*)user_active->d : nullptr,
+        user_target ? (const float*)user_target->d : nullptr, (uint8_t*)active->d, (float*)target->d);
+    FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
 int flux_collider_sdf(const flux_grid3d* g, const flux_primitive3* shapes, int count, int invert, flux_field* sdf) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(g && sdf && (shapes || count == 0) && count >= 0, "addShapes: bad arguments");
@@ -671,9 +729,24 @@ int flux_boundary3_constrain(flux_boundary3* b, int depth) {
     }
     for (int a = 0; a < 3; ++a) {
         const int64_t nf = face_count(b->g.n, a);
-        k_walls_and_clamp<<<blocks_for(nf, FTPB), FTPB, 0, g_stream>>>(g, a, b->flags, (float*)b->vel[a]->d);
+        k_walls_and_clamp<<<blocks_for(nf, FTPB), FTPB, 0, g_stream>>>(g, a, b->flags, b->inflows,
+                                                                      (float*)b->vel[a]->d);
     }
     FLUX_LAUNCH_CHECK();
+    return FLUX_OK;
+}
+
+int flux_boundary3_add_inflow(flux_boundary3* b, const flux_field* sdf, float vx, float vy, float vz, int mode) {
+    FLUX_REQUIRE(b && sdf, "setInflows: NULL argument");
+    FLUX_REQUIRE(b->inflows.count < kMaxInflows3, "setInflows: at most %d inflows", kMaxInflows3);
+    FLUX_REQUIRE(mode == 0 || mode == 1, "setInflows: mode must be 'set' (0) or 'add' (1)");
+    FLUX_TRY(check_field(sdf, FLUX_F32, (int64_t)b->g.n[0] * b->g.n[1] * b->g.n[2], "inflow sdf"));
+    const int q = b->inflows.count++;
+    b->inflows.sdf[q] = (const float*)sdf->d;
+    b->inflows.vel[q][0] = vx;
+    b->inflows.vel[q][1] = vy;
+    b->inflows.vel[q][2] = vz;
+    b->inflows.mode[q] = mode;
     return FLUX_OK;
 }
 

@@ -130,6 +130,26 @@ __device__ __forceinline__ int f2i_sat(float v) {
     return (int)v;
 }
 
+// pushOutOfCollider (grid_flip_solver3.js:945-964): a particle where the
+// collider's SDF is below `thresh` moves along the SDF's normalised
+// (trilinear) gradient by (thresh - sdf) + shift
+__global__ void k_push_out(Sdf3 col, float thresh, float shift, float* __restrict__ pos, int count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+    const float v = sdf_sample(col, pp);
+    if (!(v < thresh)) return;
+    const int n[3] = {col.n0, col.n1, col.n2};
+    const float o[3] = {col.o0, col.o1, col.o2}, h[3] = {col.h0, col.h1, col.h2};
+    float g[3];
+    sdf_gradient(col.d, n, o, h, pp, g);
+    const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+    if (!(len > 0.0f)) return;
+    const float dist = (thresh - v) + shift;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pos[3 * p + a] = pp[a] + (g[a] / len) * dist;
+}
+
 // accumulateConcentration (grid_flip_solver3.js:992-998): count and the
 // fixed-point sum of clamp(c, 0, 1) per cell (integer atomics, order-free).
 // A block takes kCellPPT * STPB consecutive particles; when their cells lie in
@@ -745,7 +765,14 @@ int flux_flip3_step(flux_flip3* s, flux_pressure_diag* diag) {
     FLUX_REQUIRE(s, "onAdvanceTimeStep: NULL solver");
     const flux_flip3_desc& d = s->d;
     // preP2G: advectParticles (933-940)
-    FLUX_TRY(flux_advect_particles(&s->g, s->positions, s->np, s->vel, d.dt, d.max_substeps));
+    FLUX_TRY(flux_advect_particles_collider(&s->g, s->positions, s->np, s->vel, d.collider_sdf, d.dt,
+                                            d.max_substeps));
+    if (d.collider_sdf && s->np > 0) {
+        k_push_out<<<blocks_for(s->np, STPB), STPB, 0, g_stream>>>(
+            sdf3_on(make_g3(&s->g), (const float*)d.collider_sdf->d), d.collider_push_thresh, d.collider_push_shift,
+            (float*)s->positions->d, (int)s->np);
+        FLUX_LAUNCH_CHECK();
+    }
     // afterP2G (1083-1114): scatter + finalize (mass-weighted with density
     // coupling), velOld snapshot, weight extrapolation, constrainVelocity,
     // fluid mask, adjacency, resampling, the concentration pass

@@ -98,9 +98,62 @@ __device__ __forceinline__ void sample_mac(const G3& g, const Mac& m, const floa
     }
 }
 
-// backTrace without a collider (advection_solver3.js:20-62)
-__device__ __forceinline__ void back_trace(const G3& g, const Mac& m, const float start[3], float dt, float hmin,
-                                           int max_sub, float direction, float out[3]) {
+// A static collider's SDF, cell-centred on the velocity grid (the collider's
+// sample(), sdf_collider3.js:33-37: collocatedValueAtPosition3); d == nullptr
+// is no collider, which samples as 1 (advection_solver3.js:78-82).
+struct Sdf3 {
+    const float* d;
+    int n0, n1, n2;
+    float o0, o1, o2;  // the cell-centre origin
+    float h0, h1, h2;
+};
+
+inline Sdf3 sdf3_on(const G3& g, const float* d) {
+    return Sdf3{d, g.n0, g.n1, g.n2, g.o0 + g.h0 * 0.5f, g.o1 + g.h1 * 0.5f, g.o2 + g.h2 * 0.5f, g.h0, g.h1, g.h2};
+}
+
+__device__ __forceinline__ float sdf_sample(const Sdf3& c, const float p[3]) {
+    if (!c.d) return 1.0f;
+    const int s[3] = {c.n0, c.n1, c.n2};
+    const float o[3] = {c.o0, c.o1, c.o2}, h[3] = {c.h0, c.h1, c.h2};
+    return sample_tri(c.d, s, o, h, p);
+}
+
+// trilinearGradientAtPosition3 (grid_math3.js:295-356)
+__device__ __forceinline__ void grad_axis(float coord, int n, int& lo, int& hi, float& t) {
+    const int idx = (int)floorf(coord);
+    int l = idx < n - 2 ? idx : n - 2;
+    l = l > 0 ? l : 0;
+    lo = l;
+    hi = l + 1 < n - 1 ? l + 1 : n - 1;
+    const float tt = coord - (float)l;
+    t = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
+}
+
+__device__ __forceinline__ void sdf_gradient(const float* __restrict__ d, const int n[3], const float o[3], const float h[3],
+                             const float p[3], float g[3]) {
+    int xl, xh, yl, yh, zl, zh;
+    float tx, ty, tz;
+    grad_axis((p[0] - o[0]) / h[0], n[0], xl, xh, tx);
+    grad_axis((p[1] - o[1]) / h[1], n[1], yl, yh, ty);
+    grad_axis((p[2] - o[2]) / h[2], n[2], zl, zh, tz);
+    auto D = [&](int i, int j, int k) { return d[i + n[0] * (j + n[1] * k)]; };
+    const float v000 = D(xl, yl, zl), v100 = D(xh, yl, zl), v010 = D(xl, yh, zl), v110 = D(xh, yh, zl);
+    const float v001 = D(xl, yl, zh), v101 = D(xh, yl, zh), v011 = D(xl, yh, zh), v111 = D(xh, yh, zh);
+    g[0] = ((((1.0f - ty) * (1.0f - tz)) * (v100 - v000) + (ty * (1.0f - tz)) * (v110 - v010)) +
+            ((1.0f - ty) * tz) * (v101 - v001) + (ty * tz) * (v111 - v011)) / h[0];
+    g[1] = ((((1.0f - tx) * (1.0f - tz)) * (v010 - v000) + (tx * (1.0f - tz)) * (v110 - v100)) +
+            ((1.0f - tx) * tz) * (v011 - v001) + (tx * tz) * (v111 - v101)) / h[1];
+    g[2] = ((((1.0f - tx) * (1.0f - ty)) * (v001 - v000) + (tx * (1.0f - ty)) * (v101 - v100)) +
+            ((1.0f - tx) * ty) * (v011 - v010) + (tx * ty) * (v111 - v110)) / h[2];
+}
+
+// backTrace (advection_solver3.js:20-62): RK2 midpoint sub-steps; with a
+// collider a sub-step whose end lies in it (phi1 <= 0) stops at the
+// SDF-weighted blend of its two ends, w = |phi1| / (|phi0| + |phi1|), and
+// ends the trace.
+__device__ __forceinline__ void back_trace_c(const G3& g, const Mac& m, const Sdf3& col, const float start[3],
+                                             float dt, float hmin, int max_sub, float direction, float out[3]) {
     float pt[3] = {start[0], start[1], start[2]};
     float rem = dt;
     for (int s = 0; s < max_sub; ++s) {
@@ -117,13 +170,33 @@ __device__ __forceinline__ void back_trace(const G3& g, const Mac& m, const floa
         for (int a = 0; a < 3; ++a) mid[a] = pt[a] - (v0[a] * half) * direction;
         float mv[3];
         sample_mac(g, m, mid, mv);
+        float next[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) pt[a] = pt[a] - (mv[a] * sub) * direction;
+        for (int a = 0; a < 3; ++a) next[a] = pt[a] - (mv[a] * sub) * direction;
+        if (col.d) {
+            const float phi0 = sdf_sample(col, pt);
+            const float phi1 = sdf_sample(col, next);
+            if (phi1 <= 0.0f) {
+                const float w = fabsf(phi1) / (fabsf(phi0) + fabsf(phi1));
+#pragma unroll
+                for (int a = 0; a < 3; ++a) pt[a] = pt[a] * w + next[a] * (1.0f - w);
+                break;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) pt[a] = next[a];
         rem = rem - sub;
     }
     out[0] = pt[0];
     out[1] = pt[1];
     out[2] = pt[2];
+}
+
+// backTrace without a collider
+__device__ __forceinline__ void back_trace(const G3& g, const Mac& m, const float start[3], float dt, float hmin,
+                                           int max_sub, float direction, float out[3]) {
+    const Sdf3 none{nullptr, 0, 0, 0, 0.f, 0.f, 0.f, 1.f, 1.f, 1.f};
+    back_trace_c(g, m, none, start, dt, hmin, max_sub, direction, out);
 }
 
 // monotonicCubic1d3 (grid_math3.js:137-157)

@@ -284,13 +284,14 @@ __global__ void k_g2p(G3 g, Mac now, Mac old, const float* __restrict__ pos, flo
 }
 
 // advectParticles (grid_flip_solver3.js:933-940)
-__global__ void k_advect_particles(G3 g, Mac m, float* __restrict__ pos, int count, float dt, int max_sub) {
+__global__ void k_advect_particles(G3 g, Mac m, Sdf3 col, float* __restrict__ pos, int count, float dt,
+                                   int max_sub) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= count) return;
     float hmin = fminf(g.h0, fminf(g.h1, g.h2));
     const float pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
     float tr[3];
-    back_trace(g, m, pp, dt, hmin, max_sub, -1.0f, tr);
+    back_trace_c(g, m, col, pp, dt, hmin, max_sub, -1.0f, tr);
     bool bad = !isfinite(tr[0]) || !isfinite(tr[1]) || !isfinite(tr[2]);
     const float o[3] = {g.o0, g.o1, g.o2}, h[3] = {g.h0, g.h1, g.h2};
     const int n[3] = {g.n0, g.n1, g.n2};
@@ -557,21 +558,30 @@ int flux_g2p(const flux_grid3* g, const flux_field* positions, flux_field* veloc
     return FLUX_OK;
 }
 
-int flux_advect_particles(const flux_grid3* g, flux_field* positions, int64_t count, const flux_field* const* vel,
-                          float dt, int max_substeps) {
+int flux_advect_particles_collider(const flux_grid3* g, flux_field* positions, int64_t count,
+                                   const flux_field* const* vel, const flux_field* collider_sdf, float dt,
+                                   int max_substeps) {
     FLUX_REQUIRE_INIT();
     FLUX_TRY(check_grid(g));
     FLUX_TRY(check_mac(g, vel, "advectParticles: velocity"));
     FLUX_REQUIRE(count >= 0 && count < (1ll << 31), "advectParticles: count out of range");
     FLUX_REQUIRE(positions && (int64_t)(positions->bytes / 12) >= count, "advectParticles: positions too small");
     FLUX_REQUIRE(max_substeps >= 1 && max_substeps <= 1024, "advectParticles: maxSubsteps out of range");
+    if (collider_sdf)
+        FLUX_TRY(check_field(collider_sdf, FLUX_F32, (int64_t)g->n[0] * g->n[1] * g->n[2], "advectParticles: collider"));
+    const G3 g3 = make_g3(g);
+    const Sdf3 col = sdf3_on(g3, collider_sdf ? (const float*)collider_sdf->d : nullptr);
     if (count > 0)
-        k_advect_particles<<<blocks_for(count, XTPB), XTPB, 0, g_stream>>>(make_g3(g), mac_of(vel),
-                                                                            (float*)positions->d, (int)count, dt,
-                                                                            max_substeps);
+        k_advect_particles<<<blocks_for(count, XTPB), XTPB, 0, g_stream>>>(g3, mac_of(vel), col, (float*)positions->d,
+                                                                            (int)count, dt, max_substeps);
     FLUX_LAUNCH_CHECK();
     FLUX_HIP(hipStreamSynchronize(g_stream));
     return FLUX_OK;
+}
+
+int flux_advect_particles(const flux_grid3* g, flux_field* positions, int64_t count, const flux_field* const* vel,
+                          float dt, int max_substeps) {
+    return flux_advect_particles_collider(g, positions, count, vel, nullptr, dt, max_substeps);
 }
 
 int flux_advect_faces(const flux_grid3* g, const flux_field* const* vel_in, flux_field* const* vel_out, float dt,

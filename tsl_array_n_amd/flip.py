@@ -28,7 +28,8 @@ class Flip3Desc(C.Structure):
                 ("carry_concentration", C.c_int), ("mixing", C.c_float), ("fade", C.c_float),
                 ("concentration_atomic_scale", C.c_float), ("ambient_density", C.c_float),
                 ("component_density", C.c_float), ("mass_weighted_transfer", C.c_int), ("pressure_pin", C.c_int),
-                ("pin_cell", C.c_int * 3), ("phase_resample", C.c_int)]
+                ("pin_cell", C.c_int * 3), ("phase_resample", C.c_int), ("collider_push_thresh", C.c_float),
+                ("collider_push_shift", C.c_float)]
 
 
 _FIELDS = {"positions": 0, "velocities": 1, "fluid_mask": 2, "u": 3, "v": 4, "w": 5, "pressure": 6,
@@ -42,7 +43,7 @@ class GridFlipSolver3:
                  collider_weights=None, friction=0.5, resample=None, pressure=None, max_substeps=32,
                  extrapolation_depth=5, carry_concentration=False, mixing=0.0, fade=0.0,
                  concentration_atomic_scale=4096.0, ambient_density=None, component_density=None,
-                 mass_weighted_transfer=False, pressure_pin=None):
+                 mass_weighted_transfer=False, pressure_pin=None, collider_push_thresh=0.0, collider_push_shift=0.0):
         if not max_particles:
             raise ValueError("createGridFlipSolver3: options.maxParticles is required.")
         if dt is None:
@@ -89,6 +90,8 @@ class GridFlipSolver3:
             for a in range(3):
                 d.collider_weights[a] = collider_weights[a].h
         d.friction = friction
+        d.collider_push_thresh = float(collider_push_thresh)
+        d.collider_push_shift = float(collider_push_shift)
         p = pressure or {}
         if "multigrid" in p:
             d.pressure.multigrid = mg_options(**p["multigrid"])

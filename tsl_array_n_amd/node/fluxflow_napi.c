@@ -672,6 +672,14 @@ static napi_value js_advection3_set_dt(napi_env env, napi_callback_info info) {
     return undef(env);
 }
 
+static napi_value js_advection3_set_collider(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_advection3_set_collider((flux_advection3*)unwrap(env, a[0], K_ADV3, K_ADV3),
+                                            is_nullish(env, a[1]) ? NULL : get_field(env, a[1])));
+    return undef(env);
+}
+
 static napi_value js_advection3_faces(napi_env env, napi_callback_info info) {
     napi_value a[3];
     args_n(env, info, 3, a);
@@ -844,6 +852,37 @@ static napi_value js_boundary3_create(napi_env env, napi_callback_info info) {
     return wrap(env, K_BND, b);
 }
 
+// boundary3AddInflow(b, sdfField, [vx, vy, vz], mode): a createSDFInflow3 object
+static napi_value js_boundary3_add_inflow(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    args_n(env, info, 4, a);
+    double v[3] = {0.0, 0.0, 0.0};
+    if (get_double_array(env, a[2], v, 3) != 3) {
+        napi_throw_type_error(env, NULL, "setInflows: velocity must be [vx, vy, vz]");
+        return NULL;
+    }
+    CHECK_FLUX(flux_boundary3_add_inflow((flux_boundary3*)unwrap(env, a[0], K_BND, K_BND), get_field(env, a[1]),
+                                         (float)v[0], (float)v[1], (float)v[2], get_int(env, a[3], 0)));
+    return undef(env);
+}
+
+// outflow3Dirichlet(grid, [sdf fields], userActive | null, userTarget | null, active, target)
+static napi_value js_outflow3_dirichlet(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    args_n(env, info, 6, a);
+    flux_grid3 g;
+    const flux_field* o[4] = {NULL, NULL, NULL, NULL};
+    const int no = get_field_array_n(env, a[1], o, 4);
+    if (get_grid(env, a[0], &g) || no < 0) {
+        napi_throw_type_error(env, NULL, "createOutflowPressureDirichlet3: bad arguments");
+        return NULL;
+    }
+    CHECK_FLUX(flux_outflow3_dirichlet(&g, o, no, is_nullish(env, a[2]) ? NULL : get_field(env, a[2]),
+                                       is_nullish(env, a[3]) ? NULL : get_field(env, a[3]), get_field(env, a[4]),
+                                       get_field(env, a[5])));
+    return undef(env);
+}
+
 static napi_value js_boundary3_set_closed(napi_env env, napi_callback_info info) {
     napi_value a[2];
     args_n(env, info, 2, a);
@@ -901,6 +940,8 @@ static napi_value js_flip3_create(napi_env env, napi_callback_info info) {
         return NULL;
     }
     if ((v = prop(env, o, "friction"))) d.friction = (float)get_double(env, v, d.friction);
+    if ((v = prop(env, o, "colliderPushThresh"))) d.collider_push_thresh = (float)get_double(env, v, 0.0);
+    if ((v = prop(env, o, "colliderPushShift"))) d.collider_push_shift = (float)get_double(env, v, 0.0);
     if ((v = prop(env, o, "multigrid"))) mg_opts(env, v, &d.pressure.multigrid);
     if ((v = prop(env, o, "preconditionerKind"))) d.pressure.preconditioner = get_int(env, v, d.pressure.preconditioner);
     if ((v = prop(env, o, "tolerance"))) d.pressure.tolerance = get_double(env, v, d.pressure.tolerance);
@@ -1514,6 +1555,9 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("colliderSdf", js_collider_sdf);
     EXPORT("boundary3Create", js_boundary3_create);
     EXPORT("boundary3SetClosed", js_boundary3_set_closed);
+    EXPORT("boundary3AddInflow", js_boundary3_add_inflow);
+    EXPORT("outflow3Dirichlet", js_outflow3_dirichlet);
+    EXPORT("advection3SetCollider", js_advection3_set_collider);
     EXPORT("boundary3Constrain", js_boundary3_constrain);
     EXPORT("flip3Create", js_flip3_create);
     EXPORT("flip3Step", js_flip3_step);

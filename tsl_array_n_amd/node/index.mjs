@@ -356,7 +356,12 @@ export function createGridPressureSolver3( {
 	}
 
 	let dirichletActive = null, dirichletTarget = null;
-	if ( typeof dirichlet === 'function' ) {
+	if ( typeof dirichlet === 'function' && dirichlet.evaluateFields ) {
+
+		// outflow Dirichlet conditions evaluated on the GPU
+		( { active: dirichletActive, target: dirichletTarget } = dirichlet.evaluateFields( resolution, gridSpacing, origin ) );
+
+	} else if ( typeof dirichlet === 'function' ) {
 
 		const { active, target } = evaluateDirichlet( dirichlet, resolution, gridSpacing, origin );
 		dirichletActive = arrayN( 'u8', resolution ).fromArray( active );
@@ -562,7 +567,17 @@ export function createCellCenteredScalarGrid3( nx, ny, nz, hx, hy, hz, ox = 0, o
 export function createSemiLagrangianAdvectionSolver3( { velocityGrid, collider, dt, maxSubsteps = 32, order = 1 } = {} ) {
 
 	if ( ! velocityGrid ) throw new Error( 'createSemiLagrangianAdvectionSolver3: options.velocityGrid is required.' );
-	if ( collider ) throw new Error( 'createSemiLagrangianAdvectionSolver3: a 3-D collider is not supported by this build.' );
+	if ( collider ) {
+
+		const cr = collider.grid.resolution, vr = velocityGrid.resolution;
+		if ( cr[ 0 ] !== vr[ 0 ] || cr[ 1 ] !== vr[ 1 ] || cr[ 2 ] !== vr[ 2 ] ) {
+
+			throw new Error( 'createSemiLagrangianAdvectionSolver3: the collider\'s grid must be the velocity grid\'s cells.' );
+
+		}
+
+	}
+
 	if ( order !== 1 && order !== 2 ) throw new Error( 'createSemiLagrangianAdvectionSolver3: order must be 1 or 2.' );
 	const readDt = () => {
 
@@ -581,6 +596,7 @@ export function createSemiLagrangianAdvectionSolver3( { velocityGrid, collider, 
 
 	const handle = addon.advection3Create( gridArg( velocityGrid ),
 		[ velocityGrid.dataU.handle, velocityGrid.dataV.handle, velocityGrid.dataW.handle ], readDt(), maxSubsteps, order );
+	if ( collider ) addon.advection3SetCollider( handle, collider.grid.data.handle );
 	const live = typeof dt !== 'number';
 	function advectFaceCentered3( input, output ) {
 
@@ -694,20 +710,150 @@ export function createSDFStaticCollider3( resolutionX, resolutionY, resolutionZ,
 
 }
 
+// createSDFInflow3 / createSDFOutflow3 (sdf_inflow_outflow3.js:15-30): SDF
+// objects on their own grid (the velocity grid's cells), an inflow with its
+// velocity [vx, vy, vz] and mode 'set' or 'add'
+export function createSDFInflow3( resolutionX, resolutionY, resolutionZ, gridSpacingX, gridSpacingY, gridSpacingZ,
+	originX = 0, originY = 0, originZ = 0, { velocity = [ 0, 0, 0 ], mode = 'set' } = {} ) {
+
+	if ( mode !== 'set' && mode !== 'add' ) throw new Error( `createSDFInflow3: mode must be 'set' or 'add', got '${ mode }'.` );
+	const sdf = createSDFStaticCollider3( resolutionX, resolutionY, resolutionZ, gridSpacingX, gridSpacingY, gridSpacingZ, originX, originY, originZ );
+	return { ...sdf, velocity, mode };
+
+}
+
+export function createSDFOutflow3( resolutionX, resolutionY, resolutionZ, gridSpacingX, gridSpacingY, gridSpacingZ,
+	originX = 0, originY = 0, originZ = 0 ) {
+
+	return createSDFStaticCollider3( resolutionX, resolutionY, resolutionZ, gridSpacingX, gridSpacingY, gridSpacingZ, originX, originY, originZ );
+
+}
+
+// the SDF's collocatedValueAtPosition3 on the host (trilinear, clamped
+// indices), from a copy read once
+function hostSampler( sdfObject ) {
+
+	let data = null;
+	const { resolution: n, gridSpacing: h, origin: o } = sdfObject.grid;
+	return ( pos ) => {
+
+		if ( ! data ) data = readFloats( sdfObject.grid.data );
+		const idx = [], f = [];
+		for ( let a = 0; a < 3; a ++ ) {
+
+			const g = ( pos[ a ] - ( o[ a ] + 0.5 * h[ a ] ) ) / h[ a ];
+			const i0 = Math.floor( g );
+			f.push( g - i0 );
+			idx.push( [ Math.min( Math.max( i0, 0 ), n[ a ] - 1 ), Math.min( Math.max( i0 + 1, 0 ), n[ a ] - 1 ) ] );
+
+		}
+
+		let acc = 0;
+		for ( let c = 0; c < 8; c ++ ) {
+
+			const bx = c & 1, by = ( c >> 1 ) & 1, bz = ( c >> 2 ) & 1;
+			const wgt = ( bx ? f[ 0 ] : 1 - f[ 0 ] ) * ( by ? f[ 1 ] : 1 - f[ 1 ] ) * ( bz ? f[ 2 ] : 1 - f[ 2 ] );
+			acc += data[ idx[ 0 ][ bx ] + n[ 0 ] * ( idx[ 1 ][ by ] + n[ 1 ] * idx[ 2 ][ bz ] ) ] * wgt;
+
+		}
+
+		return acc;
+
+	};
+
+}
+
+function readFloats( field ) {
+
+	const out = new Float32Array( field.count );
+	addon.fieldDownload( field.handle, out );
+	return out;
+
+}
+
+// createOutflowPressureDirichlet3( outflows, { target } ) and
+// combineDirichlet3( a, b ) (sdf_inflow_outflow3.js:44-94): dirichlet(pos)
+// functions of the reference's shape; with target 0 the pressure solver
+// evaluates an outflow condition (combined with any other as `a`) on the
+// GPU at its cell centres instead of calling the function per cell.
+export function createOutflowPressureDirichlet3( outflows, { target = 0 } = {} ) {
+
+	const list = Array.isArray( outflows ) ? outflows : [ outflows ];
+	const samplers = list.map( hostSampler );
+	const dirichlet = ( pos ) => ( { active: samplers.some( ( sample ) => sample( pos ) < 0 ), target } );
+	if ( target === 0 ) {
+
+		dirichlet.outflows = list;
+		dirichlet.evaluateFields = ( resolution, gridSpacing, origin, user = null ) => {
+
+			const active = arrayN( 'u8', resolution ), tgt = arrayN( 'float', resolution );
+			addon.outflow3Dirichlet( { n: resolution, h: gridSpacing, origin }, list.map( ( o ) => o.grid.data.handle ),
+				user ? user.active.handle : null, user && user.target ? user.target.handle : null, active.handle, tgt.handle );
+			return { active, target: tgt };
+
+		};
+
+	}
+
+	return dirichlet;
+
+}
+
+export function combineDirichlet3( a, b ) {
+
+	if ( ! a ) return b;
+	if ( ! b ) return a;
+	const dirichlet = ( pos ) => {
+
+		const ra = a( pos ), rb = b( pos );
+		return { active: ra.active || rb.active, target: rb.active ? rb.target : ra.target };
+
+	};
+
+	if ( a.outflows ) {
+
+		dirichlet.evaluateFields = ( resolution, gridSpacing, origin ) => {
+
+			let user;
+			if ( b.evaluateFields ) user = b.evaluateFields( resolution, gridSpacing, origin );
+			else {
+
+				const { active, target } = evaluateDirichlet( b, resolution, gridSpacing, origin );
+				user = { active: arrayN( 'u8', resolution ).fromArray( active ), target: arrayN( 'float', resolution ).fromArray( target ) };
+
+			}
+
+			return a.evaluateFields( resolution, gridSpacing, origin, user );
+
+		};
+
+	}
+
+	return dirichlet;
+
+}
+
 // createGridBlockedBoundaryConditionSolver3 (grid_blocked_boundary_condition_solver3.js:47-490)
-// for a static collider; inflow objects are not supported by this build.
+// for a static collider and createSDFInflow3 objects (their SDF on the
+// velocity grid's cells, at most 4)
 export function createGridBlockedBoundaryConditionSolver3( velocity, resolutionX, resolutionY, resolutionZ,
 	gridSpacingX, gridSpacingY, gridSpacingZ, originX = 0, originY = 0, originZ = 0,
 	colliderSDF = null, inflows = null, colliderFaceWeights = null ) {
 
-	if ( inflows ) throw new Error( 'createGridBlockedBoundaryConditionSolver3: inflows are not supported by this build.' );
 	const grid = { n: [ resolutionX, resolutionY, resolutionZ ], h: [ gridSpacingX, gridSpacingY, gridSpacingZ ], origin: [ originX, originY, originZ ] };
 	const weights = colliderFaceWeights ? [ colliderFaceWeights.fields.u.handle, colliderFaceWeights.fields.v.handle, colliderFaceWeights.fields.w.handle ] : null;
 	const handle = addon.boundary3Create( grid, [ velocity.dataU.handle, velocity.dataV.handle, velocity.dataW.handle ],
 		colliderSDF ? colliderSDF.grid.data.handle : null, weights, colliderSDF ? colliderSDF.frictionCoefficient : 0.5 );
+	const inflowList = ! inflows ? [] : ( Array.isArray( inflows ) ? inflows : [ inflows ] );
+	for ( const inflow of inflowList ) {
+
+		addon.boundary3AddInflow( handle, inflow.grid.data.handle, inflow.velocity, inflow.mode === 'add' ? 1 : 0 );
+
+	}
+
 	let flag = 63;
 	const solver = {
-		velocity, collider: colliderSDF, inputs: { colliderFaceWeights },
+		velocity, collider: colliderSDF, inflows, inputs: { colliderFaceWeights, inflowList },
 		get closedDomainBoundaryFlag() {
 
 			return flag;
@@ -738,7 +884,7 @@ export function createGridFlipSolver3( {
 	reducedPressure = true, flipRatio = 0.97, velocityDamping = 0.02, p2gAtomicScale = 65536,
 	weightEpsilon = 1e-4, closedDomainBoundaryFlag, collider, colliderFaceWeights, resample = {}, pressure = {},
 	carryConcentration = false, mixing = 0, fade = 0, concentrationAtomicScale = 4096, ambientDensity,
-	componentDensity, massWeightedTransfer = false, pressurePin = null
+	componentDensity, massWeightedTransfer = false, pressurePin = null, colliderPushThresh = 0, colliderPushShift = 0
 } = {} ) {
 
 	if ( ! velocityGrid ) throw new Error( 'createGridFlipSolver3: options.velocityGrid is required.' );
@@ -782,7 +928,7 @@ export function createGridFlipSolver3( {
 		maxPlausiblePressure: pressure.maxPlausiblePressure === undefined ? null : pressure.maxPlausiblePressure,
 		carryConcentration, mixing, fade, concentrationAtomicScale,
 		ambientDensity: coupled ? ambientDensity : null, componentDensity: coupled ? componentDensity : null,
-		massWeightedTransfer, pressurePinMode, pinCell, phaseAware
+		massWeightedTransfer, pressurePinMode, pinCell, phaseAware, colliderPushThresh, colliderPushShift
 	} );
 	const view = ( which, type, shape ) => wrapField( addon.flip3Field( handle, which ), type, shape );
 	const [ nx, ny, nz ] = resolution;

@@ -69,6 +69,14 @@ def p2g_fixed(grid, positions, velocities, count, component, numer, denom, out, 
                                C.c_void_p(denom.h), C.c_void_p(out.h), C.c_void_p(valid.h)), "p2g_fixed")
 
 
+def advect_particles_collider(grid, positions, count, vel, collider_sdf, dt, max_substeps=32):
+    """advectParticles with the FLIP solver's static collider (the trace stops
+    at the collider; collider_sdf cell-centred on the grid, or None)"""
+    _check(_lib.flux_advect_particles_collider(C.byref(grid.c), C.c_void_p(positions.h), C.c_int64(count), _mac(vel),
+                                               C.c_void_p(collider_sdf.h if collider_sdf is not None else None),
+                                               C.c_float(dt), int(max_substeps)), "advectParticles")
+
+
 def g2p(grid, positions, velocities, count, vel_new, vel_old, flip_ratio=0.97, damping=0.02):
     _check(_lib.flux_g2p(C.byref(grid.c), C.c_void_p(positions.h), C.c_void_p(velocities.h), C.c_int64(count),
                          _mac(vel_new), _mac(vel_old), C.c_float(flip_ratio), C.c_float(damping)), "g2p")
@@ -85,19 +93,24 @@ def advect_faces(grid, vel_in, vel_out, dt, max_substeps=32):
 
 
 class SemiLagrangianAdvectionSolver3:
-    """createSemiLagrangianAdvectionSolver3({velocityGrid, dt, maxSubsteps,
-    order}) (advection_solver3.js:67-362), no collider: the velocity fields
-    are bound here and read at each call; order 1 or 2 (MacCormack)."""
+    """createSemiLagrangianAdvectionSolver3({velocityGrid, collider, dt,
+    maxSubsteps, order}) (advection_solver3.js:67-362): the velocity fields
+    are bound here and read at each call; order 1 or 2 (MacCormack);
+    collider: the static collider's SDF field, cell-centred on the velocity
+    grid, or None."""
 
-    def __init__(self, grid, velocity_grid, dt, max_substeps=32, order=1):
+    def __init__(self, grid, velocity_grid, dt, max_substeps=32, order=1, collider=None):
         if order not in (1, 2):
             raise ValueError("createSemiLagrangianAdvectionSolver3: order must be 1 or 2")
-        self.grid, self.velocity_grid, self.order = grid, list(velocity_grid), order
+        self.grid, self.velocity_grid, self.order, self.collider = grid, list(velocity_grid), order, collider
         h = C.c_void_p()
         _check(_lib.flux_advection3_create(C.byref(grid.c), _mac(self.velocity_grid), C.c_float(dt),
                                            int(max_substeps), int(order), C.byref(h)),
                "createSemiLagrangianAdvectionSolver3")
         self.h = h.value
+        if collider is not None:
+            _check(_lib.flux_advection3_set_collider(C.c_void_p(self.h), C.c_void_p(collider.h)),
+                   "createSemiLagrangianAdvectionSolver3")
 
     def set_dt(self, dt):
         _check(_lib.flux_advection3_set_dt(C.c_void_p(self.h), C.c_float(dt)), "advection dt")
