@@ -69,6 +69,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form-steps", type=int, default=5, help="timed projections of each other form")
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--system", default="flip", choices=["flip", "frame0"],
+                    help="flip: the pressure system the two-phase FLIP dam break solves at step --flip-steps "
+                         "(captured from the FLIP solver); frame0: the synthetic frame-0 system (gravity on "
+                         "particles at rest)")
+    ap.add_argument("--flip-steps", type=int, default=3)
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--headline", default="mixed_f16_cs", choices=sorted(FORMS))
     ap.add_argument("--forms", default=DEFAULT_FORMS,
@@ -265,8 +270,23 @@ def main():
         del src, dst
     dev, cus = F.device_info()
     n = args.n
-    print("bench: building the %d^3 dam-break system on %s" % (n, dev), file=sys.stderr, flush=True)
-    sc = scenes.dam_break_system(n)
+    if args.system == "flip":
+        from tsl_array_n_amd import flip as FL
+        print("bench: running the %d^3 two-phase FLIP dam break for %d steps on %s" % (n, args.flip_steps, dev),
+              file=sys.stderr, flush=True)
+        fsol, parts = FL.two_phase_dam_break(n, levels=args.levels)
+        fsol.capture_projection_input(True)
+        for k in range(args.flip_steps):
+            d = fsol.on_advance_time_step()
+            print("bench: FLIP step %d: %s" % (k, d), file=sys.stderr, flush=True)
+        sc = fsol.captured_system()
+        sc["particles"] = parts["count"]
+        sc["liquid_particles"] = parts["liquid"]
+        del fsol, parts
+        F.sync()
+    else:
+        print("bench: building the %d^3 dam-break system on %s" % (n, dev), file=sys.stderr, flush=True)
+        sc = scenes.dam_break_system(n)
     shape = sc["shape"]
     cells = float(np.prod(shape))
     vin = [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
@@ -383,11 +403,15 @@ def main():
         "vs_baseline": None,
         "dtype": {"f32": "f32", "f16": "f16 storage / f32 arithmetic", "bf16": "bf16 storage / f32 arithmetic"}[
             hform["storage"]] + " V-cycle; f32 CG vectors; f64 accumulator, residual and dots",
-        "data": "synthetic: seeded 3-D two-phase dam break, frame-0 system",
-        "config": {"workload": "3D FLIP two-phase dam break %d^3, liquid:air 1000:1, pinned top-centre, MGPCG %d levels, "
-                               "form %s (%s V-cycle, %s %d/%d), cold start"
-                               % (n, args.levels, headline, hform["storage"], hform["smoother"], hform["nu"],
-                                  hform["nu"]),
+        "data": ("synthetic: the projection system of step %d of the seeded two-phase FLIP dam break "
+                 "(%d particles, %d liquid), captured from the FLIP solver" % (args.flip_steps, sc["particles"],
+                                                                               sc["liquid_particles"])
+                 if args.system == "flip" else "synthetic: seeded 3-D two-phase dam break, frame-0 system"),
+        "config": {"workload": "3D FLIP two-phase dam break %d^3, liquid:air 1000:1, pinned top-centre, %s, "
+                               "MGPCG %d levels, form %s (%s V-cycle, %s %d/%d), cold start"
+                               % (n, "system of FLIP step %d" % args.flip_steps if args.system == "flip"
+                                  else "frame-0 system", args.levels, headline, hform["storage"], hform["smoother"],
+                                  hform["nu"], hform["nu"]),
                    "cells": int(cells), "particles": sc["particles"], "iterations": hf["iterations"],
                    "parallelism": "single GPU", "device": dev},
         "speedup_vs_plain": hf.get("speedup_vs_plain"),

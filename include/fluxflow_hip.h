@@ -667,9 +667,20 @@ int flux_flip3_create(const flux_flip3_desc* d, flux_flip3** out);
 /* onAdvanceTimeStep(): advect, P2G, extrapolate, constrain, mark fluid,
  * resample, constrain, project, constrain, extrapolate, G2P */
 int flux_flip3_step(flux_flip3* s, flux_pressure_diag* diag);
+/* fields 10-14 are the last projection's Dirichlet fields and face weights
+ * (the density beta; NULL without density coupling); 15-17 its input
+ * velocity, kept only while flux_flip3_capture_projection_input is on */
 enum { FLUX_FLIP3_POSITIONS = 0, FLUX_FLIP3_VELOCITIES = 1, FLUX_FLIP3_FLUID_MASK = 2, FLUX_FLIP3_U = 3,
        FLUX_FLIP3_V = 4, FLUX_FLIP3_W = 5, FLUX_FLIP3_PRESSURE = 6, FLUX_FLIP3_CONCENTRATION = 7,
-       FLUX_FLIP3_CELL_CONCENTRATION = 8, FLUX_FLIP3_CELL_DENSITY = 9 };
+       FLUX_FLIP3_CELL_CONCENTRATION = 8, FLUX_FLIP3_CELL_DENSITY = 9, FLUX_FLIP3_DIRICHLET_ACTIVE = 10,
+       FLUX_FLIP3_DIRICHLET_TARGET = 11, FLUX_FLIP3_BETA_U = 12, FLUX_FLIP3_BETA_V = 13, FLUX_FLIP3_BETA_W = 14,
+       FLUX_FLIP3_PROJECTION_INPUT_U = 15, FLUX_FLIP3_PROJECTION_INPUT_V = 16, FLUX_FLIP3_PROJECTION_INPUT_W = 17 };
+/* Keep a copy of each step's projection input (the grid velocity after P2G,
+ * gravity and constrainVelocity), so the pressure system a frame solved can
+ * be captured with its Dirichlet fields and weights -- the frame capture of
+ * the reference's sandbox (sandbox/stalled-system/export_system.mjs) for
+ * this solver.  Costs three face-field copies per step while on. */
+int flux_flip3_capture_projection_input(flux_flip3* s, int on);
 flux_field* flux_flip3_field(flux_flip3* s, int which);
 int flux_flip3_destroy(flux_flip3* s);
 

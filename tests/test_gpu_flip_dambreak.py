@@ -76,3 +76,38 @@ def test_two_phase_dam_break_256_properties(flux):
         pos = s.positions.to_array()
         assert pos.size == 3 * p["count"] and (pos > 0).all() and (pos < 1).all()
     assert (s.fluid_mask.to_array() > 0.5).all()
+
+
+def test_captured_projection_system_reproduces_the_step(flux):
+    """the projection system a FLIP step solved, captured from the solver
+    (input velocity, Dirichlet fields, density face weights), re-solved by a
+    separate pressure solver from the same warm start gives the step's
+    pressure bit for bit; the oracle's FP64 solve of the captured system
+    agrees with it to the tolerance's conditioning"""
+    F = flux
+    n = 32
+    s, _ = flip.two_phase_dam_break(n, levels=3, particles=scenes.flip_dam_break(n, seed=2))
+    s.on_advance_time_step()
+    warm = s.pressure.to_array()
+    s.capture_projection_input(True)
+    d = s.on_advance_time_step()
+    want = s.pressure.to_array()
+    sc = s.captured_system()
+    assert sc["beta"] is not None and int(sc["active"].sum()) == 1  # the pinned top-centre cell
+    shape, h = sc["shape"], sc["spacing"]
+    ps = F.create_grid_pressure_solver3(
+        resolution=shape, grid_spacing=h,
+        dirichlet=(F.array_n("u8", shape, sc["active"]), F.array_n("float", shape, sc["target"])),
+        face_weights=[F.array_n("float", F.face_shape(shape, a), sc["beta"][a]) for a in range(3)],
+        multigrid={"levels": 3}, tolerance=1e-6)
+    ps.pressure.from_array(warm)
+    vel = [F.array_n("float", F.face_shape(shape, a), sc["vel"][a]) for a in range(3)]
+    d2 = ps.project(vel)
+    assert d2["iterations"] == d["iterations"] and d2["converged"]
+    np.testing.assert_array_equal(ps.pressure.to_array(), want)
+    A = O.pressure3_assemble(shape, h, sc["vel"], active=sc["active"], target=sc["target"], beta=sc["beta"],
+                             suffix="f64")
+    mg = O.Multigrid(shape, h, levels=3, mask=A["mask"], coarse_mask=A["vent"], weights=A["weights"], suffix="f64")
+    ref, st = O.pcg_solve(shape, h, A["b"], mask=A["mask"], weights=A["weights"], mg=mg, tol=1e-10, maxiter=400)
+    assert st["converged"]
+    assert rel(want, ref) <= 1e-4

@@ -1,10 +1,11 @@
-"""The fused one-pass red-black sweep against the two-launch form (GPU tests).
+"""The fused one-pass red-black sweeps against the two-launch form (GPU tests).
 
-The fused kernel (FLUX_FUSED_SWEEP=1) does the same arithmetic per cell as
-two single-colour launches, so one V-cycle must agree bit for bit (FP32 and
-FP16 storage); whole projections agree to the PCG tolerance (their r.z
-partials are summed over different block partitions) in the same number of
-iterations.
+Both fused kernels (FLUX_FUSED_SWEEP=1: LDS tiles on every wide level; =2:
+the register form on whole rows of the finest level) do the same arithmetic
+per cell as two single-colour launches, so one V-cycle must agree bit for
+bit (FP32 and FP16 storage); whole projections agree to the PCG tolerance
+(their r.z partials are summed over different block partitions) in the same
+number of iterations.
 """
 import os
 
@@ -19,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 def build_mg(F, sc, levels, storage, fused):
     old = os.environ.get("FLUX_FUSED_SWEEP")
-    os.environ["FLUX_FUSED_SWEEP"] = "1" if fused else "0"
+    os.environ["FLUX_FUSED_SWEEP"] = fused if isinstance(fused, str) else ("1" if fused else "0")
     try:
         n = sc["shape"]
         vel = [F.array_n("float", F.face_shape(n, a), sc["vel"][a]) for a in range(3)]
@@ -70,13 +71,16 @@ def box_scene(shape, seed):
             "target": np.zeros(nx * ny * nz, np.float32)}
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 48)])
-def test_fused_vcycle_is_bit_identical(flux, storage, shape):
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 48), (256, 24, 20), (96, 20, 12)])
+def test_fused_vcycle_is_bit_identical(flux, storage, shape, mode):
+    """including whole 256-cell rows, a partial last row tile (20 rows in
+    tiles of 8) and a short last z-chunk (12 planes in chunks of 8)"""
     F = flux
     sc = scenes.dam_break_system(64) if shape == (64, 64, 64) else box_scene(shape, 5)
     outs = []
-    for fused in (False, True):
+    for fused in ("0", mode):
         ps, vel, mg = build_mg(F, sc, 3, storage, fused)
         z = F.array_n("float", shape)
         mg(ps.b, z)
@@ -85,11 +89,12 @@ def test_fused_vcycle_is_bit_identical(flux, storage, shape):
     np.testing.assert_array_equal(outs[1], outs[0])
 
 
-def test_fused_projection_matches_two_launch(flux):
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_fused_projection_matches_two_launch(flux, mode):
     F = flux
     sc = scenes.dam_break_system(128)
     res = []
-    for fused in ("0", "1"):
+    for fused in ("0", mode):
         os.environ["FLUX_FUSED_SWEEP"] = fused
         try:
             n = sc["shape"]

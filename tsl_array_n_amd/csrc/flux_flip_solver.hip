@@ -469,6 +469,7 @@ struct flux_flip3 {
     flux_field* cell_conc = nullptr;
     flux_field* cell_density = nullptr;
     flux_field* beta[3] = {nullptr, nullptr, nullptr};
+    flux_field* captured[3] = {nullptr, nullptr, nullptr};  // the last projection's input velocity, when kept
     int* conc_accum = nullptr;
     int* conc_count = nullptr;
     int* pin_flag = nullptr;
@@ -817,6 +818,8 @@ int flux_flip3_step(flux_flip3* s, flux_pressure_diag* diag) {
         k_pin<<<1, 64, 0, g_stream>>>(s->pin_index, s->pin_flag, (uint8_t*)s->active->d, (float*)s->target->d);
     }
     FLUX_LAUNCH_CHECK();
+    if (s->captured[0])
+        for (int a = 0; a < 3; ++a) FLUX_TRY(flux_field_copy(s->captured[a], s->vel[a]));
     flux_pressure_diag pdg;
     FLUX_TRY(flux_pressure3_project(s->pressure, s->vel[0], s->vel[1], s->vel[2], s->vel[0], s->vel[1], s->vel[2],
                                     &pdg));
@@ -850,13 +853,39 @@ flux_field* flux_flip3_field(flux_flip3* s, int which) {
         case FLUX_FLIP3_CONCENTRATION: return s->conc;
         case FLUX_FLIP3_CELL_CONCENTRATION: return s->cell_conc;
         case FLUX_FLIP3_CELL_DENSITY: return s->cell_density;
+        case FLUX_FLIP3_DIRICHLET_ACTIVE: return s->active;
+        case FLUX_FLIP3_DIRICHLET_TARGET: return s->target;
+        case FLUX_FLIP3_BETA_U: return s->beta[0];
+        case FLUX_FLIP3_BETA_V: return s->beta[1];
+        case FLUX_FLIP3_BETA_W: return s->beta[2];
+        case FLUX_FLIP3_PROJECTION_INPUT_U: return s->captured[0];
+        case FLUX_FLIP3_PROJECTION_INPUT_V: return s->captured[1];
+        case FLUX_FLIP3_PROJECTION_INPUT_W: return s->captured[2];
         default: return nullptr;
     }
+}
+
+int flux_flip3_capture_projection_input(flux_flip3* s, int on) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(s, "captureProjectionInput: NULL solver");
+    for (int a = 0; a < 3; ++a) {
+        if (on && !s->captured[a]) {
+            int64_t fs[3] = {s->g.n[0] + (a == 0), s->g.n[1] + (a == 1), s->g.n[2] + (a == 2)};
+            FLUX_TRY(flux_field_create(FLUX_F32, 3, fs, &s->captured[a]));
+        } else if (!on && s->captured[a]) {
+            (void)hipStreamSynchronize(g_stream);
+            flux_field_destroy(s->captured[a]);
+            s->captured[a] = nullptr;
+        }
+    }
+    return FLUX_OK;
 }
 
 int flux_flip3_destroy(flux_flip3* s) {
     if (!s) return FLUX_OK;
     if (g_init) (void)hipStreamSynchronize(g_stream);
+    for (int a = 0; a < 3; ++a)
+        if (s->captured[a]) flux_field_destroy(s->captured[a]);
     if (s->pressure) flux_pressure3_destroy(s->pressure);
     if (s->boundary) flux_boundary3_destroy(s->boundary);
     for (int a = 0; a < 3; ++a) {

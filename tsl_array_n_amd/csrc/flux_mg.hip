@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "flux_internal.h"
@@ -1422,7 +1423,9 @@ struct flux_multigrid {
     const flux_field* coarse_mask = nullptr;
     const flux_field* w[3] = {nullptr, nullptr, nullptr};
     bool has_w = false, has_masks = false;
-    bool fused = false;  // fused sweeps on every level wide enough (FLUX_FUSED_SWEEP=1)
+    // fused red-black sweeps (FLUX_FUSED_SWEEP): 1 the LDS-tile form on every
+    // level wide enough, 2 the register form on the finest level
+    int fused = 0;
     bool tail = true;    // the last two levels in one workgroup (FLUX_VCYCLE_TAIL=0 turns it off)
     bool pairs = true;   // paired prolongation-correction on even rows (FLUX_CORRECT_PAIRS=0 turns it off)
     // mixed form: the finest level stores x and b in the storage type too
@@ -1507,8 +1510,19 @@ int refresh_t(flux_multigrid* mg) {
 }
 
 bool use_fused(const flux_multigrid* mg, const MGLevel& v) {
-    return mg->fused && mg->ndim == 3 && v.alt && v.n[0] >= FUSED_MIN_N && (v.n[0] & 1) == 0 && v.n[1] >= Sweep3::TY &&
-           v.n[2] >= 2;
+    if (mg->ndim != 3 || !v.alt || (v.n[0] & 1) != 0) return false;
+    if (mg->fused == 2) return &v == &mg->lv[0] && v.n[0] <= 2 * SweepR::PX;
+    return mg->fused == 1 && v.n[0] >= FUSED_MIN_N && v.n[1] >= Sweep3::TY && v.n[2] >= 2;
+}
+
+// planes per z-chunk of the register form (FLUX_SWEEP_KZ, default 8)
+static int sweepr_chunk() {
+    static const int v = [] {
+        const char* e = getenv("FLUX_SWEEP_KZ");
+        const long q = e ? strtol(e, nullptr, 10) : 8;
+        return (int)(q >= 2 && q <= 256 ? q : 8);
+    }();
+    return v;
 }
 
 // `iters` fused sweeps (colour A then B each), ping-ponging between v.x and
@@ -1526,6 +1540,39 @@ void fused_sweeps(flux_multigrid* mg, int l, int iters, bool reversed, bool from
     const unsigned nb = (unsigned)(gxy * gz);
     const float om = (float)mg->o.omega;
     const int colorA = reversed ? 1 : 0;
+    if constexpr (std::is_same<XT, float>::value) {
+        if (mg->fused == 2) {
+            // the register form, finest level: whole rows of cell pairs
+            const int kz = sweepr_chunk();
+            const int gyr = (v.n[1] + SweepR::TY - 1) / SweepR::TY, gzr = (v.n[2] + kz - 1) / kz;
+            const unsigned nbr = (unsigned)(gyr * gzr);
+            const float* srcr = (const float*)v.x;
+            if (!from_zero && (iters & 1)) {
+                k_copy_t<float><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(
+                    (const float*)v.x, (float*)v.alt, (int)cells_of(v.n), stop);
+                srcr = (const float*)v.alt;
+            }
+            for (int it = 0; it < iters; ++it) {
+                float* dst = (((iters - 1 - it) & 1) == 0) ? (float*)v.x : (float*)v.alt;
+                const bool zero = from_zero && it == 0;
+                const bool with_dot = dot && it == iters - 1 && nbr <= dot->capacity;
+                kprof_begin("sweep");
+                if (zero)
+                    k_sweep3r<true, false, WT><<<nbr, SweepR::NT, 0, g_stream>>>(L, nullptr, (const float*)v.b, dst, om,
+                                                                               colorA, kz, stop, nullptr);
+                else if (with_dot)
+                    k_sweep3r<false, true, WT><<<nbr, SweepR::NT, 0, g_stream>>>(L, srcr, (const float*)v.b, dst, om,
+                                                                               colorA, kz, stop, dot->partials);
+                else
+                    k_sweep3r<false, false, WT><<<nbr, SweepR::NT, 0, g_stream>>>(L, srcr, (const float*)v.b, dst, om,
+                                                                                colorA, kz, stop, nullptr);
+                kprof_end("sweep");
+                if (with_dot) dot->blocks = nbr;
+                srcr = dst;
+            }
+            return;
+        }
+    }
     const XT* src = (const XT*)v.x;
     if (!from_zero && (iters & 1)) {
         k_copy_t<XT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>((const XT*)v.x, (XT*)v.alt,
@@ -2005,7 +2052,7 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
     // in FP32, ~1 % per projection, and slower with FP16 weights; opt-in
     // (FLUX_FUSED_SWEEP=1)
     const char* fenv = getenv("FLUX_FUSED_SWEEP");
-    mg->fused = (fenv && fenv[0] == '1');
+    mg->fused = (fenv && (fenv[0] == '1' || fenv[0] == '2')) ? fenv[0] - '0' : 0;
     const char* tenv = getenv("FLUX_VCYCLE_TAIL");
     mg->tail = !(tenv && tenv[0] == '0');
     const char* penv = getenv("FLUX_CORRECT_PAIRS");
@@ -2037,7 +2084,7 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
         size_t vs = (l == 0) ? 4 : es;  // finest vectors stay FP32
         v.buf = dmalloc(cells * vs);
         ok = ok && v.buf;
-        if (ndim == 3 && mg->fused) {
+        if (ndim == 3 && (mg->fused == 1 || (mg->fused == 2 && l == 0))) {
             v.alt = dmalloc(cells * vs);
             ok = ok && v.alt;
         }

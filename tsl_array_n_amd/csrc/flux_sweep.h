@@ -262,4 +262,215 @@ __global__ void __launch_bounds__(Sweep3::NT) k_sweep3(LevelDevT<WT> L, const XT
     if (DOT) store_block_partial(dot, partials);
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused red-black sweep on cell pairs, rows whole (n0 <= 256, even).
+//
+// The same semantics and per-cell arithmetic as two k_relax3 launches on the
+// finest level (colour A, then colour B from the colour-A field; with ZERO
+// the field starts at 0): bit-identical to them.  Out of place, xin -> xout.
+// A block owns a tile of whole rows (2 groups of RG rows, 128 cell pairs per
+// row, one thread per pair and group) and a chunk of z-planes, and marches
+// in z with a one-plane lag:
+//   step s: colour A on plane s over its rows plus one ring row (group 0 the
+//           row above the tile, group 1 the row below: recomputed, so no
+//           neighbour tile is waited for), each value into an LDS plane of
+//           colour-A results;
+//           colour B on plane s-1 over the tile's rows, from the old field
+//           and the colour-A planes s-2, s-1 and s in LDS, written out as
+//           aligned pairs together with their colour-A partners.
+// The old field comes from registers: each thread keeps the cell pairs of
+// 7 rows (its colour-A rows and their y-neighbours) for planes s-1, s, s+1
+// and has plane s+2's in flight.  Colour-A planes rotate through 4 LDS
+// slots, so one barrier per step suffices (a thread still in step s-1's
+// colour-B phase reads slots s-3..s-1; step s writes slot s).
+struct SweepR {
+    static constexpr int PX = 128;        // cell pairs per row
+    static constexpr int NT = 2 * PX;     // threads per block
+    static constexpr int RG = 4;          // colour-B rows per group
+    static constexpr int TY = 2 * RG;     // rows per tile
+    static constexpr int RA = RG + 1;     // colour-A rows per group (one ring row)
+    static constexpr int WR = RA + 2;     // old-field rows per thread
+    static constexpr int LR = TY + 2;     // colour-A rows of a plane in LDS
+    static constexpr int SL = 4;          // LDS plane slots
+};
+
+// the old field's cell pairs of one plane for the WR rows of a thread's
+// window, as separate low / high registers (no dynamic indexing)
+struct PairRows {
+    float lo[SweepR::WR];
+    float hi[SweepR::WR];
+};
+
+template <bool ZERO, typename WT>
+__device__ __forceinline__ void sweepr_load(const LevelDevT<WT>& L, const float* __restrict__ xin, int p, int jw,
+                                            int k, PairRows& X) {
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+#pragma unroll
+    for (int q = 0; q < SweepR::WR; ++q) {
+        const int j = jw + q;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (!ZERO && 2 * p < n0 && j >= 0 && j < n1 && k >= 0 && k < n2)
+            v = *reinterpret_cast<const float2*>(xin + 2 * p + n0 * (j + n1 * k));
+        X.lo[q] = v.x;
+        X.hi[q] = v.y;
+    }
+}
+
+// colour B on plane kb for the group's RG rows; OFF = row index of its first
+// colour-B row in the register window (2 for group 0, 1 for group 1)
+template <int OFF, bool DOT, typename WT>
+__device__ __forceinline__ void sweepr_colour_b(const LevelDevT<WT>& L, const float* __restrict__ b,
+                                                float* __restrict__ xout, float omega, int colorA, int p, int j0,
+                                                int jb, int kb, int sm, int s0, int sp,
+                                                const float (&red)[SweepR::SL][SweepR::LR][SweepR::PX],
+                                                const PairRows& Xo, double& dot) {
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const uint8_t* __restrict__ nbits = L.nb;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    // every operand of the group's rows first, so the loads of all rows are
+    // in flight together (the stores below could otherwise hold them back)
+    float w[SweepR::RG][6];
+    float2 bp[SweepR::RG];
+    unsigned bits[SweepR::RG];
+#pragma unroll
+    for (int t = 0; t < SweepR::RG; ++t) {
+        const int j = jb + t;
+        bits[t] = 0u;
+        bp[t] = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) w[t][a] = 0.0f;
+        if (j >= n1 || 2 * p >= n0) continue;
+        const int sb = (j + kb + colorA + 1) & 1;
+        const int i = 2 * p + sb;
+        const int cp = 2 * p + n0 * (j + n1 * kb);
+        bits[t] = nbits ? nbits[cp + sb] : 0u;
+        face_w3(L, i, j, kb, cp + sb, w[t]);
+        bp[t] = *reinterpret_cast<const float2*>(b + cp);
+    }
+#pragma unroll
+    for (int t = 0; t < SweepR::RG; ++t) {
+        const int j = jb + t;
+        if (j >= n1 || 2 * p >= n0) continue;
+        const int lr = j - (j0 - 1);
+        const int sb = (j + kb + colorA + 1) & 1;  // the colour-B cell of the pair
+        const int i = 2 * p + sb;
+        const int cp = 2 * p + n0 * (j + n1 * kb);
+        const float xc = sb ? Xo.hi[OFF + t] : Xo.lo[OFF + t];
+        const float part = red[s0][lr][p];  // the pair's colour-A cell (updated)
+        float xo;                           // the x-neighbour outside the pair (colour A, updated)
+        if (sb) xo = (i < n0 - 1) ? red[s0][lr][p + 1] : xc;
+        else xo = (i > 0) ? red[s0][lr][p - 1] : xc;
+        float xn[6];
+        xn[0] = sb ? part : xo;
+        xn[1] = sb ? xo : part;
+        xn[2] = j > 0 ? red[s0][lr - 1][p] : xc;
+        xn[3] = j < n1 - 1 ? red[s0][lr + 1][p] : xc;
+        xn[4] = kb > 0 ? red[sm][lr][p] : xc;
+        xn[5] = kb + 1 < n2 ? red[sp][lr][p] : xc;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) xn[a] = ((bits[t] >> a) & 1u) ? 0.0f : xn[a];
+        const float bc = sb ? bp[t].y : bp[t].x;
+        float sum, dsum;
+        row_terms<float>(xc, xn, w[t], ih2, sum, dsum);
+        const bool dir = (bits[t] & 64u) != 0;
+        const float d = dir ? -1.0f : dsum;
+        const float ax = dir ? -xc : sum;
+        const float nv = xc + ((bc - ax) / d) * omega;
+        float2 o;
+        o.x = sb ? part : nv;
+        o.y = sb ? nv : part;
+        *reinterpret_cast<float2*>(xout + cp) = o;
+        if (DOT) {
+            dot += (double)bp[t].x * (double)o.x;
+            dot += (double)bp[t].y * (double)o.y;
+        }
+    }
+}
+
+template <bool ZERO, bool DOT, typename WT>
+__global__ void __launch_bounds__(SweepR::NT) k_sweep3r(LevelDevT<WT> L, const float* __restrict__ xin,
+                                                        const float* __restrict__ b, float* __restrict__ xout,
+                                                        float omega, int colorA, int kchunk,
+                                                        const int* __restrict__ stop, double* __restrict__ partials) {
+    using S = SweepR;
+    if (stop && *stop) return;
+    __shared__ float red[S::SL][S::LR][S::PX];
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const int gy = (n1 + S::TY - 1) / S::TY;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int by = (int)(lb % (unsigned)gy), bz = (int)(lb / (unsigned)gy);
+    const int j0 = by * S::TY;
+    const int k0 = bz * kchunk, k1 = min(k0 + kchunk, n2);
+    const int p = threadIdx.x % S::PX;
+    const int grp = threadIdx.x / S::PX;  // uniform per wavefront
+    const int jb = j0 + grp * S::RG;      // first colour-B row of the group
+    const int ja = grp ? jb : jb - 1;     // first colour-A row
+    const int jw = ja - 1;                // first row of the register window
+    const int la = ja - (j0 - 1);         // LDS row of the first colour-A row
+    const uint8_t* __restrict__ nbits = L.nb;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    // old field: planes s-1, s, s+1 and s+2 (in flight)
+    PairRows Xm, X0, Xp, Xq;
+    sweepr_load<ZERO>(L, xin, p, jw, k0 - 2, Xm);
+    sweepr_load<ZERO>(L, xin, p, jw, k0 - 1, X0);
+    sweepr_load<ZERO>(L, xin, p, jw, k0, Xp);
+    double dot = 0.0;
+    for (int s = k0 - 1; s <= k1; ++s) {
+        sweepr_load<ZERO>(L, xin, p, jw, s + 2, Xq);
+        const int slot = (s - (k0 - 1)) & (S::SL - 1);
+        // colour A on plane s
+        if (s >= 0 && s < n2) {
+#pragma unroll
+            for (int r = 0; r < S::RA; ++r) {
+                const int j = ja + r;
+                if (j < 0 || j >= n1 || 2 * p >= n0) continue;
+                const int sa = (j + s + colorA) & 1;  // the colour-A cell of the pair
+                const int i = 2 * p + sa;
+                const int cp = 2 * p + n0 * (j + n1 * s);
+                const int c = cp + sa;
+                const float xc = sa ? X0.hi[r + 1] : X0.lo[r + 1];
+                const float part = sa ? X0.lo[r + 1] : X0.hi[r + 1];
+                float xo;  // the x-neighbour outside the pair (colour B, old)
+                if (ZERO) xo = 0.0f;
+                else if (sa) xo = (i < n0 - 1) ? xin[c + 1] : xc;
+                else xo = (i > 0) ? xin[c - 1] : xc;
+                float xn[6];
+                xn[0] = sa ? part : xo;
+                xn[1] = sa ? xo : part;
+                xn[2] = j > 0 ? (sa ? X0.hi[r] : X0.lo[r]) : xc;
+                xn[3] = j < n1 - 1 ? (sa ? X0.hi[r + 2] : X0.lo[r + 2]) : xc;
+                xn[4] = s > 0 ? (sa ? Xm.hi[r + 1] : Xm.lo[r + 1]) : xc;
+                xn[5] = s + 1 < n2 ? (sa ? Xp.hi[r + 1] : Xp.lo[r + 1]) : xc;
+                const unsigned bits = nbits ? nbits[c] : 0u;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
+                float w[6];
+                face_w3(L, i, j, s, c, w);
+                const float bc = b[c];
+                float sum, dsum;
+                row_terms<float>(xc, xn, w, ih2, sum, dsum);
+                const bool dir = (bits & 64u) != 0;
+                const float d = dir ? -1.0f : dsum;
+                const float ax = dir ? -xc : sum;
+                red[slot][la + r][p] = xc + ((bc - ax) / d) * omega;
+            }
+        }
+        __syncthreads();
+        // colour B on plane s-1
+        const int kb = s - 1;
+        if (kb >= k0 && kb < k1) {
+            const int sm = (slot + S::SL - 2) & (S::SL - 1), s0 = (slot + S::SL - 1) & (S::SL - 1);
+            if (grp == 0)
+                sweepr_colour_b<2, DOT>(L, b, xout, omega, colorA, p, j0, jb, kb, sm, s0, slot, red, Xm, dot);
+            else
+                sweepr_colour_b<1, DOT>(L, b, xout, omega, colorA, p, j0, jb, kb, sm, s0, slot, red, Xm, dot);
+        }
+        Xm = X0;
+        X0 = Xp;
+        Xp = Xq;
+    }
+    if (DOT) store_block_partial(dot, partials);
+}
+
 }  // namespace flux

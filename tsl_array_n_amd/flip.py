@@ -13,7 +13,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import (F32, PREC, VEC3, Field, MGOptions, Pressure3Desc, PressureDiag, STOP_NAMES, _check, _lib,
+from . import (F32, PREC, U8, VEC3, Field, MGOptions, Pressure3Desc, PressureDiag, STOP_NAMES, _check, _lib,
                face_shape, mg_options)
 
 
@@ -33,7 +33,9 @@ class Flip3Desc(C.Structure):
 
 
 _FIELDS = {"positions": 0, "velocities": 1, "fluid_mask": 2, "u": 3, "v": 4, "w": 5, "pressure": 6,
-           "concentration": 7, "cell_concentration": 8, "cell_density": 9}
+           "concentration": 7, "cell_concentration": 8, "cell_density": 9, "dirichlet_active": 10,
+           "dirichlet_target": 11, "beta_u": 12, "beta_v": 13, "beta_w": 14, "input_u": 15, "input_v": 16,
+           "input_w": 17}
 
 
 class GridFlipSolver3:
@@ -114,6 +116,7 @@ class GridFlipSolver3:
         self.h = h.value
         n = tuple(int(x) for x in resolution)
         self.resolution = n
+        self.spacing = tuple(float(x) for x in grid_spacing)
         self.positions = self._view("positions", VEC3, (int(max_particles),))
         self.velocities = self._view("velocities", VEC3, (int(max_particles),))
         self.fluid_mask = self._view("fluid_mask", F32, n)
@@ -128,6 +131,26 @@ class GridFlipSolver3:
         _lib.flux_flip3_field.restype = C.c_void_p
         ptr = _lib.flux_flip3_field(C.c_void_p(self.h), _FIELDS[name])
         return Field(dtype, shape, handle=ptr, owned=False)
+
+    def capture_projection_input(self, on=True):
+        """keep each step's projection input velocity (see captured_system)"""
+        _check(_lib.flux_flip3_capture_projection_input(C.c_void_p(self.h), int(bool(on))), "captureProjectionInput")
+
+    def captured_system(self):
+        """the last step's pressure system on the host, in the scene form the
+        projection tools take: {"shape", "spacing", "vel" (the projection's
+        input), "beta" (None without density coupling), "active", "target"}"""
+        n = self.resolution
+        _lib.flux_flip3_field.restype = C.c_void_p
+        if not _lib.flux_flip3_field(C.c_void_p(self.h), _FIELDS["input_u"]):
+            raise RuntimeError("captured_system: call capture_projection_input() before the step")
+        vel = [self._view("input_" + c, F32, face_shape(n, a)).to_array() for a, c in enumerate("uvw")]
+        beta = None
+        if _lib.flux_flip3_field(C.c_void_p(self.h), _FIELDS["beta_u"]):
+            beta = [self._view("beta_" + c, F32, face_shape(n, a)).to_array() for a, c in enumerate("uvw")]
+        return {"shape": n, "spacing": self.spacing, "vel": vel, "beta": beta,
+                "active": self._view("dirichlet_active", U8, n).to_array(),
+                "target": self._view("dirichlet_target", F32, n).to_array()}
 
     def on_advance_time_step(self):
         dg = PressureDiag()
