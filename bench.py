@@ -64,6 +64,12 @@ FORMS = {
     # second coarse vector (taken from a plain solve before timing)
     "mixed_f16_cs_exact": {"storage": "f16", "smoother": "rbgs", "nu": 4, "omega": 1.0, "cs": "free+exact"},
     "plain_jacobi": {"storage": "f32", "smoother": "jacobi", "nu": 4, "omega": 6.0 / 7.0},
+    # brick-wise red-black: every smoothing is two passes over memory (bricks
+    # of 64x8x8 cells, then bricks shifted by half a brick), each running all
+    # of its sweeps on the brick in registers and LDS
+    "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0, "cs": "free"},
+    "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0},
+    "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 4, "omega": 1.0},
 }
 DEFAULT_FORMS = "plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_nu2,mixed_f16_cs_exact"
 # the single custom form of --storage/--smoother/--nu/--coarse-space sweeps
@@ -91,7 +97,7 @@ def parse():
                                                     "form's smoother)")
     # a single custom form (overrides --headline) for sweeps
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--smoother", choices=["rbgs", "jacobi"])
+    ap.add_argument("--smoother", choices=["rbgs", "jacobi", "brick"])
     ap.add_argument("--nu", type=int)
     ap.add_argument("--jacobi-omega", type=float)
     ap.add_argument("--coarse-space", choices=["none", "free"])
@@ -131,6 +137,12 @@ def algo_bytes_per_cell(kernel, storage):
         # the fine level's storage (FP32 for f32; FP16/BF16 in the mixed
         # form), mask 1, weights 3 x storage
         return 3 * x + 1 + 3 * s
+    if kernel == "brick":
+        # one brick pass that reads the field: x read, b read, x written
+        # (FP32 on the finest level), mask 1, weights 3 x storage; the halo
+        # of the brick (re-read from the neighbouring bricks' cells) is not
+        # counted (25 B in FP32; 19 B with FP16/BF16 weights)
+        return 3 * x + 1 + 3 * s
     if kernel == "update":
         # x64 read + write (16), p 4, b 4, r written 4, mask 1, weights 12
         return 41.0
@@ -148,6 +160,9 @@ KERNEL_SYMBOL = {("relax", "f32"): "k_relax3<1, 0, 0, float, float>",
                  ("relax", "bf16"): "k_relax3<1, 0, 0, float, __hip_bfloat16>",
                  ("jacobi", "f32"): "k_jacobi3<1, 0, 0, float, float>",
                  ("jacobi", "f16"): "k_jacobi3<1, 0, 0, float, __half>",
+                 ("brick", "f32"): "k_brick3<0, 0, float, float>",
+                 ("brick", "f16"): "k_brick3<0, 0, float, __half>",
+                 ("brick", "bf16"): "k_brick3<0, 0, float, __hip_bfloat16>",
                  ("update", "f32"): "k_pcg_update3", ("apply", "f32"): "k_pcg_apply3"}
 if fine_storage():
     KERNEL_SYMBOL.update({("relax", "f16"): "k_relax3m<0, 0, __half>",
@@ -262,7 +277,7 @@ def main():
         run = [f for f in args.forms.split(",") if f in FORMS and f != headline]
         run = [headline] + run
     hform = forms[headline]
-    kernel = args.kernel or ("jacobi" if hform["smoother"] == "jacobi" else "relax")
+    kernel = args.kernel or {"jacobi": "jacobi", "brick": "brick"}.get(hform["smoother"], "relax")
 
     F.init(int(os.environ.get("LOCAL_RANK", "0")))
     if args.copy_calibration:

@@ -118,8 +118,15 @@ typedef struct {
  * multigrid.js:626-640), or damped Jacobi, single sweeps ping-ponging through
  * a second buffer (buildJacobiRelaxKernel and relax(),
  * sandbox/jacobi-smoother/multigrid_jacobi.js:632-642, 1271-1285); the
- * Jacobi form requires even sweep counts ("sweep counts must be even"). */
-enum { FLUX_SMOOTH_RBGS = 0, FLUX_SMOOTH_JACOBI = 1 };
+ * Jacobi form requires even sweep counts ("sweep counts must be even").
+ * FLUX_SMOOTH_BRICK (no counterpart in the reference): the same red-black
+ * relaxation run brick by brick (64 x 8 x 8 cells, values outside a brick
+ * held at their values on entry), all of a smoothing's sweeps in one pass
+ * over memory; two passes per smoothing, the second on bricks shifted by
+ * half a brick, post-smoothing the adjoint (shifted first, colours
+ * reversed).  3-D levels above 4096 cells except the coarsest; other levels
+ * smooth with red-black Gauss-Seidel. */
+enum { FLUX_SMOOTH_RBGS = 0, FLUX_SMOOTH_JACOBI = 1, FLUX_SMOOTH_BRICK = 2 };
 /* fills the reference defaults */
 void flux_mg_options_default(flux_mg_options* o);
 typedef struct flux_multigrid flux_multigrid;

@@ -54,11 +54,17 @@ typedef struct {
     int n_coarsest;      /* numberOfCoarsestIterations (20) */
     int n_final;         /* numberOfFinalIterations (2) */
     double omega;        /* sorFactor (1.0); the Jacobi smoother's damping (2/3) */
-    int smoother;        /* ORC_SMOOTH_RBGS (multigrid.js) or ORC_SMOOTH_JACOBI
+    int smoother;        /* ORC_SMOOTH_RBGS (multigrid.js), ORC_SMOOTH_JACOBI or ORC_SMOOTH_BRICK
                             (sandbox/jacobi-smoother/multigrid_jacobi.js) */
 } orc_mg_options;
 
-enum { ORC_SMOOTH_RBGS = 0, ORC_SMOOTH_JACOBI = 1 };
+/* ORC_SMOOTH_BRICK: red-black sweeps brick by brick, the values outside a
+ * brick held at their values on entry (orc_brick_relax; the library's
+ * one-pass smoother, no counterpart in the reference) */
+enum { ORC_SMOOTH_RBGS = 0, ORC_SMOOTH_JACOBI = 1, ORC_SMOOTH_BRICK = 2 };
+#define ORC_BRICK_X 64
+#define ORC_BRICK_Y 8
+#define ORC_BRICK_Z 8
 
 typedef struct {
     int iterations;
@@ -152,6 +158,9 @@ void orc_two_phase_beta3(const float* liquid_fraction, const int n[3],
                        const real* const* w, double omega, int color);                            \
     void orc_jacobi_##S(const orc_grid* g, const real* xr, real* xw, const real* b,                \
                         const uint8_t* mask, const real* const* w, double omega);                 \
+    void orc_brick_relax_##S(const orc_grid* g, const real* xin, real* xout, const real* b,        \
+                             const uint8_t* mask, const real* const* w, double omega, int sweeps, \
+                             int reversed);                                                       \
     void orc_residual_##S(const orc_grid* g, const real* x, const real* b, real* r,               \
                           const uint8_t* mask, const real* const* w);                             \
     void orc_restrict_##S(const orc_grid* fine, const real* r_fine, const uint8_t* fine_mask,     \
@@ -193,6 +202,11 @@ ORC_DECL(float, f32)
  * on, a host-side switch of the stop-code tests. */
 void orc_set_post_forward(int on);
 int orc_post_forward_flag(void);
+/* brick smoother shape (cells per brick along each axis) and passes per
+ * smoothing (2: the second pass on bricks shifted by half a brick) */
+void orc_set_brick(int bx, int by, int bz, int passes);
+const int* orc_brick_dims(void);
+int orc_brick_passes(void);
 void orc_set_flexible(int on);
 int orc_pcg_solve_mixed(const orc_grid* g, const uint8_t* mask, const float* const* w, int prec,
                         void* mg, const float* b, float* x, double* x64_out, double tol, int maxiter,

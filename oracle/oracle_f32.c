@@ -63,6 +63,14 @@ static void apply_mixed(const orc_grid* g, const double* x, float* r, const floa
 static int g_flexible = 0;
 void orc_set_flexible(int on) { g_flexible = on ? 1 : 0; }
 
+static int g_brick[3] = {ORC_BRICK_X, ORC_BRICK_Y, ORC_BRICK_Z};
+static int g_brick_passes = 2;
+void orc_set_brick(int bx, int by, int bz, int passes) {
+    g_brick[0] = bx; g_brick[1] = by; g_brick[2] = bz; g_brick_passes = passes;
+}
+const int* orc_brick_dims(void) { return g_brick; }
+int orc_brick_passes(void) { return g_brick_passes; }
+
 static int g_post_forward = 0;
 void orc_set_post_forward(int on) { g_post_forward = on ? 1 : 0; }
 int orc_post_forward_flag(void) { return g_post_forward; }

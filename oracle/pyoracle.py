@@ -49,7 +49,7 @@ class MGOptions(C.Structure):
     ]
 
 
-SMOOTHERS = {"rbgs": 0, "jacobi": 1}
+SMOOTHERS = {"rbgs": 0, "jacobi": 1, "brick": 2}
 
 
 class PCGState(C.Structure):
@@ -1235,3 +1235,8 @@ def pcg_coarse_space(shape, spacing, b, mask, weights, mg, vectors, free_indicat
         rz_old = rz
     st["residual"] = float(np.sqrt(rr)) if st["iterations"] else float(np.sqrt(rr0))
     return x, st
+
+
+def set_brick(bx=64, by=8, bz=8, passes=2):
+    """Brick shape and passes per smoothing of the brick smoother."""
+    lib().orc_set_brick(int(bx), int(by), int(bz), int(passes))

@@ -424,6 +424,113 @@ void FN(orc_jacobi)(const orc_grid* g, const REAL* xr, REAL* xw, const REAL* b, 
             }
 }
 
+/* Brick-wise red-black Gauss-Seidel (no counterpart in the reference: the
+ * library's one-pass smoother).  The grid is cut into bricks of
+ * ORC_BRICK_X x ORC_BRICK_Y x ORC_BRICK_Z cells (origin at cell 0, partial
+ * bricks at the upper ends); every brick runs `sweeps` colour pairs of the
+ * reference's relax (multigrid.js:626-640, same row, same colours) on its own
+ * cells, reading a neighbour outside the brick from `xin` (the field on
+ * entry, unchanged throughout) and one inside from `xout`.  `reversed` runs
+ * colour 1 before colour 0, which makes the post-smoother the adjoint of the
+ * pre-smoother, so the V-cycle stays symmetric.  xin == NULL means the field
+ * is zero on entry. */
+void FN(orc_brick_relax_shape)(const orc_grid* g, const REAL* xin, REAL* xout, const REAL* b, const uint8_t* mask,
+                         const REAL* const* w, double omega, int sweeps, int reversed, const int* B, int shifted);
+static inline REAL FN(brick_nbr)(const REAL* xin, const REAL* xout, int64_t q, int in_brick) {
+    if (in_brick) return xout[q];
+    return xin ? xin[q] : (REAL)0;
+}
+
+static inline REAL FN(row_apply_brick)(const orc_grid* g, const REAL* xin, const REAL* xout, const int* lo,
+                                       const int* hi, const uint8_t* m, const REAL* const* w, int i, int j,
+                                       int k) {
+    const int* n = g->n;
+    int64_t c = orc_cidx(n, i, j, k);
+    REAL center = xout[c];
+    if (m && m[c]) return -center;
+    int I[3] = {i, j, k};
+    REAL sum = 0;
+    for (int a = 0; a < g->ndim; ++a) {
+        int idx = I[a];
+        REAL dLo = 0, dUp = 0;
+        int lo3[3] = {i, j, k}, up3[3] = {i, j, k};
+        lo3[a] -= 1;
+        up3[a] += 1;
+        if (idx > 0) {
+            int64_t l = orc_cidx(n, lo3[0], lo3[1], lo3[2]);
+            REAL xl = FN(brick_nbr)(xin, xout, l, lo3[a] >= lo[a]);
+            dLo = (m && m[l]) ? center : (REAL)(center - xl);
+        }
+        if (idx < n[a] - 1) {
+            int64_t u = orc_cidx(n, up3[0], up3[1], up3[2]);
+            REAL xu = FN(brick_nbr)(xin, xout, u, up3[a] < hi[a]);
+            dUp = (m && m[u]) ? -center : (REAL)(xu - center);
+        }
+        if (w) {
+            dLo = dLo * w[a][orc_fidx(n, a, i, j, k)];
+            dUp = dUp * w[a][orc_fidx(n, a, up3[0], up3[1], up3[2])];
+        }
+        REAL h2 = (REAL)(g->h[a] * g->h[a]);
+        REAL term = (REAL)(dUp - dLo) / h2;
+        sum = (a == 0) ? term : (REAL)(sum + term);
+    }
+    return sum;
+}
+
+void FN(orc_brick_relax)(const orc_grid* g, const REAL* xin, REAL* xout, const REAL* b, const uint8_t* mask,
+                         const REAL* const* w, double omega, int sweeps, int reversed) {
+    FN(orc_brick_relax_shape)(g, xin, xout, b, mask, w, omega, sweeps, reversed, orc_brick_dims(), 0);
+}
+
+void FN(orc_brick_relax_shape)(const orc_grid* g, const REAL* xin, REAL* xout, const REAL* b, const uint8_t* mask,
+                         const REAL* const* w, double omega, int sweeps, int reversed, const int* B, int shifted) {
+    const int* n = g->n;
+    int nb[3], off[3];
+    for (int a = 0; a < 3; ++a) {
+        off[a] = shifted ? B[a] / 2 : 0;
+        nb[a] = (n[a] + off[a] + B[a] - 1) / B[a];
+    }
+    REAL om = (REAL)omega;
+    const int64_t bricks = (int64_t)nb[0] * nb[1] * nb[2];
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < bricks; ++q) {
+        int bi = (int)(q % nb[0]), bj = (int)((q / nb[0]) % nb[1]), bk = (int)(q / ((int64_t)nb[0] * nb[1]));
+        int lo[3] = {bi * B[0] - off[0], bj * B[1] - off[1], bk * B[2] - off[2]};
+        int hi[3];
+        for (int a = 0; a < 3; ++a) {
+            hi[a] = lo[a] + B[a] < n[a] ? lo[a] + B[a] : n[a];
+            if (lo[a] < 0) lo[a] = 0;
+        }
+        for (int k = lo[2]; k < hi[2]; ++k)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int i = lo[0]; i < hi[0]; ++i) {
+                    int64_t c = orc_cidx(n, i, j, k);
+                    xout[c] = xin ? xin[c] : (REAL)0;
+                }
+        for (int s = 0; s < sweeps; ++s)
+            for (int h = 0; h < 2; ++h) {
+                int color = reversed ? 1 - h : h;
+                for (int k = lo[2]; k < hi[2]; ++k)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int i = lo[0] + ((lo[0] + j + k + color) & 1); i < hi[0]; i += 2) {
+                            int64_t c = orc_cidx(n, i, j, k);
+                            REAL ax = FN(row_apply_brick)(g, xin, xout, lo, hi, mask, w, i, j, k);
+                            REAL d = FN(row_diag)(g, mask, w, i, j, k);
+                            REAL upd = (REAL)((REAL)(b[c] - ax) / d) * om;
+                            xout[c] = (REAL)(xout[c] + upd);
+                        }
+            }
+    }
+}
+
+/* The levels the brick smoother covers: every 3-D level but the coarsest
+ * whose cells do not fit in one brick (a level that does, smooths as one
+ * brick would: globally). */
+static int FN(brick_level)(const FN(orc_mg)* mg, int l) {
+    return mg->o.smoother == ORC_SMOOTH_BRICK && mg->g[l].ndim == 3 && l < mg->levels - 1 &&
+           orc_cells(mg->g[l].n) > (int64_t)orc_brick_dims()[0] * orc_brick_dims()[1] * orc_brick_dims()[2];
+}
+
 /* relax (multigrid.js:1589-1607; the Jacobi form multigrid_jacobi.js:1271-1285):
  * red-black takes `iters` colour pairs, post-smoothing colours reversed; the
  * Jacobi form takes `iters` single sweeps ping-ponging through buf (an even
@@ -431,6 +538,19 @@ void FN(orc_jacobi)(const orc_grid* g, const REAL* xr, REAL* xw, const REAL* b, 
  * order, being symmetric by construction. */
 static void FN(mg_relax_n)(FN(orc_mg)* mg, int l, int iters, int reversed) {
     if (orc_post_forward_flag()) reversed = 0; /* the non-symmetric V-cycle of the stop-code tests */
+    if (FN(brick_level)(mg, l)) {
+        /* all `iters` sweeps in one brick pass; buf holds the field on entry */
+        if (iters <= 0) return;
+        const REAL* const* w = mg->w[l][0] ? (const REAL* const*)mg->w[l] : NULL;
+        const int passes = orc_brick_passes();
+        for (int ps = 0; ps < passes; ++ps) {
+            const int shifted = reversed ? (passes - 1 - ps) : ps;
+            memcpy(mg->buf[l], mg->x[l], sizeof(REAL) * orc_cells(mg->g[l].n));
+            FN(orc_brick_relax_shape)(&mg->g[l], mg->buf[l], mg->x[l], mg->b[l], mg->mask[l], w, mg->o.omega, iters,
+                                      reversed, orc_brick_dims(), shifted);
+        }
+        return;
+    }
     if (mg->o.smoother == ORC_SMOOTH_JACOBI) {
         const REAL* const* w = mg->w[l][0] ? (const REAL* const*)mg->w[l] : NULL;
         for (int it = 0; it < iters; ++it) {

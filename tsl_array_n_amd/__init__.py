@@ -198,7 +198,7 @@ def _fields3(fields, n):
     return arr
 
 
-SMOOTHER = {"rbgs": 0, "jacobi": 1}
+SMOOTHER = {"rbgs": 0, "jacobi": 1, "brick": 2}
 
 
 def mg_options(levels=1, nu_down=2, nu_up=2, n_coarsest=20, n_final=2, omega=1.0, storage="f32", smoother="rbgs",

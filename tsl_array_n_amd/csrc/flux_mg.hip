@@ -19,6 +19,7 @@
 #include "flux_stencil.h"
 #include "flux_stencil3.h"
 #include "flux_sweep.h"
+#include "flux_brick.h"
 
 namespace flux {
 
@@ -1756,6 +1757,98 @@ void correct_level(flux_multigrid* mg, const MGLevel& v, const MGLevel& c, const
 template <typename ST>
 int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot);
 
+// The levels the brick smoother covers: every 3-D level but the coarsest
+// whose cells do not fit in one brick (the smaller ones, run in one
+// workgroup, smooth globally, as one brick would).
+bool brick_level(const flux_multigrid* mg, int l) {
+    const MGLevel& v = mg->lv[l];
+    return mg->o.smoother == FLUX_SMOOTH_BRICK && mg->ndim == 3 && l + 1 < (int)mg->lv.size() &&
+           (v.n[0] & 1) == 0 && cells_of(v.n) > (int64_t)BRICK_X * BRICK_Y * BRICK_Z;
+}
+
+// One brick pass of `sweeps` colour pairs from xin (nullptr: zero) into xout.
+template <int FINE, typename XT, typename WT>
+void brick_pass(flux_multigrid* mg, const MGLevel& v, bool shifted, const XT* xin, XT* xout, int sweeps,
+                bool reversed, const int* stop, DotOut* dot) {
+    const LevelDevT<WT> L = level_dev<WT>(mg, v);
+    const BrickGrid G = brick_grid(v.n[0], v.n[1], v.n[2], shifted);
+    const unsigned nb = (unsigned)G.nbx * G.nby * G.nbz;
+    const float om = smooth_omega(mg);
+    const int rev = reversed ? 1 : 0;
+    const XT* b = (const XT*)v.b;
+    // "brick" times the passes that read the field, "brick0" the first
+    // pre-smoothing pass from zero
+    const char* tag = xin ? "brick" : "brick0";
+    if (FINE) kprof_begin(tag);
+    const bool full = L.w0 && L.w1 && L.w2 && L.nb;
+#define FLUX_BRICK(Z_, D_, F_)                                                                                 \
+    k_brick3<Z_, D_, F_, XT, WT><<<nb, BRICK_TPB, 0, g_stream>>>(L, G, Z_ ? nullptr : xin, xout, b, om, sweeps, rev, \
+                                                                 stop, D_ ? dot->partials : nullptr)
+    const bool with_dot = xin && dot && nb <= dot->capacity;
+    if (!xin) {
+        if (full) FLUX_BRICK(1, 0, 1);
+        else FLUX_BRICK(1, 0, 0);
+    } else if (with_dot) {
+        if (full) FLUX_BRICK(0, 1, 1);
+        else FLUX_BRICK(0, 1, 0);
+        dot->blocks = nb;
+    } else {
+        if (full) FLUX_BRICK(0, 0, 1);
+        else FLUX_BRICK(0, 0, 0);
+    }
+#undef FLUX_BRICK
+    if (FINE) kprof_end(tag);
+}
+
+// A V-cycle level smoothed brick-wise (FLUX_SMOOTH_BRICK).  Pre-smoothing is
+// two passes from x = 0: bricks at the origin, then bricks shifted by half a
+// brick (so every cell near a brick face of the first pass lies inside a
+// brick of the second); post-smoothing runs the adjoint: shifted bricks then
+// unshifted, colours reversed.  Each pass reads one buffer and writes
+// another: x (first pre pass) -> alt (smoothed) -> residual in buf; after
+// the correction alt -> buf -> x.
+template <int FINE, typename XT, typename ST>
+int vcycle_brick(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot) {
+    MGLevel& v = mg->lv[l];
+    MGLevel& c = mg->lv[l + 1];
+    XT* x = (XT*)v.x;
+    XT* alt = (XT*)v.alt;
+    XT* buf = (XT*)v.buf;
+    const double* scale = FINE ? rr : nullptr;
+    const int nu = mg->o.nu_down;
+    if (nu > 0) {
+        brick_pass<FINE, XT, ST>(mg, v, false, nullptr, x, nu, false, stop, nullptr);
+        brick_pass<FINE, XT, ST>(mg, v, true, x, alt, nu, false, stop, nullptr);
+    } else {
+        fill_t<XT>(alt, cells_of(v.n), 0.0f);
+    }
+    if (FINE) kprof_begin("residual");
+    const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+    k_residual3<FINE, XT, XT, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), FINE ? flip_march(M) : M, alt,
+                                                                (const XT*)v.b, buf, stop);
+    if (FINE) kprof_end("residual");
+    if (FINE) kprof_begin("restrict");
+    restrict_level<FINE, XT, ST>(mg, v, c, buf, scale, stop);
+    if (FINE) kprof_end("restrict");
+    FLUX_TRY(vcycle_t<ST>(mg, l + 1, stop, rr, nullptr));
+    // the correction adds into alt (correct_level works on v.x)
+    void* xsave = v.x;
+    v.x = alt;
+    if (FINE) kprof_begin("correct");
+    correct_level<FINE, ST, XT>(mg, v, c, scale, stop);
+    if (FINE) kprof_end("correct");
+    v.x = xsave;
+    const int nf = FINE ? mg->o.n_final : mg->o.nu_up;
+    if (nf > 0) {
+        const bool rev = !mg->post_forward;
+        brick_pass<FINE, XT, ST>(mg, v, rev, alt, buf, nf, rev, stop, nullptr);
+        brick_pass<FINE, XT, ST>(mg, v, !rev, buf, x, nf, rev, stop, dot);
+    } else {
+        k_copy_t<XT><<<blocks_for(cells_of(v.n), MTPB), MTPB, 0, g_stream>>>(alt, x, (int)cells_of(v.n), stop);
+    }
+    return FLUX_OK;
+}
+
 // The finest level of the mixed form: r (FP32) is scaled into b~ by the
 // first pre-smoothing launch, the level smooths, computes its residual and
 // corrects in ST, and the last post-smoothing launch writes z in FP32 (and
@@ -1867,7 +1960,7 @@ int vcycle_fine16(flux_multigrid* mg, const int* stop, const double* rr, DotOut*
 // type ST; coarse levels store everything in ST.
 template <typename ST>
 int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot) {
-    if (l == 0 && mg->fine16) return vcycle_fine16<ST>(mg, stop, rr, dot);
+    if (l == 0 && mg->fine16 && !brick_level(mg, 0)) return vcycle_fine16<ST>(mg, stop, rr, dot);
     MGLevel& v = mg->lv[l];
     int nl = (int)mg->lv.size();
     const bool fine = (l == 0);
@@ -1894,6 +1987,8 @@ int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOu
         }
         return FLUX_OK;
     }
+    if (brick_level(mg, l))
+        return fine ? vcycle_brick<1, float, ST>(mg, l, stop, rr, dot) : vcycle_brick<0, ST, ST>(mg, l, stop, rr, dot);
     MGLevel& c = mg->lv[l + 1];
     int64_t cells = cells_of(v.n), cc = cells_of(c.n);
     const double* scale = fine ? rr : nullptr;
@@ -2022,8 +2117,8 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
                  "multigrid: unknown storage %d", o.storage);
     FLUX_REQUIRE(o.nu_down >= 0 && o.nu_up >= 0 && o.n_coarsest >= 0 && o.n_final >= 0,
                  "multigrid: iteration counts must be >= 0");
-    FLUX_REQUIRE(o.smoother == FLUX_SMOOTH_RBGS || o.smoother == FLUX_SMOOTH_JACOBI, "multigrid: unknown smoother %d",
-                 o.smoother);
+    FLUX_REQUIRE(o.smoother == FLUX_SMOOTH_RBGS || o.smoother == FLUX_SMOOTH_JACOBI || o.smoother == FLUX_SMOOTH_BRICK,
+                 "multigrid: unknown smoother %d", o.smoother);
     FLUX_REQUIRE(o.smoother != FLUX_SMOOTH_JACOBI ||
                      !((o.nu_down | o.nu_up | o.n_coarsest | o.n_final) & 1),
                  "multigrid (jacobi): sweep counts must be even so the ping-pong lands back in x, got %d/%d/%d/%d.",
@@ -2084,7 +2179,8 @@ int flux_multigrid_create(int ndim, const int* shape, const double* spacing, con
         size_t vs = (l == 0) ? 4 : es;  // finest vectors stay FP32
         v.buf = dmalloc(cells * vs);
         ok = ok && v.buf;
-        if (ndim == 3 && (mg->fused == 1 || (mg->fused == 2 && l == 0))) {
+        const bool brick = o.smoother == FLUX_SMOOTH_BRICK && ndim == 3 && l + 1 < o.levels;
+        if (ndim == 3 && (mg->fused == 1 || (mg->fused == 2 && l == 0) || brick)) {
             v.alt = dmalloc(cells * vs);
             ok = ok && v.alt;
         }
