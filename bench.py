@@ -22,9 +22,17 @@ Every form of the solver is run on the same system, each timed on its own:
   mixed_f16_cs_exact  mixed_f16_cs with the exact solution as a second coarse
                       vector: the bound of what a learned coarse vector could
                       add
+  mixed_f16_brick_cs  (headline) mixed_f16_cs with the brick smoother: every
+                      smoothing is two passes over memory, each running 6
+                      red-black sweeps brick by brick (64x8x8 cells, the
+                      second pass on bricks shifted by half a brick) in
+                      registers and LDS; post-smoothing the adjoint
+  mixed_f16_brick_cs_nu4, mixed_f16_brick (no correction), plain_brick (FP32
+                      storage), mixed_f16_brick_cs_exact (+ the exact
+                      solution as a coarse vector)
   plain_jacobi        (on request) FP32 V-cycle, damped Jacobi 4/4, omega 6/7
                       (sandbox/jacobi-smoother/multigrid_jacobi.js)
-The headline form (`--headline`, default mixed_f16_cs) gives `value`,
+The headline form (`--headline`, default mixed_f16_brick_cs) gives `value`,
 `ms_per_step` and the roofline; `forms` carries every form's time,
 iterations and FP64 relative residual, and `speedup_vs_plain` is each form's
 time against plain_rbgs.  The CPU leg times the oracle's restatement of the
@@ -66,12 +74,20 @@ FORMS = {
     "plain_jacobi": {"storage": "f32", "smoother": "jacobi", "nu": 4, "omega": 6.0 / 7.0},
     # brick-wise red-black: every smoothing is two passes over memory (bricks
     # of 64x8x8 cells, then bricks shifted by half a brick), each running all
-    # of its sweeps on the brick in registers and LDS
-    "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0, "cs": "free"},
-    "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0},
-    "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 4, "omega": 1.0},
+    # of its sweeps on the brick in registers and LDS.  Sweeps per pass chosen
+    # by measurement at 256^3 on the FLIP system (with the free-cell
+    # correction: 2 -> 32 iterations 38.9 ms, 4 -> 20 / 27.5, 6 -> 16 / 23.9,
+    # 8 -> 15 / 24.2)
+    "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free"},
+    "mixed_f16_brick_cs_nu4": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0, "cs": "free"},
+    "mixed_f16_brick_cs_nu8": {"storage": "f16", "smoother": "brick", "nu": 8, "omega": 1.0, "cs": "free"},
+    "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0},
+    "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 6, "omega": 1.0},
+    # the bound of any learned coarse vector with the brick smoother
+    "mixed_f16_brick_cs_exact": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free+exact"},
 }
-DEFAULT_FORMS = "plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_nu2,mixed_f16_cs_exact"
+DEFAULT_FORMS = ("plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_exact,mixed_f16_brick_cs_nu4,"
+                 "mixed_f16_brick,plain_brick,mixed_f16_brick_cs_exact")
 # the single custom form of --storage/--smoother/--nu/--coarse-space sweeps
 
 
@@ -88,7 +104,7 @@ def parse():
                          "particles at rest)")
     ap.add_argument("--flip-steps", type=int, default=3)
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--headline", default="mixed_f16_cs", choices=sorted(FORMS))
+    ap.add_argument("--headline", default="mixed_f16_brick_cs", choices=sorted(FORMS))
     ap.add_argument("--forms", default=DEFAULT_FORMS,
                     help="comma-separated forms to run (the headline always runs); plain_jacobi is not in the default "
                          "set: at 256^3 its V-cycle is not a usable preconditioner for this system (the residual-growth "
