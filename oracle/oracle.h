@@ -65,6 +65,8 @@ enum { ORC_SMOOTH_RBGS = 0, ORC_SMOOTH_JACOBI = 1, ORC_SMOOTH_BRICK = 2 };
 #define ORC_BRICK_X 64
 #define ORC_BRICK_Y 8
 #define ORC_BRICK_Z 8
+/* levels of more than this many cells smooth brick-wise */
+#define ORC_BRICK_MIN_CELLS 4096
 
 typedef struct {
     int iterations;

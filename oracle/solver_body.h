@@ -524,11 +524,11 @@ void FN(orc_brick_relax_shape)(const orc_grid* g, const REAL* xin, REAL* xout, c
 }
 
 /* The levels the brick smoother covers: every 3-D level but the coarsest
- * whose cells do not fit in one brick (a level that does, smooths as one
- * brick would: globally). */
+ * with more than ORC_BRICK_MIN_CELLS cells (smaller levels smooth globally,
+ * as the library runs them in one workgroup). */
 static int FN(brick_level)(const FN(orc_mg)* mg, int l) {
     return mg->o.smoother == ORC_SMOOTH_BRICK && mg->g[l].ndim == 3 && l < mg->levels - 1 &&
-           orc_cells(mg->g[l].n) > (int64_t)orc_brick_dims()[0] * orc_brick_dims()[1] * orc_brick_dims()[2];
+           orc_cells(mg->g[l].n) > (int64_t)ORC_BRICK_MIN_CELLS;
 }
 
 /* relax (multigrid.js:1589-1607; the Jacobi form multigrid_jacobi.js:1271-1285):

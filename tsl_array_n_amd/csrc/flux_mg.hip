@@ -1758,12 +1758,12 @@ template <typename ST>
 int vcycle_t(flux_multigrid* mg, int l, const int* stop, const double* rr, DotOut* dot);
 
 // The levels the brick smoother covers: every 3-D level but the coarsest
-// whose cells do not fit in one brick (the smaller ones, run in one
-// workgroup, smooth globally, as one brick would).
+// with more than 4096 cells (the smaller ones run in one workgroup, in the
+// V-cycle tail or the coarsest solve, and smooth globally).
 bool brick_level(const flux_multigrid* mg, int l) {
     const MGLevel& v = mg->lv[l];
     return mg->o.smoother == FLUX_SMOOTH_BRICK && mg->ndim == 3 && l + 1 < (int)mg->lv.size() &&
-           (v.n[0] & 1) == 0 && cells_of(v.n) > (int64_t)BRICK_X * BRICK_Y * BRICK_Z;
+           (v.n[0] & 1) == 0 && cells_of(v.n) > (int64_t)COARSE_LDS_CELLS;
 }
 
 // One brick pass of `sweeps` colour pairs from xin (nullptr: zero) into xout.
