@@ -81,8 +81,11 @@ FORMS = {
     "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free"},
     "mixed_f16_brick_cs_nu4": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0, "cs": "free"},
     "mixed_f16_brick_cs_nu8": {"storage": "f16", "smoother": "brick", "nu": 8, "omega": 1.0, "cs": "free"},
-    "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0},
-    "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 6, "omega": 1.0},
+    # without the free-cell correction (the system is a closed box pinned at
+    # one cell, nearly singular), 6 sweeps stop on the reference's
+    # residual-growth guard near 6e-4 after 15 iterations; 4 sweeps converge
+    "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0},
+    "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 4, "omega": 1.0},
     # the bound of any learned coarse vector with the brick smoother
     "mixed_f16_brick_cs_exact": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free+exact"},
 }
