@@ -29,6 +29,7 @@
 // of consecutive words across a wavefront.
 #pragma once
 #include <cstdlib>
+#include <type_traits>
 
 #include "flux_stencil3.h"
 
@@ -229,41 +230,90 @@ __global__ void __launch_bounds__(BRICK_TPB) k_brick3(LevelDevT<WT> L, BrickGrid
     }
     __syncthreads();
 
-    // ---- the sweeps
+    // ---- the sweeps.  A cell's colour is (i + j + k) & 1 and i0 is even, so
+    // on plane kk the colour-0 cell of the pair sits at position
+    // sa = (j + k) & 1 and the colour-1 cell at 1 - sa, alternating from
+    // plane to plane.  The pair is relabelled once with its coefficients
+    // (A: colour 0, B: colour 1), so a colour phase updates A (or B) on all
+    // four planes with no per-update selection: the pair partner and the
+    // z-neighbours of A on plane kk are B on planes kk, kk - 1 and kk + 1
+    // (registers; at the thread's end planes the z-halo or the next
+    // thread's plane from LDS), the y- and side neighbours come from LDS.  A
+    // phase reads every LDS neighbour of its four updates first, then
+    // updates, then writes, so its reads are in flight together; it reads
+    // only the other colour and writes only its own.
     const int kl0 = BRICK_KT * tz + 1;  // LDS plane slot of this thread's first plane
+    float xa[BRICK_KT], xb[BRICK_KT], ba[BRICK_KT], bb[BRICK_KT];
+    float ea[BRICK_KT][6], eb[BRICK_KT][6];
+    int sa[BRICK_KT];
+#pragma unroll
+    for (int kk = 0; kk < BRICK_KT; ++kk) {
+        sa[kk] = (j + kt0 + kk) & 1;
+        const bool s = sa[kk] != 0;
+        xa[kk] = s ? xh[kk] : xl[kk];
+        xb[kk] = s ? xl[kk] : xh[kk];
+        ba[kk] = s ? bp1[kk] : bp0[kk];
+        bb[kk] = s ? bp0[kk] : bp1[kk];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            ea[kk][a] = s ? e1[kk][a] : e0[kk][a];
+            eb[kk][a] = s ? e0[kk][a] : e1[kk][a];
+        }
+    }
+    // one colour phase: C = 0 updates A, C = 1 updates B
+    auto phase = [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        float* xu = C ? xb : xa;  // the updated cells
+        const float* xq = C ? xa : xb;  // their pair partners (the other colour)
+        const float* bu = C ? bb : ba;
+        float side[BRICK_KT], ym[BRICK_KT], yp[BRICK_KT];
+#pragma unroll
+        for (int kk = 0; kk < BRICK_KT; ++kk) {
+            const int s = C ? 1 - sa[kk] : sa[kk];  // position of the updated cell
+            const int kl = kl0 + kk;
+            // the x-neighbour outside the pair: the odd cell of pair tx-1 for
+            // the even cell, the even cell of pair tx+1 for the odd one
+            side[kk] = xs[1 - s][kl][jj][pp + 2 * s - 1];
+            ym[kk] = xs[s][kl][jj - 1][pp];
+            yp[kk] = xs[s][kl][jj + 1][pp];
+        }
+        const int s0 = C ? 1 - sa[0] : sa[0], s3 = C ? 1 - sa[BRICK_KT - 1] : sa[BRICK_KT - 1];
+        const float zlo = xs[s0][kl0 - 1][jj][pp];
+        const float zhi = xs[s3][kl0 + BRICK_KT][jj][pp];
+        float nv[BRICK_KT];
+#pragma unroll
+        for (int kk = 0; kk < BRICK_KT; ++kk) {
+            const bool s = (C ? 1 - sa[kk] : sa[kk]) != 0;
+            const float* e = C ? eb[kk] : ea[kk];
+            const float xc = xu[kk], xp = xq[kk];
+            const float zm = kk > 0 ? xq[kk - 1] : zlo;
+            const float zp = kk + 1 < BRICK_KT ? xq[kk + 1] : zhi;
+            const float xm = s ? xp : side[kk];
+            const float xr = s ? side[kk] : xp;
+            float acc = bu[kk];
+            acc -= e[0] * xm;
+            acc -= e[1] * xr;
+            acc -= e[2] * ym[kk];
+            acc -= e[3] * yp[kk];
+            acc -= e[4] * zm;
+            acc -= e[5] * zp;
+            nv[kk] = xc + omega * (acc - xc);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BRICK_KT; ++kk) {
+            const int s = C ? 1 - sa[kk] : sa[kk];
+            xu[kk] = nv[kk];
+            xs[s][kl0 + kk][jj][pp] = nv[kk];
+        }
+        __syncthreads();
+    };
     for (int sw = 0; sw < sweeps; ++sw) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int color = reversed ? 1 - h : h;
-#pragma unroll
-            for (int kk = 0; kk < BRICK_KT; ++kk) {
-                const int k = kt0 + kk;
-                const int kl = kl0 + kk;
-                const int s = (j + k + color) & 1;  // which cell of the pair has the colour
-                const float xc = s ? xh[kk] : xl[kk];
-                const float xp = s ? xl[kk] : xh[kk];
-                // the x-neighbour outside the pair: the odd cell of pair tx-1 for
-                // the even cell, the even cell of pair tx+1 for the odd one
-                const float xside = xs[1 - s][kl][jj][pp + 2 * s - 1];
-                const float ym = xs[s][kl][jj - 1][pp];
-                const float yp = xs[s][kl][jj + 1][pp];
-                const float zm = kk > 0 ? (s ? xh[kk - 1] : xl[kk - 1]) : xs[s][kl - 1][jj][pp];
-                const float zp = kk + 1 < BRICK_KT ? (s ? xh[kk + 1] : xl[kk + 1]) : xs[s][kl + 1][jj][pp];
-                const float xm = s ? xp : xside;
-                const float xq = s ? xside : xp;
-                float acc = s ? bp1[kk] : bp0[kk];
-                acc -= (s ? e1[kk][0] : e0[kk][0]) * xm;
-                acc -= (s ? e1[kk][1] : e0[kk][1]) * xq;
-                acc -= (s ? e1[kk][2] : e0[kk][2]) * ym;
-                acc -= (s ? e1[kk][3] : e0[kk][3]) * yp;
-                acc -= (s ? e1[kk][4] : e0[kk][4]) * zm;
-                acc -= (s ? e1[kk][5] : e0[kk][5]) * zp;
-                const float nv = xc + omega * (acc - xc);
-                if (s) xh[kk] = nv;
-                else xl[kk] = nv;
-                xs[s][kl][jj][pp] = nv;
-            }
-            __syncthreads();
+        if (reversed) {
+            phase(std::integral_constant<int, 1>());
+            phase(std::integral_constant<int, 0>());
+        } else {
+            phase(std::integral_constant<int, 0>());
+            phase(std::integral_constant<int, 1>());
         }
     }
 
@@ -273,9 +323,10 @@ __global__ void __launch_bounds__(BRICK_TPB) k_brick3(LevelDevT<WT> L, BrickGrid
     for (int kk = 0; kk < BRICK_KT; ++kk) {
         const int k = kt0 + kk;
         if (!(col && k >= 0 && k < n2)) continue;
+        const bool s = sa[kk] != 0;
         Pair2<XT> o;
-        o.lo = to_store<XT>(xl[kk]);
-        o.hi = to_store<XT>(xh[kk]);
+        o.lo = to_store<XT>(s ? xb[kk] : xa[kk]);
+        o.hi = to_store<XT>(s ? xa[kk] : xb[kk]);
         *reinterpret_cast<Pair2<XT>*>(xout + c0 + plane * k) = o;
         if (DOT) {
             dot += (double)bo0[kk] * (double)to_f32(o.lo);
