@@ -23,11 +23,11 @@ Every form of the solver is run on the same system, each timed on its own:
                       vector: the bound of what a learned coarse vector could
                       add
   mixed_f16_brick_cs  (headline) mixed_f16_cs with the brick smoother: every
-                      smoothing is two passes over memory, each running 6
+                      smoothing is two passes over memory, each running 12
                       red-black sweeps brick by brick (64x8x8 cells, the
                       second pass on bricks shifted by half a brick) in
                       registers and LDS; post-smoothing the adjoint
-  mixed_f16_brick_cs_nu4, mixed_f16_brick (no correction), plain_brick (FP32
+  mixed_f16_brick_cs_nu6, mixed_f16_brick (no correction), plain_brick (FP32
                       storage), mixed_f16_brick_cs_exact (+ the exact
                       solution as a coarse vector)
   plain_jacobi        (on request) FP32 V-cycle, damped Jacobi 4/4, omega 6/7
@@ -75,21 +75,23 @@ FORMS = {
     # brick-wise red-black: every smoothing is two passes over memory (bricks
     # of 64x8x8 cells, then bricks shifted by half a brick), each running all
     # of its sweeps on the brick in registers and LDS.  Sweeps per pass chosen
-    # by measurement at 256^3 on the FLIP system (with the free-cell
-    # correction: 2 -> 32 iterations 38.9 ms, 4 -> 20 / 27.5, 6 -> 16 / 23.9,
-    # 8 -> 15 / 24.2)
-    "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free"},
+    # by measurement at 256^3 on the FLIP system with the free-cell
+    # correction (iterations / ms per projection: 1 -> 58 / 67.3, 2 -> 32 /
+    # 39.0, 4 -> 20 / 26.4, 6 -> 16 / 22.6, 8 -> 15 / 22.5, 10 -> 14 / 22.2,
+    # 12 -> 13 / 21.7); fewer sweeps on the coarse levels than on the finest
+    # cost iterations (12 fine / 4 coarse: 17 iterations, 26.0 ms)
+    "mixed_f16_brick_cs": {"storage": "f16", "smoother": "brick", "nu": 12, "omega": 1.0, "cs": "free"},
     "mixed_f16_brick_cs_nu4": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0, "cs": "free"},
-    "mixed_f16_brick_cs_nu8": {"storage": "f16", "smoother": "brick", "nu": 8, "omega": 1.0, "cs": "free"},
+    "mixed_f16_brick_cs_nu6": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free"},
     # without the free-cell correction (the system is a closed box pinned at
     # one cell, nearly singular), 6 sweeps stop on the reference's
     # residual-growth guard near 6e-4 after 15 iterations; 4 sweeps converge
     "mixed_f16_brick": {"storage": "f16", "smoother": "brick", "nu": 4, "omega": 1.0},
     "plain_brick": {"storage": "f32", "smoother": "brick", "nu": 4, "omega": 1.0},
     # the bound of any learned coarse vector with the brick smoother
-    "mixed_f16_brick_cs_exact": {"storage": "f16", "smoother": "brick", "nu": 6, "omega": 1.0, "cs": "free+exact"},
+    "mixed_f16_brick_cs_exact": {"storage": "f16", "smoother": "brick", "nu": 12, "omega": 1.0, "cs": "free+exact"},
 }
-DEFAULT_FORMS = ("plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_exact,mixed_f16_brick_cs_nu4,"
+DEFAULT_FORMS = ("plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mixed_f16_cs_exact,mixed_f16_brick_cs_nu6,"
                  "mixed_f16_brick,plain_brick,mixed_f16_brick_cs_exact")
 # the single custom form of --storage/--smoother/--nu/--coarse-space sweeps
 
@@ -179,9 +181,9 @@ KERNEL_SYMBOL = {("relax", "f32"): "k_relax3<1, 0, 0, float, float>",
                  ("relax", "bf16"): "k_relax3<1, 0, 0, float, __hip_bfloat16>",
                  ("jacobi", "f32"): "k_jacobi3<1, 0, 0, float, float>",
                  ("jacobi", "f16"): "k_jacobi3<1, 0, 0, float, __half>",
-                 ("brick", "f32"): "k_brick3<0, 0, float, float>",
-                 ("brick", "f16"): "k_brick3<0, 0, float, __half>",
-                 ("brick", "bf16"): "k_brick3<0, 0, float, __hip_bfloat16>",
+                 ("brick", "f32"): "k_brick3<0, 0, 1, float, float>",
+                 ("brick", "f16"): "k_brick3<0, 0, 1, float, __half>",
+                 ("brick", "bf16"): "k_brick3<0, 0, 1, float, __hip_bfloat16>",
                  ("update", "f32"): "k_pcg_update3", ("apply", "f32"): "k_pcg_apply3"}
 if fine_storage():
     KERNEL_SYMBOL.update({("relax", "f16"): "k_relax3m<0, 0, __half>",
@@ -194,7 +196,9 @@ def measured_traffic(kernel, storage):
     """HBM bytes per dispatch of the family's fine-level kernel from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
     None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_dambreak256_%s.json" % storage)
+    # the brick smoother's summary is its own (headline form mixed_f16_brick_cs)
+    name = "pmc_traffic_dambreak256_%s%s.json" % (storage, "_brick" if kernel == "brick" else "")
+    path = os.path.join(ROOT, "profiles", name)
     sym = KERNEL_SYMBOL.get((kernel, storage)) or KERNEL_SYMBOL.get((kernel, "f32"))
     if not sym or not os.path.exists(path):
         return None
