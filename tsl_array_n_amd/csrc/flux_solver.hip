@@ -163,6 +163,51 @@ __global__ void k_pcg_setup_residual(LevelDev L, const float* __restrict__ x32, 
     store_block_partial(v, partials);
 }
 
+// The same in the column-march shape of k_pcg_update3 (one cell per thread
+// and plane, z-neighbours carried), the PCG's setup at every solve.
+__global__ void __launch_bounds__(TPB) k_pcg_setup_residual3(LevelDev L, March M, const float* __restrict__ x32,
+                                                             double* __restrict__ x64, const float* __restrict__ b,
+                                                             float* __restrict__ r, double* partials) {
+    double v = 0;
+    int t, k0, k1;
+    if (march_begin(M, t, k0, k1)) {
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const int j = t / n0;
+        const int i = t - j * n0;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const double ih2[3] = {L.ih2d[0], L.ih2d[1], L.ih2d[2]};
+        double xnext = (double)x32[i + n0 * (j + n1 * k0)];
+        double xprev = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const int c = i + n0 * (j + n1 * k);
+            const double xc = xnext;
+            if (k + 1 < n2) xnext = (double)x32[c + plane];
+            const unsigned bits = nbits ? nbits[c] : 0u;
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            double xn[6];
+            xn[0] = (double)x32[i > 0 ? c - 1 : c];
+            xn[1] = (double)x32[i < n0 - 1 ? c + 1 : c];
+            xn[2] = (double)x32[j > 0 ? c - n0 : c];
+            xn[3] = (double)x32[j < n1 - 1 ? c + n0 : c];
+            xn[4] = k > k0 ? xprev : (double)x32[k > 0 ? c - plane : c];
+            xn[5] = k + 1 < n2 ? xnext : xc;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0 : xn[a];
+            double sum, dsum;
+            row_terms<double>(xc, xn, w, ih2, sum, dsum);
+            const double ax = (bits & 64u) ? -xc : sum;
+            x64[c] = xc;
+            const double rv = (double)b[c] - ax;
+            r[c] = (float)rv;
+            v += rv * rv;
+            xprev = xc;
+        }
+    }
+    store_block_partial(v, partials);
+}
+
 // Terms of a Galerkin-scaled start along a direction g (free cells only):
 // partials[blk] = g.(b - A x), partials[nb + blk] = g.(A g), in FP64.
 __global__ void k_galerkin_terms(LevelDev L, const float* __restrict__ x, const float* __restrict__ b,
@@ -270,6 +315,10 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_rz(Scal* S, const double* parti
 
 // Iteration step 1: p = z + beta p (updateP, fused); partials of p.Ap.  Ap
 // itself is not stored: the update recomputes r = b - A x from x.  Column-march shape: one cell per thread and plane.
+// FREE, NF: the coarse space's shape (free-cell indicator column, number of
+// caller fields), fixed at compile time so that every load of a cell's
+// values is issued unconditionally and all of them are in flight together.
+template <int FREE, int NF>
 __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const float* __restrict__ z,
                                                     float* __restrict__ pa, float* __restrict__ pb, const Scal* S,
                                                     CoarseSpace cs, double* partials) {
@@ -277,8 +326,9 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
     const float* __restrict__ pold = S->psel ? pb : pa;
     float* __restrict__ pnew = S->psel ? pa : pb;
     const float beta = (float)S->beta;
-    const int csm = cs.m();
+    constexpr int csm = FREE + NF;
     float ca[CS_MAX] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int q = 0; q < csm; ++q) ca[q] = (float)S->cs_a[q];
     double v = 0;
     int t, k0, k1;
@@ -294,13 +344,17 @@ __global__ void __launch_bounds__(TPB) k_pcg_apply3(LevelDev L, March M, const f
         // masked to 0 below, so only centre values need the cell's own bit)
         auto zval = [&](int q, bool dir) {
             float v = z[q];
-            for (int t = 0; t < csm; ++t) v = v + ca[t] * cs_phi(cs, t, q, dir);
+#pragma unroll
+            for (int t = 0; t < csm; ++t) {
+                const float phi = (FREE && t == 0) ? 1.0f : cs.f[t - FREE][q];
+                v = v + ca[t] * (dir ? 0.0f : phi);
+            }
             return v;
         };
         auto pval = [&](int q, bool dir) { return zval(q, dir) + pold[q] * beta; };
         // the z-neighbours are the previous and the next plane's centres
         const int c0 = i + n0 * (j + n1 * k0);
-        float pnext = pval(c0, csm && nbits && ((nbits[c0] >> 6) & 1u));
+        float pnext = pval(c0, csm > 0 && nbits && ((nbits[c0] >> 6) & 1u));
         float pprev = 0.0f;
         for (int k = k0; k < k1; ++k) {
             const int c = i + n0 * (j + n1 * k);
@@ -354,6 +408,7 @@ __global__ void __launch_bounds__(FIN_TPB) k_fin_pap(Scal* S, const double* part
 // partials[gridDim.x ...].
 // With a coarse space, the partials of Phi^T r follow (one array of
 // gridDim.x per column, after the rr and r.z_k arrays).
+template <int FREE, int NF>
 __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double* __restrict__ xa,
                                                      double* __restrict__ xb, const float* __restrict__ pa,
                                                      const float* __restrict__ pb, const float* __restrict__ b,
@@ -366,7 +421,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double
     const double alpha = S->alpha;
     double v = 0, vz = 0;
     double vc[CS_MAX] = {0, 0, 0, 0};
-    const int csm = cs.m();
+    constexpr int csm = FREE + NF;
     int t, k0, k1;
     if (march_begin(M, t, k0, k1)) {
         const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
@@ -404,12 +459,18 @@ __global__ void __launch_bounds__(TPB) k_pcg_update3(LevelDev L, March M, double
             r[c] = rf;
             v += rv * rv;
             if (zold) vz += (double)rf * (double)zold[c];
-            for (int q = 0; q < csm; ++q) vc[q] += (double)rf * (double)cs_phi(cs, q, c, (bits & 64u) != 0);
+            const bool dir = (bits & 64u) != 0;
+#pragma unroll
+            for (int q = 0; q < csm; ++q) {
+                const float phi = (FREE && q == 0) ? 1.0f : cs.f[q - FREE][c];
+                vc[q] += (double)rf * (double)(dir ? 0.0f : phi);
+            }
             xprev = xc;
         }
     }
     store_block_partial(v, partials);
     if (zold) store_block_partial(vz, partials + gridDim.x);
+#pragma unroll
     for (int q = 0; q < csm; ++q) store_block_partial(vc[q], partials + (size_t)(2 + q) * gridDim.x);
 }
 
@@ -455,6 +516,48 @@ __global__ void __launch_bounds__(TPB) k_cs_gram(LevelDev L, CoarseSpace cs, dou
             for (int q = 0; q < m; ++q) e[q * m + t] += (double)cs_phi(cs, q, c, false) * at;
         }
     }
+    for (int q = 0; q < m * m; ++q) store_block_partial(e[q], partials + (size_t)q * gridDim.x);
+}
+
+// The same Gram matrix in the column-march shape, the coarse space's shape
+// fixed at compile time: column t's values at the cell and its neighbours
+// are loaded together (phi_t = 1 on free cells for the indicator column).
+template <int FREE, int NF>
+__global__ void __launch_bounds__(TPB) k_cs_gram3(LevelDev L, March M, CoarseSpace cs, double* __restrict__ partials) {
+    constexpr int m = FREE + NF;
+    double e[CS_MAX * CS_MAX];
+#pragma unroll
+    for (int q = 0; q < m * m; ++q) e[q] = 0.0;
+    int t, k0, k1;
+    if (march_begin(M, t, k0, k1)) {
+        const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+        const int plane = n0 * n1;
+        const int j = t / n0;
+        const int i = t - j * n0;
+        const uint8_t* __restrict__ nbits = L.nb;
+        const double ih2[3] = {L.ih2d[0], L.ih2d[1], L.ih2d[2]};
+        auto phi = [&](int q, int cell) { return (FREE && q == 0) ? 1.0 : (double)cs.f[q - FREE][cell]; };
+        for (int k = k0; k < k1; ++k) {
+            const int c = i + n0 * (j + n1 * k);
+            const unsigned bits = nbits ? nbits[c] : 0u;
+            if (bits & 64u) continue;
+            float w[6];
+            face_w3(L, i, j, k, c, w);
+            const int nq[6] = {i > 0 ? c - 1 : c, i < n0 - 1 ? c + 1 : c, j > 0 ? c - n0 : c,
+                               j < n1 - 1 ? c + n0 : c, k > 0 ? c - plane : c, k + 1 < n2 ? c + plane : c};
+#pragma unroll
+            for (int tt = 0; tt < m; ++tt) {
+                double xn[6];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0 : phi(tt, nq[a]);
+                double at, dsum;
+                row_terms<double>(phi(tt, c), xn, w, ih2, at, dsum);
+#pragma unroll
+                for (int q = 0; q < m; ++q) e[q * m + tt] += phi(q, c) * at;
+            }
+        }
+    }
+#pragma unroll
     for (int q = 0; q < m * m; ++q) store_block_partial(e[q], partials + (size_t)q * gridDim.x);
 }
 
@@ -666,14 +769,24 @@ static int pcg_precondition(flux_pcg* s, DotOut* dot) {
 static int pcg_iteration(flux_pcg* s) {
     unsigned g = red_blocks(s->n);
     kprof_begin("apply");
-    k_pcg_apply3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->z, s->pa, s->pb, s->S, s->cs,
-                                                s->partials);
+#define FLUX_CS_SHAPE(KERNEL, ...)                                                                      \
+    switch (s->cs.free0 * 4 + s->cs.nf) {                                                               \
+        case 0: KERNEL<0, 0><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 1: KERNEL<0, 1><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 2: KERNEL<0, 2><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 3: KERNEL<0, 3><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 4: KERNEL<1, 0><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 5: KERNEL<1, 1><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        case 6: KERNEL<1, 2><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                           \
+        default: KERNEL<1, 3><<<s->M.nb, TPB, 0, g_stream>>>(__VA_ARGS__); break;                          \
+    }
+    FLUX_CS_SHAPE(k_pcg_apply3, s->op->L, flip_march(s->M), s->z, s->pa, s->pb, s->S, s->cs, s->partials);
     kprof_end("apply");
     k_fin_pap<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     kprof_begin("update");
-    k_pcg_update3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, flip_march(s->M), s->xa, s->xb, s->pa, s->pb,
-                                                 (const float*)s->b->d, s->r, s->flexible ? s->z : nullptr, s->S,
-                                                 s->cs, s->partials);
+    FLUX_CS_SHAPE(k_pcg_update3, s->op->L, flip_march(s->M), s->xa, s->xb, s->pa, s->pb, (const float*)s->b->d,
+                  s->r, s->flexible ? s->z : nullptr, s->S, s->cs, s->partials);
+#undef FLUX_CS_SHAPE
     kprof_end("update");
     k_fin_rr<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     DotOut dot;
@@ -747,8 +860,16 @@ int coarse_space_setup(flux_pcg* s, Scal* Sh) {
     const int m = s->cs.m();
     Sh->cs_m = m;
     if (!m) return FLUX_OK;
-    const unsigned g = red_blocks(s->n);
-    k_cs_gram<<<g, TPB, 0, g_stream>>>(s->op->L, s->cs, s->partials);
+    const unsigned g = s->M.nb;
+    switch (s->cs.free0 * 4 + s->cs.nf) {
+        case 1: k_cs_gram3<0, 1><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        case 2: k_cs_gram3<0, 2><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        case 3: k_cs_gram3<0, 3><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        case 4: k_cs_gram3<1, 0><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        case 5: k_cs_gram3<1, 1><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        case 6: k_cs_gram3<1, 2><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+        default: k_cs_gram3<1, 3><<<g, TPB, 0, g_stream>>>(s->op->L, s->M, s->cs, s->partials); break;
+    }
     FLUX_LAUNCH_CHECK();
     std::vector<double> h((size_t)m * m * g);
     FLUX_HIP(hipMemcpyAsync(h.data(), s->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, g_stream));
@@ -777,9 +898,9 @@ int pcg_solve_raw(flux_pcg* s, double tol, int maxiter, int relative, flux_pcg_s
     FLUX_HIP(hipMemcpyAsync(s->S, s->Sh, sizeof(Scal), hipMemcpyHostToDevice, g_stream));
     FLUX_HIP(hipStreamSynchronize(g_stream));
     // setup (runGpuResidentSetup, linalg.js:1613-1637)
-    k_pcg_setup_residual<<<g, TPB, 0, g_stream>>>(s->op->L, (const float*)s->x->d, s->xa, (const float*)s->b->d, s->r,
-                                                   s->partials);
-    k_fin_setup<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
+    k_pcg_setup_residual3<<<s->M.nb, TPB, 0, g_stream>>>(s->op->L, s->M, (const float*)s->x->d, s->xa,
+                                                          (const float*)s->b->d, s->r, s->partials);
+    k_fin_setup<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, s->M.nb);
     k_pcg_dot_bb<<<g, TPB, 0, g_stream>>>((const float*)s->b->d, (int)s->n, s->partials);
     k_fin_bb<<<1, FIN_TPB, 0, g_stream>>>(s->S, s->partials, g);
     if (s->cs.m()) {
@@ -858,7 +979,7 @@ int flux_pcg_create(flux_operator* op, int precond, flux_multigrid* mg, flux_fie
     // (2 + CS_MAX) arrays of the update's partials; CS_MAX^2 of the coarse
     // space's Gram matrix
     s->n_partials = (unsigned)std::max<int64_t>(
-        std::max<int64_t>((int64_t)CS_MAX * CS_MAX * RED_MAX_BLOCKS, (2 + CS_MAX) * (int64_t)s->M.nb), 4096);
+        (int64_t)CS_MAX * CS_MAX * std::max<int64_t>(RED_MAX_BLOCKS, (int64_t)s->M.nb), 4096);
     // the reference's beta unless asked otherwise (FLUX_FLEXIBLE_PCG=1 or
     // flux_pcg_set_flexible): measured on the 64^3 dam break with FP16
     // finest-level storage, the flexible beta took more iterations (63) than
