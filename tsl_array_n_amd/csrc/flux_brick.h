@@ -46,16 +46,29 @@ struct BrickGrid {
     int nbx, nby, nbz;  // bricks per axis
     int ox, oy, oz;     // shift of the brick origin (0 or half a brick)
     int order;          // block -> brick: 0 x fastest, 1 z fastest; +2: no XCD remap
+    int wrap;           // shifted bricks wrap around the grid (see brick_grid)
 };
 
+// Shifted bricks on a grid of whole bricks wrap around it (wrap_ok): the
+// brick at the origin takes the last half brick of each axis with the first
+// half brick, so the shifted pass has as many full bricks as the unshifted
+// one instead of partial ones at every face.  Its two (four, eight) parts
+// are not coupled: a face on the grid's boundary has weight 0, so the
+// coefficient of the neighbour across the seam is exactly 0 and every cell
+// is updated as in its partial brick, bit for bit.  FLUX_BRICK_WRAP=0 keeps
+// the partial bricks (read at every call).
 inline BrickGrid brick_grid(int n0, int n1, int n2, bool shifted) {
     BrickGrid g;
+    const char* we = getenv("FLUX_BRICK_WRAP");
+    const bool wrap_ok = !(we && we[0] == '0');
+    g.wrap = shifted && wrap_ok && n0 % BRICK_X == 0 && n1 % BRICK_Y == 0 && n2 % BRICK_Z == 0 ? 1 : 0;
     g.ox = shifted ? BRICK_X / 2 : 0;
     g.oy = shifted ? BRICK_Y / 2 : 0;
     g.oz = shifted ? BRICK_Z / 2 : 0;
-    g.nbx = (n0 + g.ox + BRICK_X - 1) / BRICK_X;
-    g.nby = (n1 + g.oy + BRICK_Y - 1) / BRICK_Y;
-    g.nbz = (n2 + g.oz + BRICK_Z - 1) / BRICK_Z;
+    const int ex = g.wrap ? 0 : g.ox, ey = g.wrap ? 0 : g.oy, ez = g.wrap ? 0 : g.oz;
+    g.nbx = (n0 + ex + BRICK_X - 1) / BRICK_X;
+    g.nby = (n1 + ey + BRICK_Y - 1) / BRICK_Y;
+    g.nbz = (n2 + ez + BRICK_Z - 1) / BRICK_Z;
     static const int order = [] {
         const char* e = getenv("FLUX_BRICK_ORDER");
         return e ? atoi(e) & 3 : 0;
@@ -94,8 +107,16 @@ __global__ void __launch_bounds__(BRICK_TPB) k_brick3(LevelDevT<WT> L, BrickGrid
     const int lo0 = bi * BRICK_X - G.ox, lo1 = bj * BRICK_Y - G.oy, lo2 = bk * BRICK_Z - G.oz;
     const int t = threadIdx.x;
     const int tx = t & 31, ty = (t >> 5) & 7, tz = t >> 8;
-    const int i0 = lo0 + 2 * tx, j = lo1 + ty;
-    const int kt0 = lo2 + BRICK_KT * tz;  // first plane of this thread
+    // grid coordinates of a brick position: modulo the grid when wrapping
+    // (every position then lies inside it)
+    const int wr = G.wrap;
+    auto wx = [&](int i) { return wr ? (i + n0) % n0 : i; };
+    auto wy = [&](int jv) { return wr ? (jv + n1) % n1 : jv; };
+    auto wz = [&](int k) { return wr ? (k + n2) % n2 : k; };
+    const int i0 = wx(lo0 + 2 * tx), j = wy(lo1 + ty);
+    // first plane of this thread: its 4 planes never straddle the seam (a
+    // multiple of 4, the seam a multiple of 8)
+    const int kt0 = wz(lo2 + BRICK_KT * tz);
     // n0 is even, so a pair is either wholly inside the grid or wholly outside
     const bool col = i0 >= 0 && i0 < n0 && j >= 0 && j < n1;
     const int pp = tx + 1, jj = ty + 1;
@@ -109,16 +130,17 @@ __global__ void __launch_bounds__(BRICK_TPB) k_brick3(LevelDevT<WT> L, BrickGrid
     // brick for tz = 0, above it for tz = 1), one y-halo pair (2 rows x 32
     // pairs x 8 planes over the 512 threads) and, for t < 128, one x-halo
     // cell (cells lo0 - 1 and lo0 + 64, 8 rows x 8 planes each)
-    const int kzh = tz ? lo2 + BRICK_Z : lo2 - 1;
+    const int kzh = wz(tz ? lo2 + BRICK_Z : lo2 - 1);
     const bool zok = !FROM_ZERO && col && kzh >= 0 && kzh < n2;
     const int yside = t >> 8, yp = t & 31, yk = (t >> 5) & 7;
-    const int yj = yside ? lo1 + BRICK_Y : lo1 - 1;
-    const int yi0 = lo0 + 2 * yp;
-    const bool yok = !FROM_ZERO && yj >= 0 && yj < n1 && yi0 >= 0 && yi0 < n0 && lo2 + yk >= 0 && lo2 + yk < n2;
+    const int yj = wy(yside ? lo1 + BRICK_Y : lo1 - 1);
+    const int yi0 = wx(lo0 + 2 * yp);
+    const int ykg = wz(lo2 + yk);
+    const bool yok = !FROM_ZERO && yj >= 0 && yj < n1 && yi0 >= 0 && yi0 < n0 && ykg >= 0 && ykg < n2;
     const int xsd = (t >> 6) & 1, xr = (t >> 3) & 7, xk = t & 7;
-    const int xi = xsd ? lo0 + BRICK_X : lo0 - 1;
-    const bool xok = !FROM_ZERO && t < 128 && xi >= 0 && xi < n0 && lo1 + xr >= 0 && lo1 + xr < n1 &&
-                     lo2 + xk >= 0 && lo2 + xk < n2;
+    const int xi = wx(xsd ? lo0 + BRICK_X : lo0 - 1);
+    const int xjg = wy(lo1 + xr), xkg = wz(lo2 + xk);
+    const bool xok = !FROM_ZERO && t < 128 && xi >= 0 && xi < n0 && xjg >= 0 && xjg < n1 && xkg >= 0 && xkg < n2;
     Pair2<XT> zh, yh;
     zh.lo = zh.hi = yh.lo = yh.hi = to_store<XT>(0.0f);
     XT xh1 = to_store<XT>(0.0f);
@@ -126,10 +148,10 @@ __global__ void __launch_bounds__(BRICK_TPB) k_brick3(LevelDevT<WT> L, BrickGrid
         const int kc = min(max(kzh, 0), n2 - 1);
         zh = *reinterpret_cast<const Pair2<XT>*>(xin + (int64_t)i0c + (int64_t)n0 * (jc + (int64_t)n1 * kc));
         const int yjc = min(max(yj, 0), n1 - 1), yic = min(max(yi0, 0), n0 - 2);
-        const int ykc = min(max(lo2 + yk, 0), n2 - 1);
+        const int ykc = min(max(ykg, 0), n2 - 1);
         yh = *reinterpret_cast<const Pair2<XT>*>(xin + (int64_t)yic + (int64_t)n0 * (yjc + (int64_t)n1 * ykc));
-        const int xic = min(max(xi, 0), n0 - 1), xjc = min(max(lo1 + xr, 0), n1 - 1);
-        const int xkc = min(max(lo2 + xk, 0), n2 - 1);
+        const int xic = min(max(xi, 0), n0 - 1), xjc = min(max(xjg, 0), n1 - 1);
+        const int xkc = min(max(xkg, 0), n2 - 1);
         xh1 = xin[(int64_t)xic + (int64_t)n0 * (xjc + (int64_t)n1 * xkc)];
     }
 
