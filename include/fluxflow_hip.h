@@ -689,6 +689,12 @@ enum { FLUX_FLIP3_POSITIONS = 0, FLUX_FLIP3_VELOCITIES = 1, FLUX_FLIP3_FLUID_MAS
  * this solver.  Costs three face-field copies per step while on. */
 int flux_flip3_capture_projection_input(flux_flip3* s, int on);
 flux_field* flux_flip3_field(flux_flip3* s, int which);
+/* The projection solver the step uses (owned by the FLIP solver), so a caller
+ * can set its coarse-space correction (flux_pressure3_set_coarse_space) or
+ * flexible beta.  No counterpart in the reference, whose FLIP solver builds
+ * its pressure solver internally (grid_flip_solver3.js:425-432) and exposes
+ * none of these options. */
+flux_pressure3* flux_flip3_pressure(flux_flip3* s);
 int flux_flip3_destroy(flux_flip3* s);
 
 /* createGridSmokeSolver3 (grid_smoke_solver3.js:25-154 over createGridSolver3,

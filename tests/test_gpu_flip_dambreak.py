@@ -111,3 +111,35 @@ def test_captured_projection_system_reproduces_the_step(flux):
     ref, st = O.pcg_solve(shape, h, A["b"], mask=A["mask"], weights=A["weights"], mg=mg, tol=1e-10, maxiter=400)
     assert st["converged"]
     assert rel(want, ref) <= 1e-4
+
+
+def test_two_phase_dam_break_64_headline_projection_matches_plain(flux):
+    """The FLIP step with the benchmark's headline projection form (FP16
+    V-cycle, brick smoother with 12 sweeps per pass, free-cell coarse
+    correction set through flux_flip3_pressure) against the reference's
+    method (red-black 2/2, FP32 V-cycle) on the same 64^3 two-phase dam
+    break: every projection accepted at the FP64 1e-6 relative residual in
+    fewer iterations, the fluid mask, cell concentration and density equal
+    bit for bit after three steps (the projections differ only within their
+    tolerance), positions within 5e-5 of the unit domain and velocities within
+    1e-4 of their norm (as the oracle comparison above: at 1000:1 a 1e-6
+    residual leaves velocity differences of ~1e-5)."""
+    n = 64
+    p = scenes.flip_dam_break(n, seed=2)
+    runs = {}
+    for form in ("plain", "headline"):
+        s, _ = flip.two_phase_dam_break(n, levels=4, particles=p, **flip.PROJECTION_FORMS[form])
+        its = []
+        for step in range(3):
+            d = s.on_advance_time_step()
+            print(form, "step", step, d)
+            assert d["converged"] and not d["rejected"] and d["residual"] <= 1e-6 * d["bNorm"]
+            its.append(d["iterations"])
+        runs[form] = (s, its)
+    (sp, ip), (sh, ih) = runs["plain"], runs["headline"]
+    assert all(h < q for h, q in zip(ih, ip))
+    np.testing.assert_array_equal(sh.fluid_mask.to_array(), sp.fluid_mask.to_array())
+    np.testing.assert_array_equal(sh.cell_concentration.to_array(), sp.cell_concentration.to_array())
+    np.testing.assert_array_equal(sh.cell_density.to_array(), sp.cell_density.to_array())
+    assert np.max(np.abs(sh.positions.to_array() - sp.positions.to_array())) <= 5e-5
+    assert rel(sh.velocities.to_array(), sp.velocities.to_array().astype(np.float64)) <= 1e-4

@@ -865,6 +865,8 @@ flux_field* flux_flip3_field(flux_flip3* s, int which) {
     }
 }
 
+flux_pressure3* flux_flip3_pressure(flux_flip3* s) { return s ? s->pressure : nullptr; }
+
 int flux_flip3_capture_projection_input(flux_flip3* s, int on) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(s, "captureProjectionInput: NULL solver");

@@ -152,6 +152,18 @@ class GridFlipSolver3:
                 "active": self._view("dirichlet_active", U8, n).to_array(),
                 "target": self._view("dirichlet_target", F32, n).to_array()}
 
+    def set_coarse_space(self, free_indicator=True, fields=()):
+        """the additive coarse-space correction of the projection's
+        preconditioner (flux_flip3_pressure + flux_pressure3_set_coarse_space):
+        the free-cell indicator and up to three caller fields"""
+        fields = list(fields)
+        self._cs_keep = fields
+        _lib.flux_flip3_pressure.restype = C.c_void_p
+        ps = _lib.flux_flip3_pressure(C.c_void_p(self.h))
+        arr = (C.c_void_p * max(1, len(fields)))(*[f.h for f in fields]) if fields else None
+        _check(_lib.flux_pressure3_set_coarse_space(C.c_void_p(ps), int(bool(free_indicator)), len(fields), arr),
+               "coarseSpace")
+
     def on_advance_time_step(self):
         dg = PressureDiag()
         _check(_lib.flux_flip3_step(C.c_void_p(self.h), C.byref(dg)), "onAdvanceTimeStep")
@@ -171,14 +183,25 @@ def create_grid_flip_solver3(**kw):
     return GridFlipSolver3(**kw)
 
 
+# projection forms of the dam break: the reference's method (red-black 2/2,
+# FP32 V-cycle) and the benchmark's headline form (FP16 V-cycle, brick
+# smoother with 12 sweeps per pass, free-cell coarse correction)
+PROJECTION_FORMS = {
+    "plain": {"storage": "f32", "smoother": "rbgs", "nu": 2, "coarse_space": False},
+    "headline": {"storage": "f16", "smoother": "brick", "nu": 12, "coarse_space": True},
+}
+
+
 def two_phase_dam_break(n, dt=1.0 / 60.0, levels=None, flip_ratio=0.95, ratio=1000.0, storage="f32", seed=0,
-                        particles=None):
+                        particles=None, smoother="rbgs", nu=2, coarse_space=False):
     """createGridFlipSolver3 on the two-phase dam break (the liquid block
     x < 0.4, y < 0.6 of the unit domain, gas everywhere else, 2^3 particles
     per cell, density ratio `ratio`:1, reduced pressure, gravity (0, -9.8, 0),
     FLIP ratio 0.95, the fixed-point (deterministic) scatter weighted by
     mass, phase-aware resampling, the closed box pinned at its top-centre
-    cell).  Returns (solver, particles)."""
+    cell).  The projection's V-cycle takes `storage`, `smoother` and `nu`
+    sweeps (down, up and final); `coarse_space` adds the free-cell coarse
+    correction.  Returns (solver, particles)."""
     from . import scenes
     p = particles or scenes.flip_dam_break(n, seed=seed)
     levels = levels or max(2, int(np.log2(n)) - 2)
@@ -187,7 +210,11 @@ def two_phase_dam_break(n, dt=1.0 / 60.0, levels=None, flip_ratio=0.95, ratio=10
                         carry_concentration=True, ambient_density=1.0, component_density=float(ratio),
                         mass_weighted_transfer=True, pressure_pin="topCenter",
                         resample={"enabled": True, "phaseAware": True},
-                        pressure={"multigrid": {"levels": levels, "storage": storage}, "tolerance": 1e-6})
+                        pressure={"multigrid": {"levels": levels, "storage": storage, "smoother": smoother,
+                                                "nu_down": nu, "nu_up": nu, "n_final": nu},
+                                  "tolerance": 1e-6})
+    if coarse_space:
+        s.set_coarse_space(True)
     s.positions.from_array(p["positions"])
     s.concentration.from_array(p["concentration"])
     return s, p
@@ -206,7 +233,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--storage", default="f32")
+    ap.add_argument("--storage", default=None, help="V-cycle storage (overrides the form's)")
+    ap.add_argument("--form", default="plain", choices=sorted(PROJECTION_FORMS),
+                    help="projection form: plain (the reference's method) or headline (FP16 V-cycle, brick "
+                         "smoother 12 sweeps, free-cell coarse correction)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 512 MiB field three times (the known-size kernel that scales FETCH_SIZE / "
                          "WRITE_SIZE counter readings to bytes)")
@@ -221,17 +251,22 @@ def main(argv=None):
         sync()
         del src, dst
     t0 = time.perf_counter()
-    s, p = two_phase_dam_break(a.n, storage=a.storage)
+    form = dict(PROJECTION_FORMS[a.form])
+    if a.storage:
+        form["storage"] = a.storage
+    s, p = two_phase_dam_break(a.n, **form)
     print("flip: %d^3, %d particles (%d liquid), setup %.1f s" % (a.n, p["count"], p["liquid"],
                                                                    time.perf_counter() - t0), flush=True)
     conc0 = float(p["concentration"].astype(np.float64).sum())
     ok = True
+    times = []
     for k in range(a.steps):
         sync()
         t = time.perf_counter()
         d = s.on_advance_time_step()
         sync()
         ms = (time.perf_counter() - t) * 1e3
+        times.append(ms)
         vel = s.velocities.to_array()
         conc = s.concentration.to_array()
         vmax = float(np.abs(vel).max())
@@ -240,6 +275,9 @@ def main(argv=None):
         print("flip step %d: %.1f ms, %s, max|v| %.3f, conc sum %.1f (start %.1f), fluid cells %d" % (
             k, ms, d, vmax, float(conc.astype(np.float64).sum()), conc0,
             int((s.fluid_mask.to_array() > 0.5).sum())), flush=True)
+    if len(times) > 1:
+        print("flip: form %s, mean step time over steps 1..%d: %.1f ms" % (a.form, len(times) - 1,
+                                                                        float(np.mean(times[1:]))), flush=True)
     print("flip: %s" % ("ok" if ok else "FAILED"), flush=True)
     sys.exit(0 if ok else 1)
 
