@@ -695,6 +695,11 @@ flux_field* flux_flip3_field(flux_flip3* s, int which);
  * its pressure solver internally (grid_flip_solver3.js:425-432) and exposes
  * none of these options. */
 flux_pressure3* flux_flip3_pressure(flux_flip3* s);
+/* The step's dt from the next step on: the reference reads dt as a live node
+ * each step (createGridAdaptiveTimeStep3 writes the sub-step dt into it,
+ * grid_adaptive_timestep3.js:26-40); the projection's plausible-pressure
+ * bound keeps the value derived from the creation dt. */
+int flux_flip3_set_dt(flux_flip3* s, float dt);
 int flux_flip3_destroy(flux_flip3* s);
 
 /* createGridSmokeSolver3 (grid_smoke_solver3.js:25-154 over createGridSolver3,

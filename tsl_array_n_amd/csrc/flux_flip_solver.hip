@@ -867,6 +867,14 @@ flux_field* flux_flip3_field(flux_flip3* s, int which) {
 
 flux_pressure3* flux_flip3_pressure(flux_flip3* s) { return s ? s->pressure : nullptr; }
 
+int flux_flip3_set_dt(flux_flip3* s, float dt) {
+    FLUX_REQUIRE_INIT();
+    FLUX_REQUIRE(s, "dt: NULL solver");
+    FLUX_REQUIRE(dt > 0.0f && std::isfinite(dt), "dt must be positive and finite, got %g", (double)dt);
+    s->d.dt = dt;
+    return FLUX_OK;
+}
+
 int flux_flip3_capture_projection_input(flux_flip3* s, int on) {
     FLUX_REQUIRE_INIT();
     FLUX_REQUIRE(s, "captureProjectionInput: NULL solver");

@@ -164,6 +164,10 @@ class GridFlipSolver3:
         _check(_lib.flux_pressure3_set_coarse_space(C.c_void_p(ps), int(bool(free_indicator)), len(fields), arr),
                "coarseSpace")
 
+    def set_dt(self, dt):
+        """the step's dt from the next step on (the reference's live dt node)"""
+        _check(_lib.flux_flip3_set_dt(C.c_void_p(self.h), C.c_float(dt)), "dt")
+
     def on_advance_time_step(self):
         dg = PressureDiag()
         _check(_lib.flux_flip3_step(C.c_void_p(self.h), C.byref(dg)), "onAdvanceTimeStep")
@@ -181,6 +185,33 @@ class GridFlipSolver3:
 
 def create_grid_flip_solver3(**kw):
     return GridFlipSolver3(**kw)
+
+
+class GridAdaptiveTimeStep3:
+    """createGridAdaptiveTimeStep3({velocityGrid, gridSpacing, targetDt,
+    courantNumber, maxSubSteps}) (grid_adaptive_timestep3.js:15-44): update()
+    reads max |u| over the three face arrays on the GPU, computes the
+    sub-step count and dt (computeAdaptiveSubSteps, time/cfl.js:38-48) and
+    writes the sub-step dt into the solver (the reference's live dt node);
+    returns (sub-step count, sub-step dt as f32)."""
+
+    def __init__(self, solver, target_dt, courant_number=5.0, max_sub_steps=32):
+        self.solver = solver
+        self.min_spacing = float(min(solver.spacing))
+        self.target_dt = float(target_dt)
+        self.courant_number = float(courant_number)
+        self.max_sub_steps = int(max_sub_steps)
+        self.state = {"lastNumSubSteps": 1, "lastSubDt": self.target_dt}
+
+    def update(self):
+        from .grid2 import compute_adaptive_substeps, max_abs
+        m = max_abs(list(self.solver.velocity_grid))
+        k, sub = compute_adaptive_substeps(m, self.min_spacing, self.target_dt, self.courant_number,
+                                           self.max_sub_steps)
+        sub32 = float(np.float32(sub))
+        self.solver.set_dt(sub32)
+        self.state = {"lastNumSubSteps": k, "lastSubDt": sub32, "maxVelocity": m}
+        return k, sub32
 
 
 # projection forms of the dam break: the reference's method (red-black 2/2,
@@ -237,6 +268,10 @@ def main(argv=None):
     ap.add_argument("--form", default="plain", choices=sorted(PROJECTION_FORMS),
                     help="projection form: plain (the reference's method) or headline (FP16 V-cycle, brick "
                          "smoother 12 sweeps, free-cell coarse correction)")
+    ap.add_argument("--frames", type=int, default=0,
+                    help="adaptive mode: this many frames of 1/60 s, each split into sub-steps by the CFL condition "
+                         "(createGridAdaptiveTimeStep3) instead of --steps fixed steps of 1/60 s")
+    ap.add_argument("--courant", type=float, default=1.0, help="Courant number of the adaptive mode")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 512 MiB field three times (the known-size kernel that scales FETCH_SIZE / "
                          "WRITE_SIZE counter readings to bytes)")
@@ -260,21 +295,46 @@ def main(argv=None):
     conc0 = float(p["concentration"].astype(np.float64).sum())
     ok = True
     times = []
-    for k in range(a.steps):
+
+    def check(label, d, ms):
+        vel = s.velocities.to_array()
+        conc = s.concentration.to_array()
+        vmax = float(np.abs(vel).max())
+        csum = float(conc.astype(np.float64).sum())
+        good = (d["converged"] and not d["rejected"] and np.isfinite(vmax) and vmax < 50.0 and csum == conc0 and
+                np.all(np.isfinite(s.positions.to_array())))
+        print("flip %s: %.1f ms, %s, max|v| %.3f, conc sum %.1f (start %.1f), fluid cells %d" % (
+            label, ms, d, vmax, csum, conc0, int((s.fluid_mask.to_array() > 0.5).sum())), flush=True)
+        return good
+
+    if a.frames > 0:
+        ats = GridAdaptiveTimeStep3(s, 1.0 / 60.0, courant_number=a.courant)
+        steps = 0
+        t_all = time.perf_counter()
+        for f in range(a.frames):
+            nsub, sub = ats.update()
+            for q in range(nsub):
+                sync()
+                t = time.perf_counter()
+                d = s.on_advance_time_step()
+                sync()
+                ms = (time.perf_counter() - t) * 1e3
+                times.append(ms)
+                steps += 1
+                if q == nsub - 1:
+                    ok = check("frame %d (%d sub-steps of %.3g s, max|u| %.3f)" % (
+                        f, nsub, sub, ats.state["maxVelocity"]), d, ms) and ok
+                else:
+                    ok = ok and d["converged"] and not d["rejected"]
+        print("flip: %d frames in %d steps, %.1f s" % (a.frames, steps, time.perf_counter() - t_all), flush=True)
+    for k in range(a.steps if a.frames <= 0 else 0):
         sync()
         t = time.perf_counter()
         d = s.on_advance_time_step()
         sync()
         ms = (time.perf_counter() - t) * 1e3
         times.append(ms)
-        vel = s.velocities.to_array()
-        conc = s.concentration.to_array()
-        vmax = float(np.abs(vel).max())
-        good = d["converged"] and not d["rejected"] and np.isfinite(vmax) and vmax < 50.0
-        ok = ok and good
-        print("flip step %d: %.1f ms, %s, max|v| %.3f, conc sum %.1f (start %.1f), fluid cells %d" % (
-            k, ms, d, vmax, float(conc.astype(np.float64).sum()), conc0,
-            int((s.fluid_mask.to_array() > 0.5).sum())), flush=True)
+        ok = check("step %d" % k, d, ms) and ok
     if len(times) > 1:
         print("flip: form %s, mean step time over steps 1..%d: %.1f ms" % (a.form, len(times) - 1,
                                                                         float(np.mean(times[1:]))), flush=True)
