@@ -239,3 +239,36 @@ def test_brick_wrapped_shifted_pass_is_bit_identical(flux, storage, monkeypatch)
         res.append((ps.pressure.to_array(), d))
     assert res[0][1]["iterations"] == res[1][1]["iterations"]
     assert np.array_equal(res[0][0], res[1][0])
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-6), ("f16", 1e-3)])
+def test_brick_zfolded_restriction_matches_stored_residual(flux, storage, tol, monkeypatch):
+    """The brick levels' residual folded in z into the restriction (r never
+    stored; the restriction sums the z-taps first, then the x-y taps) against
+    the stored residual and k_restrict3 (x-y taps first): the same sums in
+    another order, so a 96^3 V-cycle agrees to rounding (FP32: measured
+    ~1e-7; FP16 storage rounds the coarse right-hand sides, so to its
+    precision), and projections take the same iterations within 1."""
+    F = flux
+    sc, sh, h, A, m, v, w = _dam_break(F, 96)
+    r = np.random.default_rng(29).standard_normal(int(np.prod(sh))).astype(np.float32)
+    rd = upload(F, "float", sh, r)
+    mg = F.create_multigrid_preconditioner(sh, h, dirichlet_mask=m, coarse_dirichlet_mask=v, face_weights=w,
+                                           levels=4, nu_down=3, nu_up=3, n_final=3, smoother="brick",
+                                           storage=storage)
+    outs = []
+    for zf in ("0", "1"):
+        monkeypatch.setenv("FLUX_ZFOLD", zf)
+        z = F.array_n("float", sh)
+        mg(rd, z)
+        outs.append(z.to_array())
+    err = rel(outs[1], outs[0].astype(np.float64))
+    print(storage, "folded vs stored", err)
+    assert err <= tol
+    its = []
+    for zf in ("0", "1"):
+        monkeypatch.setenv("FLUX_ZFOLD", zf)
+        ps, d = _project(F, sc, storage)
+        assert d["converged"] and d["residual"] <= 1e-6 * d["bNorm"]
+        its.append(d["iterations"])
+    assert abs(its[0] - its[1]) <= 1

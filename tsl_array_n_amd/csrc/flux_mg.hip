@@ -691,6 +691,123 @@ __global__ void __launch_bounds__(MTPB) k_residual3(LevelDevT<WT> L, March M, co
     }
 }
 
+// The residual r = b - A x of a 3-D level (k_residual3's per-cell arithmetic)
+// folded at once into the z-taps of the full-weighting restriction:
+//   rz[K][j][i] = sum_c tz[c] r(i, j, fk_c(K)),  fk = 2K-1, 2K, 2K+1, 2K+2
+// (clamped to 2K and 2K+1 at the ends, tz = 1/8, 3/8, 3/8, 1/8, summed in
+// tap order), so r itself is never stored and the restriction that follows
+// (k_restrict_xy) reads half as many values and sums only the x-y taps.  A
+// thread marches its column over an even z-chunk [k0, k1) of fine planes
+// and also evaluates the residual at the planes k0 - 1 and k1 its first
+// and last coarse planes reach.  Dirichlet cells contribute 0, as their
+// stored residual does in the unfolded form.
+template <typename XT, typename WT>
+__global__ void __launch_bounds__(MTPB) k_residual3z(LevelDevT<WT> L, March M, const XT* __restrict__ x,
+                                                     const XT* __restrict__ b, float* __restrict__ rz,
+                                                     const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t, k0, k1;
+    if (!march_begin(M, t, k0, k1)) return;
+    const int n0 = L.n0, n1 = L.n1, n2 = L.n2;
+    const int plane = n0 * n1;
+    const int j = t / n0;
+    const int i = t - j * n0;
+    const uint8_t* __restrict__ nbits = L.nb;
+    const float ih2[3] = {L.ih2[0], L.ih2[1], L.ih2[2]};
+    const float tz[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    const int K0 = k0 >> 1, K1 = k1 >> 1;  // coarse planes of this chunk
+    const int kb = k0 > 0 ? k0 - 1 : 0;
+    const int ke = k1 < n2 ? k1 : n2 - 1;  // inclusive
+    float xnext = to_f32(x[i + n0 * (j + n1 * kb)]);
+    float xprev = 0.0f;
+    // z-sums in progress: K = m - 1 (awaiting its last tap), K = m, K = m + 1
+    float acc_prev = 0.0f, acc_cur = 0.0f, acc_next = 0.0f;
+    for (int k = kb; k <= ke; ++k) {
+        const int c = i + n0 * (j + n1 * k);
+        const float xc = xnext;
+        if (k + 1 < n2) xnext = to_f32(x[c + plane]);
+        const unsigned bits = nbits ? nbits[c] : 0u;
+        float w[6];
+        face_w3(L, i, j, k, c, w);
+        float xn[6];
+        xn[0] = to_f32(x[i > 0 ? c - 1 : c]);
+        xn[1] = to_f32(x[i < n0 - 1 ? c + 1 : c]);
+        xn[2] = to_f32(x[j > 0 ? c - n0 : c]);
+        xn[3] = to_f32(x[j < n1 - 1 ? c + n0 : c]);
+        xn[4] = k > kb ? xprev : to_f32(x[k > 0 ? c - plane : c]);
+        xn[5] = k + 1 < n2 ? xnext : xc;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) xn[a] = ((bits >> a) & 1u) ? 0.0f : xn[a];
+        float sum, dsum;
+        row_terms<float>(xc, xn, w, ih2, sum, dsum);
+        const bool dir = (bits & 64u) != 0;
+        const float r = dir ? 0.0f : to_f32(b[c]) - sum;
+        const int m = k >> 1;
+        if ((k & 1) == 0) {
+            // k = 2m: the last tap of K = m - 1, the second of K = m (for
+            // K = 0 also its first: 2K - 1 clamps to 2K)
+            if (m - 1 >= K0 && m - 1 < K1) {
+                acc_prev = acc_prev + r * tz[3];
+                rz[i + n0 * (j + n1 * (m - 1))] = acc_prev;
+            }
+            if (m >= K0 && m < K1) {
+                if (m == 0) acc_cur = r * tz[0];
+                acc_cur = acc_cur + r * tz[1];
+            }
+        } else {
+            // k = 2m + 1: the third tap of K = m (for the last K also its
+            // fourth: 2K + 2 clamps to 2K + 1), the first of K = m + 1
+            if (m >= K0 && m < K1) {
+                acc_cur = acc_cur + r * tz[2];
+                if (k == n2 - 1) {
+                    acc_cur = acc_cur + r * tz[3];
+                    rz[i + n0 * (j + n1 * m)] = acc_cur;
+                }
+            }
+            if (m + 1 >= K0 && m + 1 < K1) acc_next = r * tz[0];
+            acc_prev = acc_cur;
+            acc_cur = acc_next;
+        }
+        xprev = xc;
+    }
+}
+
+// The x-y taps of the full-weighting restriction over the z-folded residual
+// of k_residual3z (same taps and clamps as k_restrict3 per axis); masked
+// coarse cells get 0; the residual scale of the finest level when rr is set.
+template <typename OT>
+__global__ void __launch_bounds__(MTPB) k_restrict_xy(March M, int f0, int f1, int f2, const float* __restrict__ rz,
+                                                      int c0, int c1, OT* __restrict__ bc,
+                                                      const uint8_t* __restrict__ mc, const double* __restrict__ rr,
+                                                      const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t, K0, K1;
+    if (!march_begin(M, t, K0, K1)) return;
+    const int J = t / c0;
+    const int I = t - J * c0;
+    const float tw[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+    int ti[4], tj[4];
+    ti[0] = I > 0 ? 2 * I - 1 : 2 * I;
+    ti[1] = 2 * I;
+    ti[2] = 2 * I + 1;
+    ti[3] = (I + 1 < c0) ? 2 * I + 2 : 2 * I + 1;
+    tj[0] = J > 0 ? 2 * J - 1 : 2 * J;
+    tj[1] = 2 * J;
+    tj[2] = 2 * J + 1;
+    tj[3] = (J + 1 < c1) ? 2 * J + 2 : 2 * J + 1;
+    const float scale = residual_scale(rr, f0 * f1 * f2);
+    for (int K = K0; K < K1; ++K) {
+        const float* __restrict__ p = rz + (int64_t)f0 * f1 * K;
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int bq = 0; bq < 4; ++bq) sum += p[ti[a] + f0 * tj[bq]] * (tw[a] * tw[bq]);
+        const int c = I + c0 * (J + c1 * K);
+        bc[c] = to_store<OT>((mc && mc[c]) ? 0.0f : sum * scale);
+    }
+}
+
 // Full-weighting restriction (restrictionTapsForAxis 766-780 + buildRestrictKernel
 // 866-914): per axis taps 1/8,3/8,3/8,1/8 at 2c-1 (clamped to 2c), 2c, 2c+1,
 // 2c+2 (clamped to 2c+1); masked fine cells contribute 0; masked coarse cells
@@ -1822,14 +1939,37 @@ int vcycle_brick(flux_multigrid* mg, int l, const int* stop, const double* rr, D
     } else {
         fill_t<XT>(alt, cells_of(v.n), 0.0f);
     }
-    if (FINE) kprof_begin("residual");
-    const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
-    k_residual3<FINE, XT, XT, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), FINE ? flip_march(M) : M, alt,
-                                                                (const XT*)v.b, buf, stop);
-    if (FINE) kprof_end("residual");
-    if (FINE) kprof_begin("restrict");
-    restrict_level<FINE, XT, ST>(mg, v, c, buf, scale, stop);
-    if (FINE) kprof_end("restrict");
+    // the residual folded in z into the restriction (FLUX_ZFOLD=0: stored
+    // residual, then k_restrict3; read at every call so one process can
+    // compare the two); the folded sums fill half of buf's bytes
+    const char* zenv = getenv("FLUX_ZFOLD");
+    const bool zfold = !(zenv && zenv[0] == '0') && (v.n[2] & 1) == 0 && c.n[2] * 2 == v.n[2];
+    if (zfold) {
+        if (FINE) kprof_begin("residual");
+        March Mz = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+        if (Mz.kz & 1) {  // even z-chunks: a chunk holds whole coarse planes
+            Mz.kz += 1;
+            Mz.nb = Mz.bxy * (unsigned)((v.n[2] + Mz.kz - 1) / Mz.kz);
+        }
+        float* rz = reinterpret_cast<float*>(buf);
+        k_residual3z<XT, ST><<<Mz.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), FINE ? flip_march(Mz) : Mz, alt,
+                                                            (const XT*)v.b, rz, stop);
+        if (FINE) kprof_end("residual");
+        if (FINE) kprof_begin("restrict");
+        const March Mc = march_shape(c.n[0], c.n[1], c.n[2], MTPB, g_march_blocks);
+        k_restrict_xy<ST><<<Mc.nb, MTPB, 0, g_stream>>>(FINE ? flip_march(Mc) : Mc, v.n[0], v.n[1], v.n[2], rz,
+                                                         c.n[0], c.n[1], (ST*)c.b, c.mask, scale, stop);
+        if (FINE) kprof_end("restrict");
+    } else {
+        if (FINE) kprof_begin("residual");
+        const March M = march_shape(v.n[0], v.n[1], v.n[2], MTPB, g_march_blocks);
+        k_residual3<FINE, XT, XT, ST><<<M.nb, MTPB, 0, g_stream>>>(level_dev<ST>(mg, v), FINE ? flip_march(M) : M,
+                                                                    alt, (const XT*)v.b, buf, stop);
+        if (FINE) kprof_end("residual");
+        if (FINE) kprof_begin("restrict");
+        restrict_level<FINE, XT, ST>(mg, v, c, buf, scale, stop);
+        if (FINE) kprof_end("restrict");
+    }
     FLUX_TRY(vcycle_t<ST>(mg, l + 1, stop, rr, nullptr));
     // the correction adds into alt (correct_level works on v.x)
     void* xsave = v.x;
