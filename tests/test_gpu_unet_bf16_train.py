@@ -1,0 +1,105 @@
+"""BF16 training of the learned guess on the matrix cores (GPU tests).
+
+A BF16 network (flux_unet_set_precision) trains with the forward and both
+backward convolutions on MFMA: the data gradient is the implicit-GEMM
+convolution of the ReLU-masked output gradient (channels-last BF16) with the
+flipped, transposed kernel, the weight gradient an MFMA GEMM over the voxels
+(k_wgrad_mfma), FP32 accumulation, FP32 master weights, gradients and Adam.
+
+Bars (BF16 has an 8-bit mantissa; the forward alone sits at 3.4e-3 .. 3.7e-3
+of the float64 forward):
+* every parameter gradient against the float64 oracle's backward
+  (unet_ref.backward) at relative L2 <= 5e-2, on grids whose coarse levels
+  are ragged (x-rows shorter than the 8 voxels a lane holds).  The error
+  grows along the backward chain as each layer's BF16 roundings enter it
+  (measured at 24x16x8: out.w 3.1e-3, dec1a.w 1.1e-2, mid.w 2.5e-2, enc2a.w
+  3.5e-2, enc0a.w 4.7e-2; at 32x32x16 at most 2.1e-2) and shrinks as the
+  levels hold more voxels to average over (at 16x8x8, whose deepest level
+  holds 2 voxels, enc0a.w reaches 7.7e-2, so that grid is not used);
+* the MSE loss within 2e-2 of the oracle's;
+* training reduces the loss as the FP32 network does.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import unet_ref as U
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = U.init_params(seed)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.1
+    return p
+
+
+@pytest.mark.parametrize("n", [(24, 16, 8), (32, 32, 16)])
+def test_bf16_gradients_match_oracle(flux, n):
+    from tsl_array_n_amd import learned as L
+    rng = np.random.default_rng(7)
+    p = random_params(3)
+    x = rng.standard_normal((3, n[2], n[1], n[0]))
+    t = rng.standard_normal((1, n[2], n[1], n[0]))
+    net = L.UNet(n, training=True, params={k: v.astype(np.float32) for k, v in p.items()}, precision="bf16")
+    target = flux.array_n("float", (int(np.prod(n)),), t.astype(np.float32).ravel())
+    loss = net.train_step(target, x=x.astype(np.float32).ravel(), lr=0.0)
+    yr, cache = U.forward(p, x)
+    lr_, gy = U.mse_loss(yr, t)
+    g, _ = U.backward(p, cache, gy)
+    assert abs(loss - lr_) <= 2e-2 * lr_
+    grads = L.unflatten(net.get(which=1))
+    errs = {name: rel(grads[name], g[name]) for name in g}
+    print(n, "loss", loss, "oracle", lr_, " ".join("%s %.1e" % kv for kv in errs.items()))
+    bad = {k: v for k, v in errs.items() if not v <= 5e-2}
+    assert not bad, bad
+
+
+def test_bf16_training_reduces_loss(flux):
+    from tsl_array_n_amd import learned as L
+    n = (16, 16, 16)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((3, 16, 16, 16)).astype(np.float32)
+    t = np.tanh(x[0] + 0.5 * x[1]).astype(np.float32)
+    net = L.UNet(n, training=True, precision="bf16")
+    target = flux.array_n("float", (4096,), t.ravel())
+    first = net.train_step(target, x=x.ravel(), lr=1e-3)
+    for _ in range(60):
+        last = net.train_step(target, lr=1e-3)
+    print("bf16 training loss", first, "->", last)
+    assert last < 0.5 * first
+
+
+@pytest.mark.parametrize("size", [64])
+def test_bf16_training_step_time(flux, size):
+    """Time of one training step (forward, MSE, backward, Adam) of the FP32
+    and the BF16 network at size^3 (printed; the BF16 step must not be the
+    slower one)."""
+    from tsl_array_n_amd import learned as L
+    n = (size, size, size)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(3 * size ** 3).astype(np.float32)
+    target = flux.array_n("float", (size ** 3,), rng.standard_normal(size ** 3).astype(np.float32))
+    times = {}
+    for prec in ("f32", "bf16"):
+        net = L.UNet(n, training=True, precision=prec)
+        net.train_step(target, x=x, lr=1e-4)
+        flux.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            net.train_step(target, lr=1e-4)
+        flux.sync()
+        times[prec] = (time.perf_counter() - t0) / 3 * 1e3
+        del net
+    print("training step at %d^3: FP32 %.1f ms, BF16 MFMA %.1f ms" % (size, times["f32"], times["bf16"]))
+    assert times["bf16"] < times["f32"]

@@ -705,6 +705,149 @@ __global__ void k_energy_grad(const float* __restrict__ r, const float* __restri
     if (q < n) gy[q] = (mask && mask[q]) ? 0.0f : c1 * r[q] + c2 * ag[q];
 }
 
+// ------------------------------------------------ BF16 training on MFMA
+// The backward pass of a BF16 network (flux_unet_set_precision): the
+// output gradient of a layer, masked by its ReLU, goes to channels-last
+// BF16 once (k_grad_to_cl); the data gradient is the implicit-GEMM
+// convolution of that tensor with the layer's flipped, transposed kernel on
+// the matrix cores (k_conv3_mfma with its own packed weights), and the
+// weight gradient an MFMA GEMM over the voxels (k_wgrad_mfma).  Master
+// weights, their gradients and Adam stay FP32.
+
+// gy (FP32 [C][voxel]) masked by y > 0 (y channels-last BF16, or none) ->
+// g16 (channels-last BF16, cpad >= c channels, the rest 0)
+__global__ void k_grad_to_cl(const float* __restrict__ gy, const uint16_t* __restrict__ y16, int c, int cpad,
+                             int64_t nv, uint16_t* __restrict__ g16) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * cpad) return;
+    const int ch = (int)(t % cpad);
+    const int64_t v = t / cpad;
+    float g = 0.0f;
+    if (ch < c) {
+        g = gy[(int64_t)ch * nv + v];
+        // y > 0: a BF16 value with the sign bit clear and not zero
+        if (y16) {
+            const uint32_t h = y16[v * c + ch];
+            if ((h & 0x8000u) || (h & 0x7FFFu) == 0u) g = 0.0f;
+        }
+    }
+    g16[t] = f32_to_bf16(g);
+}
+
+// channels-last FP32 [voxel][c] -> [c][voxel]
+__global__ void k_cl_to_nc(const float* __restrict__ src, float* __restrict__ dst, int c, int64_t nv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * c) return;
+    const int64_t v = t / c;
+    const int ch = (int)(t % c);
+    dst[(int64_t)ch * nv + v] = src[t];
+}
+
+// Weight gradient dW[co][ci][tap] = sum_v g[v][co] x[v + off(tap)][ci] as
+// MFMA GEMMs: M = output channels (rows co0 .. co0 + 15), N = input
+// channels (columns ci0 .. ci0 + 15), K = voxels in steps of 32.  A
+// wavefront owns one tap, one (co, ci) tile and a run of `kv` voxels (whole
+// K steps).  Lane l (r = l % 16, q = l / 16) holds A[r][8q + j] =
+// g[v0 + 8q + j][co0 + r] and B[8q + j][r] = x[v0 + 8q + j + off][ci0 + r],
+// each of its 8 voxels with its own coordinates (any grid shape); zero
+// padding outside the grid, upsampling and concat as in k_conv3_mfma.  Each
+// wavefront stores its FP32 partial sums; k_wgrad_finish adds the runs in a
+// fixed order.
+__global__ void __launch_bounds__(256) k_wgrad_mfma(const uint16_t* __restrict__ g, int cpad, MfmaIn in, int co_tiles,
+                                                    int ci_tiles, int64_t kv, int runs, int nz, int ny, int nx,
+                                                    float* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t tiles = (int64_t)co_tiles * ci_tiles;
+    if (wave >= tiles * 27 * runs) return;  // wavefront-uniform
+    const int tile = (int)(wave % tiles);
+    const int tp = (int)((wave / tiles) % 27);
+    const int run = (int)(wave / (tiles * 27));
+    const int co0 = (tile / ci_tiles) * 16, ci0 = (tile % ci_tiles) * 16;
+    const int r = lane & 15, q = lane >> 4;
+    const int64_t nv = (int64_t)nz * ny * nx;
+    const int64_t vbeg = (int64_t)run * kv, vend = min(nv, vbeg + kv);
+    const int cin = in.c1 + in.c2;
+    const int hy = ny >> 1, hx = nx >> 1;
+    const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+    const int co = co0 + r, ci = ci0 + r;
+    const bool co_ok = co < cpad, ci_ok = ci < cin;
+    const bool from_a = ci < in.c1;
+    const bool up = in.up && from_a;
+    const uint16_t* base = from_a ? in.a : in.b;
+    const int64_t cs = from_a ? in.c1 : in.c2;
+    const int coff = from_a ? ci : ci - in.c1;
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int64_t v0 = vbeg; v0 < vend; v0 += 32) {
+        // this lane's 8 voxels vb .. vb + 7, walked with their coordinates
+        const int64_t vb = v0 + 8 * q;
+        const int64_t row = vb / nx;
+        int xi = (int)(vb - row * nx), yi = (int)(row % ny), zi = (int)(row / ny);
+        uint16_t av[8], bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool vin = vb + j < vend;
+            av[j] = (vin && co_ok) ? g[(vb + j) * cpad + co] : (uint16_t)0;
+            const int xx = xi + dx - 1, yy = yi + dy - 1, zz = zi + dz - 1;
+            uint16_t val = 0;
+            if (vin && ci_ok && xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                const int64_t qv = up ? ((int64_t)(zz >> 1) * hy + (yy >> 1)) * hx + (xx >> 1)
+                                      : ((int64_t)zz * ny + yy) * nx + xx;
+                val = base[qv * cs + coff];
+            }
+            bv[j] = val;
+            if (++xi == nx) {
+                xi = 0;
+                if (++yi == ny) {
+                    yi = 0;
+                    ++zi;
+                }
+            }
+        }
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            a[j] = __builtin_bit_cast(__bf16, av[j]);
+            b[j] = __builtin_bit_cast(__bf16, bv[j]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D[4q + i][r]: row co0 + 4q + i, column ci0 + r
+    const int np = ci_tiles * 16, mp = co_tiles * 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[(((int64_t)run * 27 + tp) * mp + co0 + 4 * q + i) * np + ci0 + r] = acc[i];
+}
+
+// dW[co][ci][tap] (co < cout, ci < cin) = the runs' partial sums added in run
+// order; db[co] = sum_v g[v][co] in a fixed order (one block per channel)
+__global__ void k_wgrad_finish(const float* __restrict__ part, int runs, int co_tiles, int ci_tiles, int cout,
+                               int cin, float* __restrict__ dw) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)cout * cin * 27) return;
+    const int tp = (int)(t % 27);
+    const int ci = (int)((t / 27) % cin);
+    const int co = (int)(t / (27 * (int64_t)cin));
+    const int np = ci_tiles * 16, mp = co_tiles * 16;
+    float s = 0.0f;
+    for (int k = 0; k < runs; ++k) s += part[(((int64_t)k * 27 + tp) * mp + co) * np + ci];
+    dw[t] = s;
+}
+
+__global__ void __launch_bounds__(UTPB) k_bias_grad(const uint16_t* __restrict__ g, int cpad, int64_t nv,
+                                                    float* __restrict__ db) {
+    const int co = blockIdx.x;
+    float s = 0.0f;
+    for (int64_t v = threadIdx.x; v < nv; v += blockDim.x) s += bf16_to_f32(g[v * cpad + co]);
+    __shared__ float red[UTPB];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = UTPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = red[0];
+}
+
 struct flux_unet {
     int n[3];  // nx, ny, nz of the finest level
     int64_t nparams = 0;
@@ -736,6 +879,16 @@ struct flux_unet {
     int split = 1;                 // split K on small levels (FLUX_UNET_SPLIT=0 disables)
     int tile_mt = 0, tile_nt = 0;  // forced flat-kernel tile (FLUX_UNET_TILE=MxN), 0 = chosen per layer
     float* split_part = nullptr;   // FP32 partial sums of the K splits, sized for the layers that split
+    // BF16 training (the backward pass on MFMA)
+    uint16_t* wqt = nullptr;       // packed flipped, transposed weights of every layer (data gradients)
+    int64_t off_wqt[kNumLayers];
+    int groups_t[kNumLayers], npad_t[kNumLayers];
+    int64_t packed_t_version = 0;
+    uint16_t* g16 = nullptr;       // a layer's masked output gradient, channels-last BF16
+    float* dx_cl = nullptr;        // a layer's data gradient, channels-last FP32
+    float* wpart = nullptr;        // weight-gradient partial sums
+    int64_t wpart_floats = 0;
+    float* zero_bias = nullptr;    // 128 zeros (the data gradient has no bias)
 };
 
 namespace {
@@ -1001,6 +1154,118 @@ int forward_bf16(flux_unet* u, const float* x) {
 
 int infer(flux_unet* u, const float* x) { return u->precision == FLUX_UNET_BF16 ? forward_bf16(u, x) : forward(u, x); }
 
+// ---- BF16 training: the backward pass on MFMA
+int cpad_of(int l) { return (kLayers[l].cout + 7) / 8 * 8; }
+
+// How a layer's weight gradient splits the voxels into runs: about 8192
+// wavefronts (27 per tile and run), whole K steps per run, partial sums
+// within a 64 MiB budget
+struct WgradPlan {
+    int co_tiles, ci_tiles, runs;
+    int64_t kv, nv;
+};
+WgradPlan plan_wgrad(const flux_unet* u, int l) {
+    const LayerSpec& s = kLayers[l];
+    WgradPlan p;
+    p.nv = u->nvox[s.level];
+    p.co_tiles = (s.cout + 15) / 16;
+    p.ci_tiles = (mfma_channels(l) + 15) / 16;
+    const int64_t tiles = (int64_t)p.co_tiles * p.ci_tiles;
+    const int64_t per_run = (int64_t)27 * p.co_tiles * 16 * p.ci_tiles * 16;
+    const int64_t steps = (p.nv + 31) / 32;
+    int64_t runs = std::max<int64_t>(1, 8192 / (27 * tiles));
+    runs = std::min<int64_t>(runs, std::max<int64_t>(1, ((int64_t)16 << 20) / per_run));
+    runs = std::min<int64_t>(runs, steps);
+    p.kv = (steps + runs - 1) / runs * 32;
+    p.runs = (int)((p.nv + p.kv - 1) / p.kv);
+    return p;
+}
+
+int bf16_train_setup(flux_unet* u) {
+    FLUX_TRY(bf16_setup(u));
+    if (u->wqt) return FLUX_OK;
+    FLUX_REQUIRE(u->grads, "unet: BF16 training needs a training network");
+    int64_t off = 0, g16_n = 0, dx_n = 0, wp_n = 0;
+    for (int l = 0; l < kNumLayers; ++l) {
+        const LayerSpec& s = kLayers[l];
+        const int64_t nv = u->nvox[s.level];
+        u->groups_t[l] = (27 * (cpad_of(l) / 8) + 3) / 4 * 4;
+        u->npad_t[l] = (s.cin + 15) / 16 * 16;
+        u->off_wqt[l] = off;
+        off += (int64_t)u->groups_t[l] * u->npad_t[l] * 8;
+        g16_n = std::max<int64_t>(g16_n, (int64_t)cpad_of(l) * nv);
+        if (l != L_E0A) dx_n = std::max<int64_t>(dx_n, (int64_t)s.cin * nv);
+        const WgradPlan p = plan_wgrad(u, l);
+        wp_n = std::max<int64_t>(wp_n, (int64_t)p.runs * 27 * p.co_tiles * 16 * p.ci_tiles * 16);
+    }
+    u->wqt = (uint16_t*)dmalloc(off * 2);
+    u->g16 = (uint16_t*)dmalloc(g16_n * 2);
+    u->dx_cl = (float*)dmalloc(dx_n * 4);
+    u->wpart = (float*)dmalloc(wp_n * 4);
+    u->wpart_floats = wp_n;
+    u->zero_bias = (float*)dmalloc(128 * 4);
+    if (!u->wqt || !u->g16 || !u->dx_cl || !u->wpart || !u->zero_bias)
+        return set_error(FLUX_ERR_NOMEM, "unet: BF16 training buffers: allocation failed");
+    FLUX_HIP(hipMemsetAsync(u->zero_bias, 0, 128 * 4, g_stream));
+    u->packed_t_version = 0;
+    return FLUX_OK;
+}
+
+// the flipped, transposed kernels of every layer packed for the data
+// gradients (after each change of the master weights)
+void pack_transposed(flux_unet* u) {
+    if (u->packed_t_version == u->version) return;
+    for (int l = 0; l < kNumLayers; ++l) {
+        if (l == L_E0A) continue;
+        const LayerSpec& s = kLayers[l];
+        const int cnt = s.cout * s.cin * 27;
+        k_flip_transpose<<<blocks_for(cnt, UTPB), UTPB, 0, g_stream>>>(u->params + u->off_w[l], u->wt, s.cout, s.cin);
+        const int64_t total = (int64_t)u->groups_t[l] * u->npad_t[l] * 8;
+        k_pack_weights<<<blocks_for(total, UTPB), UTPB, 0, g_stream>>>(u->wt, u->wqt + u->off_wqt[l], s.cin, s.cout,
+                                                                       cpad_of(l) / 8, u->groups_t[l], u->npad_t[l]);
+    }
+    u->packed_t_version = u->version;
+}
+
+// layer l's masked output gradient into g16 (channels-last BF16)
+void grad_to_cl(flux_unet* u, int l) {
+    const LayerSpec& s = kLayers[l];
+    const int64_t nv = u->nvox[s.level];
+    const int cp = cpad_of(l);
+    const uint16_t* y = (s.relu && l != L_OUT) ? u->y16[l] : nullptr;
+    k_grad_to_cl<<<blocks_for(nv * cp, UTPB), UTPB, 0, g_stream>>>(u->gy[l], y, s.cout, cp, nv, u->g16);
+}
+
+void weight_grad_bf16(flux_unet* u, int l) {
+    const LayerSpec& s = kLayers[l];
+    const int nx = u->n[0] >> s.level, ny = u->n[1] >> s.level, nz = u->n[2] >> s.level;
+    const WgradPlan p = plan_wgrad(u, l);
+    const int64_t waves = (int64_t)p.co_tiles * p.ci_tiles * 27 * p.runs;
+    k_wgrad_mfma<<<(unsigned)((waves + 3) / 4), 256, 0, g_stream>>>(u->g16, cpad_of(l), mfma_input(u, l), p.co_tiles,
+                                                                     p.ci_tiles, p.kv, p.runs, nz, ny, nx, u->wpart);
+    const int64_t nw = (int64_t)s.cout * s.cin * 27;
+    k_wgrad_finish<<<blocks_for(nw, UTPB), UTPB, 0, g_stream>>>(u->wpart, p.runs, p.co_tiles, p.ci_tiles, s.cout,
+                                                                  s.cin, u->grads + u->off_w[l]);
+    k_bias_grad<<<s.cout, UTPB, 0, g_stream>>>(u->g16, cpad_of(l), p.nv, u->grads + u->off_b[l]);
+}
+
+// the data gradient of layer l into dst ([cin][voxel] FP32, [up | skip])
+void data_grad_bf16(flux_unet* u, int l, float* dst) {
+    const LayerSpec& s = kLayers[l];
+    const int nx = u->n[0] >> s.level, ny = u->n[1] >> s.level, nz = u->n[2] >> s.level;
+    const int64_t nv = u->nvox[s.level];
+    const MfmaIn in{u->g16, nullptr, cpad_of(l), 0, 0};
+    const uint16_t* wq = u->wqt + u->off_wqt[l];
+    const int np = u->npad_t[l], g = u->groups_t[l];
+    if (np % 64 == 0)
+        launch_mfma<1, 4>(in, wq, u->zero_bias, u->dx_cl, s.cin, np, g, nz, ny, nx, 0, 1, nullptr, 1);
+    else if (np % 32 == 0)
+        launch_mfma<1, 2>(in, wq, u->zero_bias, u->dx_cl, s.cin, np, g, nz, ny, nx, 0, 1, nullptr, 1);
+    else
+        launch_mfma<1, 1>(in, wq, u->zero_bias, u->dx_cl, s.cin, np, g, nz, ny, nx, 0, 1, nullptr, 1);
+    k_cl_to_nc<<<blocks_for(nv * s.cin, UTPB), UTPB, 0, g_stream>>>(u->dx_cl, dst, s.cin, nv);
+}
+
 // gradient of layer l's input from gy[l]: returns into dst (full-res, cin
 // channels laid out [up part | skip part]) -- computed as a conv of the masked
 // gradient with the flipped transposed kernel
@@ -1024,6 +1289,23 @@ void weight_grad(flux_unet* u, int l, const float* x) {
 int backward(flux_unet* u, const float* x) {
     auto copy_part = [&](const float* src, float* dst, int64_t n) {
         k_ucopy<<<blocks_for(n, UTPB), UTPB, 0, g_stream>>>(src, dst, n);
+    };
+    // BF16 networks: each layer's gradients on MFMA from its masked output
+    // gradient in channels-last BF16 (weight_grad converts it, data_grad
+    // follows and reuses it)
+    const bool bf = u->precision == FLUX_UNET_BF16;
+    if (bf) pack_transposed(u);
+    auto weight_grad = [&](flux_unet* uu, int l, const float* xx) {
+        if (bf) {
+            grad_to_cl(uu, l);
+            weight_grad_bf16(uu, l);
+        } else {
+            ::weight_grad(uu, l, xx);
+        }
+    };
+    auto data_grad = [&](flux_unet* uu, int l, float* dst) {
+        if (bf) data_grad_bf16(uu, l, dst);
+        else ::data_grad(uu, l, dst);
     };
     // out
     weight_grad(u, L_OUT, x);
@@ -1107,6 +1389,11 @@ int flux_unet_destroy(flux_unet* u) {
     dfree(u->x16);
     for (int l = 0; l < kNumLayers; ++l) dfree(u->y16[l]);
     for (int q = 0; q < 3; ++q) dfree(u->pooled16[q]);
+    dfree(u->wqt);
+    dfree(u->g16);
+    dfree(u->dx_cl);
+    dfree(u->wpart);
+    dfree(u->zero_bias);
     dfree(u->part);
     dfree(u->scal);
     delete u;
@@ -1244,7 +1531,9 @@ int flux_unet_train_step(flux_unet* u, const flux_field* input, const flux_field
     FLUX_TRY(check_field(input, FLUX_F32, 3 * u->nvox[0], "unet: input"));
     FLUX_TRY(check_field(target, FLUX_F32, u->nvox[0], "unet: target"));
     const float* x = (const float*)input->d;
-    FLUX_TRY(forward(u, x));
+    // BF16 networks train on the matrix cores (forward and backward)
+    if (u->precision == FLUX_UNET_BF16) FLUX_TRY(bf16_train_setup(u));
+    FLUX_TRY(infer(u, x));
     int64_t nv = u->nvox[0];
     unsigned nb = blocks_for(nv, UTPB);
     k_mse<<<nb, UTPB, 0, g_stream>>>(u->y[L_OUT], (const float*)target->d, u->gy[L_OUT], u->part, nv);
@@ -1285,7 +1574,9 @@ int flux_unet_train_step_energy(flux_unet* u, const flux_field* input, flux_oper
     if (mask) FLUX_TRY(check_field(mask, FLUX_U8, nv, "unet: mask"));
     const uint8_t* m = mask ? (const uint8_t*)mask->d : nullptr;
     const float* x = (const float*)input->d;
-    FLUX_TRY(forward(u, x));
+    // BF16 networks train on the matrix cores (forward and backward)
+    if (u->precision == FLUX_UNET_BF16) FLUX_TRY(bf16_train_setup(u));
+    FLUX_TRY(infer(u, x));
     const unsigned nb = blocks_for(nv, UTPB);
     // g into gy[L_OUT] (scratch until the gradient overwrites it), A g into
     // gin_up[L_D0A] (48 channels at level 0; backward writes it only after

@@ -57,7 +57,9 @@ def init_params(seed=0):
 class UNet:
     def __init__(self, n, training=False, params=None, precision="f32"):
         """precision "f32" (direct FP32 convolutions) or "bf16" (implicit-GEMM
-        convolutions on the matrix cores) for forward/guess; training is FP32"""
+        convolutions on the matrix cores, BF16 activations and gradients with
+        FP32 accumulation) for forward/guess and for training, whose master
+        weights, gradients and Adam stay FP32"""
         self.n = tuple(n)
         h = C.c_void_p()
         _check(_lib.flux_unet_create(int(n[0]), int(n[1]), int(n[2]), int(bool(training)), C.byref(h)), "unet")
