@@ -1,6 +1,8 @@
 // A 3-D two-phase dam-break projection driven entirely through the Node
 // binding, the way a fluxflow page would: seed, bin, beta, build u*, project.
-// Usage: node projection.mjs <n> <outdir>
+// Usage: node projection.mjs <n> <outdir> [headline]
+// "headline": the benchmark's form (FP16 V-cycle, brick smoother with 12
+// sweeps, the free-cell coarse correction) instead of the reference's.
 import { writeFileSync, mkdirSync } from 'fs';
 import * as flux from '../../tsl_array_n_amd/node/index.mjs';
 
@@ -8,6 +10,7 @@ async function main() {
 
 	const n = Number( process.argv[ 2 ] || 16 );
 	const dir = process.argv[ 3 ];
+	const headline = process.argv[ 4 ] === 'headline';
 	mkdirSync( dir, { recursive: true } );
 	await flux.init();
 	const h = 1 / n;
@@ -29,8 +32,12 @@ async function main() {
 			active: Math.floor( pos[ 0 ] / h ) === pin[ 0 ] && Math.floor( pos[ 1 ] / h ) === pin[ 1 ] && Math.floor( pos[ 2 ] / h ) === pin[ 2 ],
 			target: 0
 		} ),
-		faceWeights: beta, multigrid: { numberOfLevels: 3 }, tolerance: 1e-6, maxIterations: 100
+		faceWeights: beta, multigrid: headline ? {
+			numberOfLevels: 3, storage: 'f16', smoother: 'brick', numberOfSmoothingIterationsDown: 12,
+			numberOfSmoothingIterationsUp: 12, numberOfFinalIterations: 12
+		} : { numberOfLevels: 3 }, tolerance: 1e-6, maxIterations: 100
 	} );
+	if ( headline ) solver.setCoarseSpace( true );
 	const dispatch = solver.project( grid, grid );
 	const diag = await dispatch();
 	writeFileSync( `${ dir }/pressure.bin`, Buffer.from( ( await solver.pressure.data.toArray() ).buffer ) );

@@ -88,6 +88,70 @@ def test_projection_through_node_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_headline_projection_through_node_matches_oracle(tmp_path):
+    """The benchmark's headline form through the Node binding (multigrid
+    options storage 'f16', smoother 'brick', 12 sweeps; setCoarseSpace(true))
+    on the 32^3 dam break: accepted at the FP64 1e-6 true residual, pressure
+    within 1e-5 of the oracle's plain solve (both within ~1e-6 of the exact
+    solution), in fewer iterations."""
+    from tsl_array_n_amd import scenes
+    n = 32
+    r = run_node("projection.mjs", n, tmp_path, "headline", timeout=300)
+    print(r.stdout)
+    meta = json.load(open(tmp_path / "diag.json"))
+    assert meta["outside"] == 0 and meta["diag"]["converged"], meta
+    sc = scenes.dam_break_system(n)
+    A = O.pressure3_assemble(sc["shape"], sc["spacing"], sc["vel"], active=sc["active"], target=sc["target"],
+                             beta=sc["beta"], suffix="f32")
+    mg = O.Multigrid(sc["shape"], sc["spacing"], levels=3, mask=A["mask"], coarse_mask=A["vent"],
+                     weights=A["weights"], suffix="f32")
+    x, x64, st = O.pcg_solve_mixed(sc["shape"], sc["spacing"], A["b"], x0=A["pressure"], mask=A["mask"],
+                                   weights=A["weights"], mg=mg, tol=1e-6, maxiter=100)
+    p = np.fromfile(tmp_path / "pressure.bin", np.float32)
+    assert np.linalg.norm(p - x) <= 1e-5 * np.linalg.norm(x)
+    assert meta["diag"]["iterations"] < st["iterations"]
+
+
+@pytest.mark.gpu
+def test_flip_headline_form_and_set_dt_through_node(flux, tmp_path):
+    """createGridFlipSolver3 through Node with the headline projection form
+    (multigrid storage 'f16', smoother 'brick', setCoarseSpace) and setDt
+    between steps gives the positions and velocities of the same run driven
+    from Python bit for bit, every projection accepted."""
+    from tsl_array_n_amd import flip as FL
+    r = run_node("flip_forms.mjs", tmp_path, timeout=300)
+    print(r.stdout)
+    meta = json.load(open(tmp_path / "diag.json"))
+    assert all(d["converged"] and not d["rejected"] for d in meta["diags"]), meta
+    n, h = 32, 1.0 / 32
+    pos = []
+    for k in range(n):
+        for j in range(19):
+            for i in range(13):
+                for sk in range(2):
+                    for sj in range(2):
+                        for si in range(2):
+                            pos.append(((i + (si + 0.5) / 2) * h, (j + (sj + 0.5) / 2) * h, (k + (sk + 0.5) / 2) * h))
+    pos = np.asarray(pos, np.float32)
+    assert len(pos) == meta["count"]
+    s = FL.GridFlipSolver3(resolution=(n, n, n), grid_spacing=(h, h, h), max_particles=len(pos), dt=1.0 / 60.0,
+                           pressure={"multigrid": {"levels": 3, "storage": "f16", "smoother": "brick", "nu_down": 12,
+                                                   "nu_up": 12, "n_final": 12}, "tolerance": 1e-6})
+    s.set_coarse_space(True)
+    s.positions.from_array(pos)
+    its = []
+    for _ in range(2):
+        its.append(s.on_advance_time_step()["iterations"])
+    s.set_dt(1.0 / 120.0)
+    for _ in range(2):
+        its.append(s.on_advance_time_step()["iterations"])
+    assert its == [d["iterations"] for d in meta["diags"]]
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "positions.bin", np.float32), s.positions.to_array().ravel())
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "velocities.bin", np.float32),
+                                  s.velocities.to_array().ravel())
+
+
+@pytest.mark.gpu
 def test_flip_and_collider_through_node_match_oracle(tmp_path):
     """createGridFlipSolver3, createSDFStaticCollider3.computeFaceWeights and
     constrainVelocity driven from Node, against the oracle."""

@@ -329,15 +329,32 @@ static void mg_opts(napi_env env, napi_value o, flux_mg_options* m) {
     if ((v = prop(env, o, "numberOfCoarsestIterations"))) m->n_coarsest = get_int(env, v, m->n_coarsest);
     if ((v = prop(env, o, "numberOfFinalIterations"))) m->n_final = get_int(env, v, m->n_final);
     if ((v = prop(env, o, "sorFactor"))) m->omega = get_double(env, v, m->omega);
-    if ((v = prop(env, o, "storage"))) m->storage = get_int(env, v, m->storage);
-    if ((v = prop(env, o, "smoother"))) {
-        /* 'rbgs' | 'jacobi' (or the FLUX_SMOOTH_* number) */
+    if ((v = prop(env, o, "storage"))) {
+        /* 'f32' | 'f16' | 'bf16' (or the FLUX_STORE_* number) */
         napi_valuetype t;
         char name[16] = {0};
         size_t len = 0;
         if (napi_typeof(env, v, &t) == napi_ok && t == napi_string) {
             napi_get_value_string_utf8(env, v, name, sizeof(name), &len);
-            m->smoother = strcmp(name, "jacobi") == 0 ? FLUX_SMOOTH_JACOBI : FLUX_SMOOTH_RBGS;
+            if (strcmp(name, "f32") == 0) m->storage = FLUX_STORE_F32;
+            else if (strcmp(name, "f16") == 0) m->storage = FLUX_STORE_F16;
+            else if (strcmp(name, "bf16") == 0) m->storage = FLUX_STORE_BF16;
+            else m->storage = -1; /* rejected by the library with its own message */
+        } else {
+            m->storage = get_int(env, v, m->storage);
+        }
+    }
+    if ((v = prop(env, o, "smoother"))) {
+        /* 'rbgs' | 'jacobi' | 'brick' (or the FLUX_SMOOTH_* number) */
+        napi_valuetype t;
+        char name[16] = {0};
+        size_t len = 0;
+        if (napi_typeof(env, v, &t) == napi_ok && t == napi_string) {
+            napi_get_value_string_utf8(env, v, name, sizeof(name), &len);
+            if (strcmp(name, "rbgs") == 0) m->smoother = FLUX_SMOOTH_RBGS;
+            else if (strcmp(name, "jacobi") == 0) m->smoother = FLUX_SMOOTH_JACOBI;
+            else if (strcmp(name, "brick") == 0) m->smoother = FLUX_SMOOTH_BRICK;
+            else m->smoother = -1; /* rejected by the library with its own message */
         } else {
             m->smoother = get_int(env, v, m->smoother);
         }
@@ -986,6 +1003,25 @@ static napi_value js_flip3_step(napi_env env, napi_callback_info info) {
     return diag_object(env, &dg);
 }
 
+// flip3SetDt(s, dt): the step's dt from the next step on (the reference's
+// live dt node, written by createGridAdaptiveTimeStep3)
+static napi_value js_flip3_set_dt(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    CHECK_FLUX(flux_flip3_set_dt((flux_flip3*)unwrap(env, a[0], K_FLIP, K_FLIP), (float)get_double(env, a[1], 0.0)));
+    return NULL;
+}
+
+// flip3SetCoarseSpace(s, freeIndicator): the coarse-space correction of the
+// step's projection (flux_flip3_pressure; no reference counterpart)
+static napi_value js_flip3_set_coarse_space(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    args_n(env, info, 2, a);
+    flux_pressure3* p = flux_flip3_pressure((flux_flip3*)unwrap(env, a[0], K_FLIP, K_FLIP));
+    CHECK_FLUX(flux_pressure3_set_coarse_space(p, get_int(env, a[1], 1), 0, NULL));
+    return NULL;
+}
+
 static napi_value js_flip3_field(napi_env env, napi_callback_info info) {
     napi_value a[2];
     args_n(env, info, 2, a);
@@ -1562,6 +1598,8 @@ static napi_value module_init(napi_env env, napi_value exports) {
     EXPORT("flip3Create", js_flip3_create);
     EXPORT("flip3Step", js_flip3_step);
     EXPORT("flip3Field", js_flip3_field);
+    EXPORT("flip3SetDt", js_flip3_set_dt);
+    EXPORT("flip3SetCoarseSpace", js_flip3_set_coarse_space);
     EXPORT("smoke3Create", js_smoke3_create);
     EXPORT("smoke3Step", js_smoke3_step);
     EXPORT("smoke3Field", js_smoke3_field);

@@ -947,8 +947,24 @@ export function createGridFlipSolver3( {
 
 	}
 
+	// the step's dt from the next step on (the reference's dt is a live node
+	// that createGridAdaptiveTimeStep3 writes)
+	function setDt( value ) {
+
+		addon.flip3SetDt( handle, value );
+
+	}
+
+	// the free-cell coarse correction of the step's projection (no reference
+	// counterpart)
+	function setCoarseSpace( freeIndicator = true ) {
+
+		addon.flip3SetCoarseSpace( handle, freeIndicator ? 1 : 0 );
+
+	}
+
 	return {
-		onAdvanceTimeStep, settings: { batchStages: true },
+		onAdvanceTimeStep, setDt, setCoarseSpace, settings: { batchStages: true },
 		positions: view( 0, 'vec3', [ maxParticles ] ), velocities: view( 1, 'vec3', [ maxParticles ] ),
 		fluidMask: view( 2, 'float', resolution ), pressure: { data: view( 6, 'float', resolution ) },
 		velocityGrid: solverGrid, diagnostics, inputs: { collider, colliderFaceWeights },
