@@ -99,7 +99,7 @@ DEFAULT_FORMS = ("plain_rbgs,plain_rbgs_nu4,mixed_f16,mixed_bf16,mixed_f16_cs,mi
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed projections of the headline form")
+    ap.add_argument("--steps", type=int, default=50, help="timed projections of the headline form (about a second)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form-steps", type=int, default=5, help="timed projections of each other form")
     ap.add_argument("--n", type=int, default=256)
