@@ -364,7 +364,8 @@ def evaluate_headline(F, train_size=64, pairs=48, steps=3000, size=256, reps=5, 
     two-phase FLIP dam break with the headline form (FP16 V-cycle, brick
     smoother with 12 sweeps, 6 levels) and the free-cell correction alone,
     with the network's guess as a second coarse vector (BF16 inference at
-    size^3 included in the time), and with the exact solution in its place
+    size^3 included in the time; also as the start, and with the flexible
+    Polak-Ribiere beta), and with the exact solution in its place
     (the bound of any vector); returns iterations and ms per projection."""
     import time
     from . import flip as FL
@@ -396,7 +397,7 @@ def evaluate_headline(F, train_size=64, pairs=48, steps=3000, size=256, reps=5, 
     vout = [array_n("float", F.face_shape(shape, a)) for a in range(3)]
     mg = {"levels": 6, "storage": "f16", "smoother": "brick", "nu_down": 12, "nu_up": 12, "n_final": 12}
     table, exact = {}, None
-    for mode in ("free", "learned", "learned+start", "exact"):
+    for mode in ("free", "learned", "learned+start", "learned+flexible", "exact"):
         ps = F.create_grid_pressure_solver3(resolution=shape, grid_spacing=sc["spacing"], dirichlet=(act, tgt),
                                             face_weights=beta, multigrid=mg, tolerance=1e-6)
         if mode == "free":
@@ -404,6 +405,8 @@ def evaluate_headline(F, train_size=64, pairs=48, steps=3000, size=256, reps=5, 
             proj = ps
         elif mode.startswith("learned"):
             proj = CoarseProjector(ps, big, 1000.0, start=mode.endswith("start"))
+            if mode.endswith("flexible"):
+                ps.set_flexible(True)
         else:
             ps.set_coarse_space(True, [array_n("float", shape, exact)])
             proj = ps
